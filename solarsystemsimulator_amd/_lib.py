@@ -1,0 +1,100 @@
+"""ctypes binding of libssim.so (the C-ABI declared in include/ssim.h).
+
+The library is loaded from the package directory, where
+``python -m solarsystemsimulator_amd.build`` puts it.  There is no fallback:
+if the shared object is missing or does not load, every GPU entry point
+raises, so a run can never silently compute on the host.
+
+Device buffers are torch tensors on the ROCm device; only their
+``data_ptr()`` crosses the boundary, together with the current HIP stream.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import c_double, c_int, c_longlong, c_size_t, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libssim.so"
+
+SSIM_OK = 0
+SSIM_EINVAL = -1
+SSIM_EHIP = -2
+
+PREC_F64 = 0
+PREC_F32C = 1
+SCHEME_KDK = 0
+SCHEME_YOSHIDA4 = 1
+
+_P = c_void_p  # device pointer / stream
+
+# name -> (restype, argtypes).  Kept in the order of include/ssim.h.
+SIGNATURES = {
+    "ssim_last_error": (ctypes.c_char_p, []),
+    "ssim_version": (ctypes.c_char_p, []),
+    "ssim_device_count": (c_int, []),
+    "ssim_allpairs_work_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
+    "ssim_allpairs": (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, c_int, _P,
+                              _P, _P, _P, _P, _P, c_double, c_double, _P,
+                              c_size_t, _P]),
+    "ssim_selfgrav_work_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
+    "ssim_selfgrav_run": (c_int, [c_int, c_int, c_int, c_int, c_int,
+                                  _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                  c_double, c_double, c_double, c_int, c_int,
+                                  _P, c_longlong, _P, c_size_t, _P]),
+    "ssim_kick": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, c_double, _P]),
+    "ssim_drift": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, c_double, _P]),
+    "ssim_energy_work_bytes": (c_size_t, [c_int, c_int]),
+    "ssim_energy": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, _P, c_double,
+                            c_double, _P, _P, c_size_t, _P]),
+}
+
+
+class SsimError(RuntimeError):
+    """A HIP runtime failure reported by libssim."""
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Open libssim.so once and attach the signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with "
+            "`python -m solarsystemsimulator_amd.build`")
+    lib = ctypes.CDLL(os.fspath(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    """Turn a libssim status code into ValueError / SsimError."""
+    if rc == SSIM_OK:
+        return
+    msg = load().ssim_last_error().decode(errors="replace")
+    if rc == SSIM_EINVAL:
+        raise ValueError(msg)
+    raise SsimError(msg or f"libssim error {rc}")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args))
+
+
+def ptr(t) -> int | None:
+    """Device address of a torch tensor (None for None)."""
+    return None if t is None else t.data_ptr()
+
+
+def stream_handle(device=None) -> int:
+    """The current torch stream of `device` as a hipStream_t value."""
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream

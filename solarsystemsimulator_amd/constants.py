@@ -1,0 +1,22 @@
+"""Module constants of the reference physics backend, same names and values.
+
+nbody_kernel.py:58-72, selfgrav_kernel.py:75-78, gpu_verbund.py:30-31.
+"""
+import numpy as np
+
+M_MAX = 64                      # massive bodies per nbody state
+K_MAX = 16                      # raster samples per step_batch call
+G_AU = 4 * np.pi * np.pi        # AU^3 / (M_sun yr^2)
+SOFTENING = 1e-6                # nbody: added to r^2 (acts as eps^2)
+MAX_SUB_DT_YEARS = 0.5 / 365.25
+MAX_SUB_STEPS_PER_FRAME = 100_000
+YOSHIDA_W1 = 1.0 / (2.0 - 2.0 ** (1.0 / 3.0))
+YOSHIDA_W0 = -(2.0 ** (1.0 / 3.0)) / (2.0 - 2.0 ** (1.0 / 3.0))
+SUB_SAMPLES = 8
+SUB_SAMPLES_MAX = 32
+
+G_MAX = 8                       # device-list limit kept for API checks
+BLOCK = 256                     # selfgrav tile width (kept for callers)
+KALIBRIER_N = 4096
+KALIBRIER_SCHRITTE = 12
+KALIBRIER_RUNDEN = 3

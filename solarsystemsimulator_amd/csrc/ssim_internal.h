@@ -1,0 +1,123 @@
+// Internal helpers shared by the libssim translation units (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/ssim.h"
+
+namespace ssim {
+
+// Thread-local message of the last failure, returned by ssim_last_error().
+void set_error(const char* fmt, ...);
+void clear_error();
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// Round a byte count up so consecutive workspace slices stay 256-B aligned.
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// All-pairs force (allpairs.hip); shared by the integrators.
+size_t allpairs_work_bytes(int dim, int ns, int nt);
+int allpairs(int dim, int prec, int ns, const double* sx, const double* sy,
+             const double* sz, const double* sm, int nt, const double* tx,
+             const double* ty, const double* tz, double* ax, double* ay,
+             double* az, double G, double eps2, void* work, size_t work_bytes,
+             hipStream_t st);
+
+// Elementwise integrator kernels (integrate.hip).
+int kick(int dim, int n, double* vx, double* vy, double* vz, const double* ax,
+         const double* ay, const double* az, double h, hipStream_t st);
+int drift(int dim, int n, double* x, double* y, double* z, const double* vx,
+          const double* vy, const double* vz, double h, hipStream_t st);
+
+// Yoshida 4th-order composition weights w1, w0 (nbody_kernel.py:64-65).
+double yoshida_w1();
+double yoshida_w0();
+
+}  // namespace ssim
+
+// Record a HIP failure and return SSIM_EHIP from the enclosing function.
+#define SSIM_HIP(call)                                                       \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      ssim::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                      __FILE__, __LINE__);                                   \
+      return SSIM_EHIP;                                                      \
+    }                                                                        \
+  } while (0)
+
+// Check the launch that was just enqueued.
+#define SSIM_LAUNCHED(name)                                                   \
+  do {                                                                        \
+    hipError_t e_ = hipGetLastError();                                        \
+    if (e_ != hipSuccess) {                                                   \
+      ssim::set_error("launch of %s failed: %s", name, hipGetErrorString(e_)); \
+      return SSIM_EHIP;                                                       \
+    }                                                                         \
+  } while (0)
+
+// Reject an argument: record the message and return SSIM_EINVAL.
+#define SSIM_REQUIRE(cond, ...)   \
+  do {                            \
+    if (!(cond)) {                \
+      ssim::set_error(__VA_ARGS__); \
+      return SSIM_EINVAL;         \
+    }                             \
+  } while (0)
+
+// Propagate a non-zero status from a nested call.
+#define SSIM_TRY(call)       \
+  do {                       \
+    int rc_ = (call);        \
+    if (rc_ != SSIM_OK) return rc_; \
+  } while (0)
+
+// ---------------------------------------------------------------------
+// Device arithmetic helpers
+// ---------------------------------------------------------------------
+
+// 1/sqrt(x) to ~1 ulp in FP64: the hardware v_rsq_f64 estimate (relative
+// error about 2^-23) refined by one third-order Householder step,
+//   y1 = y0 * (1 + e/2 + 3 e^2/8),  e = 1 - x*y0^2,
+// whose truncation error 5/16 e^3 is ~2^-69, below FP64 rounding.  The
+// library is built with -ffp-contract=off, so every fma here is explicit.
+__device__ __forceinline__ double ssim_rsqrt_f64(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double t = x * y;
+  const double e = fma(-t, y, 1.0);
+  const double p = fma(e, 0.375, 0.5);
+  const double q = e * p;
+  return fma(y, q, y);
+}
+
+// 1/sqrt(x) in FP32: v_rsq_f32 (about 1 ulp) plus one Newton step.
+__device__ __forceinline__ float ssim_rsqrt_f32(float x) {
+  const float y = __builtin_amdgcn_rsqf(x);
+  const float t = x * y;
+  const float e = fmaf(-t, y, 1.0f);
+  return fmaf(y * 0.5f, e, y);
+}
+
+// Knuth TwoSum: s + err == a + b exactly (6 flops, no ordering needed).
+template <typename F>
+__device__ __forceinline__ void ssim_two_sum(F a, F b, F& s, F& err) {
+  s = a + b;
+  const F bb = s - a;
+  err = (a - (s - bb)) + (b - bb);
+}
+
+// Accumulate b into the compensated pair (hi, lo).
+template <typename F>
+__device__ __forceinline__ void ssim_comp_add(F& hi, F& lo, F b) {
+  F s, e;
+  ssim_two_sum(hi, b, s, e);
+  hi = s;
+  lo += e;
+}
