@@ -103,6 +103,14 @@ int ssim_kick(int dim, int n, double* vx, double* vy, double* vz,
 int ssim_drift(int dim, int n, double* x, double* y, double* z,
                const double* vx, const double* vy, const double* vz, double h,
                void* stream);
+/* Half-kick v += a*h of n masses + t tracers followed (if snap != NULL) by
+ * the f32 snapshot row described at ssim_selfgrav_run; used by integrators
+ * whose force comes from elsewhere (particle mesh, NBodyPM.step_batch,
+ * pm_kernel.py:254-281). */
+int ssim_kick_snap(int dim, int n, int t, const double* x, const double* y,
+                   const double* z, double* vx, double* vy, double* vz,
+                   const double* ax, const double* ay, const double* az,
+                   double h, float* snap, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* FP64 energy diagnostic                                              */
@@ -119,6 +127,155 @@ int ssim_energy(int dim, int n, const double* x, const double* y,
                 const double* z, const double* vx, const double* vy,
                 const double* vz, const double* m, double G, double eps2,
                 double* out, void* work, size_t work_bytes, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* FFT (real <-> half-complex, NumPy layout and normalisation)         */
+/* ------------------------------------------------------------------ */
+
+/* dims: ndim (1..3) powers of two in [2, 4096], last axis contiguous.
+ * The complex half spectrum has shape (dims[0], .., dims[ndim-1]/2 + 1),
+ * interleaved (re, im) in FP64 (prec 0) or FP32 (prec 1).  `work` holds
+ * the twiddle tables: size from ssim_fft_work_bytes, filled once by
+ * ssim_fft_prepare.  r2c = numpy.fft.rfftn, c2r = numpy.fft.irfftn (scaled
+ * by 1/prod(dims)); c2r overwrites its complex input.
+ * Replaces cp.fft.rfft2 / irfft2 (pm_kernel.py:60,144-146). */
+size_t ssim_fft_work_bytes(int ndim, const int* dims, int prec);
+int ssim_fft_prepare(int ndim, const int* dims, int prec, void* work,
+                     size_t work_bytes, void* stream);
+int ssim_fft_r2c(int ndim, const int* dims, int prec, const void* in,
+                 void* out, const void* work, size_t work_bytes, void* stream);
+int ssim_fft_c2r(int ndim, const int* dims, int prec, void* in, void* out,
+                 const void* work, size_t work_bytes, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Particle-mesh building blocks                                       */
+/* ------------------------------------------------------------------ */
+
+/* Cloud-in-cell deposit of masses m[np] at (x,y,z) onto a mesh of
+ * grid_n[dim] nodes (C order, x slowest) with origin[dim] and cell size h,
+ * ADDED to `mesh` (FP64 or FP32 per mesh_prec; the caller zeroes it).
+ * Isolated mesh (periodic = 0): cell = clip(floor((x-x0)/h), 0, n-2) as in
+ * `_cic_deposit` (pm_kernel.py:63-90); periodic: floor wrapped mod n.  If
+ * cells != NULL the integer cell indices are written as cells[dim*p + c].
+ * The summation order of the atomic adds is not fixed (like cp.bincount). */
+int ssim_cic_deposit(int dim, int periodic, int np, const double* x,
+                     const double* y, const double* z, const double* m,
+                     const int* grid_n, const double* origin, double h,
+                     int mesh_prec, void* mesh, int* cells, void* stream);
+/* CIC interpolation of nfields (1..3) mesh fields at np positions, same
+ * stencil as the deposit (`_cic_gather`, pm_kernel.py:93-107). */
+int ssim_cic_gather(int dim, int periodic, int np, const double* x,
+                    const double* y, const double* z, const int* grid_n,
+                    const double* origin, double h, int mesh_prec, int nfields,
+                    const void* f0, const void* f1, const void* f2,
+                    double* out0, double* out1, double* out2, void* stream);
+/* out[2c] = min, out[2c+1] = max of array c (narrays 1..3), on the device
+ * (the reductions of `grid_fuer`, pm_kernel.py:117-118). */
+size_t ssim_minmax_work_bytes(int n);
+int ssim_minmax(int n, int narrays, const double* a0, const double* a1,
+                const double* a2, double* out, void* work, size_t work_bytes,
+                void* stream);
+/* Real-space force Green's function component `comp` on the zero-padded
+ * (2n)^dim mesh: -G d_comp / (|d|^2 + eps2)^(3/2) with wrapped offsets
+ * (build_force_kernels, pm_kernel.py:34-60, generalised to 3-D).  rs > 0
+ * keeps only the long-range part, times 1 - g(r) with the Gaussian split
+ * g(r) = erfc(r/2rs) + r/(rs sqrt(pi)) exp(-r^2/4rs^2); rs = 0 is the
+ * reference's full kernel. */
+int ssim_pm_force_kernel(int dim, int n, int comp, double h, double eps2,
+                         double G, double rs, int mesh_prec, void* out,
+                         void* stream);
+/* Divide a half spectrum on the padded mesh (padded_n^dim) by the squared
+ * CIC window prod_c sinc^4(pi f_c / padded_n) (no reference counterpart:
+ * used by the split mesh + near-field force). */
+int ssim_pm_deconvolve(int dim, int padded_n, int prec, void* spectrum,
+                       void* stream);
+/* out = a * b over ncomplex interleaved complex values (the spectral
+ * products of pm_force_field, pm_kernel.py:144-146). */
+int ssim_spectrum_mul(long long ncomplex, int prec, const void* a,
+                      const void* b, void* out, void* stream);
+/* Periodic Poisson solve in place on an rfftn half spectrum of MASS per
+ * node: phi_k = -4 pi G (m_k / h^dim) / k^2, phi_0 = 0. */
+int ssim_pm_poisson(int dim, const int* grid_n, double h, double G, int prec,
+                    void* spectrum, void* stream);
+/* Periodic central difference g_c = -(phi[+1] - phi[-1]) / (2h). */
+int ssim_pm_gradient(int dim, const int* grid_n, double h, int prec,
+                     const void* phi, void* g0, void* g1, void* g2,
+                     void* stream);
+/* crop == 0: copy an n^dim block into the corner of a (2n)^dim block (dst
+ * must be zeroed); crop == 1: copy the corner back out
+ * (pm_force_field's padding and cropping, pm_kernel.py:141-146). */
+int ssim_pm_pad(int dim, int n, int prec, const void* src, void* dst, int crop,
+                void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Cell list and near-field (particle-particle) force                  */
+/* ------------------------------------------------------------------ */
+
+/* Bin n bodies into nc[0] x nc[1] x nc[2] cubic cells of edge `cell` from
+ * `origin` (outside coordinates clamp to border cells).  cell_start gets
+ * ncells + 1 offsets, order the body indices sorted by cell (ascending
+ * index within a cell), packed (optional, float4 per body) the
+ * origin-relative FP32 position and mass in that order.  The uniform-grid
+ * hashing of the reference's collision search (film_producer.py:235-240)
+ * and the near-field source lists use it.  No hipMemset inside. */
+size_t ssim_cells_work_bytes(int n, int ncells);
+int ssim_cells_build(int n, const double* x, const double* y, const double* z,
+                     const double* m, const double* origin, double cell,
+                     const int* nc, int* cell_start, int* order, void* packed,
+                     void* work, size_t work_bytes, void* stream);
+/* ADD the near-field force G m d g(r) / (r^2 + eps2)^(3/2), r < rcut, of
+ * the binned sources (packed, from ssim_cells_build with the same grid and
+ * rcut <= cell) to nt targets' accelerations (3-D).  New: the reference has
+ * no split force. */
+int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,
+                  const double* origin, double cell, const int* nc,
+                  const int* cell_start, const void* packed, double G,
+                  double eps2, double rs, double rcut, double* ax, double* ay,
+                  double* az, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Learned near-field correction (MLP, BF16 MFMA, FP32 accumulation)   */
+/* ------------------------------------------------------------------ */
+/* None of these has a reference counterpart: the reference has no learned
+ * component.  Network 32 -> 64 -> 64 -> 64 -> 3 with SiLU; parameters are
+ * one FP32 block of ssim_mlp_param_count() floats (W1[64][32] b1 W2[64][64]
+ * b2 W3[64][64] b3 W4[16][64] b4[16], rows 3..15 of W4/b4 zero) plus its
+ * BF16 copy `wbf` (same offsets) used by the matrix cores.  Features are
+ * BF16 [nb][32]. */
+int ssim_mlp_param_count(void);
+/* wbf = BF16 (round to nearest even) of params. */
+int ssim_mlp_to_bf16(const float* params, unsigned short* wbf, void* stream);
+/* Features of nb bodies from the CIC mass mesh (mesh_n^3, C order, FP64 or
+ * FP32 per mesh_prec) with origin[3] and cell h: the 27 node masses of the
+ * 3x3x3 stencil around the nearest node divided by mass_unit, then twice the
+ * offset from that node in cells, then two zeros. */
+int ssim_mlp_features(int nb, const double* x, const double* y, const double* z,
+                      int mesh_n, const double* origin, double h,
+                      int mesh_prec, const void* rho, double mass_unit,
+                      unsigned short* feat, void* stream);
+/* Forward pass.  out (optional) receives FP32 [nb][3]; if ax != NULL,
+ * scale * out is ADDED to (ax, ay, az).  For training pass zs (FP32
+ * [3][nb][64] pre-activations) and hs (BF16 [3][nb][64] activations). */
+int ssim_mlp_forward(int nb, const unsigned short* feat,
+                     const unsigned short* wbf, const float* params,
+                     float scale, float* out, double* ax, double* ay,
+                     double* az, float* zs, unsigned short* hs, void* stream);
+/* gout = 2 (out - target) / nb; loss_part[block] = partial sums of the
+ * squared error over blocks of 256 bodies. */
+int ssim_mlp_mse_grad(int nb, const float* out, const float* target,
+                      float* gout, float* loss_part, void* stream);
+/* Backward pass: parameter gradient `grad` (FP32, param block layout) of
+ * the loss whose output gradient is gout [nb][3], from the activations
+ * recorded by ssim_mlp_forward.  Sums are in a fixed order. */
+size_t ssim_mlp_train_work_bytes(int nb);
+int ssim_mlp_backward(int nb, const unsigned short* feat,
+                      const unsigned short* wbf, const float* zs,
+                      const unsigned short* hs, const float* gout, float* grad,
+                      void* work, size_t work_bytes, void* stream);
+/* One Adam step (bias-corrected, step >= 1) on params; refreshes wbf. */
+int ssim_mlp_adam(float* params, const float* grad, float* m, float* v,
+                  float lr, float beta1, float beta2, float eps, int step,
+                  unsigned short* wbf, void* stream);
 
 #ifdef __cplusplus
 }

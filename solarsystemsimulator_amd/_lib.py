@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import c_double, c_int, c_longlong, c_size_t, c_void_p
+from ctypes import c_double, c_float, c_int, c_longlong, c_size_t, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libssim.so"
@@ -44,9 +44,48 @@ SIGNATURES = {
                                   _P, c_longlong, _P, c_size_t, _P]),
     "ssim_kick": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, c_double, _P]),
     "ssim_drift": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, c_double, _P]),
+    "ssim_kick_snap": (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P, _P,
+                               _P, _P, c_double, _P, _P]),
     "ssim_energy_work_bytes": (c_size_t, [c_int, c_int]),
     "ssim_energy": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, _P, c_double,
                             c_double, _P, _P, c_size_t, _P]),
+    "ssim_fft_work_bytes": (c_size_t, [c_int, _P, c_int]),
+    "ssim_fft_prepare": (c_int, [c_int, _P, c_int, _P, c_size_t, _P]),
+    "ssim_fft_r2c": (c_int, [c_int, _P, c_int, _P, _P, _P, c_size_t, _P]),
+    "ssim_fft_c2r": (c_int, [c_int, _P, c_int, _P, _P, _P, c_size_t, _P]),
+    "ssim_cic_deposit": (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P,
+                                 c_double, c_int, _P, _P, _P]),
+    "ssim_cic_gather": (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P,
+                                c_double, c_int, c_int, _P, _P, _P, _P, _P, _P,
+                                _P]),
+    "ssim_minmax_work_bytes": (c_size_t, [c_int]),
+    "ssim_minmax": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, c_size_t, _P]),
+    "ssim_pm_force_kernel": (c_int, [c_int, c_int, c_int, c_double, c_double,
+                                     c_double, c_double, c_int, _P, _P]),
+    "ssim_pm_deconvolve": (c_int, [c_int, c_int, c_int, _P, _P]),
+    "ssim_spectrum_mul": (c_int, [c_longlong, c_int, _P, _P, _P, _P]),
+    "ssim_pm_poisson": (c_int, [c_int, _P, c_double, c_double, c_int, _P, _P]),
+    "ssim_pm_gradient": (c_int, [c_int, _P, c_double, c_int, _P, _P, _P, _P,
+                                 _P]),
+    "ssim_pm_pad": (c_int, [c_int, c_int, c_int, _P, _P, c_int, _P]),
+    "ssim_cells_work_bytes": (c_size_t, [c_int, c_int]),
+    "ssim_cells_build": (c_int, [c_int, _P, _P, _P, _P, _P, c_double, _P, _P,
+                                 _P, _P, _P, c_size_t, _P]),
+    "ssim_pp_short": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P,
+                              c_double, c_double, c_double, c_double, _P, _P,
+                              _P, _P]),
+    "ssim_mlp_param_count": (c_int, []),
+    "ssim_mlp_to_bf16": (c_int, [_P, _P, _P]),
+    "ssim_mlp_features": (c_int, [c_int, _P, _P, _P, c_int, _P, c_double, c_int,
+                                  _P, c_double, _P, _P]),
+    "ssim_mlp_forward": (c_int, [c_int, _P, _P, _P, c_float, _P, _P, _P, _P, _P,
+                                 _P, _P]),
+    "ssim_mlp_mse_grad": (c_int, [c_int, _P, _P, _P, _P, _P]),
+    "ssim_mlp_train_work_bytes": (c_size_t, [c_int]),
+    "ssim_mlp_backward": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, c_size_t,
+                                  _P]),
+    "ssim_mlp_adam": (c_int, [_P, _P, _P, _P, c_float, c_float, c_float,
+                              c_float, c_int, _P, _P]),
 }
 
 
@@ -87,6 +126,16 @@ def check(rc: int) -> None:
 
 def call(name: str, *args) -> None:
     check(getattr(load(), name)(*args))
+
+
+def int_array(vals):
+    """A host int32 array for `const int*` arguments (kept alive by caller)."""
+    arr = (ctypes.c_int * len(vals))(*[int(v) for v in vals])
+    return arr
+
+
+def double_array(vals):
+    return (ctypes.c_double * len(vals))(*[float(v) for v in vals])
 
 
 def ptr(t) -> int | None:

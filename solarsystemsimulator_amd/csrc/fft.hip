@@ -1,0 +1,474 @@
+// Hand-written real<->complex FFTs for the particle-mesh solver (gfx950).
+//
+// Replaces the cuFFT calls of the reference (`cp.fft.rfft2/irfft2`,
+// pm_kernel.py:60,144-146) and provides the 3-D rfftn/irfftn the Poisson
+// flavour needs.  Layout and normalisation follow NumPy: real array of
+// shape dims (last axis contiguous), half spectrum of shape
+// (dims[0], ..., dims[-1]/2 + 1) complex interleaved; the inverse divides by
+// prod(dims).
+//
+// One kernel launch per axis ("pass").  A workgroup stages a tile of whole
+// lines (B lines x L points, <= 4096 complex values = 64 KB in FP64) in LDS,
+// runs all log2(L) butterfly stages there and writes the tile back, so each
+// pass streams the array through HBM once.  Lines along a strided axis are
+// grouped by consecutive inner index so every row of the tile is a
+// contiguous segment in HBM (coalesced).  The butterflies are the Stockham
+// autosort formulation (radix 4, one radix-2 stage when log2 L is odd):
+//   stage (R, p): i in [0, L/R), k = i mod p,
+//   x_r = a[i + r L/R] * w^(r k L/(p R)),  y = DFT_R(x),
+//   a'[(i - k) R + k + q p] = y_q.
+// All reads of a stage land in registers before a barrier, then all writes,
+// so one LDS buffer suffices.  Twiddles w^j = exp(-2 pi i j / L) come from a
+// table in HBM (L2-resident) filled by ssim_fft_prepare.
+#include "ssim_internal.h"
+
+namespace {
+
+constexpr int FFT_THREADS = 256;
+constexpr int FFT_TILE = 4096;   // complex values per workgroup tile
+constexpr int FFT_MAXB = 16;     // lines per tile along a strided axis
+constexpr int FFT_MAX_L = 4096;
+
+enum { MODE_C2C = 0, MODE_R2C = 1, MODE_C2R = 2 };
+
+template <typename F>
+struct cpx {
+  F re, im;
+};
+
+template <typename F>
+__device__ __forceinline__ cpx<F> cmul(cpx<F> a, cpx<F> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+template <typename F>
+__device__ __forceinline__ cpx<F> cadd(cpx<F> a, cpx<F> b) {
+  return {a.re + b.re, a.im + b.im};
+}
+
+template <typename F>
+__device__ __forceinline__ cpx<F> csub(cpx<F> a, cpx<F> b) {
+  return {a.re - b.re, a.im - b.im};
+}
+
+// multiply by -i (forward) or +i (inverse)
+template <typename F>
+__device__ __forceinline__ cpx<F> rot(cpx<F> a, bool inverse) {
+  return inverse ? cpx<F>{-a.im, a.re} : cpx<F>{a.im, -a.re};
+}
+
+int ilog2(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
+// Tile geometry of one pass (complex-element units).
+struct Pass {
+  int L, log2L;
+  int n_outer, n_inner;       // lines = n_outer * n_inner
+  long long s_outer_in, s_outer_out, s_elem;
+  int lines_per_tile;         // along the inner index (strided) or outer (contig)
+  int contig;                 // 1: lines are contiguous rows (s_elem == 1)
+};
+
+template <typename F, int MODE>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
+    const void* __restrict__ vin, void* __restrict__ vout, Pass P,
+    const cpx<F>* __restrict__ tw, int inverse, F scale) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<F>* s = reinterpret_cast<cpx<F>*>(smem_raw);
+  const int L = P.L;
+  const int nl = P.lines_per_tile;
+  const int E = nl * L;
+
+  // which lines does this workgroup own?
+  long long o0, b0;
+  int nvalid;
+  if (P.contig) {
+    o0 = (long long)blockIdx.x * nl;
+    b0 = 0;
+    nvalid = (int)min((long long)nl, (long long)P.n_outer - o0);
+  } else {
+    const int groups = (P.n_inner + nl - 1) / nl;
+    o0 = blockIdx.x / groups;
+    b0 = (long long)(blockIdx.x % groups) * nl;
+    nvalid = (int)min((long long)nl, (long long)P.n_inner - b0);
+  }
+
+  // ---- load the tile: s[l * L + e] ----
+  if (MODE == MODE_R2C) {
+    const F* in = static_cast<const F*>(vin);
+    for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
+      const int l = t / L, e = t % L;
+      F v = 0;
+      if (l < nvalid) v = in[(o0 + l) * P.s_outer_in + e];
+      s[t] = {v, F(0)};
+    }
+  } else if (MODE == MODE_C2R) {
+    // half spectrum of L/2+1 values -> Hermitian line of L values
+    const cpx<F>* in = static_cast<const cpx<F>*>(vin);
+    const int H = L / 2 + 1;
+    for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
+      const int l = t / L, e = t % L;
+      cpx<F> v = {F(0), F(0)};
+      if (l < nvalid) {
+        const cpx<F>* row = in + (o0 + l) * P.s_outer_in;
+        if (e < H) {
+          v = row[e];
+        } else {
+          const cpx<F> c = row[L - e];
+          v = {c.re, -c.im};
+        }
+      }
+      s[t] = v;
+    }
+  } else {
+    const cpx<F>* in = static_cast<const cpx<F>*>(vin);
+    if (P.contig) {
+      for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
+        const int l = t / L, e = t % L;
+        s[t] = (l < nvalid) ? in[(o0 + l) * P.s_outer_in + e] : cpx<F>{0, 0};
+      }
+    } else {
+      // consecutive threads walk the inner index: contiguous in HBM
+      for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
+        const int l = t % nl, e = t / nl;
+        cpx<F> v = {0, 0};
+        if (l < nvalid) v = in[o0 * P.s_outer_in + (b0 + l) + e * P.s_elem];
+        s[l * L + e] = v;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- Stockham stages over all lines of the tile ----
+  int p = 1;
+  if (P.log2L & 1) {  // one radix-2 stage (p == 1: k == 0, no twiddle)
+    constexpr int NB = FFT_TILE / 2 / FFT_THREADS;
+    const int T = L / 2;
+    cpx<F> r0[NB], r1[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int u = threadIdx.x + nb * FFT_THREADS;
+      if (u < E / 2) {
+        const cpx<F>* a = s + (u / T) * L;
+        const int i = u % T;
+        r0[nb] = a[i];
+        r1[nb] = a[i + T];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int u = threadIdx.x + nb * FFT_THREADS;
+      if (u < E / 2) {
+        cpx<F>* a = s + (u / T) * L;
+        const int i = u % T;
+        a[2 * i] = cadd(r0[nb], r1[nb]);
+        a[2 * i + 1] = csub(r0[nb], r1[nb]);
+      }
+    }
+    __syncthreads();
+    p = 2;
+  }
+  for (; p < L; p *= 4) {
+    constexpr int NB = FFT_TILE / 4 / FFT_THREADS;
+    const int T = L / 4;
+    const int twstep = L / (4 * p);
+    cpx<F> r[NB][4];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int u = threadIdx.x + nb * FFT_THREADS;
+      if (u < E / 4) {
+        const int i = u % T;
+        const int k = i & (p - 1);
+        const cpx<F>* a = s + (u / T) * L;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          cpx<F> x = a[i + q * T];
+          if (q > 0 && k > 0) {
+            cpx<F> w = tw[q * k * twstep];
+            if (inverse) w.im = -w.im;
+            x = cmul(x, w);
+          }
+          r[nb][q] = x;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int u = threadIdx.x + nb * FFT_THREADS;
+      if (u < E / 4) {
+        const int i = u % T;
+        const int k = i & (p - 1);
+        cpx<F>* a = s + (u / T) * L;
+        const cpx<F> s02 = cadd(r[nb][0], r[nb][2]);
+        const cpx<F> d02 = csub(r[nb][0], r[nb][2]);
+        const cpx<F> s13 = cadd(r[nb][1], r[nb][3]);
+        const cpx<F> d13 = rot(csub(r[nb][1], r[nb][3]), inverse != 0);
+        const int j = (i - k) * 4 + k;
+        a[j] = cadd(s02, s13);
+        a[j + p] = cadd(d02, d13);
+        a[j + 2 * p] = csub(s02, s13);
+        a[j + 3 * p] = csub(d02, d13);
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- store ----
+  if (MODE == MODE_R2C) {
+    cpx<F>* out = static_cast<cpx<F>*>(vout);
+    const int H = L / 2 + 1;
+    for (int t = threadIdx.x; t < nl * H; t += FFT_THREADS) {
+      const int l = t / H, e = t % H;
+      if (l < nvalid) {
+        cpx<F> v = s[l * L + e];
+        out[(o0 + l) * P.s_outer_out + e] = {v.re * scale, v.im * scale};
+      }
+    }
+  } else if (MODE == MODE_C2R) {
+    F* out = static_cast<F*>(vout);
+    for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
+      const int l = t / L, e = t % L;
+      if (l < nvalid) out[(o0 + l) * P.s_outer_out + e] = s[t].re * scale;
+    }
+  } else {
+    cpx<F>* out = static_cast<cpx<F>*>(vout);
+    if (P.contig) {
+      for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
+        const int l = t / L, e = t % L;
+        if (l < nvalid) {
+          const cpx<F> v = s[t];
+          out[(o0 + l) * P.s_outer_out + e] = {v.re * scale, v.im * scale};
+        }
+      }
+    } else {
+      for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
+        const int l = t % nl, e = t / nl;
+        if (l < nvalid) {
+          const cpx<F> v = s[l * L + e];
+          out[o0 * P.s_outer_out + (b0 + l) + e * P.s_elem] = {v.re * scale,
+                                                               v.im * scale};
+        }
+      }
+    }
+  }
+}
+
+template <typename F>
+__global__ void k_fft_twiddles(int L, cpx<F>* tw) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= L) return;
+  double sn, cs;
+  sincospi(-2.0 * (double)j / (double)L, &sn, &cs);
+  tw[j] = {(F)cs, (F)sn};
+}
+
+size_t elem_bytes(int prec) { return prec == 0 ? 16 : 8; }  // complex
+
+bool valid_dims(int ndim, const int* dims) {
+  if (ndim < 1 || ndim > 3 || !dims) return false;
+  for (int a = 0; a < ndim; ++a) {
+    const int L = dims[a];
+    if (L < 2 || L > FFT_MAX_L || (L & (L - 1))) return false;
+  }
+  return true;
+}
+
+// Twiddle table offsets: one table of L_a complex values per axis.
+size_t tw_offset(int ndim, const int* dims, int axis, int prec) {
+  size_t off = 0;
+  for (int a = 0; a < axis; ++a) off += ssim::align256(dims[a] * elem_bytes(prec));
+  (void)ndim;
+  return off;
+}
+
+template <typename F, int MODE>
+int launch_pass(const void* in, void* out, Pass P, const void* tw, int inverse,
+                F scale, hipStream_t st) {
+  int nwg;
+  if (P.contig)
+    nwg = ssim::ceil_div(P.n_outer, P.lines_per_tile);
+  else
+    nwg = P.n_outer * ssim::ceil_div(P.n_inner, P.lines_per_tile);
+  const size_t lds = (size_t)P.lines_per_tile * P.L * sizeof(cpx<F>);
+  hipLaunchKernelGGL((k_fft_pass<F, MODE>), dim3(nwg), dim3(FFT_THREADS), lds,
+                     st, in, out, P, static_cast<const cpx<F>*>(tw), inverse,
+                     scale);
+  SSIM_LAUNCHED("k_fft_pass");
+  return SSIM_OK;
+}
+
+// Pass along `axis` of the complex array with shape cdims (in place).
+Pass strided_pass(int ndim, const int* cdims, int axis) {
+  Pass P{};
+  P.L = cdims[axis];
+  P.log2L = ilog2(P.L);
+  long long inner = 1, outer = 1;
+  for (int a = axis + 1; a < ndim; ++a) inner *= cdims[a];
+  for (int a = 0; a < axis; ++a) outer *= cdims[a];
+  P.n_outer = (int)outer;
+  P.n_inner = (int)inner;
+  P.s_elem = inner;
+  P.s_outer_in = P.s_outer_out = inner * P.L;
+  P.contig = 0;
+  int b = FFT_TILE / P.L;
+  if (b > FFT_MAXB) b = FFT_MAXB;
+  if (b > P.n_inner) {
+    b = 1;
+    while (b * 2 <= P.n_inner) b *= 2;
+  }
+  P.lines_per_tile = b < 1 ? 1 : b;
+  return P;
+}
+
+Pass row_pass(long long rows, int L, long long s_in, long long s_out) {
+  Pass P{};
+  P.L = L;
+  P.log2L = ilog2(L);
+  P.n_outer = (int)rows;
+  P.n_inner = 1;
+  P.s_elem = 1;
+  P.s_outer_in = s_in;
+  P.s_outer_out = s_out;
+  P.contig = 1;
+  int b = FFT_TILE / L;
+  if (b < 1) b = 1;
+  P.lines_per_tile = b;
+  return P;
+}
+
+template <typename F>
+int fft_r2c(int ndim, const int* dims, const F* in, cpx<F>* out,
+            const unsigned char* work, int prec, hipStream_t st) {
+  int cdims[3];
+  long long rows = 1;
+  for (int a = 0; a < ndim; ++a) cdims[a] = dims[a];
+  cdims[ndim - 1] = dims[ndim - 1] / 2 + 1;
+  for (int a = 0; a < ndim - 1; ++a) rows *= dims[a];
+  const int L = dims[ndim - 1];
+  SSIM_TRY((launch_pass<F, MODE_R2C>(
+      in, out, row_pass(rows, L, L, cdims[ndim - 1]),
+      work + tw_offset(ndim, dims, ndim - 1, prec), 0, F(1), st)));
+  for (int a = ndim - 2; a >= 0; --a)
+    SSIM_TRY((launch_pass<F, MODE_C2C>(out, out, strided_pass(ndim, cdims, a),
+                                       work + tw_offset(ndim, dims, a, prec), 0,
+                                       F(1), st)));
+  return SSIM_OK;
+}
+
+template <typename F>
+int fft_c2r(int ndim, const int* dims, cpx<F>* in, F* out,
+            const unsigned char* work, int prec, hipStream_t st) {
+  int cdims[3];
+  long long rows = 1;
+  double total = 1.0;
+  for (int a = 0; a < ndim; ++a) {
+    cdims[a] = dims[a];
+    total *= dims[a];
+  }
+  cdims[ndim - 1] = dims[ndim - 1] / 2 + 1;
+  for (int a = 0; a < ndim - 1; ++a) rows *= dims[a];
+  for (int a = 0; a < ndim - 1; ++a)
+    SSIM_TRY((launch_pass<F, MODE_C2C>(in, in, strided_pass(ndim, cdims, a),
+                                       work + tw_offset(ndim, dims, a, prec), 1,
+                                       F(1), st)));
+  const int L = dims[ndim - 1];
+  SSIM_TRY((launch_pass<F, MODE_C2R>(
+      in, out, row_pass(rows, L, cdims[ndim - 1], L),
+      work + tw_offset(ndim, dims, ndim - 1, prec), 1, F(1.0 / total), st)));
+  return SSIM_OK;
+}
+
+}  // namespace
+
+namespace ssim {
+
+size_t fft_work_bytes(int ndim, const int* dims, int prec) {
+  if (!valid_dims(ndim, dims)) return 0;
+  return tw_offset(ndim, dims, ndim, prec);
+}
+
+int fft_prepare(int ndim, const int* dims, int prec, void* work,
+                size_t work_bytes, hipStream_t st) {
+  SSIM_REQUIRE(valid_dims(ndim, dims),
+               "FFT dims must be 1..3 powers of two in [2, %d]", FFT_MAX_L);
+  SSIM_REQUIRE(prec == SSIM_PREC_F64 || prec == SSIM_PREC_F32C,
+               "unknown precision %d", prec);
+  SSIM_REQUIRE(work && work_bytes >= fft_work_bytes(ndim, dims, prec),
+               "FFT workspace too small");
+  unsigned char* w = static_cast<unsigned char*>(work);
+  for (int a = 0; a < ndim; ++a) {
+    const int L = dims[a];
+    void* t = w + tw_offset(ndim, dims, a, prec);
+    if (prec == SSIM_PREC_F64)
+      hipLaunchKernelGGL(k_fft_twiddles<double>, dim3(ceil_div(L, 256)),
+                         dim3(256), 0, st, L, static_cast<cpx<double>*>(t));
+    else
+      hipLaunchKernelGGL(k_fft_twiddles<float>, dim3(ceil_div(L, 256)),
+                         dim3(256), 0, st, L, static_cast<cpx<float>*>(t));
+    SSIM_LAUNCHED("k_fft_twiddles");
+  }
+  return SSIM_OK;
+}
+
+int fft_r2c(int ndim, const int* dims, int prec, const void* in, void* out,
+            const void* work, size_t work_bytes, hipStream_t st) {
+  SSIM_REQUIRE(valid_dims(ndim, dims),
+               "FFT dims must be 1..3 powers of two in [2, %d]", FFT_MAX_L);
+  SSIM_REQUIRE(in && out && work, "null pointer");
+  SSIM_REQUIRE(work_bytes >= fft_work_bytes(ndim, dims, prec),
+               "FFT workspace too small");
+  const unsigned char* w = static_cast<const unsigned char*>(work);
+  if (prec == SSIM_PREC_F64)
+    return ::fft_r2c<double>(ndim, dims, static_cast<const double*>(in),
+                             static_cast<cpx<double>*>(out), w, prec, st);
+  return ::fft_r2c<float>(ndim, dims, static_cast<const float*>(in),
+                          static_cast<cpx<float>*>(out), w, prec, st);
+}
+
+int fft_c2r(int ndim, const int* dims, int prec, void* in, void* out,
+            const void* work, size_t work_bytes, hipStream_t st) {
+  SSIM_REQUIRE(valid_dims(ndim, dims),
+               "FFT dims must be 1..3 powers of two in [2, %d]", FFT_MAX_L);
+  SSIM_REQUIRE(in && out && work, "null pointer");
+  SSIM_REQUIRE(work_bytes >= fft_work_bytes(ndim, dims, prec),
+               "FFT workspace too small");
+  const unsigned char* w = static_cast<const unsigned char*>(work);
+  if (prec == SSIM_PREC_F64)
+    return ::fft_c2r<double>(ndim, dims, static_cast<cpx<double>*>(in),
+                             static_cast<double*>(out), w, prec, st);
+  return ::fft_c2r<float>(ndim, dims, static_cast<cpx<float>*>(in),
+                          static_cast<float*>(out), w, prec, st);
+}
+
+}  // namespace ssim
+
+extern "C" {
+
+size_t ssim_fft_work_bytes(int ndim, const int* dims, int prec) {
+  return ssim::fft_work_bytes(ndim, dims, prec);
+}
+
+int ssim_fft_prepare(int ndim, const int* dims, int prec, void* work,
+                     size_t work_bytes, void* stream) {
+  return ssim::fft_prepare(ndim, dims, prec, work, work_bytes,
+                           ssim::as_stream(stream));
+}
+
+int ssim_fft_r2c(int ndim, const int* dims, int prec, const void* in,
+                 void* out, const void* work, size_t work_bytes, void* stream) {
+  return ssim::fft_r2c(ndim, dims, prec, in, out, work, work_bytes,
+                       ssim::as_stream(stream));
+}
+
+int ssim_fft_c2r(int ndim, const int* dims, int prec, void* in, void* out,
+                 const void* work, size_t work_bytes, void* stream) {
+  return ssim::fft_c2r(ndim, dims, prec, in, out, work, work_bytes,
+                       ssim::as_stream(stream));
+}
+
+}  // extern "C"

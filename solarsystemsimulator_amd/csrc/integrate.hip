@@ -227,6 +227,27 @@ int ssim_selfgrav_run(int dim, int prec, int scheme, int n, int t, double* x,
                          work_bytes, st);
 }
 
+int ssim_kick_snap(int dim, int n, int t, const double* x, const double* y,
+                   const double* z, double* vx, double* vy, double* vz,
+                   const double* ax, const double* ay, const double* az,
+                   double h, float* snap, void* stream) {
+  SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3, got %d", dim);
+  SSIM_REQUIRE(n >= 0 && t >= 0, "negative size");
+  if (n + t == 0) return SSIM_OK;
+  SSIM_REQUIRE(x && y && vx && vy && ax && ay && (dim == 2 || (z && vz && az)),
+               "null pointer");
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid(ssim::ceil_div(n + t, EW_BLOCK));
+  if (dim == 2)
+    hipLaunchKernelGGL(k_kick_snap<2>, grid, dim3(EW_BLOCK), 0, st, n, t, x, y,
+                       z, vx, vy, vz, ax, ay, az, h, snap);
+  else
+    hipLaunchKernelGGL(k_kick_snap<3>, grid, dim3(EW_BLOCK), 0, st, n, t, x, y,
+                       z, vx, vy, vz, ax, ay, az, h, snap);
+  SSIM_LAUNCHED("k_kick_snap");
+  return SSIM_OK;
+}
+
 int ssim_kick(int dim, int n, double* vx, double* vy, double* vz,
               const double* ax, const double* ay, const double* az, double h,
               void* stream) {

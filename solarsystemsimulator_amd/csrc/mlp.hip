@@ -1,0 +1,613 @@
+// Learned near-field correction: feature extraction and a 32-64-64-64-16
+// multilayer perceptron (SiLU) with BF16 MFMA and FP32 accumulation, plus
+// its training kernels (backward, weight gradients, Adam).  No reference
+// counterpart: the reference has no learned component.
+//
+// MFMA formulation (v_mfma_f32_16x16x16_bf16, wave64).  Every layer is
+// computed transposed, H^T[out x 16 bodies] = W[out x in] X^T[in x 16], so
+// the 16x16 accumulator tile of one layer (lane l: rows 4(l/16)+i, column
+// l%16) is already the B-operand layout of the next layer (lane l: k rows
+// 4(l/16)+i, column l%16): bias + SiLU + BF16 rounding happen in registers
+// and no LDS shuffle is needed between layers.  A-operand weight tiles
+// (lane l: row l%16, k 4(l/16)+i) are 4 consecutive BF16 of one weight row,
+// loaded once per wave; all four layers' weights stay in ~88 VGPRs.
+//
+// Parameter block (FP32 master copy, same layout in BF16 for the weights):
+//   W1[64][32] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[16][64] b4[16]
+// with rows 3..15 of W4 / b4 zero (three outputs).
+#include "ssim_internal.h"
+
+namespace {
+
+typedef short s4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+constexpr int NIN = 32, NH = 64, NOUT = 16, NOUT_USED = 3;
+constexpr int OFF_W1 = 0;
+constexpr int OFF_B1 = OFF_W1 + NH * NIN;
+constexpr int OFF_W2 = OFF_B1 + NH;
+constexpr int OFF_B2 = OFF_W2 + NH * NH;
+constexpr int OFF_W3 = OFF_B2 + NH;
+constexpr int OFF_B3 = OFF_W3 + NH * NH;
+constexpr int OFF_W4 = OFF_B3 + NH;
+constexpr int OFF_B4 = OFF_W4 + NOUT * NH;
+constexpr int NPARAM = OFF_B4 + NOUT;
+
+constexpr int WAVES = 4;  // waves per workgroup
+constexpr int MB = 256;
+
+// FP32 -> BF16 bits, round to nearest even (finite inputs).
+__device__ __forceinline__ unsigned short bf16_bits(float f) {
+  unsigned int u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_float(unsigned short b) {
+  return __uint_as_float(((unsigned int)b) << 16);
+}
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+__device__ __forceinline__ f4_t mfma(s4_t a, s4_t b, f4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// A tile (row block rt, k block kb) of a row-major weight matrix [rows][K].
+__device__ __forceinline__ s4_t load_a(const unsigned short* __restrict__ w,
+                                       int K, int rt, int kb, int lane) {
+  const unsigned short* p = w + (size_t)(16 * rt + (lane & 15)) * K + 16 * kb +
+                            4 * (lane >> 4);
+  s4_t v;
+  v.x = (short)p[0];
+  v.y = (short)p[1];
+  v.z = (short)p[2];
+  v.w = (short)p[3];
+  return v;
+}
+
+struct Weights {
+  s4_t w1[4][2], w2[4][4], w3[4][4], w4[4];
+};
+
+__device__ __forceinline__ void load_weights(const unsigned short* __restrict__ wb,
+                                             int lane, Weights& W) {
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) W.w1[rt][kb] = load_a(wb + OFF_W1, NIN, rt, kb, lane);
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      W.w2[rt][kb] = load_a(wb + OFF_W2, NH, rt, kb, lane);
+      W.w3[rt][kb] = load_a(wb + OFF_W3, NH, rt, kb, lane);
+    }
+    W.w4[rt] = load_a(wb + OFF_W4, NH, 0, rt, lane);
+  }
+}
+
+// bias + SiLU on one accumulator tile; returns the BF16 B operand of the
+// next layer; optionally records Z (FP32) and H (BF16) rows of the body.
+__device__ __forceinline__ s4_t activate(f4_t acc, const float* __restrict__ bias,
+                                         int rt, int lane, float* zrow,
+                                         unsigned short* hrow) {
+  s4_t out;
+  short o[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = 16 * rt + 4 * (lane >> 4) + i;
+    const float z = acc[i] + bias[r];
+    const unsigned short hb = bf16_bits(silu(z));
+    o[i] = (short)hb;
+    if (zrow) zrow[r] = z;
+    if (hrow) hrow[r] = hb;
+  }
+  out.x = o[0];
+  out.y = o[1];
+  out.z = o[2];
+  out.w = o[3];
+  return out;
+}
+
+// Forward pass of 16 bodies (columns) starting at b0.  x: BF16 features
+// [nb][32].  Writes the 3 outputs (unscaled) and, when zs/hs != nullptr,
+// the layer activations for training: zs[l][body][64] FP32, hs[l][body][64]
+// BF16, l = 0..2.
+__device__ void forward16(const Weights& W, const float* __restrict__ bias,
+                          const unsigned short* __restrict__ x, int nb, int b0,
+                          int lane, float out3[3], bool& valid, float* zs,
+                          unsigned short* hs) {
+  const int body = b0 + (lane & 15);
+  valid = body < nb;
+  s4_t xb[2];
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) {
+    if (valid) {
+      const unsigned short* p = x + (size_t)body * NIN + 16 * kb + 4 * (lane >> 4);
+      xb[kb].x = (short)p[0];
+      xb[kb].y = (short)p[1];
+      xb[kb].z = (short)p[2];
+      xb[kb].w = (short)p[3];
+    } else {
+      xb[kb] = s4_t{0, 0, 0, 0};
+    }
+  }
+  const size_t zoff = (size_t)body * NH;
+  const size_t lstride = (size_t)nb * NH;
+  s4_t h1[4], h2[4], h3[4];
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    f4_t acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma(W.w1[rt][0], xb[0], acc);
+    acc = mfma(W.w1[rt][1], xb[1], acc);
+    h1[rt] = activate(acc, bias + OFF_B1, rt, lane,
+                      (zs && valid) ? zs + zoff : nullptr,
+                      (hs && valid) ? hs + zoff : nullptr);
+  }
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    f4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) acc = mfma(W.w2[rt][kb], h1[kb], acc);
+    h2[rt] = activate(acc, bias + OFF_B2, rt, lane,
+                      (zs && valid) ? zs + lstride + zoff : nullptr,
+                      (hs && valid) ? hs + lstride + zoff : nullptr);
+  }
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    f4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) acc = mfma(W.w3[rt][kb], h2[kb], acc);
+    h3[rt] = activate(acc, bias + OFF_B3, rt, lane,
+                      (zs && valid) ? zs + 2 * lstride + zoff : nullptr,
+                      (hs && valid) ? hs + 2 * lstride + zoff : nullptr);
+  }
+  f4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) acc = mfma(W.w4[kb], h3[kb], acc);
+  // rows 0..2 live in lanes 0..15 (4(l/16) + i with l/16 == 0)
+#pragma unroll
+  for (int i = 0; i < 3; ++i) out3[i] = acc[i] + bias[OFF_B4 + i];
+}
+
+__global__ __launch_bounds__(64 * WAVES) void k_mlp_forward(
+    int nb, const unsigned short* __restrict__ x,
+    const unsigned short* __restrict__ wbf, const float* __restrict__ params,
+    float scale, float* __restrict__ out, double* __restrict__ a0,
+    double* __restrict__ a1, double* __restrict__ a2, float* __restrict__ zs,
+    unsigned short* __restrict__ hs) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  const int nwaves = gridDim.x * WAVES;
+  Weights W;
+  load_weights(wbf, lane, W);
+  for (int b0 = wave * 16; b0 < nb; b0 += nwaves * 16) {
+    float o[3];
+    bool valid;
+    forward16(W, params, x, nb, b0, lane, o, valid, zs, hs);
+    if (lane < 16 && valid) {
+      const int body = b0 + lane;
+      if (out) {
+        out[(size_t)body * 3 + 0] = o[0];
+        out[(size_t)body * 3 + 1] = o[1];
+        out[(size_t)body * 3 + 2] = o[2];
+      }
+      if (a0) {
+        a0[body] += (double)(scale * o[0]);
+        a1[body] += (double)(scale * o[1]);
+        a2[body] += (double)(scale * o[2]);
+      }
+    }
+  }
+}
+
+// Backward through the layers for 16 bodies: given the stored activations
+// and dL/dout (3 per body), writes delta_l (BF16, [body][64]) for l = 1..3
+// (pre-activation gradients) used by the weight-gradient kernel.
+__global__ __launch_bounds__(64 * WAVES) void k_mlp_backward(
+    int nb, const unsigned short* __restrict__ wbf,
+    const float* __restrict__ zs, const float* __restrict__ gout,
+    unsigned short* __restrict__ deltas) {
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  const int nwaves = gridDim.x * WAVES;
+  // transposed weights as A operands: A[row = in][k = out] = W[out][in]
+  s4_t w4t[4], w3t[4][4], w2t[4][4];
+  for (int rt = 0; rt < 4; ++rt) {
+    const int in = 16 * rt + (lane & 15);
+    short v[4];
+    for (int i = 0; i < 4; ++i)
+      v[i] = (short)wbf[OFF_W4 + (size_t)(4 * (lane >> 4) + i) * NH + in];
+    w4t[rt] = s4_t{v[0], v[1], v[2], v[3]};
+    for (int kb = 0; kb < 4; ++kb) {
+      short u[4], q[4];
+      for (int i = 0; i < 4; ++i) {
+        const int o = 16 * kb + 4 * (lane >> 4) + i;
+        u[i] = (short)wbf[OFF_W3 + (size_t)o * NH + in];
+        q[i] = (short)wbf[OFF_W2 + (size_t)o * NH + in];
+      }
+      w3t[rt][kb] = s4_t{u[0], u[1], u[2], u[3]};
+      w2t[rt][kb] = s4_t{q[0], q[1], q[2], q[3]};
+    }
+  }
+  const size_t lstride = (size_t)nb * NH;
+  for (int b0 = wave * 16; b0 < nb; b0 += nwaves * 16) {
+    const int body = b0 + (lane & 15);
+    const bool valid = body < nb;
+    // delta4^T: rows = outputs (0..2 used), lane l holds rows 4(l/16)+i
+    short g4[4];
+    for (int i = 0; i < 4; ++i) {
+      const int r = 4 * (lane >> 4) + i;
+      const float g = (valid && r < NOUT_USED) ? gout[(size_t)body * 3 + r] : 0.f;
+      g4[i] = (short)bf16_bits(g);
+    }
+    const s4_t d4 = s4_t{g4[0], g4[1], g4[2], g4[3]};
+    s4_t d3[4], d2[4];
+    const float* z3 = zs + 2 * lstride + (size_t)body * NH;
+    const float* z2 = zs + lstride + (size_t)body * NH;
+    const float* z1 = zs + (size_t)body * NH;
+    for (int rt = 0; rt < 4; ++rt) {
+      f4_t acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mfma(w4t[rt], d4, acc);
+      short o[4];
+      for (int i = 0; i < 4; ++i) {
+        const int r = 16 * rt + 4 * (lane >> 4) + i;
+        const float d = valid ? acc[i] * silu_grad(z3[r]) : 0.f;
+        const unsigned short db = bf16_bits(d);
+        o[i] = (short)db;
+        if (valid) deltas[2 * lstride + (size_t)body * NH + r] = db;
+      }
+      d3[rt] = s4_t{o[0], o[1], o[2], o[3]};
+    }
+    for (int rt = 0; rt < 4; ++rt) {
+      f4_t acc = {0.f, 0.f, 0.f, 0.f};
+      for (int kb = 0; kb < 4; ++kb) acc = mfma(w3t[rt][kb], d3[kb], acc);
+      short o[4];
+      for (int i = 0; i < 4; ++i) {
+        const int r = 16 * rt + 4 * (lane >> 4) + i;
+        const float d = valid ? acc[i] * silu_grad(z2[r]) : 0.f;
+        const unsigned short db = bf16_bits(d);
+        o[i] = (short)db;
+        if (valid) deltas[lstride + (size_t)body * NH + r] = db;
+      }
+      d2[rt] = s4_t{o[0], o[1], o[2], o[3]};
+    }
+    for (int rt = 0; rt < 4; ++rt) {
+      f4_t acc = {0.f, 0.f, 0.f, 0.f};
+      for (int kb = 0; kb < 4; ++kb) acc = mfma(w2t[rt][kb], d2[kb], acc);
+      for (int i = 0; i < 4; ++i) {
+        const int r = 16 * rt + 4 * (lane >> 4) + i;
+        const float d = valid ? acc[i] * silu_grad(z1[r]) : 0.f;
+        if (valid) deltas[(size_t)body * NH + r] = bf16_bits(d);
+      }
+    }
+  }
+}
+
+// Weight gradients of one layer, dW[out][in] = sum_b d[b][out] h[b][in],
+// db[out] = sum_b d[b][out].  Workgroup g takes bodies [g*chunk, ...), each
+// of its waves every WAVES-th 16-body block; wave partials go to
+// part[g*WAVES + w][(out*K + in) | bias] and are summed in a fixed order by
+// k_mlp_wgrad_reduce.  d: BF16 [nb][dstride] (FP32 for the output layer),
+// h: BF16 [nb][K], the layer's inputs.
+template <int NO, int K, bool DFLOAT>
+__global__ __launch_bounds__(64 * WAVES) void k_mlp_wgrad(
+    int nb, int chunk, const void* __restrict__ dv, int dstride,
+    const unsigned short* __restrict__ h, float* __restrict__ part) {
+  constexpr int RT = (NO + 15) / 16, CT = K / 16;
+  constexpr int TOTAL = RT * 16 * K + RT * 16;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int c0 = blockIdx.x * chunk;
+  const int c1 = min(nb, c0 + chunk);
+  f4_t acc[RT][CT];
+  float dbias[RT];
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) {
+    dbias[rt] = 0.f;
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) acc[rt][ct] = f4_t{0.f, 0.f, 0.f, 0.f};
+  }
+  for (int b0 = c0 + 16 * w; b0 < c1; b0 += 16 * WAVES) {
+    // A[row = out][k = body]: lane holds 4 bodies of one output
+    // B[k = body][col = in]:  lane holds 4 bodies of one input
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      const int o = 16 * rt + (lane & 15);
+      short av[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int b = b0 + 4 * (lane >> 4) + i;
+        float d = 0.f;
+        if (b < c1 && o < NO) {
+          if (DFLOAT)
+            d = static_cast<const float*>(dv)[(size_t)b * dstride + o];
+          else
+            d = bf16_float(static_cast<const unsigned short*>(dv)[(size_t)b * dstride + o]);
+        }
+        av[i] = (short)bf16_bits(d);
+        dbias[rt] += bf16_float((unsigned short)av[i]);
+      }
+      const s4_t a = s4_t{av[0], av[1], av[2], av[3]};
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) {
+        const int in = 16 * ct + (lane & 15);
+        short bv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int b = b0 + 4 * (lane >> 4) + i;
+          bv[i] = (b < c1) ? (short)h[(size_t)b * K + in] : (short)0;
+        }
+        acc[rt][ct] = mfma(a, s4_t{bv[0], bv[1], bv[2], bv[3]}, acc[rt][ct]);
+      }
+    }
+  }
+  float* r = part + (size_t)(blockIdx.x * WAVES + w) * TOTAL;
+#pragma unroll
+  for (int rt = 0; rt < RT; ++rt) {
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int o = 16 * rt + 4 * (lane >> 4) + i;
+        const int in = 16 * ct + (lane & 15);
+        r[o * K + in] = acc[rt][ct][i];
+      }
+    // bias: lanes l, l^16, l^32, l^48 hold the same output
+    float s = dbias[rt];
+    s += __shfl_xor(s, 16);
+    s += __shfl_xor(s, 32);
+    if (lane < 16) r[RT * 16 * K + 16 * rt + lane] = s;
+  }
+}
+
+// Sum the chunk partials in order and scatter into the gradient block.
+__global__ __launch_bounds__(MB) void k_mlp_wgrad_reduce(
+    int nchunks, int total, int NOv, int K, int off_w, int off_b,
+    const float* __restrict__ part, float* __restrict__ grad) {
+  const int e = blockIdx.x * MB + threadIdx.x;
+  if (e >= total) return;
+  float s = 0.f;
+  for (int c = 0; c < nchunks; ++c) s += part[(size_t)c * total + e];
+  const int wsz = ((NOv + 15) / 16) * 16 * K;
+  if (e < wsz) {
+    const int o = e / K, in = e % K;
+    if (o < NOv) grad[off_w + o * K + in] = s;
+  } else {
+    const int o = e - wsz;
+    if (o < NOv) grad[off_b + o] = s;
+  }
+}
+
+// Adam step on the FP32 master parameters, then refresh the BF16 copy.
+__global__ __launch_bounds__(MB) void k_adam(int n, float* __restrict__ p,
+                                             const float* __restrict__ g,
+                                             float* __restrict__ m,
+                                             float* __restrict__ v, float lr,
+                                             float b1, float b2, float eps,
+                                             float c1, float c2,
+                                             unsigned short* __restrict__ wbf) {
+  const int i = blockIdx.x * MB + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float upd = lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  const float np = p[i] - upd;
+  p[i] = np;
+  wbf[i] = bf16_bits(np);
+}
+
+__global__ __launch_bounds__(MB) void k_to_bf16(int n, const float* __restrict__ p,
+                                                unsigned short* __restrict__ wbf) {
+  const int i = blockIdx.x * MB + threadIdx.x;
+  if (i < n) wbf[i] = bf16_bits(p[i]);
+}
+
+// Features of body p: the 27 node masses (/ m_unit) of the 3x3x3 stencil
+// around its nearest mesh node, then its offset from that node in cells
+// (times 2, in [-1, 1]), 2 zeros; BF16 [nb][32].
+__global__ __launch_bounds__(MB) void k_mlp_features(
+    int nb, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, int n, double ox, double oy, double oz,
+    double h, const float* __restrict__ rho32, const double* __restrict__ rho64,
+    double inv_unit, unsigned short* __restrict__ feat) {
+  const int p = blockIdx.x * MB + threadIdx.x;
+  if (p >= nb) return;
+  const double f[3] = {(x[p] - ox) / h, (y[p] - oy) / h, (z[p] - oz) / h};
+  int c[3];
+  float off[3];
+  for (int a = 0; a < 3; ++a) {
+    const double r = floor(f[a] + 0.5);
+    c[a] = min(max((int)r, 0), n - 1);
+    off[a] = (float)(2.0 * (f[a] - r));
+  }
+  unsigned short* o = feat + (size_t)p * NIN;
+  int q = 0;
+  for (int da = -1; da <= 1; ++da)
+    for (int db = -1; db <= 1; ++db)
+      for (int dc = -1; dc <= 1; ++dc, ++q) {
+        const int i = c[0] + da, j = c[1] + db, k = c[2] + dc;
+        double v = 0.0;
+        if (i >= 0 && i < n && j >= 0 && j < n && k >= 0 && k < n) {
+          const size_t idx = ((size_t)i * n + j) * n + k;
+          v = rho64 ? rho64[idx] : (double)rho32[idx];
+        }
+        o[q] = bf16_bits((float)(v * inv_unit));
+      }
+  o[27] = bf16_bits(off[0]);
+  o[28] = bf16_bits(off[1]);
+  o[29] = bf16_bits(off[2]);
+  o[30] = 0;
+  o[31] = 0;
+}
+
+int grid_waves(int nb) {
+  // enough waves for all 256 CUs but no more than one 16-body batch each
+  const int batches = ssim::ceil_div(nb, 16);
+  int wg = ssim::ceil_div(batches, WAVES);
+  return wg < 1 ? 1 : (wg > 4096 ? 4096 : wg);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssim_mlp_param_count(void) { return NPARAM; }
+
+int ssim_mlp_to_bf16(const float* params, unsigned short* wbf, void* stream) {
+  SSIM_REQUIRE(params && wbf, "null pointer");
+  hipLaunchKernelGGL(k_to_bf16, dim3(ssim::ceil_div(NPARAM, MB)), dim3(MB), 0,
+                     ssim::as_stream(stream), NPARAM, params, wbf);
+  SSIM_LAUNCHED("k_to_bf16");
+  return SSIM_OK;
+}
+
+int ssim_mlp_features(int nb, const double* x, const double* y, const double* z,
+                      int mesh_n, const double* origin, double h,
+                      int mesh_prec, const void* rho, double mass_unit,
+                      unsigned short* feat, void* stream) {
+  SSIM_REQUIRE(nb >= 0 && mesh_n >= 1 && h > 0.0 && mass_unit > 0.0,
+               "bad feature arguments");
+  if (nb == 0) return SSIM_OK;
+  SSIM_REQUIRE(x && y && z && origin && rho && feat, "null pointer");
+  const bool f64 = mesh_prec == SSIM_PREC_F64;
+  hipLaunchKernelGGL(k_mlp_features, dim3(ssim::ceil_div(nb, MB)), dim3(MB), 0,
+                     ssim::as_stream(stream), nb, x, y, z, mesh_n, origin[0],
+                     origin[1], origin[2], h,
+                     f64 ? nullptr : static_cast<const float*>(rho),
+                     f64 ? static_cast<const double*>(rho) : nullptr,
+                     1.0 / mass_unit, feat);
+  SSIM_LAUNCHED("k_mlp_features");
+  return SSIM_OK;
+}
+
+int ssim_mlp_forward(int nb, const unsigned short* feat,
+                     const unsigned short* wbf, const float* params,
+                     float scale, float* out, double* ax, double* ay,
+                     double* az, float* zs, unsigned short* hs, void* stream) {
+  SSIM_REQUIRE(nb >= 0, "nb must be >= 0");
+  if (nb == 0) return SSIM_OK;
+  SSIM_REQUIRE(feat && wbf && params, "null pointer");
+  SSIM_REQUIRE(!ax || (ay && az), "need all three acceleration arrays");
+  SSIM_REQUIRE((zs == nullptr) == (hs == nullptr), "zs and hs go together");
+  hipLaunchKernelGGL(k_mlp_forward, dim3(grid_waves(nb)), dim3(64 * WAVES), 0,
+                     ssim::as_stream(stream), nb, feat, wbf, params, scale, out,
+                     ax, ay, az, zs, hs);
+  SSIM_LAUNCHED("k_mlp_forward");
+  return SSIM_OK;
+}
+
+size_t ssim_mlp_train_work_bytes(int nb) {
+  const size_t nbs = (size_t)(nb > 0 ? nb : 1);
+  const size_t nchunks = (size_t)ssim::ceil_div(nbs, 2048);
+  const size_t part = nchunks * WAVES * (size_t)(64 * 64 + 64) * sizeof(float);
+  return ssim::align256(3 * nbs * NH * 2) + ssim::align256(part);
+}
+
+int ssim_mlp_backward(int nb, const unsigned short* feat,
+                      const unsigned short* wbf, const float* zs,
+                      const unsigned short* hs, const float* gout, float* grad,
+                      void* work, size_t work_bytes, void* stream) {
+  SSIM_REQUIRE(nb >= 1, "empty batch");
+  SSIM_REQUIRE(feat && wbf && zs && hs && gout && grad && work, "null pointer");
+  SSIM_REQUIRE(work_bytes >= ssim_mlp_train_work_bytes(nb), "workspace too small");
+  hipStream_t st = ssim::as_stream(stream);
+  unsigned short* deltas = static_cast<unsigned short*>(work);
+  float* part = reinterpret_cast<float*>(static_cast<unsigned char*>(work) +
+                                         ssim::align256((size_t)3 * nb * NH * 2));
+  hipLaunchKernelGGL(k_mlp_backward, dim3(grid_waves(nb)), dim3(64 * WAVES), 0,
+                     st, nb, wbf, zs, gout, deltas);
+  SSIM_LAUNCHED("k_mlp_backward");
+  const int chunk = 2048;
+  const int nch = ssim::ceil_div(nb, chunk);
+  const size_t lstride = (size_t)nb * NH;
+  // layer 4: d = gout (FP32, 3 used), h = H3
+  {
+    const int total = 16 * NH + 16;
+    hipLaunchKernelGGL((k_mlp_wgrad<NOUT_USED, NH, true>), dim3(nch),
+                       dim3(64 * WAVES), 0, st, nb, chunk, gout, 3,
+                       hs + 2 * lstride, part);
+    hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3(ssim::ceil_div(total, MB)),
+                       dim3(MB), 0, st, nch * WAVES, total, NOUT_USED, NH, OFF_W4,
+                       OFF_B4, part, grad);
+  }
+  // layers 3, 2: d = delta_l, h = H_{l-1}
+  const int total_h = NH * NH + NH;
+  hipLaunchKernelGGL((k_mlp_wgrad<NH, NH, false>), dim3(nch), dim3(64 * WAVES),
+                     0, st, nb, chunk, deltas + 2 * lstride, NH, hs + lstride,
+                     part);
+  hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3(ssim::ceil_div(total_h, MB)),
+                     dim3(MB), 0, st, nch * WAVES, total_h, NH, NH, OFF_W3, OFF_B3, part,
+                     grad);
+  hipLaunchKernelGGL((k_mlp_wgrad<NH, NH, false>), dim3(nch), dim3(64 * WAVES),
+                     0, st, nb, chunk, deltas + lstride, NH, hs, part);
+  hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3(ssim::ceil_div(total_h, MB)),
+                     dim3(MB), 0, st, nch * WAVES, total_h, NH, NH, OFF_W2, OFF_B2, part,
+                     grad);
+  // layer 1: d = delta_1, h = features
+  const int total_1 = NH * NIN + NH;
+  hipLaunchKernelGGL((k_mlp_wgrad<NH, NIN, false>), dim3(nch), dim3(64 * WAVES),
+                     0, st, nb, chunk, deltas, NH, feat, part);
+  hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3(ssim::ceil_div(total_1, MB)),
+                     dim3(MB), 0, st, nch * WAVES, total_1, NH, NIN, OFF_W1, OFF_B1,
+                     part, grad);
+  SSIM_LAUNCHED("k_mlp_wgrad");
+  return SSIM_OK;
+}
+
+int ssim_mlp_adam(float* params, const float* grad, float* m, float* v,
+                  float lr, float beta1, float beta2, float eps, int step,
+                  unsigned short* wbf, void* stream) {
+  SSIM_REQUIRE(params && grad && m && v && wbf && step >= 1, "bad Adam args");
+  const float c1 = 1.f - powf(beta1, (float)step);
+  const float c2 = 1.f - powf(beta2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(ssim::ceil_div(NPARAM, MB)), dim3(MB), 0,
+                     ssim::as_stream(stream), NPARAM, params, grad, m, v, lr,
+                     beta1, beta2, eps, c1, c2, wbf);
+  SSIM_LAUNCHED("k_adam");
+  return SSIM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// gout = 2 (out - y) / nb  and per-block partial sums of (out - y)^2.
+__global__ __launch_bounds__(MB) void k_mse_grad(int nb, const float* __restrict__ out,
+                                                 const float* __restrict__ y,
+                                                 float* __restrict__ gout,
+                                                 float* __restrict__ loss_part) {
+  __shared__ float red[MB];
+  const int i = blockIdx.x * MB + threadIdx.x;
+  float l = 0.f;
+  if (i < nb) {
+    for (int c = 0; c < 3; ++c) {
+      const float d = out[(size_t)i * 3 + c] - y[(size_t)i * 3 + c];
+      gout[(size_t)i * 3 + c] = 2.0f * d / (float)nb;
+      l += d * d;
+    }
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int w = MB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = red[0];
+}
+
+}  // namespace
+
+extern "C" int ssim_mlp_mse_grad(int nb, const float* out, const float* target,
+                                 float* gout, float* loss_part, void* stream) {
+  SSIM_REQUIRE(nb >= 1 && out && target && gout && loss_part, "bad arguments");
+  hipLaunchKernelGGL(k_mse_grad, dim3(ssim::ceil_div(nb, MB)), dim3(MB), 0,
+                     ssim::as_stream(stream), nb, out, target, gout, loss_part);
+  SSIM_LAUNCHED("k_mse_grad");
+  return SSIM_OK;
+}

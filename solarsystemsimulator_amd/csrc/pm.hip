@@ -1,0 +1,576 @@
+// Particle-mesh building blocks on gfx950: cloud-in-cell deposit and
+// gather, min/max reduction for the adaptive grid, Green's functions in real
+// and Fourier space, spectral products and the central-difference gradient.
+//
+// Mesh layout: C order with the x cell index slowest, flat index
+// ((i * n1) + j) * n2 + k — the reference's `i * grid_n + j` in 2-D
+// (pm_kernel.py:77-82).  Mesh values are MASS per node (what
+// `cp.bincount` returns there); the Poisson Green's function divides by the
+// cell volume.
+//
+// CIC cell indices are computed exactly like the reference: fx = (x - x0)/h
+// in IEEE FP64 (the library is built without FMA contraction), i =
+// floor(fx), clipped to [0, n-2] for an isolated mesh (pm_kernel.py:66-75)
+// or wrapped mod n for a periodic one; the weights use the clipped index
+// (tx = fx - i).  The products keep the reference's operand order
+// (m * (1-tx)) * (1-ty) ... and the gather sums the corners in its order.
+#include <cmath>
+
+#include "ssim_internal.h"
+
+namespace {
+
+constexpr int PM_BLOCK = 256;
+
+struct Grid {
+  int n[3];       // n0 (x), n1 (y), n2 (z; 1 in 2-D)
+  double o[3];    // origin
+  double h;
+  int periodic;
+};
+
+__device__ __forceinline__ void cell_of(double x, double o, double h, int n,
+                                        int periodic, int& i, int& i1,
+                                        double& t) {
+  const double f = (x - o) / h;
+  int c = (int)floor(f);
+  if (periodic) {
+    t = f - (double)c;
+    c %= n;
+    if (c < 0) c += n;
+    i = c;
+    i1 = (c + 1 == n) ? 0 : c + 1;
+  } else {
+    c = c < 0 ? 0 : (c > n - 2 ? n - 2 : c);
+    t = f - (double)c;
+    i = c;
+    i1 = c + 1;
+  }
+}
+
+template <typename M>
+__device__ __forceinline__ void atomic_add(M* p, M v) {
+  unsafeAtomicAdd(p, v);
+}
+
+template <int D, typename M>
+__global__ __launch_bounds__(PM_BLOCK) void k_cic_deposit(
+    int np, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ m, Grid g,
+    M* __restrict__ mesh, int* __restrict__ cells) {
+  const int p = blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (p >= np) return;
+  int i, i1, j, j1, k = 0, k1 = 0;
+  double tx, ty, tz = 0.0;
+  cell_of(x[p], g.o[0], g.h, g.n[0], g.periodic, i, i1, tx);
+  cell_of(y[p], g.o[1], g.h, g.n[1], g.periodic, j, j1, ty);
+  if (D == 3) cell_of(z[p], g.o[2], g.h, g.n[2], g.periodic, k, k1, tz);
+  if (cells) {
+    cells[(size_t)D * p] = i;
+    cells[(size_t)D * p + 1] = j;
+    if (D == 3) cells[(size_t)D * p + 2] = k;
+  }
+  const double w = m[p];
+  const double ux = 1.0 - tx, uy = 1.0 - ty;
+  const long long n1 = g.n[1], n2 = g.n[2];
+  if (D == 2) {
+    atomic_add(&mesh[i * n1 + j], (M)(w * ux * uy));
+    atomic_add(&mesh[i1 * n1 + j], (M)(w * tx * uy));
+    atomic_add(&mesh[i * n1 + j1], (M)(w * ux * ty));
+    atomic_add(&mesh[i1 * n1 + j1], (M)(w * tx * ty));
+  } else {
+    const double uz = 1.0 - tz;
+    const double a00 = w * ux * uy, a10 = w * tx * uy;
+    const double a01 = w * ux * ty, a11 = w * tx * ty;
+    atomic_add(&mesh[(i * n1 + j) * n2 + k], (M)(a00 * uz));
+    atomic_add(&mesh[(i1 * n1 + j) * n2 + k], (M)(a10 * uz));
+    atomic_add(&mesh[(i * n1 + j1) * n2 + k], (M)(a01 * uz));
+    atomic_add(&mesh[(i1 * n1 + j1) * n2 + k], (M)(a11 * uz));
+    atomic_add(&mesh[(i * n1 + j) * n2 + k1], (M)(a00 * tz));
+    atomic_add(&mesh[(i1 * n1 + j) * n2 + k1], (M)(a10 * tz));
+    atomic_add(&mesh[(i * n1 + j1) * n2 + k1], (M)(a01 * tz));
+    atomic_add(&mesh[(i1 * n1 + j1) * n2 + k1], (M)(a11 * tz));
+  }
+}
+
+// out_c[p] = sum over the CIC corners of field_c * weight, c < nf.
+template <int D, typename M>
+__global__ __launch_bounds__(PM_BLOCK) void k_cic_gather(
+    int np, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, Grid g, int nf, const M* __restrict__ f0,
+    const M* __restrict__ f1, const M* __restrict__ f2, double* __restrict__ o0,
+    double* __restrict__ o1, double* __restrict__ o2) {
+  const int p = blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (p >= np) return;
+  int i, i1, j, j1, k = 0, k1 = 0;
+  double tx, ty, tz = 0.0;
+  cell_of(x[p], g.o[0], g.h, g.n[0], g.periodic, i, i1, tx);
+  cell_of(y[p], g.o[1], g.h, g.n[1], g.periodic, j, j1, ty);
+  if (D == 3) cell_of(z[p], g.o[2], g.h, g.n[2], g.periodic, k, k1, tz);
+  const double ux = 1.0 - tx, uy = 1.0 - ty;
+  const long long n1 = g.n[1], n2 = g.n[2];
+  const M* fs[3] = {f0, f1, f2};
+  double* os[3] = {o0, o1, o2};
+  for (int c = 0; c < nf; ++c) {
+    const M* f = fs[c];
+    double v;
+    if (D == 2) {
+      v = (double)f[i * n1 + j] * ux * uy + (double)f[i1 * n1 + j] * tx * uy +
+          (double)f[i * n1 + j1] * ux * ty + (double)f[i1 * n1 + j1] * tx * ty;
+    } else {
+      const double uz = 1.0 - tz;
+      v = (double)f[(i * n1 + j) * n2 + k] * ux * uy * uz +
+          (double)f[(i1 * n1 + j) * n2 + k] * tx * uy * uz +
+          (double)f[(i * n1 + j1) * n2 + k] * ux * ty * uz +
+          (double)f[(i1 * n1 + j1) * n2 + k] * tx * ty * uz +
+          (double)f[(i * n1 + j) * n2 + k1] * ux * uy * tz +
+          (double)f[(i1 * n1 + j) * n2 + k1] * tx * uy * tz +
+          (double)f[(i * n1 + j1) * n2 + k1] * ux * ty * tz +
+          (double)f[(i1 * n1 + j1) * n2 + k1] * tx * ty * tz;
+    }
+    os[c][p] = v;
+  }
+}
+
+// Per-block min/max of up to 3 arrays -> part[block][2*c + {0,1}].
+__global__ __launch_bounds__(PM_BLOCK) void k_minmax_partial(
+    int n, int na, const double* __restrict__ a0, const double* __restrict__ a1,
+    const double* __restrict__ a2, double* __restrict__ part) {
+  __shared__ double smin[3][PM_BLOCK], smax[3][PM_BLOCK];
+  const double* as[3] = {a0, a1, a2};
+  for (int c = 0; c < na; ++c) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int i = blockIdx.x * PM_BLOCK + threadIdx.x; i < n;
+         i += gridDim.x * PM_BLOCK) {
+      const double v = as[c][i];
+      lo = fmin(lo, v);
+      hi = fmax(hi, v);
+    }
+    smin[c][threadIdx.x] = lo;
+    smax[c][threadIdx.x] = hi;
+  }
+  __syncthreads();
+  for (int w = PM_BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int c = 0; c < na; ++c) {
+        smin[c][threadIdx.x] = fmin(smin[c][threadIdx.x], smin[c][threadIdx.x + w]);
+        smax[c][threadIdx.x] = fmax(smax[c][threadIdx.x], smax[c][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < na; ++c) {
+      part[(size_t)blockIdx.x * 6 + 2 * c] = smin[c][0];
+      part[(size_t)blockIdx.x * 6 + 2 * c + 1] = smax[c][0];
+    }
+}
+
+__global__ void k_minmax_final(int nblk, int na, const double* __restrict__ part,
+                               double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  for (int c = 0; c < na; ++c) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int b = 0; b < nblk; ++b) {
+      lo = fmin(lo, part[(size_t)b * 6 + 2 * c]);
+      hi = fmax(hi, part[(size_t)b * 6 + 2 * c + 1]);
+    }
+    out[2 * c] = lo;
+    out[2 * c + 1] = hi;
+  }
+}
+
+// Real-space force kernel on a zero-padded (2N)^D mesh: K_c(d) =
+// -G d_c / (|d|^2 + eps2)^(3/2), d = off * h with off = idx < N ? idx :
+// idx - 2N (build_force_kernels, pm_kernel.py:42-59).  Component c of D.
+// With rs > 0 only the long-range part is kept: K_c * (1 - g(|d|)),
+// g(r) = erfc(r/2rs) + r/(rs sqrt(pi)) exp(-r^2/4rs^2) (the complement is
+// summed directly by the near-field kernel).
+template <typename M>
+__global__ __launch_bounds__(PM_BLOCK) void k_force_kernel(
+    int D, int N, int comp, double h, double eps2, double G, double rs,
+    M* __restrict__ out) {
+  const long long P = 2LL * N;
+  const long long total = (D == 2) ? P * P : P * P * P;
+  const long long t = (long long)blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (t >= total) return;
+  long long idx[3];
+  if (D == 2) {
+    idx[0] = t / P;
+    idx[1] = t % P;
+    idx[2] = 0;
+  } else {
+    idx[0] = t / (P * P);
+    idx[1] = (t / P) % P;
+    idx[2] = t % P;
+  }
+  double d[3];
+  for (int c = 0; c < 3; ++c) {
+    const double off = (double)(idx[c] < N ? idx[c] : idx[c] - P);
+    d[c] = (c < D) ? off * h : 0.0;
+  }
+  double r2 = d[0] * d[0] + d[1] * d[1];
+  if (D == 3) r2 = r2 + d[2] * d[2];
+  const double rr = r2;
+  r2 = r2 + eps2;
+  const double inv_r3 = 1.0 / (r2 * sqrt(r2));
+  double k = ((-G) * d[comp]) * inv_r3;
+  if (rs > 0.0) {
+    const double r = sqrt(rr);
+    const double x = r / (2.0 * rs);
+    const double g = erfc(x) + r / (rs * 1.7724538509055160273) * exp(-x * x);
+    k = k * (1.0 - g);
+  }
+  out[t] = (M)k;
+}
+
+// Divide a half spectrum on the padded P^D mesh by the squared CIC window,
+// prod_c sinc^4(pi f_c / P), undoing assignment + interpolation smoothing.
+template <typename M>
+__global__ __launch_bounds__(PM_BLOCK) void k_deconvolve(int D, int P,
+                                                         M* __restrict__ spec) {
+  const int W = P / 2 + 1;
+  const long long total = (D == 2) ? (long long)P * W : (long long)P * P * W;
+  const long long t = (long long)blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (t >= total) return;
+  int f[3];
+  f[D - 1] = (int)(t % W);
+  if (D == 2) {
+    const int a = (int)(t / W);
+    f[0] = a <= P / 2 ? a : a - P;
+  } else {
+    const int b = (int)((t / W) % P);
+    const int a = (int)(t / ((long long)W * P));
+    f[0] = a <= P / 2 ? a : a - P;
+    f[1] = b <= P / 2 ? b : b - P;
+  }
+  double w = 1.0;
+  for (int c = 0; c < D; ++c) {
+    if (f[c] == 0) continue;
+    const double x = M_PI * (double)f[c] / (double)P;
+    const double s = sin(x) / x;
+    w *= s * s * s * s;
+  }
+  spec[2 * t] = (M)(spec[2 * t] / w);
+  spec[2 * t + 1] = (M)(spec[2 * t + 1] / w);
+}
+
+// out = a * b on nc complex values (in place allowed).
+template <typename M>
+__global__ __launch_bounds__(PM_BLOCK) void k_cmul(long long nc,
+                                                   const M* __restrict__ a,
+                                                   const M* __restrict__ b,
+                                                   M* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (t >= nc) return;
+  const M ar = a[2 * t], ai = a[2 * t + 1];
+  const M br = b[2 * t], bi = b[2 * t + 1];
+  out[2 * t] = ar * br - ai * bi;
+  out[2 * t + 1] = ar * bi + ai * br;
+}
+
+// Periodic Poisson in Fourier space on the rfftn half spectrum:
+// phi_k = -4 pi G rho_k / k^2 with rho = mass / h^D, k = 2 pi f / (n h),
+// f the signed integer frequency; the k = 0 mode is set to 0.
+template <typename M>
+__global__ __launch_bounds__(PM_BLOCK) void k_poisson(int D, int n0, int n1,
+                                                      int n2, double h,
+                                                      double G, M* __restrict__ spec) {
+  const int nl = (D == 2) ? n1 : n2;    // last (halved) axis length
+  const int W = nl / 2 + 1;
+  const long long total = (D == 2) ? (long long)n0 * W : (long long)n0 * n1 * W;
+  const long long t = (long long)blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (t >= total) return;
+  int f[3];
+  const int c = (int)(t % W);
+  if (D == 2) {
+    const int a = (int)(t / W);
+    f[0] = a <= n0 / 2 ? a : a - n0;
+    f[1] = c;
+    f[2] = 0;
+  } else {
+    const int b = (int)((t / W) % n1);
+    const int a = (int)(t / ((long long)W * n1));
+    f[0] = a <= n0 / 2 ? a : a - n0;
+    f[1] = b <= n1 / 2 ? b : b - n1;
+    f[2] = c;
+  }
+  const int ns[3] = {n0, (D == 2) ? nl : n1, n2};
+  double k2 = 0.0;
+  for (int q = 0; q < D; ++q) {
+    const double k = 2.0 * M_PI * (double)f[q] / ((double)ns[q] * h);
+    k2 += k * k;
+  }
+  double vol = h * h;
+  if (D == 3) vol *= h;
+  double g = (k2 > 0.0) ? (-4.0 * M_PI * G) / (k2 * vol) : 0.0;
+  spec[2 * t] = (M)(spec[2 * t] * g);
+  spec[2 * t + 1] = (M)(spec[2 * t + 1] * g);
+}
+
+// Central difference a_c = -(phi[+1] - phi[-1]) / (2h), periodic wrap.
+template <typename M>
+__global__ __launch_bounds__(PM_BLOCK) void k_gradient(
+    int D, int n0, int n1, int n2, double h, const M* __restrict__ phi,
+    M* __restrict__ g0, M* __restrict__ g1, M* __restrict__ g2) {
+  const long long N = (long long)n0 * n1 * (D == 3 ? n2 : 1);
+  const long long t = (long long)blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (t >= N) return;
+  const int nz = (D == 3) ? n2 : 1;
+  const int k = (int)(t % nz);
+  const int j = (int)((t / nz) % n1);
+  const int i = (int)(t / ((long long)nz * n1));
+  auto at = [&](int a, int b, int c) -> double {
+    return (double)phi[((long long)a * n1 + b) * nz + c];
+  };
+  const double s = -1.0 / (2.0 * h);
+  const int ip = i + 1 == n0 ? 0 : i + 1, im = i == 0 ? n0 - 1 : i - 1;
+  const int jp = j + 1 == n1 ? 0 : j + 1, jm = j == 0 ? n1 - 1 : j - 1;
+  g0[t] = (M)((at(ip, j, k) - at(im, j, k)) * s);
+  g1[t] = (M)((at(i, jp, k) - at(i, jm, k)) * s);
+  if (D == 3) {
+    const int kp = k + 1 == nz ? 0 : k + 1, km = k == 0 ? nz - 1 : k - 1;
+    g2[t] = (M)((at(i, j, kp) - at(i, j, km)) * s);
+  }
+}
+
+// Copy an n^D block into the corner of a zeroed (2n)^D block (and back).
+template <typename M>
+__global__ __launch_bounds__(PM_BLOCK) void k_pad(int D, int n, const M* __restrict__ src,
+                                                  M* __restrict__ dst, int crop) {
+  const long long total = (D == 2) ? (long long)n * n : (long long)n * n * n;
+  const long long t = (long long)blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (t >= total) return;
+  const long long P = 2LL * n;
+  long long big;
+  if (D == 2)
+    big = (t / n) * P + (t % n);
+  else
+    big = ((t / ((long long)n * n)) * P + (t / n) % n) * P + (t % n);
+  if (crop)
+    dst[t] = src[big];
+  else
+    dst[big] = src[t];
+}
+
+Grid make_grid(int dim, const int* n, const double* origin, double h,
+               int periodic) {
+  Grid g{};
+  for (int c = 0; c < 3; ++c) {
+    g.n[c] = (c < dim) ? n[c] : 1;
+    g.o[c] = (c < dim) ? origin[c] : 0.0;
+  }
+  g.h = h;
+  g.periodic = periodic;
+  return g;
+}
+
+int check_grid(int dim, const int* n, double h, int periodic) {
+  SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3, got %d", dim);
+  SSIM_REQUIRE(n != nullptr, "null grid dims");
+  for (int c = 0; c < dim; ++c)
+    SSIM_REQUIRE(n[c] >= (periodic ? 1 : 2), "grid dimension %d too small", c);
+  SSIM_REQUIRE(h > 0.0, "cell size must be > 0, got %g", h);
+  return SSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssim_cic_deposit(int dim, int periodic, int np, const double* x,
+                     const double* y, const double* z, const double* m,
+                     const int* grid_n, const double* origin, double h,
+                     int mesh_prec, void* mesh, int* cells, void* stream) {
+  SSIM_TRY(check_grid(dim, grid_n, h, periodic));
+  SSIM_REQUIRE(np >= 0, "np must be >= 0");
+  if (np == 0) return SSIM_OK;
+  SSIM_REQUIRE(x && y && m && mesh && origin && (dim == 2 || z), "null pointer");
+  const Grid g = make_grid(dim, grid_n, origin, h, periodic);
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid(ssim::ceil_div(np, PM_BLOCK));
+#define LAUNCH_DEP(D, M)                                                     \
+  hipLaunchKernelGGL((k_cic_deposit<D, M>), grid, dim3(PM_BLOCK), 0, st, np, \
+                     x, y, z, m, g, static_cast<M*>(mesh), cells)
+  if (mesh_prec == SSIM_PREC_F64) {
+    if (dim == 2) LAUNCH_DEP(2, double); else LAUNCH_DEP(3, double);
+  } else {
+    if (dim == 2) LAUNCH_DEP(2, float); else LAUNCH_DEP(3, float);
+  }
+#undef LAUNCH_DEP
+  SSIM_LAUNCHED("k_cic_deposit");
+  return SSIM_OK;
+}
+
+int ssim_cic_gather(int dim, int periodic, int np, const double* x,
+                    const double* y, const double* z, const int* grid_n,
+                    const double* origin, double h, int mesh_prec, int nfields,
+                    const void* f0, const void* f1, const void* f2,
+                    double* out0, double* out1, double* out2, void* stream) {
+  SSIM_TRY(check_grid(dim, grid_n, h, periodic));
+  SSIM_REQUIRE(np >= 0 && nfields >= 1 && nfields <= 3, "bad np / nfields");
+  if (np == 0) return SSIM_OK;
+  SSIM_REQUIRE(x && y && origin && (dim == 2 || z) && f0 && out0 &&
+                   (nfields < 2 || (f1 && out1)) && (nfields < 3 || (f2 && out2)),
+               "null pointer");
+  const Grid g = make_grid(dim, grid_n, origin, h, periodic);
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid(ssim::ceil_div(np, PM_BLOCK));
+#define LAUNCH_GAT(D, M)                                                      \
+  hipLaunchKernelGGL((k_cic_gather<D, M>), grid, dim3(PM_BLOCK), 0, st, np, x, \
+                     y, z, g, nfields, static_cast<const M*>(f0),              \
+                     static_cast<const M*>(f1), static_cast<const M*>(f2),     \
+                     out0, out1, out2)
+  if (mesh_prec == SSIM_PREC_F64) {
+    if (dim == 2) LAUNCH_GAT(2, double); else LAUNCH_GAT(3, double);
+  } else {
+    if (dim == 2) LAUNCH_GAT(2, float); else LAUNCH_GAT(3, float);
+  }
+#undef LAUNCH_GAT
+  SSIM_LAUNCHED("k_cic_gather");
+  return SSIM_OK;
+}
+
+size_t ssim_minmax_work_bytes(int n) {
+  return ssim::align256((size_t)ssim::ceil_div(n < 1 ? 1 : n, PM_BLOCK * 8) * 6 *
+                        sizeof(double));
+}
+
+int ssim_minmax(int n, int narrays, const double* a0, const double* a1,
+                const double* a2, double* out, void* work, size_t work_bytes,
+                void* stream) {
+  SSIM_REQUIRE(n >= 1, "minmax of an empty array");
+  SSIM_REQUIRE(narrays >= 1 && narrays <= 3, "narrays must be 1..3");
+  SSIM_REQUIRE(a0 && out && work && (narrays < 2 || a1) && (narrays < 3 || a2),
+               "null pointer");
+  SSIM_REQUIRE(work_bytes >= ssim_minmax_work_bytes(n), "workspace too small");
+  hipStream_t st = ssim::as_stream(stream);
+  int nblk = ssim::ceil_div(n, PM_BLOCK * 8);
+  double* part = static_cast<double*>(work);
+  hipLaunchKernelGGL(k_minmax_partial, dim3(nblk), dim3(PM_BLOCK), 0, st, n,
+                     narrays, a0, a1, a2, part);
+  SSIM_LAUNCHED("k_minmax_partial");
+  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, st, nblk, narrays,
+                     part, out);
+  SSIM_LAUNCHED("k_minmax_final");
+  return SSIM_OK;
+}
+
+int ssim_pm_force_kernel(int dim, int n, int comp, double h, double eps2,
+                         double G, double rs, int mesh_prec, void* out,
+                         void* stream) {
+  SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+  SSIM_REQUIRE(n >= 1 && comp >= 0 && comp < dim, "bad n / component");
+  SSIM_REQUIRE(h > 0.0 && eps2 >= 0.0 && rs >= 0.0,
+               "need h > 0, eps2 >= 0, rs >= 0");
+  SSIM_REQUIRE(out, "null pointer");
+  const long long P = 2LL * n;
+  const long long total = dim == 2 ? P * P : P * P * P;
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid((unsigned)ssim::ceil_div(total, PM_BLOCK));
+  if (mesh_prec == SSIM_PREC_F64)
+    hipLaunchKernelGGL(k_force_kernel<double>, grid, dim3(PM_BLOCK), 0, st, dim,
+                       n, comp, h, eps2, G, rs, static_cast<double*>(out));
+  else
+    hipLaunchKernelGGL(k_force_kernel<float>, grid, dim3(PM_BLOCK), 0, st, dim,
+                       n, comp, h, eps2, G, rs, static_cast<float*>(out));
+  SSIM_LAUNCHED("k_force_kernel");
+  return SSIM_OK;
+}
+
+int ssim_pm_deconvolve(int dim, int padded_n, int prec, void* spectrum,
+                       void* stream) {
+  SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+  SSIM_REQUIRE(padded_n >= 2 && spectrum, "bad arguments");
+  const long long W = padded_n / 2 + 1;
+  const long long total = dim == 2 ? padded_n * W : (long long)padded_n * padded_n * W;
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid((unsigned)ssim::ceil_div(total, PM_BLOCK));
+  if (prec == SSIM_PREC_F64)
+    hipLaunchKernelGGL(k_deconvolve<double>, grid, dim3(PM_BLOCK), 0, st, dim,
+                       padded_n, static_cast<double*>(spectrum));
+  else
+    hipLaunchKernelGGL(k_deconvolve<float>, grid, dim3(PM_BLOCK), 0, st, dim,
+                       padded_n, static_cast<float*>(spectrum));
+  SSIM_LAUNCHED("k_deconvolve");
+  return SSIM_OK;
+}
+
+int ssim_spectrum_mul(long long ncomplex, int prec, const void* a,
+                      const void* b, void* out, void* stream) {
+  SSIM_REQUIRE(ncomplex >= 0, "bad size");
+  if (ncomplex == 0) return SSIM_OK;
+  SSIM_REQUIRE(a && b && out, "null pointer");
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid((unsigned)ssim::ceil_div(ncomplex, PM_BLOCK));
+  if (prec == SSIM_PREC_F64)
+    hipLaunchKernelGGL(k_cmul<double>, grid, dim3(PM_BLOCK), 0, st, ncomplex,
+                       static_cast<const double*>(a), static_cast<const double*>(b),
+                       static_cast<double*>(out));
+  else
+    hipLaunchKernelGGL(k_cmul<float>, grid, dim3(PM_BLOCK), 0, st, ncomplex,
+                       static_cast<const float*>(a), static_cast<const float*>(b),
+                       static_cast<float*>(out));
+  SSIM_LAUNCHED("k_cmul");
+  return SSIM_OK;
+}
+
+int ssim_pm_poisson(int dim, const int* grid_n, double h, double G, int prec,
+                    void* spectrum, void* stream) {
+  SSIM_TRY(check_grid(dim, grid_n, h, 1));
+  SSIM_REQUIRE(spectrum, "null pointer");
+  const int nl = grid_n[dim - 1];
+  const long long W = nl / 2 + 1;
+  const long long total = dim == 2 ? grid_n[0] * W : (long long)grid_n[0] * grid_n[1] * W;
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid((unsigned)ssim::ceil_div(total, PM_BLOCK));
+  const int n2 = dim == 3 ? grid_n[2] : 1;
+  if (prec == SSIM_PREC_F64)
+    hipLaunchKernelGGL(k_poisson<double>, grid, dim3(PM_BLOCK), 0, st, dim,
+                       grid_n[0], grid_n[1], n2, h, G, static_cast<double*>(spectrum));
+  else
+    hipLaunchKernelGGL(k_poisson<float>, grid, dim3(PM_BLOCK), 0, st, dim,
+                       grid_n[0], grid_n[1], n2, h, G, static_cast<float*>(spectrum));
+  SSIM_LAUNCHED("k_poisson");
+  return SSIM_OK;
+}
+
+int ssim_pm_gradient(int dim, const int* grid_n, double h, int prec,
+                     const void* phi, void* g0, void* g1, void* g2, void* stream) {
+  SSIM_TRY(check_grid(dim, grid_n, h, 1));
+  SSIM_REQUIRE(phi && g0 && g1 && (dim == 2 || g2), "null pointer");
+  const int n2 = dim == 3 ? grid_n[2] : 1;
+  const long long total = (long long)grid_n[0] * grid_n[1] * n2;
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid((unsigned)ssim::ceil_div(total, PM_BLOCK));
+  if (prec == SSIM_PREC_F64)
+    hipLaunchKernelGGL(k_gradient<double>, grid, dim3(PM_BLOCK), 0, st, dim,
+                       grid_n[0], grid_n[1], n2, h, static_cast<const double*>(phi),
+                       static_cast<double*>(g0), static_cast<double*>(g1),
+                       static_cast<double*>(g2));
+  else
+    hipLaunchKernelGGL(k_gradient<float>, grid, dim3(PM_BLOCK), 0, st, dim,
+                       grid_n[0], grid_n[1], n2, h, static_cast<const float*>(phi),
+                       static_cast<float*>(g0), static_cast<float*>(g1),
+                       static_cast<float*>(g2));
+  SSIM_LAUNCHED("k_gradient");
+  return SSIM_OK;
+}
+
+int ssim_pm_pad(int dim, int n, int prec, const void* src, void* dst, int crop,
+                void* stream) {
+  SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+  SSIM_REQUIRE(n >= 1 && src && dst, "bad arguments");
+  const long long total = dim == 2 ? (long long)n * n : (long long)n * n * n;
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid((unsigned)ssim::ceil_div(total, PM_BLOCK));
+  if (prec == SSIM_PREC_F64)
+    hipLaunchKernelGGL(k_pad<double>, grid, dim3(PM_BLOCK), 0, st, dim, n,
+                       static_cast<const double*>(src), static_cast<double*>(dst), crop);
+  else
+    hipLaunchKernelGGL(k_pad<float>, grid, dim3(PM_BLOCK), 0, st, dim, n,
+                       static_cast<const float*>(src), static_cast<float*>(dst), crop);
+  SSIM_LAUNCHED("k_pad");
+  return SSIM_OK;
+}
+
+}  // extern "C"

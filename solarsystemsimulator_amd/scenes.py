@@ -24,10 +24,11 @@ def _zeldovich(shape, rng, amp):
     white = np.fft.rfftn(rng.standard_normal(shape))
     delta_k = white / np.sqrt(k2)          # P(k) ~ k^-2
     delta_k[0, 0, 0] = 0.0
-    delta = np.fft.irfftn(delta_k, s=shape)
+    delta = np.fft.irfftn(delta_k, s=shape, axes=(0, 1, 2))
     delta_k *= amp / delta.std()
     # div psi = -delta  =>  psi_k = i k delta_k / k^2
-    psi = [np.fft.irfftn(1j * K * delta_k / k2, s=shape) for K in (KX, KY, KZ)]
+    psi = [np.fft.irfftn(1j * K * delta_k / k2, s=shape, axes=(0, 1, 2))
+           for K in (KX, KY, KZ)]
     return np.stack(psi)
 
 

@@ -1,0 +1,125 @@
+"""GPU parity of the learned-correction network: BF16-MFMA forward,
+backward (parameter gradients), Adam and feature extraction against the
+NumPy oracle that rounds at the same points, and a training smoke check."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp_np as M
+from solarsystemsimulator_amd.learned import LearnedCorrection
+
+pytestmark = pytest.mark.gpu
+
+
+def random_net(seed):
+    rng = np.random.default_rng(seed)
+    p = M.init_params(seed)
+    d = M.unpack(p.copy())
+    for b in ("b1", "b2", "b3"):
+        d[b] = rng.normal(0, 0.1, d[b].shape).astype(np.float32)
+    d["b4"][:3] = rng.normal(0, 0.1, 3)
+    return M.pack(d)
+
+
+def bf16_features(n, seed):
+    rng = np.random.default_rng(seed)
+    x = np.zeros((n, 32), np.float32)
+    x[:, :30] = rng.normal(0, 1, (n, 30))
+    return M.bf16(x)
+
+
+def to_bf16_tensor(x, dev):
+    bits = (np.asarray(x, np.float32).view(np.uint32) >> 16).astype(np.uint16)
+    return torch.from_numpy(bits.view(np.int16).copy()).to(dev)
+
+
+@pytest.mark.parametrize("n", [1, 37, 1000])
+def test_forward_matches_oracle(dev, n):
+    p = random_net(1)
+    x = bf16_features(n, 2)
+    net = LearnedCorrection(dev, p)
+    out = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    net.predict(to_bf16_tensor(x, dev), out=out)
+    ref, _ = M.forward(p, x, rounding=True)
+    got = out.cpu().numpy()
+    assert np.linalg.norm(got - ref) <= 5e-3 * np.linalg.norm(ref)
+    assert np.median(np.abs(got - ref)) <= 1e-3 * np.abs(ref).mean()
+
+
+def test_forward_adds_scaled_output(dev):
+    p = random_net(3)
+    x = bf16_features(100, 4)
+    net = LearnedCorrection(dev, p)
+    feat = to_bf16_tensor(x, dev)
+    out = torch.empty((100, 3), dtype=torch.float32, device=dev)
+    net.predict(feat, out=out)
+    a = [torch.full((100,), 1.5, dtype=torch.float64, device=dev) for _ in range(3)]
+    net.predict(feat, add_to=a, scale=2.0)
+    for c in range(3):
+        np.testing.assert_allclose(a[c].cpu().numpy(),
+                                   1.5 + 2.0 * out[:, c].cpu().numpy().astype(np.float64),
+                                   rtol=1e-6)
+
+
+def test_backward_matches_oracle(dev):
+    n = 3000
+    p = random_net(5)
+    x = bf16_features(n, 6)
+    y = np.random.default_rng(7).normal(0, 1, (n, 3)).astype(np.float32)
+    net = LearnedCorrection(dev, p)
+    loss, _ = net.gradient(to_bf16_tensor(x, dev), torch.from_numpy(y).to(dev))
+    g = net.grad.cpu().numpy().astype(np.float64)
+    out, cache = M.forward(p, x, rounding=True)
+    ref = M.backward(cache, 2 * (out - y) / n, rounding=True)
+    assert abs(loss - M.mse(out, y)) <= 1e-3 * M.mse(out, y)
+    G, R = M.unpack(g), M.unpack(ref)
+    for name, _ in M.SIZES:
+        err = np.linalg.norm(G[name] - R[name]) / max(np.linalg.norm(R[name]), 1e-30)
+        assert err <= 2e-2, (name, err)
+
+
+def test_adam_matches_oracle(dev):
+    p = random_net(8)
+    net = LearnedCorrection(dev, p)
+    g = np.random.default_rng(9).normal(0, 1, p.size).astype(np.float32)
+    net.grad.copy_(torch.from_numpy(g))
+    net.adam_step(1e-2)
+    net.adam_step(1e-2)
+    ref, m, v = M.adam(p.astype(np.float64), g, 0, 0, 1, 1e-2)
+    ref, m, v = M.adam(ref, g, m, v, 2, 1e-2)
+    np.testing.assert_allclose(net.params.cpu().numpy(), ref, rtol=1e-5, atol=1e-6)
+    wbf = net.wbf.cpu().numpy().view(np.uint16).astype(np.uint32) << 16
+    assert np.array_equal(wbf.view(np.float32), M.bf16(net.params.cpu().numpy()))
+
+
+def test_features_match_oracle(dev):
+    rng = np.random.default_rng(11)
+    n, nb = 16, 2000
+    rho = rng.uniform(0, 3, (n, n, n))
+    origin, h = [0.1, -0.2, 0.3], 0.05
+    pos = [rng.uniform(o - h, o + (n + 1) * h, nb) for o in origin]
+    net = LearnedCorrection(dev)
+    net.mass_unit = 0.75
+    feat = net.features([torch.from_numpy(c).to(dev) for c in pos],
+                        torch.from_numpy(rho).to(dev), origin, h, 0)
+    got = (feat.cpu().numpy().view(np.uint16).astype(np.uint32) << 16).view(np.float32)
+    ref = M.bf16(M.features(pos, rho, origin, h, 0.75).astype(np.float32))
+    # v / unit (oracle) vs v * (1/unit) (GPU) may differ by one FP64 ulp,
+    # which can move a BF16 rounding by one step
+    assert np.mean(got == ref) > 0.999
+    assert np.abs(got - ref).max() <= 2 ** -7 * np.abs(ref).max()
+
+
+def test_training_reduces_loss(dev):
+    rng = np.random.default_rng(12)
+    n = 65536
+    x = bf16_features(n, 13)
+    w = rng.normal(0, 1, (30, 3))
+    y = (np.tanh(x[:, :30] @ w / 6.0) + 0.1 * x[:, :3] ** 2).astype(np.float32)
+    net = LearnedCorrection(dev, seed=14)
+    feat = to_bf16_tensor(x, dev)
+    tgt = torch.from_numpy(y).to(dev)
+    l0, _ = net.gradient(feat, tgt)
+    hist = net.train(feat, tgt, epochs=6, batch=4096, lr=3e-3, lr_decay=0.7)
+    l1, _ = net.gradient(feat, tgt)
+    assert l1 < 0.1 * l0, (l0, hist, l1)
