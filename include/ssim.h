@@ -112,6 +112,43 @@ int ssim_kick_snap(int dim, int n, int t, const double* x, const double* y,
                    const double* ax, const double* ay, const double* az,
                    double h, float* snap, void* stream);
 
+/* dst[0..n) = value (a kernel, not hipMemset, so it can be graph-captured);
+ * e.g. the per-batch reset of the contact times (nbody_kernel.py:811). */
+int ssim_fill_f32(long long n, float* dst, float value, void* stream);
+/* dst[idx[k]] = vals[k], or += when add != 0 (indices unique): the point
+ * updates of the resident state (nbody_kernel.py:943-1040). */
+int ssim_scatter_f64(int n, const long long* idx, const double* vals,
+                     double* dst, int add, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Few massive bodies + many asteroids (raster batches)                */
+/* ------------------------------------------------------------------ */
+
+/* K raster samples of dt_raster in one launch (frame_kernel,
+ * nbody_kernel.py:97-560, single device: its gpuId/nGpus/GSync arguments
+ * are gone).  Asteroid arrays have n_ast entries, massive arrays m <= 64;
+ * aacc_x/y and macc_x/y are scratch (recomputed at the start, as in the
+ * reference).
+ * soft is added to r^2.  Outputs: snap f32 [k][4][n_ast] (x|y|vx|vy),
+ * snap_m f32 [k][4][m], sub_pos f32 [k][m_sub][2][n_ast] and sub_n u8
+ * [k][n_ast] (hot sub-samples; complete when sub_n == m_sub), hit_t f32
+ * [n_ast] (batch time of first contact; the caller resets it to -1 per
+ * batch) and hit_m i32 [n_ast] (massive partner).  Cooperative launch
+ * (grid barriers), deterministic back-reaction sums. */
+size_t ssim_nbody_work_bytes(int n_ast);
+int ssim_nbody_frame(int n_ast, double* ax, double* ay, double* avx,
+                     double* avy, double* aacc_x, double* aacc_y,
+                     const double* am, const unsigned char* avis,
+                     const double* arad, int m, double* mx, double* my,
+                     double* mvx, double* mvy, double* macc_x, double* macc_y,
+                     const double* mm, const unsigned char* mvis,
+                     const double* mrad, double G, double soft,
+                     double dt_raster, int k, double max_sub_dt,
+                     int max_sub_steps, double w1, double w0, int m_sub,
+                     float* snap, float* snap_m, float* sub_pos,
+                     unsigned char* sub_n, float* hit_t, int* hit_m,
+                     void* work, size_t work_bytes, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* FP64 energy diagnostic                                              */
 /* ------------------------------------------------------------------ */
@@ -234,6 +271,32 @@ int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,
                   double* az, void* stream);
 
 /* ------------------------------------------------------------------ */
+/* Uniform-grid collision search (bit-exact ordered pair list)         */
+/* ------------------------------------------------------------------ */
+
+/* Asteroid-asteroid contact pairs of one sample, exactly the list of
+ * `Erkennungskarte._hits` (film_producer.py:223-317) for the stripe
+ * [lo, hi) with halo (single device: lo = -inf, hi = +inf): same alive
+ * rule, XOR cell hash of floor(f64(x)/h), stable key order, neighbour
+ * offsets (0,0),(1,0),(0,1),(1,1),(1,-1), FP32 closest-approach prefilter
+ * over [-dt_y, dt_y] with radius sum + 1e-6, ownership min(x) < hi and the
+ * FP64 recheck, in the reference's order.  Inputs are the f32 sample
+ * (gx, gy, gvx, gvy), f32 contact radii rr, u8 flags is_ast and vis.
+ * counts[0] = candidate pairs, counts[1] = FP32 hits, counts[2] = final
+ * pairs (or -1 when f32_hit_capacity was too small: retry with
+ * counts[1]).  Pairs are written to out_i/out_j (int64) up to
+ * pair_capacity; the workspace depends on n and f32_hit_capacity.
+ * Synchronises the stream (counts are returned on the host). */
+size_t ssim_collide_work_bytes(int n, long long f32_hit_capacity);
+int ssim_collide_hits(int n, const float* gx, const float* gy, const float* gvx,
+                      const float* gvy, const float* rr,
+                      const unsigned char* is_ast, const unsigned char* vis,
+                      float lo, float hi, float halo, double dt_y, double h,
+                      long long f32_hit_capacity, long long pair_capacity,
+                      long long* out_i, long long* out_j, long long* counts,
+                      void* work, size_t work_bytes, void* stream);
+
+/* ------------------------------------------------------------------ */
 /* Learned near-field correction (MLP, BF16 MFMA, FP32 accumulation)   */
 /* ------------------------------------------------------------------ */
 /* None of these has a reference counterpart: the reference has no learned
@@ -260,6 +323,12 @@ int ssim_mlp_forward(int nb, const unsigned short* feat,
                      const unsigned short* wbf, const float* params,
                      float scale, float* out, double* ax, double* ay,
                      double* az, float* zs, unsigned short* hs, void* stream);
+/* Training target: out[i][c] = (ref_c[i] - mod_c[i]) / a_unit (FP32),
+ * the residual of the model force against the exact direct sum. */
+int ssim_mlp_residual(int nb, const double* ref_x, const double* ref_y,
+                      const double* ref_z, const double* mod_x,
+                      const double* mod_y, const double* mod_z, double a_unit,
+                      float* out, void* stream);
 /* gout = 2 (out - target) / nb; loss_part[block] = partial sums of the
  * squared error over blocks of 256 bodies. */
 int ssim_mlp_mse_grad(int nb, const float* out, const float* target,

@@ -105,3 +105,67 @@ def selfgrav_run(pos, vel, m, tpos, tvel, G, eps2, dt, steps, scheme=0,
                               _p(as_[0]), _p(as_[1]), _p(as_[2]), _p(mm), G,
                               eps2, dt, steps, 1 if acc is None else 0)
     return x, v, a
+
+
+def nbody_frames(x, y, vx, vy, mass, visible, is_ast, real_r, dt_raster, k,
+                 m_sub=8, G=4 * np.pi ** 2, soft=1e-6,
+                 max_sub_dt=0.5 / 365.25, max_sub_steps=100_000):
+    """Serial restatement of one `step_batch` (nbody_kernel.py:783-859):
+    returns (out f32 (k, 4n) in original order, state f64 (4n), sub,
+    hits) with sub/hits shaped like NBodyCuda's st["sub"] / st["hits"].
+    The inputs are not modified."""
+    c = ctypes
+    ast = np.asarray(is_ast) != 0
+    m_idx, a_idx = np.flatnonzero(~ast), np.flatnonzero(ast)
+    na, m = len(a_idx), len(m_idx)
+    n = len(x)
+    cols = [np.asarray(v, np.float64) for v in (x, y, vx, vy, mass)]
+    a = [_f64(v[a_idx]).copy() for v in cols]
+    M = [_f64(v[m_idx]).copy() for v in cols]
+    vis = np.asarray(visible).astype(np.uint8)
+    rr = np.zeros(n) if real_r is None else np.asarray(real_r, np.float64)
+    avis, mvis = np.ascontiguousarray(vis[a_idx]), np.ascontiguousarray(vis[m_idx])
+    arad, mrad = _f64(rr[a_idx]).copy(), _f64(rr[m_idx]).copy()
+    aacc = [np.zeros(max(na, 1)) for _ in range(2)]
+    macc = [np.zeros(max(m, 1)) for _ in range(2)]
+    ms = max(m_sub, 1)
+    snap = np.zeros(k * 4 * max(na, 1), np.float32)
+    snapm = np.zeros(k * 4 * max(m, 1), np.float32)
+    subpos = np.zeros(k * ms * 2 * max(na, 1), np.float32)
+    subn = np.zeros(k * max(na, 1), np.uint8)
+    hitt = np.full(max(na, 1), -1.0, np.float32)
+    hitm = np.zeros(max(na, 1), np.int32)
+    hot = np.zeros(max(na, 1), np.uint8)
+    w1 = 1.0 / (2.0 - 2.0 ** (1.0 / 3.0))
+    w0 = -(2.0 ** (1.0 / 3.0)) / (2.0 - 2.0 ** (1.0 / 3.0))
+    fn = lib().oracle_nbody_frame
+    P = c.c_void_p
+    fn.argtypes = ([c.c_int] + [P] * 9 + [c.c_int] + [P] * 9 +
+                   [c.c_double, c.c_double, c.c_double, c.c_int, c.c_double,
+                    c.c_int, c.c_double, c.c_double, c.c_int] + [P] * 7)
+    fn(na, *[_p(v) for v in a[:4]], _p(aacc[0]), _p(aacc[1]), _p(a[4]),
+       _p(avis), _p(arad), m, *[_p(v) for v in M[:4]], _p(macc[0]),
+       _p(macc[1]), _p(M[4]), _p(mvis), _p(mrad), G, soft, dt_raster, k,
+       max_sub_dt, max_sub_steps, w1, w0, m_sub, _p(snap), _p(snapm),
+       _p(subpos), _p(subn), _p(hitt), _p(hitm), _p(hot))
+    out = np.empty((k, 4 * n), np.float32)
+    sn = snap[:k * 4 * na].reshape(k, 4, na)
+    smn = snapm[:k * 4 * m].reshape(k, 4, m)
+    for f in range(4):
+        out[:, f * n + a_idx] = sn[:, f, :]
+        out[:, f * n + m_idx] = smn[:, f, :]
+    state = np.empty(4 * n)
+    for f in range(4):
+        state[f * n + a_idx] = a[f][:na]
+        state[f * n + m_idx] = M[f][:m]
+    sel = np.flatnonzero(hitt[:na] >= 0)
+    hits = (a_idx[sel], hitt[sel], m_idx[hitm[sel]])
+    sub = None
+    if m_sub > 0:
+        sub = []
+        subn_k = subn[:k * na].reshape(k, na)
+        pos = subpos[:k * ms * 2 * na].reshape(k, ms, 2, na)
+        for ks in range(k):
+            s = np.flatnonzero(subn_k[ks] == m_sub)
+            sub.append((a_idx[s], pos[ks][:, :, s]))
+    return out, state, sub, hits

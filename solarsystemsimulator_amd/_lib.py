@@ -46,6 +46,13 @@ SIGNATURES = {
     "ssim_drift": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, c_double, _P]),
     "ssim_kick_snap": (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P, _P,
                                _P, _P, c_double, _P, _P]),
+    "ssim_fill_f32": (c_int, [c_longlong, _P, c_float, _P]),
+    "ssim_scatter_f64": (c_int, [c_int, _P, _P, _P, c_int, _P]),
+    "ssim_nbody_work_bytes": (c_size_t, [c_int]),
+    "ssim_nbody_frame": (c_int, [c_int] + [_P] * 9 + [c_int] + [_P] * 9 +
+                         [c_double, c_double, c_double, c_int, c_double, c_int,
+                          c_double, c_double, c_int] + [_P] * 6 +
+                         [_P, c_size_t, _P]),
     "ssim_energy_work_bytes": (c_size_t, [c_int, c_int]),
     "ssim_energy": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, _P, c_double,
                             c_double, _P, _P, c_size_t, _P]),
@@ -74,12 +81,19 @@ SIGNATURES = {
     "ssim_pp_short": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P,
                               c_double, c_double, c_double, c_double, _P, _P,
                               _P, _P]),
+    "ssim_collide_work_bytes": (c_size_t, [c_int, c_longlong]),
+    "ssim_collide_hits": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, c_float,
+                                  c_float, c_float, c_double, c_double,
+                                  c_longlong, c_longlong, _P, _P, _P, _P,
+                                  c_size_t, _P]),
     "ssim_mlp_param_count": (c_int, []),
     "ssim_mlp_to_bf16": (c_int, [_P, _P, _P]),
     "ssim_mlp_features": (c_int, [c_int, _P, _P, _P, c_int, _P, c_double, c_int,
                                   _P, c_double, _P, _P]),
     "ssim_mlp_forward": (c_int, [c_int, _P, _P, _P, c_float, _P, _P, _P, _P, _P,
                                  _P, _P]),
+    "ssim_mlp_residual": (c_int, [c_int, _P, _P, _P, _P, _P, _P, c_double, _P,
+                                  _P]),
     "ssim_mlp_mse_grad": (c_int, [c_int, _P, _P, _P, _P, _P]),
     "ssim_mlp_train_work_bytes": (c_size_t, [c_int]),
     "ssim_mlp_backward": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, c_size_t,

@@ -1,0 +1,426 @@
+// Uniform-grid asteroid-asteroid collision search with the reference's
+// exact pair list (`Erkennungskarte._hits`, film_producer.py:223-317).
+//
+// Order and arithmetic follow the reference so that the list is bit-exact:
+//  * alive = asteroid & visible & x in [lo, hi + halo) (FP32 compares);
+//    alive bodies are enumerated in ascending index (the "rows");
+//  * cell ix = floor(f64(x) / h) as int64, key = ix*73856093 XOR
+//    iy*19349663 (int64 wrap-around, `_zellschluessel` line 320-321);
+//  * the rows sorted by key, ties by row (a stable argsort);
+//  * for every neighbour offset in (0,0),(1,0),(0,1),(1,1),(1,-1) and every
+//    row, the candidates are the sorted entries whose key equals the
+//    offset cell's key (hash collisions included, like searchsorted on the
+//    keys); pair (a, b) is kept if a < b for (0,0) and a != b otherwise;
+//  * FP32 closest-approach prefilter over [-dt, dt] with the radius sum
+//    widened by 1e-6 AU, every operation rounded separately (the reference
+//    runs one CuPy elementwise kernel per operation; built here without FMA
+//    contraction);
+//  * ownership min(x_a, x_b) < hi, then the FP64 recheck on the FP32 inputs.
+// The output is ordered by (offset, row, sorted position): count pass, an
+// exclusive scan of the per-row hit counts, fill pass — no atomics, so the
+// list is deterministic.  The sort is a bitonic network on the 96-bit
+// composite (key, row), which is a stable sort of the keys.
+#include <climits>
+
+#include "ssim_internal.h"
+
+namespace {
+
+constexpr int CB = 256;
+constexpr long long HX = 73856093LL, HY = 19349663LL;
+
+__global__ __launch_bounds__(CB) void k_alive_flags(
+    int n, const float* __restrict__ gx, const unsigned char* __restrict__ ast,
+    const unsigned char* __restrict__ vis, float lo, float hi_halo,
+    int* __restrict__ flag) {
+  const int i = blockIdx.x * CB + threadIdx.x;
+  if (i >= n) return;
+  flag[i] = (ast[i] && vis[i] != 0 && gx[i] >= lo && gx[i] < hi_halo) ? 1 : 0;
+}
+
+// Block-wise exclusive scan with per-block totals (3-phase scan).
+__global__ __launch_bounds__(1024) void k_scan_block(int n, const int* __restrict__ in,
+                                                     int* __restrict__ out,
+                                                     int* __restrict__ totals) {
+  __shared__ int s[1024];
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  const int v = i < n ? in[i] : 0;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int t = threadIdx.x >= off ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) out[i] = s[threadIdx.x] - v;
+  if (threadIdx.x == 1023) totals[blockIdx.x] = s[1023];
+}
+
+__global__ void k_scan_totals(int nb, int* __restrict__ totals, int* __restrict__ sum) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int run = 0;
+  for (int b = 0; b < nb; ++b) {
+    const int t = totals[b];
+    totals[b] = run;
+    run += t;
+  }
+  *sum = run;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_add(int n, int* __restrict__ out,
+                                                   const int* __restrict__ totals) {
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  if (i < n) out[i] += totals[blockIdx.x];
+}
+
+__global__ __launch_bounds__(CB) void k_compact_alive(int n, const int* __restrict__ flag,
+                                                      const int* __restrict__ pos,
+                                                      int* __restrict__ ai) {
+  const int i = blockIdx.x * CB + threadIdx.x;
+  if (i < n && flag[i]) ai[pos[i]] = i;
+}
+
+__device__ __forceinline__ long long cell_key(long long ix, long long iy) {
+  // int64 products wrap like NumPy/CuPy int64 arithmetic
+  const unsigned long long a = (unsigned long long)ix * (unsigned long long)HX;
+  const unsigned long long b = (unsigned long long)iy * (unsigned long long)HY;
+  return (long long)(a ^ b);
+}
+
+__global__ __launch_bounds__(CB) void k_keys(int na, int np2, const int* __restrict__ ai,
+                                             const float* __restrict__ gx,
+                                             const float* __restrict__ gy, double h,
+                                             long long* __restrict__ ix,
+                                             long long* __restrict__ iy,
+                                             unsigned long long* __restrict__ skey,
+                                             int* __restrict__ srow) {
+  const int p = blockIdx.x * CB + threadIdx.x;
+  if (p >= np2) return;
+  if (p < na) {
+    const int b = ai[p];
+    const long long cx = (long long)floor((double)gx[b] / h);
+    const long long cy = (long long)floor((double)gy[b] / h);
+    ix[p] = cx;
+    iy[p] = cy;
+    // signed order -> unsigned order by flipping the sign bit
+    skey[p] = (unsigned long long)cell_key(cx, cy) ^ 0x8000000000000000ULL;
+    srow[p] = p;
+  } else {  // padding sorts last
+    skey[p] = ULLONG_MAX;
+    srow[p] = INT_MAX;
+  }
+}
+
+__device__ __forceinline__ bool pair_less(unsigned long long ka, int ra,
+                                          unsigned long long kb, int rb) {
+  return ka < kb || (ka == kb && ra < rb);
+}
+
+// One bitonic compare-exchange step over the power-of-two array.
+__global__ __launch_bounds__(CB) void k_bitonic(int np2, int j, int k,
+                                                unsigned long long* __restrict__ key,
+                                                int* __restrict__ row) {
+  const int i = blockIdx.x * CB + threadIdx.x;
+  if (i >= np2) return;
+  const int l = i ^ j;
+  if (l <= i) return;
+  const bool up = (i & k) == 0;
+  const unsigned long long ki = key[i], kl = key[l];
+  const int ri = row[i], rl = row[l];
+  const bool swap = up ? pair_less(kl, rl, ki, ri) : pair_less(ki, ri, kl, rl);
+  if (swap) {
+    key[i] = kl;
+    key[l] = ki;
+    row[i] = rl;
+    row[l] = ri;
+  }
+}
+
+__device__ __forceinline__ int lower_bound(const unsigned long long* a, int n,
+                                           unsigned long long v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int upper_bound(const unsigned long long* a, int n,
+                                           unsigned long long v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] <= v) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+struct Pf {
+  const float *gx, *gy, *gvx, *gvy, *rr;
+  float dt, tol;
+};
+
+// FP32 prefilter of one candidate pair, one rounding per operation.
+__device__ __forceinline__ bool prefilter(const Pf& P, int a, int b) {
+  const float dpx = P.gx[b] - P.gx[a];
+  const float dpy = P.gy[b] - P.gy[a];
+  const float dvx = P.gvx[b] - P.gvx[a];
+  const float dvy = P.gvy[b] - P.gvy[a];
+  const float dv2 = dvx * dvx + dvy * dvy;
+  const float num = -(dpx * dvx + dpy * dvy);
+  const float den = dv2 > 0.0f ? dv2 : 1.0f;
+  float t = num / den;
+  t = t < -P.dt ? -P.dt : (t > P.dt ? P.dt : t);
+  const float cx = dpx + dvx * t;
+  const float cy = dpy + dvy * t;
+  const float rs = P.rr[a] + P.rr[b] + P.tol;
+  return cx * cx + cy * cy <= rs * rs;
+}
+
+// Count (pass 0) or write (pass 1) the prefilter hits of every (offset, row).
+__global__ __launch_bounds__(CB) void k_candidates(
+    int na, const int* __restrict__ ai, const long long* __restrict__ ix,
+    const long long* __restrict__ iy, const unsigned long long* __restrict__ skey,
+    const int* __restrict__ srow, Pf P, int pass, int* __restrict__ hcount,
+    unsigned long long* __restrict__ ccount, const int* __restrict__ hoff,
+    int* __restrict__ outa, int* __restrict__ outb) {
+  const int t = blockIdx.x * CB + threadIdx.x;
+  if (t >= 5 * na) return;
+  const int o = t / na, p = t % na;
+  const int ox[5] = {0, 1, 0, 1, 1}, oy[5] = {0, 0, 1, 1, -1};
+  const unsigned long long k2 =
+      (unsigned long long)cell_key(ix[p] + ox[o], iy[p] + oy[o]) ^ 0x8000000000000000ULL;
+  const int lo = lower_bound(skey, na, k2);
+  const int hi = upper_bound(skey, na, k2);
+  const int a = ai[p];
+  int w = pass ? hoff[t] : 0;
+  int cnt = 0;
+  for (int c = lo; c < hi; ++c) {
+    const int b = ai[srow[c]];
+    const bool keep = (o == 0) ? (a < b) : (a != b);
+    if (keep && prefilter(P, a, b)) {
+      if (pass) {
+        outa[w] = a;
+        outb[w] = b;
+        ++w;
+      }
+      ++cnt;
+    }
+  }
+  if (!pass) {
+    hcount[t] = cnt;
+    ccount[t] = (unsigned long long)(hi - lo);
+  }
+}
+
+// Ownership rule + FP64 recheck on the FP32 inputs -> flag per f32 hit.
+__global__ __launch_bounds__(CB) void k_recheck(int nh, const int* __restrict__ ha,
+                                                const int* __restrict__ hb, Pf P,
+                                                double dt, float hi,
+                                                int* __restrict__ flag) {
+  const int t = blockIdx.x * CB + threadIdx.x;
+  if (t >= nh) return;
+  const int a = ha[t], b = hb[t];
+  int ok = 0;
+  if (fminf(P.gx[a], P.gx[b]) < hi) {
+    const double dpx = (double)P.gx[b] - (double)P.gx[a];
+    const double dpy = (double)P.gy[b] - (double)P.gy[a];
+    const double dvx = (double)P.gvx[b] - (double)P.gvx[a];
+    const double dvy = (double)P.gvy[b] - (double)P.gvy[a];
+    const double dv2 = dvx * dvx + dvy * dvy;
+    double tm = -(dpx * dvx + dpy * dvy) / (dv2 > 0.0 ? dv2 : 1.0);
+    tm = tm < -dt ? -dt : (tm > dt ? dt : tm);
+    const double cx = dpx + dvx * tm;
+    const double cy = dpy + dvy * tm;
+    const double rs = (double)P.rr[a] + (double)P.rr[b];
+    ok = cx * cx + cy * cy <= rs * rs;
+  }
+  flag[t] = ok;
+}
+
+__global__ __launch_bounds__(CB) void k_compact_pairs(int nh, const int* __restrict__ flag,
+                                                      const int* __restrict__ pos,
+                                                      const int* __restrict__ ha,
+                                                      const int* __restrict__ hb,
+                                                      long long* __restrict__ oi,
+                                                      long long* __restrict__ oj,
+                                                      long long cap) {
+  const int t = blockIdx.x * CB + threadIdx.x;
+  if (t >= nh || !flag[t]) return;
+  const long long q = pos[t];
+  if (q < cap) {
+    oi[q] = ha[t];
+    oj[q] = hb[t];
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_sum_u64(int n, const unsigned long long* __restrict__ a,
+                                                  unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s[1024];
+  unsigned long long v = 0;
+  for (int i = threadIdx.x; i < n; i += 1024) v += a[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+// Exclusive scan of n ints (any n) with the 3-phase kernels.
+int scan(int n, const int* in, int* out, int* totals, int* sum, hipStream_t st) {
+  const int nb = ssim::ceil_div(n > 0 ? n : 1, 1024);
+  hipLaunchKernelGGL(k_scan_block, dim3(nb), dim3(1024), 0, st, n, in, out, totals);
+  hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(64), 0, st, nb, totals, sum);
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, st, n, out, totals);
+  SSIM_LAUNCHED("scan");
+  return SSIM_OK;
+}
+
+int next_pow2(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+struct Layout {
+  size_t flag, pos, ai, ix, iy, skey, srow, hcount, hoff, ccount, totals, sums,
+      ha, hb, hflag, hpos, total;
+};
+
+Layout layout(int n, long long hcap) {
+  Layout L{};
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  const size_t np2 = (size_t)next_pow2((int)nn);
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o += ssim::align256(bytes);
+    return at;
+  };
+  L.flag = take(nn * 4);
+  L.pos = take(nn * 4);
+  L.ai = take(nn * 4);
+  L.ix = take(nn * 8);
+  L.iy = take(nn * 8);
+  L.skey = take(np2 * 8);
+  L.srow = take(np2 * 4);
+  L.hcount = take(5 * nn * 4);
+  L.hoff = take(5 * nn * 4);
+  L.ccount = take(5 * nn * 8);
+  L.totals = take((5 * nn / 1024 + 2) * 4);
+  L.sums = take(64);
+  L.ha = take((size_t)(hcap > 0 ? hcap : 1) * 4);
+  L.hb = take((size_t)(hcap > 0 ? hcap : 1) * 4);
+  L.hflag = take((size_t)(hcap > 0 ? hcap : 1) * 4);
+  L.hpos = take((size_t)(hcap > 0 ? hcap : 1) * 4);
+  L.total = o;
+  return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t ssim_collide_work_bytes(int n, long long f32_hit_capacity) {
+  return layout(n, f32_hit_capacity).total;
+}
+
+int ssim_collide_hits(int n, const float* gx, const float* gy, const float* gvx,
+                      const float* gvy, const float* rr,
+                      const unsigned char* is_ast, const unsigned char* vis,
+                      float lo, float hi, float halo, double dt_y, double h,
+                      long long f32_hit_capacity, long long pair_capacity,
+                      long long* out_i, long long* out_j, long long* counts,
+                      void* work, size_t work_bytes, void* stream) {
+  SSIM_REQUIRE(n >= 0, "n must be >= 0");
+  SSIM_REQUIRE(h > 0.0, "cell size h must be > 0");
+  SSIM_REQUIRE(counts, "counts must not be null");
+  SSIM_REQUIRE(n == 0 || (gx && gy && gvx && gvy && rr && is_ast && vis),
+               "null input");
+  SSIM_REQUIRE(work && work_bytes >= ssim_collide_work_bytes(n, f32_hit_capacity),
+               "workspace too small");
+  counts[0] = counts[1] = counts[2] = 0;
+  if (n < 2) return SSIM_OK;
+  hipStream_t st = ssim::as_stream(stream);
+  const Layout L = layout(n, f32_hit_capacity);
+  unsigned char* w = static_cast<unsigned char*>(work);
+  int* flag = reinterpret_cast<int*>(w + L.flag);
+  int* pos = reinterpret_cast<int*>(w + L.pos);
+  int* ai = reinterpret_cast<int*>(w + L.ai);
+  long long* ix = reinterpret_cast<long long*>(w + L.ix);
+  long long* iy = reinterpret_cast<long long*>(w + L.iy);
+  unsigned long long* skey = reinterpret_cast<unsigned long long*>(w + L.skey);
+  int* srow = reinterpret_cast<int*>(w + L.srow);
+  int* hcount = reinterpret_cast<int*>(w + L.hcount);
+  int* hoff = reinterpret_cast<int*>(w + L.hoff);
+  unsigned long long* ccount = reinterpret_cast<unsigned long long*>(w + L.ccount);
+  int* totals = reinterpret_cast<int*>(w + L.totals);
+  int* sums = reinterpret_cast<int*>(w + L.sums);
+  unsigned long long* csum = reinterpret_cast<unsigned long long*>(w + L.sums + 8);
+  int* ha = reinterpret_cast<int*>(w + L.ha);
+  int* hb = reinterpret_cast<int*>(w + L.hb);
+  int* hflag = reinterpret_cast<int*>(w + L.hflag);
+  int* hpos = reinterpret_cast<int*>(w + L.hpos);
+
+  // alive rows in ascending body index
+  hipLaunchKernelGGL(k_alive_flags, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
+                     n, gx, is_ast, vis, lo, hi + halo, flag);
+  SSIM_TRY(scan(n, flag, pos, totals, sums, st));
+  int na = 0;
+  SSIM_HIP(hipMemcpyAsync(&na, sums, sizeof(int), hipMemcpyDeviceToHost, st));
+  SSIM_HIP(hipStreamSynchronize(st));
+  if (na < 2) return SSIM_OK;
+  hipLaunchKernelGGL(k_compact_alive, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0,
+                     st, n, flag, pos, ai);
+  // keys, stable sort by (key, row)
+  const int np2 = next_pow2(na);
+  hipLaunchKernelGGL(k_keys, dim3(ssim::ceil_div(np2, CB)), dim3(CB), 0, st, na,
+                     np2, ai, gx, gy, h, ix, iy, skey, srow);
+  for (int k = 2; k <= np2; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1)
+      hipLaunchKernelGGL(k_bitonic, dim3(ssim::ceil_div(np2, CB)), dim3(CB), 0,
+                         st, np2, j, k, skey, srow);
+  SSIM_LAUNCHED("k_bitonic");
+  // count pass
+  Pf P{gx, gy, gvx, gvy, rr, (float)dt_y, 1e-6f};
+  const int nrows = 5 * na;
+  hipLaunchKernelGGL(k_candidates, dim3(ssim::ceil_div(nrows, CB)), dim3(CB), 0,
+                     st, na, ai, ix, iy, skey, srow, P, 0, hcount, ccount,
+                     nullptr, nullptr, nullptr);
+  SSIM_LAUNCHED("k_candidates");
+  SSIM_TRY(scan(nrows, hcount, hoff, totals, sums, st));
+  hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(1024), 0, st, nrows, ccount, csum);
+  int nh = 0;
+  unsigned long long ncand = 0;
+  SSIM_HIP(hipMemcpyAsync(&nh, sums, sizeof(int), hipMemcpyDeviceToHost, st));
+  SSIM_HIP(hipMemcpyAsync(&ncand, csum, sizeof(ncand), hipMemcpyDeviceToHost, st));
+  SSIM_HIP(hipStreamSynchronize(st));
+  counts[0] = (long long)ncand;
+  counts[1] = nh;
+  if (nh == 0) return SSIM_OK;
+  if (nh > f32_hit_capacity) {  // caller retries with a larger workspace
+    counts[2] = -1;
+    return SSIM_OK;
+  }
+  hipLaunchKernelGGL(k_candidates, dim3(ssim::ceil_div(nrows, CB)), dim3(CB), 0,
+                     st, na, ai, ix, iy, skey, srow, P, 1, hcount, ccount, hoff,
+                     ha, hb);
+  hipLaunchKernelGGL(k_recheck, dim3(ssim::ceil_div(nh, CB)), dim3(CB), 0, st, nh,
+                     ha, hb, P, dt_y, hi, hflag);
+  SSIM_TRY(scan(nh, hflag, hpos, totals, sums, st));
+  hipLaunchKernelGGL(k_compact_pairs, dim3(ssim::ceil_div(nh, CB)), dim3(CB), 0,
+                     st, nh, hflag, hpos, ha, hb, out_i, out_j,
+                     (out_i && out_j) ? pair_capacity : 0);
+  SSIM_LAUNCHED("k_compact_pairs");
+  int np = 0;
+  SSIM_HIP(hipMemcpyAsync(&np, sums, sizeof(int), hipMemcpyDeviceToHost, st));
+  SSIM_HIP(hipStreamSynchronize(st));
+  counts[2] = np;
+  return SSIM_OK;
+}
+
+}  // extern "C"

