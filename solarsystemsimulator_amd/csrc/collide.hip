@@ -311,7 +311,9 @@ Layout layout(int n, long long hcap) {
   L.hcount = take(5 * nn * 4);
   L.hoff = take(5 * nn * 4);
   L.ccount = take(5 * nn * 8);
-  L.totals = take((5 * nn / 1024 + 2) * 4);
+  // block totals of the largest scan: 5 rows per body or the FP32 hits
+  const size_t scan_max = 5 * nn > (size_t)hcap ? 5 * nn : (size_t)hcap;
+  L.totals = take((scan_max / 1024 + 2) * 4);
   L.sums = take(64);
   L.ha = take((size_t)(hcap > 0 ? hcap : 1) * 4);
   L.hb = take((size_t)(hcap > 0 ? hcap : 1) * 4);
