@@ -296,6 +296,45 @@ int ssim_collide_hits(int n, const float* gx, const float* gy, const float* gvx,
                       long long* out_i, long long* out_j, long long* counts,
                       void* work, size_t work_bytes, void* stream);
 
+/* Massive-body contacts and the runaway guard of one sample
+ * (`analyze_merge`, film_producer.py:707-779), FP32 like the reference:
+ * for each of the nm alive massive bodies m_alive[r] and every visible body
+ * j != m_alive[r], does the segment prev -> cur (frac = tt - 1) or
+ * cur -> cur + v dt (frac = tt) pass within rr[m] + rr[j]?  Hits come out
+ * row-major (row r, column j ascending) as (out_row, out_col, out_frac) up
+ * to `capacity`; counts[0] = number of hits.  With do_runaway, the indices
+ * of asteroids with v^2 > 9 (two_gm / max(|x - b|, 1e-6)) (b = (bx, by),
+ * the massive barycentre) go to `runaway` in ascending order, counts[1] =
+ * their number.  Synchronises the stream. */
+size_t ssim_merge_work_bytes(int n, int nm);
+int ssim_merge_scan(int n, const float* gx, const float* gy, const float* gvx,
+                    const float* gvy, const float* px, const float* py,
+                    const float* rr, const unsigned char* vis,
+                    const unsigned char* is_ast, int nm, const int* m_alive,
+                    float dt, int do_runaway, float bx, float by, float two_gm,
+                    long long capacity, int* out_row, int* out_col,
+                    float* out_frac, long long runaway_capacity,
+                    long long* runaway, long long* counts, void* work,
+                    size_t work_bytes, void* stream);
+
+/* Bounce response of a contact list (`bounce_deltas`, film_producer.py:
+ * 916-1019, which the reference runs on the host): the greedy
+ * one-contact-per-body filter in list order, the entry time of
+ * |dp + dv t| = r_i + r_j clipped to [-dt_y, dt_y], validity, approach
+ * test, the impulse -(1+e) v_rel / (1/m_i + 1/m_j) and the mass-weighted
+ * 1.1x overlap push, in FP64 on the f32 sample.  Dense outputs ddx, ddy,
+ * ddvx, ddvy [n] (zero for untouched bodies); act_pairs = indices into the
+ * list of the pairs that bounced (counts[0] of them, list order);
+ * affected = bodies with a delta, ascending (counts[1]).  Synchronises. */
+size_t ssim_bounce_work_bytes(int n, int npairs);
+int ssim_bounce_deltas(int n, int npairs, const long long* hi, const long long* hj,
+                       const float* x, const float* y, const float* vx,
+                       const float* vy, const double* real_r, const double* mass,
+                       double dt_y, double restitution, double* ddx, double* ddy,
+                       double* ddvx, double* ddvy, long long* act_pairs,
+                       long long* affected, long long* counts, void* work,
+                       size_t work_bytes, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* Learned near-field correction (MLP, BF16 MFMA, FP32 accumulation)   */
 /* ------------------------------------------------------------------ */

@@ -323,9 +323,333 @@ Layout layout(int n, long long hcap) {
   return L;
 }
 
+// ---- massive-body merges and runaway guard (`analyze_merge`,
+// film_producer.py:707-779), FP32 like the reference's CuPy arrays ----
+
+// Does the segment p0 -> p1 pass within rsum of c?  tt = parameter of the
+// closest point in [0, 1] (seg_hit, film_producer.py:730-746).
+__device__ __forceinline__ bool seg_hit(float p0x, float p0y, float p1x,
+                                        float p1y, float cx, float cy,
+                                        float rsum2, float& tt) {
+  const float sx = p1x - p0x, sy = p1y - p0y;
+  const float seg2 = sx * sx + sy * sy;
+  float t = ((cx - p0x) * sx + (cy - p0y) * sy) / (seg2 > 0.0f ? seg2 : 1.0f);
+  t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+  tt = t;
+  const float dx = p0x + t * sx - cx;
+  const float dy = p0y + t * sy - cy;
+  return dx * dx + dy * dy <= rsum2;
+}
+
+struct MergeIn {
+  const float *gx, *gy, *gvx, *gvy, *px, *py, *rr;
+  const unsigned char *vis, *ast;
+  const int* midx;
+  int n, nm;
+  float dt;
+};
+
+// pass 0: per (row) hit counts; pass 1: ordered (row, col, frac) output.
+__global__ __launch_bounds__(CB) void k_merge_rows(MergeIn I, int pass,
+                                                   int* __restrict__ cnt,
+                                                   const int* __restrict__ off,
+                                                   int* __restrict__ orow,
+                                                   int* __restrict__ ocol,
+                                                   float* __restrict__ ofrac) {
+  // one workgroup per (row, block of columns); columns in ascending order
+  const int row = blockIdx.y;
+  const int j = blockIdx.x * CB + threadIdx.x;
+  __shared__ int s_n[CB];
+  const int g = I.midx[row];
+  bool hit = false;
+  float frac = 0.0f;
+  if (j < I.n && j != g && I.vis[j] != 0) {
+    const float cx = I.gx[g], cy = I.gy[g];
+    const float rs = I.rr[g] + I.rr[j];
+    const float rs2 = rs * rs;
+    float ta, tb;
+    const bool ha = seg_hit(I.px[j], I.py[j], I.gx[j], I.gy[j], cx, cy, rs2, ta);
+    const float q1x = I.gx[j] + I.gvx[j] * I.dt;
+    const float q1y = I.gy[j] + I.gvy[j] * I.dt;
+    const bool hb = seg_hit(I.gx[j], I.gy[j], q1x, q1y, cx, cy, rs2, tb);
+    hit = ha || hb;
+    frac = ha ? ta - 1.0f : tb;
+  }
+  // ordered position of this hit inside the workgroup (prefix count)
+  s_n[threadIdx.x] = hit ? 1 : 0;
+  __syncthreads();
+  for (int o = 1; o < CB; o <<= 1) {
+    const int v = threadIdx.x >= o ? s_n[threadIdx.x - o] : 0;
+    __syncthreads();
+    s_n[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const int slot = (int)blockIdx.y * gridDim.x + blockIdx.x;
+  if (pass == 0) {
+    if (threadIdx.x == CB - 1) cnt[slot] = s_n[CB - 1];
+  } else if (hit) {
+    const int w = off[slot] + s_n[threadIdx.x] - 1;
+    orow[w] = row;
+    ocol[w] = j;
+    ofrac[w] = frac;
+  }
+}
+
+__global__ __launch_bounds__(CB) void k_runaway(int n, const float* __restrict__ gx,
+                                                const float* __restrict__ gy,
+                                                const float* __restrict__ gvx,
+                                                const float* __restrict__ gvy,
+                                                const unsigned char* __restrict__ ast,
+                                                const unsigned char* __restrict__ vis,
+                                                float bx, float by, float two_gm,
+                                                int* __restrict__ flag) {
+  const int i = blockIdx.x * CB + threadIdx.x;
+  if (i >= n) return;
+  const float dx = gx[i] - bx, dy = gy[i] - by;
+  float r = hypotf(dx, dy);
+  r = r > 1e-6f ? r : 1e-6f;
+  const float v2 = gvx[i] * gvx[i] + gvy[i] * gvy[i];
+  const float vesc2 = two_gm / r;
+  flag[i] = (v2 > 9.0f * vesc2 && ast[i] && vis[i] != 0) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(CB) void k_compact_idx(int n, const int* __restrict__ flag,
+                                                    const int* __restrict__ pos,
+                                                    long long* __restrict__ out) {
+  const int i = blockIdx.x * CB + threadIdx.x;
+  if (i < n && flag[i]) out[pos[i]] = i;
+}
+
+// ---- bounce response (`bounce_deltas`, film_producer.py:916-1019) ----
+
+// Greedy one-contact-per-body filter in list order: a sequential rule, run
+// by one thread (the lists hold at most thousands of pairs).
+__global__ void k_bounce_greedy(int np, const long long* __restrict__ hi,
+                                const long long* __restrict__ hj,
+                                unsigned char* __restrict__ used,
+                                int* __restrict__ keep) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int t = 0; t < np; ++t) {
+    const long long a = hi[t], b = hj[t];
+    if (used[a] || used[b]) {
+      keep[t] = 0;
+      continue;
+    }
+    used[a] = 1;
+    used[b] = 1;
+    keep[t] = 1;
+  }
+}
+
+__global__ __launch_bounds__(CB) void k_zero_u8(long long n, unsigned char* __restrict__ a) {
+  const long long i = (long long)blockIdx.x * CB + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+__global__ __launch_bounds__(CB) void k_zero_f64(long long n, double* __restrict__ a) {
+  const long long i = (long long)blockIdx.x * CB + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
+// Impulse and overlap push of every kept pair; each body is in at most one
+// kept pair, so the dense delta arrays are written without conflicts.
+__global__ __launch_bounds__(CB) void k_bounce_pairs(
+    int np, const long long* __restrict__ hi, const long long* __restrict__ hj,
+    const int* __restrict__ keep, const float* __restrict__ x,
+    const float* __restrict__ y, const float* __restrict__ vx,
+    const float* __restrict__ vy, const double* __restrict__ rr,
+    const double* __restrict__ mass, double dt, double e,
+    double* __restrict__ ddx, double* __restrict__ ddy, double* __restrict__ ddvx,
+    double* __restrict__ ddvy, int* __restrict__ act, unsigned char* __restrict__ aff) {
+  const int t = blockIdx.x * CB + threadIdx.x;
+  if (t >= np) return;
+  act[t] = 0;
+  if (!keep[t]) return;
+  const long long a = hi[t], b = hj[t];
+  const double dpx = (double)x[b] - (double)x[a];
+  const double dpy = (double)y[b] - (double)y[a];
+  const double dvx = (double)vx[b] - (double)vx[a];
+  const double dvy = (double)vy[b] - (double)vy[a];
+  const double dv2 = dvx * dvx + dvy * dvy;
+  const double touch = rr[a] + rr[b];
+  const double c = dpx * dpx + dpy * dpy - touch * touch;
+  const double bq = 2.0 * (dpx * dvx + dpy * dvy);
+  const double disc = bq * bq - 4.0 * dv2 * c;
+  const double sq = sqrt(disc > 0.0 ? disc : 0.0);
+  double tin = (-bq - sq) / (2.0 * (dv2 > 0.0 ? dv2 : 1.0));
+  if (tin != tin) tin = 0.0;  // nan_to_num
+  else if (isinf(tin)) tin = tin > 0 ? 1.7976931348623157e308 : -1.7976931348623157e308;
+  const double tc = c < 0.0 ? 0.0 : (tin < -dt ? -dt : (tin > dt ? dt : tin));
+  const bool valid = (c < 0.0) || (disc >= -(touch * touch) * 0.01);
+  const double ncx = dpx + dvx * tc, ncy = dpy + dvy * tc;
+  double dist = hypot(ncx, ncy);
+  dist = dist > 1e-30 ? dist : 1.0;
+  const double nx = ncx / dist, ny = ncy / dist;
+  const double vrel = dvx * nx + dvy * ny;
+  if (!(valid && vrel < 0.0)) return;
+  act[t] = 1;
+  const double mi = mass[a], mj = mass[b];
+  double imp = -(1.0 + e) * vrel / (1.0 / mi + 1.0 / mj);
+  if (imp != imp || isinf(imp)) imp = imp != imp ? 0.0 : (imp > 0 ? 1.7976931348623157e308 : -1.7976931348623157e308);
+  ddvx[a] -= imp * nx / mi;
+  ddvy[a] -= imp * ny / mi;
+  ddvx[b] += imp * nx / mj;
+  ddvy[b] += imp * ny / mj;
+  const double pdist = hypot(dpx, dpy);
+  const double overlap = touch - pdist;
+  if (overlap > 0.0 && pdist > 1e-30) {
+    const double pnx = dpx / pdist, pny = dpy / pdist;
+    const double mt = mi + mj;
+    const double shift = overlap * 1.1;
+    ddx[a] -= pnx * shift * (mj / mt);
+    ddy[a] -= pny * shift * (mj / mt);
+    ddx[b] += pnx * shift * (mi / mt);
+    ddy[b] += pny * shift * (mi / mt);
+  }
+  aff[a] = 1;
+  aff[b] = 1;
+}
+
+__global__ __launch_bounds__(CB) void k_u8_to_i32(int n, const unsigned char* __restrict__ a,
+                                                  int* __restrict__ b) {
+  const int i = blockIdx.x * CB + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
 }  // namespace
 
 extern "C" {
+
+size_t ssim_bounce_work_bytes(int n, int npairs) {
+  const size_t nn = (size_t)(n > 0 ? n : 1), np = (size_t)(npairs > 0 ? npairs : 1);
+  const size_t big = nn > np ? nn : np;
+  return ssim::align256(nn) * 2 + ssim::align256(np * 4) * 3 +
+         ssim::align256(nn * 4) * 2 + ssim::align256((big / 1024 + 2) * 4) + 256;
+}
+
+int ssim_bounce_deltas(int n, int npairs, const long long* hi, const long long* hj,
+                       const float* x, const float* y, const float* vx,
+                       const float* vy, const double* real_r, const double* mass,
+                       double dt_y, double restitution, double* ddx, double* ddy,
+                       double* ddvx, double* ddvy, long long* act_pairs,
+                       long long* affected, long long* counts, void* work,
+                       size_t work_bytes, void* stream) {
+  SSIM_REQUIRE(n >= 0 && npairs >= 0, "bad sizes");
+  SSIM_REQUIRE(counts && work && work_bytes >= ssim_bounce_work_bytes(n, npairs),
+               "workspace too small");
+  counts[0] = counts[1] = 0;
+  if (npairs == 0 || n == 0) return SSIM_OK;
+  SSIM_REQUIRE(hi && hj && x && y && vx && vy && real_r && mass && ddx && ddy &&
+                   ddvx && ddvy && act_pairs && affected,
+               "null pointer");
+  hipStream_t st = ssim::as_stream(stream);
+  const size_t nn = (size_t)n, np = (size_t)npairs;
+  unsigned char* w = static_cast<unsigned char*>(work);
+  size_t o = 0;
+  unsigned char* used = w + o; o += ssim::align256(nn);
+  unsigned char* aff = w + o; o += ssim::align256(nn);
+  int* keep = reinterpret_cast<int*>(w + o); o += ssim::align256(np * 4);
+  int* act = reinterpret_cast<int*>(w + o); o += ssim::align256(np * 4);
+  int* apos = reinterpret_cast<int*>(w + o); o += ssim::align256(np * 4);
+  int* affi = reinterpret_cast<int*>(w + o); o += ssim::align256(nn * 4);
+  int* affp = reinterpret_cast<int*>(w + o); o += ssim::align256(nn * 4);
+  int* totals = reinterpret_cast<int*>(w + o);
+  int* sums = reinterpret_cast<int*>(w + work_bytes - 256);
+  const dim3 gn(ssim::ceil_div(n, CB)), gp(ssim::ceil_div(npairs, CB));
+  hipLaunchKernelGGL(k_zero_u8, gn, dim3(CB), 0, st, (long long)n, used);
+  hipLaunchKernelGGL(k_zero_u8, gn, dim3(CB), 0, st, (long long)n, aff);
+  for (double* d : {ddx, ddy, ddvx, ddvy})
+    hipLaunchKernelGGL(k_zero_f64, gn, dim3(CB), 0, st, (long long)n, d);
+  hipLaunchKernelGGL(k_bounce_greedy, dim3(1), dim3(64), 0, st, npairs, hi, hj,
+                     used, keep);
+  hipLaunchKernelGGL(k_bounce_pairs, gp, dim3(CB), 0, st, npairs, hi, hj, keep,
+                     x, y, vx, vy, real_r, mass, dt_y, restitution, ddx, ddy,
+                     ddvx, ddvy, act, aff);
+  SSIM_LAUNCHED("k_bounce_pairs");
+  // active pairs in list order, affected bodies ascending
+  SSIM_TRY(scan(npairs, act, apos, totals, sums, st));
+  int na = 0;
+  SSIM_HIP(hipMemcpyAsync(&na, sums, sizeof(int), hipMemcpyDeviceToHost, st));
+  hipLaunchKernelGGL(k_compact_idx, gp, dim3(CB), 0, st, npairs, act, apos,
+                     act_pairs);
+  hipLaunchKernelGGL(k_u8_to_i32, gn, dim3(CB), 0, st, n, aff, affi);
+  SSIM_TRY(scan(n, affi, affp, totals, sums + 1, st));
+  int nf = 0;
+  SSIM_HIP(hipMemcpyAsync(&nf, sums + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  hipLaunchKernelGGL(k_compact_idx, gn, dim3(CB), 0, st, n, affi, affp, affected);
+  SSIM_LAUNCHED("k_compact_idx");
+  SSIM_HIP(hipStreamSynchronize(st));
+  counts[0] = na;
+  counts[1] = nf;
+  return SSIM_OK;
+}
+
+size_t ssim_merge_work_bytes(int n, int nm) {
+  const size_t slots = (size_t)(nm > 0 ? nm : 1) * ssim::ceil_div(n > 0 ? n : 1, CB);
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  return 2 * ssim::align256(slots * 4) + 2 * ssim::align256(nn * 4) +
+         ssim::align256(((slots > nn ? slots : nn) / 1024 + 2) * 4) + 256;
+}
+
+int ssim_merge_scan(int n, const float* gx, const float* gy, const float* gvx,
+                    const float* gvy, const float* px, const float* py,
+                    const float* rr, const unsigned char* vis,
+                    const unsigned char* is_ast, int nm, const int* m_alive,
+                    float dt, int do_runaway, float bx, float by, float two_gm,
+                    long long capacity, int* out_row, int* out_col,
+                    float* out_frac, long long runaway_capacity,
+                    long long* runaway, long long* counts, void* work,
+                    size_t work_bytes, void* stream) {
+  SSIM_REQUIRE(n >= 0 && nm >= 0 && nm <= 64, "bad sizes (n=%d nm=%d)", n, nm);
+  SSIM_REQUIRE(counts && work && work_bytes >= ssim_merge_work_bytes(n, nm),
+               "workspace too small");
+  SSIM_REQUIRE(n == 0 || (gx && gy && gvx && gvy && px && py && rr && vis && is_ast),
+               "null input");
+  counts[0] = counts[1] = 0;
+  if (n == 0) return SSIM_OK;
+  hipStream_t st = ssim::as_stream(stream);
+  const int cols = ssim::ceil_div(n, CB);
+  const size_t slots = (size_t)(nm > 0 ? nm : 1) * cols;
+  const size_t nn = (size_t)n;
+  unsigned char* w = static_cast<unsigned char*>(work);
+  int* cnt = reinterpret_cast<int*>(w);
+  int* off = reinterpret_cast<int*>(w + ssim::align256(slots * 4));
+  int* flag = reinterpret_cast<int*>(w + 2 * ssim::align256(slots * 4));
+  int* pos = reinterpret_cast<int*>(w + 2 * ssim::align256(slots * 4) + ssim::align256(nn * 4));
+  int* totals = reinterpret_cast<int*>(w + 2 * ssim::align256(slots * 4) +
+                                       2 * ssim::align256(nn * 4));
+  int* sums = reinterpret_cast<int*>(w + work_bytes - 256);
+  if (nm > 0) {
+    SSIM_REQUIRE(m_alive, "null massive index list");
+    MergeIn I{gx, gy, gvx, gvy, px, py, rr, vis, is_ast, m_alive, n, nm, dt};
+    hipLaunchKernelGGL(k_merge_rows, dim3(cols, nm), dim3(CB), 0, st, I, 0, cnt,
+                       nullptr, nullptr, nullptr, nullptr);
+    SSIM_TRY(scan((int)slots, cnt, off, totals, sums, st));
+    int nh = 0;
+    SSIM_HIP(hipMemcpyAsync(&nh, sums, sizeof(int), hipMemcpyDeviceToHost, st));
+    SSIM_HIP(hipStreamSynchronize(st));
+    counts[0] = nh;
+    if (nh > 0 && nh <= capacity && out_row && out_col && out_frac)
+      hipLaunchKernelGGL(k_merge_rows, dim3(cols, nm), dim3(CB), 0, st, I, 1,
+                         cnt, off, out_row, out_col, out_frac);
+    SSIM_LAUNCHED("k_merge_rows");
+  }
+  if (do_runaway) {
+    hipLaunchKernelGGL(k_runaway, dim3(cols), dim3(CB), 0, st, n, gx, gy, gvx,
+                       gvy, is_ast, vis, bx, by, two_gm, flag);
+    SSIM_TRY(scan(n, flag, pos, totals, sums, st));
+    int nr = 0;
+    SSIM_HIP(hipMemcpyAsync(&nr, sums, sizeof(int), hipMemcpyDeviceToHost, st));
+    SSIM_HIP(hipStreamSynchronize(st));
+    counts[1] = nr;
+    if (nr > 0 && nr <= runaway_capacity && runaway)
+      hipLaunchKernelGGL(k_compact_idx, dim3(cols), dim3(CB), 0, st, n, flag, pos,
+                         runaway);
+    SSIM_LAUNCHED("k_runaway");
+  }
+  SSIM_HIP(hipStreamSynchronize(st));
+  return SSIM_OK;
+}
 
 size_t ssim_collide_work_bytes(int n, long long f32_hit_capacity) {
   return layout(n, f32_hit_capacity).total;

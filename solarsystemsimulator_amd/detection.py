@@ -99,6 +99,81 @@ class Erkennungskarte:
         return (oi[:npairs].cpu().numpy(), oj[:npairs].cpu().numpy()), cand
 
 
+def merge_scan(dev, gx, gy, gvx, gvy, px, py, g_rr, g_vis, g_ast, m_alive,
+               dt_years, barycentre=None, two_gm=0.0):
+    """Device part of `analyze_merge` (film_producer.py:707-779).
+    Returns ([(row, j, frac)], runaway indices | None); rows index
+    `m_alive`.  barycentre=(bx, by) enables the runaway guard."""
+    torch = _torch()
+    n = gx.numel()
+    nm = len(m_alive)
+    lib = _lib.load()
+    wb = lib.ssim_merge_work_bytes(n, nm)
+    work = D.workspace(wb, dev)
+    midx = D.to_dev(np.asarray(m_alive, np.int32), dev, torch.int32) if nm else None
+    counts = (_lib.ctypes.c_longlong * 2)()
+    cap = 4096
+    while True:
+        rows = torch.empty(cap, dtype=torch.int32, device=dev)
+        cols = torch.empty(cap, dtype=torch.int32, device=dev)
+        fr = torch.empty(cap, dtype=torch.float32, device=dev)
+        run = torch.empty(cap, dtype=torch.int64, device=dev)
+        bx, by = barycentre if barycentre is not None else (0.0, 0.0)
+        _lib.call("ssim_merge_scan", n, *[_lib.ptr(a) for a in
+                                          (gx, gy, gvx, gvy, px, py)],
+                  _lib.ptr(g_rr), _lib.ptr(g_vis), _lib.ptr(g_ast), nm,
+                  _lib.ptr(midx), float(np.float32(dt_years)),
+                  int(barycentre is not None), float(bx), float(by),
+                  float(np.float32(two_gm)), cap, _lib.ptr(rows), _lib.ptr(cols),
+                  _lib.ptr(fr), cap, _lib.ptr(run),
+                  _lib.ctypes.addressof(counts), _lib.ptr(work), wb,
+                  D.stream(dev))
+        if max(counts[0], counts[1]) <= cap:
+            break
+        cap = int(max(counts[0], counts[1]))
+    nh, nr = int(counts[0]), int(counts[1])
+    pairs = list(zip(rows[:nh].cpu().numpy().tolist(), cols[:nh].cpu().numpy().tolist(),
+                     fr[:nh].cpu().numpy().tolist()))
+    runaway = run[:nr].cpu().numpy() if nr else None
+    return pairs, runaway
+
+
+def bounce_deltas(dev, sample, hits, real_r, mass, dt_y, restitution=0.6):
+    """Bounce response of a contact list on the device (the host function
+    `bounce_deltas`, film_producer.py:916-1019): (hi, hj, affected,
+    deltas[k, 4]) or None.  `sample` is the f32 (4n) [x|y|vx|vy] host row."""
+    torch = _torch()
+    n = len(real_r)
+    hi_h = np.asarray(hits[0], np.int64)
+    hj_h = np.asarray(hits[1], np.int64)
+    npairs = len(hi_h)
+    if npairs == 0:
+        return None
+    s = np.asarray(sample, np.float32)
+    x, y, vx, vy = (D.to_dev(s[k * n:(k + 1) * n], dev, torch.float32) for k in range(4))
+    rr = D.to_dev(np.asarray(real_r, np.float64), dev)
+    ms = D.to_dev(np.asarray(mass, np.float64), dev)
+    hi, hj = D.to_dev(hi_h, dev, torch.int64), D.to_dev(hj_h, dev, torch.int64)
+    d = [torch.empty(n, dtype=torch.float64, device=dev) for _ in range(4)]
+    act = torch.empty(npairs, dtype=torch.int64, device=dev)
+    aff = torch.empty(n, dtype=torch.int64, device=dev)
+    wb = _lib.load().ssim_bounce_work_bytes(n, npairs)
+    work = D.workspace(wb, dev)
+    counts = (_lib.ctypes.c_longlong * 2)()
+    _lib.call("ssim_bounce_deltas", n, npairs, _lib.ptr(hi), _lib.ptr(hj),
+              _lib.ptr(x), _lib.ptr(y), _lib.ptr(vx), _lib.ptr(vy), _lib.ptr(rr),
+              _lib.ptr(ms), float(dt_y), float(restitution),
+              *[_lib.ptr(c) for c in d], _lib.ptr(act), _lib.ptr(aff),
+              _lib.ctypes.addressof(counts), _lib.ptr(work), wb, D.stream(dev))
+    na, nf = int(counts[0]), int(counts[1])
+    if na == 0:
+        return None
+    a = act[:na].cpu().numpy()
+    f = aff[:nf].cpu().numpy()
+    deltas = np.stack([c.cpu().numpy()[f] for c in d], axis=1)
+    return hi_h[a], hj_h[a], f.astype(np.int64), deltas
+
+
 def _streifengrenzen(gx_lebt, anteile):
     """x stripes holding `anteile` of the alive asteroids (outer bounds
     infinite), film_producer.py:382-405."""
