@@ -347,27 +347,33 @@ int ssim_bounce_deltas(int n, int npairs, const long long* hi, const long long* 
 int ssim_mlp_param_count(void);
 /* wbf = BF16 (round to nearest even) of params. */
 int ssim_mlp_to_bf16(const float* params, unsigned short* wbf, void* stream);
-/* Features of nb bodies from the CIC mass mesh (mesh_n^3, C order, FP64 or
- * FP32 per mesh_prec) with origin[3] and cell h: the 27 node masses of the
- * 3x3x3 stencil around the nearest node divided by mass_unit, then twice the
- * offset from that node in cells, then two zeros. */
-int ssim_mlp_features(int nb, const double* x, const double* y, const double* z,
-                      int mesh_n, const double* origin, double h,
+/* Features of nb bodies (the first n_mass are masses, the rest tracers)
+ * from the CIC mass mesh (mesh_n^3, C order, FP64 or FP32 per mesh_prec)
+ * with origin[3] and cell h: the 27 node masses of the 3x3x3 stencil around
+ * the nearest node divided by mass_unit, twice the offset from that node in
+ * cells, 1 for a mass / 0 for a tracer, and a zero. */
+int ssim_mlp_features(int nb, int n_mass, const double* x, const double* y,
+                      const double* z, int mesh_n, const double* origin, double h,
                       int mesh_prec, const void* rho, double mass_unit,
                       unsigned short* feat, void* stream);
 /* Forward pass.  out (optional) receives FP32 [nb][3]; if ax != NULL,
- * scale * out is ADDED to (ax, ay, az).  For training pass zs (FP32
- * [3][nb][64] pre-activations) and hs (BF16 [3][nb][64] activations). */
+ * u * out is ADDED to (ax, ay, az) with u = scale, or, if relative != 0,
+ * u = scale * |a| of the acceleration being corrected.  For training pass
+ * zs (FP32 [3][nb][64] pre-activations) and hs (BF16 [3][nb][64]
+ * activations). */
 int ssim_mlp_forward(int nb, const unsigned short* feat,
                      const unsigned short* wbf, const float* params,
-                     float scale, float* out, double* ax, double* ay,
-                     double* az, float* zs, unsigned short* hs, void* stream);
-/* Training target: out[i][c] = (ref_c[i] - mod_c[i]) / a_unit (FP32),
- * the residual of the model force against the exact direct sum. */
+                     float scale, int relative, float* out, double* ax,
+                     double* ay, double* az, float* zs, unsigned short* hs,
+                     void* stream);
+/* Training target: out[i][c] = (ref_c[i] - mod_c[i]) / u (FP32), the
+ * residual of the model force against the exact direct sum, with u = a_unit
+ * or, if relative != 0, u = a_unit * |mod[i]|; clipped to [-clip, clip]
+ * when clip > 0. */
 int ssim_mlp_residual(int nb, const double* ref_x, const double* ref_y,
                       const double* ref_z, const double* mod_x,
                       const double* mod_y, const double* mod_z, double a_unit,
-                      float* out, void* stream);
+                      int relative, double clip, float* out, void* stream);
 /* gout = 2 (out - target) / nb; loss_part[block] = partial sums of the
  * squared error over blocks of 256 bodies. */
 int ssim_mlp_mse_grad(int nb, const float* out, const float* target,

@@ -116,8 +116,9 @@ def adam(p, g, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
     return p - lr * (m / c1) / (np.sqrt(v / c2) + eps), m, v
 
 
-def features(pos, rho, origin, h, mass_unit):
-    """27 stencil node masses around the nearest node + 2x offsets (n, 32)."""
+def features(pos, rho, origin, h, mass_unit, n_mass=0):
+    """27 stencil node masses around the nearest node + 2x offsets + the
+    mass flag (1 for the first n_mass bodies) (n, 32)."""
     n = rho.shape[0]
     f = np.stack([(np.asarray(pos[a]) - origin[a]) / h for a in range(3)])
     c = np.floor(f + 0.5)
@@ -135,4 +136,5 @@ def features(pos, rho, origin, h, mass_unit):
                 out[:, q] = v / mass_unit
                 q += 1
     out[:, 27:30] = off.T
+    out[:n_mass, 30] = 1.0
     return out

@@ -95,12 +95,12 @@ SIGNATURES = {
                            + [_P] * 7 + [_P, c_size_t, _P]),
     "ssim_mlp_param_count": (c_int, []),
     "ssim_mlp_to_bf16": (c_int, [_P, _P, _P]),
-    "ssim_mlp_features": (c_int, [c_int, _P, _P, _P, c_int, _P, c_double, c_int,
-                                  _P, c_double, _P, _P]),
-    "ssim_mlp_forward": (c_int, [c_int, _P, _P, _P, c_float, _P, _P, _P, _P, _P,
-                                 _P, _P]),
-    "ssim_mlp_residual": (c_int, [c_int, _P, _P, _P, _P, _P, _P, c_double, _P,
-                                  _P]),
+    "ssim_mlp_features": (c_int, [c_int, c_int, _P, _P, _P, c_int, _P, c_double,
+                                  c_int, _P, c_double, _P, _P]),
+    "ssim_mlp_forward": (c_int, [c_int, _P, _P, _P, c_float, c_int, _P, _P, _P,
+                                 _P, _P, _P, _P]),
+    "ssim_mlp_residual": (c_int, [c_int, _P, _P, _P, _P, _P, _P, c_double, c_int,
+                                  c_double, _P, _P]),
     "ssim_mlp_mse_grad": (c_int, [c_int, _P, _P, _P, _P, _P]),
     "ssim_mlp_train_work_bytes": (c_size_t, [c_int]),
     "ssim_mlp_backward": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, c_size_t,

@@ -7,9 +7,9 @@
 //   order[n]                body indices sorted by cell, ascending body
 //                           index inside a cell (deterministic),
 //   packed[n]               float4 (x - ox, y - oy, z - oz, m) in that order.
-// Steps: key + histogram (atomics on the counts), one-workgroup exclusive
-// scan, scatter through per-cell cursors (atomics, arbitrary order), then a
-// per-cell insertion sort by body index, which restores a fixed order.
+// Steps: key + histogram (atomics on the counts), a stable radix sort of
+// (key, index) (sort.hip) that leaves ascending body indices inside a cell,
+// an exclusive scan of the counts, and the packed copy.
 #include "ssim_internal.h"
 
 namespace {
@@ -29,7 +29,7 @@ __global__ __launch_bounds__(CB) void k_cell_key(
     int n, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, double ox, double oy, double oz,
     double inv_cell, int nc0, int nc1, int nc2, int* __restrict__ key,
-    int* __restrict__ count) {
+    int* __restrict__ order, int* __restrict__ count) {
   const int i = blockIdx.x * CB + threadIdx.x;
   if (i >= n) return;
   const int a = clampi((int)floor((x[i] - ox) * inv_cell), 0, nc0 - 1);
@@ -37,68 +37,20 @@ __global__ __launch_bounds__(CB) void k_cell_key(
   const int c = clampi((int)floor((z[i] - oz) * inv_cell), 0, nc2 - 1);
   const int k = (a * nc1 + b) * nc2 + c;
   key[i] = k;
+  order[i] = i;
   atomicAdd(&count[k], 1);
 }
 
-// Exclusive scan of count[0..nc) into start[0..nc], one workgroup.
-__global__ __launch_bounds__(1024) void k_scan(int nc, const int* __restrict__ count,
-                                               int* __restrict__ start) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int per = (nc + 1023) / 1024;
-  const int b0 = min(nc, t * per), b1 = min(nc, b0 + per);
-  int s = 0;
-  for (int i = b0; i < b1; ++i) s += count[i];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
-    const int v = t >= off ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int run = t == 0 ? 0 : part[t - 1];
-  for (int i = b0; i < b1; ++i) {
-    start[i] = run;
-    run += count[i];
-  }
-  if (t == 1023) start[nc] = part[1023];
-}
-
-__global__ __launch_bounds__(CB) void k_scatter(int n, const int* __restrict__ key,
-                                                const int* __restrict__ start,
-                                                int* __restrict__ cursor,
-                                                int* __restrict__ order) {
-  const int i = blockIdx.x * CB + threadIdx.x;
-  if (i >= n) return;
-  const int k = key[i];
-  const int slot = atomicAdd(&cursor[k], 1);
-  order[start[k] + slot] = i;
-}
-
-__global__ __launch_bounds__(CB) void k_sort_cells_pack(
-    int nc, const int* __restrict__ start, int* __restrict__ order,
-    const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, const double* __restrict__ m, double ox,
-    double oy, double oz, float4* __restrict__ packed) {
-  const int k = blockIdx.x * CB + threadIdx.x;
-  if (k >= nc) return;
-  const int b = start[k], e = start[k + 1];
-  for (int i = b + 1; i < e; ++i) {  // insertion sort: cells hold few bodies
-    const int v = order[i];
-    int j = i - 1;
-    while (j >= b && order[j] > v) {
-      order[j + 1] = order[j];
-      --j;
-    }
-    order[j + 1] = v;
-  }
-  if (!packed) return;
-  for (int i = b; i < e; ++i) {
-    const int p = order[i];
-    packed[i] = make_float4((float)(x[p] - ox), (float)(y[p] - oy),
-                            (float)(z[p] - oz), m ? (float)m[p] : 0.0f);
-  }
+__global__ __launch_bounds__(CB) void k_pack(
+    int n, const int* __restrict__ order, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, double ox, double oy, double oz,
+    float4* __restrict__ packed) {
+  const int s = blockIdx.x * CB + threadIdx.x;
+  if (s >= n) return;
+  const int p = order[s];
+  packed[s] = make_float4((float)(x[p] - ox), (float)(y[p] - oy),
+                          (float)(z[p] - oz), m ? (float)m[p] : 0.0f);
 }
 
 }  // namespace
@@ -107,7 +59,8 @@ extern "C" {
 
 size_t ssim_cells_work_bytes(int n, int ncells) {
   return ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int)) +
-         2 * ssim::align256((size_t)(ncells > 0 ? ncells : 1) * sizeof(int));
+         ssim::align256((size_t)(ncells > 0 ? ncells : 1) * sizeof(int)) +
+         ssim::radix_sort_work_bytes(n);
 }
 
 int ssim_cells_build(int n, const double* x, const double* y, const double* z,
@@ -126,34 +79,31 @@ int ssim_cells_build(int n, const double* x, const double* y, const double* z,
   hipStream_t st = ssim::as_stream(stream);
   unsigned char* w = static_cast<unsigned char*>(work);
   int* key = reinterpret_cast<int*>(w);
-  int* count = reinterpret_cast<int*>(w + ssim::align256((size_t)(n > 0 ? n : 1) * 4));
-  int* cursor = count + ssim::align256((size_t)ncells * 4) / 4;
+  w += ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int));
+  int* count = reinterpret_cast<int*>(w);
+  w += ssim::align256((size_t)ncells * sizeof(int));
   // zeroed by a kernel (not hipMemsetAsync) so the sequence can be captured
   // in a HIP graph
   hipLaunchKernelGGL(k_zero_i32, dim3(ssim::ceil_div(ncells, CB)), dim3(CB), 0,
                      st, ncells, count);
-  hipLaunchKernelGGL(k_zero_i32, dim3(ssim::ceil_div(ncells, CB)), dim3(CB), 0,
-                     st, ncells, cursor);
   SSIM_LAUNCHED("k_zero_i32");
   if (n > 0) {
     hipLaunchKernelGGL(k_cell_key, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
                        n, x, y, z, origin[0], origin[1], origin[2], 1.0 / cell,
-                       nc[0], nc[1], nc[2], key, count);
+                       nc[0], nc[1], nc[2], key, order, count);
     SSIM_LAUNCHED("k_cell_key");
+    int bits = 1;
+    while (bits < 31 && (1LL << bits) < ncl) ++bits;
+    SSIM_TRY(ssim::radix_sort_pairs(n, bits, key, order, w,
+                                    ssim::radix_sort_work_bytes(n), st));
   }
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, ncells, count,
-                     cell_start);
-  SSIM_LAUNCHED("k_scan");
-  if (n > 0) {
-    hipLaunchKernelGGL(k_scatter, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
-                       n, key, cell_start, cursor, order);
-    SSIM_LAUNCHED("k_scatter");
+  SSIM_TRY(ssim::scan_exclusive(ncells, count, cell_start, st));
+  if (n > 0 && packed) {
+    hipLaunchKernelGGL(k_pack, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st, n,
+                       order, x, y, z, m, origin[0], origin[1], origin[2],
+                       static_cast<float4*>(packed));
+    SSIM_LAUNCHED("k_pack");
   }
-  hipLaunchKernelGGL(k_sort_cells_pack, dim3(ssim::ceil_div(ncells, CB)),
-                     dim3(CB), 0, st, ncells, cell_start, order, x, y, z, m,
-                     origin[0], origin[1], origin[2],
-                     static_cast<float4*>(packed));
-  SSIM_LAUNCHED("k_sort_cells_pack");
   return SSIM_OK;
 }
 

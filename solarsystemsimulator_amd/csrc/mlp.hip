@@ -175,7 +175,7 @@ __device__ void forward16(const Weights& W, const float* __restrict__ bias,
 __global__ __launch_bounds__(64 * WAVES) void k_mlp_forward(
     int nb, const unsigned short* __restrict__ x,
     const unsigned short* __restrict__ wbf, const float* __restrict__ params,
-    float scale, float* __restrict__ out, double* __restrict__ a0,
+    float scale, int relative, float* __restrict__ out, double* __restrict__ a0,
     double* __restrict__ a1, double* __restrict__ a2, float* __restrict__ zs,
     unsigned short* __restrict__ hs) {
   const int lane = threadIdx.x & 63;
@@ -195,9 +195,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_forward(
         out[(size_t)body * 3 + 2] = o[2];
       }
       if (a0) {
-        a0[body] += (double)(scale * o[0]);
-        a1[body] += (double)(scale * o[1]);
-        a2[body] += (double)(scale * o[2]);
+        // relative: the output is in units of scale * |a| of the force it
+        // corrects (read before the update)
+        const double u = relative ? (double)scale * sqrt(a0[body] * a0[body] +
+                                                         a1[body] * a1[body] +
+                                                         a2[body] * a2[body])
+                                  : (double)scale;
+        a0[body] += u * (double)o[0];
+        a1[body] += u * (double)o[1];
+        a2[body] += u * (double)o[2];
       }
     }
   }
@@ -410,9 +416,10 @@ __global__ __launch_bounds__(MB) void k_to_bf16(int n, const float* __restrict__
 
 // Features of body p: the 27 node masses (/ m_unit) of the 3x3x3 stencil
 // around its nearest mesh node, then its offset from that node in cells
-// (times 2, in [-1, 1]), 2 zeros; BF16 [nb][32].
+// (times 2, in [-1, 1]), 1 for a mass (p < n_mass, its own mass is in the
+// stencil) or 0 for a tracer, and a zero; BF16 [nb][32].
 __global__ __launch_bounds__(MB) void k_mlp_features(
-    int nb, const double* __restrict__ x, const double* __restrict__ y,
+    int nb, int n_mass, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, int n, double ox, double oy, double oz,
     double h, const float* __restrict__ rho32, const double* __restrict__ rho64,
     double inv_unit, unsigned short* __restrict__ feat) {
@@ -442,7 +449,7 @@ __global__ __launch_bounds__(MB) void k_mlp_features(
   o[27] = bf16_bits(off[0]);
   o[28] = bf16_bits(off[1]);
   o[29] = bf16_bits(off[2]);
-  o[30] = 0;
+  o[30] = p < n_mass ? (unsigned short)0x3F80 : (unsigned short)0;  // 1.0
   o[31] = 0;
 }
 
@@ -467,8 +474,8 @@ int ssim_mlp_to_bf16(const float* params, unsigned short* wbf, void* stream) {
   return SSIM_OK;
 }
 
-int ssim_mlp_features(int nb, const double* x, const double* y, const double* z,
-                      int mesh_n, const double* origin, double h,
+int ssim_mlp_features(int nb, int n_mass, const double* x, const double* y,
+                      const double* z, int mesh_n, const double* origin, double h,
                       int mesh_prec, const void* rho, double mass_unit,
                       unsigned short* feat, void* stream) {
   SSIM_REQUIRE(nb >= 0 && mesh_n >= 1 && h > 0.0 && mass_unit > 0.0,
@@ -477,7 +484,7 @@ int ssim_mlp_features(int nb, const double* x, const double* y, const double* z,
   SSIM_REQUIRE(x && y && z && origin && rho && feat, "null pointer");
   const bool f64 = mesh_prec == SSIM_PREC_F64;
   hipLaunchKernelGGL(k_mlp_features, dim3(ssim::ceil_div(nb, MB)), dim3(MB), 0,
-                     ssim::as_stream(stream), nb, x, y, z, mesh_n, origin[0],
+                     ssim::as_stream(stream), nb, n_mass, x, y, z, mesh_n, origin[0],
                      origin[1], origin[2], h,
                      f64 ? nullptr : static_cast<const float*>(rho),
                      f64 ? static_cast<const double*>(rho) : nullptr,
@@ -488,15 +495,17 @@ int ssim_mlp_features(int nb, const double* x, const double* y, const double* z,
 
 int ssim_mlp_forward(int nb, const unsigned short* feat,
                      const unsigned short* wbf, const float* params,
-                     float scale, float* out, double* ax, double* ay,
-                     double* az, float* zs, unsigned short* hs, void* stream) {
+                     float scale, int relative, float* out, double* ax,
+                     double* ay, double* az, float* zs, unsigned short* hs,
+                     void* stream) {
   SSIM_REQUIRE(nb >= 0, "nb must be >= 0");
   if (nb == 0) return SSIM_OK;
   SSIM_REQUIRE(feat && wbf && params, "null pointer");
   SSIM_REQUIRE(!ax || (ay && az), "need all three acceleration arrays");
   SSIM_REQUIRE((zs == nullptr) == (hs == nullptr), "zs and hs go together");
   hipLaunchKernelGGL(k_mlp_forward, dim3(grid_waves(nb)), dim3(64 * WAVES), 0,
-                     ssim::as_stream(stream), nb, feat, wbf, params, scale, out,
+                     ssim::as_stream(stream), nb, feat, wbf, params, scale,
+                     relative, out,
                      ax, ay, az, zs, hs);
   SSIM_LAUNCHED("k_mlp_forward");
   return SSIM_OK;
@@ -610,26 +619,33 @@ __global__ __launch_bounds__(MB) void k_residual(int nb, const double* __restric
                                                  const double* __restrict__ m0,
                                                  const double* __restrict__ m1,
                                                  const double* __restrict__ m2,
-                                                 double inv_unit, float* __restrict__ out) {
+                                                 double unit, int relative,
+                                                 double clip, float* __restrict__ out) {
   const int i = blockIdx.x * MB + threadIdx.x;
   if (i >= nb) return;
-  out[(size_t)i * 3 + 0] = (float)((r0[i] - m0[i]) * inv_unit);
-  out[(size_t)i * 3 + 1] = (float)((r1[i] - m1[i]) * inv_unit);
-  out[(size_t)i * 3 + 2] = (float)((r2[i] - m2[i]) * inv_unit);
+  double u = unit;
+  if (relative) u *= sqrt(m0[i] * m0[i] + m1[i] * m1[i] + m2[i] * m2[i]);
+  const double d[3] = {r0[i] - m0[i], r1[i] - m1[i], r2[i] - m2[i]};
+  for (int c = 0; c < 3; ++c) {
+    double v = u > 0.0 ? d[c] / u : 0.0;
+    if (clip > 0.0) v = fmin(fmax(v, -clip), clip);
+    out[(size_t)i * 3 + c] = (float)v;
+  }
 }
 }  // namespace
 
 extern "C" int ssim_mlp_residual(int nb, const double* ref_x, const double* ref_y,
                                  const double* ref_z, const double* mod_x,
                                  const double* mod_y, const double* mod_z,
-                                 double a_unit, float* out, void* stream) {
-  SSIM_REQUIRE(nb >= 0 && a_unit > 0.0, "bad arguments");
+                                 double a_unit, int relative, double clip,
+                                 float* out, void* stream) {
+  SSIM_REQUIRE(nb >= 0 && a_unit > 0.0 && clip >= 0.0, "bad arguments");
   if (nb == 0) return SSIM_OK;
   SSIM_REQUIRE(ref_x && ref_y && ref_z && mod_x && mod_y && mod_z && out,
                "null pointer");
   hipLaunchKernelGGL(k_residual, dim3(ssim::ceil_div(nb, MB)), dim3(MB), 0,
                      ssim::as_stream(stream), nb, ref_x, ref_y, ref_z, mod_x,
-                     mod_y, mod_z, 1.0 / a_unit, out);
+                     mod_y, mod_z, a_unit, relative, clip, out);
   SSIM_LAUNCHED("k_residual");
   return SSIM_OK;
 }

@@ -40,6 +40,14 @@ int drift(int dim, int n, double* x, double* y, double* z, const double* vx,
 double yoshida_w1();
 double yoshida_w0();
 
+// Exclusive scan of n ints into out[0..n] (out[n] = total), sort.hip.
+int scan_exclusive(int n, const int* in, int* out, hipStream_t st);
+
+// Stable LSD radix sort of (keys, vals) in place on keys in [0, 2^bits).
+size_t radix_sort_work_bytes(int n);
+int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
+                     size_t work_bytes, hipStream_t st);
+
 }  // namespace ssim
 
 // Record a HIP failure and return SSIM_EHIP from the enclosing function.

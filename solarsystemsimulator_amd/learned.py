@@ -3,8 +3,10 @@
 A 32-64-64-64-3 SiLU network (libssim, BF16 matrix cores, FP32
 accumulation) maps local features of a body — the 27 CIC node masses of
 the 3x3x3 mesh stencil around its nearest node and its offset from that
-node — to the residual between the exact direct sum and the split mesh +
-near-field force, in units of a_unit = G m_body / h^2.  It is trained on
+node, and whether the body is a mass — to the residual between the exact
+direct sum and the split mesh + near-field force, in units of 1e-3 of the
+split force's magnitude at that body (targets clipped at 3 %; an absolute
+mode in units of G m_body / h^2 is kept).  It is trained on
 snapshots whose exact forces the FP64 all-pairs kernel computes, with the
 project's own forward/backward/Adam kernels, and at run time its output is
 added to the split force; the direct sum stays available as the fallback
@@ -59,6 +61,11 @@ class LearnedCorrection:
         self.step = 0
         self.a_unit = 1.0      # set by the caller: G m_body / h^2 of training
         self.mass_unit = 1.0   # m_body
+        # output units: relative -> out_scale * |a| of the corrected force
+        # (the residual in units of 1e-3 of the force), else a_unit
+        self.relative = True
+        self.out_scale = 1e-3
+        self.clip = 30.0       # training targets clipped to +-3 % of |a|
         self.refresh()
 
     def refresh(self):
@@ -69,26 +76,29 @@ class LearnedCorrection:
         return self.params.cpu().numpy().copy()
 
     # -- features / inference ---------------------------------------------
-    def features(self, x, mesh, origin, h, mesh_prec, out=None):
-        """BF16 feature rows [nb][32] (int16 storage) of the bodies x."""
+    def features(self, x, mesh, origin, h, mesh_prec, out=None, n_mass=0):
+        """BF16 feature rows [nb][32] (int16 storage) of the bodies x, the
+        first n_mass of which are masses."""
         torch = _torch()
         nb = x[0].numel()
         if out is None:
             out = torch.empty((nb, NIN), dtype=torch.int16, device=self.dev)
         org = _lib.double_array(origin)
-        _lib.call("ssim_mlp_features", nb, *[_lib.ptr(c) for c in x],
+        _lib.call("ssim_mlp_features", nb, int(n_mass), *[_lib.ptr(c) for c in x],
                   mesh.shape[0], ctypes.addressof(org), float(h), mesh_prec,
                   _lib.ptr(mesh), float(self.mass_unit), _lib.ptr(out),
                   D.stream(self.dev))
         return out
 
-    def predict(self, feat, out=None, add_to=None, scale=None, zs=None, hs=None):
-        """Network outputs [nb][3] (FP32) and/or scale*out added to add_to."""
+    def predict(self, feat, out=None, add_to=None, scale=None, zs=None, hs=None,
+                relative=False):
+        """Network outputs [nb][3] (FP32) and/or u*out added to add_to, with
+        u = scale, or scale * |add_to| per body when `relative`."""
         nb = feat.shape[0]
         s = self.a_unit if scale is None else scale
         a = add_to if add_to is not None else [None, None, None]
         _lib.call("ssim_mlp_forward", nb, _lib.ptr(feat), _lib.ptr(self.wbf),
-                  _lib.ptr(self.params), float(s), _lib.ptr(out),
+                  _lib.ptr(self.params), float(s), int(relative), _lib.ptr(out),
                   *[_lib.ptr(c) for c in a], _lib.ptr(zs), _lib.ptr(hs),
                   D.stream(self.dev))
         return out

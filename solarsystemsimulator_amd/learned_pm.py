@@ -9,7 +9,7 @@ of the force error (`force_errors`).
 `make_training_set` builds the learned part's data from the method's own
 run: it integrates with the split force and at every snapshot the FP64
 all-pairs kernel computes the exact forces whose residual the network
-learns (in units of G m_body / h^2).
+learns (relative to the split force, see learned.py).
 """
 from __future__ import annotations
 
@@ -29,6 +29,12 @@ DEFAULT_WEIGHTS = DATA / "learned_cube_250k.npz"
 
 # split parameters of the 250k-mass configuration (mesh 128^3, FP32 mesh)
 SPLIT = dict(mesh_n=128, rs_cells=1.0, rcut_rs=5.0, prec=P.F32)
+
+# time step of the 250k-mass cube (G = 1, total mass 1 in the unit cube):
+# 1000 steps end at t = 0.25, before the cold cube's free-fall collapse at
+# t_ff = sqrt(3 pi / 32) = 0.54, where every method's cost and error are
+# dominated by the singular core.
+DT = 2.5e-4
 
 
 def _torch():
@@ -72,20 +78,27 @@ class PMLearnedSystem:
     def a_unit(self) -> float:
         return self.G * self.mass_unit / self.split.h ** 2
 
+    def out_unit(self) -> float:
+        """The network's output unit: out_scale (x |a| per body) in the
+        relative mode, else G m_body / h^2."""
+        return self.net.out_scale if self.net.relative else self.a_unit()
+
     def features(self):
         if self.feat is None:
             self.feat = _torch().empty((self.n + self.t, 32), dtype=_torch().int16,
                                        device=self.dev)
         self.net.mass_unit = self.mass_unit
         return self.net.features(self.x, self.split.rho, self.split.origin,
-                                 self.split.h, self.split.prec, out=self.feat)
+                                 self.split.h, self.split.prec, out=self.feat,
+                                 n_mass=self.n)
 
     def force(self, learned: bool = True):
         """a = split force (+ learned correction) at the current positions."""
         self.split.accelerations(self.x, self.n, self.m, self.a)
         if learned and self.net is not None:
             feat = self.features()
-            self.net.predict(feat, add_to=self.a, scale=self.a_unit())
+            self.net.predict(feat, add_to=self.a, scale=self.out_unit(),
+                             relative=self.net.relative)
         self.acc_valid = True
         return self.a
 
@@ -156,7 +169,8 @@ def make_training_set(dev, pos, m, tpos, eps2, dt, n_snap=64, every=15,
         sysm.features()
         feats[s * nb:(s + 1) * nb].copy_(sysm.feat)
         _lib.call("ssim_mlp_residual", nb, *[_lib.ptr(c) for c in a_ref],
-                  *[_lib.ptr(c) for c in a_split], sysm.a_unit(),
+                  *[_lib.ptr(c) for c in a_split], sysm.out_unit(),
+                  int(net.relative), float(net.clip),
                   _lib.ptr(tgts[s * nb:(s + 1) * nb]), D.stream(dev))
         if log:
             log(f"snapshot {s + 1}/{n_snap}")
@@ -166,9 +180,18 @@ def make_training_set(dev, pos, m, tpos, eps2, dt, n_snap=64, every=15,
 def save_weights(net: LearnedCorrection, path=DEFAULT_WEIGHTS, **meta):
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
-    np.savez(path, params=net.host_params(), **{k: np.asarray(v) for k, v in meta.items()})
+    np.savez(path, params=net.host_params(), relative=np.int32(net.relative),
+             out_scale=np.float64(net.out_scale), clip=np.float64(net.clip),
+             **{k: np.asarray(v) for k, v in meta.items()})
 
 
 def load_weights(dev, path=DEFAULT_WEIGHTS) -> LearnedCorrection:
     with np.load(path, allow_pickle=False) as f:
-        return LearnedCorrection(dev, f["params"])
+        net = LearnedCorrection(dev, f["params"])
+        if "relative" in f.files:
+            net.relative = bool(f["relative"])
+            net.out_scale = float(f["out_scale"])
+            net.clip = float(f["clip"])
+        else:                       # absolute units (G m_body / h^2)
+            net.relative = False
+    return net

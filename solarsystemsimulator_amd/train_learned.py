@@ -3,7 +3,7 @@
     python -m solarsystemsimulator_amd.train_learned [--small] [--out PATH]
 
 1. the 250,000-mass / 500,000-tracer perturbed cube (seeds 5 / 55),
-   eps = 1e-3, dt = 1e-3, is integrated with the split PM force; every
+   eps = 1e-3, dt = 2.5e-4, is integrated with the split PM force; every
    `--every` steps the FP64 all-pairs kernel computes the exact forces of
    all bodies (64 snapshots);
 2. the network is trained on the residual exact - split (units G m / h^2)
@@ -24,7 +24,7 @@ import numpy as np
 from . import device as D
 from . import scenes
 from .learned import LearnedCorrection
-from .learned_pm import (DEFAULT_WEIGHTS, PMLearnedSystem, direct_system,
+from .learned_pm import (DEFAULT_WEIGHTS, DT, PMLearnedSystem, direct_system,
                          exact_accelerations, force_errors, make_training_set,
                          save_weights)
 
@@ -33,7 +33,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def evaluate(dev, net, pos, m, tpos, eps2, dt, times=(0, 100, 300, 600)):
+def evaluate(dev, net, pos, m, tpos, eps2, dt, times=(0, 250, 500, 1000)):
     """Median / p90 relative force errors of the split force with and
     without the correction at several steps of a run of the method."""
     n, t = len(m), len(tpos[0])
@@ -65,6 +65,7 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--dt", type=float, default=DT)
     ap.add_argument("--out", default=str(DEFAULT_WEIGHTS))
     args = ap.parse_args(argv)
     dev = D.require_gpu(0)
@@ -73,7 +74,7 @@ def main(argv=None):
     else:
         shapes = ((100, 50, 50), (100, 100, 50))
     pos, m, tpos = scenes.learned_scene(*shapes, seed=5, tracer_seed=55)
-    eps2, dt = 1e-6, 1e-3
+    eps2, dt = 1e-6, args.dt
     t0 = time.time()
     net = LearnedCorrection(dev, seed=1)
     feat, tgt = make_training_set(dev, pos, m, tpos, eps2, dt, args.snapshots,

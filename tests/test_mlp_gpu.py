@@ -59,6 +59,13 @@ def test_forward_adds_scaled_output(dev):
         np.testing.assert_allclose(a[c].cpu().numpy(),
                                    1.5 + 2.0 * out[:, c].cpu().numpy().astype(np.float64),
                                    rtol=1e-6)
+    # relative mode: u = scale * |a| of the acceleration being corrected
+    a = [torch.full((100,), v, dtype=torch.float64, device=dev) for v in (3.0, 0.0, 4.0)]
+    net.predict(feat, add_to=a, scale=1e-3, relative=True)
+    o = out.cpu().numpy().astype(np.float64)
+    for c, v in enumerate((3.0, 0.0, 4.0)):
+        np.testing.assert_allclose(a[c].cpu().numpy(), v + 5e-3 * o[:, c],
+                                   rtol=1e-6, atol=1e-12)
 
 
 def test_backward_matches_oracle(dev):
@@ -101,9 +108,9 @@ def test_features_match_oracle(dev):
     net = LearnedCorrection(dev)
     net.mass_unit = 0.75
     feat = net.features([torch.from_numpy(c).to(dev) for c in pos],
-                        torch.from_numpy(rho).to(dev), origin, h, 0)
+                        torch.from_numpy(rho).to(dev), origin, h, 0, n_mass=700)
     got = (feat.cpu().numpy().view(np.uint16).astype(np.uint32) << 16).view(np.float32)
-    ref = M.bf16(M.features(pos, rho, origin, h, 0.75).astype(np.float32))
+    ref = M.bf16(M.features(pos, rho, origin, h, 0.75, 700).astype(np.float32))
     # v / unit (oracle) vs v * (1/unit) (GPU) may differ by one FP64 ulp,
     # which can move a BF16 rounding by one step
     assert np.mean(got == ref) > 0.999

@@ -60,3 +60,4 @@ def test_features_pick_the_nearest_node_stencil():
     assert f[0, 0] == rho[2, 3, 4] / 2.0
     assert f[0, 26] == rho[4, 5, 6] / 2.0
     np.testing.assert_allclose(f[0, 27:30], [0.6, 0.6, -0.4], atol=1e-12)
+    assert f[0, 30] == 0.0 and M.features(pos, rho, origin, h, 2.0, 1)[0, 30] == 1.0
