@@ -3,7 +3,7 @@ direct FP64 all-pairs vs PM + learned correction.
 
 Workload (G = 1, unit cube, cold start): 250,000 equal masses on a perturbed
 100x50x50 lattice (seed 5) and 500,000 tracers on a 100x100x50 lattice
-(seed 55), Plummer softening eps = 1e-3, dt = 1e-3, leapfrog KDK with FP64
+(seed 55), Plummer softening eps = 1e-3, dt = 2.5e-4, leapfrog KDK with FP64
 state.  A step is one KDK step of all 750,000 bodies: half-kick + drift,
 one force evaluation, half-kick (+ the f32 snapshot for the direct run),
 all on the GPU with inputs resident in HBM.
@@ -105,7 +105,7 @@ def main():
     from solarsystemsimulator_amd import _lib, scenes
     from solarsystemsimulator_amd import device as D
     from solarsystemsimulator_amd.learned_pm import (
-        DEFAULT_WEIGHTS, PMLearnedSystem, force_errors, load_weights)
+        DEFAULT_WEIGHTS, DT, PMLearnedSystem, force_errors, load_weights)
     from solarsystemsimulator_amd.selfgrav import SelfGravSystem
 
     dev = D.require_gpu(0)
@@ -116,7 +116,7 @@ def main():
         pos, m, tpos = scenes.learned_scene()
     n, t = len(m), len(tpos[0])
     nb = n + t
-    eps2, dt = 1e-6, 1e-3
+    eps2, dt = 1e-6, DT
     zeros_n, zeros_t = [np.zeros(n)] * 3, [np.zeros(t)] * 3
 
     # ---- direct FP64 ------------------------------------------------------
@@ -213,7 +213,7 @@ def main():
         "precision": "f64 state; f32 mesh and near field; bf16 MLP with f32 "
                      "accumulation; direct run all f64",
         "config": {"workload": f"{n} masses + {t} tracers, perturbed unit cube, "
-                               "eps=1e-3, dt=1e-3, leapfrog KDK, PM 128^3 "
+                               f"eps=1e-3, dt={dt:g}, leapfrog KDK, PM 128^3 "
                                "isolated + near field + learned correction",
                    "masses": n, "tracers": t, "method": "pm_learned"},
         "direct_f64_body_steps_per_s": direct_rate,

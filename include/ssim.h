@@ -215,12 +215,15 @@ int ssim_minmax(int n, int narrays, const double* a0, const double* a1,
 /* Real-space force Green's function component `comp` on the zero-padded
  * (2n)^dim mesh: -G d_comp / (|d|^2 + eps2)^(3/2) with wrapped offsets
  * (build_force_kernels, pm_kernel.py:34-60, generalised to 3-D).  rs > 0
- * keeps only the long-range part, times 1 - g(r) with the Gaussian split
- * g(r) = erfc(r/2rs) + r/(rs sqrt(pi)) exp(-r^2/4rs^2); rs = 0 is the
- * reference's full kernel. */
+ * keeps only the long-range part: times 1 - g(r) with the Gaussian split
+ * g(r) = erfc(r/2rs) + r/(rs sqrt(pi)) exp(-r^2/4rs^2) for r < rcut (all
+ * r if rcut <= 0) and the full kernel beyond rcut, where the near-field sum
+ * stops; it is zero where an offset is -n (never used by two nodes of the
+ * n-mesh), which makes it exactly odd.  rs = 0 is the reference's full
+ * kernel (rcut ignored). */
 int ssim_pm_force_kernel(int dim, int n, int comp, double h, double eps2,
-                         double G, double rs, int mesh_prec, void* out,
-                         void* stream);
+                         double G, double rs, double rcut, int mesh_prec,
+                         void* out, void* stream);
 /* Divide a half spectrum on the padded mesh (padded_n^dim) by the squared
  * CIC window prod_c sinc^4(pi f_c / padded_n) (no reference counterpart:
  * used by the split mesh + near-field force). */
@@ -230,6 +233,23 @@ int ssim_pm_deconvolve(int dim, int padded_n, int prec, void* spectrum,
  * products of pm_force_field, pm_kernel.py:144-146). */
 int ssim_spectrum_mul(long long ncomplex, int prec, const void* a,
                       const void* b, void* out, void* stream);
+/* out[i] = Im a[i] over ncomplex values (the split kernel's spectrum is
+ * imaginary; ssim_pm_conv_isolated3 keeps only that part). */
+int ssim_spectrum_imag(long long ncomplex, int prec, const void* a, void* out,
+                       void* stream);
+/* Isolated 3-D convolution of the n^3 mass mesh rho with three kernels
+ * given by their imaginary spectra kim_* on the padded (2n)^3 mesh
+ * (rfftn half-spectrum layout, real values): field_c = corner n^3 of
+ * irfftn(rfftn(pad(rho)) * i kim_c), the result of ssim_pm_pad +
+ * ssim_fft_r2c + ssim_spectrum_mul + ssim_fft_c2r + crop, computed without
+ * the padding or the discarded part (pruned passes).  spec and tmp hold
+ * (2n)^2 (n+1) complex values each; fft_work is prepared for dims
+ * (2n, 2n, 2n).  No reference counterpart (split force). */
+int ssim_pm_conv_isolated3(int n, int prec, const void* rho, const void* kim_x,
+                           const void* kim_y, const void* kim_z, void* fx,
+                           void* fy, void* fz, void* spec, void* tmp,
+                           const void* fft_work, size_t fft_work_bytes,
+                           void* stream);
 /* Periodic Poisson solve in place on an rfftn half spectrum of MASS per
  * node: phi_k = -4 pi G (m_k / h^dim) / k^2, phi_0 = 0. */
 int ssim_pm_poisson(int dim, const int* grid_n, double h, double G, int prec,
@@ -261,14 +281,19 @@ int ssim_cells_build(int n, const double* x, const double* y, const double* z,
                      const int* nc, int* cell_start, int* order, void* packed,
                      void* work, size_t work_bytes, void* stream);
 /* ADD the near-field force G m d g(r) / (r^2 + eps2)^(3/2), r < rcut, of
- * the binned sources (packed, from ssim_cells_build with the same grid and
- * rcut <= cell) to nt targets' accelerations (3-D).  New: the reference has
- * no split force. */
+ * the binned sources (cell_start + packed, from ssim_cells_build) to nt
+ * targets' accelerations (3-D).  The targets are binned on the same grid
+ * (tcell_start, torder from ssim_cells_build on the targets, packed = NULL)
+ * and rcut <= cell.  Each target's sum runs over the sources in their sorted
+ * order, so the result is deterministic.  New: the reference has no split
+ * force. */
+size_t ssim_pp_work_bytes(int nt, int ncells);
 int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,
                   const double* origin, double cell, const int* nc,
-                  const int* cell_start, const void* packed, double G,
+                  const int* cell_start, const void* packed,
+                  const int* tcell_start, const int* torder, double G,
                   double eps2, double rs, double rcut, double* ax, double* ay,
-                  double* az, void* stream);
+                  double* az, void* work, size_t work_bytes, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Uniform-grid collision search (bit-exact ordered pair list)         */

@@ -68,9 +68,13 @@ SIGNATURES = {
     "ssim_minmax_work_bytes": (c_size_t, [c_int]),
     "ssim_minmax": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, c_size_t, _P]),
     "ssim_pm_force_kernel": (c_int, [c_int, c_int, c_int, c_double, c_double,
-                                     c_double, c_double, c_int, _P, _P]),
+                                     c_double, c_double, c_double, c_int, _P,
+                                     _P]),
     "ssim_pm_deconvolve": (c_int, [c_int, c_int, c_int, _P, _P]),
     "ssim_spectrum_mul": (c_int, [c_longlong, c_int, _P, _P, _P, _P]),
+    "ssim_spectrum_imag": (c_int, [c_longlong, c_int, _P, _P, _P]),
+    "ssim_pm_conv_isolated3": (c_int, [c_int, c_int] + [_P] * 9 + [_P, c_size_t,
+                                                                  _P]),
     "ssim_pm_poisson": (c_int, [c_int, _P, c_double, c_double, c_int, _P, _P]),
     "ssim_pm_gradient": (c_int, [c_int, _P, c_double, c_int, _P, _P, _P, _P,
                                  _P]),
@@ -78,9 +82,10 @@ SIGNATURES = {
     "ssim_cells_work_bytes": (c_size_t, [c_int, c_int]),
     "ssim_cells_build": (c_int, [c_int, _P, _P, _P, _P, _P, c_double, _P, _P,
                                  _P, _P, _P, c_size_t, _P]),
-    "ssim_pp_short": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P,
-                              c_double, c_double, c_double, c_double, _P, _P,
-                              _P, _P]),
+    "ssim_pp_work_bytes": (c_size_t, [c_int, c_int]),
+    "ssim_pp_short": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P,
+                              _P, c_double, c_double, c_double, c_double, _P,
+                              _P, _P, _P, c_size_t, _P]),
     "ssim_collide_work_bytes": (c_size_t, [c_int, c_longlong]),
     "ssim_collide_hits": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, c_float,
                                   c_float, c_float, c_double, c_double,

@@ -60,7 +60,7 @@ extern "C" {
 size_t ssim_cells_work_bytes(int n, int ncells) {
   return ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int)) +
          ssim::align256((size_t)(ncells > 0 ? ncells : 1) * sizeof(int)) +
-         ssim::radix_sort_work_bytes(n);
+         ssim::scan_work_bytes(ncells) + ssim::radix_sort_work_bytes(n);
 }
 
 int ssim_cells_build(int n, const double* x, const double* y, const double* z,
@@ -82,6 +82,8 @@ int ssim_cells_build(int n, const double* x, const double* y, const double* z,
   w += ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int));
   int* count = reinterpret_cast<int*>(w);
   w += ssim::align256((size_t)ncells * sizeof(int));
+  void* scan_work = w;
+  w += ssim::scan_work_bytes(ncells);
   // zeroed by a kernel (not hipMemsetAsync) so the sequence can be captured
   // in a HIP graph
   hipLaunchKernelGGL(k_zero_i32, dim3(ssim::ceil_div(ncells, CB)), dim3(CB), 0,
@@ -97,7 +99,7 @@ int ssim_cells_build(int n, const double* x, const double* y, const double* z,
     SSIM_TRY(ssim::radix_sort_pairs(n, bits, key, order, w,
                                     ssim::radix_sort_work_bytes(n), st));
   }
-  SSIM_TRY(ssim::scan_exclusive(ncells, count, cell_start, st));
+  SSIM_TRY(ssim::scan_exclusive(ncells, count, cell_start, scan_work, st));
   if (n > 0 && packed) {
     hipLaunchKernelGGL(k_pack, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st, n,
                        order, x, y, z, m, origin[0], origin[1], origin[2],

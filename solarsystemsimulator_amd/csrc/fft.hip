@@ -63,19 +63,32 @@ int ilog2(int v) {
   return l;
 }
 
-// Tile geometry of one pass (complex-element units).
+// Tile geometry of one pass (complex-element units).  Only the first
+// in_count points of a line are read (the rest are zeros) and only the first
+// out_count are written, which prunes the zero padding and the cropped part
+// of an isolated convolution.  Contiguous rows r map to offsets
+// (r / R1) * s_hi + (r % R1) * s_outer, so a pass can read or write an
+// n x n block of rows inside a larger array.
 struct Pass {
   int L, log2L;
   int n_outer, n_inner;       // lines = n_outer * n_inner
   long long s_outer_in, s_outer_out, s_elem;
   int lines_per_tile;         // along the inner index (strided) or outer (contig)
   int contig;                 // 1: lines are contiguous rows (s_elem == 1)
+  int in_count, out_count;    // points read / written per line
+  long long R1, s_in_hi, s_out_hi;
 };
+
+__device__ __forceinline__ long long row_off(long long r, long long R1,
+                                             long long s_hi, long long s_lo) {
+  return (r / R1) * s_hi + (r % R1) * s_lo;
+}
 
 template <typename F, int MODE>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     const void* __restrict__ vin, void* __restrict__ vout, Pass P,
-    const cpx<F>* __restrict__ tw, int inverse, F scale) {
+    const cpx<F>* __restrict__ tw, int inverse, F scale,
+    const F* __restrict__ mul) {
   extern __shared__ unsigned char smem_raw[];
   cpx<F>* s = reinterpret_cast<cpx<F>*>(smem_raw);
   const int L = P.L;
@@ -102,7 +115,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
       const int l = t / L, e = t % L;
       F v = 0;
-      if (l < nvalid) v = in[(o0 + l) * P.s_outer_in + e];
+      if (l < nvalid && e < P.in_count)
+        v = in[row_off(o0 + l, P.R1, P.s_in_hi, P.s_outer_in) + e];
       s[t] = {v, F(0)};
     }
   } else if (MODE == MODE_C2R) {
@@ -113,7 +127,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
       const int l = t / L, e = t % L;
       cpx<F> v = {F(0), F(0)};
       if (l < nvalid) {
-        const cpx<F>* row = in + (o0 + l) * P.s_outer_in;
+        const cpx<F>* row = in + row_off(o0 + l, P.R1, P.s_in_hi, P.s_outer_in);
         if (e < H) {
           v = row[e];
         } else {
@@ -128,14 +142,23 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     if (P.contig) {
       for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
         const int l = t / L, e = t % L;
-        s[t] = (l < nvalid) ? in[(o0 + l) * P.s_outer_in + e] : cpx<F>{0, 0};
+        s[t] = (l < nvalid && e < P.in_count)
+                   ? in[row_off(o0 + l, P.R1, P.s_in_hi, P.s_outer_in) + e]
+                   : cpx<F>{0, 0};
       }
     } else {
       // consecutive threads walk the inner index: contiguous in HBM
       for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
         const int l = t % nl, e = t / nl;
         cpx<F> v = {0, 0};
-        if (l < nvalid) v = in[o0 * P.s_outer_in + (b0 + l) + e * P.s_elem];
+        if (l < nvalid && e < P.in_count) {
+          const long long idx = o0 * P.s_outer_in + (b0 + l) + e * P.s_elem;
+          v = in[idx];
+          if (mul) {  // times i * mul (an imaginary spectrum)
+            const F m = mul[idx];
+            v = {-v.im * m, v.re * m};
+          }
+        }
         s[l * L + e] = v;
       }
     }
@@ -226,29 +249,32 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
       const int l = t / H, e = t % H;
       if (l < nvalid) {
         cpx<F> v = s[l * L + e];
-        out[(o0 + l) * P.s_outer_out + e] = {v.re * scale, v.im * scale};
+        out[row_off(o0 + l, P.R1, P.s_out_hi, P.s_outer_out) + e] = {v.re * scale,
+                                                                     v.im * scale};
       }
     }
   } else if (MODE == MODE_C2R) {
     F* out = static_cast<F*>(vout);
     for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
       const int l = t / L, e = t % L;
-      if (l < nvalid) out[(o0 + l) * P.s_outer_out + e] = s[t].re * scale;
+      if (l < nvalid && e < P.out_count)
+        out[row_off(o0 + l, P.R1, P.s_out_hi, P.s_outer_out) + e] = s[t].re * scale;
     }
   } else {
     cpx<F>* out = static_cast<cpx<F>*>(vout);
     if (P.contig) {
       for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
         const int l = t / L, e = t % L;
-        if (l < nvalid) {
+        if (l < nvalid && e < P.out_count) {
           const cpx<F> v = s[t];
-          out[(o0 + l) * P.s_outer_out + e] = {v.re * scale, v.im * scale};
+          out[row_off(o0 + l, P.R1, P.s_out_hi, P.s_outer_out) + e] = {v.re * scale,
+                                                                       v.im * scale};
         }
       }
     } else {
       for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
         const int l = t % nl, e = t / nl;
-        if (l < nvalid) {
+        if (l < nvalid && e < P.out_count) {
           const cpx<F> v = s[l * L + e];
           out[o0 * P.s_outer_out + (b0 + l) + e * P.s_elem] = {v.re * scale,
                                                                v.im * scale};
@@ -288,7 +314,7 @@ size_t tw_offset(int ndim, const int* dims, int axis, int prec) {
 
 template <typename F, int MODE>
 int launch_pass(const void* in, void* out, Pass P, const void* tw, int inverse,
-                F scale, hipStream_t st) {
+                F scale, hipStream_t st, const F* mul = nullptr) {
   int nwg;
   if (P.contig)
     nwg = ssim::ceil_div(P.n_outer, P.lines_per_tile);
@@ -297,7 +323,7 @@ int launch_pass(const void* in, void* out, Pass P, const void* tw, int inverse,
   const size_t lds = (size_t)P.lines_per_tile * P.L * sizeof(cpx<F>);
   hipLaunchKernelGGL((k_fft_pass<F, MODE>), dim3(nwg), dim3(FFT_THREADS), lds,
                      st, in, out, P, static_cast<const cpx<F>*>(tw), inverse,
-                     scale);
+                     scale, mul);
   SSIM_LAUNCHED("k_fft_pass");
   return SSIM_OK;
 }
@@ -315,6 +341,9 @@ Pass strided_pass(int ndim, const int* cdims, int axis) {
   P.s_elem = inner;
   P.s_outer_in = P.s_outer_out = inner * P.L;
   P.contig = 0;
+  P.in_count = P.out_count = P.L;
+  P.R1 = 1;
+  P.s_in_hi = P.s_out_hi = 0;
   int b = FFT_TILE / P.L;
   if (b > FFT_MAXB) b = FFT_MAXB;
   if (b > P.n_inner) {
@@ -335,6 +364,9 @@ Pass row_pass(long long rows, int L, long long s_in, long long s_out) {
   P.s_outer_in = s_in;
   P.s_outer_out = s_out;
   P.contig = 1;
+  P.in_count = P.out_count = L;
+  P.R1 = rows > 0 ? rows : 1;
+  P.s_in_hi = P.s_out_hi = 0;
   int b = FFT_TILE / L;
   if (b < 1) b = 1;
   P.lines_per_tile = b;
@@ -380,6 +412,60 @@ int fft_c2r(int ndim, const int* dims, cpx<F>* in, F* out,
   SSIM_TRY((launch_pass<F, MODE_C2R>(
       in, out, row_pass(rows, L, cdims[ndim - 1], L),
       work + tw_offset(ndim, dims, ndim - 1, prec), 1, F(1.0 / total), st)));
+  return SSIM_OK;
+}
+
+// Isolated convolution of an n^3 mesh with three imaginary kernel spectra
+// on the zero-padded P = 2n mesh, keeping only the n^3 corner:
+//   forward: R2C rows (x, y < n) read straight from rho (n points, zeros
+//            beyond), y lines of the planes x < n (n points read), x lines
+//            (n points read);
+//   per component: x lines read spec * i kim_c and write x < n into tmp,
+//            y lines of the planes x < n write y < n, C2R rows (x, y < n)
+//            write z < n into the field, scaled by 1/P^3.
+// The padding never exists in memory and nothing outside the corner is
+// computed in the inverse: about 2.6x fewer HBM bytes than pad + rfftn +
+// 3 x (product + irfftn + crop).
+template <typename F>
+int conv_isolated3(int n, const F* rho, const F* const* kim, F* const* fields,
+                   cpx<F>* spec, cpx<F>* tmp, const unsigned char* tw, int prec,
+                   hipStream_t st) {
+  const int P = 2 * n, H = n + 1;
+  const int dims[3] = {P, P, P};
+  const int cd[3] = {P, P, H};
+  const unsigned char* twx = tw + tw_offset(3, dims, 0, prec);
+  const unsigned char* twy = tw + tw_offset(3, dims, 1, prec);
+  const unsigned char* twz = tw + tw_offset(3, dims, 2, prec);
+  const long long rows = (long long)n * n;
+  Pass fz = row_pass(rows, P, n, H);
+  fz.R1 = n;
+  fz.s_in_hi = rows;
+  fz.s_out_hi = (long long)P * H;
+  fz.in_count = n;
+  SSIM_TRY((launch_pass<F, MODE_R2C>(rho, spec, fz, twz, 0, F(1), st)));
+  Pass fy = strided_pass(3, cd, 1);
+  fy.n_outer = n;
+  fy.in_count = n;
+  SSIM_TRY((launch_pass<F, MODE_C2C>(spec, spec, fy, twy, 0, F(1), st)));
+  Pass fx = strided_pass(3, cd, 0);
+  fx.in_count = n;
+  SSIM_TRY((launch_pass<F, MODE_C2C>(spec, spec, fx, twx, 0, F(1), st)));
+  const F scale = F(1.0 / ((double)P * P * P));
+  for (int c = 0; c < 3; ++c) {
+    Pass ix = strided_pass(3, cd, 0);
+    ix.out_count = n;
+    SSIM_TRY((launch_pass<F, MODE_C2C>(spec, tmp, ix, twx, 1, F(1), st, kim[c])));
+    Pass iy = strided_pass(3, cd, 1);
+    iy.n_outer = n;
+    iy.out_count = n;
+    SSIM_TRY((launch_pass<F, MODE_C2C>(tmp, tmp, iy, twy, 1, F(1), st)));
+    Pass iz = row_pass(rows, P, H, n);
+    iz.R1 = n;
+    iz.s_in_hi = (long long)P * H;
+    iz.s_out_hi = rows;
+    iz.out_count = n;
+    SSIM_TRY((launch_pass<F, MODE_C2R>(tmp, fields[c], iz, twz, 1, scale, st)));
+  }
   return SSIM_OK;
 }
 
@@ -451,6 +537,43 @@ extern "C" {
 
 size_t ssim_fft_work_bytes(int ndim, const int* dims, int prec) {
   return ssim::fft_work_bytes(ndim, dims, prec);
+}
+
+int ssim_pm_conv_isolated3(int n, int prec, const void* rho, const void* kim_x,
+                           const void* kim_y, const void* kim_z, void* fx,
+                           void* fy, void* fz, void* spec, void* tmp,
+                           const void* fft_work, size_t fft_work_bytes,
+                           void* stream) {
+  SSIM_REQUIRE(n >= 1 && 2 * n <= FFT_MAX_L && (n & (n - 1)) == 0,
+               "n must be a power of two with 2n <= %d", FFT_MAX_L);
+  SSIM_REQUIRE(prec == SSIM_PREC_F64 || prec == SSIM_PREC_F32C,
+               "unknown precision %d", prec);
+  SSIM_REQUIRE(rho && kim_x && kim_y && kim_z && fx && fy && fz && spec && tmp &&
+                   fft_work,
+               "null pointer");
+  const int dims[3] = {2 * n, 2 * n, 2 * n};
+  SSIM_REQUIRE(fft_work_bytes >= ssim::fft_work_bytes(3, dims, prec),
+               "FFT workspace too small (prepare it for the padded 2n^3 mesh)");
+  const unsigned char* tw = static_cast<const unsigned char*>(fft_work);
+  hipStream_t st = ssim::as_stream(stream);
+  if (prec == SSIM_PREC_F64) {
+    const double* k[3] = {static_cast<const double*>(kim_x),
+                          static_cast<const double*>(kim_y),
+                          static_cast<const double*>(kim_z)};
+    double* f[3] = {static_cast<double*>(fx), static_cast<double*>(fy),
+                    static_cast<double*>(fz)};
+    return conv_isolated3<double>(n, static_cast<const double*>(rho), k, f,
+                                  static_cast<cpx<double>*>(spec),
+                                  static_cast<cpx<double>*>(tmp), tw, prec, st);
+  }
+  const float* k[3] = {static_cast<const float*>(kim_x),
+                       static_cast<const float*>(kim_y),
+                       static_cast<const float*>(kim_z)};
+  float* f[3] = {static_cast<float*>(fx), static_cast<float*>(fy),
+                 static_cast<float*>(fz)};
+  return conv_isolated3<float>(n, static_cast<const float*>(rho), k, f,
+                               static_cast<cpx<float>*>(spec),
+                               static_cast<cpx<float>*>(tmp), tw, prec, st);
 }
 
 int ssim_fft_prepare(int ndim, const int* dims, int prec, void* work,

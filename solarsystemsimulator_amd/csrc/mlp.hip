@@ -433,10 +433,14 @@ __global__ __launch_bounds__(MB) void k_mlp_features(
     c[a] = min(max((int)r, 0), n - 1);
     off[a] = (float)(2.0 * (f[a] - r));
   }
-  unsigned short* o = feat + (size_t)p * NIN;
+  // 32 BF16 values packed into 16 words, written as four 16-byte stores
+  unsigned int wd[16];
   int q = 0;
+#pragma unroll
   for (int da = -1; da <= 1; ++da)
+#pragma unroll
     for (int db = -1; db <= 1; ++db)
+#pragma unroll
       for (int dc = -1; dc <= 1; ++dc, ++q) {
         const int i = c[0] + da, j = c[1] + db, k = c[2] + dc;
         double v = 0.0;
@@ -444,13 +448,15 @@ __global__ __launch_bounds__(MB) void k_mlp_features(
           const size_t idx = ((size_t)i * n + j) * n + k;
           v = rho64 ? rho64[idx] : (double)rho32[idx];
         }
-        o[q] = bf16_bits((float)(v * inv_unit));
+        const unsigned int b = bf16_bits((float)(v * inv_unit));
+        if (q & 1) wd[q >> 1] |= b << 16; else wd[q >> 1] = b;
       }
-  o[27] = bf16_bits(off[0]);
-  o[28] = bf16_bits(off[1]);
-  o[29] = bf16_bits(off[2]);
-  o[30] = p < n_mass ? (unsigned short)0x3F80 : (unsigned short)0;  // 1.0
-  o[31] = 0;
+  wd[13] |= (unsigned int)bf16_bits(off[0]) << 16;                 // 27
+  wd[14] = bf16_bits(off[1]) | ((unsigned int)bf16_bits(off[2]) << 16);
+  wd[15] = p < n_mass ? 0x3F80u : 0u;                              // 30, 31
+  uint4* o = reinterpret_cast<uint4*>(feat + (size_t)p * NIN);
+  for (int v = 0; v < 4; ++v)
+    o[v] = make_uint4(wd[4 * v], wd[4 * v + 1], wd[4 * v + 2], wd[4 * v + 3]);
 }
 
 int grid_waves(int nb) {

@@ -165,30 +165,46 @@ __global__ __launch_bounds__(PM_BLOCK) void k_minmax_partial(
     }
 }
 
-__global__ void k_minmax_final(int nblk, int na, const double* __restrict__ part,
-                               double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+// One workgroup folds the per-block extrema (min and max are exact, so the
+// order of the fold does not matter).
+__global__ __launch_bounds__(PM_BLOCK) void k_minmax_final(
+    int nblk, int na, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double smin[PM_BLOCK], smax[PM_BLOCK];
   for (int c = 0; c < na; ++c) {
     double lo = INFINITY, hi = -INFINITY;
-    for (int b = 0; b < nblk; ++b) {
+    for (int b = threadIdx.x; b < nblk; b += PM_BLOCK) {
       lo = fmin(lo, part[(size_t)b * 6 + 2 * c]);
       hi = fmax(hi, part[(size_t)b * 6 + 2 * c + 1]);
     }
-    out[2 * c] = lo;
-    out[2 * c + 1] = hi;
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int w = PM_BLOCK / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) {
+        smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + w]);
+        smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      out[2 * c] = smin[0];
+      out[2 * c + 1] = smax[0];
+    }
+    __syncthreads();
   }
 }
 
 // Real-space force kernel on a zero-padded (2N)^D mesh: K_c(d) =
 // -G d_c / (|d|^2 + eps2)^(3/2), d = off * h with off = idx < N ? idx :
 // idx - 2N (build_force_kernels, pm_kernel.py:42-59).  Component c of D.
-// With rs > 0 only the long-range part is kept: K_c * (1 - g(|d|)),
-// g(r) = erfc(r/2rs) + r/(rs sqrt(pi)) exp(-r^2/4rs^2) (the complement is
-// summed directly by the near-field kernel).
+// With rs > 0 only the long-range part is kept: K_c * (1 - g(|d|)) for
+// |d| < rcut (all |d| if rcut <= 0), the full K_c beyond,
+// g(r) = erfc(r/2rs) + r/(rs sqrt(pi)) exp(-r^2/4rs^2) (the complement
+// K_c g inside rcut is summed directly by the near-field kernel).
 template <typename M>
 __global__ __launch_bounds__(PM_BLOCK) void k_force_kernel(
     int D, int N, int comp, double h, double eps2, double G, double rs,
-    M* __restrict__ out) {
+    double rcut, M* __restrict__ out) {
   const long long P = 2LL * N;
   const long long total = (D == 2) ? P * P : P * P * P;
   const long long t = (long long)blockIdx.x * PM_BLOCK + threadIdx.x;
@@ -218,7 +234,13 @@ __global__ __launch_bounds__(PM_BLOCK) void k_force_kernel(
     const double r = sqrt(rr);
     const double x = r / (2.0 * rs);
     const double g = erfc(x) + r / (rs * 1.7724538509055160273) * exp(-x * x);
-    k = k * (1.0 - g);
+    // beyond rcut (the near-field sum stops there) the mesh carries the
+    // whole force, so no part of any pair is dropped
+    if (rcut <= 0.0 || r < rcut) k = k * (1.0 - g);
+    // offset -N pairs no two nodes of the N-mesh; zero there the kernel is
+    // exactly odd, so its spectrum is imaginary
+    for (int c = 0; c < D; ++c)
+      if (idx[c] == N) k = 0.0;
   }
   out[t] = (M)k;
 }
@@ -266,6 +288,13 @@ __global__ __launch_bounds__(PM_BLOCK) void k_cmul(long long nc,
   const M br = b[2 * t], bi = b[2 * t + 1];
   out[2 * t] = ar * br - ai * bi;
   out[2 * t + 1] = ar * bi + ai * br;
+}
+
+template <typename M>
+__global__ __launch_bounds__(PM_BLOCK) void k_imag(long long nc, const M* __restrict__ a,
+                                                   M* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (t < nc) out[t] = a[2 * t + 1];
 }
 
 // Periodic Poisson in Fourier space on the rfftn half spectrum:
@@ -449,15 +478,15 @@ int ssim_minmax(int n, int narrays, const double* a0, const double* a1,
   hipLaunchKernelGGL(k_minmax_partial, dim3(nblk), dim3(PM_BLOCK), 0, st, n,
                      narrays, a0, a1, a2, part);
   SSIM_LAUNCHED("k_minmax_partial");
-  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(64), 0, st, nblk, narrays,
+  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(PM_BLOCK), 0, st, nblk, narrays,
                      part, out);
   SSIM_LAUNCHED("k_minmax_final");
   return SSIM_OK;
 }
 
 int ssim_pm_force_kernel(int dim, int n, int comp, double h, double eps2,
-                         double G, double rs, int mesh_prec, void* out,
-                         void* stream) {
+                         double G, double rs, double rcut, int mesh_prec,
+                         void* out, void* stream) {
   SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
   SSIM_REQUIRE(n >= 1 && comp >= 0 && comp < dim, "bad n / component");
   SSIM_REQUIRE(h > 0.0 && eps2 >= 0.0 && rs >= 0.0,
@@ -469,10 +498,10 @@ int ssim_pm_force_kernel(int dim, int n, int comp, double h, double eps2,
   const dim3 grid((unsigned)ssim::ceil_div(total, PM_BLOCK));
   if (mesh_prec == SSIM_PREC_F64)
     hipLaunchKernelGGL(k_force_kernel<double>, grid, dim3(PM_BLOCK), 0, st, dim,
-                       n, comp, h, eps2, G, rs, static_cast<double*>(out));
+                       n, comp, h, eps2, G, rs, rcut, static_cast<double*>(out));
   else
     hipLaunchKernelGGL(k_force_kernel<float>, grid, dim3(PM_BLOCK), 0, st, dim,
-                       n, comp, h, eps2, G, rs, static_cast<float*>(out));
+                       n, comp, h, eps2, G, rs, rcut, static_cast<float*>(out));
   SSIM_LAUNCHED("k_force_kernel");
   return SSIM_OK;
 }
@@ -511,6 +540,23 @@ int ssim_spectrum_mul(long long ncomplex, int prec, const void* a,
                        static_cast<const float*>(a), static_cast<const float*>(b),
                        static_cast<float*>(out));
   SSIM_LAUNCHED("k_cmul");
+  return SSIM_OK;
+}
+
+int ssim_spectrum_imag(long long ncomplex, int prec, const void* a, void* out,
+                       void* stream) {
+  SSIM_REQUIRE(ncomplex >= 0, "bad size");
+  if (ncomplex == 0) return SSIM_OK;
+  SSIM_REQUIRE(a && out, "null pointer");
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid((unsigned)ssim::ceil_div(ncomplex, PM_BLOCK));
+  if (prec == SSIM_PREC_F64)
+    hipLaunchKernelGGL(k_imag<double>, grid, dim3(PM_BLOCK), 0, st, ncomplex,
+                       static_cast<const double*>(a), static_cast<double*>(out));
+  else
+    hipLaunchKernelGGL(k_imag<float>, grid, dim3(PM_BLOCK), 0, st, ncomplex,
+                       static_cast<const float*>(a), static_cast<float*>(out));
+  SSIM_LAUNCHED("k_imag");
   return SSIM_OK;
 }
 

@@ -4,8 +4,16 @@
 // the Gaussian filter of the long-range mesh part,
 //   g(r) = erfc(r / 2rs) + r / (rs sqrt(pi)) exp(-r^2 / 4rs^2),
 //   F = F g(r)  [near, summed here over r < rcut]  +  F (1 - g(r))  [mesh].
-// Sources come from a cell list (cells.hip) with cells no smaller than rcut,
-// so the 27 cells around a target's cell hold every source within rcut.
+//
+// Sources and targets are binned on the same grid of cells no smaller than
+// rcut (cells.hip), so the 27 cells around a target's cell hold every source
+// within rcut.  Work is cut into chunks of <= 64 targets of one cell; one
+// wave takes a chunk, and all its lanes walk the same source list (the
+// three contiguous cells of each of the 9 neighbour rows, in the sources'
+// sorted order), so the source loads are wave-uniform (scalar loads) and
+// the loop has no divergence.  Each target's sum is made by one lane in the
+// fixed source order, so results do not depend on scheduling.
+//
 // Terms are FP32 on origin-relative coordinates (error ~1e-7 of each term,
 // far below the method's 1e-3 budget); the per-target sum is ADDED to the
 // FP64 accelerations.
@@ -13,76 +21,130 @@
 
 namespace {
 
-constexpr int PB = 256;
+constexpr int PB = 256;            // 4 waves per workgroup
+constexpr int WAVES = PB / 64;
+constexpr int CHUNK = 64;          // targets per wave
 
-__global__ __launch_bounds__(PB) void k_pp_short(
-    int nt, const double* __restrict__ tx, const double* __restrict__ ty,
-    const double* __restrict__ tz, double ox, double oy, double oz,
-    double inv_cell, int nc0, int nc1, int nc2,
-    const int* __restrict__ cell_start, const float4* __restrict__ src,
-    float eps2, float inv_2rs, float c_lin, float rcut2, double G,
-    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az) {
-  const int i = blockIdx.x * PB + threadIdx.x;
-  if (i >= nt) return;
-  const double rx = tx[i] - ox, ry = ty[i] - oy, rz = tz[i] - oz;
-  const float px = (float)rx, py = (float)ry, pz = (float)rz;
-  const int a = min(max((int)floor(rx * inv_cell), 0), nc0 - 1);
-  const int b = min(max((int)floor(ry * inv_cell), 0), nc1 - 1);
-  const int c = min(max((int)floor(rz * inv_cell), 0), nc2 - 1);
+__global__ __launch_bounds__(PB) void k_chunk_count(int ncells,
+                                                    const int* __restrict__ tstart,
+                                                    int* __restrict__ nchunk) {
+  const int c = blockIdx.x * PB + threadIdx.x;
+  if (c >= ncells) return;
+  nchunk[c] = (tstart[c + 1] - tstart[c] + CHUNK - 1) / CHUNK;
+}
+
+__global__ __launch_bounds__(PB) void k_chunk_map(int ncells,
+                                                  const int* __restrict__ cstart,
+                                                  int* __restrict__ map) {
+  const int c = blockIdx.x * PB + threadIdx.x;
+  if (c >= ncells) return;
+  for (int k = cstart[c]; k < cstart[c + 1]; ++k) map[k] = c;
+}
+
+__global__ __launch_bounds__(PB) void k_pp_cells(
+    const double* __restrict__ tx, const double* __restrict__ ty,
+    const double* __restrict__ tz, double ox, double oy, double oz, int nc0,
+    int nc1, int nc2, const int* __restrict__ cell_start,
+    const float4* __restrict__ src, const int* __restrict__ tstart,
+    const int* __restrict__ torder, const int* __restrict__ cstart,
+    const int* __restrict__ cmap, int ncells, float eps2, float inv_2rs,
+    float c_lin, float rcut2, double G, double* __restrict__ ax,
+    double* __restrict__ ay, double* __restrict__ az) {
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + (threadIdx.x >> 6));
+  if (w >= cstart[ncells]) return;
+  const int cell = __builtin_amdgcn_readfirstlane(cmap[w]);
+  const int first = __builtin_amdgcn_readfirstlane(
+      tstart[cell] + (w - cstart[cell]) * CHUNK);
+  const int count = __builtin_amdgcn_readfirstlane(min(CHUNK, tstart[cell + 1] - first));
+  const bool live = lane < count;
+  const int tgt = live ? torder[first + lane] : 0;
+  const float px = live ? (float)(tx[tgt] - ox) : 0.f;
+  const float py = live ? (float)(ty[tgt] - oy) : 0.f;
+  const float pz = live ? (float)(tz[tgt] - oz) : 0.f;
+  const int a = cell / (nc1 * nc2), b = (cell / nc2) % nc1, c = cell % nc2;
+  const int c0 = max(c - 1, 0), c1 = min(c + 1, nc2 - 1);
   float sx = 0.f, sy = 0.f, sz = 0.f;
   for (int da = max(a - 1, 0); da <= min(a + 1, nc0 - 1); ++da)
     for (int db = max(b - 1, 0); db <= min(b + 1, nc1 - 1); ++db) {
       const int row = (da * nc1 + db) * nc2;
-      const int c0 = max(c - 1, 0), c1 = min(c + 1, nc2 - 1);
-      // the cells c0..c1 of one row are contiguous in the sorted order
       const int s0 = cell_start[row + c0], s1 = cell_start[row + c1 + 1];
       for (int s = s0; s < s1; ++s) {
         const float4 q = src[s];
         const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        if (r2 >= rcut2) continue;
         const float r = sqrtf(r2);
         const float x = r * inv_2rs;
         const float g = erfcf(x) + c_lin * r * __expf(-x * x);
         const float inv = ssim_rsqrt_f32(r2 + eps2);
-        const float f = q.w * (inv * inv) * inv * g;
+        const float f = r2 < rcut2 ? q.w * (inv * inv) * inv * g : 0.f;
         sx = fmaf(f, dx, sx);
         sy = fmaf(f, dy, sy);
         sz = fmaf(f, dz, sz);
       }
     }
-  ax[i] += G * (double)sx;
-  ay[i] += G * (double)sy;
-  az[i] += G * (double)sz;
+  if (live) {
+    ax[tgt] += G * (double)sx;
+    ay[tgt] += G * (double)sy;
+    az[tgt] += G * (double)sz;
+  }
 }
 
 }  // namespace
 
 extern "C" {
 
+size_t ssim_pp_work_bytes(int nt, int ncells) {
+  const size_t nc = (size_t)(ncells > 0 ? ncells : 1);
+  const size_t chunks = (size_t)(nt > 0 ? nt : 1) / CHUNK + nc + 1;
+  return ssim::align256(nc * sizeof(int)) + ssim::align256((nc + 1) * sizeof(int)) +
+         ssim::align256(chunks * sizeof(int)) + ssim::scan_work_bytes((int)nc);
+}
+
 int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,
                   const double* origin, double cell, const int* nc,
-                  const int* cell_start, const void* packed, double G,
+                  const int* cell_start, const void* packed,
+                  const int* tcell_start, const int* torder, double G,
                   double eps2, double rs, double rcut, double* ax, double* ay,
-                  double* az, void* stream) {
+                  double* az, void* work, size_t work_bytes, void* stream) {
   SSIM_REQUIRE(nt >= 0, "nt must be >= 0");
   if (nt == 0) return SSIM_OK;
-  SSIM_REQUIRE(tx && ty && tz && origin && nc && cell_start && packed && ax &&
-                   ay && az,
+  SSIM_REQUIRE(tx && ty && tz && origin && nc && cell_start && packed &&
+                   tcell_start && torder && ax && ay && az && work,
                "null pointer");
+  SSIM_REQUIRE(nc[0] >= 1 && nc[1] >= 1 && nc[2] >= 1, "bad cell counts");
+  const long long ncl = (long long)nc[0] * nc[1] * nc[2];
+  SSIM_REQUIRE(ncl <= (1 << 26), "too many cells: %lld", ncl);
+  const int ncells = (int)ncl;
   SSIM_REQUIRE(rs > 0.0 && rcut > 0.0 && rcut <= cell,
                "need rs > 0 and 0 < rcut <= cell (rs=%g rcut=%g cell=%g)", rs,
                rcut, cell);
   SSIM_REQUIRE(eps2 > 0.0, "softening eps2 must be > 0");
+  SSIM_REQUIRE(work_bytes >= ssim_pp_work_bytes(nt, ncells), "workspace too small");
   hipStream_t st = ssim::as_stream(stream);
+  unsigned char* w = static_cast<unsigned char*>(work);
+  int* nchunk = reinterpret_cast<int*>(w);
+  w += ssim::align256((size_t)ncells * sizeof(int));
+  int* cstart = reinterpret_cast<int*>(w);
+  w += ssim::align256(((size_t)ncells + 1) * sizeof(int));
+  int* cmap = reinterpret_cast<int*>(w);
+  const size_t max_chunks = (size_t)nt / CHUNK + ncells + 1;
+  w += ssim::align256(max_chunks * sizeof(int));
+  const dim3 gc(ssim::ceil_div(ncells, PB));
+  hipLaunchKernelGGL(k_chunk_count, gc, dim3(PB), 0, st, ncells, tcell_start, nchunk);
+  SSIM_LAUNCHED("k_chunk_count");
+  SSIM_TRY(ssim::scan_exclusive(ncells, nchunk, cstart, w, st));
+  hipLaunchKernelGGL(k_chunk_map, gc, dim3(PB), 0, st, ncells, cstart, cmap);
+  SSIM_LAUNCHED("k_chunk_map");
+  // chunks <= nt / 64 + ncells: launch that bound, surplus waves exit
   const float c_lin = (float)(1.0 / (rs * 1.7724538509055160273));
-  hipLaunchKernelGGL(k_pp_short, dim3(ssim::ceil_div(nt, PB)), dim3(PB), 0, st,
-                     nt, tx, ty, tz, origin[0], origin[1], origin[2],
-                     1.0 / cell, nc[0], nc[1], nc[2], cell_start,
-                     static_cast<const float4*>(packed), (float)eps2,
-                     (float)(0.5 / rs), c_lin, (float)(rcut * rcut), G, ax, ay,
-                     az);
-  SSIM_LAUNCHED("k_pp_short");
+  hipLaunchKernelGGL(k_pp_cells, dim3(ssim::ceil_div((long long)max_chunks, WAVES)),
+                     dim3(PB), 0, st, tx, ty, tz, origin[0], origin[1], origin[2],
+                     nc[0], nc[1], nc[2], cell_start,
+                     static_cast<const float4*>(packed), tcell_start, torder,
+                     cstart, cmap, ncells, (float)eps2, (float)(0.5 / rs), c_lin,
+                     (float)(rcut * rcut), G, ax, ay, az);
+  SSIM_LAUNCHED("k_pp_cells");
   return SSIM_OK;
 }
 

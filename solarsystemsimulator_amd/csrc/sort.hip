@@ -65,6 +65,7 @@ __global__ __launch_bounds__(RT) void k_copy_pairs(int n, const int* __restrict_
 // Exclusive scan of in[0..n) into out[0..n], one workgroup: every thread
 // sums a contiguous chunk, a Hillis-Steele scan over the chunk sums gives
 // each chunk's base, and the chunks are written out.  out[n] = total.
+// Used directly for short arrays and for the block sums of long ones.
 __global__ __launch_bounds__(SCAN_T) void k_scan(int n, const int* __restrict__ in,
                                                  int* __restrict__ out) {
   __shared__ int part[SCAN_T];
@@ -90,20 +91,92 @@ __global__ __launch_bounds__(SCAN_T) void k_scan(int n, const int* __restrict__ 
   if (t == SCAN_T - 1) out[n] = part[SCAN_T - 1];
 }
 
+// Long arrays: blocks of SB = 4 * RT elements.  k_block_sum writes each
+// block's total, k_scan turns them into block offsets, k_block_scan scans a
+// block in LDS and adds its offset.
+constexpr int SB = 4 * RT;
+
+__device__ __forceinline__ int block_reduce_sum(int v, int* red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int w = RT / 2; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+__global__ __launch_bounds__(RT) void k_block_sum(int n, const int* __restrict__ in,
+                                                  int* __restrict__ bsum) {
+  __shared__ int red[RT];
+  const int base = blockIdx.x * SB + 4 * threadIdx.x;
+  int v = 0;
+  for (int k = 0; k < 4; ++k)
+    if (base + k < n) v += in[base + k];
+  const int tot = block_reduce_sum(v, red);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(RT) void k_block_scan(int n, const int* __restrict__ in,
+                                                   const int* __restrict__ boff,
+                                                   int nblocks, int* __restrict__ out) {
+  __shared__ int part[RT];
+  const int t = threadIdx.x;
+  const int base = blockIdx.x * SB + 4 * t;
+  int v[4], s = 0;
+  for (int k = 0; k < 4; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0;
+    s += v[k];
+  }
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < RT; off <<= 1) {
+    const int add = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += add;
+    __syncthreads();
+  }
+  int run = boff[blockIdx.x] + (t == 0 ? 0 : part[t - 1]);
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (blockIdx.x == 0 && t == 0) out[n] = boff[nblocks];
+}
+
 }  // namespace
 
 namespace ssim {
 
-int scan_exclusive(int n, const int* in, int* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_T), 0, st, n, in, out);
+size_t scan_work_bytes(int n) {
+  return 2 * align256(((size_t)ceil_div(n > 0 ? n : 1, SB) + 1) * sizeof(int));
+}
+
+int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st) {
+  if (n <= 4 * SB) {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_T), 0, st, n, in, out);
+    SSIM_LAUNCHED("k_scan");
+    return SSIM_OK;
+  }
+  SSIM_REQUIRE(work, "scan: workspace missing");
+  const int nb = ceil_div(n, SB);
+  int* bsum = static_cast<int*>(work);
+  int* boff = bsum + align256(((size_t)nb + 1) * sizeof(int)) / sizeof(int);
+  hipLaunchKernelGGL(k_block_sum, dim3(nb), dim3(RT), 0, st, n, in, bsum);
+  SSIM_LAUNCHED("k_block_sum");
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_T), 0, st, nb, bsum, boff);
   SSIM_LAUNCHED("k_scan");
+  hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(RT), 0, st, n, in, boff, nb, out);
+  SSIM_LAUNCHED("k_block_scan");
   return SSIM_OK;
 }
 
 size_t radix_sort_work_bytes(int n) {
   const size_t nn = (size_t)(n > 0 ? n : 1);
   const size_t tiles = (nn + RT - 1) / RT;
-  return 2 * align256(nn * sizeof(int)) + 2 * align256((256 * tiles + 1) * sizeof(int));
+  return 2 * align256(nn * sizeof(int)) + 2 * align256((256 * tiles + 1) * sizeof(int)) +
+         scan_work_bytes((int)(256 * tiles));
 }
 
 int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
@@ -121,6 +194,8 @@ int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
   int* hist = reinterpret_cast<int*>(w);
   w += align256((256 * (size_t)tiles + 1) * sizeof(int));
   int* offs = reinterpret_cast<int*>(w);
+  w += align256((256 * (size_t)tiles + 1) * sizeof(int));
+  void* scan_work = w;
   int *ka = keys, *va = vals, *kb = kt, *vb = vt;
   const int passes = (bits + 7) / 8;
   for (int p = 0; p < passes; ++p) {
@@ -128,7 +203,7 @@ int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
     hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(RT), 0, st, n, ka, shift,
                        tiles, hist);
     SSIM_LAUNCHED("k_radix_hist");
-    SSIM_TRY(scan_exclusive(256 * tiles, hist, offs, st));
+    SSIM_TRY(scan_exclusive(256 * tiles, hist, offs, scan_work, st));
     hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(RT), 0, st, n, ka, va,
                        shift, tiles, offs, kb, vb);
     SSIM_LAUNCHED("k_radix_scatter");

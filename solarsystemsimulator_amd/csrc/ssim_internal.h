@@ -40,8 +40,10 @@ int drift(int dim, int n, double* x, double* y, double* z, const double* vx,
 double yoshida_w1();
 double yoshida_w0();
 
-// Exclusive scan of n ints into out[0..n] (out[n] = total), sort.hip.
-int scan_exclusive(int n, const int* in, int* out, hipStream_t st);
+// Exclusive scan of n ints into out[0..n] (out[n] = total), sort.hip;
+// `work` holds scan_work_bytes(n) (unused for n <= 4096).
+size_t scan_work_bytes(int n);
+int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st);
 
 // Stable LSD radix sort of (keys, vals) in place on keys in [0, 2^bits).
 size_t radix_sort_work_bytes(int n);

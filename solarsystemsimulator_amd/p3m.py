@@ -6,9 +6,14 @@ a = a_mesh + a_near (+ a_learned, see learned.py):
   the masses, convolution with the long-range Plummer force
   F(d) (1 - g(|d|)), g(r) = erfc(r/2rs) + r/(rs sqrt(pi)) exp(-r^2/4rs^2),
   whose spectrum is divided by the squared CIC window, CIC gather at every
-  body (masses and tracers);
+  body (masses and tracers).  The kernel is odd, so its spectrum is kept as
+  its imaginary part, and the convolution runs as pruned FFT passes that
+  never touch the padding (ssim_pm_conv_isolated3);
 * a_near — the complementary F(d) g(|d|) summed directly over sources
-  within rcut through a cell list.
+  within rcut through a cell list.  Beyond rcut the mesh kernel is the
+  full F(d), so no part of a pair force is dropped (a hard cut would leave
+  out F g(rcut) ~ 0.6 % of the pair force at 5 rs, always attractive, a
+  bias that shows up in the energy of a contracting system).
 
 The reference's PM (pm_kernel.py:127-229) convolves with the full softened
 force on an adaptive mesh; this class keeps that adaptive, isolated mesh
@@ -36,26 +41,28 @@ class SplitForce:
     rcut_rs    near-field cutoff in units of rs
     margin     empty mesh cells kept around the bodies
     prec       mesh precision (pm.F64 or pm.F32)
+    freeze     keep the mesh placement while every body stays at least one
+               cell inside it (the force field is then a fixed function of
+               the positions between re-placements)
     """
 
     def __init__(self, dev, mesh_n: int = 128, eps2: float = 1e-6,
                  G: float = 1.0, rs_cells: float = 1.25, rcut_rs: float = 6.0,
-                 margin: float = 2.0, prec: int = P.F64):
+                 margin: float = 2.0, prec: int = P.F64, freeze: bool = False):
         import torch
         self.dev, self.n, self.eps2, self.G = dev, int(mesh_n), float(eps2), float(G)
         self.rs_cells, self.rcut_rs = float(rs_cells), float(rcut_rs)
-        self.margin, self.prec = float(margin), prec
+        self.margin, self.prec, self.freeze = float(margin), prec, bool(freeze)
         self.Pn = 2 * self.n
         self.plan = P.FFTPlan.get([self.Pn] * 3, prec, dev)
         dt = P._mesh_dtype(prec)
         self.rho = torch.zeros([self.n] * 3, dtype=dt, device=dev)
-        self.pad = torch.zeros([self.Pn] * 3, dtype=dt, device=dev)
-        self.full = torch.empty([self.Pn] * 3, dtype=dt, device=dev)
         self.spec = self.plan.new_complex()
         self.prod = self.plan.new_complex()
         self.fields = [torch.empty([self.n] * 3, dtype=dt, device=dev)
                        for _ in range(3)]
-        self.kspec = None
+        self.kim = None
+        self.fused = True     # False: pad + rfftn + product + irfftn + crop
         self.level = None
         self.h = None
         self.origin = None
@@ -65,6 +72,11 @@ class SplitForce:
     def _place(self, x):
         """Mesh origin and ladder cell size covering every body."""
         box = P.minmax(x)
+        if self.freeze and self.level is not None:
+            h, o = self.h, self.origin
+            if all(lo >= o[c] + h and hi <= o[c] + (self.n - 2) * h
+                   for c, (lo, hi) in enumerate(box)):
+                return self.level, h, o, box
         span = max(hi - lo for lo, hi in box)
         need = max(span, 1e-30) / (self.n - 1 - 2 * self.margin)
         level = math.ceil(64.0 * math.log2(need))
@@ -73,18 +85,23 @@ class SplitForce:
         return level, h, origin, box
 
     def _rebuild_kernels(self, h):
+        """Imaginary spectra of the 3 deconvolved long-range kernels."""
         import torch
         rs = self.rs_cells * h
-        real = self.full
-        self.kspec = []
+        st = D.stream(self.dev)
+        real = torch.empty([self.Pn] * 3, dtype=self.rho.dtype, device=self.dev)
+        nc = self.spec.numel() // 2
+        self.kim = []
         for c in range(3):
             _lib.call("ssim_pm_force_kernel", 3, self.n, c, h, self.eps2,
-                      self.G, rs, self.prec, _lib.ptr(real), D.stream(self.dev))
-            spec = self.plan.r2c(real)
+                      self.G, rs, self.rcut_rs * rs, self.prec, _lib.ptr(real), st)
+            self.plan.r2c(real, self.prod)
             _lib.call("ssim_pm_deconvolve", 3, self.Pn, self.prec,
-                      _lib.ptr(spec), D.stream(self.dev))
-            self.kspec.append(spec)
-        del torch
+                      _lib.ptr(self.prod), st)
+            kim = torch.empty(nc, dtype=self.rho.dtype, device=self.dev)
+            _lib.call("ssim_spectrum_imag", nc, self.prec, _lib.ptr(self.prod),
+                      _lib.ptr(kim), st)
+            self.kim.append(kim)
 
     # -- force --------------------------------------------------------------
     def mesh_accel(self, x, n_mass, m, a):
@@ -97,21 +114,36 @@ class SplitForce:
         src = [c[:n_mass] for c in x]
         P.deposit(src, m[:n_mass], [self.n] * 3, origin, h, False, self.prec,
                   mesh=self.rho)
-        self.pad.zero_()
-        st = D.stream(self.dev)
-        _lib.call("ssim_pm_pad", 3, self.n, self.prec, _lib.ptr(self.rho),
-                  _lib.ptr(self.pad), 0, st)
-        self.plan.r2c(self.pad, self.spec)
-        nc = self.spec.numel() // 2
-        for c in range(3):
-            _lib.call("ssim_spectrum_mul", nc, self.prec, _lib.ptr(self.spec),
-                      _lib.ptr(self.kspec[c]), _lib.ptr(self.prod), st)
-            self.plan.c2r(self.prod, self.full)
-            _lib.call("ssim_pm_pad", 3, self.n, self.prec, _lib.ptr(self.full),
-                      _lib.ptr(self.fields[c]), 1, st)
+        self.convolve()
         P.gather(self.fields, x, [self.n] * 3, origin, h, False, self.prec,
                  outs=a)
         return a
+
+    def convolve(self):
+        """fields = long-range acceleration meshes of self.rho."""
+        import torch
+        st = D.stream(self.dev)
+        if self.fused:
+            _lib.call("ssim_pm_conv_isolated3", self.n, self.prec,
+                      _lib.ptr(self.rho), *[_lib.ptr(k) for k in self.kim],
+                      *[_lib.ptr(f) for f in self.fields], _lib.ptr(self.spec),
+                      _lib.ptr(self.prod), _lib.ptr(self.plan.work),
+                      self.plan.work_bytes, st)
+            return self.fields
+        pad = torch.zeros([self.Pn] * 3, dtype=self.rho.dtype, device=self.dev)
+        full = torch.empty_like(pad)
+        _lib.call("ssim_pm_pad", 3, self.n, self.prec, _lib.ptr(self.rho),
+                  _lib.ptr(pad), 0, st)
+        self.plan.r2c(pad, self.spec)
+        kc = torch.zeros_like(self.prod)
+        for c in range(3):
+            kc[..., 1] = self.kim[c].view(kc.shape[:-1])
+            _lib.call("ssim_spectrum_mul", self.spec.numel() // 2, self.prec,
+                      _lib.ptr(self.spec), _lib.ptr(kc), _lib.ptr(self.prod), st)
+            self.plan.c2r(self.prod, full)
+            _lib.call("ssim_pm_pad", 3, self.n, self.prec, _lib.ptr(full),
+                      _lib.ptr(self.fields[c]), 1, st)
+        return self.fields
 
     def near_accel(self, x, n_mass, m, a):
         """a += near-field term (cell list of the masses)."""
@@ -124,16 +156,20 @@ class SplitForce:
         nc = [max(1, int(s // rcut)) for s in span]
         cell = max(max(s / k for s, k in zip(span, nc)), rcut)
         ncells = nc[0] * nc[1] * nc[2]
-        if self._cells_cap < ncells or self._np != n_mass:
-            self.cell_start = torch.empty(ncells + 1, dtype=torch.int32,
-                                          device=self.dev)
-            self.order = torch.empty(max(n_mass, 1), dtype=torch.int32,
-                                     device=self.dev)
+        nt = x[0].numel()
+        if self._cells_cap < ncells or self._np != n_mass or self._nt != nt:
+            i32 = dict(dtype=torch.int32, device=self.dev)
+            self.cell_start = torch.empty(ncells + 1, **i32)
+            self.order = torch.empty(max(n_mass, 1), **i32)
             self.packed = torch.empty((max(n_mass, 1), 4), dtype=torch.float32,
                                       device=self.dev)
-            wb = _lib.load().ssim_cells_work_bytes(n_mass, ncells)
+            self.tcell_start = torch.empty(ncells + 1, **i32)
+            self.torder = torch.empty(max(nt, 1), **i32)
+            lib = _lib.load()
+            wb = max(lib.ssim_cells_work_bytes(max(n_mass, nt), ncells),
+                     lib.ssim_pp_work_bytes(nt, ncells))
             self.cwork = D.workspace(wb, self.dev)
-            self._cells_cap, self._np = ncells, n_mass
+            self._cells_cap, self._np, self._nt = ncells, n_mass, nt
         gn = _lib.int_array(nc)
         org = _lib.double_array(lo)
         st = D.stream(self.dev)
@@ -142,14 +178,20 @@ class SplitForce:
                   _lib.ptr(self.cell_start), _lib.ptr(self.order),
                   _lib.ptr(self.packed), _lib.ptr(self.cwork),
                   self.cwork.numel(), st)
-        nt = x[0].numel()
+        _lib.call("ssim_cells_build", nt, *[_lib.ptr(c) for c in x], None,
+                  P._addr(org), cell, P._addr(gn), _lib.ptr(self.tcell_start),
+                  _lib.ptr(self.torder), None, _lib.ptr(self.cwork),
+                  self.cwork.numel(), st)
         _lib.call("ssim_pp_short", nt, *[_lib.ptr(c) for c in x], P._addr(org),
                   cell, P._addr(gn), _lib.ptr(self.cell_start),
-                  _lib.ptr(self.packed), self.G, self.eps2, rs, rcut,
-                  *[_lib.ptr(c) for c in a], st)
+                  _lib.ptr(self.packed), _lib.ptr(self.tcell_start),
+                  _lib.ptr(self.torder), self.G, self.eps2, rs, rcut,
+                  *[_lib.ptr(c) for c in a], _lib.ptr(self.cwork),
+                  self.cwork.numel(), st)
         return a
 
     _np = -1
+    _nt = -1
 
     def accelerations(self, x, n_mass, m, a):
         """a = mesh + near-field acceleration of all bodies in x (n_mass

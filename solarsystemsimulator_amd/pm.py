@@ -166,7 +166,7 @@ def build_force_kernels(grid_n: int, h: float, eps2: float, G: float = G_AU,
     out = []
     for c in range(dim):
         _lib.call("ssim_pm_force_kernel", dim, int(grid_n), c, float(h),
-                  float(eps2), float(G), 0.0, prec, _lib.ptr(real),
+                  float(eps2), float(G), 0.0, 0.0, prec, _lib.ptr(real),
                   D.stream(dev))
         out.append(plan.r2c(real))
     return tuple(out)

@@ -43,6 +43,61 @@ def test_cell_list_bit_exact(dev):
     np.testing.assert_array_equal(pk[:, 3], m[ro].astype(np.float32))
 
 
+@pytest.mark.parametrize("n,nc,clump", [(300_000, [40, 33, 29], 0.3),
+                                         (5, [2, 1, 3], 0.0)])
+def test_cell_list_large_and_clumped(dev, n, nc, clump):
+    """Radix passes over > 4096-entry histograms (multi-block scans), a
+    clump holding a third of the bodies in one cell, and no packed copy."""
+    print(f"cell list n={n} nc={nc}", flush=True)
+    rng = np.random.default_rng(4)
+    pos = [rng.uniform(0.0, 1.0, n) for _ in range(3)]
+    k = int(clump * n)
+    for c in range(3):
+        pos[c][:k] = 0.5 + rng.uniform(0.0, 1e-3, k)
+    origin, cell = [0.0, 0.0, 0.0], 1.0 / 29
+    ncells = int(np.prod(nc))
+    start = torch.empty(ncells + 1, dtype=torch.int32, device=dev)
+    order = torch.empty(n, dtype=torch.int32, device=dev)
+    w = D.workspace(_lib.load().ssim_cells_work_bytes(n, ncells), dev)
+    import ctypes
+    gn, org = _lib.int_array(nc), _lib.double_array(origin)
+    x = [D.to_dev(c, dev) for c in pos]
+    _lib.call("ssim_cells_build", n, *[_lib.ptr(c) for c in x], None,
+              ctypes.addressof(org), cell, ctypes.addressof(gn), _lib.ptr(start),
+              _lib.ptr(order), None, _lib.ptr(w), w.numel(), D.stream(dev))
+    rs, ro = p3m_np.cell_list(pos, origin, cell, nc)
+    assert np.array_equal(start.cpu().numpy(), rs)
+    assert np.array_equal(order.cpu().numpy(), ro)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 1e-5)])
+def test_pruned_convolution_matches_padded_fft(dev, prec, tol):
+    """ssim_pm_conv_isolated3 (pruned passes, imaginary kernel spectrum)
+    equals the corner of irfftn(rfftn(pad(rho)) * i kim) in NumPy, and the
+    unfused device path."""
+    print(f"pruned convolution prec={prec}", flush=True)
+    n = 32
+    sf = SplitForce(dev, mesh_n=n, eps2=1e-6, rs_cells=1.25, rcut_rs=4.5, prec=prec)
+    sf._rebuild_kernels(0.03)
+    rng = np.random.default_rng(5)
+    rho = rng.uniform(0.0, 1.0, (n, n, n))
+    rho[rng.uniform(size=rho.shape) < 0.5] = 0.0
+    sf.rho.copy_(torch.from_numpy(rho).to(sf.rho.dtype))
+    got = [f.cpu().numpy().astype(np.float64) for f in sf.convolve()]
+    sf.fused = False
+    ref_dev = [f.cpu().numpy().astype(np.float64) for f in sf.convolve()]
+    P = 2 * n
+    pad = np.zeros((P, P, P))
+    pad[:n, :n, :n] = sf.rho.cpu().numpy().astype(np.float64)
+    spec = np.fft.rfftn(pad)
+    for c in range(3):
+        kim = sf.kim[c].cpu().numpy().astype(np.float64).reshape(P, P, n + 1)
+        ref = np.fft.irfftn(spec * (1j * kim), s=(P, P, P), axes=(0, 1, 2))[:n, :n, :n]
+        scale = np.abs(ref).max()
+        assert np.abs(got[c] - ref).max() <= tol * scale
+        assert np.abs(ref_dev[c] - ref).max() <= tol * scale
+
+
 def test_near_field_matches_oracle(dev):
     n, t = 6000, 3000
     pos, m = cloud(n, 2)
