@@ -5,35 +5,43 @@ Workload (G = 1, unit cube, cold start): 250,000 equal masses on a perturbed
 100x50x50 lattice (seed 5) and 500,000 tracers on a 100x100x50 lattice
 (seed 55), Plummer softening eps = 1e-3, dt = 2.5e-4, leapfrog KDK with FP64
 state.  A step is one KDK step of all 750,000 bodies: half-kick + drift,
-one force evaluation, half-kick (+ the f32 snapshot for the direct run),
-all on the GPU with inputs resident in HBM.
+one force evaluation, half-kick, all on the GPU with inputs resident in HBM.
 
 * direct FP64: all-pairs force, 250k x 750k = 1.875e11 pair interactions
-  per step;
-* PM + learned: isolated 128^3 mesh (FP32, zero-padded 256^3 FFTs) for the
-  long range, near-field sum inside 5 rs, and the trained MLP correction
-  (BF16 MFMA); its median relative force error against the FP64 direct sum
-  is measured on the state the timed run ends in.
+  per step (+ the f32 snapshot row, as the reference's selfgrav kernel);
+* PM + learned: isolated 128^3 mesh (FP32, pruned 256^3 FFT convolution)
+  for the long range, near-field sum inside rcut, and the trained MLP
+  correction (BF16 MFMA); its median relative force error against the FP64
+  direct sum is measured on the state the timed run ends in.
 
 `value` is the PM + learned rate (the method the metric asks for); the
 direct rate, the ratio and the force errors are in the same JSON line.
+Per-kernel durations come from libssim's launch timing (HIP events around
+each launch on its stream) over `--profile-steps` extra steps after the
+timed region; `roofline` is the kernel with the largest time per step.
 Options: --gpus N (1 only), --steps K (PM steps timed), --warmup W,
---direct-steps, --small (1/50 of the bodies), --no-cpu.
+--direct-steps, --small (1/50 of the bodies), --no-cpu, --probe (a short
+run for hardware-counter passes: a 256 MiB copy, then three PM steps).
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
 FP64_PEAK_TFLOPS = 78.6          # MI355X vector FP64
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3         # MI355X vector FP32
+BF16_PEAK_TFLOPS = 2500.0        # MI355X dense BF16 matrix
 HBM_PEAK_GBS = 8000.0
-FLOPS_PER_PAIR = 20              # 3-D pair interaction convention
+FLOPS_PER_PAIR = 20              # 3-D pair interaction convention (direct)
+FLOPS_PER_NEAR_PAIR = 28         # near-field pair: 20 + table split function
+TRAFFIC_FILE = Path(__file__).resolve().parent / "profiles" / "pmc_traffic.json"
 
 
 def parse():
@@ -42,8 +50,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--direct-steps", type=int, default=3)
+    ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--small", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", action="store_true")
     ap.add_argument("--weights", default=None)
     return ap.parse_args()
 
@@ -52,25 +62,17 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-class Timer:
-    """HIP events on the current torch stream (the stream libssim uses)."""
-
-    def __init__(self, dev):
-        import torch
-        self.torch, self.dev = torch, dev
-
-    def __call__(self, fn, reps):
-        torch = self.torch
-        s = torch.cuda.current_stream(self.dev)
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        fn()
-        e0.record(s)
-        for _ in range(reps):
-            fn()
-        e1.record(s)
-        e1.synchronize()
-        return e0.elapsed_time(e1) / 1e3 / reps
+def kernel_times(lib, names):
+    """{name: (total ms, launches)} of the launches recorded since
+    ssim_timing(1)."""
+    out = {}
+    for k in names:
+        ms, cnt = ctypes.c_double(), ctypes.c_longlong()
+        rc = lib.ssim_timing_report(k.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+        if rc != 0:
+            raise RuntimeError(lib.ssim_last_error().decode())
+        out[k] = (ms.value, cnt.value)
+    return out
 
 
 def cpu_baseline(pos, m, tpos, eps2):
@@ -95,6 +97,21 @@ def cpu_baseline(pos, m, tpos, eps2):
                       f"evaluation ({dt:.1f} s), extrapolated to all targets"}
 
 
+def mesh_pass_bytes(n, mesh_bytes=4):
+    """Algorithmic HBM bytes of the 12 pruned FFT passes of one isolated
+    convolution of an n^3 FP32 mesh (padded P = 2n, half spectra of H =
+    n + 1, complex 2 x mesh_bytes, kernel spectra real mesh_bytes)."""
+    P, H = 2 * n, n + 1
+    c = 2 * mesh_bytes
+    fwd = [n ** 3 * mesh_bytes + n * n * H * c,             # R2C rows from rho
+           n * n * H * c + n * P * H * c,                   # y lines, x < n
+           n * P * H * c + P * P * H * c]                   # x lines
+    inv = [P * P * H * (c + mesh_bytes) + n * P * H * c,    # x lines * i kim
+           n * P * H * c + n * n * H * c,                   # y lines
+           n * n * H * c + n ** 3 * mesh_bytes]             # C2R rows
+    return fwd + 3 * inv
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -105,11 +122,11 @@ def main():
     from solarsystemsimulator_amd import _lib, scenes
     from solarsystemsimulator_amd import device as D
     from solarsystemsimulator_amd.learned_pm import (
-        DEFAULT_WEIGHTS, DT, PMLearnedSystem, force_errors, load_weights)
+        DEFAULT_WEIGHTS, DT, SPLIT, PMLearnedSystem, force_errors, load_weights)
     from solarsystemsimulator_amd.selfgrav import SelfGravSystem
 
     dev = D.require_gpu(0)
-    timer = Timer(dev)
+    lib = _lib.load()
     if args.small:
         pos, m, tpos = scenes.learned_scene((20, 25, 10), (20, 50, 10))
     else:
@@ -118,6 +135,21 @@ def main():
     nb = n + t
     eps2, dt = 1e-6, DT
     zeros_n, zeros_t = [np.zeros(n)] * 3, [np.zeros(t)] * 3
+    net = load_weights(dev, args.weights or DEFAULT_WEIGHTS)
+
+    if args.probe:
+        # hardware-counter passes: one 256 MiB copy (the calibration kernel),
+        # then three PM + learned steps
+        a = torch.empty(64 << 20, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize(dev)
+        torch.mul(a, 1.0, out=b)          # reads 256 MiB, writes 256 MiB
+        fast = PMLearnedSystem(dev, n, t, eps2, 1.0, net=net)
+        fast.set_bodies(pos, zeros_n, m, tpos, zeros_t)
+        fast.run(dt, 3)
+        torch.cuda.synchronize(dev)
+        print(json.dumps({"probe": "done", "copy_bytes": a.numel() * 4}))
+        return
 
     # ---- direct FP64 ------------------------------------------------------
     direct = SelfGravSystem(dev, 3, n, t, _lib.PREC_F64, _lib.SCHEME_KDK, 1.0, eps2)
@@ -133,18 +165,21 @@ def main():
     direct_rate = nb * args.direct_steps / wall_d
     log(f"direct FP64: {wall_d / args.direct_steps * 1e3:.2f} ms/step, "
         f"{direct_rate:.4g} body-steps/s")
-    kt = timer(direct.accelerations, 2)
+    lib.ssim_timing(1)
+    direct.accelerations()
+    direct.accelerations()
+    ms, cnt = kernel_times(lib, ["k_allpairs_f64"])["k_allpairs_f64"]
+    lib.ssim_timing(0)
+    kt = ms / cnt / 1e3
     pairs = n * nb
     ach = FLOPS_PER_PAIR * pairs / kt / 1e12
     roof_direct = {"bound": "valu", "kernel": "k_allpairs_f64<3,2>",
                    "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                    "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
-                   "launch_ms": kt * 1e3,
+                   "launch_us": kt * 1e6,
                    "work": f"{FLOPS_PER_PAIR} flop/pair x {pairs:.4g} pairs"}
 
     # ---- PM + learned -----------------------------------------------------
-    wpath = args.weights or DEFAULT_WEIGHTS
-    net = load_weights(dev, wpath)
     fast = PMLearnedSystem(dev, n, t, eps2, 1.0, net=net)
     fast.set_bodies(pos, zeros_n, m, tpos, zeros_t)
     fast.run(dt, max(args.warmup, 1))
@@ -168,41 +203,74 @@ def main():
     log(f"median rel force error: learned {np.median(err_l):.3e}, "
         f"without MLP {np.median(err_p):.3e}")
 
-    # dominant kernels of one PM + learned force evaluation, timed in isolation
-    parts = {
-        "mesh (deposit, FFTs, Green's function, gather)":
-            timer(lambda: fast.split.mesh_accel(fast.x, n, fast.m, fast.a), 5),
-        "near field (cell list + k_pp_short)":
-            timer(lambda: fast.split.near_accel(fast.x, n, fast.m, fast.a), 5),
-        "MLP (features + k_mlp_forward)":
-            timer(lambda: fast.net.predict(fast.features(), add_to=fast.a,
-                                           scale=fast.a_unit()), 5),
+    # ---- per-kernel durations (after the timed region) ---------------------
+    sf = fast.split
+    sf.bin_bodies(fast.x, n, fast.m)
+    tested, inside = sf.near_pair_counts(fast.x)
+    names = ["k_pp_cells", "k_fft_pass", "k_tsc_deposit", "k_tsc_gather",
+             "k_cic_deposit", "k_cic_gather", "k_mlp_forward", "k_mlp_features"]
+    lib.ssim_timing(1)
+    fast.run(dt, args.profile_steps)
+    times = kernel_times(lib, names)
+    lib.ssim_timing(0)
+    steps_p = args.profile_steps
+    n_mesh = sf.n
+    pass_bytes = mesh_pass_bytes(n_mesh)
+    mlp_flops = 2 * nb * (64 * 32 + 2 * 64 * 64 + 16 * 64)
+    work = {
+        "k_pp_cells": ("valu", FLOPS_PER_NEAR_PAIR * inside, FP32_PEAK_TFLOPS,
+                       "TFLOP/s", f"{FLOPS_PER_NEAR_PAIR} flop x {inside:.4g} "
+                       f"pairs inside rcut ({tested:.4g} pairs tested)"),
+        "k_fft_pass": ("hbm", sum(pass_bytes) / len(pass_bytes), HBM_PEAK_GBS,
+                       "GB/s", f"{sum(pass_bytes) / 1e6:.1f} MB over 12 pruned "
+                       f"passes of one {2 * n_mesh}^3 isolated convolution"),
+        "k_tsc_deposit": ("hbm", n * 32 + 27 * 4 * n, HBM_PEAK_GBS, "GB/s",
+                          "32 B/mass in + 27 FP32 node updates"),
+        "k_tsc_gather": ("hbm", nb * (24 + 24) + 3 * 4 * n_mesh ** 3,
+                         HBM_PEAK_GBS, "GB/s",
+                         "48 B/body (position in, acceleration out) + 3 meshes"),
+        "k_cic_deposit": ("hbm", n * 32 + 8 * 4 * n, HBM_PEAK_GBS, "GB/s",
+                          "32 B/mass in + 8 FP32 node updates"),
+        "k_cic_gather": ("hbm", nb * (24 + 24) + 3 * 4 * n_mesh ** 3,
+                         HBM_PEAK_GBS, "GB/s",
+                         "48 B/body (position in, acceleration out) + 3 meshes"),
+        "k_mlp_forward": ("mfma", mlp_flops, BF16_PEAK_TFLOPS, "TFLOP/s",
+                          "2 x 11,264 MAC per body, BF16"),
+        "k_mlp_features": ("hbm", nb * (24 + 64), HBM_PEAK_GBS, "GB/s",
+                           "24 B position in + 64 B feature row out per body"),
     }
-    for k_, v_ in parts.items():
-        log(f"  {k_}: {v_ * 1e3:.3f} ms")
-    # algorithmic HBM bytes of the mesh part per evaluation (FP32 mesh):
-    # deposit (n x 32 B + mesh 4 B/node), 1 forward + 3 inverse 3-D FFTs on
-    # the padded mesh (3 passes each, read+write the half spectrum), 3
-    # spectral products, 3 crops, gather (nb x (24 B in + 24 B out) + 3 x
-    # mesh 4 B/node)
-    P_ = 2 * fast.split.n
-    spec = P_ * P_ * (P_ // 2 + 1) * 8
-    mesh4 = fast.split.n ** 3 * 4
-    mesh_bytes = (n * 32 + mesh4 + 4 * 3 * 2 * spec + 3 * 3 * spec
-                  + 3 * 2 * mesh4 + nb * 48 + 3 * mesh4)
-    mt = list(parts.values())[0]
-    roof = {"bound": "hbm", "kernel": "PM mesh pipeline (k_fft_pass x12, "
-            "k_cic_deposit, k_cmul x3, k_pm_pad, k_cic_gather)",
-            "achieved": mesh_bytes / mt / 1e9, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": mesh_bytes / mt / 1e9 / HBM_PEAK_GBS,
-            "traffic": None, "launch_ms": mt * 1e3,
-            "work": f"{mesh_bytes / 1e6:.1f} MB algorithmic per evaluation"}
+    kernels = {}
+    for k, (tot_ms, cnt) in times.items():
+        if cnt == 0:
+            continue
+        bound, w, peak, unit, desc = work[k]
+        avg_s = tot_ms / cnt / 1e3
+        scale = 1e9 if unit == "GB/s" else 1e12
+        ach = w / avg_s / scale
+        kernels[k] = {"bound": bound, "avg_us": avg_s * 1e6,
+                      "launches_per_step": cnt / steps_p,
+                      "us_per_step": tot_ms / steps_p * 1e3,
+                      "achieved": ach, "peak": peak, "unit": unit,
+                      "frac": ach / peak, "work": desc}
+        log(f"  {k}: {cnt / steps_p:.0f} x {avg_s * 1e6:.1f} us/step, "
+            f"{ach:.1f} {unit} ({ach / peak:.1%} of peak)")
+    dom = max(kernels, key=lambda k: kernels[k]["us_per_step"])
+    d = kernels[dom]
+    traffic = None
+    if TRAFFIC_FILE.exists():
+        rec = json.loads(TRAFFIC_FILE.read_text())
+        if rec.get("kernel") == dom and rec.get("workload") == "250k_cube":
+            traffic = rec.get("traffic_bytes_per_launch")
+    roof = {"bound": d["bound"], "kernel": dom, "achieved": d["achieved"],
+            "peak": d["peak"], "unit": d["unit"], "frac": d["frac"],
+            "traffic": traffic, "launch_us": d["avg_us"], "work": d["work"]}
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pos, m, tpos, eps2)
         log(f"cpu baseline: {cpu['value']:.4g} body-steps/s")
 
+    split = {k: v for k, v in SPLIT.items() if k != "prec"}
     out = {
         "metric": "body-steps/s at 250k masses: direct f64 vs PM+learned at "
                   "<=1e-3 force err",
@@ -213,15 +281,17 @@ def main():
         "precision": "f64 state; f32 mesh and near field; bf16 MLP with f32 "
                      "accumulation; direct run all f64",
         "config": {"workload": f"{n} masses + {t} tracers, perturbed unit cube, "
-                               f"eps=1e-3, dt={dt:g}, leapfrog KDK, PM 128^3 "
-                               "isolated + near field + learned correction",
-                   "masses": n, "tracers": t, "method": "pm_learned"},
+                               f"eps=1e-3, dt={dt:g}, leapfrog KDK, PM "
+                               f"{n_mesh}^3 isolated + near field + learned "
+                               "correction",
+                   "masses": n, "tracers": t, "method": "pm_learned",
+                   "split": split},
         "direct_f64_body_steps_per_s": direct_rate,
         "speedup_vs_direct_f64": value / direct_rate,
         "force_err_median": float(np.median(err_l)),
         "force_err_p90": float(np.percentile(err_l, 90)),
         "force_err_median_without_mlp": float(np.median(err_p)),
-        "ms_per_eval_parts": {k_: v_ * 1e3 for k_, v_ in parts.items()},
+        "kernels": kernels,
         "roofline": roof,
         "roofline_direct": roof_direct,
         "cpu_baseline": cpu,

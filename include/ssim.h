@@ -51,6 +51,15 @@ const char* ssim_last_error(void);
 const char* ssim_version(void);
 /* Number of HIP devices visible (0 on a host without GPU, never fails). */
 int ssim_device_count(void);
+/* Launch timing for measurement runs: ssim_timing(1) clears the records and
+ * brackets every later launch of the library's main kernels (k_allpairs_f64,
+ * k_fft_pass, k_pp_cells, k_tsc_deposit, k_tsc_gather, k_cic_deposit,
+ * k_cic_gather, k_mlp_features, k_mlp_forward) with HIP events on the
+ * launch's stream; ssim_timing(0) stops recording.  ssim_timing_report
+ * waits for the recorded launches of `kernel` and returns their number and
+ * summed duration.  Not thread-safe against concurrent launches. */
+int ssim_timing(int enable);
+int ssim_timing_report(const char* kernel, double* total_ms, long long* launches);
 
 /* ------------------------------------------------------------------ */
 /* All-pairs softened gravity                                          */
@@ -225,10 +234,25 @@ int ssim_pm_force_kernel(int dim, int n, int comp, double h, double eps2,
                          double G, double rs, double rcut, int mesh_prec,
                          void* out, void* stream);
 /* Divide a half spectrum on the padded mesh (padded_n^dim) by the squared
- * CIC window prod_c sinc^4(pi f_c / padded_n) (no reference counterpart:
- * used by the split mesh + near-field force). */
-int ssim_pm_deconvolve(int dim, int padded_n, int prec, void* spectrum,
-                       void* stream);
+ * assignment window prod_c sinc^(2 order)(pi f_c / padded_n), order 2 for
+ * CIC and 3 for TSC (no reference counterpart: used by the split mesh +
+ * near-field force). */
+int ssim_pm_deconvolve(int dim, int padded_n, int prec, int order,
+                       void* spectrum, void* stream);
+/* Triangular-shaped-cloud deposit of np masses on an isolated grid_n^3
+ * mesh (ADDS to mesh): weights (1/2 - d)^2/2, 3/4 - d^2, (1/2 + d)^2/2 on
+ * the nodes c-1, c, c+1 of each axis around the nearest node c, d the
+ * offset from it in cells; nodes outside the mesh are skipped.  No
+ * reference counterpart (the split force's assignment). */
+int ssim_tsc_deposit(int np, const double* x, const double* y, const double* z,
+                     const double* m, int grid_n, const double* origin, double h,
+                     int mesh_prec, void* mesh, void* stream);
+/* TSC interpolation of three meshes at np points: o_c = sum w f_c
+ * (overwrites o_c, FP64). */
+int ssim_tsc_gather(int np, const double* x, const double* y, const double* z,
+                    int grid_n, const double* origin, double h, int mesh_prec,
+                    const void* f0, const void* f1, const void* f2, double* o0,
+                    double* o1, double* o2, void* stream);
 /* out = a * b over ncomplex interleaved complex values (the spectral
  * products of pm_force_field, pm_kernel.py:144-146). */
 int ssim_spectrum_mul(long long ncomplex, int prec, const void* a,
@@ -280,11 +304,19 @@ int ssim_cells_build(int n, const double* x, const double* y, const double* z,
                      const double* m, const double* origin, double cell,
                      const int* nc, int* cell_start, int* order, void* packed,
                      void* work, size_t work_bytes, void* stream);
+/* The same binning without the fixed order inside a cell (slots handed
+ * out by atomics): for lists whose consumer does not depend on the order,
+ * such as the near-field targets.  No hipMemset inside. */
+size_t ssim_cells_bin_work_bytes(int n, int ncells);
+int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
+                   const double* origin, double cell, const int* nc,
+                   int* cell_start, int* order, void* work, size_t work_bytes,
+                   void* stream);
 /* ADD the near-field force G m d g(r) / (r^2 + eps2)^(3/2), r < rcut, of
  * the binned sources (cell_start + packed, from ssim_cells_build) to nt
  * targets' accelerations (3-D).  The targets are binned on the same grid
- * (tcell_start, torder from ssim_cells_build on the targets, packed = NULL)
- * and rcut <= cell.  Each target's sum runs over the sources in their sorted
+ * (tcell_start, torder from ssim_cells_bin or ssim_cells_build) and
+ * rcut <= cell.  Each target's sum runs over the sources in their sorted
  * order, so the result is deterministic.  New: the reference has no split
  * force. */
 size_t ssim_pp_work_bytes(int nt, int ncells);
@@ -294,6 +326,15 @@ int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,
                   const int* tcell_start, const int* torder, double G,
                   double eps2, double rs, double rcut, double* ax, double* ay,
                   double* az, void* work, size_t work_bytes, void* stream);
+/* Work accounting of ssim_pp_short with the same arguments: counts[0] =
+ * source-target pairs tested, counts[1] = pairs closer than rcut (a body
+ * with itself excluded); counts is a device array of 2, no forces. */
+int ssim_pp_count(int nt, const double* tx, const double* ty, const double* tz,
+                  const double* origin, double cell, const int* nc,
+                  const int* cell_start, const void* packed,
+                  const int* tcell_start, const int* torder, double eps2,
+                  double rs, double rcut, unsigned long long* counts, void* work,
+                  size_t work_bytes, void* stream);
 
 /* ------------------------------------------------------------------ */
 /* Uniform-grid collision search (bit-exact ordered pair list)         */

@@ -5,11 +5,20 @@ INFRASTRUCTURE ONLY.
   the grid, stable argsort of the flat cell key (ascending body index within
   a cell), exclusive prefix of the counts.
 * `near_field` — brute force over all source/target pairs of
-  G m d g(r) / (r^2 + eps2)^(3/2) for r < rcut with the Gaussian split
-  g(r) = erfc(r/2rs) + r/(rs sqrt(pi)) exp(-r^2/4rs^2).
-Pinned in tests/test_p3m_oracle.py: g -> 1 as rs -> infinity turns the
-near field into the plain all-pairs sum (checked against the long-double
-oracle), and mesh part + near part of a pair reproduce the Plummer force.
+  G m d gS(r) / (r^2 + eps2)^(3/2) for r < rcut with the Gaussian split
+  g(r) = erfc(r/2rs) + r/(rs sqrt(pi)) exp(-r^2/4rs^2) tapered to zero over
+  [rcut - rs, rcut] by S(u) = 1 - u^3 (10 - 15u + 6u^2).
+* `tsc_deposit` / `tsc_gather` — triangular-shaped-cloud assignment on an
+  isolated 3-D mesh (3 nodes per axis around the nearest node).
+Pinned in tests/test_p3m_oracle.py: the cell list against a brute-force
+binning; g -> 1 as rs -> infinity turns the near field into the plain
+all-pairs sum (checked against the long-double oracle); the taper is 1
+below rcut - rs, 0 at rcut, with continuous first and second derivatives
+(finite differences); 1 - g(r) equals
+the mass fraction of a Gaussian cloud (variance 2 rs^2) inside r, computed
+by quadrature; TSC conserves mass, puts 3/4 per axis on the node a body
+sits on, interpolates linear fields exactly and is the adjoint of its
+gather.
 """
 from __future__ import annotations
 
@@ -36,6 +45,16 @@ def split_g(r, rs):
     return erfc(x) + r / (rs * math.sqrt(math.pi)) * np.exp(-x * x)
 
 
+def taper(r, rs, rcut):
+    u = np.clip((np.asarray(r, np.float64) - (rcut - rs)) / rs, 0.0, 1.0)
+    return np.maximum(1.0 - u ** 3 * (10.0 - 15.0 * u + 6.0 * u * u), 0.0)
+
+
+def split_gs(r, rs, rcut):
+    """The tapered split function the near field sums."""
+    return np.where(np.asarray(r) < rcut, split_g(r, rs) * taper(r, rs, rcut), 0.0)
+
+
 def near_field(src, m, tgt, G, eps2, rs, rcut, block=2048):
     src = [np.asarray(c, np.float64) for c in src]
     out = np.zeros((3, len(tgt[0])))
@@ -45,7 +64,44 @@ def near_field(src, m, tgt, G, eps2, rs, rcut, block=2048):
         r2 = d[0] ** 2 + d[1] ** 2 + d[2] ** 2
         r = np.sqrt(r2)
         w = np.where(r2 < rcut * rcut,
-                     m[None, :] * split_g(r, rs) / (r2 + eps2) ** 1.5, 0.0)
+                     m[None, :] * split_gs(r, rs, rcut) / (r2 + eps2) ** 1.5, 0.0)
         for c in range(3):
             out[c, b0:b0 + block] = G * (w * d[c]).sum(1)
+    return out
+
+
+def tsc_weights(f):
+    """Nearest node c = floor(f + 1/2) and the weights of c-1, c, c+1."""
+    c = np.floor(f + 0.5)
+    d = f - c
+    return c.astype(np.int64), np.stack([0.5 * (0.5 - d) ** 2, 0.75 - d * d,
+                                         0.5 * (0.5 + d) ** 2])
+
+
+def _tsc_stencil(pos, n, origin, h):
+    cw = [tsc_weights((np.asarray(pos[a], np.float64) - origin[a]) / h)
+          for a in range(3)]
+    for a in range(3):
+        for b in range(3):
+            for e in range(3):
+                idx = [cw[0][0] + a - 1, cw[1][0] + b - 1, cw[2][0] + e - 1]
+                ok = np.ones(len(idx[0]), bool)
+                for q in range(3):
+                    ok &= (idx[q] >= 0) & (idx[q] < n)
+                w = cw[0][1][a] * cw[1][1][b] * cw[2][1][e]
+                yield idx, ok, w
+
+
+def tsc_deposit(pos, m, n, origin, h):
+    rho = np.zeros((n, n, n))
+    m = np.asarray(m, np.float64)
+    for idx, ok, w in _tsc_stencil(pos, n, origin, h):
+        np.add.at(rho, tuple(i[ok] for i in idx), (m * w)[ok])
+    return rho
+
+
+def tsc_gather(field, pos, n, origin, h):
+    out = np.zeros(len(pos[0]))
+    for idx, ok, w in _tsc_stencil(pos, n, origin, h):
+        out[ok] += w[ok] * field[tuple(i[ok] for i in idx)]
     return out

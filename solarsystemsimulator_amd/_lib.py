@@ -33,6 +33,8 @@ SIGNATURES = {
     "ssim_last_error": (ctypes.c_char_p, []),
     "ssim_version": (ctypes.c_char_p, []),
     "ssim_device_count": (c_int, []),
+    "ssim_timing": (c_int, [c_int]),
+    "ssim_timing_report": (c_int, [ctypes.c_char_p, _P, _P]),
     "ssim_allpairs_work_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
     "ssim_allpairs": (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, c_int, _P,
                               _P, _P, _P, _P, _P, c_double, c_double, _P,
@@ -70,7 +72,11 @@ SIGNATURES = {
     "ssim_pm_force_kernel": (c_int, [c_int, c_int, c_int, c_double, c_double,
                                      c_double, c_double, c_double, c_int, _P,
                                      _P]),
-    "ssim_pm_deconvolve": (c_int, [c_int, c_int, c_int, _P, _P]),
+    "ssim_pm_deconvolve": (c_int, [c_int, c_int, c_int, c_int, _P, _P]),
+    "ssim_tsc_deposit": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double,
+                                 c_int, _P, _P]),
+    "ssim_tsc_gather": (c_int, [c_int, _P, _P, _P, c_int, _P, c_double, c_int,
+                                _P, _P, _P, _P, _P, _P, _P]),
     "ssim_spectrum_mul": (c_int, [c_longlong, c_int, _P, _P, _P, _P]),
     "ssim_spectrum_imag": (c_int, [c_longlong, c_int, _P, _P, _P]),
     "ssim_pm_conv_isolated3": (c_int, [c_int, c_int] + [_P] * 9 + [_P, c_size_t,
@@ -82,10 +88,16 @@ SIGNATURES = {
     "ssim_cells_work_bytes": (c_size_t, [c_int, c_int]),
     "ssim_cells_build": (c_int, [c_int, _P, _P, _P, _P, _P, c_double, _P, _P,
                                  _P, _P, _P, c_size_t, _P]),
+    "ssim_cells_bin_work_bytes": (c_size_t, [c_int, c_int]),
+    "ssim_cells_bin": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P,
+                               c_size_t, _P]),
     "ssim_pp_work_bytes": (c_size_t, [c_int, c_int]),
     "ssim_pp_short": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P,
                               _P, c_double, c_double, c_double, c_double, _P,
                               _P, _P, _P, c_size_t, _P]),
+    "ssim_pp_count": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P,
+                              _P, c_double, c_double, c_double, _P, _P, c_size_t,
+                              _P]),
     "ssim_collide_work_bytes": (c_size_t, [c_int, c_longlong]),
     "ssim_collide_hits": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, c_float,
                                   c_float, c_float, c_double, c_double,

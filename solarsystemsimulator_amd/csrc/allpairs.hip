@@ -295,9 +295,10 @@ int launch_allpairs(int prec, int ns, const double* sx, const double* sy,
   dim3 grid(ssim::ceil_div(nt, AP_BLOCK * AP_T), S_eff);
   double* part = static_cast<double*>(work);
   if (prec == SSIM_PREC_F64)
-    hipLaunchKernelGGL((k_allpairs_f64<D, AP_T>), grid, dim3(AP_BLOCK), 0, st,
-                       sx, sy, sz, sm, ns, chunk, tx, ty, tz, nt, ax, ay, az,
-                       part, G, eps2);
+    SSIM_TIMED("k_allpairs_f64", st,
+               hipLaunchKernelGGL((k_allpairs_f64<D, AP_T>), grid, dim3(AP_BLOCK),
+                                  0, st, sx, sy, sz, sm, ns, chunk, tx, ty, tz, nt,
+                                  ax, ay, az, part, G, eps2));
   else
     hipLaunchKernelGGL((k_allpairs_f32c<D, AP_T>), grid, dim3(AP_BLOCK), 0, st,
                        sx, sy, sz, sm, ns, chunk, tx, ty, tz, nt, ax, ay, az,

@@ -41,6 +41,17 @@ __global__ __launch_bounds__(CB) void k_cell_key(
   atomicAdd(&count[k], 1);
 }
 
+// Unstable placement: a cursor per cell hands out the slots.
+__global__ __launch_bounds__(CB) void k_scatter_any(int n, const int* __restrict__ key,
+                                                    const int* __restrict__ start,
+                                                    int* __restrict__ cursor,
+                                                    int* __restrict__ order) {
+  const int i = blockIdx.x * CB + threadIdx.x;
+  if (i >= n) return;
+  const int k = key[i];
+  order[start[k] + atomicAdd(&cursor[k], 1)] = i;
+}
+
 __global__ __launch_bounds__(CB) void k_pack(
     int n, const int* __restrict__ order, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z,
@@ -105,6 +116,52 @@ int ssim_cells_build(int n, const double* x, const double* y, const double* z,
                        order, x, y, z, m, origin[0], origin[1], origin[2],
                        static_cast<float4*>(packed));
     SSIM_LAUNCHED("k_pack");
+  }
+  return SSIM_OK;
+}
+
+size_t ssim_cells_bin_work_bytes(int n, int ncells) {
+  return ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int)) +
+         2 * ssim::align256((size_t)(ncells > 0 ? ncells : 1) * sizeof(int)) +
+         ssim::scan_work_bytes(ncells);
+}
+
+int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
+                   const double* origin, double cell, const int* nc,
+                   int* cell_start, int* order, void* work, size_t work_bytes,
+                   void* stream) {
+  SSIM_REQUIRE(n >= 0, "n must be >= 0");
+  SSIM_REQUIRE(origin && nc && cell > 0.0, "bad cell grid");
+  SSIM_REQUIRE(nc[0] >= 1 && nc[1] >= 1 && nc[2] >= 1, "bad cell counts");
+  const long long ncl = (long long)nc[0] * nc[1] * nc[2];
+  SSIM_REQUIRE(ncl <= (1 << 26), "too many cells: %lld", ncl);
+  const int ncells = (int)ncl;
+  SSIM_REQUIRE(cell_start && order && (n == 0 || (x && y && z)), "null pointer");
+  SSIM_REQUIRE(work && work_bytes >= ssim_cells_bin_work_bytes(n, ncells),
+               "workspace too small");
+  hipStream_t st = ssim::as_stream(stream);
+  unsigned char* w = static_cast<unsigned char*>(work);
+  int* key = reinterpret_cast<int*>(w);
+  w += ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int));
+  int* count = reinterpret_cast<int*>(w);
+  w += ssim::align256((size_t)ncells * sizeof(int));
+  int* cursor = reinterpret_cast<int*>(w);
+  w += ssim::align256((size_t)ncells * sizeof(int));
+  const dim3 gc(ssim::ceil_div(ncells, CB));
+  hipLaunchKernelGGL(k_zero_i32, gc, dim3(CB), 0, st, ncells, count);
+  hipLaunchKernelGGL(k_zero_i32, gc, dim3(CB), 0, st, ncells, cursor);
+  SSIM_LAUNCHED("k_zero_i32");
+  if (n > 0) {
+    hipLaunchKernelGGL(k_cell_key, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
+                       n, x, y, z, origin[0], origin[1], origin[2], 1.0 / cell,
+                       nc[0], nc[1], nc[2], key, order, count);
+    SSIM_LAUNCHED("k_cell_key");
+  }
+  SSIM_TRY(ssim::scan_exclusive(ncells, count, cell_start, w, st));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter_any, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
+                       n, key, cell_start, cursor, order);
+    SSIM_LAUNCHED("k_scatter_any");
   }
   return SSIM_OK;
 }

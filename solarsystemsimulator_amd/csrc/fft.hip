@@ -25,8 +25,8 @@
 namespace {
 
 constexpr int FFT_THREADS = 256;
-constexpr int FFT_TILE = 4096;   // complex values per workgroup tile
-constexpr int FFT_MAXB = 16;     // lines per tile along a strided axis
+constexpr int FFT_TILE_BYTES = 65536;  // LDS tile of one workgroup
+constexpr int FFT_MAXB = 64;           // lines per tile along a strided axis
 constexpr int FFT_MAX_L = 4096;
 
 enum { MODE_C2C = 0, MODE_R2C = 1, MODE_C2R = 2 };
@@ -35,6 +35,14 @@ template <typename F>
 struct cpx {
   F re, im;
 };
+
+// complex values per tile: 8192 in FP32, 4096 in FP64 (64 KB of LDS, two
+// workgroups per CU); a strided pass then moves rows of 32 (FP32) or 16
+// (FP64) consecutive lines, 256 contiguous bytes
+template <typename F>
+constexpr int tile_elems() {
+  return FFT_TILE_BYTES / (int)sizeof(cpx<F>);
+}
 
 template <typename F>
 __device__ __forceinline__ cpx<F> cmul(cpx<F> a, cpx<F> b) {
@@ -168,7 +176,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
   // ---- Stockham stages over all lines of the tile ----
   int p = 1;
   if (P.log2L & 1) {  // one radix-2 stage (p == 1: k == 0, no twiddle)
-    constexpr int NB = FFT_TILE / 2 / FFT_THREADS;
+    constexpr int NB = tile_elems<F>() / 2 / FFT_THREADS;
     const int T = L / 2;
     cpx<F> r0[NB], r1[NB];
 #pragma unroll
@@ -196,7 +204,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     p = 2;
   }
   for (; p < L; p *= 4) {
-    constexpr int NB = FFT_TILE / 4 / FFT_THREADS;
+    constexpr int NB = tile_elems<F>() / 4 / FFT_THREADS;
     const int T = L / 4;
     const int twstep = L / (4 * p);
     cpx<F> r[NB][4];
@@ -321,15 +329,16 @@ int launch_pass(const void* in, void* out, Pass P, const void* tw, int inverse,
   else
     nwg = P.n_outer * ssim::ceil_div(P.n_inner, P.lines_per_tile);
   const size_t lds = (size_t)P.lines_per_tile * P.L * sizeof(cpx<F>);
-  hipLaunchKernelGGL((k_fft_pass<F, MODE>), dim3(nwg), dim3(FFT_THREADS), lds,
-                     st, in, out, P, static_cast<const cpx<F>*>(tw), inverse,
-                     scale, mul);
+  SSIM_TIMED("k_fft_pass", st,
+             hipLaunchKernelGGL((k_fft_pass<F, MODE>), dim3(nwg), dim3(FFT_THREADS),
+                                lds, st, in, out, P, static_cast<const cpx<F>*>(tw),
+                                inverse, scale, mul));
   SSIM_LAUNCHED("k_fft_pass");
   return SSIM_OK;
 }
 
 // Pass along `axis` of the complex array with shape cdims (in place).
-Pass strided_pass(int ndim, const int* cdims, int axis) {
+Pass strided_pass(int ndim, const int* cdims, int axis, int tile) {
   Pass P{};
   P.L = cdims[axis];
   P.log2L = ilog2(P.L);
@@ -344,7 +353,7 @@ Pass strided_pass(int ndim, const int* cdims, int axis) {
   P.in_count = P.out_count = P.L;
   P.R1 = 1;
   P.s_in_hi = P.s_out_hi = 0;
-  int b = FFT_TILE / P.L;
+  int b = tile / P.L;
   if (b > FFT_MAXB) b = FFT_MAXB;
   if (b > P.n_inner) {
     b = 1;
@@ -354,7 +363,7 @@ Pass strided_pass(int ndim, const int* cdims, int axis) {
   return P;
 }
 
-Pass row_pass(long long rows, int L, long long s_in, long long s_out) {
+Pass row_pass(long long rows, int L, long long s_in, long long s_out, int tile) {
   Pass P{};
   P.L = L;
   P.log2L = ilog2(L);
@@ -367,7 +376,7 @@ Pass row_pass(long long rows, int L, long long s_in, long long s_out) {
   P.in_count = P.out_count = L;
   P.R1 = rows > 0 ? rows : 1;
   P.s_in_hi = P.s_out_hi = 0;
-  int b = FFT_TILE / L;
+  int b = tile / L;
   if (b < 1) b = 1;
   P.lines_per_tile = b;
   return P;
@@ -383,12 +392,12 @@ int fft_r2c(int ndim, const int* dims, const F* in, cpx<F>* out,
   for (int a = 0; a < ndim - 1; ++a) rows *= dims[a];
   const int L = dims[ndim - 1];
   SSIM_TRY((launch_pass<F, MODE_R2C>(
-      in, out, row_pass(rows, L, L, cdims[ndim - 1]),
+      in, out, row_pass(rows, L, L, cdims[ndim - 1], tile_elems<F>()),
       work + tw_offset(ndim, dims, ndim - 1, prec), 0, F(1), st)));
   for (int a = ndim - 2; a >= 0; --a)
-    SSIM_TRY((launch_pass<F, MODE_C2C>(out, out, strided_pass(ndim, cdims, a),
-                                       work + tw_offset(ndim, dims, a, prec), 0,
-                                       F(1), st)));
+    SSIM_TRY((launch_pass<F, MODE_C2C>(
+        out, out, strided_pass(ndim, cdims, a, tile_elems<F>()),
+        work + tw_offset(ndim, dims, a, prec), 0, F(1), st)));
   return SSIM_OK;
 }
 
@@ -405,12 +414,12 @@ int fft_c2r(int ndim, const int* dims, cpx<F>* in, F* out,
   cdims[ndim - 1] = dims[ndim - 1] / 2 + 1;
   for (int a = 0; a < ndim - 1; ++a) rows *= dims[a];
   for (int a = 0; a < ndim - 1; ++a)
-    SSIM_TRY((launch_pass<F, MODE_C2C>(in, in, strided_pass(ndim, cdims, a),
-                                       work + tw_offset(ndim, dims, a, prec), 1,
-                                       F(1), st)));
+    SSIM_TRY((launch_pass<F, MODE_C2C>(
+        in, in, strided_pass(ndim, cdims, a, tile_elems<F>()),
+        work + tw_offset(ndim, dims, a, prec), 1, F(1), st)));
   const int L = dims[ndim - 1];
   SSIM_TRY((launch_pass<F, MODE_C2R>(
-      in, out, row_pass(rows, L, cdims[ndim - 1], L),
+      in, out, row_pass(rows, L, cdims[ndim - 1], L, tile_elems<F>()),
       work + tw_offset(ndim, dims, ndim - 1, prec), 1, F(1.0 / total), st)));
   return SSIM_OK;
 }
@@ -437,29 +446,29 @@ int conv_isolated3(int n, const F* rho, const F* const* kim, F* const* fields,
   const unsigned char* twy = tw + tw_offset(3, dims, 1, prec);
   const unsigned char* twz = tw + tw_offset(3, dims, 2, prec);
   const long long rows = (long long)n * n;
-  Pass fz = row_pass(rows, P, n, H);
+  Pass fz = row_pass(rows, P, n, H, tile_elems<F>());
   fz.R1 = n;
   fz.s_in_hi = rows;
   fz.s_out_hi = (long long)P * H;
   fz.in_count = n;
   SSIM_TRY((launch_pass<F, MODE_R2C>(rho, spec, fz, twz, 0, F(1), st)));
-  Pass fy = strided_pass(3, cd, 1);
+  Pass fy = strided_pass(3, cd, 1, tile_elems<F>());
   fy.n_outer = n;
   fy.in_count = n;
   SSIM_TRY((launch_pass<F, MODE_C2C>(spec, spec, fy, twy, 0, F(1), st)));
-  Pass fx = strided_pass(3, cd, 0);
+  Pass fx = strided_pass(3, cd, 0, tile_elems<F>());
   fx.in_count = n;
   SSIM_TRY((launch_pass<F, MODE_C2C>(spec, spec, fx, twx, 0, F(1), st)));
   const F scale = F(1.0 / ((double)P * P * P));
   for (int c = 0; c < 3; ++c) {
-    Pass ix = strided_pass(3, cd, 0);
+    Pass ix = strided_pass(3, cd, 0, tile_elems<F>());
     ix.out_count = n;
     SSIM_TRY((launch_pass<F, MODE_C2C>(spec, tmp, ix, twx, 1, F(1), st, kim[c])));
-    Pass iy = strided_pass(3, cd, 1);
+    Pass iy = strided_pass(3, cd, 1, tile_elems<F>());
     iy.n_outer = n;
     iy.out_count = n;
     SSIM_TRY((launch_pass<F, MODE_C2C>(tmp, tmp, iy, twy, 1, F(1), st)));
-    Pass iz = row_pass(rows, P, H, n);
+    Pass iz = row_pass(rows, P, H, n, tile_elems<F>());
     iz.R1 = n;
     iz.s_in_hi = (long long)P * H;
     iz.s_out_hi = rows;

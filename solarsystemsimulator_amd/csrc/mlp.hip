@@ -46,9 +46,14 @@ __device__ __forceinline__ float bf16_float(unsigned short b) {
   return __uint_as_float(((unsigned int)b) << 16);
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// sigmoid with the hardware reciprocal (1 ulp) instead of a full-precision
+// division: the BF16 rounding of the activations is 2^-9, far coarser
+__device__ __forceinline__ float sigmoid(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float silu_grad(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -489,12 +494,14 @@ int ssim_mlp_features(int nb, int n_mass, const double* x, const double* y,
   if (nb == 0) return SSIM_OK;
   SSIM_REQUIRE(x && y && z && origin && rho && feat, "null pointer");
   const bool f64 = mesh_prec == SSIM_PREC_F64;
-  hipLaunchKernelGGL(k_mlp_features, dim3(ssim::ceil_div(nb, MB)), dim3(MB), 0,
-                     ssim::as_stream(stream), nb, n_mass, x, y, z, mesh_n, origin[0],
-                     origin[1], origin[2], h,
-                     f64 ? nullptr : static_cast<const float*>(rho),
-                     f64 ? static_cast<const double*>(rho) : nullptr,
-                     1.0 / mass_unit, feat);
+  hipStream_t st = ssim::as_stream(stream);
+  SSIM_TIMED("k_mlp_features", st,
+             hipLaunchKernelGGL(k_mlp_features, dim3(ssim::ceil_div(nb, MB)),
+                                dim3(MB), 0, st, nb, n_mass, x, y, z, mesh_n,
+                                origin[0], origin[1], origin[2], h,
+                                f64 ? nullptr : static_cast<const float*>(rho),
+                                f64 ? static_cast<const double*>(rho) : nullptr,
+                                1.0 / mass_unit, feat));
   SSIM_LAUNCHED("k_mlp_features");
   return SSIM_OK;
 }
@@ -509,10 +516,11 @@ int ssim_mlp_forward(int nb, const unsigned short* feat,
   SSIM_REQUIRE(feat && wbf && params, "null pointer");
   SSIM_REQUIRE(!ax || (ay && az), "need all three acceleration arrays");
   SSIM_REQUIRE((zs == nullptr) == (hs == nullptr), "zs and hs go together");
-  hipLaunchKernelGGL(k_mlp_forward, dim3(grid_waves(nb)), dim3(64 * WAVES), 0,
-                     ssim::as_stream(stream), nb, feat, wbf, params, scale,
-                     relative, out,
-                     ax, ay, az, zs, hs);
+  hipStream_t st = ssim::as_stream(stream);
+  SSIM_TIMED("k_mlp_forward", st,
+             hipLaunchKernelGGL(k_mlp_forward, dim3(grid_waves(nb)),
+                                dim3(64 * WAVES), 0, st, nb, feat, wbf, params,
+                                scale, relative, out, ax, ay, az, zs, hs));
   SSIM_LAUNCHED("k_mlp_forward");
   return SSIM_OK;
 }

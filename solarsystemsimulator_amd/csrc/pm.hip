@@ -165,6 +165,94 @@ __global__ __launch_bounds__(PM_BLOCK) void k_minmax_partial(
     }
 }
 
+// Triangular-shaped-cloud (TSC) assignment on an isolated 3-D mesh, for the
+// split force: a body at f = (x - o)/h (node units) spreads over the 3 nodes
+// c-1, c, c+1 of each axis around its nearest node c = floor(f + 1/2), with
+// d = f - c in [-1/2, 1/2) and weights
+//   w(-1) = (1/2 - d)^2 / 2,  w(0) = 3/4 - d^2,  w(+1) = (1/2 + d)^2 / 2,
+// which sum to 1 and reproduce linear functions exactly (sum w(k) k = d).
+// Its window is sinc^3 per axis against sinc^2 for CIC, so aliasing of the
+// mesh force drops several-fold.  Nodes outside the mesh are skipped (the
+// split force keeps two empty cells around the bodies).
+__device__ __forceinline__ void tsc_axis(double x, double o, double h, int& c,
+                                         double w[3]) {
+  const double f = (x - o) / h;
+  const double r = floor(f + 0.5);
+  const double d = f - r;
+  c = (int)r;
+  const double a = 0.5 - d, b = 0.5 + d;
+  w[0] = 0.5 * a * a;
+  w[1] = 0.75 - d * d;
+  w[2] = 0.5 * b * b;
+}
+
+template <typename M>
+__global__ __launch_bounds__(PM_BLOCK) void k_tsc_deposit(
+    int np, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ m, Grid g,
+    M* __restrict__ mesh) {
+  const int p = blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (p >= np) return;
+  int c[3];
+  double w[3][3];
+  tsc_axis(x[p], g.o[0], g.h, c[0], w[0]);
+  tsc_axis(y[p], g.o[1], g.h, c[1], w[1]);
+  tsc_axis(z[p], g.o[2], g.h, c[2], w[2]);
+  const double mp = m[p];
+  for (int a = 0; a < 3; ++a) {
+    const int i = c[0] + a - 1;
+    if (i < 0 || i >= g.n[0]) continue;
+    for (int b = 0; b < 3; ++b) {
+      const int j = c[1] + b - 1;
+      if (j < 0 || j >= g.n[1]) continue;
+      const double wab = mp * w[0][a] * w[1][b];
+      const long long row = ((long long)i * g.n[1] + j) * g.n[2];
+      for (int e = 0; e < 3; ++e) {
+        const int k = c[2] + e - 1;
+        if (k < 0 || k >= g.n[2]) continue;
+        atomic_add(&mesh[row + k], (M)(wab * w[2][e]));
+      }
+    }
+  }
+}
+
+template <typename M>
+__global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather(
+    int np, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, Grid g, const M* __restrict__ f0,
+    const M* __restrict__ f1, const M* __restrict__ f2, double* __restrict__ o0,
+    double* __restrict__ o1, double* __restrict__ o2) {
+  const int p = blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (p >= np) return;
+  int c[3];
+  double w[3][3];
+  tsc_axis(x[p], g.o[0], g.h, c[0], w[0]);
+  tsc_axis(y[p], g.o[1], g.h, c[1], w[1]);
+  tsc_axis(z[p], g.o[2], g.h, c[2], w[2]);
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const int i = c[0] + a - 1;
+    if (i < 0 || i >= g.n[0]) continue;
+    for (int b = 0; b < 3; ++b) {
+      const int j = c[1] + b - 1;
+      if (j < 0 || j >= g.n[1]) continue;
+      const double wab = w[0][a] * w[1][b];
+      const long long row = ((long long)i * g.n[1] + j) * g.n[2];
+      for (int e = 0; e < 3; ++e) {
+        const int k = c[2] + e - 1;
+        if (k < 0 || k >= g.n[2]) continue;
+        const double wt = wab * w[2][e];
+        v0 += wt * (double)f0[row + k];
+        v1 += wt * (double)f1[row + k];
+        v2 += wt * (double)f2[row + k];
+      }
+    }
+  }
+  o0[p] = v0;
+  o1[p] = v1;
+  o2[p] = v2;
+}
+
 // One workgroup folds the per-block extrema (min and max are exact, so the
 // order of the fold does not matter).
 __global__ __launch_bounds__(PM_BLOCK) void k_minmax_final(
@@ -197,10 +285,10 @@ __global__ __launch_bounds__(PM_BLOCK) void k_minmax_final(
 // Real-space force kernel on a zero-padded (2N)^D mesh: K_c(d) =
 // -G d_c / (|d|^2 + eps2)^(3/2), d = off * h with off = idx < N ? idx :
 // idx - 2N (build_force_kernels, pm_kernel.py:42-59).  Component c of D.
-// With rs > 0 only the long-range part is kept: K_c * (1 - g(|d|)) for
-// |d| < rcut (all |d| if rcut <= 0), the full K_c beyond,
-// g(r) = erfc(r/2rs) + r/(rs sqrt(pi)) exp(-r^2/4rs^2) (the complement
-// K_c g inside rcut is summed directly by the near-field kernel).
+// With rs > 0 only the long-range part is kept: K_c * (1 - gS(|d|)), gS the
+// tapered split function (ssim_split_g; zero beyond rcut, so the mesh has
+// the full K_c there); the complement K_c gS is summed directly by the
+// near-field kernel.
 template <typename M>
 __global__ __launch_bounds__(PM_BLOCK) void k_force_kernel(
     int D, int N, int comp, double h, double eps2, double G, double rs,
@@ -231,12 +319,9 @@ __global__ __launch_bounds__(PM_BLOCK) void k_force_kernel(
   const double inv_r3 = 1.0 / (r2 * sqrt(r2));
   double k = ((-G) * d[comp]) * inv_r3;
   if (rs > 0.0) {
-    const double r = sqrt(rr);
-    const double x = r / (2.0 * rs);
-    const double g = erfc(x) + r / (rs * 1.7724538509055160273) * exp(-x * x);
-    // beyond rcut (the near-field sum stops there) the mesh carries the
-    // whole force, so no part of any pair is dropped
-    if (rcut <= 0.0 || r < rcut) k = k * (1.0 - g);
+    // the near-field sum takes F g S; the mesh keeps the rest, which is the
+    // whole force beyond rcut, so no part of any pair is dropped
+    k = k * (1.0 - ssim_split_g(sqrt(rr), rs, rcut));
     // offset -N pairs no two nodes of the N-mesh; zero there the kernel is
     // exactly odd, so its spectrum is imaginary
     for (int c = 0; c < D; ++c)
@@ -245,10 +330,11 @@ __global__ __launch_bounds__(PM_BLOCK) void k_force_kernel(
   out[t] = (M)k;
 }
 
-// Divide a half spectrum on the padded P^D mesh by the squared CIC window,
-// prod_c sinc^4(pi f_c / P), undoing assignment + interpolation smoothing.
+// Divide a half spectrum on the padded P^D mesh by the squared assignment
+// window, prod_c sinc^(2 order)(pi f_c / P) (order 2: CIC, 3: TSC), undoing
+// assignment + interpolation smoothing.
 template <typename M>
-__global__ __launch_bounds__(PM_BLOCK) void k_deconvolve(int D, int P,
+__global__ __launch_bounds__(PM_BLOCK) void k_deconvolve(int D, int P, int order,
                                                          M* __restrict__ spec) {
   const int W = P / 2 + 1;
   const long long total = (D == 2) ? (long long)P * W : (long long)P * P * W;
@@ -270,7 +356,7 @@ __global__ __launch_bounds__(PM_BLOCK) void k_deconvolve(int D, int P,
     if (f[c] == 0) continue;
     const double x = M_PI * (double)f[c] / (double)P;
     const double s = sin(x) / x;
-    w *= s * s * s * s;
+    for (int q = 0; q < 2 * order; ++q) w *= s;
   }
   spec[2 * t] = (M)(spec[2 * t] / w);
   spec[2 * t + 1] = (M)(spec[2 * t + 1] / w);
@@ -418,8 +504,10 @@ int ssim_cic_deposit(int dim, int periodic, int np, const double* x,
   hipStream_t st = ssim::as_stream(stream);
   const dim3 grid(ssim::ceil_div(np, PM_BLOCK));
 #define LAUNCH_DEP(D, M)                                                     \
-  hipLaunchKernelGGL((k_cic_deposit<D, M>), grid, dim3(PM_BLOCK), 0, st, np, \
-                     x, y, z, m, g, static_cast<M*>(mesh), cells)
+  SSIM_TIMED("k_cic_deposit", st,                                             \
+             hipLaunchKernelGGL((k_cic_deposit<D, M>), grid, dim3(PM_BLOCK), 0, \
+                                st, np, x, y, z, m, g, static_cast<M*>(mesh),    \
+                                cells))
   if (mesh_prec == SSIM_PREC_F64) {
     if (dim == 2) LAUNCH_DEP(2, double); else LAUNCH_DEP(3, double);
   } else {
@@ -445,10 +533,12 @@ int ssim_cic_gather(int dim, int periodic, int np, const double* x,
   hipStream_t st = ssim::as_stream(stream);
   const dim3 grid(ssim::ceil_div(np, PM_BLOCK));
 #define LAUNCH_GAT(D, M)                                                      \
-  hipLaunchKernelGGL((k_cic_gather<D, M>), grid, dim3(PM_BLOCK), 0, st, np, x, \
-                     y, z, g, nfields, static_cast<const M*>(f0),              \
-                     static_cast<const M*>(f1), static_cast<const M*>(f2),     \
-                     out0, out1, out2)
+  SSIM_TIMED("k_cic_gather", st,                                              \
+             hipLaunchKernelGGL((k_cic_gather<D, M>), grid, dim3(PM_BLOCK), 0,  \
+                                st, np, x, y, z, g, nfields,                    \
+                                static_cast<const M*>(f0),                      \
+                                static_cast<const M*>(f1),                      \
+                                static_cast<const M*>(f2), out0, out1, out2))
   if (mesh_prec == SSIM_PREC_F64) {
     if (dim == 2) LAUNCH_GAT(2, double); else LAUNCH_GAT(3, double);
   } else {
@@ -506,9 +596,10 @@ int ssim_pm_force_kernel(int dim, int n, int comp, double h, double eps2,
   return SSIM_OK;
 }
 
-int ssim_pm_deconvolve(int dim, int padded_n, int prec, void* spectrum,
-                       void* stream) {
+int ssim_pm_deconvolve(int dim, int padded_n, int prec, int order,
+                       void* spectrum, void* stream) {
   SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+  SSIM_REQUIRE(order == 2 || order == 3, "order must be 2 (CIC) or 3 (TSC)");
   SSIM_REQUIRE(padded_n >= 2 && spectrum, "bad arguments");
   const long long W = padded_n / 2 + 1;
   const long long total = dim == 2 ? padded_n * W : (long long)padded_n * padded_n * W;
@@ -516,10 +607,10 @@ int ssim_pm_deconvolve(int dim, int padded_n, int prec, void* spectrum,
   const dim3 grid((unsigned)ssim::ceil_div(total, PM_BLOCK));
   if (prec == SSIM_PREC_F64)
     hipLaunchKernelGGL(k_deconvolve<double>, grid, dim3(PM_BLOCK), 0, st, dim,
-                       padded_n, static_cast<double*>(spectrum));
+                       padded_n, order, static_cast<double*>(spectrum));
   else
     hipLaunchKernelGGL(k_deconvolve<float>, grid, dim3(PM_BLOCK), 0, st, dim,
-                       padded_n, static_cast<float*>(spectrum));
+                       padded_n, order, static_cast<float*>(spectrum));
   SSIM_LAUNCHED("k_deconvolve");
   return SSIM_OK;
 }
@@ -540,6 +631,56 @@ int ssim_spectrum_mul(long long ncomplex, int prec, const void* a,
                        static_cast<const float*>(a), static_cast<const float*>(b),
                        static_cast<float*>(out));
   SSIM_LAUNCHED("k_cmul");
+  return SSIM_OK;
+}
+
+int ssim_tsc_deposit(int np, const double* x, const double* y, const double* z,
+                     const double* m, int grid_n, const double* origin, double h,
+                     int mesh_prec, void* mesh, void* stream) {
+  SSIM_REQUIRE(np >= 0 && grid_n >= 1 && h > 0.0, "bad TSC arguments");
+  if (np == 0) return SSIM_OK;
+  SSIM_REQUIRE(x && y && z && m && origin && mesh, "null pointer");
+  const int ns[3] = {grid_n, grid_n, grid_n};
+  const Grid g = make_grid(3, ns, origin, h, 0);
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid(ssim::ceil_div(np, PM_BLOCK));
+  if (mesh_prec == SSIM_PREC_F64)
+    SSIM_TIMED("k_tsc_deposit", st,
+               hipLaunchKernelGGL(k_tsc_deposit<double>, grid, dim3(PM_BLOCK), 0,
+                                  st, np, x, y, z, m, g, static_cast<double*>(mesh)));
+  else
+    SSIM_TIMED("k_tsc_deposit", st,
+               hipLaunchKernelGGL(k_tsc_deposit<float>, grid, dim3(PM_BLOCK), 0,
+                                  st, np, x, y, z, m, g, static_cast<float*>(mesh)));
+  SSIM_LAUNCHED("k_tsc_deposit");
+  return SSIM_OK;
+}
+
+int ssim_tsc_gather(int np, const double* x, const double* y, const double* z,
+                    int grid_n, const double* origin, double h, int mesh_prec,
+                    const void* f0, const void* f1, const void* f2, double* o0,
+                    double* o1, double* o2, void* stream) {
+  SSIM_REQUIRE(np >= 0 && grid_n >= 1 && h > 0.0, "bad TSC arguments");
+  if (np == 0) return SSIM_OK;
+  SSIM_REQUIRE(x && y && z && origin && f0 && f1 && f2 && o0 && o1 && o2,
+               "null pointer");
+  const int ns[3] = {grid_n, grid_n, grid_n};
+  const Grid g = make_grid(3, ns, origin, h, 0);
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid(ssim::ceil_div(np, PM_BLOCK));
+  if (mesh_prec == SSIM_PREC_F64)
+    SSIM_TIMED("k_tsc_gather", st,
+               hipLaunchKernelGGL(k_tsc_gather<double>, grid, dim3(PM_BLOCK), 0, st,
+                                  np, x, y, z, g, static_cast<const double*>(f0),
+                                  static_cast<const double*>(f1),
+                                  static_cast<const double*>(f2), o0, o1, o2));
+  else
+    SSIM_TIMED("k_tsc_gather", st,
+               hipLaunchKernelGGL(k_tsc_gather<float>, grid, dim3(PM_BLOCK), 0, st,
+                                  np, x, y, z, g, static_cast<const float*>(f0),
+                                  static_cast<const float*>(f1),
+                                  static_cast<const float*>(f2), o0, o1, o2));
+  SSIM_LAUNCHED("k_tsc_gather");
   return SSIM_OK;
 }
 

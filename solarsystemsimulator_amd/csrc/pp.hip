@@ -1,9 +1,10 @@
 // Near-field (particle-particle) part of the split particle-mesh force.
 //
 // The Plummer pair force F(d) = G m d / (|d|^2 + eps2)^(3/2) is split with
-// the Gaussian filter of the long-range mesh part,
+// the Gaussian filter of the long-range mesh part, tapered to zero at rcut
+// (ssim_split_g),
 //   g(r) = erfc(r / 2rs) + r / (rs sqrt(pi)) exp(-r^2 / 4rs^2),
-//   F = F g(r)  [near, summed here over r < rcut]  +  F (1 - g(r))  [mesh].
+//   F = F gS(r)  [near, summed here over r < rcut]  +  F (1 - gS(r))  [mesh].
 //
 // Sources and targets are binned on the same grid of cells no smaller than
 // rcut (cells.hip), so the 27 cells around a target's cell hold every source
@@ -16,7 +17,11 @@
 //
 // Terms are FP32 on origin-relative coordinates (error ~1e-7 of each term,
 // far below the method's 1e-3 budget); the per-target sum is ADDED to the
-// FP64 accelerations.
+// FP64 accelerations.  g(r) on [0, rcut] comes from a table of GT + 1
+// values (computed in FP64 per call) in LDS, linearly interpolated in r:
+// the interpolation error h^2 max|g''| / 8 is ~1e-6 of the pair force at
+// 1024 intervals, and a pair then costs one square root, one reciprocal
+// square root and a table read instead of erfc and exp.
 #include "ssim_internal.h"
 
 namespace {
@@ -24,6 +29,17 @@ namespace {
 constexpr int PB = 256;            // 4 waves per workgroup
 constexpr int WAVES = PB / 64;
 constexpr int CHUNK = 64;          // targets per wave
+constexpr int GT = 1024;           // intervals of the g(r) table
+
+// tab[i] = (g(r_i), g(r_{i+1}) - g(r_i)), r_i = i rcut / GT
+__global__ __launch_bounds__(PB) void k_gtab(double rs, double rcut,
+                                             float2* __restrict__ tab) {
+  const int i = blockIdx.x * PB + threadIdx.x;
+  if (i > GT) return;
+  const double r0 = rcut * i / GT, r1 = rcut * (i + 1) / GT;
+  const double g0 = ssim_split_g(r0, rs, rcut);
+  tab[i] = make_float2((float)g0, (float)(ssim_split_g(r1, rs, rcut) - g0));
+}
 
 __global__ __launch_bounds__(PB) void k_chunk_count(int ncells,
                                                     const int* __restrict__ tstart,
@@ -41,15 +57,22 @@ __global__ __launch_bounds__(PB) void k_chunk_map(int ncells,
   for (int k = cstart[c]; k < cstart[c + 1]; ++k) map[k] = c;
 }
 
+// COUNT: instead of forces, count the pairs tested and the pairs inside
+// rcut (excluding a body with itself), for the work accounting of a run.
+template <bool COUNT>
 __global__ __launch_bounds__(PB) void k_pp_cells(
     const double* __restrict__ tx, const double* __restrict__ ty,
     const double* __restrict__ tz, double ox, double oy, double oz, int nc0,
     int nc1, int nc2, const int* __restrict__ cell_start,
     const float4* __restrict__ src, const int* __restrict__ tstart,
     const int* __restrict__ torder, const int* __restrict__ cstart,
-    const int* __restrict__ cmap, int ncells, float eps2, float inv_2rs,
-    float c_lin, float rcut2, double G, double* __restrict__ ax,
-    double* __restrict__ ay, double* __restrict__ az) {
+    const int* __restrict__ cmap, int ncells, float eps2,
+    const float2* __restrict__ gtab, float tab_scale, float rcut2, double G,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    unsigned long long* __restrict__ counts) {
+  __shared__ float2 tab[GT + 1];
+  for (int i = threadIdx.x; i <= GT; i += PB) tab[i] = gtab[i];
+  __syncthreads();  // before any wave leaves
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + (threadIdx.x >> 6));
   if (w >= cstart[ncells]) return;
@@ -65,6 +88,7 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
   const int a = cell / (nc1 * nc2), b = (cell / nc2) % nc1, c = cell % nc2;
   const int c0 = max(c - 1, 0), c1 = min(c + 1, nc2 - 1);
   float sx = 0.f, sy = 0.f, sz = 0.f;
+  unsigned int n_test = 0, n_in = 0;
   for (int da = max(a - 1, 0); da <= min(a + 1, nc0 - 1); ++da)
     for (int db = max(b - 1, 0); db <= min(b + 1, nc1 - 1); ++db) {
       const int row = (da * nc1 + db) * nc2;
@@ -73,16 +97,30 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
         const float4 q = src[s];
         const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        const float r = sqrtf(r2);
-        const float x = r * inv_2rs;
-        const float g = erfcf(x) + c_lin * r * __expf(-x * x);
+        const bool in = r2 < rcut2;
+        if (COUNT) {
+          n_test += 1;
+          n_in += (in && r2 > 0.f) ? 1 : 0;
+          continue;
+        }
+        const float t = sqrtf(r2) * tab_scale;
+        const int k = in ? (int)t : 0;
+        const float2 e = tab[k];
+        const float g = fmaf(t - (float)k, e.y, e.x);
         const float inv = ssim_rsqrt_f32(r2 + eps2);
-        const float f = r2 < rcut2 ? q.w * (inv * inv) * inv * g : 0.f;
+        const float f = in ? q.w * (inv * inv) * inv * g : 0.f;
         sx = fmaf(f, dx, sx);
         sy = fmaf(f, dy, sy);
         sz = fmaf(f, dz, sz);
       }
     }
+  if (COUNT) {
+    if (live) {
+      atomicAdd(&counts[0], (unsigned long long)n_test);
+      atomicAdd(&counts[1], (unsigned long long)n_in);
+    }
+    return;
+  }
   if (live) {
     ax[tgt] += G * (double)sx;
     ay[tgt] += G * (double)sy;
@@ -98,19 +136,28 @@ size_t ssim_pp_work_bytes(int nt, int ncells) {
   const size_t nc = (size_t)(ncells > 0 ? ncells : 1);
   const size_t chunks = (size_t)(nt > 0 ? nt : 1) / CHUNK + nc + 1;
   return ssim::align256(nc * sizeof(int)) + ssim::align256((nc + 1) * sizeof(int)) +
-         ssim::align256(chunks * sizeof(int)) + ssim::scan_work_bytes((int)nc);
+         ssim::align256(chunks * sizeof(int)) + ssim::align256((GT + 1) * sizeof(float2)) +
+         ssim::scan_work_bytes((int)nc);
 }
 
-int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,
-                  const double* origin, double cell, const int* nc,
-                  const int* cell_start, const void* packed,
-                  const int* tcell_start, const int* torder, double G,
-                  double eps2, double rs, double rcut, double* ax, double* ay,
-                  double* az, void* work, size_t work_bytes, void* stream) {
+}  // extern "C"
+
+namespace {
+
+__global__ void k_zero_u64(unsigned long long* c) {
+  if (threadIdx.x < 2) c[threadIdx.x] = 0;
+}
+
+int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
+            const double* origin, double cell, const int* nc,
+            const int* cell_start, const void* packed, const int* tcell_start,
+            const int* torder, double G, double eps2, double rs, double rcut,
+            double* ax, double* ay, double* az, unsigned long long* counts,
+            void* work, size_t work_bytes, void* stream) {
   SSIM_REQUIRE(nt >= 0, "nt must be >= 0");
   if (nt == 0) return SSIM_OK;
   SSIM_REQUIRE(tx && ty && tz && origin && nc && cell_start && packed &&
-                   tcell_start && torder && ax && ay && az && work,
+                   tcell_start && torder && work && (counts || (ax && ay && az)),
                "null pointer");
   SSIM_REQUIRE(nc[0] >= 1 && nc[1] >= 1 && nc[2] >= 1, "bad cell counts");
   const long long ncl = (long long)nc[0] * nc[1] * nc[2];
@@ -130,6 +177,11 @@ int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,
   int* cmap = reinterpret_cast<int*>(w);
   const size_t max_chunks = (size_t)nt / CHUNK + ncells + 1;
   w += ssim::align256(max_chunks * sizeof(int));
+  float2* gtab = reinterpret_cast<float2*>(w);
+  w += ssim::align256((GT + 1) * sizeof(float2));
+  hipLaunchKernelGGL(k_gtab, dim3(ssim::ceil_div(GT + 1, PB)), dim3(PB), 0, st, rs,
+                     rcut, gtab);
+  SSIM_LAUNCHED("k_gtab");
   const dim3 gc(ssim::ceil_div(ncells, PB));
   hipLaunchKernelGGL(k_chunk_count, gc, dim3(PB), 0, st, ncells, tcell_start, nchunk);
   SSIM_LAUNCHED("k_chunk_count");
@@ -137,15 +189,55 @@ int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,
   hipLaunchKernelGGL(k_chunk_map, gc, dim3(PB), 0, st, ncells, cstart, cmap);
   SSIM_LAUNCHED("k_chunk_map");
   // chunks <= nt / 64 + ncells: launch that bound, surplus waves exit
-  const float c_lin = (float)(1.0 / (rs * 1.7724538509055160273));
-  hipLaunchKernelGGL(k_pp_cells, dim3(ssim::ceil_div((long long)max_chunks, WAVES)),
-                     dim3(PB), 0, st, tx, ty, tz, origin[0], origin[1], origin[2],
-                     nc[0], nc[1], nc[2], cell_start,
-                     static_cast<const float4*>(packed), tcell_start, torder,
-                     cstart, cmap, ncells, (float)eps2, (float)(0.5 / rs), c_lin,
-                     (float)(rcut * rcut), G, ax, ay, az);
+  const dim3 grid(ssim::ceil_div((long long)max_chunks, WAVES));
+  if (counts)
+    hipLaunchKernelGGL(k_pp_cells<true>, grid, dim3(PB), 0, st, tx, ty, tz,
+                       origin[0], origin[1], origin[2], nc[0], nc[1], nc[2],
+                       cell_start, static_cast<const float4*>(packed), tcell_start,
+                       torder, cstart, cmap, ncells, (float)eps2, gtab,
+                       (float)(GT / rcut), (float)(rcut * rcut), G, ax, ay, az,
+                       counts);
+  else
+    SSIM_TIMED("k_pp_cells", st,
+               hipLaunchKernelGGL(k_pp_cells<false>, grid, dim3(PB), 0, st, tx, ty,
+                                  tz, origin[0], origin[1], origin[2], nc[0], nc[1],
+                                  nc[2], cell_start,
+                                  static_cast<const float4*>(packed), tcell_start,
+                                  torder, cstart, cmap, ncells, (float)eps2, gtab,
+                                  (float)(GT / rcut), (float)(rcut * rcut), G, ax,
+                                  ay, az, nullptr));
   SSIM_LAUNCHED("k_pp_cells");
   return SSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,
+                  const double* origin, double cell, const int* nc,
+                  const int* cell_start, const void* packed,
+                  const int* tcell_start, const int* torder, double G,
+                  double eps2, double rs, double rcut, double* ax, double* ay,
+                  double* az, void* work, size_t work_bytes, void* stream) {
+  return pp_impl(nt, tx, ty, tz, origin, cell, nc, cell_start, packed,
+                 tcell_start, torder, G, eps2, rs, rcut, ax, ay, az, nullptr,
+                 work, work_bytes, stream);
+}
+
+int ssim_pp_count(int nt, const double* tx, const double* ty, const double* tz,
+                  const double* origin, double cell, const int* nc,
+                  const int* cell_start, const void* packed,
+                  const int* tcell_start, const int* torder, double eps2,
+                  double rs, double rcut, unsigned long long* counts, void* work,
+                  size_t work_bytes, void* stream) {
+  SSIM_REQUIRE(counts, "null pointer");
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, ssim::as_stream(stream),
+                     counts);
+  SSIM_LAUNCHED("k_zero_u64");
+  return pp_impl(nt, tx, ty, tz, origin, cell, nc, cell_start, packed,
+                 tcell_start, torder, 1.0, eps2, rs, rcut, nullptr, nullptr,
+                 nullptr, counts, work, work_bytes, stream);
 }
 
 }  // extern "C"

@@ -45,6 +45,11 @@ double yoshida_w0();
 size_t scan_work_bytes(int n);
 int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st);
 
+// Launch timing (abi.hip): SSIM_TIMED wraps a launch statement.
+bool timing_on();
+void* timing_start(hipStream_t st);
+void timing_stop(const char* name, void* start, hipStream_t st);
+
 // Stable LSD radix sort of (keys, vals) in place on keys in [0, 2^bits).
 size_t radix_sort_work_bytes(int n);
 int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
@@ -82,6 +87,15 @@ int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
     }                             \
   } while (0)
 
+// Enqueue `launch` (a hipLaunchKernelGGL statement) on stream st; while
+// timing is enabled it is bracketed by events recorded under `name`.
+#define SSIM_TIMED(name, st, launch)                                \
+  do {                                                              \
+    void* t0_ = ssim::timing_on() ? ssim::timing_start(st) : nullptr; \
+    launch;                                                         \
+    if (t0_) ssim::timing_stop(name, t0_, st);                      \
+  } while (0)
+
 // Propagate a non-zero status from a nested call.
 #define SSIM_TRY(call)       \
   do {                       \
@@ -113,6 +127,25 @@ __device__ __forceinline__ float ssim_rsqrt_f32(float x) {
   const float t = x * y;
   const float e = fmaf(-t, y, 1.0f);
   return fmaf(y * 0.5f, e, y);
+}
+
+// Split function of the particle-mesh + near-field force, shared by the
+// mesh kernel (pm.hip) and the near-field table (pp.hip):
+//   g(r) = erfc(r/2rs) + r/(rs sqrt(pi)) exp(-r^2/4rs^2)   (Gaussian split)
+// tapered to zero over the last rs before rcut with the C2 step
+// S(u) = 1 - u^3 (10 - 15u + 6u^2), u = (r - (rcut - rs)) / rs, so the
+// near part F g S vanishes smoothly at rcut and the mesh part F (1 - g S)
+// is continuous (a hard cut leaves a jump of F g(rcut) in the mesh kernel,
+// sampled unevenly by the mesh nodes).  rcut <= 0: no taper.
+__host__ __device__ inline double ssim_split_g(double r, double rs, double rcut) {
+  const double x = r / (2.0 * rs);
+  double g = erfc(x) + r / (rs * 1.7724538509055160273) * exp(-x * x);
+  if (rcut > 0.0) {
+    if (r >= rcut) return 0.0;
+    const double u = (r - (rcut - rs)) / rs;
+    if (u > 0.0) g *= fmax(1.0 - u * u * u * (10.0 - 15.0 * u + 6.0 * u * u), 0.0);
+  }
+  return g;
 }
 
 // Knuth TwoSum: s + err == a + b exactly (6 flops, no ordering needed).

@@ -77,14 +77,15 @@ class Erkennungskarte:
         n = gx.numel()
         lib = _lib.load()
         counts = (_lib.ctypes.c_longlong * 3)()
+        # contiguous copies (if any) must outlive the launch: keep them here
+        cols = [a.contiguous() for a in (gx, gy, gvx, gvy)]
         while True:
             wb = lib.ssim_collide_work_bytes(n, self._cap)
             if self._work is None or self._work.numel() < wb:
                 self._work = D.workspace(wb, self._dev)
             oi = torch.empty(self._cap, dtype=torch.int64, device=self._dev)
             oj = torch.empty(self._cap, dtype=torch.int64, device=self._dev)
-            _lib.call("ssim_collide_hits", n, *[_lib.ptr(a.contiguous()) for a in
-                                                (gx, gy, gvx, gvy)],
+            _lib.call("ssim_collide_hits", n, *[_lib.ptr(a) for a in cols],
                       _lib.ptr(self.g_rr), _lib.ptr(self.g_ast),
                       _lib.ptr(self.g_vis), float(self.lo), float(self.hi),
                       float(halo), float(dt_y), float(h), self._cap, self._cap,

@@ -55,7 +55,8 @@ class PMLearnedSystem:
         cfg.update(split)
         self.split = SplitForce(dev, cfg["mesh_n"], eps2, G, cfg["rs_cells"],
                                 cfg["rcut_rs"], margin=cfg.get("margin", 2.0),
-                                prec=cfg["prec"], freeze=cfg.get("freeze", False))
+                                prec=cfg["prec"], freeze=cfg.get("freeze", False),
+                                assign=cfg.get("assign", "cic"))
         self.net = net
         nb = self.n + self.t
         self.x = [torch.zeros(nb, dtype=torch.float64, device=dev) for _ in range(3)]

@@ -29,10 +29,11 @@ def test_cell_list_bit_exact(dev):
     wb = _lib.load().ssim_cells_work_bytes(20000, ncells)
     w = D.workspace(wb, dev)
     x = [D.to_dev(c, dev) for c in pos]
+    md = D.to_dev(m, dev)
     gn, org = _lib.int_array(nc), _lib.double_array(origin)
     import ctypes
     _lib.call("ssim_cells_build", 20000, *[_lib.ptr(c) for c in x],
-              _lib.ptr(D.to_dev(m, dev)), ctypes.addressof(org), cell,
+              _lib.ptr(md), ctypes.addressof(org), cell,
               ctypes.addressof(gn), _lib.ptr(start), _lib.ptr(order),
               _lib.ptr(packed), _lib.ptr(w), wb, D.stream(dev))
     rs, ro = p3m_np.cell_list(pos, origin, cell, nc)
@@ -68,6 +69,84 @@ def test_cell_list_large_and_clumped(dev, n, nc, clump):
     rs, ro = p3m_np.cell_list(pos, origin, cell, nc)
     assert np.array_equal(start.cpu().numpy(), rs)
     assert np.array_equal(order.cpu().numpy(), ro)
+
+
+def test_cell_bin_same_cells_any_order(dev):
+    """ssim_cells_bin: same cell_start as the stable list and the same set
+    of bodies in every cell (the order inside a cell is free)."""
+    n, nc = 200_000, [23, 31, 17]
+    rng = np.random.default_rng(8)
+    pos = [rng.normal(0.5, 0.2, n) for _ in range(3)]
+    origin, cell = [0.0, 0.0, 0.0], 1.0 / 23
+    ncells = int(np.prod(nc))
+    start = torch.empty(ncells + 1, dtype=torch.int32, device=dev)
+    order = torch.empty(n, dtype=torch.int32, device=dev)
+    w = D.workspace(_lib.load().ssim_cells_bin_work_bytes(n, ncells), dev)
+    import ctypes
+    gn, org = _lib.int_array(nc), _lib.double_array(origin)
+    x = [D.to_dev(c, dev) for c in pos]
+    _lib.call("ssim_cells_bin", n, *[_lib.ptr(c) for c in x],
+              ctypes.addressof(org), cell, ctypes.addressof(gn), _lib.ptr(start),
+              _lib.ptr(order), _lib.ptr(w), w.numel(), D.stream(dev))
+    rs, ro = p3m_np.cell_list(pos, origin, cell, nc)
+    got_s, got_o = start.cpu().numpy(), order.cpu().numpy()
+    assert np.array_equal(got_s, rs)
+    # sorting each cell's segment by body index gives the stable list
+    keys = np.repeat(np.arange(ncells), np.diff(rs))
+    assert np.array_equal(got_o[np.lexsort((got_o, keys))], ro)
+
+
+def test_tsc_deposit_and_gather_match_oracle(dev):
+    print("tsc deposit/gather", flush=True)
+    n, h, origin = 24, 0.05, [0.1, -0.2, 0.3]
+    rng = np.random.default_rng(6)
+    pos = [origin[a] + rng.uniform(1.6, n - 2.6, 30000) * h for a in range(3)]
+    m = rng.uniform(0.5, 2.0, 30000)
+    x = [D.to_dev(c, dev) for c in pos]
+    mesh = torch.zeros((n, n, n), dtype=torch.float64, device=dev)
+    import ctypes
+    org = _lib.double_array(origin)
+    md = D.to_dev(m, dev)
+    _lib.call("ssim_tsc_deposit", 30000, *[_lib.ptr(c) for c in x],
+              _lib.ptr(md), n, ctypes.addressof(org), h, 0,
+              _lib.ptr(mesh), D.stream(dev))
+    ref = p3m_np.tsc_deposit(pos, m, n, origin, h)
+    np.testing.assert_allclose(mesh.cpu().numpy(), ref, rtol=1e-12, atol=1e-12 * ref.max())
+    fields = [rng.normal(size=(n, n, n)) for _ in range(3)]
+    fd = [D.to_dev(f, dev) for f in fields]        # keep the device copies alive
+    out = [torch.empty(30000, dtype=torch.float64, device=dev) for _ in range(3)]
+    _lib.call("ssim_tsc_gather", 30000, *[_lib.ptr(c) for c in x], n,
+              ctypes.addressof(org), h, 0, *[_lib.ptr(f) for f in fd],
+              *[_lib.ptr(o) for o in out], D.stream(dev))
+    for c in range(3):
+        np.testing.assert_allclose(out[c].cpu().numpy(),
+                                   p3m_np.tsc_gather(fields[c], pos, n, origin, h),
+                                   rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("assign", ["cic", "tsc"])
+def test_split_force_assignments_on_cube(dev, assign):
+    """Median force error of the split force on a 64k-mass cube against the
+    FP64 direct sum: CIC < 1e-2, TSC below CIC's."""
+    print(f"split force {assign}", flush=True)
+    pos, m, tpos = scenes.learned_scene((40, 40, 40), (40, 40, 40), seed=5,
+                                        tracer_seed=55)
+    n = len(m)
+    allx = [np.concatenate([pos[c], tpos[c]]) for c in range(3)]
+    x = [D.to_dev(c, dev) for c in allx]
+    md = D.to_dev(m, dev)
+    idx = np.random.default_rng(0).choice(len(allx[0]), 2000, replace=False)
+    ref = np.array(oracle.allpairs(pos, m, [c[idx] for c in allx], 1.0, 1e-6, plain=True))
+    errs = {}
+    for a_ in ("cic", assign):
+        sf = SplitForce(dev, mesh_n=64, eps2=1e-6, rs_cells=1.0, rcut_rs=5.0, assign=a_)
+        a = [torch.zeros(len(allx[0]), dtype=torch.float64, device=dev) for _ in range(3)]
+        sf.accelerations(x, n, md, a)
+        got = np.array([c.cpu().numpy()[idx] for c in a])
+        errs[a_] = np.median(relative_errors(got, ref))
+    assert errs["cic"] < 1e-2
+    if assign == "tsc":
+        assert errs["tsc"] < 0.5 * errs["cic"], errs
 
 
 @pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 1e-5)])
