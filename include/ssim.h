@@ -247,6 +247,15 @@ int ssim_pm_deconvolve(int dim, int padded_n, int prec, int order,
 int ssim_tsc_deposit(int np, const double* x, const double* y, const double* z,
                      const double* m, int grid_n, const double* origin, double h,
                      int mesh_prec, void* mesh, void* stream);
+/* The TSC deposit without atomics (OVERWRITES mesh): the masses are binned
+ * by nearest node (ssim_cells_build with origin - h/2, cell h and
+ * grid_n^3 cells: bin_start, bin_order) and every node sums the bodies of
+ * the 27 bins around it in a fixed order, so the mesh is bit-reproducible.
+ * Same weights as ssim_tsc_deposit. */
+int ssim_tsc_deposit_binned(const double* x, const double* y, const double* z,
+                            const double* m, int grid_n, const double* origin,
+                            double h, const int* bin_start, const int* bin_order,
+                            int mesh_prec, void* mesh, void* stream);
 /* TSC interpolation of three meshes at np points: o_c = sum w f_c
  * (overwrites o_c, FP64). */
 int ssim_tsc_gather(int np, const double* x, const double* y, const double* z,
@@ -423,14 +432,17 @@ int ssim_mlp_features(int nb, int n_mass, const double* x, const double* y,
                       int mesh_prec, const void* rho, double mass_unit,
                       unsigned short* feat, void* stream);
 /* Forward pass.  out (optional) receives FP32 [nb][3]; if ax != NULL,
- * u * out is ADDED to (ax, ay, az) with u = scale, or, if relative != 0,
- * u = scale * |a| of the acceleration being corrected.  For training pass
+ * d = u * out is ADDED to (ax, ay, az) with u = scale, or, if relative != 0,
+ * u = scale * |a| of the acceleration being corrected; for the first n_proj
+ * bodies (the masses) the component of d along the velocity (vx, vy, vz) is
+ * removed first, so the correction does no work on them.  For training pass
  * zs (FP32 [3][nb][64] pre-activations) and hs (BF16 [3][nb][64]
  * activations). */
 int ssim_mlp_forward(int nb, const unsigned short* feat,
                      const unsigned short* wbf, const float* params,
                      float scale, int relative, float* out, double* ax,
-                     double* ay, double* az, float* zs, unsigned short* hs,
+                     double* ay, double* az, const double* vx, const double* vy,
+                     const double* vz, int n_proj, float* zs, unsigned short* hs,
                      void* stream);
 /* Training target: out[i][c] = (ref_c[i] - mod_c[i]) / u (FP32), the
  * residual of the model force against the exact direct sum, with u = a_unit

@@ -75,6 +75,8 @@ SIGNATURES = {
     "ssim_pm_deconvolve": (c_int, [c_int, c_int, c_int, c_int, _P, _P]),
     "ssim_tsc_deposit": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double,
                                  c_int, _P, _P]),
+    "ssim_tsc_deposit_binned": (c_int, [_P, _P, _P, _P, c_int, _P, c_double, _P,
+                                        _P, c_int, _P, _P]),
     "ssim_tsc_gather": (c_int, [c_int, _P, _P, _P, c_int, _P, c_double, c_int,
                                 _P, _P, _P, _P, _P, _P, _P]),
     "ssim_spectrum_mul": (c_int, [c_longlong, c_int, _P, _P, _P, _P]),
@@ -115,7 +117,7 @@ SIGNATURES = {
     "ssim_mlp_features": (c_int, [c_int, c_int, _P, _P, _P, c_int, _P, c_double,
                                   c_int, _P, c_double, _P, _P]),
     "ssim_mlp_forward": (c_int, [c_int, _P, _P, _P, c_float, c_int, _P, _P, _P,
-                                 _P, _P, _P, _P]),
+                                 _P, _P, _P, _P, c_int, _P, _P, _P]),
     "ssim_mlp_residual": (c_int, [c_int, _P, _P, _P, _P, _P, _P, c_double, c_int,
                                   c_double, _P, _P]),
     "ssim_mlp_mse_grad": (c_int, [c_int, _P, _P, _P, _P, _P]),

@@ -25,7 +25,7 @@
 namespace {
 
 constexpr int FFT_THREADS = 256;
-constexpr int FFT_TILE_BYTES = 65536;  // LDS tile of one workgroup
+constexpr int FFT_TILE_BYTES = 32768;  // LDS tile of one workgroup
 constexpr int FFT_MAXB = 64;           // lines per tile along a strided axis
 constexpr int FFT_MAX_L = 4096;
 
@@ -36,12 +36,19 @@ struct cpx {
   F re, im;
 };
 
-// complex values per tile: 8192 in FP32, 4096 in FP64 (64 KB of LDS, two
-// workgroups per CU); a strided pass then moves rows of 32 (FP32) or 16
-// (FP64) consecutive lines, 256 contiguous bytes
+// complex values per tile: 4096 in FP32, 2048 in FP64 (32 KB of LDS plus
+// the row padding); a strided pass of 256-point lines then moves rows of 16
+// (FP32) or 8 (FP64) consecutive lines, 128 contiguous bytes
 template <typename F>
 constexpr int tile_elems() {
   return FFT_TILE_BYTES / (int)sizeof(cpx<F>);
+}
+
+// the largest tile a workgroup may hold: a single line can be longer than
+// the tile (up to FFT_MAX_L points, 64 KB in FP64)
+template <typename F>
+constexpr int max_elems() {
+  return tile_elems<F>() > FFT_MAX_L ? tile_elems<F>() : FFT_MAX_L;
 }
 
 template <typename F>
@@ -102,6 +109,9 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
   const int L = P.L;
   const int nl = P.lines_per_tile;
   const int E = nl * L;
+  // lines sit L + 1 apart in LDS: the strided loads and stores, where
+  // consecutive lanes take consecutive lines, then fall in different banks
+  const int LS = nl > 1 ? L + 1 : L;
 
   // which lines does this workgroup own?
   long long o0, b0;
@@ -117,7 +127,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     nvalid = (int)min((long long)nl, (long long)P.n_inner - b0);
   }
 
-  // ---- load the tile: s[l * L + e] ----
+  // ---- load the tile: s[l * LS + e] ----
   if (MODE == MODE_R2C) {
     const F* in = static_cast<const F*>(vin);
     for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
@@ -125,7 +135,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
       F v = 0;
       if (l < nvalid && e < P.in_count)
         v = in[row_off(o0 + l, P.R1, P.s_in_hi, P.s_outer_in) + e];
-      s[t] = {v, F(0)};
+      s[l * LS + e] = {v, F(0)};
     }
   } else if (MODE == MODE_C2R) {
     // half spectrum of L/2+1 values -> Hermitian line of L values
@@ -143,14 +153,14 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
           v = {c.re, -c.im};
         }
       }
-      s[t] = v;
+      s[l * LS + e] = v;
     }
   } else {
     const cpx<F>* in = static_cast<const cpx<F>*>(vin);
     if (P.contig) {
       for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
         const int l = t / L, e = t % L;
-        s[t] = (l < nvalid && e < P.in_count)
+        s[l * LS + e] = (l < nvalid && e < P.in_count)
                    ? in[row_off(o0 + l, P.R1, P.s_in_hi, P.s_outer_in) + e]
                    : cpx<F>{0, 0};
       }
@@ -167,7 +177,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
             v = {-v.im * m, v.re * m};
           }
         }
-        s[l * L + e] = v;
+        s[l * LS + e] = v;
       }
     }
   }
@@ -176,14 +186,14 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
   // ---- Stockham stages over all lines of the tile ----
   int p = 1;
   if (P.log2L & 1) {  // one radix-2 stage (p == 1: k == 0, no twiddle)
-    constexpr int NB = tile_elems<F>() / 2 / FFT_THREADS;
+    constexpr int NB = max_elems<F>() / 2 / FFT_THREADS;
     const int T = L / 2;
     cpx<F> r0[NB], r1[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) {
       const int u = threadIdx.x + nb * FFT_THREADS;
       if (u < E / 2) {
-        const cpx<F>* a = s + (u / T) * L;
+        const cpx<F>* a = s + (u / T) * LS;
         const int i = u % T;
         r0[nb] = a[i];
         r1[nb] = a[i + T];
@@ -194,7 +204,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     for (int nb = 0; nb < NB; ++nb) {
       const int u = threadIdx.x + nb * FFT_THREADS;
       if (u < E / 2) {
-        cpx<F>* a = s + (u / T) * L;
+        cpx<F>* a = s + (u / T) * LS;
         const int i = u % T;
         a[2 * i] = cadd(r0[nb], r1[nb]);
         a[2 * i + 1] = csub(r0[nb], r1[nb]);
@@ -204,7 +214,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     p = 2;
   }
   for (; p < L; p *= 4) {
-    constexpr int NB = tile_elems<F>() / 4 / FFT_THREADS;
+    constexpr int NB = max_elems<F>() / 4 / FFT_THREADS;
     const int T = L / 4;
     const int twstep = L / (4 * p);
     cpx<F> r[NB][4];
@@ -214,7 +224,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
       if (u < E / 4) {
         const int i = u % T;
         const int k = i & (p - 1);
-        const cpx<F>* a = s + (u / T) * L;
+        const cpx<F>* a = s + (u / T) * LS;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           cpx<F> x = a[i + q * T];
@@ -234,7 +244,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
       if (u < E / 4) {
         const int i = u % T;
         const int k = i & (p - 1);
-        cpx<F>* a = s + (u / T) * L;
+        cpx<F>* a = s + (u / T) * LS;
         const cpx<F> s02 = cadd(r[nb][0], r[nb][2]);
         const cpx<F> d02 = csub(r[nb][0], r[nb][2]);
         const cpx<F> s13 = cadd(r[nb][1], r[nb][3]);
@@ -256,7 +266,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     for (int t = threadIdx.x; t < nl * H; t += FFT_THREADS) {
       const int l = t / H, e = t % H;
       if (l < nvalid) {
-        cpx<F> v = s[l * L + e];
+        cpx<F> v = s[l * LS + e];
         out[row_off(o0 + l, P.R1, P.s_out_hi, P.s_outer_out) + e] = {v.re * scale,
                                                                      v.im * scale};
       }
@@ -266,7 +276,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
       const int l = t / L, e = t % L;
       if (l < nvalid && e < P.out_count)
-        out[row_off(o0 + l, P.R1, P.s_out_hi, P.s_outer_out) + e] = s[t].re * scale;
+        out[row_off(o0 + l, P.R1, P.s_out_hi, P.s_outer_out) + e] = s[l * LS + e].re * scale;
     }
   } else {
     cpx<F>* out = static_cast<cpx<F>*>(vout);
@@ -274,7 +284,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
       for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
         const int l = t / L, e = t % L;
         if (l < nvalid && e < P.out_count) {
-          const cpx<F> v = s[t];
+          const cpx<F> v = s[l * LS + e];
           out[row_off(o0 + l, P.R1, P.s_out_hi, P.s_outer_out) + e] = {v.re * scale,
                                                                        v.im * scale};
         }
@@ -283,7 +293,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
       for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
         const int l = t % nl, e = t / nl;
         if (l < nvalid && e < P.out_count) {
-          const cpx<F> v = s[l * L + e];
+          const cpx<F> v = s[l * LS + e];
           out[o0 * P.s_outer_out + (b0 + l) + e * P.s_elem] = {v.re * scale,
                                                                v.im * scale};
         }
@@ -328,7 +338,8 @@ int launch_pass(const void* in, void* out, Pass P, const void* tw, int inverse,
     nwg = ssim::ceil_div(P.n_outer, P.lines_per_tile);
   else
     nwg = P.n_outer * ssim::ceil_div(P.n_inner, P.lines_per_tile);
-  const size_t lds = (size_t)P.lines_per_tile * P.L * sizeof(cpx<F>);
+  const size_t lds = (size_t)P.lines_per_tile *
+                    (P.lines_per_tile > 1 ? P.L + 1 : P.L) * sizeof(cpx<F>);
   SSIM_TIMED("k_fft_pass", st,
              hipLaunchKernelGGL((k_fft_pass<F, MODE>), dim3(nwg), dim3(FFT_THREADS),
                                 lds, st, in, out, P, static_cast<const cpx<F>*>(tw),

@@ -120,6 +120,7 @@ __device__ __forceinline__ s4_t activate(f4_t acc, const float* __restrict__ bia
 // [nb][32].  Writes the 3 outputs (unscaled) and, when zs/hs != nullptr,
 // the layer activations for training: zs[l][body][64] FP32, hs[l][body][64]
 // BF16, l = 0..2.
+// `bias` holds b1 | b2 | b3 (64 each) | b4 (16) contiguously (LDS).
 __device__ void forward16(const Weights& W, const float* __restrict__ bias,
                           const unsigned short* __restrict__ x, int nb, int b0,
                           int lane, float out3[3], bool& valid, float* zs,
@@ -129,15 +130,10 @@ __device__ void forward16(const Weights& W, const float* __restrict__ bias,
   s4_t xb[2];
 #pragma unroll
   for (int kb = 0; kb < 2; ++kb) {
-    if (valid) {
-      const unsigned short* p = x + (size_t)body * NIN + 16 * kb + 4 * (lane >> 4);
-      xb[kb].x = (short)p[0];
-      xb[kb].y = (short)p[1];
-      xb[kb].z = (short)p[2];
-      xb[kb].w = (short)p[3];
-    } else {
-      xb[kb] = s4_t{0, 0, 0, 0};
-    }
+    // 4 consecutive BF16 features: one 8-byte load
+    xb[kb] = valid ? *reinterpret_cast<const s4_t*>(x + (size_t)body * NIN + 16 * kb +
+                                                    4 * (lane >> 4))
+                   : s4_t{0, 0, 0, 0};
   }
   const size_t zoff = (size_t)body * NH;
   const size_t lstride = (size_t)nb * NH;
@@ -147,7 +143,7 @@ __device__ void forward16(const Weights& W, const float* __restrict__ bias,
     f4_t acc = {0.f, 0.f, 0.f, 0.f};
     acc = mfma(W.w1[rt][0], xb[0], acc);
     acc = mfma(W.w1[rt][1], xb[1], acc);
-    h1[rt] = activate(acc, bias + OFF_B1, rt, lane,
+    h1[rt] = activate(acc, bias, rt, lane,
                       (zs && valid) ? zs + zoff : nullptr,
                       (hs && valid) ? hs + zoff : nullptr);
   }
@@ -156,7 +152,7 @@ __device__ void forward16(const Weights& W, const float* __restrict__ bias,
     f4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) acc = mfma(W.w2[rt][kb], h1[kb], acc);
-    h2[rt] = activate(acc, bias + OFF_B2, rt, lane,
+    h2[rt] = activate(acc, bias + NH, rt, lane,
                       (zs && valid) ? zs + lstride + zoff : nullptr,
                       (hs && valid) ? hs + lstride + zoff : nullptr);
   }
@@ -165,7 +161,7 @@ __device__ void forward16(const Weights& W, const float* __restrict__ bias,
     f4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) acc = mfma(W.w3[rt][kb], h2[kb], acc);
-    h3[rt] = activate(acc, bias + OFF_B3, rt, lane,
+    h3[rt] = activate(acc, bias + 2 * NH, rt, lane,
                       (zs && valid) ? zs + 2 * lstride + zoff : nullptr,
                       (hs && valid) ? hs + 2 * lstride + zoff : nullptr);
   }
@@ -174,15 +170,25 @@ __device__ void forward16(const Weights& W, const float* __restrict__ bias,
   for (int kb = 0; kb < 4; ++kb) acc = mfma(W.w4[kb], h3[kb], acc);
   // rows 0..2 live in lanes 0..15 (4(l/16) + i with l/16 == 0)
 #pragma unroll
-  for (int i = 0; i < 3; ++i) out3[i] = acc[i] + bias[OFF_B4 + i];
+  for (int i = 0; i < 3; ++i) out3[i] = acc[i] + bias[3 * NH + i];
 }
 
 __global__ __launch_bounds__(64 * WAVES) void k_mlp_forward(
     int nb, const unsigned short* __restrict__ x,
     const unsigned short* __restrict__ wbf, const float* __restrict__ params,
     float scale, int relative, float* __restrict__ out, double* __restrict__ a0,
-    double* __restrict__ a1, double* __restrict__ a2, float* __restrict__ zs,
-    unsigned short* __restrict__ hs) {
+    double* __restrict__ a1, double* __restrict__ a2, const double* __restrict__ v0,
+    const double* __restrict__ v1, const double* __restrict__ v2, int n_proj,
+    float* __restrict__ zs, unsigned short* __restrict__ hs) {
+  __shared__ float sbias[3 * NH + NOUT];
+  for (int i = threadIdx.x; i < 3 * NH + NOUT; i += 64 * WAVES) {
+    const int off = i < NH ? OFF_B1 + i
+                  : i < 2 * NH ? OFF_B2 + i - NH
+                  : i < 3 * NH ? OFF_B3 + i - 2 * NH
+                  : OFF_B4 + i - 3 * NH;
+    sbias[i] = params[off];
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * WAVES + (threadIdx.x >> 6);
   const int nwaves = gridDim.x * WAVES;
@@ -191,7 +197,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_forward(
   for (int b0 = wave * 16; b0 < nb; b0 += nwaves * 16) {
     float o[3];
     bool valid;
-    forward16(W, params, x, nb, b0, lane, o, valid, zs, hs);
+    forward16(W, sbias, x, nb, b0, lane, o, valid, zs, hs);
     if (lane < 16 && valid) {
       const int body = b0 + lane;
       if (out) {
@@ -206,9 +212,21 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_forward(
                                                          a1[body] * a1[body] +
                                                          a2[body] * a2[body])
                                   : (double)scale;
-        a0[body] += u * (double)o[0];
-        a1[body] += u * (double)o[1];
-        a2[body] += u * (double)o[2];
+        double d0 = u * (double)o[0], d1 = u * (double)o[1], d2 = u * (double)o[2];
+        if (body < n_proj) {
+          // no component along the velocity: the correction does no work
+          const double w0 = v0[body], w1 = v1[body], w2 = v2[body];
+          const double vv = w0 * w0 + w1 * w1 + w2 * w2;
+          if (vv > 0.0) {
+            const double c = (d0 * w0 + d1 * w1 + d2 * w2) / vv;
+            d0 -= c * w0;
+            d1 -= c * w1;
+            d2 -= c * w2;
+          }
+        }
+        a0[body] += d0;
+        a1[body] += d1;
+        a2[body] += d2;
       }
     }
   }
@@ -509,18 +527,23 @@ int ssim_mlp_features(int nb, int n_mass, const double* x, const double* y,
 int ssim_mlp_forward(int nb, const unsigned short* feat,
                      const unsigned short* wbf, const float* params,
                      float scale, int relative, float* out, double* ax,
-                     double* ay, double* az, float* zs, unsigned short* hs,
+                     double* ay, double* az, const double* vx, const double* vy,
+                     const double* vz, int n_proj, float* zs, unsigned short* hs,
                      void* stream) {
   SSIM_REQUIRE(nb >= 0, "nb must be >= 0");
   if (nb == 0) return SSIM_OK;
   SSIM_REQUIRE(feat && wbf && params, "null pointer");
   SSIM_REQUIRE(!ax || (ay && az), "need all three acceleration arrays");
   SSIM_REQUIRE((zs == nullptr) == (hs == nullptr), "zs and hs go together");
+  SSIM_REQUIRE(n_proj >= 0 && n_proj <= nb, "n_proj must be in [0, nb]");
+  SSIM_REQUIRE(n_proj == 0 || (ax && vx && vy && vz),
+               "the projection needs the accelerations and velocities");
   hipStream_t st = ssim::as_stream(stream);
   SSIM_TIMED("k_mlp_forward", st,
              hipLaunchKernelGGL(k_mlp_forward, dim3(grid_waves(nb)),
                                 dim3(64 * WAVES), 0, st, nb, feat, wbf, params,
-                                scale, relative, out, ax, ay, az, zs, hs));
+                                scale, relative, out, ax, ay, az, vx, vy, vz,
+                                n_proj, zs, hs));
   SSIM_LAUNCHED("k_mlp_forward");
   return SSIM_OK;
 }

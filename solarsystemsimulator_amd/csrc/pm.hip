@@ -186,6 +186,26 @@ __device__ __forceinline__ void tsc_axis(double x, double o, double h, int& c,
   w[2] = 0.5 * b * b;
 }
 
+// TSC weight of the body at x on node `node` of its axis (0 beyond the
+// 3-node stencil); the same arithmetic as tsc_axis.
+__device__ __forceinline__ double tsc_node_weight(double x, double o, double h,
+                                                  int node) {
+  const double f = (x - o) / h;
+  const double r = floor(f + 0.5);
+  const double d = f - r;
+  const int off = node - (int)r;
+  if (off == 0) return 0.75 - d * d;
+  if (off == -1) {
+    const double a = 0.5 - d;
+    return 0.5 * a * a;
+  }
+  if (off == 1) {
+    const double b = 0.5 + d;
+    return 0.5 * b * b;
+  }
+  return 0.0;
+}
+
 template <typename M>
 __global__ __launch_bounds__(PM_BLOCK) void k_tsc_deposit(
     int np, const double* __restrict__ x, const double* __restrict__ y,
@@ -214,6 +234,45 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_deposit(
       }
     }
   }
+}
+
+// The same deposit without atomics: one thread per node sums, in a fixed
+// order, the bodies binned at the 27 nodes around it (bins = nearest node,
+// stable order inside a bin), so the mesh is bit-reproducible.  A body binned
+// one node off by rounding at a bin edge has weight ~0 on the nodes it is
+// then missed by (the TSC weights vanish at |d| = 1/2 on the far side).
+template <typename M>
+__global__ __launch_bounds__(PM_BLOCK) void k_tsc_deposit_binned(
+    const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ m, Grid g,
+    const int* __restrict__ bin_start, const int* __restrict__ bin_order,
+    M* __restrict__ mesh) {
+  const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+  const long long node = (long long)blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (node >= (long long)n0 * n1 * n2) return;
+  const int k = (int)(node % n2), j = (int)((node / n2) % n1),
+            i = (int)(node / ((long long)n1 * n2));
+  double acc = 0.0;
+  for (int a = -1; a <= 1; ++a) {
+    const int bi = i + a;
+    if (bi < 0 || bi >= n0) continue;
+    for (int b = -1; b <= 1; ++b) {
+      const int bj = j + b;
+      if (bj < 0 || bj >= n1) continue;
+      const long long row = ((long long)bi * n1 + bj) * n2;
+      const int k0 = max(k - 1, 0), k1 = min(k + 1, n2 - 1);
+      // bins k0..k1 of one row are contiguous in the sorted order
+      const int s0 = bin_start[row + k0], s1 = bin_start[row + k1 + 1];
+      for (int s = s0; s < s1; ++s) {
+        const int p = bin_order[s];
+        const double wx = tsc_node_weight(x[p], g.o[0], g.h, i);
+        const double wy = tsc_node_weight(y[p], g.o[1], g.h, j);
+        const double wz = tsc_node_weight(z[p], g.o[2], g.h, k);
+        acc += m[p] * wx * wy * wz;
+      }
+    }
+  }
+  mesh[node] = (M)acc;
 }
 
 template <typename M>
@@ -653,6 +712,32 @@ int ssim_tsc_deposit(int np, const double* x, const double* y, const double* z,
                hipLaunchKernelGGL(k_tsc_deposit<float>, grid, dim3(PM_BLOCK), 0,
                                   st, np, x, y, z, m, g, static_cast<float*>(mesh)));
   SSIM_LAUNCHED("k_tsc_deposit");
+  return SSIM_OK;
+}
+
+int ssim_tsc_deposit_binned(const double* x, const double* y, const double* z,
+                            const double* m, int grid_n, const double* origin,
+                            double h, const int* bin_start, const int* bin_order,
+                            int mesh_prec, void* mesh, void* stream) {
+  SSIM_REQUIRE(grid_n >= 1 && h > 0.0, "bad TSC arguments");
+  SSIM_REQUIRE(x && y && z && m && origin && bin_start && bin_order && mesh,
+               "null pointer");
+  const int ns[3] = {grid_n, grid_n, grid_n};
+  const Grid g = make_grid(3, ns, origin, h, 0);
+  hipStream_t st = ssim::as_stream(stream);
+  const long long nodes = (long long)grid_n * grid_n * grid_n;
+  const dim3 grid((unsigned)ssim::ceil_div(nodes, PM_BLOCK));
+  if (mesh_prec == SSIM_PREC_F64)
+    SSIM_TIMED("k_tsc_deposit", st,
+               hipLaunchKernelGGL(k_tsc_deposit_binned<double>, grid, dim3(PM_BLOCK),
+                                  0, st, x, y, z, m, g, bin_start, bin_order,
+                                  static_cast<double*>(mesh)));
+  else
+    SSIM_TIMED("k_tsc_deposit", st,
+               hipLaunchKernelGGL(k_tsc_deposit_binned<float>, grid, dim3(PM_BLOCK),
+                                  0, st, x, y, z, m, g, bin_start, bin_order,
+                                  static_cast<float*>(mesh)));
+  SSIM_LAUNCHED("k_tsc_deposit_binned");
   return SSIM_OK;
 }
 

@@ -17,11 +17,13 @@
 //
 // Terms are FP32 on origin-relative coordinates (error ~1e-7 of each term,
 // far below the method's 1e-3 budget); the per-target sum is ADDED to the
-// FP64 accelerations.  g(r) on [0, rcut] comes from a table of GT + 1
-// values (computed in FP64 per call) in LDS, linearly interpolated in r:
-// the interpolation error h^2 max|g''| / 8 is ~1e-6 of the pair force at
-// 1024 intervals, and a pair then costs one square root, one reciprocal
-// square root and a table read instead of erfc and exp.
+// FP64 accelerations.  gS(r) comes from a table over u = r^2 in [0, rcut^2]
+// of GT + 1 values (computed in FP64 per call) in LDS, linearly
+// interpolated in u: at 4096 intervals the error is below 7e-6 of the pair
+// force (largest at r ~ rs/20, where g - 1 ~ r^3 is least smooth in u) and
+// ~1e-7 elsewhere, and a tested pair then costs one table read and one
+// hardware reciprocal square root (v_rsq_f32, 1 ulp) instead of erfc, exp
+// and two square roots.
 #include "ssim_internal.h"
 
 namespace {
@@ -29,14 +31,14 @@ namespace {
 constexpr int PB = 256;            // 4 waves per workgroup
 constexpr int WAVES = PB / 64;
 constexpr int CHUNK = 64;          // targets per wave
-constexpr int GT = 1024;           // intervals of the g(r) table
+constexpr int GT = 4096;           // intervals of the gS(sqrt(u)) table
 
-// tab[i] = (g(r_i), g(r_{i+1}) - g(r_i)), r_i = i rcut / GT
+// tab[i] = (gS(r_i), gS(r_{i+1}) - gS(r_i)), r_i = rcut sqrt(i / GT)
 __global__ __launch_bounds__(PB) void k_gtab(double rs, double rcut,
                                              float2* __restrict__ tab) {
   const int i = blockIdx.x * PB + threadIdx.x;
   if (i > GT) return;
-  const double r0 = rcut * i / GT, r1 = rcut * (i + 1) / GT;
+  const double r0 = rcut * sqrt((double)i / GT), r1 = rcut * sqrt((double)(i + 1) / GT);
   const double g0 = ssim_split_g(r0, rs, rcut);
   tab[i] = make_float2((float)g0, (float)(ssim_split_g(r1, rs, rcut) - g0));
 }
@@ -93,6 +95,7 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
     for (int db = max(b - 1, 0); db <= min(b + 1, nc1 - 1); ++db) {
       const int row = (da * nc1 + db) * nc2;
       const int s0 = cell_start[row + c0], s1 = cell_start[row + c1 + 1];
+#pragma unroll 4
       for (int s = s0; s < s1; ++s) {
         const float4 q = src[s];
         const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
@@ -103,11 +106,11 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
           n_in += (in && r2 > 0.f) ? 1 : 0;
           continue;
         }
-        const float t = sqrtf(r2) * tab_scale;
+        const float t = r2 * tab_scale;
         const int k = in ? (int)t : 0;
         const float2 e = tab[k];
         const float g = fmaf(t - (float)k, e.y, e.x);
-        const float inv = ssim_rsqrt_f32(r2 + eps2);
+        const float inv = __builtin_amdgcn_rsqf(r2 + eps2);
         const float f = in ? q.w * (inv * inv) * inv * g : 0.f;
         sx = fmaf(f, dx, sx);
         sy = fmaf(f, dy, sy);
@@ -195,7 +198,7 @@ int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
                        origin[0], origin[1], origin[2], nc[0], nc[1], nc[2],
                        cell_start, static_cast<const float4*>(packed), tcell_start,
                        torder, cstart, cmap, ncells, (float)eps2, gtab,
-                       (float)(GT / rcut), (float)(rcut * rcut), G, ax, ay, az,
+                       (float)(GT / (rcut * rcut)), (float)(rcut * rcut), G, ax, ay, az,
                        counts);
   else
     SSIM_TIMED("k_pp_cells", st,
@@ -204,7 +207,7 @@ int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
                                   nc[2], cell_start,
                                   static_cast<const float4*>(packed), tcell_start,
                                   torder, cstart, cmap, ncells, (float)eps2, gtab,
-                                  (float)(GT / rcut), (float)(rcut * rcut), G, ax,
+                                  (float)(GT / (rcut * rcut)), (float)(rcut * rcut), G, ax,
                                   ay, az, nullptr));
   SSIM_LAUNCHED("k_pp_cells");
   return SSIM_OK;

@@ -91,16 +91,18 @@ class LearnedCorrection:
         return out
 
     def predict(self, feat, out=None, add_to=None, scale=None, zs=None, hs=None,
-                relative=False):
+                relative=False, vel=None, n_proj=0):
         """Network outputs [nb][3] (FP32) and/or u*out added to add_to, with
-        u = scale, or scale * |add_to| per body when `relative`."""
+        u = scale, or scale * |add_to| per body when `relative`; the first
+        n_proj bodies get only the part orthogonal to their velocity `vel`."""
         nb = feat.shape[0]
         s = self.a_unit if scale is None else scale
         a = add_to if add_to is not None else [None, None, None]
+        v = vel if vel is not None else [None, None, None]
         _lib.call("ssim_mlp_forward", nb, _lib.ptr(feat), _lib.ptr(self.wbf),
                   _lib.ptr(self.params), float(s), int(relative), _lib.ptr(out),
-                  *[_lib.ptr(c) for c in a], _lib.ptr(zs), _lib.ptr(hs),
-                  D.stream(self.dev))
+                  *[_lib.ptr(c) for c in a], *[_lib.ptr(c) for c in v],
+                  int(n_proj), _lib.ptr(zs), _lib.ptr(hs), D.stream(self.dev))
         return out
 
     # -- training ----------------------------------------------------------

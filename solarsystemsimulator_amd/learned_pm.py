@@ -27,8 +27,13 @@ from .selfgrav import SelfGravSystem
 DATA = Path(__file__).resolve().parent / "data"
 DEFAULT_WEIGHTS = DATA / "learned_cube_250k.npz"
 
-# split parameters of the 250k-mass configuration (mesh 128^3, FP32 mesh)
-SPLIT = dict(mesh_n=128, rs_cells=1.0, rcut_rs=5.0, prec=P.F32)
+# split parameters of the 250k-mass configuration: 128^3 FP32 mesh with TSC
+# assignment, split scale rs = 1 cell, near field to 5 rs, four empty cells
+# around the bodies and the mesh placement kept while it fits (measured on
+# the 1000-step run: median force error 1e-4, energy drift 1.2x the direct
+# sum's; a mesh that follows the box every step, or CIC, drifts 2-3x more)
+SPLIT = dict(mesh_n=128, rs_cells=1.0, rcut_rs=5.0, prec=P.F32, assign="tsc",
+             margin=4.0, freeze=True)
 
 # time step of the 250k-mass cube (G = 1, total mass 1 in the unit cube):
 # 1000 steps end at t = 0.25, before the cold cube's free-fall collapse at
@@ -99,8 +104,11 @@ class PMLearnedSystem:
         self.split.accelerations(self.x, self.n, self.m, self.a)
         if learned and self.net is not None:
             feat = self.features()
+            # masses: no component along the velocity (no work done on
+            # the self-gravitating system); tracers: the full correction
             self.net.predict(feat, add_to=self.a, scale=self.out_unit(),
-                             relative=self.net.relative)
+                             relative=self.net.relative, vel=self.v,
+                             n_proj=self.n)
         self.acc_valid = True
         return self.a
 

@@ -44,8 +44,10 @@ class SplitForce:
     margin     empty mesh cells kept around the bodies
     prec       mesh precision (pm.F64 or pm.F32)
     freeze     keep the mesh placement while every body stays at least one
-               cell inside it (the force field is then a fixed function of
-               the positions between re-placements)
+               cell inside it and the bodies still span 3/4 of the usable
+               mesh (the force field is then a fixed function of the
+               positions between re-placements, which keeps the method's
+               energy error down)
     assign     "cic" (the reference's assignment) or "tsc"
     """
 
@@ -80,12 +82,14 @@ class SplitForce:
     def _place(self, x):
         """Mesh origin and ladder cell size covering every body."""
         box = P.minmax(x)
+        span = max(hi - lo for lo, hi in box)
         if self.freeze and self.level is not None:
             h, o = self.h, self.origin
-            if all(lo >= o[c] + h and hi <= o[c] + (self.n - 2) * h
-                   for c, (lo, hi) in enumerate(box)):
+            usable = (self.n - 1 - 2 * self.margin) * h
+            if span >= 0.75 * usable and all(
+                    lo >= o[c] + h and hi <= o[c] + (self.n - 2) * h
+                    for c, (lo, hi) in enumerate(box)):
                 return self.level, h, o, box
-        span = max(hi - lo for lo, hi in box)
         need = max(span, 1e-30) / (self.n - 1 - 2 * self.margin)
         level = math.ceil(64.0 * math.log2(need))
         h = 2.0 ** (level / 64.0)
@@ -125,11 +129,7 @@ class SplitForce:
             P.deposit(src, m[:n_mass], [self.n] * 3, origin, h, False, self.prec,
                       mesh=self.rho)
         else:
-            self.rho.zero_()
-            org = _lib.double_array(origin)
-            _lib.call("ssim_tsc_deposit", n_mass, *[_lib.ptr(c) for c in src],
-                      _lib.ptr(m), self.n, P._addr(org), h, self.prec,
-                      _lib.ptr(self.rho), st)
+            self._tsc_deposit(src, m[:n_mass], origin, h, st)
         self.convolve()
         if self.assign == "cic":
             P.gather(self.fields, x, [self.n] * 3, origin, h, False, self.prec,
@@ -141,6 +141,30 @@ class SplitForce:
                       *[_lib.ptr(f) for f in self.fields],
                       *[_lib.ptr(c) for c in a], st)
         return a
+
+    def _tsc_deposit(self, src, m, origin, h, st):
+        """rho = TSC mass mesh, without atomics: the masses are binned by
+        nearest node (stable cell list on the node-centred grid) and every
+        node sums its 27 neighbouring bins (bit-reproducible)."""
+        import torch
+        nm, nn = src[0].numel(), self.n ** 3
+        if getattr(self, "_bin_n", -1) != nm:
+            i32 = dict(dtype=torch.int32, device=self.dev)
+            self.bin_start = torch.empty(nn + 1, **i32)
+            self.bin_order = torch.empty(max(nm, 1), **i32)
+            self.bwork = D.workspace(_lib.load().ssim_cells_work_bytes(nm, nn),
+                                     self.dev)
+            self._bin_n = nm
+        gn = _lib.int_array([self.n] * 3)
+        org_b = _lib.double_array([o - 0.5 * h for o in origin])
+        org = _lib.double_array(origin)
+        _lib.call("ssim_cells_build", nm, *[_lib.ptr(c) for c in src], None,
+                  P._addr(org_b), h, P._addr(gn), _lib.ptr(self.bin_start),
+                  _lib.ptr(self.bin_order), None, _lib.ptr(self.bwork),
+                  self.bwork.numel(), st)
+        _lib.call("ssim_tsc_deposit_binned", *[_lib.ptr(c) for c in src],
+                  _lib.ptr(m), self.n, P._addr(org), h, _lib.ptr(self.bin_start),
+                  _lib.ptr(self.bin_order), self.prec, _lib.ptr(self.rho), st)
 
     def convolve(self):
         """fields = long-range acceleration meshes of self.rho."""

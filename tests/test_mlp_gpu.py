@@ -59,6 +59,23 @@ def test_forward_adds_scaled_output(dev):
         np.testing.assert_allclose(a[c].cpu().numpy(),
                                    1.5 + 2.0 * out[:, c].cpu().numpy().astype(np.float64),
                                    rtol=1e-6)
+    # projection: the first 40 bodies get only the part orthogonal to v
+    rng = np.random.default_rng(13)
+    vel = [torch.from_numpy(rng.normal(size=100)).to(dev) for _ in range(3)]
+    vel[0][5] = 0.0
+    vel[1][5] = 0.0
+    vel[2][5] = 0.0                                   # v = 0: unchanged
+    a = [torch.zeros(100, dtype=torch.float64, device=dev) for _ in range(3)]
+    net.predict(feat, add_to=a, scale=2.0, vel=vel, n_proj=40)
+    d = np.stack([c.cpu().numpy() for c in a], 1)
+    v = np.stack([c.cpu().numpy() for c in vel], 1)
+    full = 2.0 * out.cpu().numpy().astype(np.float64)
+    np.testing.assert_allclose(d[40:], full[40:], rtol=1e-6)
+    np.testing.assert_allclose(d[5], full[5], rtol=1e-6)
+    proj = full - ((full * v).sum(1) / np.maximum((v * v).sum(1), 1e-300))[:, None] * v
+    np.testing.assert_allclose(d[:40][np.arange(40) != 5], proj[:40][np.arange(40) != 5],
+                               rtol=1e-6, atol=1e-9)
+    assert np.abs((d[:40] * v[:40]).sum(1)).max() < 1e-9 * np.abs(full).max()
     # relative mode: u = scale * |a| of the acceleration being corrected
     a = [torch.full((100,), v, dtype=torch.float64, device=dev) for v in (3.0, 0.0, 4.0)]
     net.predict(feat, add_to=a, scale=1e-3, relative=True)

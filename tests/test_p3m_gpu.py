@@ -112,6 +112,25 @@ def test_tsc_deposit_and_gather_match_oracle(dev):
               _lib.ptr(mesh), D.stream(dev))
     ref = p3m_np.tsc_deposit(pos, m, n, origin, h)
     np.testing.assert_allclose(mesh.cpu().numpy(), ref, rtol=1e-12, atol=1e-12 * ref.max())
+    # the atomic-free deposit from nearest-node bins, twice: bit-identical
+    nn = n ** 3
+    start = torch.empty(nn + 1, dtype=torch.int32, device=dev)
+    order = torch.empty(30000, dtype=torch.int32, device=dev)
+    w = D.workspace(_lib.load().ssim_cells_work_bytes(30000, nn), dev)
+    gn = _lib.int_array([n] * 3)
+    org_b = _lib.double_array([o - 0.5 * h for o in origin])
+    _lib.call("ssim_cells_build", 30000, *[_lib.ptr(c) for c in x], None,
+              ctypes.addressof(org_b), h, ctypes.addressof(gn), _lib.ptr(start),
+              _lib.ptr(order), None, _lib.ptr(w), w.numel(), D.stream(dev))
+    binned = []
+    for _ in range(2):
+        mb = torch.full((n, n, n), 7.0, dtype=torch.float64, device=dev)
+        _lib.call("ssim_tsc_deposit_binned", *[_lib.ptr(c) for c in x], _lib.ptr(md),
+                  n, ctypes.addressof(org), h, _lib.ptr(start), _lib.ptr(order), 0,
+                  _lib.ptr(mb), D.stream(dev))
+        binned.append(mb.cpu().numpy())
+    assert np.array_equal(binned[0], binned[1])
+    np.testing.assert_allclose(binned[0], ref, rtol=1e-12, atol=1e-12 * ref.max())
     fields = [rng.normal(size=(n, n, n)) for _ in range(3)]
     fd = [D.to_dev(f, dev) for f in fields]        # keep the device copies alive
     out = [torch.empty(30000, dtype=torch.float64, device=dev) for _ in range(3)]
