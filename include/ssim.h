@@ -257,7 +257,8 @@ int ssim_tsc_deposit_binned(const double* x, const double* y, const double* z,
                             double h, const int* bin_start, const int* bin_order,
                             int mesh_prec, void* mesh, void* stream);
 /* TSC interpolation of three meshes at np points: o_c = sum w f_c
- * (overwrites o_c, FP64). */
+ * (overwrites o_c, FP64).  f1 = f2 = NULL: f0 holds the three FP32 fields
+ * interleaved as float4 [grid_n^3] (ssim_pm_conv_isolated3's layout). */
 int ssim_tsc_gather(int np, const double* x, const double* y, const double* z,
                     int grid_n, const double* origin, double h, int mesh_prec,
                     const void* f0, const void* f1, const void* f2, double* o0,
@@ -275,7 +276,9 @@ int ssim_spectrum_imag(long long ncomplex, int prec, const void* a, void* out,
  * (rfftn half-spectrum layout, real values): field_c = corner n^3 of
  * irfftn(rfftn(pad(rho)) * i kim_c), the result of ssim_pm_pad +
  * ssim_fft_r2c + ssim_spectrum_mul + ssim_fft_c2r + crop, computed without
- * the padding or the discarded part (pruned passes).  spec and tmp hold
+ * the padding or the discarded part (pruned passes).  With fy = fz = NULL
+ * the three fields are written interleaved into fx as [n^3][4] (component
+ * c at index 4 i + c; the 4th slot is left as it is).  spec and tmp hold
  * (2n)^2 (n+1) complex values each; fft_work is prepared for dims
  * (2n, 2n, 2n).  No reference counterpart (split force). */
 int ssim_pm_conv_isolated3(int n, int prec, const void* rho, const void* kim_x,

@@ -92,11 +92,125 @@ struct Pass {
   int contig;                 // 1: lines are contiguous rows (s_elem == 1)
   int in_count, out_count;    // points read / written per line
   long long R1, s_in_hi, s_out_hi;
+  int tw_lds;                 // 1: the twiddle table is staged in LDS
+  int out_es, out_c;          // C2R: element e goes to out[(off + e) * es + c]
 };
 
 __device__ __forceinline__ long long row_off(long long r, long long R1,
                                              long long s_hi, long long s_lo) {
   return (r / R1) * s_hi + (r % R1) * s_lo;
+}
+
+// ---- 256-point lines: four-step FFT with 16-point DFTs in registers ----
+// x[n1 + 16 n2] -> Y[n1][k2] = DFT16 over n2 (thread n1), times
+// W256^(n1 k2), then X[k2 + 16 k1] = DFT16 over n1 of Y[.][k2] (thread k2):
+// two LDS exchanges instead of four radix-4 stages.  Y[n1][k2] sits at
+// 16 n1 + (k2 ^ n1) so both exchanges are free of bank conflicts.
+
+template <typename F>
+__device__ __forceinline__ void dft4(cpx<F>& x0, cpx<F>& x1, cpx<F>& x2, cpx<F>& x3,
+                                     bool inverse) {
+  const cpx<F> s02 = cadd(x0, x2), d02 = csub(x0, x2);
+  const cpx<F> s13 = cadd(x1, x3), d13 = rot(csub(x1, x3), inverse);
+  x0 = cadd(s02, s13);
+  x1 = cadd(d02, d13);
+  x2 = csub(s02, s13);
+  x3 = csub(d02, d13);
+}
+
+// in-place 16-point DFT (forward: exp(-2 pi i nk/16)), natural order out
+template <typename F>
+__device__ __forceinline__ void dft16(cpx<F> v[16], bool inverse) {
+  // W16^j, j = 0..9, as (cos, -sin) of 2 pi j / 16
+  constexpr double C1 = 0.92387953251128675613, S1 = 0.38268343236508977173;
+  constexpr double C2 = 0.70710678118654752440;
+  const F sg = inverse ? F(1) : F(-1);
+  const cpx<F> w[10] = {{F(1), F(0)},          {F(C1), sg * F(S1)},
+                        {F(C2), sg * F(C2)},   {F(S1), sg * F(C1)},
+                        {F(0), sg},            {F(-S1), sg * F(C1)},
+                        {F(-C2), sg * F(C2)},  {F(-C1), sg * F(S1)},
+                        {F(-1), F(0)},         {F(-C1), -sg * F(S1)}};
+#pragma unroll
+  for (int b = 0; b < 4; ++b) dft4(v[b], v[b + 4], v[b + 8], v[b + 12], inverse);
+  // v[b + 4 k1] = U_b[k1]; twiddle by W16^(b k1)
+#pragma unroll
+  for (int b = 1; b < 4; ++b)
+#pragma unroll
+    for (int k1 = 1; k1 < 4; ++k1) v[b + 4 * k1] = cmul(v[b + 4 * k1], w[b * k1]);
+  cpx<F> o[16];
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cpx<F> t0 = v[4 * k1], t1 = v[4 * k1 + 1], t2 = v[4 * k1 + 2], t3 = v[4 * k1 + 3];
+    dft4(t0, t1, t2, t3, inverse);
+    o[k1] = t0;
+    o[k1 + 4] = t1;
+    o[k1 + 8] = t2;
+    o[k1 + 12] = t3;
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = o[k];
+}
+
+template <typename F>
+__device__ void four_step_256(cpx<F>* s, int LS, int nl, const cpx<F>* tw,
+                              bool inverse) {
+  const int jobs = nl * 16;
+  constexpr int MAXJ = 4096 / 16 / FFT_THREADS > 0 ? 4096 / 16 / FFT_THREADS : 1;
+  cpx<F> v[MAXJ][16];
+  // step 1: DFT16 over n2 of x[n1 + 16 n2], thread (l, n1)
+#pragma unroll
+  for (int q = 0; q < MAXJ; ++q) {
+    const int job = threadIdx.x + q * FFT_THREADS;
+    if (job < jobs) {
+      const cpx<F>* a = s + (job >> 4) * LS;
+      const int n1 = job & 15;
+#pragma unroll
+      for (int n2 = 0; n2 < 16; ++n2) v[q][n2] = a[n1 + 16 * n2];
+      dft16(v[q], inverse);
+#pragma unroll
+      for (int k2 = 1; k2 < 16; ++k2) {
+        cpx<F> w = tw[n1 * k2];
+        if (inverse) w.im = -w.im;
+        v[q][k2] = cmul(v[q][k2], w);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXJ; ++q) {
+    const int job = threadIdx.x + q * FFT_THREADS;
+    if (job < jobs) {
+      cpx<F>* a = s + (job >> 4) * LS;
+      const int n1 = job & 15;
+#pragma unroll
+      for (int k2 = 0; k2 < 16; ++k2) a[16 * n1 + (k2 ^ n1)] = v[q][k2];
+    }
+  }
+  __syncthreads();
+  // step 3: DFT16 over n1 of Y[n1][k2], thread (l, k2)
+#pragma unroll
+  for (int q = 0; q < MAXJ; ++q) {
+    const int job = threadIdx.x + q * FFT_THREADS;
+    if (job < jobs) {
+      const cpx<F>* a = s + (job >> 4) * LS;
+      const int k2 = job & 15;
+#pragma unroll
+      for (int n1 = 0; n1 < 16; ++n1) v[q][n1] = a[16 * n1 + (k2 ^ n1)];
+      dft16(v[q], inverse);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXJ; ++q) {
+    const int job = threadIdx.x + q * FFT_THREADS;
+    if (job < jobs) {
+      cpx<F>* a = s + (job >> 4) * LS;
+      const int k2 = job & 15;
+#pragma unroll
+      for (int k1 = 0; k1 < 16; ++k1) a[k2 + 16 * k1] = v[q][k1];
+    }
+  }
+  __syncthreads();
 }
 
 template <typename F, int MODE>
@@ -126,6 +240,12 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     b0 = (long long)(blockIdx.x % groups) * nl;
     nvalid = (int)min((long long)nl, (long long)P.n_inner - b0);
   }
+
+  // twiddles: a copy of the table after the tile when it fits in LDS
+  cpx<F>* stw = s + nl * LS;
+  if (P.tw_lds)
+    for (int t = threadIdx.x; t < L; t += FFT_THREADS) stw[t] = tw[t];
+  const cpx<F>* twp = P.tw_lds ? stw : tw;
 
   // ---- load the tile: s[l * LS + e] ----
   if (MODE == MODE_R2C) {
@@ -183,80 +303,84 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
   }
   __syncthreads();
 
-  // ---- Stockham stages over all lines of the tile ----
-  int p = 1;
-  if (P.log2L & 1) {  // one radix-2 stage (p == 1: k == 0, no twiddle)
-    constexpr int NB = max_elems<F>() / 2 / FFT_THREADS;
-    const int T = L / 2;
-    cpx<F> r0[NB], r1[NB];
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
-      const int u = threadIdx.x + nb * FFT_THREADS;
-      if (u < E / 2) {
-        const cpx<F>* a = s + (u / T) * LS;
-        const int i = u % T;
-        r0[nb] = a[i];
-        r1[nb] = a[i + T];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
-      const int u = threadIdx.x + nb * FFT_THREADS;
-      if (u < E / 2) {
-        cpx<F>* a = s + (u / T) * LS;
-        const int i = u % T;
-        a[2 * i] = cadd(r0[nb], r1[nb]);
-        a[2 * i + 1] = csub(r0[nb], r1[nb]);
-      }
-    }
-    __syncthreads();
-    p = 2;
-  }
-  for (; p < L; p *= 4) {
-    constexpr int NB = max_elems<F>() / 4 / FFT_THREADS;
-    const int T = L / 4;
-    const int twstep = L / (4 * p);
-    cpx<F> r[NB][4];
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
-      const int u = threadIdx.x + nb * FFT_THREADS;
-      if (u < E / 4) {
-        const int i = u % T;
-        const int k = i & (p - 1);
-        const cpx<F>* a = s + (u / T) * LS;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          cpx<F> x = a[i + q * T];
-          if (q > 0 && k > 0) {
-            cpx<F> w = tw[q * k * twstep];
-            if (inverse) w.im = -w.im;
-            x = cmul(x, w);
-          }
-          r[nb][q] = x;
+  if (L == 256) {
+    four_step_256(s, LS, nl, twp, inverse != 0);
+  } else {
+    // ---- Stockham stages over all lines of the tile ----
+    int p = 1;
+    if (P.log2L & 1) {  // one radix-2 stage (p == 1: k == 0, no twiddle)
+      constexpr int NB = max_elems<F>() / 2 / FFT_THREADS;
+      const int T = L / 2;
+      cpx<F> r0[NB], r1[NB];
+  #pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int u = threadIdx.x + nb * FFT_THREADS;
+        if (u < E / 2) {
+          const cpx<F>* a = s + (u / T) * LS;
+          const int i = u % T;
+          r0[nb] = a[i];
+          r1[nb] = a[i + T];
         }
       }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int nb = 0; nb < NB; ++nb) {
-      const int u = threadIdx.x + nb * FFT_THREADS;
-      if (u < E / 4) {
-        const int i = u % T;
-        const int k = i & (p - 1);
-        cpx<F>* a = s + (u / T) * LS;
-        const cpx<F> s02 = cadd(r[nb][0], r[nb][2]);
-        const cpx<F> d02 = csub(r[nb][0], r[nb][2]);
-        const cpx<F> s13 = cadd(r[nb][1], r[nb][3]);
-        const cpx<F> d13 = rot(csub(r[nb][1], r[nb][3]), inverse != 0);
-        const int j = (i - k) * 4 + k;
-        a[j] = cadd(s02, s13);
-        a[j + p] = cadd(d02, d13);
-        a[j + 2 * p] = csub(s02, s13);
-        a[j + 3 * p] = csub(d02, d13);
+      __syncthreads();
+  #pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int u = threadIdx.x + nb * FFT_THREADS;
+        if (u < E / 2) {
+          cpx<F>* a = s + (u / T) * LS;
+          const int i = u % T;
+          a[2 * i] = cadd(r0[nb], r1[nb]);
+          a[2 * i + 1] = csub(r0[nb], r1[nb]);
+        }
       }
+      __syncthreads();
+      p = 2;
     }
-    __syncthreads();
+    for (; p < L; p *= 4) {
+      constexpr int NB = max_elems<F>() / 4 / FFT_THREADS;
+      const int T = L / 4;
+      const int twstep = L / (4 * p);
+      cpx<F> r[NB][4];
+  #pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int u = threadIdx.x + nb * FFT_THREADS;
+        if (u < E / 4) {
+          const int i = u % T;
+          const int k = i & (p - 1);
+          const cpx<F>* a = s + (u / T) * LS;
+  #pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            cpx<F> x = a[i + q * T];
+            if (q > 0 && k > 0) {
+              cpx<F> w = twp[q * k * twstep];
+              if (inverse) w.im = -w.im;
+              x = cmul(x, w);
+            }
+            r[nb][q] = x;
+          }
+        }
+      }
+      __syncthreads();
+  #pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int u = threadIdx.x + nb * FFT_THREADS;
+        if (u < E / 4) {
+          const int i = u % T;
+          const int k = i & (p - 1);
+          cpx<F>* a = s + (u / T) * LS;
+          const cpx<F> s02 = cadd(r[nb][0], r[nb][2]);
+          const cpx<F> d02 = csub(r[nb][0], r[nb][2]);
+          const cpx<F> s13 = cadd(r[nb][1], r[nb][3]);
+          const cpx<F> d13 = rot(csub(r[nb][1], r[nb][3]), inverse != 0);
+          const int j = (i - k) * 4 + k;
+          a[j] = cadd(s02, s13);
+          a[j + p] = cadd(d02, d13);
+          a[j + 2 * p] = csub(s02, s13);
+          a[j + 3 * p] = csub(d02, d13);
+        }
+      }
+      __syncthreads();
+    }
   }
 
   // ---- store ----
@@ -276,7 +400,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     for (int t = threadIdx.x; t < E; t += FFT_THREADS) {
       const int l = t / L, e = t % L;
       if (l < nvalid && e < P.out_count)
-        out[row_off(o0 + l, P.R1, P.s_out_hi, P.s_outer_out) + e] = s[l * LS + e].re * scale;
+        out[(row_off(o0 + l, P.R1, P.s_out_hi, P.s_outer_out) + e) * P.out_es + P.out_c] =
+            s[l * LS + e].re * scale;
     }
   } else {
     cpx<F>* out = static_cast<cpx<F>*>(vout);
@@ -338,8 +463,10 @@ int launch_pass(const void* in, void* out, Pass P, const void* tw, int inverse,
     nwg = ssim::ceil_div(P.n_outer, P.lines_per_tile);
   else
     nwg = P.n_outer * ssim::ceil_div(P.n_inner, P.lines_per_tile);
-  const size_t lds = (size_t)P.lines_per_tile *
-                    (P.lines_per_tile > 1 ? P.L + 1 : P.L) * sizeof(cpx<F>);
+  size_t lds = (size_t)P.lines_per_tile *
+               (P.lines_per_tile > 1 ? P.L + 1 : P.L) * sizeof(cpx<F>);
+  P.tw_lds = lds + (size_t)P.L * sizeof(cpx<F>) <= 65536 ? 1 : 0;
+  if (P.tw_lds) lds += (size_t)P.L * sizeof(cpx<F>);
   SSIM_TIMED("k_fft_pass", st,
              hipLaunchKernelGGL((k_fft_pass<F, MODE>), dim3(nwg), dim3(FFT_THREADS),
                                 lds, st, in, out, P, static_cast<const cpx<F>*>(tw),
@@ -364,6 +491,8 @@ Pass strided_pass(int ndim, const int* cdims, int axis, int tile) {
   P.in_count = P.out_count = P.L;
   P.R1 = 1;
   P.s_in_hi = P.s_out_hi = 0;
+  P.out_es = 1;
+  P.out_c = 0;
   int b = tile / P.L;
   if (b > FFT_MAXB) b = FFT_MAXB;
   if (b > P.n_inner) {
@@ -387,6 +516,8 @@ Pass row_pass(long long rows, int L, long long s_in, long long s_out, int tile) 
   P.in_count = P.out_count = L;
   P.R1 = rows > 0 ? rows : 1;
   P.s_in_hi = P.s_out_hi = 0;
+  P.out_es = 1;
+  P.out_c = 0;
   int b = tile / L;
   if (b < 1) b = 1;
   P.lines_per_tile = b;
@@ -446,10 +577,13 @@ int fft_c2r(int ndim, const int* dims, cpx<F>* in, F* out,
 // The padding never exists in memory and nothing outside the corner is
 // computed in the inverse: about 2.6x fewer HBM bytes than pad + rfftn +
 // 3 x (product + irfftn + crop).
+// fields[1] == fields[2] == nullptr: the three results are interleaved in
+// fields[0] as [n^3][4] (component c at 4 i + c, slot 3 untouched).
 template <typename F>
 int conv_isolated3(int n, const F* rho, const F* const* kim, F* const* fields,
                    cpx<F>* spec, cpx<F>* tmp, const unsigned char* tw, int prec,
                    hipStream_t st) {
+  const bool inter = fields[1] == nullptr && fields[2] == nullptr;
   const int P = 2 * n, H = n + 1;
   const int dims[3] = {P, P, P};
   const int cd[3] = {P, P, H};
@@ -484,7 +618,12 @@ int conv_isolated3(int n, const F* rho, const F* const* kim, F* const* fields,
     iz.s_in_hi = (long long)P * H;
     iz.s_out_hi = rows;
     iz.out_count = n;
-    SSIM_TRY((launch_pass<F, MODE_C2R>(tmp, fields[c], iz, twz, 1, scale, st)));
+    if (inter) {
+      iz.out_es = 4;
+      iz.out_c = c;
+    }
+    SSIM_TRY((launch_pass<F, MODE_C2R>(tmp, inter ? fields[0] : fields[c], iz, twz, 1,
+                                       scale, st)));
   }
   return SSIM_OK;
 }
@@ -568,9 +707,10 @@ int ssim_pm_conv_isolated3(int n, int prec, const void* rho, const void* kim_x,
                "n must be a power of two with 2n <= %d", FFT_MAX_L);
   SSIM_REQUIRE(prec == SSIM_PREC_F64 || prec == SSIM_PREC_F32C,
                "unknown precision %d", prec);
-  SSIM_REQUIRE(rho && kim_x && kim_y && kim_z && fx && fy && fz && spec && tmp &&
-                   fft_work,
+  SSIM_REQUIRE(rho && kim_x && kim_y && kim_z && fx && spec && tmp && fft_work,
                "null pointer");
+  SSIM_REQUIRE((fy == nullptr) == (fz == nullptr),
+               "fy and fz are both given or both NULL (interleaved)");
   const int dims[3] = {2 * n, 2 * n, 2 * n};
   SSIM_REQUIRE(fft_work_bytes >= ssim::fft_work_bytes(3, dims, prec),
                "FFT workspace too small (prepare it for the padded 2n^3 mesh)");

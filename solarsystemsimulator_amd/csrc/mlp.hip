@@ -173,6 +173,67 @@ __device__ void forward16(const Weights& W, const float* __restrict__ bias,
   for (int i = 0; i < 3; ++i) out3[i] = acc[i] + bias[3 * NH + i];
 }
 
+// Inference on two 16-body column blocks at once: the MFMA chains of the
+// two blocks are independent, so the matrix cores and the SiLU work of one
+// block overlap with the other's.
+__device__ void forward2x16(const Weights& W, const float* __restrict__ bias,
+                            const unsigned short* __restrict__ x, int nb, int b0,
+                            int lane, float out3[2][3], bool valid[2]) {
+  s4_t xb[2][2], h1[2][4], h2[2][4], h3[2][4];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int body = b0 + 16 * u + (lane & 15);
+    valid[u] = body < nb;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+      xb[u][kb] = valid[u] ? *reinterpret_cast<const s4_t*>(
+                                 x + (size_t)body * NIN + 16 * kb + 4 * (lane >> 4))
+                           : s4_t{0, 0, 0, 0};
+  }
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    f4_t acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) acc[u] = mfma(W.w1[rt][kb], xb[u][kb], acc[u]);
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      h1[u][rt] = activate(acc[u], bias, rt, lane, nullptr, nullptr);
+  }
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    f4_t acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) acc[u] = mfma(W.w2[rt][kb], h1[u][kb], acc[u]);
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      h2[u][rt] = activate(acc[u], bias + NH, rt, lane, nullptr, nullptr);
+  }
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    f4_t acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) acc[u] = mfma(W.w3[rt][kb], h2[u][kb], acc[u]);
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      h3[u][rt] = activate(acc[u], bias + 2 * NH, rt, lane, nullptr, nullptr);
+  }
+  f4_t acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+    for (int u = 0; u < 2; ++u) acc[u] = mfma(W.w4[kb], h3[u][kb], acc[u]);
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out3[u][i] = acc[u][i] + bias[3 * NH + i];
+}
+
 __global__ __launch_bounds__(64 * WAVES) void k_mlp_forward(
     int nb, const unsigned short* __restrict__ x,
     const unsigned short* __restrict__ wbf, const float* __restrict__ params,
@@ -194,41 +255,54 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_forward(
   const int nwaves = gridDim.x * WAVES;
   Weights W;
   load_weights(wbf, lane, W);
+  // one body's result: store / add the (projected) correction
+  auto emit = [&](int body, const float o[3]) {
+    if (out) {
+      out[(size_t)body * 3 + 0] = o[0];
+      out[(size_t)body * 3 + 1] = o[1];
+      out[(size_t)body * 3 + 2] = o[2];
+    }
+    if (a0) {
+      // relative: the output is in units of scale * |a| of the force it
+      // corrects (read before the update)
+      const double u = relative ? (double)scale * sqrt(a0[body] * a0[body] +
+                                                       a1[body] * a1[body] +
+                                                       a2[body] * a2[body])
+                                : (double)scale;
+      double d0 = u * (double)o[0], d1 = u * (double)o[1], d2 = u * (double)o[2];
+      if (body < n_proj) {
+        // no component along the velocity: the correction does no work
+        const double w0 = v0[body], w1 = v1[body], w2 = v2[body];
+        const double vv = w0 * w0 + w1 * w1 + w2 * w2;
+        if (vv > 0.0) {
+          const double c = (d0 * w0 + d1 * w1 + d2 * w2) / vv;
+          d0 -= c * w0;
+          d1 -= c * w1;
+          d2 -= c * w2;
+        }
+      }
+      a0[body] += d0;
+      a1[body] += d1;
+      a2[body] += d2;
+    }
+  };
+  if (zs == nullptr) {  // inference: two column blocks per iteration
+    for (int b0 = wave * 32; b0 < nb; b0 += nwaves * 32) {
+      float o[2][3];
+      bool valid[2];
+      forward2x16(W, sbias, x, nb, b0, lane, o, valid);
+      if (lane < 16) {
+        if (valid[0]) emit(b0 + lane, o[0]);
+        if (valid[1]) emit(b0 + 16 + lane, o[1]);
+      }
+    }
+    return;
+  }
   for (int b0 = wave * 16; b0 < nb; b0 += nwaves * 16) {
     float o[3];
     bool valid;
     forward16(W, sbias, x, nb, b0, lane, o, valid, zs, hs);
-    if (lane < 16 && valid) {
-      const int body = b0 + lane;
-      if (out) {
-        out[(size_t)body * 3 + 0] = o[0];
-        out[(size_t)body * 3 + 1] = o[1];
-        out[(size_t)body * 3 + 2] = o[2];
-      }
-      if (a0) {
-        // relative: the output is in units of scale * |a| of the force it
-        // corrects (read before the update)
-        const double u = relative ? (double)scale * sqrt(a0[body] * a0[body] +
-                                                         a1[body] * a1[body] +
-                                                         a2[body] * a2[body])
-                                  : (double)scale;
-        double d0 = u * (double)o[0], d1 = u * (double)o[1], d2 = u * (double)o[2];
-        if (body < n_proj) {
-          // no component along the velocity: the correction does no work
-          const double w0 = v0[body], w1 = v1[body], w2 = v2[body];
-          const double vv = w0 * w0 + w1 * w1 + w2 * w2;
-          if (vv > 0.0) {
-            const double c = (d0 * w0 + d1 * w1 + d2 * w2) / vv;
-            d0 -= c * w0;
-            d1 -= c * w1;
-            d2 -= c * w2;
-          }
-        }
-        a0[body] += d0;
-        a1[body] += d1;
-        a2[body] += d2;
-      }
-    }
+    if (lane < 16 && valid) emit(b0 + lane, o);
   }
 }
 

@@ -73,6 +73,9 @@ class SplitForce:
                        for _ in range(3)]
         self.kim = None
         self.fused = True     # False: pad + rfftn + product + irfftn + crop
+        # TSC on an FP32 mesh gathers the three fields as one float4 mesh
+        self.f4 = (torch.empty((self.n ** 3, 4), dtype=torch.float32, device=dev)
+                   if assign == "tsc" and prec == P.F32 else None)
         self.level = None
         self.h = None
         self.origin = None
@@ -130,16 +133,16 @@ class SplitForce:
                       mesh=self.rho)
         else:
             self._tsc_deposit(src, m[:n_mass], origin, h, st)
-        self.convolve()
+        self.convolve(interleaved=self.f4 is not None)
         if self.assign == "cic":
             P.gather(self.fields, x, [self.n] * 3, origin, h, False, self.prec,
                      outs=a)
         else:
             org = _lib.double_array(origin)
+            fs = ([self.f4, None, None] if self.f4 is not None else self.fields)
             _lib.call("ssim_tsc_gather", x[0].numel(), *[_lib.ptr(c) for c in x],
                       self.n, P._addr(org), h, self.prec,
-                      *[_lib.ptr(f) for f in self.fields],
-                      *[_lib.ptr(c) for c in a], st)
+                      *[_lib.ptr(f) for f in fs], *[_lib.ptr(c) for c in a], st)
         return a
 
     def _tsc_deposit(self, src, m, origin, h, st):
@@ -166,17 +169,19 @@ class SplitForce:
                   _lib.ptr(m), self.n, P._addr(org), h, _lib.ptr(self.bin_start),
                   _lib.ptr(self.bin_order), self.prec, _lib.ptr(self.rho), st)
 
-    def convolve(self):
-        """fields = long-range acceleration meshes of self.rho."""
+    def convolve(self, interleaved=False):
+        """fields = long-range acceleration meshes of self.rho (or, with
+        `interleaved`, self.f4 = the three as float4 [n^3])."""
         import torch
         st = D.stream(self.dev)
         if self.fused:
+            outs = [self.f4, None, None] if interleaved else self.fields
             _lib.call("ssim_pm_conv_isolated3", self.n, self.prec,
                       _lib.ptr(self.rho), *[_lib.ptr(k) for k in self.kim],
-                      *[_lib.ptr(f) for f in self.fields], _lib.ptr(self.spec),
+                      *[_lib.ptr(f) for f in outs], _lib.ptr(self.spec),
                       _lib.ptr(self.prod), _lib.ptr(self.plan.work),
                       self.plan.work_bytes, st)
-            return self.fields
+            return self.f4 if interleaved else self.fields
         pad = torch.zeros([self.Pn] * 3, dtype=self.rho.dtype, device=self.dev)
         full = torch.empty_like(pad)
         _lib.call("ssim_pm_pad", 3, self.n, self.prec, _lib.ptr(self.rho),

@@ -141,6 +141,21 @@ def test_tsc_deposit_and_gather_match_oracle(dev):
         np.testing.assert_allclose(out[c].cpu().numpy(),
                                    p3m_np.tsc_gather(fields[c], pos, n, origin, h),
                                    rtol=1e-12, atol=1e-12)
+    # FP32 fields: three arrays and the interleaved float4 mesh agree exactly
+    f32 = [D.to_dev(f, dev, torch.float32) for f in fields]
+    f4 = torch.zeros((n ** 3, 4), dtype=torch.float32, device=dev)
+    for c in range(3):
+        f4[:, c] = f32[c].reshape(-1)
+    o3 = [torch.empty(30000, dtype=torch.float64, device=dev) for _ in range(3)]
+    o4 = [torch.empty(30000, dtype=torch.float64, device=dev) for _ in range(3)]
+    _lib.call("ssim_tsc_gather", 30000, *[_lib.ptr(c) for c in x], n,
+              ctypes.addressof(org), h, 1, *[_lib.ptr(f) for f in f32],
+              *[_lib.ptr(o) for o in o3], D.stream(dev))
+    _lib.call("ssim_tsc_gather", 30000, *[_lib.ptr(c) for c in x], n,
+              ctypes.addressof(org), h, 1, _lib.ptr(f4), None, None,
+              *[_lib.ptr(o) for o in o4], D.stream(dev))
+    for c in range(3):
+        assert torch.equal(o3[c], o4[c])
 
 
 @pytest.mark.parametrize("assign", ["cic", "tsc"])
@@ -168,13 +183,12 @@ def test_split_force_assignments_on_cube(dev, assign):
         assert errs["tsc"] < 0.5 * errs["cic"], errs
 
 
-@pytest.mark.parametrize("prec,tol", [(0, 1e-12), (1, 1e-5)])
-def test_pruned_convolution_matches_padded_fft(dev, prec, tol):
+@pytest.mark.parametrize("prec,tol,n", [(0, 1e-12, 32), (1, 1e-5, 32), (1, 1e-5, 128)])
+def test_pruned_convolution_matches_padded_fft(dev, prec, tol, n):
     """ssim_pm_conv_isolated3 (pruned passes, imaginary kernel spectrum)
     equals the corner of irfftn(rfftn(pad(rho)) * i kim) in NumPy, and the
     unfused device path."""
-    print(f"pruned convolution prec={prec}", flush=True)
-    n = 32
+    print(f"pruned convolution prec={prec} n={n}", flush=True)
     sf = SplitForce(dev, mesh_n=n, eps2=1e-6, rs_cells=1.25, rcut_rs=4.5, prec=prec)
     sf._rebuild_kernels(0.03)
     rng = np.random.default_rng(5)
@@ -182,6 +196,12 @@ def test_pruned_convolution_matches_padded_fft(dev, prec, tol):
     rho[rng.uniform(size=rho.shape) < 0.5] = 0.0
     sf.rho.copy_(torch.from_numpy(rho).to(sf.rho.dtype))
     got = [f.cpu().numpy().astype(np.float64) for f in sf.convolve()]
+    if prec == 1:                       # the interleaved float4 output too
+        sf.f4 = torch.full((n ** 3, 4), 9.0, dtype=torch.float32, device=dev)
+        f4 = sf.convolve(interleaved=True).cpu().numpy()
+        for c in range(3):
+            assert np.array_equal(f4[:, c].reshape(n, n, n), got[c].astype(np.float32))
+        assert np.all(f4[:, 3] == 9.0)
     sf.fused = False
     ref_dev = [f.cpu().numpy().astype(np.float64) for f in sf.convolve()]
     P = 2 * n

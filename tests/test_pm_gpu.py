@@ -20,9 +20,9 @@ def dev_t(a, dev, dtype=torch.float64):
 
 
 @pytest.mark.parametrize("prec", [P.F64, P.F32])
-@pytest.mark.parametrize("dims", [[2], [8], [4096], [2, 4], [16, 8], [64, 128],
-                                  [1024, 512], [4, 8, 2], [32, 32, 32],
-                                  [128, 64, 256]])
+@pytest.mark.parametrize("dims", [[2], [8], [256], [4096], [2, 4], [16, 8],
+                                  [64, 128], [256, 16], [1024, 512], [4, 8, 2],
+                                  [32, 32, 32], [128, 64, 256], [256, 4, 256]])
 def test_fft_matches_numpy(dev, dims, prec):
     print(f"fft dims={dims} prec={prec}", flush=True)
     rng = np.random.default_rng(len(dims) * 100 + dims[-1])
