@@ -247,20 +247,27 @@ int ssim_pm_deconvolve(int dim, int padded_n, int prec, int order,
 int ssim_tsc_deposit(int np, const double* x, const double* y, const double* z,
                      const double* m, int grid_n, const double* origin, double h,
                      int mesh_prec, void* mesh, void* stream);
-/* The TSC deposit without atomics (OVERWRITES mesh): the masses are binned
+/* The TSC deposit without atomics (OVERWRITES mesh) of np masses (work:
+ * ssim_tsc_deposit_work_bytes(np), their packed copy in bin order): the
+ * masses are binned
  * by nearest node (ssim_cells_build with origin - h/2, cell h and
  * grid_n^3 cells: bin_start, bin_order) and every node sums the bodies of
  * the 27 bins around it in a fixed order, so the mesh is bit-reproducible.
  * Same weights as ssim_tsc_deposit. */
-int ssim_tsc_deposit_binned(const double* x, const double* y, const double* z,
-                            const double* m, int grid_n, const double* origin,
-                            double h, const int* bin_start, const int* bin_order,
-                            int mesh_prec, void* mesh, void* stream);
+size_t ssim_tsc_deposit_work_bytes(int np);
+int ssim_tsc_deposit_binned(int np, const double* x, const double* y,
+                            const double* z, const double* m, int grid_n,
+                            const double* origin, double h, const int* bin_start,
+                            const int* bin_order, int mesh_prec, void* mesh,
+                            void* work, size_t work_bytes, void* stream);
 /* TSC interpolation of three meshes at np points: o_c = sum w f_c
- * (overwrites o_c, FP64).  f1 = f2 = NULL: f0 holds the three FP32 fields
+ * (overwrites o_c, FP64); thread i handles point order[i] (order = NULL:
+ * point i) — a cell-sorted order makes neighbouring threads read
+ * neighbouring nodes.  f1 = f2 = NULL: f0 holds the three FP32 fields
  * interleaved as float4 [grid_n^3] (ssim_pm_conv_isolated3's layout). */
-int ssim_tsc_gather(int np, const double* x, const double* y, const double* z,
-                    int grid_n, const double* origin, double h, int mesh_prec,
+int ssim_tsc_gather(int np, const int* order, const double* x, const double* y,
+                    const double* z, int grid_n, const double* origin, double h,
+                    int mesh_prec,
                     const void* f0, const void* f1, const void* f2, double* o0,
                     double* o1, double* o2, void* stream);
 /* out = a * b over ncomplex interleaved complex values (the spectral
@@ -425,13 +432,15 @@ int ssim_bounce_deltas(int n, int npairs, const long long* hi, const long long* 
 int ssim_mlp_param_count(void);
 /* wbf = BF16 (round to nearest even) of params. */
 int ssim_mlp_to_bf16(const float* params, unsigned short* wbf, void* stream);
-/* Features of nb bodies (the first n_mass are masses, the rest tracers)
+/* Features of nb bodies (the first n_mass are masses, the rest tracers;
+ * thread i fills body order[i]'s row, order = NULL: body i)
  * from the CIC mass mesh (mesh_n^3, C order, FP64 or FP32 per mesh_prec)
  * with origin[3] and cell h: the 27 node masses of the 3x3x3 stencil around
  * the nearest node divided by mass_unit, twice the offset from that node in
  * cells, 1 for a mass / 0 for a tracer, and a zero. */
-int ssim_mlp_features(int nb, int n_mass, const double* x, const double* y,
-                      const double* z, int mesh_n, const double* origin, double h,
+int ssim_mlp_features(int nb, int n_mass, const int* order, const double* x,
+                      const double* y, const double* z, int mesh_n,
+                      const double* origin, double h,
                       int mesh_prec, const void* rho, double mass_unit,
                       unsigned short* feat, void* stream);
 /* Forward pass.  out (optional) receives FP32 [nb][3]; if ax != NULL,

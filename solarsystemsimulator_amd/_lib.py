@@ -75,9 +75,10 @@ SIGNATURES = {
     "ssim_pm_deconvolve": (c_int, [c_int, c_int, c_int, c_int, _P, _P]),
     "ssim_tsc_deposit": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double,
                                  c_int, _P, _P]),
-    "ssim_tsc_deposit_binned": (c_int, [_P, _P, _P, _P, c_int, _P, c_double, _P,
-                                        _P, c_int, _P, _P]),
-    "ssim_tsc_gather": (c_int, [c_int, _P, _P, _P, c_int, _P, c_double, c_int,
+    "ssim_tsc_deposit_work_bytes": (c_size_t, [c_int]),
+    "ssim_tsc_deposit_binned": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double,
+                                        _P, _P, c_int, _P, _P, c_size_t, _P]),
+    "ssim_tsc_gather": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double, c_int,
                                 _P, _P, _P, _P, _P, _P, _P]),
     "ssim_spectrum_mul": (c_int, [c_longlong, c_int, _P, _P, _P, _P]),
     "ssim_spectrum_imag": (c_int, [c_longlong, c_int, _P, _P, _P]),
@@ -114,8 +115,8 @@ SIGNATURES = {
                            + [_P] * 7 + [_P, c_size_t, _P]),
     "ssim_mlp_param_count": (c_int, []),
     "ssim_mlp_to_bf16": (c_int, [_P, _P, _P]),
-    "ssim_mlp_features": (c_int, [c_int, c_int, _P, _P, _P, c_int, _P, c_double,
-                                  c_int, _P, c_double, _P, _P]),
+    "ssim_mlp_features": (c_int, [c_int, c_int, _P, _P, _P, _P, c_int, _P,
+                                  c_double, c_int, _P, c_double, _P, _P]),
     "ssim_mlp_forward": (c_int, [c_int, _P, _P, _P, c_float, c_int, _P, _P, _P,
                                  _P, _P, _P, _P, c_int, _P, _P, _P]),
     "ssim_mlp_residual": (c_int, [c_int, _P, _P, _P, _P, _P, _P, c_double, c_int,

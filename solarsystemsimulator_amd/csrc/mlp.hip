@@ -516,12 +516,14 @@ __global__ __launch_bounds__(MB) void k_to_bf16(int n, const float* __restrict__
 // (times 2, in [-1, 1]), 1 for a mass (p < n_mass, its own mass is in the
 // stencil) or 0 for a tracer, and a zero; BF16 [nb][32].
 __global__ __launch_bounds__(MB) void k_mlp_features(
-    int nb, int n_mass, const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, int n, double ox, double oy, double oz,
+    int nb, int n_mass, const int* __restrict__ order, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, int n, double ox,
+    double oy, double oz,
     double h, const float* __restrict__ rho32, const double* __restrict__ rho64,
     double inv_unit, unsigned short* __restrict__ feat) {
-  const int p = blockIdx.x * MB + threadIdx.x;
-  if (p >= nb) return;
+  const int i_ = blockIdx.x * MB + threadIdx.x;
+  if (i_ >= nb) return;
+  const int p = order ? order[i_] : i_;
   const double f[3] = {(x[p] - ox) / h, (y[p] - oy) / h, (z[p] - oz) / h};
   int c[3];
   float off[3];
@@ -577,8 +579,9 @@ int ssim_mlp_to_bf16(const float* params, unsigned short* wbf, void* stream) {
   return SSIM_OK;
 }
 
-int ssim_mlp_features(int nb, int n_mass, const double* x, const double* y,
-                      const double* z, int mesh_n, const double* origin, double h,
+int ssim_mlp_features(int nb, int n_mass, const int* order, const double* x,
+                      const double* y, const double* z, int mesh_n,
+                      const double* origin, double h,
                       int mesh_prec, const void* rho, double mass_unit,
                       unsigned short* feat, void* stream) {
   SSIM_REQUIRE(nb >= 0 && mesh_n >= 1 && h > 0.0 && mass_unit > 0.0,
@@ -589,7 +592,7 @@ int ssim_mlp_features(int nb, int n_mass, const double* x, const double* y,
   hipStream_t st = ssim::as_stream(stream);
   SSIM_TIMED("k_mlp_features", st,
              hipLaunchKernelGGL(k_mlp_features, dim3(ssim::ceil_div(nb, MB)),
-                                dim3(MB), 0, st, nb, n_mass, x, y, z, mesh_n,
+                                dim3(MB), 0, st, nb, n_mass, order, x, y, z, mesh_n,
                                 origin[0], origin[1], origin[2], h,
                                 f64 ? nullptr : static_cast<const float*>(rho),
                                 f64 ? static_cast<const double*>(rho) : nullptr,

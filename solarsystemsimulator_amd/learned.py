@@ -76,15 +76,19 @@ class LearnedCorrection:
         return self.params.cpu().numpy().copy()
 
     # -- features / inference ---------------------------------------------
-    def features(self, x, mesh, origin, h, mesh_prec, out=None, n_mass=0):
+    def features(self, x, mesh, origin, h, mesh_prec, out=None, n_mass=0,
+                 order=None):
         """BF16 feature rows [nb][32] (int16 storage) of the bodies x, the
-        first n_mass of which are masses."""
+        first n_mass of which are masses; `order` (int32 device array) is
+        the order the threads visit the bodies in (a cell-sorted order reads
+        the mesh with locality), None for index order."""
         torch = _torch()
         nb = x[0].numel()
         if out is None:
             out = torch.empty((nb, NIN), dtype=torch.int16, device=self.dev)
         org = _lib.double_array(origin)
-        _lib.call("ssim_mlp_features", nb, int(n_mass), *[_lib.ptr(c) for c in x],
+        _lib.call("ssim_mlp_features", nb, int(n_mass), _lib.ptr(order),
+                  *[_lib.ptr(c) for c in x],
                   mesh.shape[0], ctypes.addressof(org), float(h), mesh_prec,
                   _lib.ptr(mesh), float(self.mass_unit), _lib.ptr(out),
                   D.stream(self.dev))

@@ -119,13 +119,25 @@ class SplitForce:
             self.kim.append(kim)
 
     # -- force --------------------------------------------------------------
-    def mesh_accel(self, x, n_mass, m, a):
-        """a = long-range mesh acceleration at all bodies (overwrites a)."""
+    def place(self, x):
+        """Mesh placement for the bodies x (rebuilds the kernel spectra when
+        the cell size changes)."""
         level, h, origin, box = self._place(x)
         if level != self.level:
             self._rebuild_kernels(h)
             self.level = level
         self.h, self.origin, self.box = h, origin, box
+
+    def visit_order(self):
+        """Cell-sorted body order of the last bin_bodies (None before)."""
+        return self.torder if self._grid is not None else None
+
+    def mesh_accel(self, x, n_mass, m, a, order=None, placed=False):
+        """a = long-range mesh acceleration at all bodies (overwrites a);
+        `order`: the order the gather visits the bodies in."""
+        if not placed:
+            self.place(x)
+        h, origin = self.h, self.origin
         src = [c[:n_mass] for c in x]
         st = D.stream(self.dev)
         if self.assign == "cic":
@@ -140,7 +152,8 @@ class SplitForce:
         else:
             org = _lib.double_array(origin)
             fs = ([self.f4, None, None] if self.f4 is not None else self.fields)
-            _lib.call("ssim_tsc_gather", x[0].numel(), *[_lib.ptr(c) for c in x],
+            _lib.call("ssim_tsc_gather", x[0].numel(), _lib.ptr(order),
+                      *[_lib.ptr(c) for c in x],
                       self.n, P._addr(org), h, self.prec,
                       *[_lib.ptr(f) for f in fs], *[_lib.ptr(c) for c in a], st)
         return a
@@ -155,7 +168,9 @@ class SplitForce:
             i32 = dict(dtype=torch.int32, device=self.dev)
             self.bin_start = torch.empty(nn + 1, **i32)
             self.bin_order = torch.empty(max(nm, 1), **i32)
-            self.bwork = D.workspace(_lib.load().ssim_cells_work_bytes(nm, nn),
+            lib = _lib.load()
+            self.bwork = D.workspace(max(lib.ssim_cells_work_bytes(nm, nn),
+                                         lib.ssim_tsc_deposit_work_bytes(nm)),
                                      self.dev)
             self._bin_n = nm
         gn = _lib.int_array([self.n] * 3)
@@ -165,9 +180,10 @@ class SplitForce:
                   P._addr(org_b), h, P._addr(gn), _lib.ptr(self.bin_start),
                   _lib.ptr(self.bin_order), None, _lib.ptr(self.bwork),
                   self.bwork.numel(), st)
-        _lib.call("ssim_tsc_deposit_binned", *[_lib.ptr(c) for c in src],
+        _lib.call("ssim_tsc_deposit_binned", nm, *[_lib.ptr(c) for c in src],
                   _lib.ptr(m), self.n, P._addr(org), h, _lib.ptr(self.bin_start),
-                  _lib.ptr(self.bin_order), self.prec, _lib.ptr(self.rho), st)
+                  _lib.ptr(self.bin_order), self.prec, _lib.ptr(self.rho),
+                  _lib.ptr(self.bwork), self.bwork.numel(), st)
 
     def convolve(self, interleaved=False):
         """fields = long-range acceleration meshes of self.rho (or, with
@@ -269,12 +285,17 @@ class SplitForce:
 
     _np = -1
     _nt = -1
+    _grid = None
 
     def accelerations(self, x, n_mass, m, a):
         """a = mesh + near-field acceleration of all bodies in x (n_mass
-        masses first, tracers after them)."""
-        self.mesh_accel(x, n_mass, m, a)
-        self.near_accel(x, n_mass, m, a)
+        masses first, tracers after them).  The near-field cell lists are
+        built first, so the mesh gather (and the learned features) visit the
+        bodies in cell order."""
+        self.place(x)
+        self.bin_bodies(x, n_mass, m)
+        self.mesh_accel(x, n_mass, m, a, order=self.torder, placed=True)
+        self.near_sum(x, a)
         return a
 
 

@@ -125,16 +125,18 @@ def test_tsc_deposit_and_gather_match_oracle(dev):
     binned = []
     for _ in range(2):
         mb = torch.full((n, n, n), 7.0, dtype=torch.float64, device=dev)
-        _lib.call("ssim_tsc_deposit_binned", *[_lib.ptr(c) for c in x], _lib.ptr(md),
-                  n, ctypes.addressof(org), h, _lib.ptr(start), _lib.ptr(order), 0,
-                  _lib.ptr(mb), D.stream(dev))
+        bw = D.workspace(_lib.load().ssim_tsc_deposit_work_bytes(30000), dev)
+        _lib.call("ssim_tsc_deposit_binned", 30000, *[_lib.ptr(c) for c in x],
+                  _lib.ptr(md), n, ctypes.addressof(org), h, _lib.ptr(start),
+                  _lib.ptr(order), 0, _lib.ptr(mb), _lib.ptr(bw), bw.numel(),
+                  D.stream(dev))
         binned.append(mb.cpu().numpy())
     assert np.array_equal(binned[0], binned[1])
     np.testing.assert_allclose(binned[0], ref, rtol=1e-12, atol=1e-12 * ref.max())
     fields = [rng.normal(size=(n, n, n)) for _ in range(3)]
     fd = [D.to_dev(f, dev) for f in fields]        # keep the device copies alive
     out = [torch.empty(30000, dtype=torch.float64, device=dev) for _ in range(3)]
-    _lib.call("ssim_tsc_gather", 30000, *[_lib.ptr(c) for c in x], n,
+    _lib.call("ssim_tsc_gather", 30000, None, *[_lib.ptr(c) for c in x], n,
               ctypes.addressof(org), h, 0, *[_lib.ptr(f) for f in fd],
               *[_lib.ptr(o) for o in out], D.stream(dev))
     for c in range(3):
@@ -148,14 +150,22 @@ def test_tsc_deposit_and_gather_match_oracle(dev):
         f4[:, c] = f32[c].reshape(-1)
     o3 = [torch.empty(30000, dtype=torch.float64, device=dev) for _ in range(3)]
     o4 = [torch.empty(30000, dtype=torch.float64, device=dev) for _ in range(3)]
-    _lib.call("ssim_tsc_gather", 30000, *[_lib.ptr(c) for c in x], n,
+    _lib.call("ssim_tsc_gather", 30000, None, *[_lib.ptr(c) for c in x], n,
               ctypes.addressof(org), h, 1, *[_lib.ptr(f) for f in f32],
               *[_lib.ptr(o) for o in o3], D.stream(dev))
-    _lib.call("ssim_tsc_gather", 30000, *[_lib.ptr(c) for c in x], n,
+    _lib.call("ssim_tsc_gather", 30000, None, *[_lib.ptr(c) for c in x], n,
               ctypes.addressof(org), h, 1, _lib.ptr(f4), None, None,
               *[_lib.ptr(o) for o in o4], D.stream(dev))
     for c in range(3):
         assert torch.equal(o3[c], o4[c])
+    # any iteration order gives the same per-body results
+    perm = D.to_dev(np.random.default_rng(7).permutation(30000), dev, torch.int32)
+    o5 = [torch.empty(30000, dtype=torch.float64, device=dev) for _ in range(3)]
+    _lib.call("ssim_tsc_gather", 30000, _lib.ptr(perm), *[_lib.ptr(c) for c in x], n,
+              ctypes.addressof(org), h, 1, _lib.ptr(f4), None, None,
+              *[_lib.ptr(o) for o in o5], D.stream(dev))
+    for c in range(3):
+        assert torch.equal(o3[c], o5[c])
 
 
 @pytest.mark.parametrize("assign", ["cic", "tsc"])
