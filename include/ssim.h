@@ -261,15 +261,15 @@ int ssim_tsc_deposit_binned(int np, const double* x, const double* y,
                             const int* bin_order, int mesh_prec, void* mesh,
                             void* work, size_t work_bytes, void* stream);
 /* TSC interpolation of three meshes at np points: o_c = sum w f_c
- * (overwrites o_c, FP64); thread i handles point order[i] (order = NULL:
+ * (overwrites o_c, FP64, or adds to it when accumulate != 0); thread i handles point order[i] (order = NULL:
  * point i) — a cell-sorted order makes neighbouring threads read
  * neighbouring nodes.  f1 = f2 = NULL: f0 holds the three FP32 fields
  * interleaved as float4 [grid_n^3] (ssim_pm_conv_isolated3's layout). */
 int ssim_tsc_gather(int np, const int* order, const double* x, const double* y,
                     const double* z, int grid_n, const double* origin, double h,
-                    int mesh_prec,
-                    const void* f0, const void* f1, const void* f2, double* o0,
-                    double* o1, double* o2, void* stream);
+                    int mesh_prec, const void* f0, const void* f1, const void* f2,
+                    double* o0, double* o1, double* o2, int accumulate,
+                    void* stream);
 /* out = a * b over ncomplex interleaved complex values (the spectral
  * products of pm_force_field, pm_kernel.py:144-146). */
 int ssim_spectrum_mul(long long ncomplex, int prec, const void* a,

@@ -1,0 +1,165 @@
+"""Throughput of the other benchmark configurations on one MI355X.
+
+    python -m solarsystemsimulator_amd.bench_configs [--quick]
+
+Prints one JSON object with a record per configuration (the headline
+250k-mass comparison is bench.py's):
+
+* solar_system — star + 8 planets + 10,000 test particles (2-D), raster
+  integrator (frame kernel, Yoshida-4 sub-steps of <= 0.5 day), dt = 1 day,
+  1000 frames: frames/s and body-steps/s;
+* allpairs_65k — 65,536 equal masses in the perturbed unit cube, Yoshida-4,
+  dt = 1e-3, eps = 1e-3: ms/step in FP64 and in FP32 with compensated
+  sums, and the FP32 path's force error against FP64 (median / p99);
+* periodic_pm_262k — 262,144 bodies in the periodic cube, 128^3 mesh, CIC
+  deposit, R2C Poisson, central-difference gradient, CIC gather: ms per
+  force evaluation and per leapfrog step, FP32 and FP64 mesh;
+* collisions_50k — star + 4 planets + 50,000 asteroids in a cloud at
+  2.5 AU: raster frames/s of the frame kernel, and the uniform-grid
+  collision search of one sample (candidates, contact pairs, ms).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import _lib, scenes
+from . import device as D
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed(fn, reps=1):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps, out
+
+
+def solar_system(frames):
+    from .nbody import NBodyCuda
+    from .constants import K_MAX
+    x, y, vx, vy, m, vis, ast = scenes.solar_system(10_000, seed=1)
+    sim = NBodyCuda(0)
+    st = sim.load_state(x, y, vx, vy, m, vis, ast)
+    day = 1.0 / 365.25
+    sim.step_batch(st, day, K_MAX)                        # warm-up
+    batches = max(1, frames // K_MAX)
+    t, _ = timed(lambda: [sim.step_batch(st, day, K_MAX) for _ in range(batches)])
+    nf = batches * K_MAX
+    return {"bodies": len(x), "frames": nf, "dt": "1 day", "s": t,
+            "frames_per_s": nf / t, "body_frames_per_s": nf * len(x) / t}
+
+
+def allpairs_65k(steps):
+    from .selfgrav import SelfGravSystem
+    pos, m = scenes.selfgrav_cube()
+    n = len(m)
+    zeros = [np.zeros(n)] * 3
+    dev = D.require_gpu(0)
+    out = {"bodies": n, "steps": steps, "scheme": "yoshida4", "dt": 1e-3}
+    acc = {}
+    for name, prec in (("f64", _lib.PREC_F64), ("f32c", _lib.PREC_F32C)):
+        s = SelfGravSystem(dev, 3, n, 0, prec, _lib.SCHEME_YOSHIDA4, 1.0, 1e-6)
+        s.set_bodies(pos, zeros, m)
+        acc[name] = np.array([c.cpu().numpy() for c in s.accelerations()])
+        s.run(1e-3, 1)
+        t, _ = timed(lambda: s.run(1e-3, steps))
+        out[f"ms_per_step_{name}"] = t / steps * 1e3
+    err = (np.linalg.norm(acc["f32c"] - acc["f64"], axis=0) /
+           np.linalg.norm(acc["f64"], axis=0))
+    out["f32c_force_err_median"] = float(np.median(err))
+    out["f32c_force_err_p99"] = float(np.percentile(err, 99))
+    return out
+
+
+def periodic_pm(steps):
+    import torch
+    from . import pm as P
+    dev = D.require_gpu(0)
+    pos = scenes.perturbed_cube(64, 64, 64, seed=3, periodic=True)
+    n = pos[0].size
+    m = D.to_dev(np.full(n, 1.0 / n), dev)
+    out = {"bodies": n, "mesh": 128, "steps": steps}
+    for name, prec in (("f64", P.F64), ("f32", P.F32)):
+        x = [D.to_dev(c, dev) for c in pos]
+        v = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3)]
+        a = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3)]
+        solver = P.PeriodicPM3D(dev, 128, prec=prec)
+        solver.accelerations(x, m, x, outs=a)
+        t_eval, _ = timed(lambda: solver.accelerations(x, m, x, outs=a), 5)
+        st = D.stream(dev)
+        ps = lambda lst: [_lib.ptr(c) for c in lst]  # noqa: E731
+
+        def step(dt=1e-3):
+            _lib.call("ssim_kick", 3, n, *ps(v), *ps(a), 0.5 * dt, st)
+            _lib.call("ssim_drift", 3, n, *ps(x), *ps(v), dt, st)
+            for c in range(3):                              # periodic wrap
+                torch.remainder(x[c], 1.0, out=x[c])
+            solver.accelerations(x, m, x, outs=a)
+            _lib.call("ssim_kick", 3, n, *ps(v), *ps(a), 0.5 * dt, st)
+        step()                                              # warm-up
+        t_step, _ = timed(lambda: [step() for _ in range(steps)])
+        out[f"ms_per_eval_{name}"] = t_eval * 1e3
+        out[f"ms_per_step_{name}"] = t_step / steps * 1e3
+    return out
+
+
+def collisions_50k(frames):
+    import torch
+    from .constants import K_MAX
+    from .detection import Erkennungskarte, bounce_suche
+    from .nbody import NBodyCuda
+    x, y, vx, vy, m, rr, ast = scenes.asteroid_cloud(50_000, seed=4)
+    n = len(x)
+    vis = np.ones(n, np.uint8)
+    sim = NBodyCuda(0)
+    st = sim.load_state(x, y, vx, vy, m, vis, ast, real_r=rr)
+    day = 1.0 / 365.25
+    sim.step_batch(st, day, K_MAX)
+    batches = max(1, frames // K_MAX)
+    t, rows = timed(lambda: [sim.step_batch(st, day, K_MAX) for _ in range(batches)])
+    out = {"bodies": n, "frames": batches * K_MAX, "frames_per_s": batches * K_MAX / t}
+    dev = D.require_gpu(0)
+    det = Erkennungskarte(0, ast, -np.inf, np.inf)
+    det.stammdaten(vis, rr)
+    sample = rows[-1][-1]
+    g = det.hochladen(sample)
+    row = (g[0:n], g[n:2 * n], g[2 * n:3 * n], g[3 * n:4 * n])
+    bounce_suche([det], None, [row], day, float(rr.max()))
+    t, res = timed(lambda: bounce_suche([det], None, [row], day, float(rr.max())), 5)
+    hits, cand, h, _ = res
+    out.update({"search_ms": t * 1e3, "candidates": int(cand),
+                "contact_pairs": 0 if hits is None else int(len(hits[0])),
+                "cell_h_au": h})
+    del torch, dev
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--quick", action="store_true")
+    args = ap.parse_args(argv)
+    q = args.quick
+    res = {}
+    for name, fn, arg in (("solar_system", solar_system, 160 if q else 1000),
+                          ("allpairs_65k", allpairs_65k, 5 if q else 20),
+                          ("periodic_pm_262k", periodic_pm, 5 if q else 50),
+                          ("collisions_50k", collisions_50k, 32 if q else 160)):
+        log(f"== {name}")
+        res[name] = fn(arg)
+        log(json.dumps(res[name]))
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

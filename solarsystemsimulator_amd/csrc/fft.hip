@@ -591,7 +591,10 @@ int conv_isolated3(int n, const F* rho, const F* const* kim, F* const* fields,
   const unsigned char* twy = tw + tw_offset(3, dims, 1, prec);
   const unsigned char* twz = tw + tw_offset(3, dims, 2, prec);
   const long long rows = (long long)n * n;
-  Pass fz = row_pass(rows, P, n, H, tile_elems<F>());
+  // row passes use half tiles: twice the workgroups, so loads, butterflies
+  // and stores of different workgroups overlap (a full-tile row pass of the
+  // 128^2 rows is a single round of 1024 workgroups)
+  Pass fz = row_pass(rows, P, n, H, tile_elems<F>() / 2);
   fz.R1 = n;
   fz.s_in_hi = rows;
   fz.s_out_hi = (long long)P * H;
@@ -613,7 +616,7 @@ int conv_isolated3(int n, const F* rho, const F* const* kim, F* const* fields,
     iy.n_outer = n;
     iy.out_count = n;
     SSIM_TRY((launch_pass<F, MODE_C2C>(tmp, tmp, iy, twy, 1, F(1), st)));
-    Pass iz = row_pass(rows, P, H, n, tile_elems<F>());
+    Pass iz = row_pass(rows, P, H, n, tile_elems<F>() / 2);
     iz.R1 = n;
     iz.s_in_hi = (long long)P * H;
     iz.s_out_hi = rows;

@@ -559,10 +559,12 @@ __global__ __launch_bounds__(MB) void k_mlp_features(
 }
 
 int grid_waves(int nb) {
-  // enough waves for all 256 CUs but no more than one 16-body batch each
+  // workgroups: one per 16-body batch per wave, at most 1024 (4096 waves,
+  // about one resident set on 256 CUs): every wave then loads its 88
+  // registers of weights once for several batches
   const int batches = ssim::ceil_div(nb, 16);
   int wg = ssim::ceil_div(batches, WAVES);
-  return wg < 1 ? 1 : (wg > 4096 ? 4096 : wg);
+  return wg < 1 ? 1 : (wg > 1024 ? 1024 : wg);
 }
 
 }  // namespace

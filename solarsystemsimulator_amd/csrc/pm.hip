@@ -290,7 +290,7 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather(
     int np, const int* __restrict__ order, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, Grid g, const M* __restrict__ f0,
     const M* __restrict__ f1, const M* __restrict__ f2, double* __restrict__ o0,
-    double* __restrict__ o1, double* __restrict__ o2) {
+    double* __restrict__ o1, double* __restrict__ o2, int accumulate) {
   const int i_ = blockIdx.x * PM_BLOCK + threadIdx.x;
   if (i_ >= np) return;
   const int p = order ? order[i_] : i_;
@@ -318,9 +318,15 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather(
       }
     }
   }
-  o0[p] = v0;
-  o1[p] = v1;
-  o2[p] = v2;
+  if (accumulate) {
+    o0[p] += v0;
+    o1[p] += v1;
+    o2[p] += v2;
+  } else {
+    o0[p] = v0;
+    o1[p] = v1;
+    o2[p] = v2;
+  }
 }
 
 // The same gather from three FP32 fields interleaved as float4 [n^3]
@@ -328,7 +334,8 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather(
 __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
     int np, const int* __restrict__ order, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, Grid g, const float4* __restrict__ f,
-    double* __restrict__ o0, double* __restrict__ o1, double* __restrict__ o2) {
+    double* __restrict__ o0, double* __restrict__ o1, double* __restrict__ o2,
+    int accumulate) {
   const int i_ = blockIdx.x * PM_BLOCK + threadIdx.x;
   if (i_ >= np) return;
   const int p = order ? order[i_] : i_;
@@ -360,9 +367,15 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
       }
     }
   }
-  o0[p] = v0;
-  o1[p] = v1;
-  o2[p] = v2;
+  if (accumulate) {
+    o0[p] += v0;
+    o1[p] += v1;
+    o2[p] += v2;
+  } else {
+    o0[p] = v0;
+    o1[p] = v1;
+    o2[p] = v2;
+  }
 }
 
 // One workgroup folds the per-block extrema (min and max are exact, so the
@@ -807,9 +820,10 @@ int ssim_tsc_deposit_binned(int np, const double* x, const double* y,
 }
 
 int ssim_tsc_gather(int np, const int* order, const double* x, const double* y,
-                    const double* z, int grid_n, const double* origin, double h, int mesh_prec,
-                    const void* f0, const void* f1, const void* f2, double* o0,
-                    double* o1, double* o2, void* stream) {
+                    const double* z, int grid_n, const double* origin, double h,
+                    int mesh_prec, const void* f0, const void* f1, const void* f2,
+                    double* o0, double* o1, double* o2, int accumulate,
+                    void* stream) {
   SSIM_REQUIRE(np >= 0 && grid_n >= 1 && h > 0.0, "bad TSC arguments");
   if (np == 0) return SSIM_OK;
   SSIM_REQUIRE(x && y && z && origin && f0 && o0 && o1 && o2, "null pointer");
@@ -824,7 +838,8 @@ int ssim_tsc_gather(int np, const int* order, const double* x, const double* y,
     SSIM_TIMED("k_tsc_gather", st,
                hipLaunchKernelGGL(k_tsc_gather4, dim3(ssim::ceil_div(np, PM_BLOCK)),
                                   dim3(PM_BLOCK), 0, st, np, order, x, y, z, g,
-                                  static_cast<const float4*>(f0), o0, o1, o2));
+                                  static_cast<const float4*>(f0), o0, o1, o2,
+                                  accumulate));
     SSIM_LAUNCHED("k_tsc_gather4");
     return SSIM_OK;
   }
@@ -837,13 +852,15 @@ int ssim_tsc_gather(int np, const int* order, const double* x, const double* y,
                hipLaunchKernelGGL(k_tsc_gather<double>, grid, dim3(PM_BLOCK), 0, st,
                                   np, order, x, y, z, g, static_cast<const double*>(f0),
                                   static_cast<const double*>(f1),
-                                  static_cast<const double*>(f2), o0, o1, o2));
+                                  static_cast<const double*>(f2), o0, o1, o2,
+                                  accumulate));
   else
     SSIM_TIMED("k_tsc_gather", st,
                hipLaunchKernelGGL(k_tsc_gather<float>, grid, dim3(PM_BLOCK), 0, st,
                                   np, order, x, y, z, g, static_cast<const float*>(f0),
                                   static_cast<const float*>(f1),
-                                  static_cast<const float*>(f2), o0, o1, o2));
+                                  static_cast<const float*>(f2), o0, o1, o2,
+                                  accumulate));
   SSIM_LAUNCHED("k_tsc_gather");
   return SSIM_OK;
 }

@@ -132,31 +132,40 @@ class SplitForce:
         """Cell-sorted body order of the last bin_bodies (None before)."""
         return self.torder if self._grid is not None else None
 
+    def mesh_fields(self, x, n_mass, m):
+        """Deposit the masses and convolve: the acceleration meshes."""
+        src = [c[:n_mass] for c in x]
+        if self.assign == "cic":
+            P.deposit(src, m[:n_mass], [self.n] * 3, self.origin, self.h, False,
+                      self.prec, mesh=self.rho)
+        else:
+            self._tsc_deposit(src, m[:n_mass], self.origin, self.h,
+                              D.stream(self.dev))
+        self.convolve(interleaved=self.f4 is not None)
+
+    def mesh_gather(self, x, a, order=None, accumulate=False):
+        """a (+)= the meshes interpolated at the bodies x."""
+        if self.assign == "cic":
+            if accumulate:
+                raise ValueError("the CIC gather overwrites")
+            P.gather(self.fields, x, [self.n] * 3, self.origin, self.h, False,
+                     self.prec, outs=a)
+            return a
+        org = _lib.double_array(self.origin)
+        fs = ([self.f4, None, None] if self.f4 is not None else self.fields)
+        _lib.call("ssim_tsc_gather", x[0].numel(), _lib.ptr(order),
+                  *[_lib.ptr(c) for c in x], self.n, P._addr(org), self.h,
+                  self.prec, *[_lib.ptr(f) for f in fs],
+                  *[_lib.ptr(c) for c in a], int(accumulate), D.stream(self.dev))
+        return a
+
     def mesh_accel(self, x, n_mass, m, a, order=None, placed=False):
         """a = long-range mesh acceleration at all bodies (overwrites a);
         `order`: the order the gather visits the bodies in."""
         if not placed:
             self.place(x)
-        h, origin = self.h, self.origin
-        src = [c[:n_mass] for c in x]
-        st = D.stream(self.dev)
-        if self.assign == "cic":
-            P.deposit(src, m[:n_mass], [self.n] * 3, origin, h, False, self.prec,
-                      mesh=self.rho)
-        else:
-            self._tsc_deposit(src, m[:n_mass], origin, h, st)
-        self.convolve(interleaved=self.f4 is not None)
-        if self.assign == "cic":
-            P.gather(self.fields, x, [self.n] * 3, origin, h, False, self.prec,
-                     outs=a)
-        else:
-            org = _lib.double_array(origin)
-            fs = ([self.f4, None, None] if self.f4 is not None else self.fields)
-            _lib.call("ssim_tsc_gather", x[0].numel(), _lib.ptr(order),
-                      *[_lib.ptr(c) for c in x],
-                      self.n, P._addr(org), h, self.prec,
-                      *[_lib.ptr(f) for f in fs], *[_lib.ptr(c) for c in a], st)
-        return a
+        self.mesh_fields(x, n_mass, m)
+        return self.mesh_gather(x, a, order)
 
     def _tsc_deposit(self, src, m, origin, h, st):
         """rho = TSC mass mesh, without atomics: the masses are binned by
@@ -289,14 +298,40 @@ class SplitForce:
 
     def accelerations(self, x, n_mass, m, a):
         """a = mesh + near-field acceleration of all bodies in x (n_mass
-        masses first, tracers after them).  The near-field cell lists are
-        built first, so the mesh gather (and the learned features) visit the
-        bodies in cell order."""
+        masses first, tracers after them).
+
+        With TSC the near field runs on a second stream while the current
+        one deposits and convolves: the near-field kernel is bound by the
+        vector ALU, the FFT passes by memory and latency, so they overlap.
+        The side stream zeroes a, builds the cell lists and adds the near
+        field; the current stream then waits for it and adds the mesh gather
+        (in cell order).  With CIC everything runs in order on one stream."""
+        import torch
         self.place(x)
-        self.bin_bodies(x, n_mass, m)
-        self.mesh_accel(x, n_mass, m, a, order=self.torder, placed=True)
-        self.near_sum(x, a)
+        if self.assign != "tsc":
+            self.bin_bodies(x, n_mass, m)
+            self.mesh_accel(x, n_mass, m, a, order=self.torder, placed=True)
+            self.near_sum(x, a)
+            return a
+        main = torch.cuda.current_stream(self.dev)
+        if self._side is None:
+            self._side = torch.cuda.Stream(device=self.dev)  # default priority
+        start = torch.cuda.Event()
+        start.record(main)
+        self._side.wait_event(start)
+        with torch.cuda.stream(self._side):
+            for c in a:
+                c.zero_()
+            self.bin_bodies(x, n_mass, m)
+            self.near_sum(x, a)
+        done = torch.cuda.Event()
+        done.record(self._side)
+        self.mesh_fields(x, n_mass, m)
+        main.wait_event(done)
+        self.mesh_gather(x, a, order=self.torder, accumulate=True)
         return a
+
+    _side = None
 
 
 def relative_errors(a, ref):

@@ -7,6 +7,10 @@
   into [0, 1).  Used for the 65,536-body all-pairs case (64x32x32, seed 2),
   the 262,144-body particle-mesh case (64^3, seed 3, periodic) and the
   250k-mass + 500k-tracer case (100x50x50 seed 5 / 100x100x50 seed 55).
+* `solar_system` — star + the 8 planets on Kepler ellipses + test
+  particles in the asteroid belt (2-D, AU, years, G = 4 pi^2).
+* `asteroid_cloud` — star + 4 planets + a Gaussian asteroid cloud at
+  2.5 AU for the collision search.
 """
 from __future__ import annotations
 
@@ -61,3 +65,71 @@ def learned_scene(mass_shape=(100, 50, 50), tracer_shape=(100, 100, 50),
     tpos = perturbed_cube(*tracer_shape, seed=tracer_seed, amp=amp)
     m = np.full(pos[0].size, 1.0 / pos[0].size)
     return pos, m, tpos
+
+
+# planet masses (solar masses) and semi-major axes (AU), Mercury..Neptune
+PLANETS = ((1.66e-7, 0.387), (2.45e-6, 0.723), (3.00e-6, 1.000),
+           (3.23e-7, 1.524), (9.55e-4, 5.203), (2.86e-4, 9.537),
+           (4.37e-5, 19.19), (5.15e-5, 30.07))
+G_AU = 4.0 * np.pi * np.pi
+
+
+def _kepler_2d(a, e, nu, omega, gm):
+    """Position and velocity on the ellipse (a, e) at true anomaly nu, with
+    periapsis direction omega (2-D)."""
+    p = a * (1.0 - e * e)
+    r = p / (1.0 + e * np.cos(nu))
+    vr = np.sqrt(gm / p) * e * np.sin(nu)
+    vt = np.sqrt(gm / p) * (1.0 + e * np.cos(nu))
+    ang = nu + omega
+    c, s = np.cos(ang), np.sin(ang)
+    return r * c, r * s, vr * c - vt * s, vr * s + vt * c
+
+
+def solar_system(n_test: int = 10_000, seed: int = 1):
+    """(x, y, vx, vy, mass, visible, is_ast): star of mass 1 at rest, the
+    8 planets with e ~ U[0, 0.1] and random phases, n_test massless test
+    particles with a ~ U[2.1, 3.3] AU, e ~ U[0, 0.2], random angles."""
+    rng = np.random.default_rng(seed)
+    m = np.array([1.0] + [pm for pm, _ in PLANETS] + [0.0] * n_test)
+    a = np.array([pa for _, pa in PLANETS] + list(rng.uniform(2.1, 3.3, n_test)))
+    e = np.concatenate([rng.uniform(0.0, 0.1, 8), rng.uniform(0.0, 0.2, n_test)])
+    nu = rng.uniform(0.0, 2 * np.pi, 8 + n_test)
+    om = rng.uniform(0.0, 2 * np.pi, 8 + n_test)
+    x, y, vx, vy = _kepler_2d(a, e, nu, om, G_AU)
+    x, y = np.concatenate([[0.0], x]), np.concatenate([[0.0], y])
+    vx, vy = np.concatenate([[0.0], vx]), np.concatenate([[0.0], vy])
+    n = len(m)
+    visible = np.ones(n, np.uint8)
+    is_ast = np.zeros(n, np.uint8)
+    is_ast[9:] = 1
+    return x, y, vx, vy, m, visible, is_ast
+
+
+def asteroid_cloud(n: int = 50_000, seed: int = 4):
+    """Star (mass 1) + Mercury..Mars + n asteroids in a Gaussian cloud
+    (sigma 0.05 AU) at 2.5 AU moving with the circular velocity, dispersion
+    0.02 of it; radii log-uniform in [1e-7, 1e-5] AU, densities U[1, 3]
+    (masses from radius and density in solar masses per AU^3 scaled to
+    g/cm^3).  Returns (x, y, vx, vy, mass, radius, is_ast)."""
+    rng = np.random.default_rng(seed)
+    vc = np.sqrt(G_AU / 2.5)
+    ax = 2.5 + rng.normal(0.0, 0.05, n)
+    ay = rng.normal(0.0, 0.05, n)
+    avx = rng.normal(0.0, 0.02 * vc, n)
+    avy = vc + rng.normal(0.0, 0.02 * vc, n)
+    rad = np.exp(rng.uniform(np.log(1e-7), np.log(1e-5), n))
+    rho = rng.uniform(1.0, 3.0, n)                       # g / cm^3
+    au_cm, msun_g = 1.495978707e13, 1.98847e33
+    am = 4.0 / 3.0 * np.pi * (rad * au_cm) ** 3 * rho / msun_g
+    px, py, pvx, pvy = _kepler_2d(np.array([pa for _, pa in PLANETS[:4]]),
+                                  np.zeros(4), rng.uniform(0, 2 * np.pi, 4),
+                                  np.zeros(4), G_AU)
+    x = np.concatenate([[0.0], px, ax])
+    y = np.concatenate([[0.0], py, ay])
+    vx = np.concatenate([[0.0], pvx, avx])
+    vy = np.concatenate([[0.0], pvy, avy])
+    mass = np.concatenate([[1.0], [pm for pm, _ in PLANETS[:4]], am])
+    radius = np.concatenate([[0.00465], [1.6e-5, 4.0e-5, 4.3e-5, 2.3e-5], rad])
+    is_ast = np.concatenate([np.zeros(5, np.uint8), np.ones(n, np.uint8)])
+    return x, y, vx, vy, mass, radius, is_ast

@@ -138,7 +138,7 @@ def test_tsc_deposit_and_gather_match_oracle(dev):
     out = [torch.empty(30000, dtype=torch.float64, device=dev) for _ in range(3)]
     _lib.call("ssim_tsc_gather", 30000, None, *[_lib.ptr(c) for c in x], n,
               ctypes.addressof(org), h, 0, *[_lib.ptr(f) for f in fd],
-              *[_lib.ptr(o) for o in out], D.stream(dev))
+              *[_lib.ptr(o) for o in out], 0, D.stream(dev))
     for c in range(3):
         np.testing.assert_allclose(out[c].cpu().numpy(),
                                    p3m_np.tsc_gather(fields[c], pos, n, origin, h),
@@ -152,10 +152,10 @@ def test_tsc_deposit_and_gather_match_oracle(dev):
     o4 = [torch.empty(30000, dtype=torch.float64, device=dev) for _ in range(3)]
     _lib.call("ssim_tsc_gather", 30000, None, *[_lib.ptr(c) for c in x], n,
               ctypes.addressof(org), h, 1, *[_lib.ptr(f) for f in f32],
-              *[_lib.ptr(o) for o in o3], D.stream(dev))
+              *[_lib.ptr(o) for o in o3], 0, D.stream(dev))
     _lib.call("ssim_tsc_gather", 30000, None, *[_lib.ptr(c) for c in x], n,
               ctypes.addressof(org), h, 1, _lib.ptr(f4), None, None,
-              *[_lib.ptr(o) for o in o4], D.stream(dev))
+              *[_lib.ptr(o) for o in o4], 0, D.stream(dev))
     for c in range(3):
         assert torch.equal(o3[c], o4[c])
     # any iteration order gives the same per-body results
@@ -163,9 +163,14 @@ def test_tsc_deposit_and_gather_match_oracle(dev):
     o5 = [torch.empty(30000, dtype=torch.float64, device=dev) for _ in range(3)]
     _lib.call("ssim_tsc_gather", 30000, _lib.ptr(perm), *[_lib.ptr(c) for c in x], n,
               ctypes.addressof(org), h, 1, _lib.ptr(f4), None, None,
-              *[_lib.ptr(o) for o in o5], D.stream(dev))
+              *[_lib.ptr(o) for o in o5], 0, D.stream(dev))
     for c in range(3):
         assert torch.equal(o3[c], o5[c])
+    _lib.call("ssim_tsc_gather", 30000, None, *[_lib.ptr(c) for c in x], n,
+              ctypes.addressof(org), h, 1, _lib.ptr(f4), None, None,
+              *[_lib.ptr(o) for o in o5], 1, D.stream(dev))      # accumulate
+    for c in range(3):
+        assert torch.equal(o5[c], 2.0 * o3[c])
 
 
 @pytest.mark.parametrize("assign", ["cic", "tsc"])
@@ -188,6 +193,18 @@ def test_split_force_assignments_on_cube(dev, assign):
         sf.accelerations(x, n, md, a)
         got = np.array([c.cpu().numpy()[idx] for c in a])
         errs[a_] = np.median(relative_errors(got, ref))
+        # the two-stream evaluation (TSC) equals mesh then near field in
+        # order; bit for bit, as the binned TSC deposit is reproducible (the
+        # CIC deposit's atomics are not, to rounding)
+        b = [torch.zeros_like(c) for c in a]
+        sf.mesh_accel(x, n, md, b)
+        sf.near_accel(x, n, md, b)
+        for c in range(3):
+            if a_ == "tsc":
+                assert torch.equal(a[c], b[c])
+            else:
+                scale = float(a[c].abs().max())
+                torch.testing.assert_close(a[c], b[c], rtol=1e-12, atol=1e-12 * scale)
     assert errs["cic"] < 1e-2
     if assign == "tsc":
         assert errs["tsc"] < 0.5 * errs["cic"], errs
