@@ -97,19 +97,20 @@ def cpu_baseline(pos, m, tpos, eps2):
                       f"evaluation ({dt:.1f} s), extrapolated to all targets"}
 
 
-def mesh_pass_bytes(n, mesh_bytes=4):
-    """Algorithmic HBM bytes of the 12 pruned FFT passes of one isolated
-    convolution of an n^3 FP32 mesh (padded P = 2n, half spectra of H =
-    n + 1, complex 2 x mesh_bytes, kernel spectra real mesh_bytes)."""
+def conv_bytes(n, mesh_bytes=4):
+    """Algorithmic HBM bytes of one isolated convolution of an n^3 FP32 mesh
+    (padded P = 2n, half spectra of H = n + 1, complex 2 x mesh_bytes,
+    kernel spectra real mesh_bytes): the 8 pruned FFT passes (R2C rows,
+    y lines, and per component y lines + C2R rows) and the fused x-axis
+    kernel (x < n half in, three kernel spectra in, three x < n halves out)."""
     P, H = 2 * n, n + 1
     c = 2 * mesh_bytes
-    fwd = [n ** 3 * mesh_bytes + n * n * H * c,             # R2C rows from rho
-           n * n * H * c + n * P * H * c,                   # y lines, x < n
-           n * P * H * c + P * P * H * c]                   # x lines
-    inv = [P * P * H * (c + mesh_bytes) + n * P * H * c,    # x lines * i kim
-           n * P * H * c + n * n * H * c,                   # y lines
-           n * n * H * c + n ** 3 * mesh_bytes]             # C2R rows
-    return fwd + 3 * inv
+    passes = [n ** 3 * mesh_bytes + n * n * H * c,          # R2C rows from rho
+              n * n * H * c + n * P * H * c]                # y lines, x < n
+    passes += 3 * [n * P * H * c + n * n * H * c,           # y lines, x < n
+                   n * n * H * c + n ** 3 * mesh_bytes]     # C2R rows
+    x3 = n * P * H * c + 3 * P * P * H * mesh_bytes + 3 * n * P * H * c
+    return passes, x3
 
 
 def main():
@@ -207,7 +208,7 @@ def main():
     sf = fast.split
     sf.bin_bodies(fast.x, n, fast.m)
     tested, inside = sf.near_pair_counts(fast.x)
-    names = ["k_pp_cells", "k_fft_pass", "k_tsc_deposit", "k_tsc_gather",
+    names = ["k_pp_cells", "k_fft_pass", "k_conv_x3", "k_tsc_deposit", "k_tsc_gather",
              "k_cic_deposit", "k_cic_gather", "k_mlp_forward", "k_mlp_features"]
     lib.ssim_timing(1)
     fast.run(dt, args.profile_steps)
@@ -215,15 +216,18 @@ def main():
     lib.ssim_timing(0)
     steps_p = args.profile_steps
     n_mesh = sf.n
-    pass_bytes = mesh_pass_bytes(n_mesh)
+    pass_bytes, x3_bytes = conv_bytes(n_mesh)
     mlp_flops = 2 * nb * (64 * 32 + 2 * 64 * 64 + 16 * 64)
     work = {
         "k_pp_cells": ("valu", FLOPS_PER_NEAR_PAIR * inside, FP32_PEAK_TFLOPS,
                        "TFLOP/s", f"{FLOPS_PER_NEAR_PAIR} flop x {inside:.4g} "
                        f"pairs inside rcut ({tested:.4g} pairs tested)"),
         "k_fft_pass": ("hbm", sum(pass_bytes) / len(pass_bytes), HBM_PEAK_GBS,
-                       "GB/s", f"{sum(pass_bytes) / 1e6:.1f} MB over 12 pruned "
-                       f"passes of one {2 * n_mesh}^3 isolated convolution"),
+                       "GB/s", f"{sum(pass_bytes) / 1e6:.1f} MB over the 8 pruned "
+                       f"FFT passes of one {2 * n_mesh}^3 isolated convolution"),
+        "k_conv_x3": ("hbm", x3_bytes, HBM_PEAK_GBS, "GB/s",
+                      f"{x3_bytes / 1e6:.1f} MB: x < {n_mesh} half in, 3 kernel "
+                      f"spectra in, 3 x < {n_mesh} halves out"),
         "k_tsc_deposit": ("hbm", n * 32 + 27 * 4 * n, HBM_PEAK_GBS, "GB/s",
                           "32 B/mass in + 27 FP32 node updates"),
         "k_tsc_gather": ("hbm", nb * (24 + 24) + 3 * 4 * n_mesh ** 3,

@@ -325,7 +325,9 @@ int ssim_cells_build(int n, const double* x, const double* y, const double* z,
                      void* work, size_t work_bytes, void* stream);
 /* The same binning without the fixed order inside a cell (slots handed
  * out by atomics): for lists whose consumer does not depend on the order,
- * such as the near-field targets.  No hipMemset inside. */
+ * such as the near-field targets.  A cell's bodies come grouped by the
+ * octant of the cell they lie in (x, then y, then z halves), so any run of
+ * consecutive bodies of a cell is spatially compact.  No hipMemset inside. */
 size_t ssim_cells_bin_work_bytes(int n, int ncells);
 int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
                    const double* origin, double cell, const int* nc,

@@ -41,6 +41,38 @@ __global__ __launch_bounds__(CB) void k_cell_key(
   atomicAdd(&count[k], 1);
 }
 
+// Key of the target binning: cell * 8 + octant of the body inside its cell
+// (bit 2: upper half in x, bit 1: in y, bit 0: in z), so the unstable
+// placement below leaves each cell's targets grouped by octant.
+__device__ __forceinline__ int octant_bit(double f, int a) {
+  return clampi((int)floor(2.0 * f) - 2 * a, 0, 1);
+}
+
+__global__ __launch_bounds__(CB) void k_cell_key8(
+    int n, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, double ox, double oy, double oz,
+    double inv_cell, int nc0, int nc1, int nc2, int* __restrict__ key,
+    int* __restrict__ count) {
+  const int i = blockIdx.x * CB + threadIdx.x;
+  if (i >= n) return;
+  const double fa = (x[i] - ox) * inv_cell, fb = (y[i] - oy) * inv_cell,
+               fc = (z[i] - oz) * inv_cell;
+  const int a = clampi((int)floor(fa), 0, nc0 - 1);
+  const int b = clampi((int)floor(fb), 0, nc1 - 1);
+  const int c = clampi((int)floor(fc), 0, nc2 - 1);
+  const int k = ((a * nc1 + b) * nc2 + c) * 8 + octant_bit(fa, a) * 4 +
+                octant_bit(fb, b) * 2 + octant_bit(fc, c);
+  key[i] = k;
+  atomicAdd(&count[k], 1);
+}
+
+// cell_start[c] = sub_start[8 c], c = 0..ncells
+__global__ __launch_bounds__(CB) void k_every8(int ncells, const int* __restrict__ sub,
+                                               int* __restrict__ cell_start) {
+  const int c = blockIdx.x * CB + threadIdx.x;
+  if (c <= ncells) cell_start[c] = sub[8 * (long long)c];
+}
+
 // Unstable placement: a cursor per cell hands out the slots.
 __global__ __launch_bounds__(CB) void k_scatter_any(int n, const int* __restrict__ key,
                                                     const int* __restrict__ start,
@@ -121,9 +153,10 @@ int ssim_cells_build(int n, const double* x, const double* y, const double* z,
 }
 
 size_t ssim_cells_bin_work_bytes(int n, int ncells) {
+  const size_t nsub = 8 * (size_t)(ncells > 0 ? ncells : 1);
   return ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int)) +
-         2 * ssim::align256((size_t)(ncells > 0 ? ncells : 1) * sizeof(int)) +
-         ssim::scan_work_bytes(ncells);
+         2 * ssim::align256(nsub * sizeof(int)) +
+         ssim::align256((nsub + 1) * sizeof(int)) + ssim::scan_work_bytes((int)nsub);
 }
 
 int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
@@ -140,27 +173,33 @@ int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
   SSIM_REQUIRE(work && work_bytes >= ssim_cells_bin_work_bytes(n, ncells),
                "workspace too small");
   hipStream_t st = ssim::as_stream(stream);
+  const int nsub = 8 * ncells;
   unsigned char* w = static_cast<unsigned char*>(work);
   int* key = reinterpret_cast<int*>(w);
   w += ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int));
   int* count = reinterpret_cast<int*>(w);
-  w += ssim::align256((size_t)ncells * sizeof(int));
+  w += ssim::align256((size_t)nsub * sizeof(int));
   int* cursor = reinterpret_cast<int*>(w);
-  w += ssim::align256((size_t)ncells * sizeof(int));
-  const dim3 gc(ssim::ceil_div(ncells, CB));
-  hipLaunchKernelGGL(k_zero_i32, gc, dim3(CB), 0, st, ncells, count);
-  hipLaunchKernelGGL(k_zero_i32, gc, dim3(CB), 0, st, ncells, cursor);
+  w += ssim::align256((size_t)nsub * sizeof(int));
+  int* sub_start = reinterpret_cast<int*>(w);
+  w += ssim::align256(((size_t)nsub + 1) * sizeof(int));
+  const dim3 gc(ssim::ceil_div(nsub, CB));
+  hipLaunchKernelGGL(k_zero_i32, gc, dim3(CB), 0, st, nsub, count);
+  hipLaunchKernelGGL(k_zero_i32, gc, dim3(CB), 0, st, nsub, cursor);
   SSIM_LAUNCHED("k_zero_i32");
   if (n > 0) {
-    hipLaunchKernelGGL(k_cell_key, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
+    hipLaunchKernelGGL(k_cell_key8, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
                        n, x, y, z, origin[0], origin[1], origin[2], 1.0 / cell,
-                       nc[0], nc[1], nc[2], key, order, count);
-    SSIM_LAUNCHED("k_cell_key");
+                       nc[0], nc[1], nc[2], key, count);
+    SSIM_LAUNCHED("k_cell_key8");
   }
-  SSIM_TRY(ssim::scan_exclusive(ncells, count, cell_start, w, st));
+  SSIM_TRY(ssim::scan_exclusive(nsub, count, sub_start, w, st));
+  hipLaunchKernelGGL(k_every8, dim3(ssim::ceil_div(ncells + 1, CB)), dim3(CB), 0, st,
+                     ncells, sub_start, cell_start);
+  SSIM_LAUNCHED("k_every8");
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter_any, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
-                       n, key, cell_start, cursor, order);
+                       n, key, sub_start, cursor, order);
     SSIM_LAUNCHED("k_scatter_any");
   }
   return SSIM_OK;

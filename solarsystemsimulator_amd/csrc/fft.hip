@@ -427,6 +427,85 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
   }
 }
 
+// ---- fused x axis of the isolated convolution (256-point lines) ----
+// One workgroup owns NL consecutive x lines (consecutive inner index, so
+// every row access is NL contiguous values).  Thread (l = t % NL, j = t / NL)
+// reads x[j + 16 n2], n2 < 8, of line l straight from HBM (x >= n = 128 is
+// the zero padding), and the forward four-step FFT leaves X[j + 16 k1] in
+// its registers.  For each component c it multiplies by i kim_c, runs the
+// inverse four-step from those registers (the DFT16 inputs of the inverse
+// are exactly the values the thread holds) and stores x = j + 16 k1 < n of
+// the result.  LDS only carries the two DFT16 exchanges; rows are padded to
+// 258 so the exchanges of NL lines are free of bank conflicts.  In HBM this
+// replaces four passes (forward x and three product + inverse x) by one
+// read of the x < n half, three reads of the kernel spectra and three
+// half-size writes.
+constexpr int HUB_LS = 258;
+
+template <typename F, int NL>
+__device__ __forceinline__ void hub_exchange(cpx<F> v[16], cpx<F>* s, int l, int j,
+                                             const cpx<F>* tw, bool inverse) {
+  // v = DFT16 outputs of index n1 = j; twiddle, exchange, DFT16 over n1
+#pragma unroll
+  for (int k2 = 1; k2 < 16; ++k2) {
+    cpx<F> w = tw[j * k2];
+    if (inverse) w.im = -w.im;
+    v[k2] = cmul(v[k2], w);
+  }
+  __syncthreads();  // the previous exchange's reads are done
+  cpx<F>* a = s + l * HUB_LS;
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) a[16 * j + (k2 ^ j)] = v[k2];
+  __syncthreads();
+#pragma unroll
+  for (int n1 = 0; n1 < 16; ++n1) v[n1] = a[16 * n1 + (j ^ n1)];
+  dft16(v, inverse);
+}
+
+template <typename F, int NL>
+__global__ __launch_bounds__(NL * 16) void k_conv_x3(
+    const cpx<F>* in, const F* __restrict__ kim0,  // in == out0: updated in place
+    const F* __restrict__ kim1, const F* __restrict__ kim2, cpx<F>* out0,
+    cpx<F>* out1, cpx<F>* out2, const cpx<F>* __restrict__ tw, int lines) {
+  // 256-point lines of the padded 2n = 256 mesh: x < 128 read and written;
+  // the host launches lines / NL workgroups (lines = 256 * 129 divides)
+  constexpr int L = 256;
+  __shared__ cpx<F> s[NL * HUB_LS];
+  __shared__ cpx<F> stw[L];
+  for (int t = threadIdx.x; t < L; t += NL * 16) stw[t] = tw[t];
+  __syncthreads();
+  const int l = threadIdx.x % NL, j = threadIdx.x / NL;
+  const int line = blockIdx.x * NL + l;
+  // element (x, line) at line + x * lines (< 2^31, checked by the host)
+  const unsigned int base = (unsigned int)line + (unsigned int)j * (unsigned int)lines;
+  const unsigned int step = 16u * (unsigned int)lines;
+  cpx<F> v[16];
+#pragma unroll
+  for (int n2 = 0; n2 < 8; ++n2) v[n2] = in[base + n2 * step];
+#pragma unroll
+  for (int n2 = 8; n2 < 16; ++n2) v[n2] = {F(0), F(0)};
+  dft16(v, false);
+  hub_exchange<F, NL>(v, s, l, j, stw, false);
+  // X[j + 16 k1] = v[k1]
+  cpx<F> X[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) X[k] = v[k];
+#pragma unroll 1
+  for (int c = 0; c < 3; ++c) {
+    const F* kim = c == 0 ? kim0 : (c == 1 ? kim1 : kim2);
+    cpx<F>* out = c == 0 ? out0 : (c == 1 ? out1 : out2);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) {
+      const F m = kim[base + k1 * step];
+      v[k1] = {-X[k1].im * m, X[k1].re * m};
+    }
+    dft16(v, true);
+    hub_exchange<F, NL>(v, s, l, j, stw, true);
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) out[base + k1 * step] = v[k1];
+  }
+}
+
 template <typename F>
 __global__ void k_fft_twiddles(int L, cpx<F>* tw) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -576,7 +655,10 @@ int fft_c2r(int ndim, const int* dims, cpx<F>* in, F* out,
 //            write z < n into the field, scaled by 1/P^3.
 // The padding never exists in memory and nothing outside the corner is
 // computed in the inverse: about 2.6x fewer HBM bytes than pad + rfftn +
-// 3 x (product + irfftn + crop).
+// 3 x (product + irfftn + crop).  With 256-point lines (n = 128) the
+// forward x lines and the three product + inverse x lines are one kernel
+// (k_conv_x3) that keeps each line's spectrum in registers; its outputs
+// go to spec (component 0, in place) and the two halves of tmp.
 // fields[1] == fields[2] == nullptr: the three results are interleaved in
 // fields[0] as [n^3][4] (component c at 4 i + c, slot 3 untouched).
 template <typename F>
@@ -604,10 +686,42 @@ int conv_isolated3(int n, const F* rho, const F* const* kim, F* const* fields,
   fy.n_outer = n;
   fy.in_count = n;
   SSIM_TRY((launch_pass<F, MODE_C2C>(spec, spec, fy, twy, 0, F(1), st)));
+  const F scale = F(1.0 / ((double)P * P * P));
+  if (P == 256) {
+    // fused x axis: component 0 in place in spec (x < n), components 1 and
+    // 2 in the two halves of tmp, each an [n][P][H] block
+    constexpr int NL = sizeof(F) == 4 ? 16 : 8;
+    const long long lines = (long long)P * H;
+    static_assert((256LL * 129) % NL == 0, "whole tiles of x lines");
+    cpx<F>* blk[3] = {spec, tmp, tmp + (long long)n * lines};
+    SSIM_TIMED("k_conv_x3", st,
+               hipLaunchKernelGGL((k_conv_x3<F, NL>), dim3((int)(lines / NL)),
+                                  dim3(NL * 16), 0, st, spec, kim[0], kim[1], kim[2],
+                                  blk[0], blk[1], blk[2],
+                                  reinterpret_cast<const cpx<F>*>(twx), (int)lines));
+    SSIM_LAUNCHED("k_conv_x3");
+    for (int c = 0; c < 3; ++c) {
+      Pass iy = strided_pass(3, cd, 1, tile_elems<F>());
+      iy.n_outer = n;
+      iy.out_count = n;
+      SSIM_TRY((launch_pass<F, MODE_C2C>(blk[c], blk[c], iy, twy, 1, F(1), st)));
+      Pass iz = row_pass(rows, P, H, n, tile_elems<F>() / 2);
+      iz.R1 = n;
+      iz.s_in_hi = (long long)P * H;
+      iz.s_out_hi = rows;
+      iz.out_count = n;
+      if (inter) {
+        iz.out_es = 4;
+        iz.out_c = c;
+      }
+      SSIM_TRY((launch_pass<F, MODE_C2R>(blk[c], inter ? fields[0] : fields[c], iz, twz,
+                                         1, scale, st)));
+    }
+    return SSIM_OK;
+  }
   Pass fx = strided_pass(3, cd, 0, tile_elems<F>());
   fx.in_count = n;
   SSIM_TRY((launch_pass<F, MODE_C2C>(spec, spec, fx, twx, 0, F(1), st)));
-  const F scale = F(1.0 / ((double)P * P * P));
   for (int c = 0; c < 3; ++c) {
     Pass ix = strided_pass(3, cd, 0, tile_elems<F>());
     ix.out_count = n;

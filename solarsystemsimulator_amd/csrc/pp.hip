@@ -12,8 +12,12 @@
 // wave takes a chunk, and all its lanes walk the same source list (the
 // three contiguous cells of each of the 9 neighbour rows, in the sources'
 // sorted order), so the source loads are wave-uniform (scalar loads) and
-// the loop has no divergence.  Each target's sum is made by one lane in the
-// fixed source order, so results do not depend on scheduling.
+// the loop has no divergence.  A source that lies beyond rcut of every
+// target of the chunk is skipped after the distance test (a wave-uniform
+// branch); the targets of a cell come grouped by octant (ssim_cells_bin),
+// so a cell's partial last chunk is compact and skips most of the
+// 27-cell block.  Each target's sum is made by one lane in the fixed source
+// order, so results do not depend on scheduling.
 //
 // Terms are FP32 on origin-relative coordinates (error ~1e-7 of each term,
 // far below the method's 1e-3 budget); the per-target sum is ADDED to the
@@ -28,9 +32,10 @@
 
 namespace {
 
-constexpr int PB = 256;            // 4 waves per workgroup
+constexpr int PB = 512;            // 8 waves share one LDS table
 constexpr int WAVES = PB / 64;
 constexpr int CHUNK = 64;          // targets per wave
+constexpr int SG = 4;              // sources loaded and tested per group
 constexpr int GT = 4096;           // intervals of the gS(sqrt(u)) table
 
 // tab[i] = (gS(r_i), gS(r_{i+1}) - gS(r_i)), r_i = rcut sqrt(i / GT)
@@ -59,6 +64,22 @@ __global__ __launch_bounds__(PB) void k_chunk_map(int ncells,
   for (int k = cstart[c]; k < cstart[c + 1]; ++k) map[k] = c;
 }
 
+// s += m d gS(r) / (r^2 + eps2)^(3/2) for a source inside rcut (in), else
+// exactly nothing; gS from the LDS table, linear in u = r^2
+__device__ __forceinline__ void pp_term(float m, float dx, float dy, float dz, float r2,
+                                        bool in, const float2* tab, float tab_scale,
+                                        float eps2, float& sx, float& sy, float& sz) {
+  const float t = r2 * tab_scale;
+  const int k = in ? (int)t : 0;
+  const float2 e = tab[k];
+  const float g = fmaf(t - (float)k, e.y, e.x);
+  const float inv = __builtin_amdgcn_rsqf(r2 + eps2);
+  const float f = in ? m * (inv * inv) * inv * g : 0.f;
+  sx = fmaf(f, dx, sx);
+  sy = fmaf(f, dy, sy);
+  sz = fmaf(f, dz, sz);
+}
+
 // COUNT: instead of forces, count the pairs tested and the pairs inside
 // rcut (excluding a body with itself), for the work accounting of a run.
 template <bool COUNT>
@@ -84,9 +105,10 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
   const int count = __builtin_amdgcn_readfirstlane(min(CHUNK, tstart[cell + 1] - first));
   const bool live = lane < count;
   const int tgt = live ? torder[first + lane] : 0;
-  const float px = live ? (float)(tx[tgt] - ox) : 0.f;
-  const float py = live ? (float)(ty[tgt] - oy) : 0.f;
-  const float pz = live ? (float)(tz[tgt] - oz) : 0.f;
+  // idle lanes sit far outside the grid, so no source is ever within rcut
+  const float px = live ? (float)(tx[tgt] - ox) : 1e18f;
+  const float py = live ? (float)(ty[tgt] - oy) : 1e18f;
+  const float pz = live ? (float)(tz[tgt] - oz) : 1e18f;
   const int a = cell / (nc1 * nc2), b = (cell / nc2) % nc1, c = cell % nc2;
   const int c0 = max(c - 1, 0), c1 = min(c + 1, nc2 - 1);
   float sx = 0.f, sy = 0.f, sz = 0.f;
@@ -95,8 +117,39 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
     for (int db = max(b - 1, 0); db <= min(b + 1, nc1 - 1); ++db) {
       const int row = (da * nc1 + db) * nc2;
       const int s0 = cell_start[row + c0], s1 = cell_start[row + c1 + 1];
-#pragma unroll 4
-      for (int s = s0; s < s1; ++s) {
+      // groups of SG sources: their (scalar) loads are issued together, and
+      // a group beyond rcut of every lane adds exactly zero and is skipped
+      int s = s0;
+      for (; s + SG <= s1; s += SG) {
+        float4 q[SG];
+#pragma unroll
+        for (int j = 0; j < SG; ++j) q[j] = src[s + j];
+        float dx[SG], dy[SG], dz[SG], r2[SG];
+        bool in[SG], any = false;
+#pragma unroll
+        for (int j = 0; j < SG; ++j) {
+          dx[j] = q[j].x - px;
+          dy[j] = q[j].y - py;
+          dz[j] = q[j].z - pz;
+          r2[j] = fmaf(dx[j], dx[j], fmaf(dy[j], dy[j], dz[j] * dz[j]));
+          in[j] = r2[j] < rcut2;
+          any = any || in[j];
+        }
+        if (COUNT) {
+#pragma unroll
+          for (int j = 0; j < SG; ++j) {
+            n_test += 1;
+            n_in += (in[j] && r2[j] > 0.f) ? 1 : 0;
+          }
+          continue;
+        }
+        if (__ballot(any) == 0) continue;
+#pragma unroll
+        for (int j = 0; j < SG; ++j)
+          pp_term(q[j].w, dx[j], dy[j], dz[j], r2[j], in[j], tab, tab_scale, eps2, sx,
+                  sy, sz);
+      }
+      for (; s < s1; ++s) {
         const float4 q = src[s];
         const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -106,15 +159,7 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
           n_in += (in && r2 > 0.f) ? 1 : 0;
           continue;
         }
-        const float t = r2 * tab_scale;
-        const int k = in ? (int)t : 0;
-        const float2 e = tab[k];
-        const float g = fmaf(t - (float)k, e.y, e.x);
-        const float inv = __builtin_amdgcn_rsqf(r2 + eps2);
-        const float f = in ? q.w * (inv * inv) * inv * g : 0.f;
-        sx = fmaf(f, dx, sx);
-        sy = fmaf(f, dy, sy);
-        sz = fmaf(f, dz, sz);
+        pp_term(q.w, dx, dy, dz, r2, in, tab, tab_scale, eps2, sx, sy, sz);
       }
     }
   if (COUNT) {

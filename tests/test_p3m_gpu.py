@@ -94,6 +94,13 @@ def test_cell_bin_same_cells_any_order(dev):
     # sorting each cell's segment by body index gives the stable list
     keys = np.repeat(np.arange(ncells), np.diff(rs))
     assert np.array_equal(got_o[np.lexsort((got_o, keys))], ro)
+    # inside a cell the bodies come grouped by octant (x, y, z halves)
+    f = [(pos[c] - origin[c]) * (1.0 / cell) for c in range(3)]
+    ijk = [np.clip(np.floor(f[c]).astype(np.int64), 0, nc[c] - 1) for c in range(3)]
+    octant = sum(np.clip(np.floor(2.0 * f[c]).astype(np.int64) - 2 * ijk[c], 0, 1)
+                 << (2 - c) for c in range(3))
+    key8 = ((ijk[0] * nc[1] + ijk[1]) * nc[2] + ijk[2]) * 8 + octant
+    assert np.all(np.diff(key8[got_o]) >= 0)
 
 
 def test_tsc_deposit_and_gather_match_oracle(dev):
@@ -210,7 +217,8 @@ def test_split_force_assignments_on_cube(dev, assign):
         assert errs["tsc"] < 0.5 * errs["cic"], errs
 
 
-@pytest.mark.parametrize("prec,tol,n", [(0, 1e-12, 32), (1, 1e-5, 32), (1, 1e-5, 128)])
+@pytest.mark.parametrize("prec,tol,n", [(0, 1e-12, 32), (1, 1e-5, 32), (1, 1e-5, 128),
+                                        (0, 1e-12, 128)])
 def test_pruned_convolution_matches_padded_fft(dev, prec, tol, n):
     """ssim_pm_conv_isolated3 (pruned passes, imaginary kernel spectrum)
     equals the corner of irfftn(rfftn(pad(rho)) * i kim) in NumPy, and the
