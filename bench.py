@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true")
     ap.add_argument("--weights", default=None)
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every kernel of a step instead of replaying "
+                         "the captured step")
     return ap.parse_args()
 
 
@@ -147,7 +150,7 @@ def main():
         torch.mul(a, 1.0, out=b)          # reads 256 MiB, writes 256 MiB
         fast = PMLearnedSystem(dev, n, t, eps2, 1.0, net=net)
         fast.set_bodies(pos, zeros_n, m, tpos, zeros_t)
-        fast.run(dt, 3)
+        fast.run(dt, 3, graph=False)
         torch.cuda.synchronize(dev)
         print(json.dumps({"probe": "done", "copy_bytes": a.numel() * 4}))
         return
@@ -183,10 +186,11 @@ def main():
     # ---- PM + learned -----------------------------------------------------
     fast = PMLearnedSystem(dev, n, t, eps2, 1.0, net=net)
     fast.set_bodies(pos, zeros_n, m, tpos, zeros_t)
-    fast.run(dt, max(args.warmup, 1))
+    graph = not args.eager
+    fast.run(dt, max(args.warmup, 1), graph=graph)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
-    fast.run(dt, args.steps)
+    fast.run(dt, args.steps, graph=graph)
     torch.cuda.synchronize(dev)
     wall = time.perf_counter() - t0
     value = nb * args.steps / wall
@@ -210,8 +214,8 @@ def main():
     tested, inside = sf.near_pair_counts(fast.x)
     names = ["k_pp_cells", "k_fft_pass", "k_conv_x3", "k_tsc_deposit", "k_tsc_gather",
              "k_cic_deposit", "k_cic_gather", "k_mlp_forward", "k_mlp_features"]
-    lib.ssim_timing(1)
-    fast.run(dt, args.profile_steps)
+    lib.ssim_timing(1)                  # HIP events per launch: eager steps
+    fast.run(dt, args.profile_steps, graph=False)
     times = kernel_times(lib, names)
     lib.ssim_timing(0)
     steps_p = args.profile_steps
@@ -289,7 +293,8 @@ def main():
                                f"{n_mesh}^3 isolated + near field + learned "
                                "correction",
                    "masses": n, "tracers": t, "method": "pm_learned",
-                   "split": split},
+                   "split": split,
+                   "launch": "eager" if args.eager else "hip_graph"},
         "direct_f64_body_steps_per_s": direct_rate,
         "speedup_vs_direct_f64": value / direct_rate,
         "force_err_median": float(np.median(err_l)),

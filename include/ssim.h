@@ -53,7 +53,7 @@ const char* ssim_version(void);
 int ssim_device_count(void);
 /* Launch timing for measurement runs: ssim_timing(1) clears the records and
  * brackets every later launch of the library's main kernels (k_allpairs_f64,
- * k_fft_pass, k_pp_cells, k_tsc_deposit, k_tsc_gather, k_cic_deposit,
+ * k_fft_pass, k_conv_x3, k_pp_cells, k_tsc_deposit, k_tsc_gather, k_cic_deposit,
  * k_cic_gather, k_mlp_features, k_mlp_forward) with HIP events on the
  * launch's stream; ssim_timing(0) stops recording.  ssim_timing_report
  * waits for the recorded launches of `kernel` and returns their number and
@@ -221,6 +221,18 @@ size_t ssim_minmax_work_bytes(int n);
 int ssim_minmax(int n, int narrays, const double* a0, const double* a1,
                 const double* a2, double* out, void* work, size_t work_bytes,
                 void* stream);
+/* out[2c], out[2c+1] = min, max over the n bodies of the positions that
+ * the next half kick (v += a half) and drift (x += v full) will produce,
+ * x_c + (v_c + a_c half) full, with the kernels' own operations, so the
+ * box is exact (device output, workspace ssim_minmax_work_bytes(n)).  The
+ * split-force integrator places its mesh from it before the step, so a
+ * step needs no host round trip.  New: the reference computes the box of
+ * the current positions (`grid_fuer`, pm_kernel.py:117-118). */
+int ssim_box_after_kdk(int dim, int n, const double* x, const double* y,
+                       const double* z, const double* vx, const double* vy,
+                       const double* vz, const double* ax, const double* ay,
+                       const double* az, double half, double full, double* out,
+                       void* work, size_t work_bytes, void* stream);
 /* Real-space force Green's function component `comp` on the zero-padded
  * (2n)^dim mesh: -G d_comp / (|d|^2 + eps2)^(3/2) with wrapped offsets
  * (build_force_kernels, pm_kernel.py:34-60, generalised to 3-D).  rs > 0

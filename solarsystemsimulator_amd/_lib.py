@@ -69,6 +69,8 @@ SIGNATURES = {
                                 _P]),
     "ssim_minmax_work_bytes": (c_size_t, [c_int]),
     "ssim_minmax": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, c_size_t, _P]),
+    "ssim_box_after_kdk": (c_int, [c_int, c_int] + [_P] * 9 + [c_double, c_double, _P,
+                                                             _P, c_size_t, _P]),
     "ssim_pm_force_kernel": (c_int, [c_int, c_int, c_int, c_double, c_double,
                                      c_double, c_double, c_double, c_int, _P,
                                      _P]),

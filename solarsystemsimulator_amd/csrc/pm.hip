@@ -165,6 +165,48 @@ __global__ __launch_bounds__(PM_BLOCK) void k_minmax_partial(
     }
 }
 
+// Per-block extrema of the positions the next kick(half) + drift(full)
+// produces, p = x + (v + a half) full, with the same operations (no fused
+// multiply-add) as k_kick_snap and k_drift, so the box is exact.
+__global__ __launch_bounds__(PM_BLOCK) void k_box_kdk_partial(
+    int n, int na, const double* __restrict__ x0, const double* __restrict__ x1,
+    const double* __restrict__ x2, const double* __restrict__ v0,
+    const double* __restrict__ v1, const double* __restrict__ v2,
+    const double* __restrict__ a0, const double* __restrict__ a1,
+    const double* __restrict__ a2, double half, double full,
+    double* __restrict__ part) {
+  __shared__ double smin[3][PM_BLOCK], smax[3][PM_BLOCK];
+  const double* xs[3] = {x0, x1, x2};
+  const double* vs[3] = {v0, v1, v2};
+  const double* as[3] = {a0, a1, a2};
+  for (int c = 0; c < na; ++c) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int i = blockIdx.x * PM_BLOCK + threadIdx.x; i < n;
+         i += gridDim.x * PM_BLOCK) {
+      const double nv = vs[c][i] + as[c][i] * half;
+      const double p = xs[c][i] + nv * full;
+      lo = fmin(lo, p);
+      hi = fmax(hi, p);
+    }
+    smin[c][threadIdx.x] = lo;
+    smax[c][threadIdx.x] = hi;
+  }
+  __syncthreads();
+  for (int w = PM_BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int c = 0; c < na; ++c) {
+        smin[c][threadIdx.x] = fmin(smin[c][threadIdx.x], smin[c][threadIdx.x + w]);
+        smax[c][threadIdx.x] = fmax(smax[c][threadIdx.x], smax[c][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < na; ++c) {
+      part[(size_t)blockIdx.x * 6 + 2 * c] = smin[c][0];
+      part[(size_t)blockIdx.x * 6 + 2 * c + 1] = smax[c][0];
+    }
+}
+
 // Triangular-shaped-cloud (TSC) assignment on an isolated 3-D mesh, for the
 // split force: a body at f = (x - o)/h (node units) spreads over the 3 nodes
 // c-1, c, c+1 of each axis around its nearest node c = floor(f + 1/2), with
@@ -695,6 +737,29 @@ int ssim_minmax(int n, int narrays, const double* a0, const double* a1,
   SSIM_LAUNCHED("k_minmax_partial");
   hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(PM_BLOCK), 0, st, nblk, narrays,
                      part, out);
+  SSIM_LAUNCHED("k_minmax_final");
+  return SSIM_OK;
+}
+
+int ssim_box_after_kdk(int dim, int n, const double* x, const double* y,
+                       const double* z, const double* vx, const double* vy,
+                       const double* vz, const double* ax, const double* ay,
+                       const double* az, double half, double full, double* out,
+                       void* work, size_t work_bytes, void* stream) {
+  SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+  SSIM_REQUIRE(n >= 1, "box of an empty set");
+  SSIM_REQUIRE(x && y && vx && vy && ax && ay && out && work &&
+                   (dim == 2 || (z && vz && az)),
+               "null pointer");
+  SSIM_REQUIRE(work_bytes >= ssim_minmax_work_bytes(n), "workspace too small");
+  hipStream_t st = ssim::as_stream(stream);
+  int nblk = ssim::ceil_div(n, PM_BLOCK * 8);
+  double* part = static_cast<double*>(work);
+  hipLaunchKernelGGL(k_box_kdk_partial, dim3(nblk), dim3(PM_BLOCK), 0, st, n, dim, x,
+                     y, z, vx, vy, vz, ax, ay, az, half, full, part);
+  SSIM_LAUNCHED("k_box_kdk_partial");
+  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(PM_BLOCK), 0, st, nblk, dim, part,
+                     out);
   SSIM_LAUNCHED("k_minmax_final");
   return SSIM_OK;
 }

@@ -99,9 +99,10 @@ class PMLearnedSystem:
                                  self.split.h, self.split.prec, out=self.feat,
                                  n_mass=self.n)
 
-    def force(self, learned: bool = True):
-        """a = split force (+ learned correction) at the current positions."""
-        self.split.accelerations(self.x, self.n, self.m, self.a)
+    def force(self, learned: bool = True, placed: bool = False):
+        """a = split force (+ learned correction) at the current positions;
+        `placed`: the mesh is placed for them already (run's steps)."""
+        self.split.accelerations(self.x, self.n, self.m, self.a, placed=placed)
         if learned and self.net is not None:
             feat = self.features()
             # masses: no component along the velocity (no work done on
@@ -112,22 +113,90 @@ class PMLearnedSystem:
         self.acc_valid = True
         return self.a
 
-    def run(self, dt: float, steps: int, snap=None, learned: bool = True):
-        """`steps` KDK steps; optional f32 snapshot tensor (steps, 6n+3t)."""
-        if not self.acc_valid:
-            self.force(learned)
+    # -- time stepping -------------------------------------------------------
+    # A step is a fixed sequence of launches once the mesh is placed: the
+    # box of the positions the step's drift will produce is computed on the
+    # device at the end of the previous step (ssim_box_after_kdk, exact),
+    # the host reads those six numbers, places the mesh (place_box), and
+    # the step runs without another host round trip.  While the placement
+    # is kept the step is replayed from a captured HIP graph, which removes
+    # the ~100 kernel launches per step from the host's critical path; the
+    # eager and the replayed steps run the same kernels with the same
+    # arguments, so they give identical results.
+    use_graph = True
+    captures = 0          # step graphs captured so far (one per placement)
+
+    def _box_buffers(self):
+        torch = _torch()
+        if getattr(self, "_box_dev", None) is None:
+            nb = self.n + self.t
+            self._box_dev = torch.empty(6, dtype=torch.float64, device=self.dev)
+            self._box_host = torch.empty(6, dtype=torch.float64, pin_memory=True)
+            self._box_ws = D.workspace(_lib.load().ssim_minmax_work_bytes(nb), self.dev)
+            self._box_ev = torch.cuda.Event()
+
+    def _next_box(self, half, full):
+        ps = lambda lst: [_lib.ptr(c) for c in lst]  # noqa: E731
+        _lib.call("ssim_box_after_kdk", 3, self.n + self.t, *ps(self.x), *ps(self.v),
+                  *ps(self.a), half, full, _lib.ptr(self._box_dev),
+                  _lib.ptr(self._box_ws), self._box_ws.numel(), D.stream(self.dev))
+
+    def _read_box(self):
+        self._box_host.copy_(self._box_dev, non_blocking=True)
+        self._box_ev.record()
+        self._box_ev.synchronize()
+        b = self._box_host.numpy()
+        return [(float(b[2 * c]), float(b[2 * c + 1])) for c in range(3)]
+
+    def _step(self, dt, learned, snap_ptr=None):
+        """Launches of one KDK step (the mesh already placed for the drifted
+        positions), ending with the box of the next step's drift."""
         nb = self.n + self.t
         st = D.stream(self.dev)
         ps = lambda lst: [_lib.ptr(c) for c in lst]  # noqa: E731
-        half, full = 0.5 * dt, dt
-        for s in range(steps):
-            _lib.call("ssim_kick_snap", 3, self.n, self.t, *ps(self.x),
-                      *ps(self.v), *ps(self.a), half, None, st)
-            _lib.call("ssim_drift", 3, nb, *ps(self.x), *ps(self.v), full, st)
+        half = 0.5 * dt
+        _lib.call("ssim_kick_snap", 3, self.n, self.t, *ps(self.x), *ps(self.v),
+                  *ps(self.a), half, None, st)
+        _lib.call("ssim_drift", 3, nb, *ps(self.x), *ps(self.v), dt, st)
+        self.force(learned, placed=True)
+        _lib.call("ssim_kick_snap", 3, self.n, self.t, *ps(self.x), *ps(self.v),
+                  *ps(self.a), half, snap_ptr, st)
+        self._next_box(half, dt)
+
+    def _graph_for(self, dt, learned):
+        """The captured step for the current placement (captured anew when
+        the placement, dt or the learned flag changed)."""
+        torch = _torch()
+        key = (self.split.placement(), float(dt), bool(learned))
+        if getattr(self, "_graph_key", None) != key:
+            self._graph = None
+            g = torch.cuda.CUDAGraph()
+            side = torch.cuda.Stream(device=self.dev)    # default priority
+            side.wait_stream(torch.cuda.current_stream(self.dev))
+            with torch.cuda.graph(g, stream=side):
+                self._step(dt, learned)
+            torch.cuda.current_stream(self.dev).wait_stream(side)
+            self._graph, self._graph_key = g, key
+            self.captures += 1
+        return self._graph
+
+    def run(self, dt: float, steps: int, snap=None, learned: bool = True,
+            graph: bool | None = None):
+        """`steps` KDK steps; optional f32 snapshot tensor (steps, 6n+3t).
+        graph: replay captured steps (default: use_graph, with a kept mesh
+        placement and no snapshots); False launches every kernel."""
+        self._box_buffers()
+        if not self.acc_valid:
             self.force(learned)
-            _lib.call("ssim_kick_snap", 3, self.n, self.t, *ps(self.x),
-                      *ps(self.v), *ps(self.a), half,
-                      None if snap is None else _lib.ptr(snap[s]), st)
+        use = self.use_graph if graph is None else graph
+        use = use and snap is None and self.split.freeze
+        self._next_box(0.5 * dt, dt)
+        for s in range(steps):
+            self.split.place_box(self._read_box())
+            if use:
+                self._graph_for(dt, learned).replay()
+            else:
+                self._step(dt, learned, None if snap is None else _lib.ptr(snap[s]))
 
     def host_state(self):
         return ([c.cpu().numpy() for c in self.x], [c.cpu().numpy() for c in self.v])

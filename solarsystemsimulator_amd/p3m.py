@@ -82,9 +82,8 @@ class SplitForce:
         self._cells_cap = 0
 
     # -- grid ---------------------------------------------------------------
-    def _place(self, x):
-        """Mesh origin and ladder cell size covering every body."""
-        box = P.minmax(x)
+    def _place(self, box):
+        """Mesh origin and ladder cell size covering the box [(lo, hi)]."""
         span = max(hi - lo for lo, hi in box)
         if self.freeze and self.level is not None:
             h, o = self.h, self.origin
@@ -122,11 +121,22 @@ class SplitForce:
     def place(self, x):
         """Mesh placement for the bodies x (rebuilds the kernel spectra when
         the cell size changes)."""
-        level, h, origin, box = self._place(x)
+        return self.place_box(P.minmax(x))
+
+    def place_box(self, box):
+        """Mesh placement for bodies inside box [(lo, hi)] x 3; returns True
+        when the placement (cell size or origin) changed."""
+        level, h, origin, box = self._place(box)
+        changed = level != self.level or origin != self.origin
         if level != self.level:
             self._rebuild_kernels(h)
             self.level = level
         self.h, self.origin, self.box = h, origin, box
+        return changed
+
+    def placement(self):
+        """(ladder level, origin) of the current mesh placement."""
+        return self.level, tuple(self.origin) if self.origin is not None else None
 
     def visit_order(self):
         """Cell-sorted body order of the last bin_bodies (None before)."""
@@ -224,14 +234,17 @@ class SplitForce:
 
     def bin_bodies(self, x, n_mass, m):
         """Cell lists of the sources (stable, packed) and of all targets on
-        a grid of cells >= rcut over the current box; returns the grid."""
+        a grid of cells >= rcut over the mesh (a function of the placement
+        only, so the launches of a step repeat while the mesh is kept);
+        returns the grid."""
         import torch
         rs = self.rs_cells * self.h
         rcut = self.rcut_rs * rs
-        lo = [b[0] for b in self.box]
-        span = [b[1] - b[0] for b in self.box]
-        nc = [max(1, int(s // rcut)) for s in span]
-        cell = max(max(s / k for s, k in zip(span, nc)), rcut)
+        lo = list(self.origin)
+        extent = (self.n - 1) * self.h
+        k = max(1, int(extent // rcut))
+        nc = [k, k, k]
+        cell = max(extent / k, rcut)
         ncells = nc[0] * nc[1] * nc[2]
         nt = x[0].numel()
         if self._cells_cap < ncells or self._np != n_mass or self._nt != nt:
@@ -296,9 +309,10 @@ class SplitForce:
     _nt = -1
     _grid = None
 
-    def accelerations(self, x, n_mass, m, a):
+    def accelerations(self, x, n_mass, m, a, placed=False):
         """a = mesh + near-field acceleration of all bodies in x (n_mass
-        masses first, tracers after them).
+        masses first, tracers after them); `placed`: the caller has placed
+        the mesh for x already (place_box), so nothing waits for the host.
 
         With TSC the near field runs on a second stream while the current
         one deposits and convolves: the near-field kernel is bound by the
@@ -307,7 +321,8 @@ class SplitForce:
         field; the current stream then waits for it and adds the mesh gather
         (in cell order).  With CIC everything runs in order on one stream."""
         import torch
-        self.place(x)
+        if not placed:
+            self.place(x)
         if self.assign != "tsc":
             self.bin_bodies(x, n_mass, m)
             self.mesh_accel(x, n_mass, m, a, order=self.torder, placed=True)

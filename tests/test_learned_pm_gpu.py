@@ -1,0 +1,72 @@
+"""Time stepping of the PM + learned method: the exact box of the next
+drift (the mesh placement is made from it before a step), and steps
+replayed from a captured HIP graph against steps launched one by one."""
+import numpy as np
+import pytest
+import torch
+
+from solarsystemsimulator_amd import _lib, scenes
+from solarsystemsimulator_amd import device as D
+from solarsystemsimulator_amd.learned_pm import PMLearnedSystem, load_weights
+
+pytestmark = pytest.mark.gpu
+
+
+def _ptrs(arrs):
+    return [_lib.ptr(c) for c in arrs]
+
+
+def test_box_after_kdk_is_exact(dev):
+    print("box after kick + drift", flush=True)
+    rng = np.random.default_rng(3)
+    n = 100_003
+    x = [rng.normal(0.0, 1.0, n) for _ in range(3)]
+    v = [rng.normal(0.0, 10.0, n) for _ in range(3)]
+    a = [rng.normal(0.0, 100.0, n) for _ in range(3)]
+    half, full = 1.25e-3, 2.5e-3
+    xd, vd, ad = ([D.to_dev(c, dev) for c in arr] for arr in (x, v, a))
+    out = torch.empty(6, dtype=torch.float64, device=dev)
+    w = D.workspace(_lib.load().ssim_minmax_work_bytes(n), dev)
+    st = D.stream(dev)
+    _lib.call("ssim_box_after_kdk", 3, n, *_ptrs(xd), *_ptrs(vd), *_ptrs(ad), half,
+              full, _lib.ptr(out), _lib.ptr(w), w.numel(), st)
+    got = out.cpu().numpy()
+    # the kick and drift kernels themselves, then the box of their result
+    _lib.call("ssim_kick", 3, n, *_ptrs(vd), *_ptrs(ad), half, st)
+    _lib.call("ssim_drift", 3, n, *_ptrs(xd), *_ptrs(vd), full, st)
+    for c in range(3):
+        p = xd[c].cpu().numpy()
+        assert got[2 * c] == p.min() and got[2 * c + 1] == p.max()
+        # NumPy's arithmetic (no fused multiply-add) gives the same positions
+        assert np.array_equal(x[c] + (v[c] + a[c] * half) * full, p)
+
+
+def _system(dev, pos, m, tpos, vel, tvel):
+    s = PMLearnedSystem(dev, len(m), len(tpos[0]), 1e-6, 1.0,
+                        net=load_weights(dev), mesh_n=32)
+    s.set_bodies(pos, vel, m, tpos, tvel)
+    return s
+
+
+def test_replayed_steps_equal_eager_steps(dev):
+    print("graph-replayed steps vs eager steps", flush=True)
+    pos, m, tpos = scenes.learned_scene((20, 20, 10), (20, 20, 20), seed=7,
+                                        tracer_seed=77)
+    n, t = len(m), len(tpos[0])
+    # a bulk drift of about one mesh cell per step (cell ~ 1/23 of the
+    # cube): the kept placement runs out every few steps, so the run
+    # re-places the mesh and captures the step again several times
+    dt, vb = 2.5e-4, 170.0
+    vel = [np.full(n, vb), np.zeros(n), np.zeros(n)]
+    tvel = [np.full(t, vb), np.zeros(t), np.zeros(t)]
+    eager = _system(dev, pos, m, tpos, vel, tvel)
+    replay = _system(dev, pos, m, tpos, vel, tvel)
+    eager.run(dt, 12, graph=False)
+    replay.run(dt, 12, graph=True)
+    assert eager.captures == 0 and replay.captures >= 3
+    for c in range(3):
+        assert torch.equal(eager.x[c], replay.x[c])
+        assert torch.equal(eager.v[c], replay.v[c])
+        assert torch.equal(eager.a[c], replay.a[c])
+    # the eager run re-placed the mesh the same way
+    assert eager.split.placement() == replay.split.placement()
