@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", action="store_true")
     ap.add_argument("--weights", default=None)
+    ap.add_argument("--serial", action="store_true",
+                    help="one stream: no overlap of the near field with the mesh")
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel of a step instead of replaying "
                          "the captured step")
@@ -186,6 +188,7 @@ def main():
     # ---- PM + learned -----------------------------------------------------
     fast = PMLearnedSystem(dev, n, t, eps2, 1.0, net=net)
     fast.set_bodies(pos, zeros_n, m, tpos, zeros_t)
+    fast.split.overlap = not args.serial
     graph = not args.eager
     fast.run(dt, max(args.warmup, 1), graph=graph)
     torch.cuda.synchronize(dev)
@@ -214,10 +217,14 @@ def main():
     tested, inside = sf.near_pair_counts(fast.x)
     names = ["k_pp_cells", "k_fft_pass", "k_conv_x3", "k_tsc_deposit", "k_tsc_gather",
              "k_cic_deposit", "k_cic_gather", "k_mlp_forward", "k_mlp_features"]
-    lib.ssim_timing(1)                  # HIP events per launch: eager steps
+    # HIP events per launch: eager steps on one stream, so each duration is
+    # the kernel's own and not that of two kernels sharing the GPU
+    fast.split.overlap = False
+    lib.ssim_timing(1)
     fast.run(dt, args.profile_steps, graph=False)
     times = kernel_times(lib, names)
     lib.ssim_timing(0)
+    fast.split.overlap = not args.serial
     steps_p = args.profile_steps
     n_mesh = sf.n
     pass_bytes, x3_bytes = conv_bytes(n_mesh)
@@ -294,7 +301,8 @@ def main():
                                "correction",
                    "masses": n, "tracers": t, "method": "pm_learned",
                    "split": split,
-                   "launch": "eager" if args.eager else "hip_graph"},
+                   "launch": "eager" if args.eager else "hip_graph",
+                   "streams": 1 if args.serial else 2},
         "direct_f64_body_steps_per_s": direct_rate,
         "speedup_vs_direct_f64": value / direct_rate,
         "force_err_median": float(np.median(err_l)),

@@ -272,6 +272,13 @@ int ssim_tsc_deposit_binned(int np, const double* x, const double* y,
                             const double* origin, double h, const int* bin_start,
                             const int* bin_order, int mesh_prec, void* mesh,
                             void* work, size_t work_bytes, void* stream);
+/* The same atomic-free deposit (OVERWRITES mesh) over the blocked bins of
+ * ssim_split_bin (bin_start, and body = its double4 copy of the masses in
+ * bin order) with the same mesh and block edge (>= 2); bit-reproducible.
+ * The masses must lie inside the mesh.  No reference counterpart. */
+int ssim_tsc_deposit_blocked(int grid_n, const double* origin, double h, int block,
+                             const int* bin_start, const void* body, int mesh_prec,
+                             void* mesh, void* stream);
 /* TSC interpolation of three meshes at np points: o_c = sum w f_c
  * (overwrites o_c, FP64, or adds to it when accumulate != 0); thread i handles point order[i] (order = NULL:
  * point i) — a cell-sorted order makes neighbouring threads read
@@ -345,6 +352,24 @@ int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
                    const double* origin, double cell, const int* nc,
                    int* cell_start, int* order, void* work, size_t work_bytes,
                    void* stream);
+/* One stable binning of the n masses (x, y, z, m) for both halves of the
+ * split force on a grid_n^3 mesh (origin, h): every mass goes to the TSC
+ * node its deposit stencil is centred on, r = floor((x - o)/h + 1/2) per
+ * axis, and the nodes are grouped in blocks of block^3 (nb = ceil(grid_n /
+ * block) per axis) that serve as the near-field cells, edge block * h
+ * (>= rcut), grid origin o - h/2.  Bins are ordered block-major
+ * (ssim_tsc_deposit_blocked's key), bodies inside a bin by index.
+ * Outputs: bin_start[nb^3 block^3 + 1], cell_start[nb^3 + 1] (the block
+ * starts, for ssim_pp_short), order[n] (mass indices in bin order),
+ * packed[n] float4 (x, y, z relative to o - h/2, m) for ssim_pp_short and
+ * body[n] double4 ((x - o)/h per axis, m) for ssim_tsc_deposit_blocked.  It replaces
+ * the two cell lists (nodes and near-field cells) of ssim_cells_build.  No
+ * hipMemset inside.  No reference counterpart. */
+size_t ssim_split_bin_work_bytes(int n, int grid_n, int block);
+int ssim_split_bin(int n, const double* x, const double* y, const double* z,
+                   const double* m, int grid_n, const double* origin, double h,
+                   int block, int* bin_start, int* cell_start, int* order, void* packed,
+                   void* body, void* work, size_t work_bytes, void* stream);
 /* ADD the near-field force G m d g(r) / (r^2 + eps2)^(3/2), r < rcut, of
  * the binned sources (cell_start + packed, from ssim_cells_build) to nt
  * targets' accelerations (3-D).  The targets are binned on the same grid

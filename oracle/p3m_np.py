@@ -10,6 +10,8 @@ INFRASTRUCTURE ONLY.
   [rcut - rs, rcut] by S(u) = 1 - u^3 (10 - 15u + 6u^2).
 * `tsc_deposit` / `tsc_gather` — triangular-shaped-cloud assignment on an
   isolated 3-D mesh (3 nodes per axis around the nearest node).
+* `split_bins` — the masses binned by TSC stencil centre, nodes grouped in
+  blocks (the near-field cells), stable order.
 Pinned in tests/test_p3m_oracle.py: the cell list against a brute-force
 binning; g -> 1 as rs -> infinity turns the near field into the plain
 all-pairs sum (checked against the long-double oracle); the taper is 1
@@ -18,7 +20,9 @@ below rcut - rs, 0 at rcut, with continuous first and second derivatives
 the mass fraction of a Gaussian cloud (variance 2 rs^2) inside r, computed
 by quadrature; TSC conserves mass, puts 3/4 per axis on the node a body
 sits on, interpolates linear fields exactly and is the adjoint of its
-gather.
+gather; split_bins matches a brute-force binning (nearest node by search,
+lexicographic sort of (block, node in block, index)), and summing every
+node's 27 neighbouring bins with the TSC weights gives tsc_deposit.
 """
 from __future__ import annotations
 
@@ -37,6 +41,26 @@ def cell_list(pos, origin, cell, nc):
     counts = np.bincount(keys, minlength=int(np.prod(nc)))
     start = np.concatenate([[0], np.cumsum(counts)])
     return start, order
+
+
+def split_bins(pos, n, origin, h, block):
+    """The split force's mass binning (ssim_split_bin, a binning of this
+    build; its inputs are the reference's body arrays): per axis the TSC
+    stencil centre r = floor((x - o)/h + 1/2) clamped to [0, n - 1]; nodes
+    grouped in blocks of block^3 (nb = ceil(n / block) per axis), key
+    ((r0//B nb + r1//B) nb + r2//B) B^3 + ((r0%B) B + r1%B) B + r2%B; stable
+    order; bin starts over all nb^3 B^3 keys; block starts."""
+    nb = -(-n // block)
+    r = [np.clip(np.floor((np.asarray(pos[c]) - origin[c]) / h + 0.5), 0,
+                 n - 1).astype(np.int64) for c in range(3)]
+    blk = [rc // block for rc in r]
+    loc = [rc % block for rc in r]
+    b3 = block ** 3
+    key = (((blk[0] * nb + blk[1]) * nb + blk[2]) * b3 +
+           (loc[0] * block + loc[1]) * block + loc[2])
+    order = np.argsort(key, kind="stable")
+    start = np.concatenate([[0], np.cumsum(np.bincount(key, minlength=nb ** 3 * b3))])
+    return start, start[::b3], order
 
 
 def split_g(r, rs):

@@ -80,6 +80,8 @@ SIGNATURES = {
     "ssim_tsc_deposit_work_bytes": (c_size_t, [c_int]),
     "ssim_tsc_deposit_binned": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double,
                                         _P, _P, c_int, _P, _P, c_size_t, _P]),
+    "ssim_tsc_deposit_blocked": (c_int, [c_int, _P, c_double, c_int, _P, _P, c_int,
+                                         _P, _P]),
     "ssim_tsc_gather": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double, c_int,
                                 _P, _P, _P, _P, _P, _P, c_int, _P]),
     "ssim_spectrum_mul": (c_int, [c_longlong, c_int, _P, _P, _P, _P]),
@@ -96,6 +98,9 @@ SIGNATURES = {
     "ssim_cells_bin_work_bytes": (c_size_t, [c_int, c_int]),
     "ssim_cells_bin": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P,
                                c_size_t, _P]),
+    "ssim_split_bin_work_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "ssim_split_bin": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double, c_int, _P,
+                               _P, _P, _P, _P, _P, c_size_t, _P]),
     "ssim_pp_work_bytes": (c_size_t, [c_int, c_int]),
     "ssim_pp_short": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P,
                               _P, c_double, c_double, c_double, c_double, _P,

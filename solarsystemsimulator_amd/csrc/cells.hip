@@ -66,11 +66,51 @@ __global__ __launch_bounds__(CB) void k_cell_key8(
   atomicAdd(&count[k], 1);
 }
 
-// cell_start[c] = sub_start[8 c], c = 0..ncells
-__global__ __launch_bounds__(CB) void k_every8(int ncells, const int* __restrict__ sub,
-                                               int* __restrict__ cell_start) {
+// cell_start[c] = sub_start[stride c], c = 0..ncells
+__global__ __launch_bounds__(CB) void k_every(int ncells, int stride,
+                                              const int* __restrict__ sub,
+                                              int* __restrict__ cell_start) {
   const int c = blockIdx.x * CB + threadIdx.x;
-  if (c <= ncells) cell_start[c] = sub[8 * (long long)c];
+  if (c <= ncells) cell_start[c] = sub[(long long)stride * c];
+}
+
+// Split-force mass bins: the TSC stencil centre r = floor((x - o)/h + 1/2)
+// of each axis (the operations of the deposit's weights, so a body's bin is
+// always its stencil's centre), clamped to the mesh, as a blocked key.
+__global__ __launch_bounds__(CB) void k_split_key(
+    int n, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, double ox, double oy, double oz, double h, int gn,
+    int B, int nb, int* __restrict__ key, int* __restrict__ order,
+    int* __restrict__ count) {
+  const int i = blockIdx.x * CB + threadIdx.x;
+  if (i >= n) return;
+  const double top = gn - 1;
+  const int a = (int)fmin(fmax(floor((x[i] - ox) / h + 0.5), 0.0), top);
+  const int b = (int)fmin(fmax(floor((y[i] - oy) / h + 0.5), 0.0), top);
+  const int c = (int)fmin(fmax(floor((z[i] - oz) / h + 0.5), 0.0), top);
+  const int k = ssim_blocked_key(a, b, c, B, nb);
+  key[i] = k;
+  order[i] = i;
+  atomicAdd(&count[k], 1);
+}
+
+// Both packed copies in bin order: float4 (x - cx, y - cy, z - cz, m) for
+// the near field (relative to the cell grid's origin) and double4
+// ((x - ox)/h, (y - oy)/h, (z - oz)/h, m) for the deposit (the node-unit
+// coordinates its weights start from, computed with the same operations).
+__global__ __launch_bounds__(CB) void k_split_pack(
+    int n, const int* __restrict__ order, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, double ox, double oy, double oz, double h,
+    double cx, double cy, double cz, float4* __restrict__ packed,
+    double4* __restrict__ body) {
+  const int s = blockIdx.x * CB + threadIdx.x;
+  if (s >= n) return;
+  const int p = order[s];
+  const double px = x[p], py = y[p], pz = z[p], pm = m[p];
+  packed[s] = make_float4((float)(px - cx), (float)(py - cy), (float)(pz - cz),
+                          (float)pm);
+  body[s] = make_double4((px - ox) / h, (py - oy) / h, (pz - oz) / h, pm);
 }
 
 // Unstable placement: a cursor per cell hands out the slots.
@@ -194,13 +234,80 @@ int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
     SSIM_LAUNCHED("k_cell_key8");
   }
   SSIM_TRY(ssim::scan_exclusive(nsub, count, sub_start, w, st));
-  hipLaunchKernelGGL(k_every8, dim3(ssim::ceil_div(ncells + 1, CB)), dim3(CB), 0, st,
-                     ncells, sub_start, cell_start);
-  SSIM_LAUNCHED("k_every8");
+  hipLaunchKernelGGL(k_every, dim3(ssim::ceil_div(ncells + 1, CB)), dim3(CB), 0, st,
+                     ncells, 8, sub_start, cell_start);
+  SSIM_LAUNCHED("k_every");
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter_any, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
                        n, key, sub_start, cursor, order);
     SSIM_LAUNCHED("k_scatter_any");
+  }
+  return SSIM_OK;
+}
+
+static long long split_bins(int grid_n, int block) {
+  const long long nb = (grid_n + block - 1) / block;
+  return nb * nb * nb * block * block * block;
+}
+
+size_t ssim_split_bin_work_bytes(int n, int grid_n, int block) {
+  if (grid_n < 1 || block < 1) return 0;
+  const long long K = split_bins(grid_n, block);
+  if (K > (1LL << 30)) return 0;
+  return ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int)) +
+         ssim::align256((size_t)K * sizeof(int)) + ssim::scan_work_bytes((int)K) +
+         ssim::radix_sort_work_bytes(n);
+}
+
+int ssim_split_bin(int n, const double* x, const double* y, const double* z,
+                   const double* m, int grid_n, const double* origin, double h,
+                   int block, int* bin_start, int* cell_start, int* order, void* packed,
+                   void* body, void* work, size_t work_bytes, void* stream) {
+  SSIM_REQUIRE(n >= 0, "n must be >= 0");
+  SSIM_REQUIRE(grid_n >= 1 && block >= 1 && h > 0.0 && origin, "bad mesh");
+  const long long K = split_bins(grid_n, block);
+  SSIM_REQUIRE(K <= (1LL << 30), "too many bins: %lld", K);
+  SSIM_REQUIRE(bin_start && cell_start && order && packed && body &&
+                   (n == 0 || (x && y && z && m)),
+               "null pointer");
+  SSIM_REQUIRE(work && work_bytes >= ssim_split_bin_work_bytes(n, grid_n, block),
+               "workspace too small");
+  hipStream_t st = ssim::as_stream(stream);
+  const int nb = (grid_n + block - 1) / block;
+  const int ncells = nb * nb * nb;
+  const int B3 = block * block * block;
+  unsigned char* w = static_cast<unsigned char*>(work);
+  int* key = reinterpret_cast<int*>(w);
+  w += ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int));
+  int* count = reinterpret_cast<int*>(w);
+  w += ssim::align256((size_t)K * sizeof(int));
+  void* scan_work = w;
+  w += ssim::scan_work_bytes((int)K);
+  hipLaunchKernelGGL(k_zero_i32, dim3(ssim::ceil_div(K, CB)), dim3(CB), 0, st, (int)K,
+                     count);
+  SSIM_LAUNCHED("k_zero_i32");
+  if (n > 0) {
+    hipLaunchKernelGGL(k_split_key, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st, n, x,
+                       y, z, origin[0], origin[1], origin[2], h, grid_n, block, nb, key,
+                       order, count);
+    SSIM_LAUNCHED("k_split_key");
+    int bits = 1;
+    while (bits < 31 && (1LL << bits) < K) ++bits;
+    SSIM_TRY(ssim::radix_sort_pairs(n, bits, key, order, w,
+                                    ssim::radix_sort_work_bytes(n), st));
+  }
+  SSIM_TRY(ssim::scan_exclusive((int)K, count, bin_start, scan_work, st));
+  hipLaunchKernelGGL(k_every, dim3(ssim::ceil_div(ncells + 1, CB)), dim3(CB), 0, st,
+                     ncells, B3, bin_start, cell_start);
+  SSIM_LAUNCHED("k_every");
+  if (n > 0) {
+    const double hh = 0.5 * h;
+    hipLaunchKernelGGL(k_split_pack, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st, n,
+                       order, x, y, z, m, origin[0], origin[1], origin[2], h,
+                       origin[0] - hh, origin[1] - hh, origin[2] - hh,
+                       static_cast<float4*>(packed),
+                       static_cast<double4*>(body));
+    SSIM_LAUNCHED("k_split_pack");
   }
   return SSIM_OK;
 }

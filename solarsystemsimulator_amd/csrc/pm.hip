@@ -327,6 +327,80 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_deposit_binned(
   mesh[node] = (M)acc;
 }
 
+// TSC weight on node `node` of a body at node-unit coordinate f = (x-o)/h
+// (the value tsc_node_weight computes first).
+__device__ __forceinline__ double tsc_weight_f(double f, int node) {
+  const double r = floor(f + 0.5);
+  const double d = f - r;
+  const int off = node - (int)r;
+  if (off == 0) return 0.75 - d * d;
+  if (off < -1 || off > 1) return 0.0;
+  const double a = off < 0 ? 0.5 - d : 0.5 + d;
+  return 0.5 * a * a;
+}
+
+// The atomic-free TSC deposit over blocked bins: node (i, j, k) sums the
+// bodies of the 27 bins around it (a bin holds the bodies whose stencil is
+// centred on its node, so every body of them touches the node).  The bin
+// key is a sum of per-axis terms, key = X(i) + Y(j) + Z(k) with
+//   X(i) = (i/B) nb^2 B^3 + (i%B) B^2,  Y(j) = (j/B) nb B^3 + (j%B) B,
+//   Z(k) = (k/B) B^3 + k%B,
+// so each node needs nine divisions.  The z bins k-1..k+1 of a row are one
+// run of the sorted order, or two when they straddle a block boundary
+// (B >= 2: never more); both are walked as one index range so every lane
+// of a wave runs one loop.  body[] holds (f_x, f_y, f_z, m) with
+// f = (x - o)/h computed by ssim_split_bin.
+template <typename M>
+__global__ __launch_bounds__(PM_BLOCK) void k_tsc_deposit_blocked(
+    const double4* __restrict__ body, Grid g, int B, int nb,
+    const int* __restrict__ bin_start, M* __restrict__ mesh) {
+  const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+  const long long node = (long long)blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (node >= (long long)n0 * n1 * n2) return;
+  const int k = (int)(node % n2), j = (int)((node / n2) % n1),
+            i = (int)(node / ((long long)n1 * n2));
+  const int B3 = B * B * B;
+  int X[3], Y[3], Z[3], zb[3];
+  for (int q = 0; q < 3; ++q) {
+    const int ii = max(i + q - 1, 0), jj = max(j + q - 1, 0), kk = max(k + q - 1, 0);
+    const int bi = ii / B, bj = jj / B, bk = kk / B;
+    X[q] = bi * nb * nb * B3 + (ii - bi * B) * B * B;
+    Y[q] = bj * nb * B3 + (jj - bj * B) * B;
+    Z[q] = bk * B3 + (kk - bk * B);
+    zb[q] = bk;
+  }
+  const int q0 = k > 0 ? 0 : 1, q1 = k + 1 < n2 ? 2 : 1;
+  // first z bin of the second run (q1 + 1: one run)
+  int qs = q1 + 1;
+  for (int q = q0 + 1; q <= q1; ++q)
+    if (zb[q] != zb[q0] && qs > q1) qs = q;
+  double acc = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const int bi = i + a - 1;
+    if (bi < 0 || bi >= n0) continue;
+    for (int b = 0; b < 3; ++b) {
+      const int bj = j + b - 1;
+      if (bj < 0 || bj >= n1) continue;
+      const int xy = X[a] + Y[b];
+      const int s0 = bin_start[xy + Z[q0]];
+      int s1, t0 = 0, t1 = 0;
+      if (qs > q1) {
+        s1 = bin_start[xy + Z[q1] + 1];
+      } else {
+        s1 = bin_start[xy + Z[qs - 1] + 1];
+        t0 = bin_start[xy + Z[qs]];
+        t1 = bin_start[xy + Z[q1] + 1];
+      }
+      const int la = s1 - s0, len = la + (t1 - t0);
+      for (int u = 0; u < len; ++u) {
+        const double4 p = body[u < la ? s0 + u : t0 + (u - la)];
+        acc += p.w * tsc_weight_f(p.x, i) * tsc_weight_f(p.y, j) * tsc_weight_f(p.z, k);
+      }
+    }
+  }
+  mesh[node] = (M)acc;
+}
+
 template <typename M>
 __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather(
     int np, const int* __restrict__ order, const double* __restrict__ x, const double* __restrict__ y,
@@ -881,6 +955,32 @@ int ssim_tsc_deposit_binned(int np, const double* x, const double* y,
                                   0, st, body, g, bin_start,
                                   static_cast<float*>(mesh)));
   SSIM_LAUNCHED("k_tsc_deposit_binned");
+  return SSIM_OK;
+}
+
+int ssim_tsc_deposit_blocked(int grid_n, const double* origin, double h, int block,
+                             const int* bin_start, const void* body, int mesh_prec,
+                             void* mesh, void* stream) {
+  SSIM_REQUIRE(grid_n >= 1 && h > 0.0 && block >= 2, "bad TSC arguments (block >= 2)");
+  SSIM_REQUIRE(origin && bin_start && body && mesh, "null pointer");
+  const int ns[3] = {grid_n, grid_n, grid_n};
+  const Grid g = make_grid(3, ns, origin, h, 0);
+  const int nb = (grid_n + block - 1) / block;
+  hipStream_t st = ssim::as_stream(stream);
+  const long long nodes = (long long)grid_n * grid_n * grid_n;
+  const dim3 grid((unsigned)ssim::ceil_div(nodes, PM_BLOCK));
+  const double4* bp = static_cast<const double4*>(body);
+  if (mesh_prec == SSIM_PREC_F64)
+    SSIM_TIMED("k_tsc_deposit", st,
+               hipLaunchKernelGGL(k_tsc_deposit_blocked<double>, grid, dim3(PM_BLOCK), 0,
+                                  st, bp, g, block, nb, bin_start,
+                                  static_cast<double*>(mesh)));
+  else
+    SSIM_TIMED("k_tsc_deposit", st,
+               hipLaunchKernelGGL(k_tsc_deposit_blocked<float>, grid, dim3(PM_BLOCK), 0,
+                                  st, bp, g, block, nb, bin_start,
+                                  static_cast<float*>(mesh)));
+  SSIM_LAUNCHED("k_tsc_deposit_blocked");
   return SSIM_OK;
 }
 

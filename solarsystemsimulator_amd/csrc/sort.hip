@@ -1,55 +1,149 @@
 // Stable LSD radix sort of (int key, int value) pairs and an exclusive scan,
-// shared by the cell list (cells.hip).
+// shared by the cell lists (cells.hip) and the near-field chunk map (pp.hip).
 //
-// Each pass sorts on 8 bits of the key:
-//   1. k_radix_hist   — every 256-element tile counts its digits in LDS and
-//                       writes the counts digit-major: hist[d * tiles + b];
-//   2. scan_exclusive — the exclusive prefix of hist is the first output
-//                       slot of digit d coming from tile b;
-//   3. k_radix_scatter— an element's slot is that prefix plus the number of
-//                       elements before it in its tile with the same digit.
-// Tiles are taken in index order and ranks inside a tile follow the index,
-// so equal keys keep their input order: the sort is stable, and the result
-// does not depend on scheduling.
+// The sort takes 8 key bits per pass over tiles of RTILE = 4096 elements
+// (16 rounds of one element per thread of a 256-thread workgroup):
+//   k_radix_hist0   — per-tile digit counts of the first pass (tile-major,
+//                     hist[b * 256 + d]); it also clears the count arrays
+//                     of the later passes;
+//   k_radix_scatter — the tile's first slot per digit (its own exclusive
+//                     prefix over the tile counts, or k_radix_offs's when
+//                     there are many tiles), then per round the rank of an
+//                     element among the lanes of its wave with the same
+//                     digit (nine ballots), wave totals folded in wave
+//                     order, and the store; the destination tile's count
+//                     of the NEXT pass's digit is added as it goes, so only
+//                     the first pass needs a counting kernel.
+// Tiles, rounds, waves and lanes are taken in index order, so equal keys
+// keep their input order: the sort is stable, and the result does not
+// depend on scheduling (the next-pass counts are sums of ones).  A 3-pass
+// sort is 4 launches, and the passes alternate between two scratch pairs
+// so the last one writes the caller's arrays.
 #include "ssim_internal.h"
 
 namespace {
 
-constexpr int RT = 256;       // tile = threads per workgroup
-constexpr int SCAN_T = 1024;  // threads of the single-workgroup scan
+constexpr int RT = 256;            // threads per workgroup (one per digit)
+constexpr int RIPT = 16;           // rounds per tile
+constexpr int RTILE = RT * RIPT;   // elements per tile
+constexpr int FUSE_TILES = 256;    // up to this many tiles a scatter block
+                                   // computes its own offsets
+constexpr int SCAN_T = 1024;       // threads of the single-workgroup scan
 
-__global__ __launch_bounds__(RT) void k_radix_hist(int n, const int* __restrict__ keys,
-                                                   int shift, int tiles,
-                                                   int* __restrict__ hist) {
+__global__ __launch_bounds__(RT) void k_radix_hist0(int n, const int* __restrict__ keys,
+                                                    int tiles, int* __restrict__ hist,
+                                                    int* __restrict__ zero, int nzero) {
   __shared__ int cnt[256];
   const int t = threadIdx.x;
-  const int i = blockIdx.x * RT + t;
+  for (int i = blockIdx.x * RT + t; i < nzero; i += gridDim.x * RT) zero[i] = 0;
   cnt[t] = 0;
   __syncthreads();
-  if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255], 1);
+  const int base = blockIdx.x * RTILE;
+#pragma unroll
+  for (int r = 0; r < RIPT; ++r) {
+    const int i = base + r * RT + t;
+    if (i < n) atomicAdd(&cnt[keys[i] & 255], 1);
+  }
   __syncthreads();
-  hist[(size_t)t * tiles + blockIdx.x] = cnt[t];
+  hist[(size_t)blockIdx.x * 256 + t] = cnt[t];
+}
+
+// Exclusive prefix of v over the 256 threads of the workgroup (s: 256 ints
+// of LDS, free again on return).
+__device__ int excl_scan256(int v, int* s) {
+  const int t = threadIdx.x;
+  s[t] = v;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int a = t >= off ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += a;
+    __syncthreads();
+  }
+  const int incl = s[t];
+  __syncthreads();
+  return incl - v;
+}
+
+// One workgroup: offs[b * 256 + d] = first slot of digit d from tile b.
+__global__ __launch_bounds__(RT) void k_radix_offs(const int* __restrict__ hist,
+                                                   int tiles, int* __restrict__ offs) {
+  __shared__ int s[256];
+  const int d = threadIdx.x;
+  int tot = 0;
+  for (int j = 0; j < tiles; ++j) tot += hist[(size_t)j * 256 + d];
+  int run = excl_scan256(tot, s);
+  for (int j = 0; j < tiles; ++j) {
+    const int c = hist[(size_t)j * 256 + d];
+    offs[(size_t)j * 256 + d] = run;
+    run += c;
+  }
 }
 
 __global__ __launch_bounds__(RT) void k_radix_scatter(
     int n, const int* __restrict__ keys, const int* __restrict__ vals, int shift,
-    int tiles, const int* __restrict__ offs, int* __restrict__ keys_out,
-    int* __restrict__ vals_out) {
-  __shared__ int dig[RT];
-  const int t = threadIdx.x;
-  const int i = blockIdx.x * RT + t;
-  const int k = i < n ? keys[i] : 0;
-  const int d = i < n ? (k >> shift) & 255 : -1;
-  dig[t] = d;
+    int tiles, const int* __restrict__ hist, const int* __restrict__ offs,
+    int* __restrict__ keys_out, int* __restrict__ vals_out, int* __restrict__ hist_next,
+    int shift_next) {
+  constexpr int W = RT / 64;
+  __shared__ int s[256];
+  __shared__ int run[256];
+  __shared__ int wc[W][257];  // per-wave digit counts (257: idle lanes)
+  __shared__ int wb[W][256];  // first slot of (wave, digit) in this round
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int base = blockIdx.x * RTILE;
+  int k[RIPT], v[RIPT];
+#pragma unroll
+  for (int r = 0; r < RIPT; ++r) {
+    const int i = base + r * RT + t;
+    k[r] = i < n ? keys[i] : 0;
+    v[r] = i < n ? vals[i] : 0;
+  }
+  if (offs) {
+    run[t] = offs[(size_t)blockIdx.x * 256 + t];
+  } else {
+    int tot = 0, pre = 0;
+    for (int j = 0; j < tiles; ++j) {
+      const int c = hist[(size_t)j * 256 + t];
+      pre += j < (int)blockIdx.x ? c : 0;
+      tot += c;
+    }
+    run[t] = excl_scan256(tot, s) + pre;
+  }
+  for (int q = t; q < W * 257; q += RT) (&wc[0][0])[q] = 0;
   __syncthreads();
-  if (i >= n) return;
-  // rank among the earlier elements of the tile with the same digit; every
-  // thread reads the same dig[j] in step j (an LDS broadcast)
-  int rank = 0;
-  for (int j = 0; j < RT; ++j) rank += (j < t) & (dig[j] == d);
-  const int slot = offs[(size_t)d * tiles + blockIdx.x] + rank;
-  keys_out[slot] = k;
-  vals_out[slot] = vals[i];
+  const unsigned long long lt = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int r = 0; r < RIPT; ++r) {
+    const bool valid = base + r * RT + t < n;
+    const int d = valid ? (k[r] >> shift) & 255 : 256;
+    unsigned long long peers = __ballot(1);
+#pragma unroll
+    for (int b = 0; b < 9; ++b) {
+      const unsigned long long bal = __ballot((d >> b) & 1);
+      peers &= ((d >> b) & 1) ? bal : ~bal;
+    }
+    const int rank = __popcll(peers & lt);
+    if (rank == 0) wc[w][d] = __popcll(peers);
+    __syncthreads();
+    int r0 = run[t];
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      wb[q][t] = r0;
+      r0 += wc[q][t];
+      wc[q][t] = 0;
+    }
+    run[t] = r0;
+    __syncthreads();
+    if (valid) {
+      const int slot = wb[w][d] + rank;
+      keys_out[slot] = k[r];
+      vals_out[slot] = v[r];
+      if (hist_next)
+        atomicAdd(&hist_next[(size_t)(slot / RTILE) * 256 + ((k[r] >> shift_next) & 255)],
+                  1);
+    }
+  }
 }
 
 __global__ __launch_bounds__(RT) void k_copy_pairs(int n, const int* __restrict__ ka,
@@ -174,9 +268,9 @@ int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st) {
 
 size_t radix_sort_work_bytes(int n) {
   const size_t nn = (size_t)(n > 0 ? n : 1);
-  const size_t tiles = (nn + RT - 1) / RT;
-  return 2 * align256(nn * sizeof(int)) + 2 * align256((256 * tiles + 1) * sizeof(int)) +
-         scan_work_bytes((int)(256 * tiles));
+  const size_t tiles = (nn + RTILE - 1) / RTILE;
+  return 4 * align256(nn * sizeof(int)) + align256(4 * 256 * tiles * sizeof(int)) +
+         align256(256 * tiles * sizeof(int));
 }
 
 int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
@@ -185,33 +279,46 @@ int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
   SSIM_REQUIRE(bits >= 1 && bits <= 31, "radix sort: bad key width %d", bits);
   SSIM_REQUIRE(work && work_bytes >= radix_sort_work_bytes(n),
                "radix sort: workspace too small");
-  const int tiles = ceil_div(n, RT);
-  unsigned char* w = static_cast<unsigned char*>(work);
-  int* kt = reinterpret_cast<int*>(w);
-  w += align256((size_t)n * sizeof(int));
-  int* vt = reinterpret_cast<int*>(w);
-  w += align256((size_t)n * sizeof(int));
-  int* hist = reinterpret_cast<int*>(w);
-  w += align256((256 * (size_t)tiles + 1) * sizeof(int));
-  int* offs = reinterpret_cast<int*>(w);
-  w += align256((256 * (size_t)tiles + 1) * sizeof(int));
-  void* scan_work = w;
-  int *ka = keys, *va = vals, *kb = kt, *vb = vt;
+  const int tiles = ceil_div(n, RTILE);
   const int passes = (bits + 7) / 8;
-  for (int p = 0; p < passes; ++p) {
-    const int shift = 8 * p;
-    hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(RT), 0, st, n, ka, shift,
-                       tiles, hist);
-    SSIM_LAUNCHED("k_radix_hist");
-    SSIM_TRY(scan_exclusive(256 * tiles, hist, offs, scan_work, st));
-    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(RT), 0, st, n, ka, va,
-                       shift, tiles, offs, kb, vb);
-    SSIM_LAUNCHED("k_radix_scatter");
-    int* tk = ka; ka = kb; kb = tk;
-    int* tv = va; va = vb; vb = tv;
+  const size_t ht = 256 * (size_t)tiles;
+  unsigned char* w = static_cast<unsigned char*>(work);
+  int* pk[2];
+  int* pv[2];
+  for (int q = 0; q < 2; ++q) {
+    pk[q] = reinterpret_cast<int*>(w);
+    w += align256((size_t)n * sizeof(int));
+    pv[q] = reinterpret_cast<int*>(w);
+    w += align256((size_t)n * sizeof(int));
   }
-  if (ka != keys) {
-    hipLaunchKernelGGL(k_copy_pairs, dim3(tiles), dim3(RT), 0, st, n, ka, va,
+  int* hist = reinterpret_cast<int*>(w);
+  w += align256(4 * ht * sizeof(int));
+  int* offs = reinterpret_cast<int*>(w);
+  hipLaunchKernelGGL(k_radix_hist0, dim3(tiles), dim3(RT), 0, st, n, keys, tiles, hist,
+                     hist + ht, (int)((passes - 1) * ht));
+  SSIM_LAUNCHED("k_radix_hist0");
+  const int* sk = keys;
+  const int* sv = vals;
+  for (int p = 0; p < passes; ++p) {
+    const bool last = p == passes - 1 && passes >= 2;
+    int* dk = last ? keys : pk[p & 1];
+    int* dv = last ? vals : pv[p & 1];
+    const int* h = hist + p * ht;
+    const int* o = nullptr;
+    if (tiles > FUSE_TILES) {
+      hipLaunchKernelGGL(k_radix_offs, dim3(1), dim3(RT), 0, st, h, tiles, offs);
+      SSIM_LAUNCHED("k_radix_offs");
+      o = offs;
+    }
+    hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(RT), 0, st, n, sk, sv, 8 * p,
+                       tiles, h, o, dk, dv,
+                       p + 1 < passes ? hist + (p + 1) * ht : nullptr, 8 * (p + 1));
+    SSIM_LAUNCHED("k_radix_scatter");
+    sk = dk;
+    sv = dv;
+  }
+  if (passes == 1) {
+    hipLaunchKernelGGL(k_copy_pairs, dim3(ceil_div(n, RT)), dim3(RT), 0, st, n, sk, sv,
                        keys, vals);
     SSIM_LAUNCHED("k_copy_pairs");
   }

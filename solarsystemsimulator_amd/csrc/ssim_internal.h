@@ -148,6 +148,15 @@ __host__ __device__ inline double ssim_split_g(double r, double rs, double rcut)
   return g;
 }
 
+// Bin key of mesh node (i, j, k) in ssim_split_bin's blocked layout:
+// blocks of B^3 nodes, nb blocks per axis, block-major then node-major,
+//   ((i/B nb + j/B) nb + k/B) B^3 + ((i%B) B + j%B) B + k%B.
+__host__ __device__ inline int ssim_blocked_key(int i, int j, int k, int B, int nb) {
+  const int bi = i / B, bj = j / B, bk = k / B;
+  return ((bi * nb + bj) * nb + bk) * (B * B * B) +
+         ((i - bi * B) * B + (j - bj * B)) * B + (k - bk * B);
+}
+
 // Knuth TwoSum: s + err == a + b exactly (6 flops, no ordering needed).
 template <typename F>
 __device__ __forceinline__ void ssim_two_sum(F a, F b, F& s, F& err) {

@@ -80,6 +80,9 @@ class SplitForce:
         self.h = None
         self.origin = None
         self._cells_cap = 0
+        # near-field cells of TSC runs: blocks of block^3 mesh nodes, edge
+        # block * h >= rcut = rcut_rs * rs_cells * h
+        self.block = max(1, math.ceil(self.rcut_rs * self.rs_cells - 1e-9))
 
     # -- grid ---------------------------------------------------------------
     def _place(self, box):
@@ -142,15 +145,20 @@ class SplitForce:
         """Cell-sorted body order of the last bin_bodies (None before)."""
         return self.torder if self._grid is not None else None
 
-    def mesh_fields(self, x, n_mass, m):
-        """Deposit the masses and convolve: the acceleration meshes."""
+    def mesh_fields(self, x, n_mass, m, binned=False):
+        """Deposit the masses and convolve: the acceleration meshes.  TSC:
+        `binned` = the masses were binned (mass_bins) for these positions."""
         src = [c[:n_mass] for c in x]
         if self.assign == "cic":
             P.deposit(src, m[:n_mass], [self.n] * 3, self.origin, self.h, False,
                       self.prec, mesh=self.rho)
         else:
-            self._tsc_deposit(src, m[:n_mass], self.origin, self.h,
-                              D.stream(self.dev))
+            if not binned:
+                self.mass_bins(x, n_mass, m)
+            org = _lib.double_array(self.origin)
+            _lib.call("ssim_tsc_deposit_blocked", self.n, P._addr(org), self.h,
+                      self.block, _lib.ptr(self.bin_start), _lib.ptr(self.body),
+                      self.prec, _lib.ptr(self.rho), D.stream(self.dev))
         self.convolve(interleaved=self.f4 is not None)
 
     def mesh_gather(self, x, a, order=None, accumulate=False):
@@ -177,32 +185,32 @@ class SplitForce:
         self.mesh_fields(x, n_mass, m)
         return self.mesh_gather(x, a, order)
 
-    def _tsc_deposit(self, src, m, origin, h, st):
-        """rho = TSC mass mesh, without atomics: the masses are binned by
-        nearest node (stable cell list on the node-centred grid) and every
-        node sums its 27 neighbouring bins (bit-reproducible)."""
+    def mass_bins(self, x, n_mass, m):
+        """TSC runs: one stable binning of the masses (ssim_split_bin) that
+        serves the atomic-free deposit (bins = the node each mass's stencil
+        is centred on) and the near field (the node blocks are its cells)."""
         import torch
-        nm, nn = src[0].numel(), self.n ** 3
+        nm = n_mass
         if getattr(self, "_bin_n", -1) != nm:
+            nb = -(-self.n // self.block)
+            K = nb ** 3 * self.block ** 3
             i32 = dict(dtype=torch.int32, device=self.dev)
-            self.bin_start = torch.empty(nn + 1, **i32)
-            self.bin_order = torch.empty(max(nm, 1), **i32)
-            lib = _lib.load()
-            self.bwork = D.workspace(max(lib.ssim_cells_work_bytes(nm, nn),
-                                         lib.ssim_tsc_deposit_work_bytes(nm)),
-                                     self.dev)
+            self.bin_start = torch.empty(K + 1, **i32)
+            self.cell_start = torch.empty(nb ** 3 + 1, **i32)
+            self.order = torch.empty(max(nm, 1), **i32)
+            self.packed = torch.empty((max(nm, 1), 4), dtype=torch.float32,
+                                      device=self.dev)
+            self.body = torch.empty((max(nm, 1), 4), dtype=torch.float64,
+                                    device=self.dev)
+            self.bwork = D.workspace(
+                _lib.load().ssim_split_bin_work_bytes(nm, self.n, self.block), self.dev)
             self._bin_n = nm
-        gn = _lib.int_array([self.n] * 3)
-        org_b = _lib.double_array([o - 0.5 * h for o in origin])
-        org = _lib.double_array(origin)
-        _lib.call("ssim_cells_build", nm, *[_lib.ptr(c) for c in src], None,
-                  P._addr(org_b), h, P._addr(gn), _lib.ptr(self.bin_start),
-                  _lib.ptr(self.bin_order), None, _lib.ptr(self.bwork),
-                  self.bwork.numel(), st)
-        _lib.call("ssim_tsc_deposit_binned", nm, *[_lib.ptr(c) for c in src],
-                  _lib.ptr(m), self.n, P._addr(org), h, _lib.ptr(self.bin_start),
-                  _lib.ptr(self.bin_order), self.prec, _lib.ptr(self.rho),
-                  _lib.ptr(self.bwork), self.bwork.numel(), st)
+        org = _lib.double_array(self.origin)
+        _lib.call("ssim_split_bin", nm, *[_lib.ptr(c) for c in x], _lib.ptr(m), self.n,
+                  P._addr(org), self.h, self.block, _lib.ptr(self.bin_start),
+                  _lib.ptr(self.cell_start), _lib.ptr(self.order),
+                  _lib.ptr(self.packed), _lib.ptr(self.body), _lib.ptr(self.bwork),
+                  self.bwork.numel(), D.stream(self.dev))
 
     def convolve(self, interleaved=False):
         """fields = long-range acceleration meshes of self.rho (or, with
@@ -240,6 +248,9 @@ class SplitForce:
         import torch
         rs = self.rs_cells * self.h
         rcut = self.rcut_rs * rs
+        if self.assign == "tsc":
+            self.mass_bins(x, n_mass, m)
+            return self.bin_targets(x)
         lo = list(self.origin)
         extent = (self.n - 1) * self.h
         k = max(1, int(extent // rcut))
@@ -273,6 +284,36 @@ class SplitForce:
                   P._addr(org), cell, P._addr(gn), _lib.ptr(self.tcell_start),
                   _lib.ptr(self.torder), _lib.ptr(self.cwork),
                   self.cwork.numel(), st)
+        self._grid = (gn, org, cell, rs, rcut)
+        return self._grid
+
+    def bin_targets(self, x):
+        """TSC runs: cell list of all targets on the grid of node blocks
+        (origin o - h/2, edge block * h); returns the grid."""
+        import torch
+        rs = self.rs_cells * self.h
+        rcut = self.rcut_rs * rs
+        nb = -(-self.n // self.block)
+        nc = [nb, nb, nb]
+        ncells = nb ** 3
+        cell = self.block * self.h
+        lo = [o - 0.5 * self.h for o in self.origin]
+        nt = x[0].numel()
+        if self._nt != nt or self._cells_cap != ncells:
+            i32 = dict(dtype=torch.int32, device=self.dev)
+            self.tcell_start = torch.empty(ncells + 1, **i32)
+            self.torder = torch.empty(max(nt, 1), **i32)
+            lib = _lib.load()
+            wb = max(lib.ssim_cells_bin_work_bytes(nt, ncells),
+                     lib.ssim_pp_work_bytes(nt, ncells))
+            self.cwork = D.workspace(wb, self.dev)
+            self._cells_cap, self._nt = ncells, nt
+        gn = _lib.int_array(nc)
+        org = _lib.double_array(lo)
+        _lib.call("ssim_cells_bin", nt, *[_lib.ptr(c) for c in x],
+                  P._addr(org), cell, P._addr(gn), _lib.ptr(self.tcell_start),
+                  _lib.ptr(self.torder), _lib.ptr(self.cwork),
+                  self.cwork.numel(), D.stream(self.dev))
         self._grid = (gn, org, cell, rs, rcut)
         return self._grid
 
@@ -314,15 +355,25 @@ class SplitForce:
         masses first, tracers after them); `placed`: the caller has placed
         the mesh for x already (place_box), so nothing waits for the host.
 
-        With TSC the near field runs on a second stream while the current
-        one deposits and convolves: the near-field kernel is bound by the
-        vector ALU, the FFT passes by memory and latency, so they overlap.
-        The side stream zeroes a, builds the cell lists and adds the near
-        field; the current stream then waits for it and adds the mesh gather
-        (in cell order).  With CIC everything runs in order on one stream."""
+        With TSC the work runs on two streams: the side stream zeroes a and
+        bins the targets while the current one bins the masses (one binning
+        for the deposit and the near-field sources); then the side stream
+        adds the near field (bound by the vector ALU) while the current one
+        deposits and convolves (bound by memory and latency), and the
+        current stream waits for it and adds the mesh gather (in cell
+        order).  With CIC everything runs in order on one stream."""
         import torch
         if not placed:
             self.place(x)
+        if self.assign == "tsc" and not self.overlap:
+            for c in a:
+                c.zero_()
+            self.bin_targets(x)
+            self.mass_bins(x, n_mass, m)
+            self.near_sum(x, a)
+            self.mesh_fields(x, n_mass, m, binned=True)
+            self.mesh_gather(x, a, order=self.torder, accumulate=True)
+            return a
         if self.assign != "tsc":
             self.bin_bodies(x, n_mass, m)
             self.mesh_accel(x, n_mass, m, a, order=self.torder, placed=True)
@@ -337,16 +388,22 @@ class SplitForce:
         with torch.cuda.stream(self._side):
             for c in a:
                 c.zero_()
-            self.bin_bodies(x, n_mass, m)
+            self.bin_targets(x)
+        self.mass_bins(x, n_mass, m)
+        binned = torch.cuda.Event()
+        binned.record(main)
+        self._side.wait_event(binned)
+        with torch.cuda.stream(self._side):
             self.near_sum(x, a)
         done = torch.cuda.Event()
         done.record(self._side)
-        self.mesh_fields(x, n_mass, m)
+        self.mesh_fields(x, n_mass, m, binned=True)
         main.wait_event(done)
         self.mesh_gather(x, a, order=self.torder, accumulate=True)
         return a
 
     _side = None
+    overlap = True        # TSC: near field on a second stream
 
 
 def relative_errors(a, ref):

@@ -217,6 +217,55 @@ def test_split_force_assignments_on_cube(dev, assign):
         assert errs["tsc"] < 0.5 * errs["cic"], errs
 
 
+@pytest.mark.parametrize("B", [5, 3, 2])
+def test_split_bins_and_blocked_deposit(dev, B):
+    """ssim_split_bin equals the oracle's binning exactly (stable order, bin
+    and block starts, both packed copies); the blocked deposit matches the
+    TSC oracle and is bit-identical run to run."""
+    print(f"split bins B={B}", flush=True)
+    import ctypes
+    n, h, nbody = 24, 0.05, 40_000
+    origin = [0.1, -0.2, 0.3]
+    rng = np.random.default_rng(12)
+    pos = [origin[a] + rng.uniform(1.0, n - 2.0, nbody) * h for a in range(3)]
+    pos[0][:500] = origin[0] + (np.arange(500) % 20 + 0.5) * h   # on bin edges
+    m = rng.uniform(0.5, 2.0, nbody)
+    x = [D.to_dev(c, dev) for c in pos]
+    md = D.to_dev(m, dev)
+    nb = -(-n // B)
+    K = nb ** 3 * B ** 3
+    i32 = dict(dtype=torch.int32, device=dev)
+    start = torch.empty(K + 1, **i32)
+    cstart = torch.empty(nb ** 3 + 1, **i32)
+    order = torch.empty(nbody, **i32)
+    packed = torch.empty((nbody, 4), dtype=torch.float32, device=dev)
+    body = torch.empty((nbody, 4), dtype=torch.float64, device=dev)
+    w = D.workspace(_lib.load().ssim_split_bin_work_bytes(nbody, n, B), dev)
+    org = _lib.double_array(origin)
+    _lib.call("ssim_split_bin", nbody, *[_lib.ptr(c) for c in x], _lib.ptr(md), n,
+              ctypes.addressof(org), h, B, _lib.ptr(start), _lib.ptr(cstart),
+              _lib.ptr(order), _lib.ptr(packed), _lib.ptr(body), _lib.ptr(w),
+              w.numel(), D.stream(dev))
+    rs, rc, ro = p3m_np.split_bins(pos, n, origin, h, B)
+    assert np.array_equal(start.cpu().numpy(), rs)
+    assert np.array_equal(cstart.cpu().numpy(), rc)
+    assert np.array_equal(order.cpu().numpy(), ro)
+    pk, bd = packed.cpu().numpy(), body.cpu().numpy()
+    for c in range(3):
+        assert np.array_equal(bd[:, c], (pos[c][ro] - origin[c]) / h)
+        assert np.array_equal(pk[:, c], (pos[c][ro] - (origin[c] - 0.5 * h)).astype(np.float32))
+    assert np.array_equal(bd[:, 3], m[ro])
+    meshes = []
+    for _ in range(2):
+        mb = torch.full((n, n, n), 7.0, dtype=torch.float64, device=dev)
+        _lib.call("ssim_tsc_deposit_blocked", n, ctypes.addressof(org), h, B,
+                  _lib.ptr(start), _lib.ptr(body), 0, _lib.ptr(mb), D.stream(dev))
+        meshes.append(mb.cpu().numpy())
+    assert np.array_equal(meshes[0], meshes[1])
+    ref = p3m_np.tsc_deposit(pos, m, n, origin, h)
+    np.testing.assert_allclose(meshes[0], ref, rtol=1e-12, atol=1e-12 * ref.max())
+
+
 @pytest.mark.parametrize("prec,tol,n", [(0, 1e-12, 32), (1, 1e-5, 32), (1, 1e-5, 128),
                                         (0, 1e-12, 128)])
 def test_pruned_convolution_matches_padded_fft(dev, prec, tol, n):

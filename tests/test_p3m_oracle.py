@@ -87,3 +87,60 @@ def test_tsc_mass_node_weights_linear_and_adjoint():
     # gather is the adjoint of deposit
     f = rng.normal(size=(n, n, n))
     assert abs((rho * f).sum() - (m * S.tsc_gather(f, pos, n, origin, h)).sum()) < 1e-10
+
+
+def test_split_bins_brute_force_and_tsc_consistency():
+    """split_bins against a brute-force binning (nearest node found by
+    search, Python's sort of (block, node in block, index) tuples), and the
+    deposit summed bin by bin over every node's 27 neighbouring bins equals
+    tsc_deposit (so a body's bin is always its stencil's centre)."""
+    rng = np.random.default_rng(11)
+    n, h, B, nbody = 12, 0.1, 5, 2000
+    origin = [0.2, -0.3, 0.05]
+    pos = [origin[c] + rng.uniform(1.0, n - 2.0, nbody) * h for c in range(3)]
+    m = rng.uniform(0.5, 2.0, nbody)
+    start, cstart, order = S.split_bins(pos, n, origin, h, B)
+    nb = -(-n // B)
+    nodes = np.arange(n)
+    recs = []
+    for i in range(nbody):
+        r = [int(nodes[np.argmin(np.abs((pos[c][i] - origin[c]) / h - nodes))])
+             for c in range(3)]
+        recs.append((tuple(rc // B for rc in r), tuple(rc % B for rc in r), i))
+    recs.sort()
+    assert [rec[2] for rec in recs] == list(order)
+    # bin sizes in key order, block sizes in block order
+    bins = {}
+    for blk, loc, _ in recs:
+        bins[(blk, loc)] = bins.get((blk, loc), 0) + 1
+    keys = [(bi, bj, bk, li, lj, lk) for bi in range(nb) for bj in range(nb)
+            for bk in range(nb) for li in range(B) for lj in range(B) for lk in range(B)]
+    sizes = [bins.get((k[:3], k[3:]), 0) for k in keys]
+    assert np.array_equal(np.diff(start), sizes)
+    assert np.array_equal(np.diff(cstart), np.add.reduceat(sizes, np.arange(0, len(keys), B ** 3)))
+    # deposit from the bins
+    mesh = np.zeros((n, n, n))
+    key_of = {k: q for q, k in enumerate(keys)}
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                acc = 0.0
+                for a in (-1, 0, 1):
+                    for b in (-1, 0, 1):
+                        for c in (-1, 0, 1):
+                            node = (i + a, j + b, k + c)
+                            if min(node) < 0 or max(node) >= n:
+                                continue
+                            q = key_of[tuple(x // B for x in node) + tuple(x % B for x in node)]
+                            for s in order[start[q]:start[q + 1]]:
+                                w = 1.0
+                                for ax, nd in zip(range(3), (i, j, k)):
+                                    f = (pos[ax][s] - origin[ax]) / h
+                                    cc, wts = S.tsc_weights(np.array([f]))
+                                    off = nd - int(cc[0])
+                                    assert -1 <= off <= 1
+                                    w *= wts[off + 1][0]
+                                acc += m[s] * w
+                mesh[i, j, k] = acc
+    np.testing.assert_allclose(mesh, S.tsc_deposit(pos, m, n, origin, h), rtol=1e-12,
+                               atol=1e-12)
