@@ -289,6 +289,20 @@ int ssim_tsc_gather(int np, const int* order, const double* x, const double* y,
                     int mesh_prec, const void* f0, const void* f1, const void* f2,
                     double* o0, double* o1, double* o2, int accumulate,
                     void* stream);
+/* ssim_tsc_gather of the interleaved FP32 fields (accumulate as there)
+ * with the learned correction folded in: corr[np][3] are the network's
+ * outputs (ssim_mlp_forward with out != NULL and no ax), s the gathered
+ * split force of a body, u = scale |s| (relative != 0) or scale, d = u corr
+ * without its component along (v0, v1, v2) for bodies below n_proj, and
+ * s + d is stored — what ssim_mlp_forward would add afterwards, with the
+ * same operations, in one pass over the bodies.  No reference counterpart. */
+int ssim_tsc_gather_corrected(int np, const int* order, const double* x,
+                              const double* y, const double* z, int grid_n,
+                              const double* origin, double h, const void* f4,
+                              double* o0, double* o1, double* o2, int accumulate,
+                              const float* corr, float scale, int relative,
+                              const double* v0, const double* v1, const double* v2,
+                              int n_proj, void* stream);
 /* out = a * b over ncomplex interleaved complex values (the spectral
  * products of pm_force_field, pm_kernel.py:144-146). */
 int ssim_spectrum_mul(long long ncomplex, int prec, const void* a,

@@ -84,6 +84,9 @@ SIGNATURES = {
                                          _P, _P]),
     "ssim_tsc_gather": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double, c_int,
                                 _P, _P, _P, _P, _P, _P, c_int, _P]),
+    "ssim_tsc_gather_corrected": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double,
+                                          _P, _P, _P, _P, c_int, _P, c_float, c_int,
+                                          _P, _P, _P, c_int, _P]),
     "ssim_spectrum_mul": (c_int, [c_longlong, c_int, _P, _P, _P, _P]),
     "ssim_spectrum_imag": (c_int, [c_longlong, c_int, _P, _P, _P]),
     "ssim_pm_conv_isolated3": (c_int, [c_int, c_int] + [_P] * 9 + [_P, c_size_t,

@@ -447,11 +447,25 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather(
 
 // The same gather from three FP32 fields interleaved as float4 [n^3]
 // (one 16-byte load per node instead of three 4-byte loads).
+// Learned correction folded into the gather (k_tsc_gather4<true>): corr
+// holds the network's three outputs per body; with the split force s at the
+// body (after the accumulate), u = scale |s| (relative) or scale, d = u corr,
+// bodies below n_proj lose the component of d along their velocity, and
+// s + d is stored: the operations of ssim_mlp_forward's own update.
+struct Corr {
+  const float* out;
+  float scale;
+  int relative;
+  const double *v0, *v1, *v2;
+  int n_proj;
+};
+
+template <bool CORR>
 __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
     int np, const int* __restrict__ order, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, Grid g, const float4* __restrict__ f,
     double* __restrict__ o0, double* __restrict__ o1, double* __restrict__ o2,
-    int accumulate) {
+    int accumulate, Corr cr) {
   const int i_ = blockIdx.x * PM_BLOCK + threadIdx.x;
   if (i_ >= np) return;
   const int p = order ? order[i_] : i_;
@@ -483,15 +497,32 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
       }
     }
   }
-  if (accumulate) {
-    o0[p] += v0;
-    o1[p] += v1;
-    o2[p] += v2;
-  } else {
-    o0[p] = v0;
-    o1[p] = v1;
-    o2[p] = v2;
+  double s0 = accumulate ? o0[p] + v0 : v0;
+  double s1 = accumulate ? o1[p] + v1 : v1;
+  double s2 = accumulate ? o2[p] + v2 : v2;
+  if (CORR) {
+    const double u = cr.relative ? (double)cr.scale * sqrt(s0 * s0 + s1 * s1 + s2 * s2)
+                                 : (double)cr.scale;
+    double d0 = u * (double)cr.out[(size_t)p * 3 + 0];
+    double d1 = u * (double)cr.out[(size_t)p * 3 + 1];
+    double d2 = u * (double)cr.out[(size_t)p * 3 + 2];
+    if (p < cr.n_proj) {
+      const double w0 = cr.v0[p], w1 = cr.v1[p], w2 = cr.v2[p];
+      const double vv = w0 * w0 + w1 * w1 + w2 * w2;
+      if (vv > 0.0) {
+        const double cc = (d0 * w0 + d1 * w1 + d2 * w2) / vv;
+        d0 -= cc * w0;
+        d1 -= cc * w1;
+        d2 -= cc * w2;
+      }
+    }
+    s0 += d0;
+    s1 += d1;
+    s2 += d2;
   }
+  o0[p] = s0;
+  o1[p] = s1;
+  o2[p] = s2;
 }
 
 // One workgroup folds the per-block extrema (min and max are exact, so the
@@ -1001,10 +1032,10 @@ int ssim_tsc_gather(int np, const int* order, const double* x, const double* y,
     const Grid g = make_grid(3, ns, origin, h, 0);
     hipStream_t st = ssim::as_stream(stream);
     SSIM_TIMED("k_tsc_gather", st,
-               hipLaunchKernelGGL(k_tsc_gather4, dim3(ssim::ceil_div(np, PM_BLOCK)),
+               hipLaunchKernelGGL(k_tsc_gather4<false>, dim3(ssim::ceil_div(np, PM_BLOCK)),
                                   dim3(PM_BLOCK), 0, st, np, order, x, y, z, g,
                                   static_cast<const float4*>(f0), o0, o1, o2,
-                                  accumulate));
+                                  accumulate, Corr{}));
     SSIM_LAUNCHED("k_tsc_gather4");
     return SSIM_OK;
   }
@@ -1027,6 +1058,30 @@ int ssim_tsc_gather(int np, const int* order, const double* x, const double* y,
                                   static_cast<const float*>(f2), o0, o1, o2,
                                   accumulate));
   SSIM_LAUNCHED("k_tsc_gather");
+  return SSIM_OK;
+}
+
+int ssim_tsc_gather_corrected(int np, const int* order, const double* x,
+                              const double* y, const double* z, int grid_n,
+                              const double* origin, double h, const void* f4,
+                              double* o0, double* o1, double* o2, int accumulate,
+                              const float* corr, float scale, int relative,
+                              const double* v0, const double* v1, const double* v2,
+                              int n_proj, void* stream) {
+  SSIM_REQUIRE(np >= 0 && grid_n >= 1 && h > 0.0, "bad TSC arguments");
+  if (np == 0) return SSIM_OK;
+  SSIM_REQUIRE(x && y && z && origin && f4 && o0 && o1 && o2 && corr, "null pointer");
+  SSIM_REQUIRE(n_proj <= 0 || (v0 && v1 && v2), "n_proj > 0 needs the velocities");
+  const int ns[3] = {grid_n, grid_n, grid_n};
+  const Grid g = make_grid(3, ns, origin, h, 0);
+  hipStream_t st = ssim::as_stream(stream);
+  const Corr cr{corr, scale, relative, v0, v1, v2, n_proj};
+  SSIM_TIMED("k_tsc_gather", st,
+             hipLaunchKernelGGL(k_tsc_gather4<true>, dim3(ssim::ceil_div(np, PM_BLOCK)),
+                                dim3(PM_BLOCK), 0, st, np, order, x, y, z, g,
+                                static_cast<const float4*>(f4), o0, o1, o2, accumulate,
+                                cr));
+  SSIM_LAUNCHED("k_tsc_gather4");
   return SSIM_OK;
 }
 

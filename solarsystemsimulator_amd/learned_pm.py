@@ -99,11 +99,29 @@ class PMLearnedSystem:
                                  self.split.h, self.split.prec, out=self.feat,
                                  n_mass=self.n)
 
+    # fold the learned correction into the mesh gather (features and the
+    # network run beside the convolution); False: a separate update after it
+    fold = True
+
+    def _correction(self):
+        """Features and network outputs (no update) for the gather."""
+        torch = _torch()
+        feat = self.features()
+        if getattr(self, "_corr_out", None) is None:
+            self._corr_out = torch.empty((self.n + self.t, 3), dtype=torch.float32,
+                                         device=self.dev)
+        self.net.predict(feat, out=self._corr_out)
+        return dict(out=self._corr_out, scale=self.out_unit(),
+                    relative=self.net.relative, vel=self.v, n_proj=self.n)
+
     def force(self, learned: bool = True, placed: bool = False):
         """a = split force (+ learned correction) at the current positions;
         `placed`: the mesh is placed for them already (run's steps)."""
-        self.split.accelerations(self.x, self.n, self.m, self.a, placed=placed)
-        if learned and self.net is not None:
+        fold = (learned and self.net is not None and self.fold and
+                self.split.can_fold_correction())
+        self.split.accelerations(self.x, self.n, self.m, self.a, placed=placed,
+                                 correction=self._correction if fold else None)
+        if learned and self.net is not None and not fold:
             feat = self.features()
             # masses: no component along the velocity (no work done on
             # the self-gravitating system); tracers: the full correction
@@ -167,7 +185,8 @@ class PMLearnedSystem:
         """The captured step for the current placement (captured anew when
         the placement, dt or the learned flag changed)."""
         torch = _torch()
-        key = (self.split.placement(), float(dt), bool(learned))
+        key = (self.split.placement(), float(dt), bool(learned), self.fold,
+               self.split.overlap)
         if getattr(self, "_graph_key", None) != key:
             self._graph = None
             g = torch.cuda.CUDAGraph()

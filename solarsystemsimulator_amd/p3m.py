@@ -145,9 +145,9 @@ class SplitForce:
         """Cell-sorted body order of the last bin_bodies (None before)."""
         return self.torder if self._grid is not None else None
 
-    def mesh_fields(self, x, n_mass, m, binned=False):
-        """Deposit the masses and convolve: the acceleration meshes.  TSC:
-        `binned` = the masses were binned (mass_bins) for these positions."""
+    def deposit(self, x, n_mass, m, binned=False):
+        """rho = the mass mesh.  TSC: `binned` = the masses were binned
+        (mass_bins) for these positions."""
         src = [c[:n_mass] for c in x]
         if self.assign == "cic":
             P.deposit(src, m[:n_mass], [self.n] * 3, self.origin, self.h, False,
@@ -159,10 +159,32 @@ class SplitForce:
             _lib.call("ssim_tsc_deposit_blocked", self.n, P._addr(org), self.h,
                       self.block, _lib.ptr(self.bin_start), _lib.ptr(self.body),
                       self.prec, _lib.ptr(self.rho), D.stream(self.dev))
+
+    def mesh_fields(self, x, n_mass, m, binned=False):
+        """Deposit the masses and convolve: the acceleration meshes."""
+        self.deposit(x, n_mass, m, binned)
         self.convolve(interleaved=self.f4 is not None)
 
-    def mesh_gather(self, x, a, order=None, accumulate=False):
-        """a (+)= the meshes interpolated at the bodies x."""
+    def can_fold_correction(self):
+        """True when the gather can add a learned correction itself (TSC
+        with the interleaved FP32 fields)."""
+        return self.assign == "tsc" and self.f4 is not None
+
+    def mesh_gather(self, x, a, order=None, accumulate=False, corr=None):
+        """a (+)= the meshes interpolated at the bodies x; `corr` (a dict of
+        out, scale, relative, vel, n_proj) folds a learned correction in."""
+        if corr is not None:
+            if not self.can_fold_correction():
+                raise ValueError("the correction folds into the TSC float4 gather only")
+            org = _lib.double_array(self.origin)
+            v = corr.get("vel") or [None, None, None]
+            _lib.call("ssim_tsc_gather_corrected", x[0].numel(), _lib.ptr(order),
+                      *[_lib.ptr(c) for c in x], self.n, P._addr(org), self.h,
+                      _lib.ptr(self.f4), *[_lib.ptr(c) for c in a], int(accumulate),
+                      _lib.ptr(corr["out"]), float(corr["scale"]),
+                      int(corr["relative"]), *[_lib.ptr(c) for c in v],
+                      int(corr.get("n_proj", 0)), D.stream(self.dev))
+            return a
         if self.assign == "cic":
             if accumulate:
                 raise ValueError("the CIC gather overwrites")
@@ -350,29 +372,38 @@ class SplitForce:
     _nt = -1
     _grid = None
 
-    def accelerations(self, x, n_mass, m, a, placed=False):
+    def accelerations(self, x, n_mass, m, a, placed=False, correction=None):
         """a = mesh + near-field acceleration of all bodies in x (n_mass
         masses first, tracers after them); `placed`: the caller has placed
         the mesh for x already (place_box), so nothing waits for the host.
+        `correction` (can_fold_correction() only): a callable, launched
+        once the mass mesh is deposited, that enqueues a learned correction
+        on the current stream and returns mesh_gather's `corr` dict; the
+        gather then adds it.
 
         With TSC the work runs on two streams: the side stream zeroes a and
         bins the targets while the current one bins the masses (one binning
         for the deposit and the near-field sources); then the side stream
-        adds the near field (bound by the vector ALU) while the current one
-        deposits and convolves (bound by memory and latency), and the
-        current stream waits for it and adds the mesh gather (in cell
-        order).  With CIC everything runs in order on one stream."""
+        adds the near field (bound by the vector ALU) and, once the mass
+        mesh is deposited, runs the correction, while the current one
+        deposits and convolves (bound by memory and latency); the current
+        stream waits for it and adds the mesh gather (in cell order).  With
+        CIC everything runs in order on one stream."""
         import torch
         if not placed:
             self.place(x)
+        if correction is not None and not self.can_fold_correction():
+            raise ValueError("correction needs TSC with interleaved FP32 fields")
         if self.assign == "tsc" and not self.overlap:
             for c in a:
                 c.zero_()
             self.bin_targets(x)
             self.mass_bins(x, n_mass, m)
             self.near_sum(x, a)
-            self.mesh_fields(x, n_mass, m, binned=True)
-            self.mesh_gather(x, a, order=self.torder, accumulate=True)
+            self.deposit(x, n_mass, m, binned=True)
+            corr = correction() if correction is not None else None
+            self.convolve(interleaved=self.f4 is not None)
+            self.mesh_gather(x, a, order=self.torder, accumulate=True, corr=corr)
             return a
         if self.assign != "tsc":
             self.bin_bodies(x, n_mass, m)
@@ -395,11 +426,19 @@ class SplitForce:
         self._side.wait_event(binned)
         with torch.cuda.stream(self._side):
             self.near_sum(x, a)
+        self.deposit(x, n_mass, m, binned=True)
+        corr = None
+        if correction is not None:
+            deposited = torch.cuda.Event()
+            deposited.record(main)
+            self._side.wait_event(deposited)
+            with torch.cuda.stream(self._side):
+                corr = correction()
         done = torch.cuda.Event()
         done.record(self._side)
-        self.mesh_fields(x, n_mass, m, binned=True)
+        self.convolve(interleaved=self.f4 is not None)
         main.wait_event(done)
-        self.mesh_gather(x, a, order=self.torder, accumulate=True)
+        self.mesh_gather(x, a, order=self.torder, accumulate=True, corr=corr)
         return a
 
     _side = None

@@ -70,3 +70,27 @@ def test_replayed_steps_equal_eager_steps(dev):
         assert torch.equal(eager.a[c], replay.a[c])
     # the eager run re-placed the mesh the same way
     assert eager.split.placement() == replay.split.placement()
+
+
+@pytest.mark.parametrize("overlap", [True, False])
+def test_folded_correction_equals_separate_update(dev, overlap):
+    """The learned correction added inside the mesh gather (network beside
+    the convolution) equals the separate update after it, bit for bit,
+    with moving masses (the velocity projection) and tracers."""
+    print(f"folded correction overlap={overlap}", flush=True)
+    pos, m, tpos = scenes.learned_scene((20, 20, 10), (20, 20, 20), seed=8,
+                                        tracer_seed=88)
+    n, t = len(m), len(tpos[0])
+    rng = np.random.default_rng(9)
+    vel = [rng.normal(0.0, 0.3, n) for _ in range(3)]
+    tvel = [rng.normal(0.0, 0.3, t) for _ in range(3)]
+    s = _system(dev, pos, m, tpos, vel, tvel)
+    s.split.overlap = overlap
+    s.fold = True
+    folded = [c.clone() for c in s.force(learned=True)]
+    s.fold = False
+    separate = [c.clone() for c in s.force(learned=True)]
+    plain = [c.clone() for c in s.force(learned=False)]
+    for c in range(3):
+        assert torch.equal(folded[c], separate[c])
+        assert not torch.equal(folded[c], plain[c])
