@@ -112,6 +112,11 @@ int ssim_kick(int dim, int n, double* vx, double* vy, double* vz,
 int ssim_drift(int dim, int n, double* x, double* y, double* z,
                const double* vx, const double* vy, const double* vz, double h,
                void* stream);
+/* v += a*half then x += v*full (the opening half of a KDK step) in one
+ * pass, with the rounding of ssim_kick followed by ssim_drift. */
+int ssim_kick_drift(int dim, int n, double* x, double* y, double* z, double* vx,
+                    double* vy, double* vz, const double* ax, const double* ay,
+                    const double* az, double half, double full, void* stream);
 /* Half-kick v += a*h of n masses + t tracers followed (if snap != NULL) by
  * the f32 snapshot row described at ssim_selfgrav_run; used by integrators
  * whose force comes from elsewhere (particle mesh, NBodyPM.step_batch,
@@ -233,6 +238,17 @@ int ssim_box_after_kdk(int dim, int n, const double* x, const double* y,
                        const double* vz, const double* ax, const double* ay,
                        const double* az, double half, double full, double* out,
                        void* work, size_t work_bytes, void* stream);
+/* The closing half kick of a step, v += a half (as ssim_kick), fused with
+ * ssim_box_after_kdk for the next step (out = the box of x + (v + a half)
+ * full with the kicked v), in one launch.  The first 4 bytes of work (size
+ * ssim_kick_box_work_bytes(n)) must be zero before the first call (a
+ * zero-filled buffer); every call leaves them zero.  Calls sharing a work
+ * buffer must be ordered.  No reference counterpart. */
+size_t ssim_kick_box_work_bytes(int n);
+int ssim_kick_box(int dim, int n, const double* x, const double* y, const double* z,
+                  double* vx, double* vy, double* vz, const double* ax,
+                  const double* ay, const double* az, double half, double full,
+                  double* out, void* work, size_t work_bytes, void* stream);
 /* Real-space force Green's function component `comp` on the zero-padded
  * (2n)^dim mesh: -G d_comp / (|d|^2 + eps2)^(3/2) with wrapped offsets
  * (build_force_kernels, pm_kernel.py:34-60, generalised to 3-D).  rs > 0
@@ -384,9 +400,10 @@ int ssim_split_bin(int n, const double* x, const double* y, const double* z,
                    const double* m, int grid_n, const double* origin, double h,
                    int block, int* bin_start, int* cell_start, int* order, void* packed,
                    void* body, void* work, size_t work_bytes, void* stream);
-/* ADD the near-field force G m d g(r) / (r^2 + eps2)^(3/2), r < rcut, of
- * the binned sources (cell_start + packed, from ssim_cells_build) to nt
- * targets' accelerations (3-D).  The targets are binned on the same grid
+/* ADD (accumulate != 0) or STORE the near-field force G m d g(r) /
+ * (r^2 + eps2)^(3/2), r < rcut, of the binned sources (cell_start +
+ * packed, from ssim_cells_build or ssim_split_bin) to / into nt targets'
+ * accelerations (3-D); every target is written once.  The targets are binned on the same grid
  * (tcell_start, torder from ssim_cells_bin or ssim_cells_build) and
  * rcut <= cell.  Each target's sum runs over the sources in their sorted
  * order, so the result is deterministic.  New: the reference has no split
@@ -397,7 +414,8 @@ int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,
                   const int* cell_start, const void* packed,
                   const int* tcell_start, const int* torder, double G,
                   double eps2, double rs, double rcut, double* ax, double* ay,
-                  double* az, void* work, size_t work_bytes, void* stream);
+                  double* az, int accumulate, void* work, size_t work_bytes,
+                  void* stream);
 /* Work accounting of ssim_pp_short with the same arguments: counts[0] =
  * source-target pairs tested, counts[1] = pairs closer than rcut (a body
  * with itself excluded); counts is a device array of 2, no forces. */

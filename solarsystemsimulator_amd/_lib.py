@@ -46,6 +46,7 @@ SIGNATURES = {
                                   _P, c_longlong, _P, c_size_t, _P]),
     "ssim_kick": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, c_double, _P]),
     "ssim_drift": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, c_double, _P]),
+    "ssim_kick_drift": (c_int, [c_int, c_int] + [_P] * 9 + [c_double, c_double, _P]),
     "ssim_kick_snap": (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P, _P,
                                _P, _P, c_double, _P, _P]),
     "ssim_fill_f32": (c_int, [c_longlong, _P, c_float, _P]),
@@ -71,6 +72,9 @@ SIGNATURES = {
     "ssim_minmax": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, c_size_t, _P]),
     "ssim_box_after_kdk": (c_int, [c_int, c_int] + [_P] * 9 + [c_double, c_double, _P,
                                                              _P, c_size_t, _P]),
+    "ssim_kick_box_work_bytes": (c_size_t, [c_int]),
+    "ssim_kick_box": (c_int, [c_int, c_int] + [_P] * 9 + [c_double, c_double, _P, _P,
+                                                        c_size_t, _P]),
     "ssim_pm_force_kernel": (c_int, [c_int, c_int, c_int, c_double, c_double,
                                      c_double, c_double, c_double, c_int, _P,
                                      _P]),
@@ -107,7 +111,7 @@ SIGNATURES = {
     "ssim_pp_work_bytes": (c_size_t, [c_int, c_int]),
     "ssim_pp_short": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P,
                               _P, c_double, c_double, c_double, c_double, _P,
-                              _P, _P, _P, c_size_t, _P]),
+                              _P, _P, c_int, _P, c_size_t, _P]),
     "ssim_pp_count": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P,
                               _P, c_double, c_double, c_double, _P, _P, c_size_t,
                               _P]),

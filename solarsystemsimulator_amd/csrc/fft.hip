@@ -442,7 +442,7 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
 // half-size writes.
 constexpr int HUB_LS = 258;
 
-template <typename F, int NL>
+template <typename F, int NL, int LS = HUB_LS>
 __device__ __forceinline__ void hub_exchange(cpx<F> v[16], cpx<F>* s, int l, int j,
                                              const cpx<F>* tw, bool inverse) {
   // v = DFT16 outputs of index n1 = j; twiddle, exchange, DFT16 over n1
@@ -453,7 +453,7 @@ __device__ __forceinline__ void hub_exchange(cpx<F> v[16], cpx<F>* s, int l, int
     v[k2] = cmul(v[k2], w);
   }
   __syncthreads();  // the previous exchange's reads are done
-  cpx<F>* a = s + l * HUB_LS;
+  cpx<F>* a = s + l * LS;
 #pragma unroll
   for (int k2 = 0; k2 < 16; ++k2) a[16 * j + (k2 ^ j)] = v[k2];
   __syncthreads();
@@ -503,6 +503,98 @@ __global__ __launch_bounds__(NL * 16) void k_conv_x3(
     hub_exchange<F, NL>(v, s, l, j, stw, true);
 #pragma unroll
     for (int k1 = 0; k1 < 8; ++k1) out[base + k1 * step] = v[k1];
+  }
+}
+
+// ---- 256-point lines of the isolated convolution, register-direct ----
+// The y and z passes of the padded 2n = 256 mesh (n = 128, H = 129) in the
+// style of k_conv_x3: thread (line l, index j) loads its 16 (or 8) inputs
+// straight from HBM, runs the four-step FFT with one LDS exchange, and stores
+// its outputs straight from registers; only the x < n planes exist.
+//   LY_FWD: y lines of plane x (kz = line % H), inputs ky < n, all 256 out;
+//   LY_INV: the same lines, all 256 in, ky < n out;
+//   LZ_R2C: rows (x, y) of the real n^3 mesh (z < n, zeros beyond), the
+//           H = n + 1 half-spectrum values out;
+//   LZ_C2R: rows (x, y) of the half spectrum (Hermitian completion), z < n
+//           real out, times `scale`, at out[(row * n + z) * es + ec].
+// y lines: lanes take consecutive lines (consecutive kz: 128-byte rows);
+// z rows: lanes take consecutive indices of one row.
+enum { LY_FWD = 0, LY_INV = 1, LZ_R2C = 2, LZ_C2R = 3 };
+
+template <typename F, int NL, int KIND>
+__global__ __launch_bounds__(NL * 16) void k_line256(
+    const void* vin, void* vout, const cpx<F>* __restrict__ tw,  // y passes: in place
+    int lines, F scale, int es, int ec) {
+  constexpr int n = 128, H = 129, PH = 256 * 129;
+  constexpr bool YL = KIND == LY_FWD || KIND == LY_INV;
+  // exchange rows: 258 apart when lanes take consecutive lines, 272 when
+  // they take consecutive indices of a line (both free of bank conflicts)
+  constexpr int LS = YL ? HUB_LS : 272;
+  __shared__ cpx<F> s[NL * LS];
+  __shared__ cpx<F> stw[256];
+  for (int t = threadIdx.x; t < 256; t += NL * 16) stw[t] = tw[t];
+  __syncthreads();
+  const int l = YL ? threadIdx.x % NL : threadIdx.x / 16;
+  const int j = YL ? threadIdx.x / NL : threadIdx.x % 16;
+  const int line = blockIdx.x * NL + l;
+  const bool live = line < lines;
+  const int lc = live ? line : 0;
+  cpx<F> v[16];
+  if (YL) {
+    // line = x * H + kz; element ky at x * PH + ky * H + kz
+    const int x = lc / H, kz = lc - x * H;
+    const cpx<F>* in = static_cast<const cpx<F>*>(vin) + (size_t)x * PH + kz;
+    const int nin = KIND == LY_FWD ? 8 : 16;  // n2 < 8: ky = j + 16 n2 < n
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2)
+      v[n2] = (n2 < nin && live) ? in[(j + 16 * n2) * H] : cpx<F>{F(0), F(0)};
+  } else if (KIND == LZ_R2C) {
+    const F* in = static_cast<const F*>(vin) + (size_t)lc * n;
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2)
+      v[n2] = {(n2 < 8 && live) ? in[j + 16 * n2] : F(0), F(0)};
+  } else {
+    // row (x, y) = lc of the half spectrum at x * PH + y * H; e > n mirrored
+    const int x = lc / n, y = lc - x * n;
+    const cpx<F>* row = static_cast<const cpx<F>*>(vin) + (size_t)x * PH + y * H;
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) {
+      const int e = j + 16 * n2;
+      cpx<F> c = {F(0), F(0)};
+      if (live) {
+        if (e <= n) {
+          c = row[e];
+        } else {
+          const cpx<F> m = row[256 - e];
+          c = {m.re, -m.im};
+        }
+      }
+      v[n2] = c;
+    }
+  }
+  const bool inverse = KIND == LY_INV || KIND == LZ_C2R;
+  dft16(v, inverse);
+  hub_exchange<F, NL, LS>(v, s, l, j, stw, inverse);
+  // v[k1] = X[j + 16 k1]
+  if (!live) return;
+  if (YL) {
+    const int x = lc / H, kz = lc - x * H;
+    cpx<F>* out = static_cast<cpx<F>*>(vout) + (size_t)x * PH + kz;
+    const int nout = KIND == LY_FWD ? 16 : 8;
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1)
+      if (k1 < nout) out[(j + 16 * k1) * H] = v[k1];
+  } else if (KIND == LZ_R2C) {
+    const int x = lc / n, y = lc - x * n;
+    cpx<F>* out = static_cast<cpx<F>*>(vout) + (size_t)x * PH + y * H;
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) out[j + 16 * k1] = v[k1];
+    if (j == 0) out[n] = v[8];
+  } else {
+    F* out = static_cast<F*>(vout);
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1)
+      out[((size_t)lc * n + j + 16 * k1) * es + ec] = v[k1].re * scale;
   }
 }
 
@@ -645,6 +737,52 @@ int fft_c2r(int ndim, const int* dims, cpx<F>* in, F* out,
   return SSIM_OK;
 }
 
+// The 256^3 padded convolution (n = 128) from register-direct kernels:
+// R2C rows, forward y lines, the fused x axis, and per component inverse y
+// lines + C2R rows; spec and tmp as in conv_isolated3.
+template <typename F>
+int conv256(const F* rho, const F* const* kim, F* const* fields, cpx<F>* spec,
+            cpx<F>* tmp, const unsigned char* twx, const unsigned char* twy,
+            const unsigned char* twz, bool inter, F scale, hipStream_t st) {
+  constexpr int n = 128, H = 129, NL = sizeof(F) == 4 ? 16 : 8;
+  const cpx<F>* tx = reinterpret_cast<const cpx<F>*>(twx);
+  const cpx<F>* ty = reinterpret_cast<const cpx<F>*>(twy);
+  const cpx<F>* tz = reinterpret_cast<const cpx<F>*>(twz);
+  const int rows = n * n, ylines = n * H;
+  const dim3 blk(NL * 16);
+  SSIM_TIMED("k_fft_pass", st,
+             hipLaunchKernelGGL((k_line256<F, NL, LZ_R2C>), dim3(ssim::ceil_div(rows, NL)),
+                                blk, 0, st, rho, spec, tz, rows, F(1), 1, 0));
+  SSIM_LAUNCHED("k_line256");
+  SSIM_TIMED("k_fft_pass", st,
+             hipLaunchKernelGGL((k_line256<F, NL, LY_FWD>), dim3(ssim::ceil_div(ylines, NL)),
+                                blk, 0, st, spec, spec, ty, ylines, F(1), 1, 0));
+  SSIM_LAUNCHED("k_line256");
+  // fused x axis: component 0 in place in spec (x < n), components 1 and 2
+  // in the two halves of tmp, each an [n][P][H] block
+  const long long lines = 256LL * H;
+  static_assert((256LL * 129) % NL == 0, "whole tiles of x lines");
+  cpx<F>* blk3[3] = {spec, tmp, tmp + (long long)n * lines};
+  SSIM_TIMED("k_conv_x3", st,
+             hipLaunchKernelGGL((k_conv_x3<F, NL>), dim3((int)(lines / NL)),
+                                dim3(NL * 16), 0, st, spec, kim[0], kim[1], kim[2],
+                                blk3[0], blk3[1], blk3[2], tx, (int)lines));
+  SSIM_LAUNCHED("k_conv_x3");
+  for (int c = 0; c < 3; ++c) {
+    SSIM_TIMED("k_fft_pass", st,
+               hipLaunchKernelGGL((k_line256<F, NL, LY_INV>),
+                                  dim3(ssim::ceil_div(ylines, NL)), blk, 0, st, blk3[c],
+                                  blk3[c], ty, ylines, F(1), 1, 0));
+    SSIM_LAUNCHED("k_line256");
+    SSIM_TIMED("k_fft_pass", st,
+               hipLaunchKernelGGL((k_line256<F, NL, LZ_C2R>), dim3(ssim::ceil_div(rows, NL)),
+                                  blk, 0, st, blk3[c], inter ? fields[0] : fields[c], tz,
+                                  rows, scale, inter ? 4 : 1, inter ? c : 0));
+    SSIM_LAUNCHED("k_line256");
+  }
+  return SSIM_OK;
+}
+
 // Isolated convolution of an n^3 mesh with three imaginary kernel spectra
 // on the zero-padded P = 2n mesh, keeping only the n^3 corner:
 //   forward: R2C rows (x, y < n) read straight from rho (n points, zeros
@@ -672,6 +810,9 @@ int conv_isolated3(int n, const F* rho, const F* const* kim, F* const* fields,
   const unsigned char* twx = tw + tw_offset(3, dims, 0, prec);
   const unsigned char* twy = tw + tw_offset(3, dims, 1, prec);
   const unsigned char* twz = tw + tw_offset(3, dims, 2, prec);
+  if (P == 256)
+    return conv256<F>(rho, kim, fields, spec, tmp, twx, twy, twz, inter,
+                      F(1.0 / ((double)P * P * P)), st);
   const long long rows = (long long)n * n;
   // row passes use half tiles: twice the workgroups, so loads, butterflies
   // and stores of different workgroups overlap (a full-tile row pass of the
@@ -687,38 +828,6 @@ int conv_isolated3(int n, const F* rho, const F* const* kim, F* const* fields,
   fy.in_count = n;
   SSIM_TRY((launch_pass<F, MODE_C2C>(spec, spec, fy, twy, 0, F(1), st)));
   const F scale = F(1.0 / ((double)P * P * P));
-  if (P == 256) {
-    // fused x axis: component 0 in place in spec (x < n), components 1 and
-    // 2 in the two halves of tmp, each an [n][P][H] block
-    constexpr int NL = sizeof(F) == 4 ? 16 : 8;
-    const long long lines = (long long)P * H;
-    static_assert((256LL * 129) % NL == 0, "whole tiles of x lines");
-    cpx<F>* blk[3] = {spec, tmp, tmp + (long long)n * lines};
-    SSIM_TIMED("k_conv_x3", st,
-               hipLaunchKernelGGL((k_conv_x3<F, NL>), dim3((int)(lines / NL)),
-                                  dim3(NL * 16), 0, st, spec, kim[0], kim[1], kim[2],
-                                  blk[0], blk[1], blk[2],
-                                  reinterpret_cast<const cpx<F>*>(twx), (int)lines));
-    SSIM_LAUNCHED("k_conv_x3");
-    for (int c = 0; c < 3; ++c) {
-      Pass iy = strided_pass(3, cd, 1, tile_elems<F>());
-      iy.n_outer = n;
-      iy.out_count = n;
-      SSIM_TRY((launch_pass<F, MODE_C2C>(blk[c], blk[c], iy, twy, 1, F(1), st)));
-      Pass iz = row_pass(rows, P, H, n, tile_elems<F>() / 2);
-      iz.R1 = n;
-      iz.s_in_hi = (long long)P * H;
-      iz.s_out_hi = rows;
-      iz.out_count = n;
-      if (inter) {
-        iz.out_es = 4;
-        iz.out_c = c;
-      }
-      SSIM_TRY((launch_pass<F, MODE_C2R>(blk[c], inter ? fields[0] : fields[c], iz, twz,
-                                         1, scale, st)));
-    }
-    return SSIM_OK;
-  }
   Pass fx = strided_pass(3, cd, 0, tile_elems<F>());
   fx.in_count = n;
   SSIM_TRY((launch_pass<F, MODE_C2C>(spec, spec, fx, twx, 0, F(1), st)));

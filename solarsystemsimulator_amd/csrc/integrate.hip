@@ -300,4 +300,24 @@ int ssim_drift(int dim, int n, double* x, double* y, double* z,
   return ssim::drift(dim, n, x, y, z, vx, vy, vz, h, ssim::as_stream(stream));
 }
 
+int ssim_kick_drift(int dim, int n, double* x, double* y, double* z, double* vx,
+                    double* vy, double* vz, const double* ax, const double* ay,
+                    const double* az, double half, double full, void* stream) {
+  SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3, got %d", dim);
+  SSIM_REQUIRE(n >= 0, "negative size");
+  if (n == 0) return SSIM_OK;
+  SSIM_REQUIRE(x && y && vx && vy && ax && ay && (dim == 2 || (z && vz && az)),
+               "null pointer");
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid(ssim::ceil_div(n, EW_BLOCK));
+  if (dim == 2)
+    hipLaunchKernelGGL(k_kick_drift<2>, grid, dim3(EW_BLOCK), 0, st, n, x, y, z, vx, vy,
+                       vz, ax, ay, az, half, full);
+  else
+    hipLaunchKernelGGL(k_kick_drift<3>, grid, dim3(EW_BLOCK), 0, st, n, x, y, z, vx, vy,
+                       vz, ax, ay, az, half, full);
+  SSIM_LAUNCHED("k_kick_drift");
+  return SSIM_OK;
+}
+
 }  // extern "C"

@@ -207,6 +207,87 @@ __global__ __launch_bounds__(PM_BLOCK) void k_box_kdk_partial(
     }
 }
 
+// The closing half kick of a step (v += a half, stored) fused with the box
+// of the positions the next step's kick + drift will produce (as
+// k_box_kdk_partial), in one launch: every workgroup publishes its extrema,
+// and the last one to finish (a counter in work, reset by that workgroup)
+// folds them into out.
+__global__ __launch_bounds__(PM_BLOCK) void k_kick_box(
+    int n, int na, const double* __restrict__ x0, const double* __restrict__ x1,
+    const double* __restrict__ x2, double* __restrict__ v0, double* __restrict__ v1,
+    double* __restrict__ v2, const double* __restrict__ a0,
+    const double* __restrict__ a1, const double* __restrict__ a2, double half,
+    double full, double* part, unsigned int* counter, double* __restrict__ out) {
+  __shared__ double smin[3][PM_BLOCK], smax[3][PM_BLOCK];
+  __shared__ int last;
+  const double* xs[3] = {x0, x1, x2};
+  double* vs[3] = {v0, v1, v2};
+  const double* as[3] = {a0, a1, a2};
+  for (int c = 0; c < na; ++c) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int i = blockIdx.x * PM_BLOCK + threadIdx.x; i < n;
+         i += gridDim.x * PM_BLOCK) {
+      const double a = as[c][i];
+      const double v = vs[c][i] + a * half;   // this step's closing kick
+      vs[c][i] = v;
+      const double nv = v + a * half;          // the next step's opening kick
+      const double p = xs[c][i] + nv * full;   // and its drift
+      lo = fmin(lo, p);
+      hi = fmax(hi, p);
+    }
+    smin[c][threadIdx.x] = lo;
+    smax[c][threadIdx.x] = hi;
+  }
+  __syncthreads();
+  for (int w = PM_BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int c = 0; c < na; ++c) {
+        smin[c][threadIdx.x] = fmin(smin[c][threadIdx.x], smin[c][threadIdx.x + w]);
+        smax[c][threadIdx.x] = fmax(smax[c][threadIdx.x], smax[c][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < na; ++c) {
+      part[(size_t)blockIdx.x * 6 + 2 * c] = smin[c][0];
+      part[(size_t)blockIdx.x * 6 + 2 * c + 1] = smax[c][0];
+    }
+    const unsigned int done = __hip_atomic_fetch_add(
+        counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = done == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();  // every workgroup's extrema are visible to this one
+  for (int c = 0; c < na; ++c) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += PM_BLOCK) {
+      lo = fmin(lo, __hip_atomic_load(&part[(size_t)b * 6 + 2 * c], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT));
+      hi = fmax(hi, __hip_atomic_load(&part[(size_t)b * 6 + 2 * c + 1],
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    smin[c][threadIdx.x] = lo;
+    smax[c][threadIdx.x] = hi;
+  }
+  __syncthreads();
+  for (int w = PM_BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int c = 0; c < na; ++c) {
+        smin[c][threadIdx.x] = fmin(smin[c][threadIdx.x], smin[c][threadIdx.x + w]);
+        smax[c][threadIdx.x] = fmax(smax[c][threadIdx.x], smax[c][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < na; ++c) {
+      out[2 * c] = smin[c][0];
+      out[2 * c + 1] = smax[c][0];
+    }
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 // Triangular-shaped-cloud (TSC) assignment on an isolated 3-D mesh, for the
 // split force: a body at f = (x - o)/h (node units) spreads over the 3 nodes
 // c-1, c, c+1 of each axis around its nearest node c = floor(f + 1/2), with
@@ -866,6 +947,30 @@ int ssim_box_after_kdk(int dim, int n, const double* x, const double* y,
   hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(PM_BLOCK), 0, st, nblk, dim, part,
                      out);
   SSIM_LAUNCHED("k_minmax_final");
+  return SSIM_OK;
+}
+
+size_t ssim_kick_box_work_bytes(int n) {
+  return 256 + ssim_minmax_work_bytes(n);
+}
+
+int ssim_kick_box(int dim, int n, const double* x, const double* y, const double* z,
+                  double* vx, double* vy, double* vz, const double* ax,
+                  const double* ay, const double* az, double half, double full,
+                  double* out, void* work, size_t work_bytes, void* stream) {
+  SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+  SSIM_REQUIRE(n >= 1, "box of an empty set");
+  SSIM_REQUIRE(x && y && vx && vy && ax && ay && out && work &&
+                   (dim == 2 || (z && vz && az)),
+               "null pointer");
+  SSIM_REQUIRE(work_bytes >= ssim_kick_box_work_bytes(n), "workspace too small");
+  hipStream_t st = ssim::as_stream(stream);
+  const int nblk = ssim::ceil_div(n, PM_BLOCK * 8);
+  unsigned int* counter = static_cast<unsigned int*>(work);
+  double* part = reinterpret_cast<double*>(static_cast<unsigned char*>(work) + 256);
+  hipLaunchKernelGGL(k_kick_box, dim3(nblk), dim3(PM_BLOCK), 0, st, n, dim, x, y, z, vx,
+                     vy, vz, ax, ay, az, half, full, part, counter, out);
+  SSIM_LAUNCHED("k_kick_box");
   return SSIM_OK;
 }
 

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
     const int* __restrict__ cmap, int ncells, float eps2,
     const float2* __restrict__ gtab, float tab_scale, float rcut2, double G,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
-    unsigned long long* __restrict__ counts) {
+    int accumulate, unsigned long long* __restrict__ counts) {
   __shared__ float2 tab[GT + 1];
   for (int i = threadIdx.x; i <= GT; i += PB) tab[i] = gtab[i];
   __syncthreads();  // before any wave leaves
@@ -170,9 +170,15 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
     return;
   }
   if (live) {
-    ax[tgt] += G * (double)sx;
-    ay[tgt] += G * (double)sy;
-    az[tgt] += G * (double)sz;
+    if (accumulate) {
+      ax[tgt] += G * (double)sx;
+      ay[tgt] += G * (double)sy;
+      az[tgt] += G * (double)sz;
+    } else {
+      ax[tgt] = G * (double)sx;
+      ay[tgt] = G * (double)sy;
+      az[tgt] = G * (double)sz;
+    }
   }
 }
 
@@ -200,8 +206,8 @@ int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
             const double* origin, double cell, const int* nc,
             const int* cell_start, const void* packed, const int* tcell_start,
             const int* torder, double G, double eps2, double rs, double rcut,
-            double* ax, double* ay, double* az, unsigned long long* counts,
-            void* work, size_t work_bytes, void* stream) {
+            double* ax, double* ay, double* az, int accumulate,
+            unsigned long long* counts, void* work, size_t work_bytes, void* stream) {
   SSIM_REQUIRE(nt >= 0, "nt must be >= 0");
   if (nt == 0) return SSIM_OK;
   SSIM_REQUIRE(tx && ty && tz && origin && nc && cell_start && packed &&
@@ -244,7 +250,7 @@ int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
                        cell_start, static_cast<const float4*>(packed), tcell_start,
                        torder, cstart, cmap, ncells, (float)eps2, gtab,
                        (float)(GT / (rcut * rcut)), (float)(rcut * rcut), G, ax, ay, az,
-                       counts);
+                       accumulate, counts);
   else
     SSIM_TIMED("k_pp_cells", st,
                hipLaunchKernelGGL(k_pp_cells<false>, grid, dim3(PB), 0, st, tx, ty,
@@ -253,7 +259,7 @@ int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
                                   static_cast<const float4*>(packed), tcell_start,
                                   torder, cstart, cmap, ncells, (float)eps2, gtab,
                                   (float)(GT / (rcut * rcut)), (float)(rcut * rcut), G, ax,
-                                  ay, az, nullptr));
+                                  ay, az, accumulate, nullptr));
   SSIM_LAUNCHED("k_pp_cells");
   return SSIM_OK;
 }
@@ -267,10 +273,11 @@ int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,
                   const int* cell_start, const void* packed,
                   const int* tcell_start, const int* torder, double G,
                   double eps2, double rs, double rcut, double* ax, double* ay,
-                  double* az, void* work, size_t work_bytes, void* stream) {
+                  double* az, int accumulate, void* work, size_t work_bytes,
+                  void* stream) {
   return pp_impl(nt, tx, ty, tz, origin, cell, nc, cell_start, packed,
-                 tcell_start, torder, G, eps2, rs, rcut, ax, ay, az, nullptr,
-                 work, work_bytes, stream);
+                 tcell_start, torder, G, eps2, rs, rcut, ax, ay, az, accumulate,
+                 nullptr, work, work_bytes, stream);
 }
 
 int ssim_pp_count(int nt, const double* tx, const double* ty, const double* tz,
@@ -285,7 +292,7 @@ int ssim_pp_count(int nt, const double* tx, const double* ty, const double* tz,
   SSIM_LAUNCHED("k_zero_u64");
   return pp_impl(nt, tx, ty, tz, origin, cell, nc, cell_start, packed,
                  tcell_start, torder, 1.0, eps2, rs, rcut, nullptr, nullptr,
-                 nullptr, counts, work, work_bytes, stream);
+                 nullptr, 1, counts, work, work_bytes, stream);
 }
 
 }  // extern "C"

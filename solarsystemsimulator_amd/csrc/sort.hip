@@ -3,9 +3,9 @@
 //
 // The sort takes 8 key bits per pass over tiles of RTILE = 4096 elements
 // (16 rounds of one element per thread of a 256-thread workgroup):
-//   k_radix_hist0   — per-tile digit counts of the first pass (tile-major,
-//                     hist[b * 256 + d]); it also clears the count arrays
-//                     of the later passes;
+//   k_radix_hist0   — per-tile digit counts of the first pass (digit-major,
+//                     hist[d * hrow + b], rows padded to a multiple of 4);
+//                     it also clears the count arrays of the later passes;
 //   k_radix_scatter — the tile's first slot per digit (its own exclusive
 //                     prefix over the tile counts, or k_radix_offs's when
 //                     there are many tiles), then per round the rank of an
@@ -31,7 +31,7 @@ constexpr int FUSE_TILES = 256;    // up to this many tiles a scatter block
 constexpr int SCAN_T = 1024;       // threads of the single-workgroup scan
 
 __global__ __launch_bounds__(RT) void k_radix_hist0(int n, const int* __restrict__ keys,
-                                                    int tiles, int* __restrict__ hist,
+                                                    int hrow, int* __restrict__ hist,
                                                     int* __restrict__ zero, int nzero) {
   __shared__ int cnt[256];
   const int t = threadIdx.x;
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(RT) void k_radix_hist0(int n, const int* __restrict
     if (i < n) atomicAdd(&cnt[keys[i] & 255], 1);
   }
   __syncthreads();
-  hist[(size_t)blockIdx.x * 256 + t] = cnt[t];
+  hist[(size_t)t * hrow + blockIdx.x] = cnt[t];
 }
 
 // Exclusive prefix of v over the 256 threads of the workgroup (s: 256 ints
@@ -65,24 +65,44 @@ __device__ int excl_scan256(int v, int* s) {
   return incl - v;
 }
 
-// One workgroup: offs[b * 256 + d] = first slot of digit d from tile b.
+// Counts are digit-major, hist[d * hrow + b], rows padded to hrow (a
+// multiple of 4, zero beyond the tiles) so a thread sums its digit's row
+// with 16-byte loads.  Returns the row total; *pre = the tiles before b.
+__device__ __forceinline__ int row_sums(const int* __restrict__ row, int hrow, int b,
+                                        int* pre) {
+  const int4* r4 = reinterpret_cast<const int4*>(row);
+  int tot = 0, p = 0;
+#pragma unroll 4
+  for (int q = 0; q < hrow / 4; ++q) {
+    const int4 c = r4[q];
+    const int j = 4 * q;
+    tot += c.x + c.y + c.z + c.w;
+    p += (j < b ? c.x : 0) + (j + 1 < b ? c.y : 0) + (j + 2 < b ? c.z : 0) +
+         (j + 3 < b ? c.w : 0);
+  }
+  *pre = p;
+  return tot;
+}
+
+// One workgroup: offs[d * hrow + b] = first slot of digit d from tile b.
 __global__ __launch_bounds__(RT) void k_radix_offs(const int* __restrict__ hist,
-                                                   int tiles, int* __restrict__ offs) {
+                                                   int tiles, int hrow,
+                                                   int* __restrict__ offs) {
   __shared__ int s[256];
   const int d = threadIdx.x;
-  int tot = 0;
-  for (int j = 0; j < tiles; ++j) tot += hist[(size_t)j * 256 + d];
+  int pre;
+  const int tot = row_sums(hist + (size_t)d * hrow, hrow, 0, &pre);
   int run = excl_scan256(tot, s);
   for (int j = 0; j < tiles; ++j) {
-    const int c = hist[(size_t)j * 256 + d];
-    offs[(size_t)j * 256 + d] = run;
+    const int c = hist[(size_t)d * hrow + j];
+    offs[(size_t)d * hrow + j] = run;
     run += c;
   }
 }
 
 __global__ __launch_bounds__(RT) void k_radix_scatter(
     int n, const int* __restrict__ keys, const int* __restrict__ vals, int shift,
-    int tiles, const int* __restrict__ hist, const int* __restrict__ offs,
+    int hrow, const int* __restrict__ hist, const int* __restrict__ offs,
     int* __restrict__ keys_out, int* __restrict__ vals_out, int* __restrict__ hist_next,
     int shift_next) {
   constexpr int W = RT / 64;
@@ -100,14 +120,10 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
     v[r] = i < n ? vals[i] : 0;
   }
   if (offs) {
-    run[t] = offs[(size_t)blockIdx.x * 256 + t];
+    run[t] = offs[(size_t)t * hrow + blockIdx.x];
   } else {
-    int tot = 0, pre = 0;
-    for (int j = 0; j < tiles; ++j) {
-      const int c = hist[(size_t)j * 256 + t];
-      pre += j < (int)blockIdx.x ? c : 0;
-      tot += c;
-    }
+    int pre;
+    const int tot = row_sums(hist + (size_t)t * hrow, hrow, blockIdx.x, &pre);
     run[t] = excl_scan256(tot, s) + pre;
   }
   for (int q = t; q < W * 257; q += RT) (&wc[0][0])[q] = 0;
@@ -140,7 +156,7 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
       keys_out[slot] = k[r];
       vals_out[slot] = v[r];
       if (hist_next)
-        atomicAdd(&hist_next[(size_t)(slot / RTILE) * 256 + ((k[r] >> shift_next) & 255)],
+        atomicAdd(&hist_next[(size_t)((k[r] >> shift_next) & 255) * hrow + slot / RTILE],
                   1);
     }
   }
@@ -269,8 +285,9 @@ int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st) {
 size_t radix_sort_work_bytes(int n) {
   const size_t nn = (size_t)(n > 0 ? n : 1);
   const size_t tiles = (nn + RTILE - 1) / RTILE;
-  return 4 * align256(nn * sizeof(int)) + align256(4 * 256 * tiles * sizeof(int)) +
-         align256(256 * tiles * sizeof(int));
+  const size_t hrow = (tiles + 3) & ~size_t(3);
+  return 4 * align256(nn * sizeof(int)) + align256(4 * 256 * hrow * sizeof(int)) +
+         align256(256 * hrow * sizeof(int));
 }
 
 int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
@@ -280,8 +297,9 @@ int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
   SSIM_REQUIRE(work && work_bytes >= radix_sort_work_bytes(n),
                "radix sort: workspace too small");
   const int tiles = ceil_div(n, RTILE);
+  const int hrow = (tiles + 3) & ~3;  // padded row of one digit's tile counts
   const int passes = (bits + 7) / 8;
-  const size_t ht = 256 * (size_t)tiles;
+  const size_t ht = 256 * (size_t)hrow;
   unsigned char* w = static_cast<unsigned char*>(work);
   int* pk[2];
   int* pv[2];
@@ -294,7 +312,8 @@ int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
   int* hist = reinterpret_cast<int*>(w);
   w += align256(4 * ht * sizeof(int));
   int* offs = reinterpret_cast<int*>(w);
-  hipLaunchKernelGGL(k_radix_hist0, dim3(tiles), dim3(RT), 0, st, n, keys, tiles, hist,
+  // pass 0's rows: the padding stays zero (tiles >= hrow - 3 write the rest)
+  hipLaunchKernelGGL(k_radix_hist0, dim3(hrow), dim3(RT), 0, st, n, keys, hrow, hist,
                      hist + ht, (int)((passes - 1) * ht));
   SSIM_LAUNCHED("k_radix_hist0");
   const int* sk = keys;
@@ -306,12 +325,12 @@ int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
     const int* h = hist + p * ht;
     const int* o = nullptr;
     if (tiles > FUSE_TILES) {
-      hipLaunchKernelGGL(k_radix_offs, dim3(1), dim3(RT), 0, st, h, tiles, offs);
+      hipLaunchKernelGGL(k_radix_offs, dim3(1), dim3(RT), 0, st, h, tiles, hrow, offs);
       SSIM_LAUNCHED("k_radix_offs");
       o = offs;
     }
     hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(RT), 0, st, n, sk, sv, 8 * p,
-                       tiles, h, o, dk, dv,
+                       hrow, h, o, dk, dv,
                        p + 1 < passes ? hist + (p + 1) * ht : nullptr, 8 * (p + 1));
     SSIM_LAUNCHED("k_radix_scatter");
     sk = dk;

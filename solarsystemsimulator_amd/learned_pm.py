@@ -150,7 +150,11 @@ class PMLearnedSystem:
             nb = self.n + self.t
             self._box_dev = torch.empty(6, dtype=torch.float64, device=self.dev)
             self._box_host = torch.empty(6, dtype=torch.float64, pin_memory=True)
-            self._box_ws = D.workspace(_lib.load().ssim_minmax_work_bytes(nb), self.dev)
+            lib = _lib.load()
+            self._box_ws = D.workspace(lib.ssim_minmax_work_bytes(nb), self.dev)
+            # ssim_kick_box's completion counter starts at zero
+            self._kb_ws = torch.zeros(lib.ssim_kick_box_work_bytes(nb), dtype=torch.uint8,
+                                      device=self.dev)
             self._box_ev = torch.cuda.Event()
 
     def _next_box(self, half, full):
@@ -173,10 +177,14 @@ class PMLearnedSystem:
         st = D.stream(self.dev)
         ps = lambda lst: [_lib.ptr(c) for c in lst]  # noqa: E731
         half = 0.5 * dt
-        _lib.call("ssim_kick_snap", 3, self.n, self.t, *ps(self.x), *ps(self.v),
-                  *ps(self.a), half, None, st)
-        _lib.call("ssim_drift", 3, nb, *ps(self.x), *ps(self.v), dt, st)
+        _lib.call("ssim_kick_drift", 3, nb, *ps(self.x), *ps(self.v), *ps(self.a),
+                  half, dt, st)
         self.force(learned, placed=True)
+        if snap_ptr is None:          # closing kick + next box in one launch
+            _lib.call("ssim_kick_box", 3, nb, *ps(self.x), *ps(self.v), *ps(self.a),
+                      half, dt, _lib.ptr(self._box_dev), _lib.ptr(self._kb_ws),
+                      self._kb_ws.numel(), st)
+            return
         _lib.call("ssim_kick_snap", 3, self.n, self.t, *ps(self.x), *ps(self.v),
                   *ps(self.a), half, snap_ptr, st)
         self._next_box(half, dt)

@@ -339,14 +339,15 @@ class SplitForce:
         self._grid = (gn, org, cell, rs, rcut)
         return self._grid
 
-    def near_sum(self, x, a):
-        """a += near-field term over the lists of the last bin_bodies."""
+    def near_sum(self, x, a, accumulate=True):
+        """a += (or, accumulate=False, a =) the near-field term over the
+        lists of the last bin_bodies."""
         gn, org, cell, rs, rcut = self._grid
         _lib.call("ssim_pp_short", x[0].numel(), *[_lib.ptr(c) for c in x],
                   P._addr(org), cell, P._addr(gn), _lib.ptr(self.cell_start),
                   _lib.ptr(self.packed), _lib.ptr(self.tcell_start),
                   _lib.ptr(self.torder), self.G, self.eps2, rs, rcut,
-                  *[_lib.ptr(c) for c in a], _lib.ptr(self.cwork),
+                  *[_lib.ptr(c) for c in a], int(accumulate), _lib.ptr(self.cwork),
                   self.cwork.numel(), D.stream(self.dev))
         return a
 
@@ -381,13 +382,14 @@ class SplitForce:
         on the current stream and returns mesh_gather's `corr` dict; the
         gather then adds it.
 
-        With TSC the work runs on two streams: the side stream zeroes a and
-        bins the targets while the current one bins the masses (one binning
+        With TSC the work runs on two streams: the side stream bins the
+        targets while the current one bins the masses (one binning
         for the deposit and the near-field sources); then the side stream
         adds the near field (bound by the vector ALU) and, once the mass
         mesh is deposited, runs the correction, while the current one
-        deposits and convolves (bound by memory and latency); the current
-        stream waits for it and adds the mesh gather (in cell order).  With
+        deposits and convolves (bound by memory and latency); the near field
+        is stored into a, and the current stream waits for the side stream
+        and adds the mesh gather (in cell order).  With
         CIC everything runs in order on one stream."""
         import torch
         if not placed:
@@ -395,11 +397,9 @@ class SplitForce:
         if correction is not None and not self.can_fold_correction():
             raise ValueError("correction needs TSC with interleaved FP32 fields")
         if self.assign == "tsc" and not self.overlap:
-            for c in a:
-                c.zero_()
             self.bin_targets(x)
             self.mass_bins(x, n_mass, m)
-            self.near_sum(x, a)
+            self.near_sum(x, a, accumulate=False)
             self.deposit(x, n_mass, m, binned=True)
             corr = correction() if correction is not None else None
             self.convolve(interleaved=self.f4 is not None)
@@ -417,15 +417,13 @@ class SplitForce:
         start.record(main)
         self._side.wait_event(start)
         with torch.cuda.stream(self._side):
-            for c in a:
-                c.zero_()
             self.bin_targets(x)
         self.mass_bins(x, n_mass, m)
         binned = torch.cuda.Event()
         binned.record(main)
         self._side.wait_event(binned)
         with torch.cuda.stream(self._side):
-            self.near_sum(x, a)
+            self.near_sum(x, a, accumulate=False)
         self.deposit(x, n_mass, m, binned=True)
         corr = None
         if correction is not None:

@@ -41,6 +41,45 @@ def test_box_after_kdk_is_exact(dev):
         assert np.array_equal(x[c] + (v[c] + a[c] * half) * full, p)
 
 
+def test_fused_kicks_match_separate_launches(dev):
+    """ssim_kick_drift = ssim_kick + ssim_drift and ssim_kick_box = ssim_kick
+    + ssim_box_after_kdk, bit for bit, on two calls in a row (the fused box
+    resets its completion counter)."""
+    print("fused kicks", flush=True)
+    rng = np.random.default_rng(4)
+    n = 70_001
+    st = D.stream(dev)
+    lib = _lib.load()
+    half, full = 1.25e-3, 2.5e-3
+    for call in range(2):
+        x = [rng.normal(0.0, 1.0, n) for _ in range(3)]
+        v = [rng.normal(0.0, 10.0, n) for _ in range(3)]
+        a = [rng.normal(0.0, 100.0, n) for _ in range(3)]
+        xa, va, aa = ([D.to_dev(c, dev) for c in arr] for arr in (x, v, a))
+        xb, vb = [c.clone() for c in xa], [c.clone() for c in va]
+        _lib.call("ssim_kick_drift", 3, n, *_ptrs(xa), *_ptrs(va), *_ptrs(aa), half, full,
+                  st)
+        _lib.call("ssim_kick", 3, n, *_ptrs(vb), *_ptrs(aa), half, st)
+        _lib.call("ssim_drift", 3, n, *_ptrs(xb), *_ptrs(vb), full, st)
+        for c in range(3):
+            assert torch.equal(xa[c], xb[c]) and torch.equal(va[c], vb[c])
+        if call == 0:
+            kw = torch.zeros(lib.ssim_kick_box_work_bytes(n), dtype=torch.uint8,
+                             device=dev)
+        out_a = torch.empty(6, dtype=torch.float64, device=dev)
+        out_b = torch.empty(6, dtype=torch.float64, device=dev)
+        _lib.call("ssim_kick_box", 3, n, *_ptrs(xa), *_ptrs(va), *_ptrs(aa), half, full,
+                  _lib.ptr(out_a), _lib.ptr(kw), kw.numel(), st)
+        _lib.call("ssim_kick", 3, n, *_ptrs(vb), *_ptrs(aa), half, st)
+        w = D.workspace(lib.ssim_minmax_work_bytes(n), dev)
+        _lib.call("ssim_box_after_kdk", 3, n, *_ptrs(xb), *_ptrs(vb), *_ptrs(aa), half,
+                  full, _lib.ptr(out_b), _lib.ptr(w), w.numel(), st)
+        for c in range(3):
+            assert torch.equal(va[c], vb[c])
+        assert torch.equal(out_a, out_b)
+        assert int(kw[:4].sum()) == 0
+
+
 def _system(dev, pos, m, tpos, vel, tvel):
     s = PMLearnedSystem(dev, len(m), len(tpos[0]), 1e-6, 1.0,
                         net=load_weights(dev), mesh_n=32)
