@@ -140,7 +140,11 @@ class PMLearnedSystem:
     # is kept the step is replayed from a captured HIP graph, which removes
     # the ~100 kernel launches per step from the host's critical path; the
     # eager and the replayed steps run the same kernels with the same
-    # arguments, so they give identical results.
+    # arguments, so they give identical results.  The graph executor runs
+    # the branches of a two-stream graph in capture order rather than side
+    # by side, so the overlapped force is captured with the side stream's
+    # binning first (split.side_first): the near field then runs beside the
+    # deposit and the convolution.
     use_graph = True
     captures = 0          # step graphs captured so far (one per placement)
 
@@ -200,8 +204,12 @@ class PMLearnedSystem:
             g = torch.cuda.CUDAGraph()
             side = torch.cuda.Stream(device=self.dev)    # default priority
             side.wait_stream(torch.cuda.current_stream(self.dev))
-            with torch.cuda.graph(g, stream=side):
-                self._step(dt, learned)
+            self.split.side_first = True
+            try:
+                with torch.cuda.graph(g, stream=side):
+                    self._step(dt, learned)
+            finally:
+                self.split.side_first = False
             torch.cuda.current_stream(self.dev).wait_stream(side)
             self._graph, self._graph_key = g, key
             self.captures += 1

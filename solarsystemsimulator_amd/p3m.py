@@ -411,36 +411,69 @@ class SplitForce:
             self.near_sum(x, a)
             return a
         main = torch.cuda.current_stream(self.dev)
-        if self._side is None:
-            self._side = torch.cuda.Stream(device=self.dev)  # default priority
         start = torch.cuda.Event()
         start.record(main)
-        self._side.wait_event(start)
-        with torch.cuda.stream(self._side):
-            self.bin_targets(x)
-        self.mass_bins(x, n_mass, m)
-        binned = torch.cuda.Event()
-        binned.record(main)
-        self._side.wait_event(binned)
-        with torch.cuda.stream(self._side):
-            self.near_sum(x, a, accumulate=False)
-        self.deposit(x, n_mass, m, binned=True)
-        corr = None
-        if correction is not None:
-            deposited = torch.cuda.Event()
-            deposited.record(main)
-            self._side.wait_event(deposited)
-            with torch.cuda.stream(self._side):
-                corr = correction()
-        done = torch.cuda.Event()
-        done.record(self._side)
-        self.convolve(interleaved=self.f4 is not None)
-        main.wait_event(done)
-        self.mesh_gather(x, a, order=self.torder, accumulate=True, corr=corr)
+        self.run_segments(self.tsc_segments(x, n_mass, m, a, correction),
+                          {"start": start})
         return a
+
+    def side_stream(self):
+        import torch
+        if self._side is None:
+            self._side = torch.cuda.Stream(device=self.dev)  # default priority
+        return self._side
+
+    def tsc_segments(self, x, n_mass, m, a, correction=None):
+        """The overlapped TSC force as ordered segments (stream, fn, waits,
+        signal): stream "main" (the current one) or "side"; before a
+        segment runs, its stream waits for the named events, and afterwards
+        records `signal`.  The caller records "start" on main before the
+        first segment.  The mass binning comes first (it feeds both the
+        deposit and the near field; side_first puts the target binning
+        before it, the order a captured graph needs); the target binning
+        and the near field run on the side stream beside the deposit and
+        the convolution, and so does the correction once the mass mesh
+        exists; the gather waits for the side stream."""
+        held = {}
+
+        def corr():
+            held["corr"] = correction()
+
+        mass = ("main", lambda: self.mass_bins(x, n_mass, m), (), "bins")
+        targets = ("side", lambda: self.bin_targets(x), ("start",), None)
+        segs = [targets, mass] if self.side_first else [mass, targets]
+        segs += [("side", lambda: self.near_sum(x, a, accumulate=False), ("bins",), None),
+                 ("main", lambda: self.deposit(x, n_mass, m, binned=True), (), "deposited")]
+        if correction is not None:
+            segs.append(("side", corr, ("deposited",), None))
+        last = segs[-1] if correction is not None else segs[2]
+        segs[segs.index(last)] = last[:3] + ("side_done",)
+        segs += [("main", lambda: self.convolve(interleaved=self.f4 is not None), (), None),
+                 ("main", lambda: self.mesh_gather(x, a, order=self.torder, accumulate=True,
+                                                   corr=held.get("corr")),
+                  ("side_done",), None)]
+        return segs
+
+    def run_segments(self, segs, events):
+        """Launch segments (see tsc_segments) on the current and the side
+        stream; `events` maps names to torch events (created as needed)."""
+        import torch
+        main = torch.cuda.current_stream(self.dev)
+        side = self.side_stream()
+        for sname, fn, waits, sig in segs:
+            st = main if sname == "main" else side
+            for w in waits:
+                st.wait_event(events[w])
+            with torch.cuda.stream(st):
+                fn()
+            if sig:
+                if sig not in events:
+                    events[sig] = torch.cuda.Event()
+                events[sig].record(st)
 
     _side = None
     overlap = True        # TSC: near field on a second stream
+    side_first = False    # enqueue the target binning before the mass binning
 
 
 def relative_errors(a, ref):
