@@ -258,14 +258,12 @@ __global__ __launch_bounds__(PM_BLOCK) void k_kick_box(
   }
   __syncthreads();
   if (!last) return;
-  __threadfence();  // every workgroup's extrema are visible to this one
+  __threadfence();  // acquire: every workgroup's extrema are visible here
   for (int c = 0; c < na; ++c) {
     double lo = INFINITY, hi = -INFINITY;
     for (int b = threadIdx.x; b < (int)gridDim.x; b += PM_BLOCK) {
-      lo = fmin(lo, __hip_atomic_load(&part[(size_t)b * 6 + 2 * c], __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT));
-      hi = fmax(hi, __hip_atomic_load(&part[(size_t)b * 6 + 2 * c + 1],
-                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      lo = fmin(lo, part[(size_t)b * 6 + 2 * c]);
+      hi = fmax(hi, part[(size_t)b * 6 + 2 * c + 1]);
     }
     smin[c][threadIdx.x] = lo;
     smax[c][threadIdx.x] = hi;
@@ -455,30 +453,34 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_deposit_blocked(
   int qs = q1 + 1;
   for (int q = q0 + 1; q <= q1; ++q)
     if (zb[q] != zb[q0] && qs > q1) qs = q;
-  double acc = 0.0;
-  for (int a = 0; a < 3; ++a) {
-    const int bi = i + a - 1;
-    if (bi < 0 || bi >= n0) continue;
+  // every row's bin range first (rows outside the mesh: empty), so the
+  // bin_start loads are in flight together; then the bodies
+  const bool split = qs <= q1;
+  const int ze = split ? Z[qs - 1] : Z[q1];
+  // second run: [Z[qs], Z[q1]] when split, else an empty range
+  const int zb0 = split ? Z[qs] : Z[q0], zb1 = split ? Z[q1] + 1 : Z[q0];
+  int s0[9], la[9], t0[9], len[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
     for (int b = 0; b < 3; ++b) {
-      const int bj = j + b - 1;
-      if (bj < 0 || bj >= n1) continue;
-      const int xy = X[a] + Y[b];
-      const int s0 = bin_start[xy + Z[q0]];
-      int s1, t0 = 0, t1 = 0;
-      if (qs > q1) {
-        s1 = bin_start[xy + Z[q1] + 1];
-      } else {
-        s1 = bin_start[xy + Z[qs - 1] + 1];
-        t0 = bin_start[xy + Z[qs]];
-        t1 = bin_start[xy + Z[q1] + 1];
-      }
-      const int la = s1 - s0, len = la + (t1 - t0);
-      for (int u = 0; u < len; ++u) {
-        const double4 p = body[u < la ? s0 + u : t0 + (u - la)];
-        acc += p.w * tsc_weight_f(p.x, i) * tsc_weight_f(p.y, j) * tsc_weight_f(p.z, k);
-      }
+      const int r = 3 * a + b;
+      const bool in = i + a - 1 >= 0 && i + a - 1 < n0 && j + b - 1 >= 0 && j + b - 1 < n1;
+      const int xy = in ? X[a] + Y[b] : 0;
+      const int a0 = bin_start[xy + Z[q0]], a1 = bin_start[xy + ze + 1];
+      const int b0 = bin_start[xy + zb0], b1 = bin_start[xy + zb1];
+      s0[r] = a0;
+      la[r] = in ? a1 - a0 : 0;
+      t0[r] = b0;
+      len[r] = in ? la[r] + (b1 - b0) : 0;
     }
-  }
+  double acc = 0.0;
+#pragma unroll
+  for (int r = 0; r < 9; ++r)
+    for (int u = 0; u < len[r]; ++u) {
+      const double4 p = body[u < la[r] ? s0[r] + u : t0[r] + (u - la[r])];
+      acc += p.w * tsc_weight_f(p.x, i) * tsc_weight_f(p.y, j) * tsc_weight_f(p.z, k);
+    }
   mesh[node] = (M)acc;
 }
 
@@ -555,29 +557,41 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
   tsc_axis(x[p], g.o[0], g.h, c[0], w[0]);
   tsc_axis(y[p], g.o[1], g.h, c[1], w[1]);
   tsc_axis(z[p], g.o[2], g.h, c[2], w[2]);
-  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  // branch-free stencil: nodes outside the mesh are clamped into it with
+  // weight 0 (an exact zero term), so all 27 loads can be in flight at once
+  int ix[3][3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const int i = c[0] + a - 1;
-    if (i < 0 || i >= g.n[0]) continue;
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int i = c[d] + a - 1;
+      const bool in = i >= 0 && i < g.n[d];
+      ix[d][a] = in ? i : 0;
+      w[d][a] = in ? w[d][a] : 0.0;
+    }
+  float4 q[3][3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      const int j = c[1] + b - 1;
-      if (j < 0 || j >= g.n[1]) continue;
+      const long long row = ((long long)ix[0][a] * g.n[1] + ix[1][b]) * g.n[2];
+#pragma unroll
+      for (int e = 0; e < 3; ++e) q[a][b][e] = f[row + ix[2][e]];
+    }
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
       const double wab = w[0][a] * w[1][b];
-      const long long row = ((long long)i * g.n[1] + j) * g.n[2];
 #pragma unroll
       for (int e = 0; e < 3; ++e) {
-        const int k = c[2] + e - 1;
-        if (k < 0 || k >= g.n[2]) continue;
         const double wt = wab * w[2][e];
-        const float4 q = f[row + k];
-        v0 += wt * (double)q.x;
-        v1 += wt * (double)q.y;
-        v2 += wt * (double)q.z;
+        v0 += wt * (double)q[a][b][e].x;
+        v1 += wt * (double)q[a][b][e].y;
+        v2 += wt * (double)q[a][b][e].z;
       }
     }
-  }
   double s0 = accumulate ? o0[p] + v0 : v0;
   double s1 = accumulate ? o1[p] + v1 : v1;
   double s2 = accumulate ? o2[p] + v2 : v2;
@@ -950,8 +964,13 @@ int ssim_box_after_kdk(int dim, int n, const double* x, const double* y,
   return SSIM_OK;
 }
 
+// ssim_kick_box: eight bodies per thread (the minmax partial pass's grid):
+// more workgroups lengthen the last workgroup's fold more than they gain
+constexpr int KB_ITEMS = 8;
+
 size_t ssim_kick_box_work_bytes(int n) {
-  return 256 + ssim_minmax_work_bytes(n);
+  const int nblk = ssim::ceil_div(n < 1 ? 1 : n, PM_BLOCK * KB_ITEMS);
+  return 256 + ssim::align256((size_t)nblk * 6 * sizeof(double));
 }
 
 int ssim_kick_box(int dim, int n, const double* x, const double* y, const double* z,
@@ -965,7 +984,7 @@ int ssim_kick_box(int dim, int n, const double* x, const double* y, const double
                "null pointer");
   SSIM_REQUIRE(work_bytes >= ssim_kick_box_work_bytes(n), "workspace too small");
   hipStream_t st = ssim::as_stream(stream);
-  const int nblk = ssim::ceil_div(n, PM_BLOCK * 8);
+  const int nblk = ssim::ceil_div(n, PM_BLOCK * KB_ITEMS);
   unsigned int* counter = static_cast<unsigned int*>(work);
   double* part = reinterpret_cast<double*>(static_cast<unsigned char*>(work) + 256);
   hipLaunchKernelGGL(k_kick_box, dim3(nblk), dim3(PM_BLOCK), 0, st, n, dim, x, y, z, vx,

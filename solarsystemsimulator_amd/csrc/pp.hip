@@ -38,7 +38,8 @@ constexpr int CHUNK = 64;          // targets per wave
 constexpr int SG = 4;              // sources loaded and tested per group
 constexpr int GT = 4096;           // intervals of the gS(sqrt(u)) table
 
-// tab[i] = (gS(r_i), gS(r_{i+1}) - gS(r_i)), r_i = rcut sqrt(i / GT)
+// tab[i] = (gS(r_i), gS(r_{i+1}) - gS(r_i)), r_i = rcut sqrt(i / GT);
+// tab[GT] = (0, 0): gS vanishes from rcut on
 __global__ __launch_bounds__(PB) void k_gtab(double rs, double rcut,
                                              float2* __restrict__ tab) {
   const int i = blockIdx.x * PB + threadIdx.x;
@@ -64,17 +65,18 @@ __global__ __launch_bounds__(PB) void k_chunk_map(int ncells,
   for (int k = cstart[c]; k < cstart[c + 1]; ++k) map[k] = c;
 }
 
-// s += m d gS(r) / (r^2 + eps2)^(3/2) for a source inside rcut (in), else
-// exactly nothing; gS from the LDS table, linear in u = r^2
+// s += m d gS(r) / (r^2 + eps2)^(3/2); gS from the LDS table, linear in
+// u = r^2.  t = u GT / rcut^2 is clamped to GT, whose entry is (0, 0), so a
+// source at or beyond rcut adds exactly nothing without a select, and the
+// interpolation weight t - floor(t) is one v_fract_f32.
 __device__ __forceinline__ void pp_term(float m, float dx, float dy, float dz, float r2,
-                                        bool in, const float2* tab, float tab_scale,
-                                        float eps2, float& sx, float& sy, float& sz) {
-  const float t = r2 * tab_scale;
-  const int k = in ? (int)t : 0;
-  const float2 e = tab[k];
-  const float g = fmaf(t - (float)k, e.y, e.x);
+                                        const float2* tab, float tab_scale, float eps2,
+                                        float& sx, float& sy, float& sz) {
+  const float t = fminf(r2 * tab_scale, (float)GT);
+  const float2 e = tab[(int)t];
+  const float g = fmaf(__builtin_amdgcn_fractf(t), e.y, e.x);
   const float inv = __builtin_amdgcn_rsqf(r2 + eps2);
-  const float f = in ? m * (inv * inv) * inv * g : 0.f;
+  const float f = m * (inv * inv) * inv * g;
   sx = fmaf(f, dx, sx);
   sy = fmaf(f, dy, sy);
   sz = fmaf(f, dz, sz);
@@ -125,29 +127,27 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
 #pragma unroll
         for (int j = 0; j < SG; ++j) q[j] = src[s + j];
         float dx[SG], dy[SG], dz[SG], r2[SG];
-        bool in[SG], any = false;
+        unsigned long long any = 0;  // lanes with a source of the group inside
 #pragma unroll
         for (int j = 0; j < SG; ++j) {
           dx[j] = q[j].x - px;
           dy[j] = q[j].y - py;
           dz[j] = q[j].z - pz;
           r2[j] = fmaf(dx[j], dx[j], fmaf(dy[j], dy[j], dz[j] * dz[j]));
-          in[j] = r2[j] < rcut2;
-          any = any || in[j];
+          any |= __ballot(r2[j] < rcut2);
         }
         if (COUNT) {
 #pragma unroll
           for (int j = 0; j < SG; ++j) {
             n_test += 1;
-            n_in += (in[j] && r2[j] > 0.f) ? 1 : 0;
+            n_in += (r2[j] < rcut2 && r2[j] > 0.f) ? 1 : 0;
           }
           continue;
         }
-        if (__ballot(any) == 0) continue;
+        if (any == 0) continue;
 #pragma unroll
         for (int j = 0; j < SG; ++j)
-          pp_term(q[j].w, dx[j], dy[j], dz[j], r2[j], in[j], tab, tab_scale, eps2, sx,
-                  sy, sz);
+          pp_term(q[j].w, dx[j], dy[j], dz[j], r2[j], tab, tab_scale, eps2, sx, sy, sz);
       }
       for (; s < s1; ++s) {
         const float4 q = src[s];
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
           n_in += (in && r2 > 0.f) ? 1 : 0;
           continue;
         }
-        pp_term(q.w, dx, dy, dz, r2, in, tab, tab_scale, eps2, sx, sy, sz);
+        if (in) pp_term(q.w, dx, dy, dz, r2, tab, tab_scale, eps2, sx, sy, sz);
       }
     }
   if (COUNT) {
