@@ -515,12 +515,12 @@ __global__ __launch_bounds__(MB) void k_to_bf16(int n, const float* __restrict__
 // around its nearest mesh node, then its offset from that node in cells
 // (times 2, in [-1, 1]), 1 for a mass (p < n_mass, its own mass is in the
 // stencil) or 0 for a tracer, and a zero; BF16 [nb][32].
+template <typename M>
 __global__ __launch_bounds__(MB) void k_mlp_features(
     int nb, int n_mass, const int* __restrict__ order, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, int n, double ox,
-    double oy, double oz,
-    double h, const float* __restrict__ rho32, const double* __restrict__ rho64,
-    double inv_unit, unsigned short* __restrict__ feat) {
+    double oy, double oz, double h, const M* __restrict__ rho, double inv_unit,
+    unsigned short* __restrict__ feat) {
   const int i_ = blockIdx.x * MB + threadIdx.x;
   if (i_ >= nb) return;
   const int p = order ? order[i_] : i_;
@@ -532,24 +532,35 @@ __global__ __launch_bounds__(MB) void k_mlp_features(
     c[a] = min(max((int)r, 0), n - 1);
     off[a] = (float)(2.0 * (f[a] - r));
   }
+  // branch-free stencil: nodes outside the mesh are read clamped and
+  // count as 0, so all 27 loads can be in flight at once
+  int ix[3][3];
+  bool ok[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int i = c[a] + d - 1;
+      ok[a][d] = i >= 0 && i < n;
+      ix[a][d] = ok[a][d] ? i : 0;
+    }
+  M m[27];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        m[9 * a + 3 * b + d] = rho[((size_t)ix[0][a] * n + ix[1][b]) * n + ix[2][d]];
   // 32 BF16 values packed into 16 words, written as four 16-byte stores
   unsigned int wd[16];
-  int q = 0;
 #pragma unroll
-  for (int da = -1; da <= 1; ++da)
-#pragma unroll
-    for (int db = -1; db <= 1; ++db)
-#pragma unroll
-      for (int dc = -1; dc <= 1; ++dc, ++q) {
-        const int i = c[0] + da, j = c[1] + db, k = c[2] + dc;
-        double v = 0.0;
-        if (i >= 0 && i < n && j >= 0 && j < n && k >= 0 && k < n) {
-          const size_t idx = ((size_t)i * n + j) * n + k;
-          v = rho64 ? rho64[idx] : (double)rho32[idx];
-        }
-        const unsigned int b = bf16_bits((float)(v * inv_unit));
-        if (q & 1) wd[q >> 1] |= b << 16; else wd[q >> 1] = b;
-      }
+  for (int q = 0; q < 27; ++q) {
+    const bool in = ok[0][q / 9] && ok[1][(q / 3) % 3] && ok[2][q % 3];
+    const double v = in ? (double)m[q] : 0.0;
+    const unsigned int b = bf16_bits((float)(v * inv_unit));
+    if (q & 1) wd[q >> 1] |= b << 16; else wd[q >> 1] = b;
+  }
   wd[13] |= (unsigned int)bf16_bits(off[0]) << 16;                 // 27
   wd[14] = bf16_bits(off[1]) | ((unsigned int)bf16_bits(off[2]) << 16);
   wd[15] = p < n_mass ? 0x3F80u : 0u;                              // 30, 31
@@ -592,13 +603,18 @@ int ssim_mlp_features(int nb, int n_mass, const int* order, const double* x,
   SSIM_REQUIRE(x && y && z && origin && rho && feat, "null pointer");
   const bool f64 = mesh_prec == SSIM_PREC_F64;
   hipStream_t st = ssim::as_stream(stream);
-  SSIM_TIMED("k_mlp_features", st,
-             hipLaunchKernelGGL(k_mlp_features, dim3(ssim::ceil_div(nb, MB)),
-                                dim3(MB), 0, st, nb, n_mass, order, x, y, z, mesh_n,
-                                origin[0], origin[1], origin[2], h,
-                                f64 ? nullptr : static_cast<const float*>(rho),
-                                f64 ? static_cast<const double*>(rho) : nullptr,
-                                1.0 / mass_unit, feat));
+  if (f64)
+    SSIM_TIMED("k_mlp_features", st,
+               hipLaunchKernelGGL(k_mlp_features<double>, dim3(ssim::ceil_div(nb, MB)),
+                                  dim3(MB), 0, st, nb, n_mass, order, x, y, z, mesh_n,
+                                  origin[0], origin[1], origin[2], h,
+                                  static_cast<const double*>(rho), 1.0 / mass_unit, feat));
+  else
+    SSIM_TIMED("k_mlp_features", st,
+               hipLaunchKernelGGL(k_mlp_features<float>, dim3(ssim::ceil_div(nb, MB)),
+                                  dim3(MB), 0, st, nb, n_mass, order, x, y, z, mesh_n,
+                                  origin[0], origin[1], origin[2], h,
+                                  static_cast<const float*>(rho), 1.0 / mass_unit, feat));
   SSIM_LAUNCHED("k_mlp_features");
   return SSIM_OK;
 }
