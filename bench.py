@@ -118,7 +118,62 @@ def conv_bytes(n, mesh_bytes=4):
     return passes, x3
 
 
+# rocprofv3 kernel names of the bench's timed kernels (PMC passes)
+PMC_KERNELS = {"k_pp_cells": "k_pp_cells<false>", "k_fft_pass": "k_line256",
+               "k_conv_x3": "k_conv_x3", "k_tsc_deposit": "k_tsc_deposit_blocked",
+               "k_tsc_gather": "k_tsc_gather4<true>", "k_mlp_forward": "k_mlp_forward",
+               "k_mlp_features": "k_mlp_features"}
+
+
+def pmc_summary(fetch_dir, write_dir, out):
+    """HBM bytes per launch of the PM kernels from two counter passes over
+    `bench.py --probe` (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch),
+    each scaled by what the counter reads for the probe's 256 MiB copy
+    kernel (torch.mul: 256 MiB in, 256 MiB out)."""
+    import csv
+    rows = {k: list(csv.DictReader(open(Path(d) / "run_counter_collection.csv")))
+            for k, d in (("fetch", fetch_dir), ("write", write_dir))}
+    copy = {k: [r for r in v if "elementwise" in r["Kernel_Name"]
+                and "Fill" not in r["Kernel_Name"]][-1] for k, v in rows.items()}
+    copy_kib = 256 * 1024
+    factor = {k: copy_kib / float(copy[k]["Counter_Value"]) for k in rows}
+    n_mesh = 128
+    passes, x3 = conv_bytes(n_mesh)
+    alg = {"k_fft_pass": sum(passes) / len(passes), "k_conv_x3": x3}
+    rec = {"workload": "250k_cube", "copy_bytes": copy_kib * 1024,
+           "copy_fetch_kib": float(copy["fetch"]["Counter_Value"]),
+           "copy_write_kib": float(copy["write"]["Counter_Value"]),
+           "factor_fetch": factor["fetch"], "factor_write": factor["write"],
+           "kernels": {},
+           "commands": [
+               f"rocprofv3 --pmc FETCH_SIZE -d {fetch_dir} -o run -f csv -- "
+               "python -u bench.py --probe",
+               f"rocprofv3 --pmc WRITE_SIZE -d {write_dir} -o run -f csv -- "
+               "python -u bench.py --probe",
+               f"python bench.py --pmc-summary {fetch_dir} {write_dir} {out}"]}
+    for name, pat in PMC_KERNELS.items():
+        kib = {}
+        for k, v in rows.items():
+            vals = [float(r["Counter_Value"]) for r in v if pat in r["Kernel_Name"]]
+            if not vals:
+                break
+            last = vals[-(len(vals) // 4 or 1):]      # the last of the 4 steps
+            kib[k] = sum(last) / len(last)
+        if len(kib) < 2:
+            continue
+        traffic = (kib["fetch"] * factor["fetch"] + kib["write"] * factor["write"]) * 1024
+        rec["kernels"][name] = {"rocprof_name": pat, "fetch_kib_per_launch": kib["fetch"],
+                                "write_kib_per_launch": kib["write"],
+                                "traffic_bytes_per_launch": traffic,
+                                "algorithmic_bytes_per_launch": alg.get(name)}
+    Path(out).write_text(json.dumps(rec, indent=1))
+    print(json.dumps(rec, indent=1))
+
+
 def main():
+    if len(sys.argv) == 5 and sys.argv[1] == "--pmc-summary":
+        pmc_summary(*sys.argv[2:5])
+        return
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; "
@@ -152,6 +207,7 @@ def main():
         torch.mul(a, 1.0, out=b)          # reads 256 MiB, writes 256 MiB
         fast = PMLearnedSystem(dev, n, t, eps2, 1.0, net=net)
         fast.set_bodies(pos, zeros_n, m, tpos, zeros_t)
+        fast.split.overlap = False        # one kernel at a time on the GPU
         fast.run(dt, 3, graph=False)
         torch.cuda.synchronize(dev)
         print(json.dumps({"probe": "done", "copy_bytes": a.numel() * 4}))
@@ -217,14 +273,13 @@ def main():
     tested, inside = sf.near_pair_counts(fast.x)
     names = ["k_pp_cells", "k_fft_pass", "k_conv_x3", "k_tsc_deposit", "k_tsc_gather",
              "k_cic_deposit", "k_cic_gather", "k_mlp_forward", "k_mlp_features"]
-    # HIP events per launch: eager steps on one stream, so each duration is
-    # the kernel's own and not that of two kernels sharing the GPU
-    fast.split.overlap = False
+    # HIP events per launch over eager steps in the timed run's stream
+    # configuration (events cannot sit inside the replayed graph), so each
+    # duration is the kernel's own under the same sharing of the GPU
     lib.ssim_timing(1)
     fast.run(dt, args.profile_steps, graph=False)
     times = kernel_times(lib, names)
     lib.ssim_timing(0)
-    fast.split.overlap = not args.serial
     steps_p = args.profile_steps
     n_mesh = sf.n
     pass_bytes, x3_bytes = conv_bytes(n_mesh)
@@ -274,8 +329,8 @@ def main():
     traffic = None
     if TRAFFIC_FILE.exists():
         rec = json.loads(TRAFFIC_FILE.read_text())
-        if rec.get("kernel") == dom and rec.get("workload") == "250k_cube":
-            traffic = rec.get("traffic_bytes_per_launch")
+        if rec.get("workload") == "250k_cube" and not args.small:
+            traffic = rec.get("kernels", {}).get(dom, {}).get("traffic_bytes_per_launch")
     roof = {"bound": d["bound"], "kernel": dom, "achieved": d["achieved"],
             "peak": d["peak"], "unit": d["unit"], "frac": d["frac"],
             "traffic": traffic, "launch_us": d["avg_us"], "work": d["work"]}

@@ -536,7 +536,7 @@ __global__ __launch_bounds__(NL * 16) void k_line256(
   __syncthreads();
   const int l = YL ? threadIdx.x % NL : threadIdx.x / 16;
   const int j = YL ? threadIdx.x / NL : threadIdx.x % 16;
-  const int line = blockIdx.x * NL + l;
+  const int line = ssim_xcd_block(blockIdx.x, gridDim.x) * NL + l;  // XCD-contiguous
   const bool live = line < lines;
   const int lc = live ? line : 0;
   cpx<F> v[16];

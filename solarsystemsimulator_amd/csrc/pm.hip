@@ -549,7 +549,9 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
     const double* __restrict__ z, Grid g, const float4* __restrict__ f,
     double* __restrict__ o0, double* __restrict__ o1, double* __restrict__ o2,
     int accumulate, Corr cr) {
-  const int i_ = blockIdx.x * PM_BLOCK + threadIdx.x;
+  // XCD-contiguous ranges of the (cell-ordered) bodies: an XCD's L2 then
+  // holds the part of the mesh its bodies touch
+  const int i_ = ssim_xcd_block(blockIdx.x, gridDim.x) * PM_BLOCK + threadIdx.x;
   if (i_ >= np) return;
   const int p = order ? order[i_] : i_;
   int c[3];

@@ -148,6 +148,19 @@ __host__ __device__ inline double ssim_split_g(double r, double rs, double rcut)
   return g;
 }
 
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
+// XCDs (block b runs on XCD b % 8), each with its own L2; a kernel whose
+// neighbouring blocks share data (a mesh stencil, a source list, a cache
+// line) renumbers them so that XCD x takes the contiguous logical range
+// [start(x), start(x) + count(x)) in dispatch order.  A bijection on
+// [0, nblocks).
+constexpr int SSIM_XCDS = 8;
+__device__ __forceinline__ int ssim_xcd_block(int b, int nblocks) {
+  const int per = nblocks / SSIM_XCDS, rem = nblocks % SSIM_XCDS;
+  const int x = b % SSIM_XCDS, i = b / SSIM_XCDS;
+  return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + i;
+}
+
 // Bin key of mesh node (i, j, k) in ssim_split_bin's blocked layout:
 // blocks of B^3 nodes, nb blocks per axis, block-major then node-major,
 //   ((i/B nb + j/B) nb + k/B) B^3 + ((i%B) B + j%B) B + k%B.
