@@ -1,8 +1,8 @@
 // Stable LSD radix sort of (int key, int value) pairs and an exclusive scan,
 // shared by the cell lists (cells.hip) and the near-field chunk map (pp.hip).
 //
-// The sort takes 8 key bits per pass over tiles of RTILE = 4096 elements
-// (16 rounds of one element per thread of a 256-thread workgroup):
+// The sort takes 8 key bits per pass over tiles of RTILE = 1024 elements
+// (4 rounds of one element per thread of a 256-thread workgroup):
 //   k_radix_hist0   — per-tile digit counts of the first pass (digit-major,
 //                     hist[d * hrow + b], rows padded to a multiple of 4);
 //                     it also clears the count arrays of the later passes;
@@ -24,7 +24,7 @@
 namespace {
 
 constexpr int RT = 256;            // threads per workgroup (one per digit)
-constexpr int RIPT = 16;           // rounds per tile
+constexpr int RIPT = 4;            // rounds per tile
 constexpr int RTILE = RT * RIPT;   // elements per tile
 constexpr int FUSE_TILES = 256;    // up to this many tiles a scatter block
                                    // computes its own offsets
