@@ -207,6 +207,10 @@ __global__ __launch_bounds__(PM_BLOCK) void k_box_kdk_partial(
     }
 }
 
+// ssim_kick_box: eight bodies per thread (the minmax partial pass's grid):
+// more workgroups lengthen the last workgroup's fold more than they gain
+constexpr int KB_ITEMS = 8;
+
 // The closing half kick of a step (v += a half, stored) fused with the box
 // of the positions the next step's kick + drift will produce (as
 // k_box_kdk_partial), in one launch: every workgroup publishes its extrema,
@@ -223,17 +227,30 @@ __global__ __launch_bounds__(PM_BLOCK) void k_kick_box(
   const double* xs[3] = {x0, x1, x2};
   double* vs[3] = {v0, v1, v2};
   const double* as[3] = {a0, a1, a2};
+  // a tile of KB_ITEMS bodies per thread (stride PM_BLOCK), all loads of a
+  // component issued before the arithmetic
+  const int base = blockIdx.x * PM_BLOCK * KB_ITEMS + threadIdx.x;
   for (int c = 0; c < na; ++c) {
     double lo = INFINITY, hi = -INFINITY;
-    for (int i = blockIdx.x * PM_BLOCK + threadIdx.x; i < n;
-         i += gridDim.x * PM_BLOCK) {
-      const double a = as[c][i];
-      const double v = vs[c][i] + a * half;   // this step's closing kick
-      vs[c][i] = v;
-      const double nv = v + a * half;          // the next step's opening kick
-      const double p = xs[c][i] + nv * full;   // and its drift
-      lo = fmin(lo, p);
-      hi = fmax(hi, p);
+    double av[KB_ITEMS], vv[KB_ITEMS], xv[KB_ITEMS];
+#pragma unroll
+    for (int q = 0; q < KB_ITEMS; ++q) {
+      const int i = min(base + q * PM_BLOCK, n - 1);
+      av[q] = as[c][i];
+      vv[q] = vs[c][i];
+      xv[q] = xs[c][i];
+    }
+#pragma unroll
+    for (int q = 0; q < KB_ITEMS; ++q) {
+      const int i = base + q * PM_BLOCK;
+      if (i < n) {
+        const double v = vv[q] + av[q] * half;  // this step's closing kick
+        vs[c][i] = v;
+        const double nv = v + av[q] * half;     // the next step's opening kick
+        const double p = xv[q] + nv * full;     // and its drift
+        lo = fmin(lo, p);
+        hi = fmax(hi, p);
+      }
     }
     smin[c][threadIdx.x] = lo;
     smax[c][threadIdx.x] = hi;
@@ -429,10 +446,13 @@ __device__ __forceinline__ double tsc_weight_f(double f, int node) {
 // (B >= 2: never more); both are walked as one index range so every lane
 // of a wave runs one loop.  body[] holds (f_x, f_y, f_z, m) with
 // f = (x - o)/h computed by ssim_split_bin.
-template <typename M>
+// BC > 0: the block edge as a compile-time constant (its divisions become
+// multiply-shifts); BC == 0: the runtime value Brt.
+template <typename M, int BC>
 __global__ __launch_bounds__(PM_BLOCK) void k_tsc_deposit_blocked(
-    const double4* __restrict__ body, Grid g, int B, int nb,
+    const double4* __restrict__ body, Grid g, int Brt, int nb,
     const int* __restrict__ bin_start, M* __restrict__ mesh) {
+  const int B = BC > 0 ? BC : Brt;
   const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
   const long long node = (long long)blockIdx.x * PM_BLOCK + threadIdx.x;
   if (node >= (long long)n0 * n1 * n2) return;
@@ -966,9 +986,6 @@ int ssim_box_after_kdk(int dim, int n, const double* x, const double* y,
   return SSIM_OK;
 }
 
-// ssim_kick_box: eight bodies per thread (the minmax partial pass's grid):
-// more workgroups lengthen the last workgroup's fold more than they gain
-constexpr int KB_ITEMS = 8;
 
 size_t ssim_kick_box_work_bytes(int n) {
   const int nblk = ssim::ceil_div(n < 1 ? 1 : n, PM_BLOCK * KB_ITEMS);
@@ -1127,16 +1144,26 @@ int ssim_tsc_deposit_blocked(int grid_n, const double* origin, double h, int blo
   const long long nodes = (long long)grid_n * grid_n * grid_n;
   const dim3 grid((unsigned)ssim::ceil_div(nodes, PM_BLOCK));
   const double4* bp = static_cast<const double4*>(body);
-  if (mesh_prec == SSIM_PREC_F64)
-    SSIM_TIMED("k_tsc_deposit", st,
-               hipLaunchKernelGGL(k_tsc_deposit_blocked<double>, grid, dim3(PM_BLOCK), 0,
-                                  st, bp, g, block, nb, bin_start,
-                                  static_cast<double*>(mesh)));
-  else
-    SSIM_TIMED("k_tsc_deposit", st,
-               hipLaunchKernelGGL(k_tsc_deposit_blocked<float>, grid, dim3(PM_BLOCK), 0,
-                                  st, bp, g, block, nb, bin_start,
-                                  static_cast<float*>(mesh)));
+#define LAUNCH_DEPB(M, BC)                                                          \
+  SSIM_TIMED("k_tsc_deposit", st,                                                   \
+             hipLaunchKernelGGL((k_tsc_deposit_blocked<M, BC>), grid, dim3(PM_BLOCK), 0, \
+                                st, bp, g, block, nb, bin_start, static_cast<M*>(mesh)))
+#define LAUNCH_DEPB_B(M)                       \
+  switch (block) {                             \
+    case 2: LAUNCH_DEPB(M, 2); break;          \
+    case 3: LAUNCH_DEPB(M, 3); break;          \
+    case 4: LAUNCH_DEPB(M, 4); break;          \
+    case 5: LAUNCH_DEPB(M, 5); break;          \
+    case 6: LAUNCH_DEPB(M, 6); break;          \
+    default: LAUNCH_DEPB(M, 0); break;         \
+  }
+  if (mesh_prec == SSIM_PREC_F64) {
+    LAUNCH_DEPB_B(double)
+  } else {
+    LAUNCH_DEPB_B(float)
+  }
+#undef LAUNCH_DEPB_B
+#undef LAUNCH_DEPB
   SSIM_LAUNCHED("k_tsc_deposit_blocked");
   return SSIM_OK;
 }

@@ -167,8 +167,11 @@ class PMLearnedSystem:
                   *ps(self.a), half, full, _lib.ptr(self._box_dev),
                   _lib.ptr(self._box_ws), self._box_ws.numel(), D.stream(self.dev))
 
-    def _read_box(self):
-        self._box_host.copy_(self._box_dev, non_blocking=True)
+    def _read_box(self, in_host=False):
+        """The box of the next drift: from the pinned host buffer (in_host:
+        written there by ssim_kick_box) or copied from the device one."""
+        if not in_host:
+            self._box_host.copy_(self._box_dev, non_blocking=True)
         self._box_ev.record()
         self._box_ev.synchronize()
         b = self._box_host.numpy()
@@ -184,9 +187,12 @@ class PMLearnedSystem:
         _lib.call("ssim_kick_drift", 3, nb, *ps(self.x), *ps(self.v), *ps(self.a),
                   half, dt, st)
         self.force(learned, placed=True)
-        if snap_ptr is None:          # closing kick + next box in one launch
+        if snap_ptr is None:
+            # closing kick + next box in one launch; the box goes straight
+            # into the pinned host buffer (kernels can write it), so reading
+            # it needs no copy
             _lib.call("ssim_kick_box", 3, nb, *ps(self.x), *ps(self.v), *ps(self.a),
-                      half, dt, _lib.ptr(self._box_dev), _lib.ptr(self._kb_ws),
+                      half, dt, _lib.ptr(self._box_host), _lib.ptr(self._kb_ws),
                       self._kb_ws.numel(), st)
             return
         _lib.call("ssim_kick_snap", 3, self.n, self.t, *ps(self.x), *ps(self.v),
@@ -226,8 +232,10 @@ class PMLearnedSystem:
         use = self.use_graph if graph is None else graph
         use = use and snap is None and self.split.freeze
         self._next_box(0.5 * dt, dt)
+        in_host = False
         for s in range(steps):
-            self.split.place_box(self._read_box())
+            self.split.place_box(self._read_box(in_host))
+            in_host = snap is None    # the step's ssim_kick_box writes it
             if use:
                 self._graph_for(dt, learned).replay()
             else:
