@@ -334,7 +334,7 @@ int ssim_spectrum_imag(long long ncomplex, int prec, const void* a, void* out,
  * ssim_fft_r2c + ssim_spectrum_mul + ssim_fft_c2r + crop, computed without
  * the padding or the discarded part (pruned passes).  With fy = fz = NULL
  * the three fields are written interleaved into fx as [n^3][4] (component
- * c at index 4 i + c; the 4th slot is left as it is).  spec and tmp hold
+ * c at index 4 i + c; the 4th slot is either left as it is or set to 0).  spec and tmp hold
  * (2n)^2 (n+1) complex values each; fft_work is prepared for dims
  * (2n, 2n, 2n).  No reference counterpart (split force). */
 int ssim_pm_conv_isolated3(int n, int prec, const void* rho, const void* kim_x,

@@ -506,41 +506,21 @@ __global__ __launch_bounds__(NL * 16) void k_conv_x3(
   }
 }
 
-// ---- 256-point lines of the isolated convolution, register-direct ----
-// The y and z passes of the padded 2n = 256 mesh (n = 128, H = 129) in the
-// style of k_conv_x3: thread (line l, index j) loads its 16 (or 8) inputs
-// straight from HBM, runs the four-step FFT with one LDS exchange, and stores
-// its outputs straight from registers; only the x < n planes exist.
-//   LY_FWD: y lines of plane x (kz = line % H), inputs ky < n, all 256 out;
-//   LY_INV: the same lines, all 256 in, ky < n out;
-//   LZ_R2C: rows (x, y) of the real n^3 mesh (z < n, zeros beyond), the
-//           H = n + 1 half-spectrum values out;
-//   LZ_C2R: rows (x, y) of the half spectrum (Hermitian completion), z < n
-//           real out, times `scale`, at out[(row * n + z) * es + ec].
-// y lines: lanes take consecutive lines (consecutive kz: 128-byte rows);
-// z rows: lanes take consecutive indices of one row.
-enum { LY_FWD = 0, LY_INV = 1, LZ_R2C = 2, LZ_C2R = 3 };
+enum { LY_FWD = 0, LY_INV = 1, LZ_R2C = 2, LZ_C2R = 3, LZ_C2R3 = 4 };
 
-template <typename F, int NL, int KIND>
-__global__ __launch_bounds__(NL * 16) void k_line256(
-    const void* vin, void* vout, const cpx<F>* __restrict__ tw,  // y passes: in place
-    int lines, F scale, int es, int ec) {
+// Up to three arrays of one pass: component c's lines are lines
+// [c * per, (c + 1) * per) of the launch.
+struct Lines3 {
+  const void* in[3];
+  void* out[3];
+};
+
+// load the 16 inputs x[j + 16 n2] of a line (see k_line256)
+template <typename F, int KIND>
+__device__ __forceinline__ void line_load(const void* vin, int lc, int j, bool live,
+                                          cpx<F> v[16]) {
   constexpr int n = 128, H = 129, PH = 256 * 129;
-  constexpr bool YL = KIND == LY_FWD || KIND == LY_INV;
-  // exchange rows: 258 apart when lanes take consecutive lines, 272 when
-  // they take consecutive indices of a line (both free of bank conflicts)
-  constexpr int LS = YL ? HUB_LS : 272;
-  __shared__ cpx<F> s[NL * LS];
-  __shared__ cpx<F> stw[256];
-  for (int t = threadIdx.x; t < 256; t += NL * 16) stw[t] = tw[t];
-  __syncthreads();
-  const int l = YL ? threadIdx.x % NL : threadIdx.x / 16;
-  const int j = YL ? threadIdx.x / NL : threadIdx.x % 16;
-  const int line = ssim_xcd_block(blockIdx.x, gridDim.x) * NL + l;  // XCD-contiguous
-  const bool live = line < lines;
-  const int lc = live ? line : 0;
-  cpx<F> v[16];
-  if (YL) {
+  if (KIND == LY_FWD || KIND == LY_INV) {
     // line = x * H + kz; element ky at x * PH + ky * H + kz
     const int x = lc / H, kz = lc - x * H;
     const cpx<F>* in = static_cast<const cpx<F>*>(vin) + (size_t)x * PH + kz;
@@ -572,26 +552,83 @@ __global__ __launch_bounds__(NL * 16) void k_line256(
       v[n2] = c;
     }
   }
-  const bool inverse = KIND == LY_INV || KIND == LZ_C2R;
+}
+
+// ---- 256-point lines of the isolated convolution, register-direct ----
+// The y and z passes of the padded 2n = 256 mesh (n = 128, H = 129) in the
+// style of k_conv_x3: thread (line l, index j) loads its 16 (or 8) inputs
+// straight from HBM, runs the four-step FFT with one LDS exchange, and stores
+// its outputs straight from registers; only the x < n planes exist.
+//   LY_FWD:  y lines of plane x (kz = line % H), inputs ky < n, all 256 out;
+//   LY_INV:  the same lines, all 256 in, ky < n out (`per` lines per
+//            component, up to three components in one launch);
+//   LZ_R2C:  rows (x, y) of the real n^3 mesh (z < n, zeros beyond), the
+//            H = n + 1 half-spectrum values out;
+//   LZ_C2R:  rows (x, y) of the half spectrum (Hermitian completion), z < n
+//            real out, times `scale`, at out[(row * n + z) * es + ec];
+//   LZ_C2R3: the same rows of all three components, written as float4
+//            (c0, c1, c2, 0) [n^3]: whole 16-byte slots, one launch.
+// y lines: lanes take consecutive lines (consecutive kz: 128-byte rows);
+// z rows: lanes take consecutive indices of one row.
+template <typename F, int NL, int KIND>
+__global__ __launch_bounds__(NL * 16) void k_line256(
+    Lines3 P, const cpx<F>* __restrict__ tw,  // y passes: in place
+    int lines, int per, F scale, int es, int ec) {
+  constexpr int n = 128, H = 129, PH = 256 * 129;
+  constexpr bool YL = KIND == LY_FWD || KIND == LY_INV;
+  // exchange rows: 258 apart when lanes take consecutive lines, 272 when
+  // they take consecutive indices of a line (both free of bank conflicts)
+  constexpr int LS = YL ? HUB_LS : 272;
+  __shared__ cpx<F> s[NL * LS];
+  __shared__ cpx<F> stw[256];
+  for (int t = threadIdx.x; t < 256; t += NL * 16) stw[t] = tw[t];
+  __syncthreads();
+  const int l = YL ? threadIdx.x % NL : threadIdx.x / 16;
+  const int j = YL ? threadIdx.x / NL : threadIdx.x % 16;
+  const int line = ssim_xcd_block(blockIdx.x, gridDim.x) * NL + l;  // XCD-contiguous
+  const bool live = line < lines;
+  const int lg = live ? line : 0;
+  const int comp = lg / per, lc = lg - comp * per;  // component, line in it
+  const bool inverse = KIND == LY_INV || KIND == LZ_C2R || KIND == LZ_C2R3;
+  cpx<F> v[16];
+  if (KIND == LZ_C2R3) {
+    F res[3][8];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      line_load<F, LZ_C2R>(P.in[c], lc, j, live, v);
+      dft16(v, true);
+      hub_exchange<F, NL, LS>(v, s, l, j, stw, true);
+#pragma unroll
+      for (int k1 = 0; k1 < 8; ++k1) res[c][k1] = v[k1].re * scale;
+    }
+    if (!live) return;
+    float4* out = static_cast<float4*>(P.out[0]);
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1)
+      out[(size_t)lc * n + j + 16 * k1] =
+          make_float4((float)res[0][k1], (float)res[1][k1], (float)res[2][k1], 0.f);
+    return;
+  }
+  line_load<F, KIND>(P.in[comp], lc, j, live, v);
   dft16(v, inverse);
   hub_exchange<F, NL, LS>(v, s, l, j, stw, inverse);
   // v[k1] = X[j + 16 k1]
   if (!live) return;
   if (YL) {
     const int x = lc / H, kz = lc - x * H;
-    cpx<F>* out = static_cast<cpx<F>*>(vout) + (size_t)x * PH + kz;
+    cpx<F>* out = static_cast<cpx<F>*>(P.out[comp]) + (size_t)x * PH + kz;
     const int nout = KIND == LY_FWD ? 16 : 8;
 #pragma unroll
     for (int k1 = 0; k1 < 16; ++k1)
       if (k1 < nout) out[(j + 16 * k1) * H] = v[k1];
   } else if (KIND == LZ_R2C) {
     const int x = lc / n, y = lc - x * n;
-    cpx<F>* out = static_cast<cpx<F>*>(vout) + (size_t)x * PH + y * H;
+    cpx<F>* out = static_cast<cpx<F>*>(P.out[0]) + (size_t)x * PH + y * H;
 #pragma unroll
     for (int k1 = 0; k1 < 8; ++k1) out[j + 16 * k1] = v[k1];
     if (j == 0) out[n] = v[8];
   } else {
-    F* out = static_cast<F*>(vout);
+    F* out = static_cast<F*>(P.out[0]);
 #pragma unroll
     for (int k1 = 0; k1 < 8; ++k1)
       out[((size_t)lc * n + j + 16 * k1) * es + ec] = v[k1].re * scale;
@@ -750,13 +787,15 @@ int conv256(const F* rho, const F* const* kim, F* const* fields, cpx<F>* spec,
   const cpx<F>* tz = reinterpret_cast<const cpx<F>*>(twz);
   const int rows = n * n, ylines = n * H;
   const dim3 blk(NL * 16);
+  Lines3 L{{rho, nullptr, nullptr}, {spec, nullptr, nullptr}};
   SSIM_TIMED("k_fft_pass", st,
              hipLaunchKernelGGL((k_line256<F, NL, LZ_R2C>), dim3(ssim::ceil_div(rows, NL)),
-                                blk, 0, st, rho, spec, tz, rows, F(1), 1, 0));
+                                blk, 0, st, L, tz, rows, rows, F(1), 1, 0));
   SSIM_LAUNCHED("k_line256");
+  L = Lines3{{spec, nullptr, nullptr}, {spec, nullptr, nullptr}};
   SSIM_TIMED("k_fft_pass", st,
              hipLaunchKernelGGL((k_line256<F, NL, LY_FWD>), dim3(ssim::ceil_div(ylines, NL)),
-                                blk, 0, st, spec, spec, ty, ylines, F(1), 1, 0));
+                                blk, 0, st, L, ty, ylines, ylines, F(1), 1, 0));
   SSIM_LAUNCHED("k_line256");
   // fused x axis: component 0 in place in spec (x < n), components 1 and 2
   // in the two halves of tmp, each an [n][P][H] block
@@ -768,16 +807,28 @@ int conv256(const F* rho, const F* const* kim, F* const* fields, cpx<F>* spec,
                                 dim3(NL * 16), 0, st, spec, kim[0], kim[1], kim[2],
                                 blk3[0], blk3[1], blk3[2], tx, (int)lines));
   SSIM_LAUNCHED("k_conv_x3");
-  for (int c = 0; c < 3; ++c) {
+  // inverse y lines of the three components in one launch
+  L = Lines3{{blk3[0], blk3[1], blk3[2]}, {blk3[0], blk3[1], blk3[2]}};
+  SSIM_TIMED("k_fft_pass", st,
+             hipLaunchKernelGGL((k_line256<F, NL, LY_INV>),
+                                dim3(ssim::ceil_div(3 * ylines, NL)), blk, 0, st, L, ty,
+                                3 * ylines, ylines, F(1), 1, 0));
+  SSIM_LAUNCHED("k_line256");
+  if (inter && sizeof(F) == 4) {
+    // C2R rows of the three components, whole float4 slots
+    L = Lines3{{blk3[0], blk3[1], blk3[2]}, {fields[0], nullptr, nullptr}};
     SSIM_TIMED("k_fft_pass", st,
-               hipLaunchKernelGGL((k_line256<F, NL, LY_INV>),
-                                  dim3(ssim::ceil_div(ylines, NL)), blk, 0, st, blk3[c],
-                                  blk3[c], ty, ylines, F(1), 1, 0));
+               hipLaunchKernelGGL((k_line256<F, NL, LZ_C2R3>), dim3(ssim::ceil_div(rows, NL)),
+                                  blk, 0, st, L, tz, rows, rows, scale, 4, 0));
     SSIM_LAUNCHED("k_line256");
+    return SSIM_OK;
+  }
+  for (int c = 0; c < 3; ++c) {
+    L = Lines3{{blk3[c], nullptr, nullptr}, {inter ? fields[0] : fields[c], nullptr, nullptr}};
     SSIM_TIMED("k_fft_pass", st,
                hipLaunchKernelGGL((k_line256<F, NL, LZ_C2R>), dim3(ssim::ceil_div(rows, NL)),
-                                  blk, 0, st, blk3[c], inter ? fields[0] : fields[c], tz,
-                                  rows, scale, inter ? 4 : 1, inter ? c : 0));
+                                  blk, 0, st, L, tz, rows, rows, scale, inter ? 4 : 1,
+                                  inter ? c : 0));
     SSIM_LAUNCHED("k_line256");
   }
   return SSIM_OK;

@@ -285,7 +285,7 @@ def test_pruned_convolution_matches_padded_fft(dev, prec, tol, n):
         f4 = sf.convolve(interleaved=True).cpu().numpy()
         for c in range(3):
             assert np.array_equal(f4[:, c].reshape(n, n, n), got[c].astype(np.float32))
-        assert np.all(f4[:, 3] == 9.0)
+        assert np.all(f4[:, 3] == (0.0 if n == 128 else 9.0))   # whole slots at n = 128
     sf.fused = False
     ref_dev = [f.cpu().numpy().astype(np.float64) for f in sf.convolve()]
     P = 2 * n
