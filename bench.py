@@ -105,15 +105,16 @@ def cpu_baseline(pos, m, tpos, eps2):
 def conv_bytes(n, mesh_bytes=4):
     """Algorithmic HBM bytes of one isolated convolution of an n^3 FP32 mesh
     (padded P = 2n, half spectra of H = n + 1, complex 2 x mesh_bytes,
-    kernel spectra real mesh_bytes): the 8 pruned FFT passes (R2C rows,
-    y lines, and per component y lines + C2R rows) and the fused x-axis
-    kernel (x < n half in, three kernel spectra in, three x < n halves out)."""
+    kernel spectra real mesh_bytes): the 4 pruned FFT launches (R2C rows,
+    y lines, the three components' inverse y lines, the three components'
+    C2R rows written as one float4 field) and the fused x-axis kernel (x < n
+    half in, three kernel spectra in, three x < n halves out)."""
     P, H = 2 * n, n + 1
     c = 2 * mesh_bytes
     passes = [n ** 3 * mesh_bytes + n * n * H * c,          # R2C rows from rho
-              n * n * H * c + n * P * H * c]                # y lines, x < n
-    passes += 3 * [n * P * H * c + n * n * H * c,           # y lines, x < n
-                   n * n * H * c + n ** 3 * mesh_bytes]     # C2R rows
+              n * n * H * c + n * P * H * c,                # y lines, x < n
+              3 * (n * P * H * c + n * n * H * c),          # inverse y lines
+              3 * n * n * H * c + n ** 3 * 4 * mesh_bytes]  # C2R rows, float4
     x3 = n * P * H * c + 3 * P * P * H * mesh_bytes + 3 * n * P * H * c
     return passes, x3
 
@@ -289,7 +290,7 @@ def main():
                        "TFLOP/s", f"{FLOPS_PER_NEAR_PAIR} flop x {inside:.4g} "
                        f"pairs inside rcut ({tested:.4g} pairs tested)"),
         "k_fft_pass": ("hbm", sum(pass_bytes) / len(pass_bytes), HBM_PEAK_GBS,
-                       "GB/s", f"{sum(pass_bytes) / 1e6:.1f} MB over the 8 pruned "
+                       "GB/s", f"{sum(pass_bytes) / 1e6:.1f} MB over the 4 pruned "
                        f"FFT passes of one {2 * n_mesh}^3 isolated convolution"),
         "k_conv_x3": ("hbm", x3_bytes, HBM_PEAK_GBS, "GB/s",
                       f"{x3_bytes / 1e6:.1f} MB: x < {n_mesh} half in, 3 kernel "

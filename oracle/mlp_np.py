@@ -5,7 +5,10 @@ mathematics only: its f64 backward pass is checked against central finite
 differences (tests/test_mlp_oracle.py), and its BF16 mode rounds exactly
 where the GPU kernels round (inputs, weights, activations, output
 gradients and layer deltas to BF16, round to nearest even) so the GPU's
-forward, gradients and Adam step can be compared to it.
+forward, gradients and Adam step can be compared to it.  The inference
+kernel rounds the activations scaled by -log2(e) (`act_scale`); its result
+is also checked against the pure-f64 network, within the error of the
+BF16-rounded network itself.
 
 Parameter block layout (float32, NPARAM values):
   W1[64][32] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[16][64] b4[16]
@@ -64,8 +67,18 @@ def silu_grad(z):
     return s * (1.0 + z * (1.0 - s))
 
 
-def forward(p, x, rounding=True):
-    """Returns (out [n,3], cache).  rounding=False is the pure-f64 network."""
+# activation scale of the inference kernel: it keeps the BF16 rounding of
+# -log2(e) silu(z) (its hidden layers carry t = -log2(e) z), the training
+# kernels the rounding of silu(z) itself
+INFER_ACT_SCALE = -1.4426950408889634
+
+
+def forward(p, x, rounding=True, act_scale=1.0):
+    """Returns (out [n,3], cache).  rounding=False is the pure-f64 network.
+
+    With rounding, each hidden activation h = silu(z) enters the next layer
+    as bf16(act_scale h) / act_scale (exact in f64 for the scale): 1 for the
+    training kernels, INFER_ACT_SCALE for the inference kernel."""
     r = bf16 if rounding else (lambda a: np.asarray(a, np.float64))
     P = unpack(np.asarray(p, np.float64) if not rounding else p)
     W = {k: (r(v) if k.startswith("W") else np.asarray(v, np.float64))
@@ -77,7 +90,10 @@ def forward(p, x, rounding=True):
         z = h @ W[f"W{l}"].T.astype(np.float64) + W[f"b{l}"]
         if rounding:
             z = z.astype(np.float32).astype(np.float64)
-        h = r(silu(z).astype(np.float32) if rounding else silu(z)).astype(np.float64)
+        if rounding:
+            h = bf16((act_scale * silu(z)).astype(np.float32)).astype(np.float64) / act_scale
+        else:
+            h = silu(z)
         zs.append(z)
         hs.append(h)
     out = h @ W["W4"].T.astype(np.float64) + W["b4"]

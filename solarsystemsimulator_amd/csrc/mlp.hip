@@ -173,67 +173,6 @@ __device__ void forward16(const Weights& W, const float* __restrict__ bias,
   for (int i = 0; i < 3; ++i) out3[i] = acc[i] + bias[3 * NH + i];
 }
 
-// Inference on two 16-body column blocks at once: the MFMA chains of the
-// two blocks are independent, so the matrix cores and the SiLU work of one
-// block overlap with the other's.
-__device__ void forward2x16(const Weights& W, const float* __restrict__ bias,
-                            const unsigned short* __restrict__ x, int nb, int b0,
-                            int lane, float out3[2][3], bool valid[2]) {
-  s4_t xb[2][2], h1[2][4], h2[2][4], h3[2][4];
-#pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    const int body = b0 + 16 * u + (lane & 15);
-    valid[u] = body < nb;
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb)
-      xb[u][kb] = valid[u] ? *reinterpret_cast<const s4_t*>(
-                                 x + (size_t)body * NIN + 16 * kb + 4 * (lane >> 4))
-                           : s4_t{0, 0, 0, 0};
-  }
-#pragma unroll
-  for (int rt = 0; rt < 4; ++rt) {
-    f4_t acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb)
-#pragma unroll
-      for (int u = 0; u < 2; ++u) acc[u] = mfma(W.w1[rt][kb], xb[u][kb], acc[u]);
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-      h1[u][rt] = activate(acc[u], bias, rt, lane, nullptr, nullptr);
-  }
-#pragma unroll
-  for (int rt = 0; rt < 4; ++rt) {
-    f4_t acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb)
-#pragma unroll
-      for (int u = 0; u < 2; ++u) acc[u] = mfma(W.w2[rt][kb], h1[u][kb], acc[u]);
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-      h2[u][rt] = activate(acc[u], bias + NH, rt, lane, nullptr, nullptr);
-  }
-#pragma unroll
-  for (int rt = 0; rt < 4; ++rt) {
-    f4_t acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb)
-#pragma unroll
-      for (int u = 0; u < 2; ++u) acc[u] = mfma(W.w3[rt][kb], h2[u][kb], acc[u]);
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-      h3[u][rt] = activate(acc[u], bias + 2 * NH, rt, lane, nullptr, nullptr);
-  }
-  f4_t acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-#pragma unroll
-  for (int kb = 0; kb < 4; ++kb)
-#pragma unroll
-    for (int u = 0; u < 2; ++u) acc[u] = mfma(W.w4[kb], h3[u][kb], acc[u]);
-#pragma unroll
-  for (int u = 0; u < 2; ++u)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) out3[u][i] = acc[u][i] + bias[3 * NH + i];
-}
-
 __global__ __launch_bounds__(64 * WAVES) void k_mlp_forward(
     int nb, const unsigned short* __restrict__ x,
     const unsigned short* __restrict__ wbf, const float* __restrict__ params,
@@ -286,23 +225,184 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_forward(
       a2[body] += d2;
     }
   };
-  if (zs == nullptr) {  // inference: two column blocks per iteration
-    for (int b0 = wave * 32; b0 < nb; b0 += nwaves * 32) {
-      float o[2][3];
-      bool valid[2];
-      forward2x16(W, sbias, x, nb, b0, lane, o, valid);
-      if (lane < 16) {
-        if (valid[0]) emit(b0 + lane, o[0]);
-        if (valid[1]) emit(b0 + 16 + lane, o[1]);
-      }
-    }
-    return;
-  }
   for (int b0 = wave * 16; b0 < nb; b0 += nwaves * 16) {
     float o[3];
     bool valid;
     forward16(W, sbias, x, nb, b0, lane, o, valid, zs, hs);
     if (lane < 16 && valid) emit(b0 + lane, o);
+  }
+}
+
+// ---- inference ----
+// The hidden layers carry t = -log2(e) z instead of z: sigmoid(z) =
+// 1 / (1 + 2^t) is one v_exp_f32 and one v_rcp_f32, and the activation
+// kept (BF16, rounded by v_cvt_pk_bf16_f32) is h' = t sigmoid(z) =
+// -log2(e) silu(z).  The next layer's W h' + b' with b' = -log2(e) b is then
+// exactly its own t (log2(e) ln(2) = 1), so the weights stay as trained:
+// layer 1 scales its accumulator once, and layer 4 gives
+// out = -ln(2) (W4 h3') + b4.  Biases enter as the MFMA's C operand.
+typedef __bf16 bf2_t __attribute__((ext_vector_type(2)));
+typedef float f2_t __attribute__((ext_vector_type(2)));
+
+#ifndef SSIM_IU
+#define SSIM_IU 2
+#endif
+#ifndef SSIM_INFER_ATTR
+#define SSIM_INFER_ATTR
+#endif
+constexpr int IU = SSIM_IU;  // column blocks per iteration
+constexpr float NEG_LOG2E = -1.4426950408889634f;
+constexpr float NEG_LN2 = -0.6931471805599453f;
+
+__device__ __forceinline__ s4_t act_t(f4_t t) {
+  float h[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    h[i] = t[i] * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t[i]));
+  const bf2_t lo = __builtin_convertvector((f2_t{h[0], h[1]}), bf2_t);
+  const bf2_t hi = __builtin_convertvector((f2_t{h[2], h[3]}), bf2_t);
+  const unsigned int ul = __builtin_bit_cast(unsigned int, lo);
+  const unsigned int uh = __builtin_bit_cast(unsigned int, hi);
+  s4_t out;
+  out.x = (short)(ul & 0xFFFFu);
+  out.y = (short)(ul >> 16);
+  out.z = (short)(uh & 0xFFFFu);
+  out.w = (short)(uh >> 16);
+  return out;
+}
+
+// U 16-body column blocks per iteration (independent MFMA chains);
+// sb: b1 | -log2(e) b2 | -log2(e) b3 | b4
+template <int U>
+__device__ __forceinline__ void infer16(const Weights& W, const float* __restrict__ sb,
+                                        const unsigned short* __restrict__ x, int nb,
+                                        int b0, int lane, float out3[U][3], bool valid[U]) {
+  s4_t xb[U][2], h1[U][4], h2[U][4], h3[U][4];
+  const int g4 = 4 * (lane >> 4);
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int body = b0 + 16 * u + (lane & 15);
+    valid[u] = body < nb;
+    const int bl = valid[u] ? body : 0;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+      const s4_t v = *reinterpret_cast<const s4_t*>(x + (size_t)bl * NIN + 16 * kb + g4);
+      xb[u][kb] = valid[u] ? v : s4_t{0, 0, 0, 0};
+    }
+  }
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    const f4_t b = *reinterpret_cast<const f4_t*>(sb + 16 * rt + g4);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      f4_t acc = mfma(W.w1[rt][0], xb[u][0], b);
+      acc = mfma(W.w1[rt][1], xb[u][1], acc);
+      h1[u][rt] = act_t(acc * NEG_LOG2E);
+    }
+  }
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    const f4_t b = *reinterpret_cast<const f4_t*>(sb + NH + 16 * rt + g4);
+    f4_t acc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc[u] = b;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc[u] = mfma(W.w2[rt][kb], h1[u][kb], acc[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) h2[u][rt] = act_t(acc[u]);
+  }
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    const f4_t b = *reinterpret_cast<const f4_t*>(sb + 2 * NH + 16 * rt + g4);
+    f4_t acc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc[u] = b;
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc[u] = mfma(W.w3[rt][kb], h2[u][kb], acc[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) h3[u][rt] = act_t(acc[u]);
+  }
+  f4_t acc[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) acc[u] = f4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc[u] = mfma(W.w4[kb], h3[u][kb], acc[u]);
+  // rows 0..2 live in lanes 0..15 (4(l/16) + i with l/16 == 0)
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) out3[u][i] = fmaf(acc[u][i], NEG_LN2, sb[3 * NH + i]);
+}
+
+// ADD: add the (scaled, projected) correction to a as k_mlp_forward does;
+// otherwise only store the raw outputs.
+template <bool ADD>
+__global__ __launch_bounds__(64 * WAVES) SSIM_INFER_ATTR void k_mlp_infer(
+    int nb, const unsigned short* __restrict__ x, const unsigned short* __restrict__ wbf,
+    const float* __restrict__ params, float scale, int relative, float* __restrict__ out,
+    double* __restrict__ a0, double* __restrict__ a1, double* __restrict__ a2,
+    const double* __restrict__ v0, const double* __restrict__ v1,
+    const double* __restrict__ v2, int n_proj) {
+  __shared__ __align__(16) float sb[3 * NH + NOUT];
+  for (int i = threadIdx.x; i < 3 * NH + NOUT; i += 64 * WAVES) {
+    const int off = i < NH ? OFF_B1 + i
+                  : i < 2 * NH ? OFF_B2 + i - NH
+                  : i < 3 * NH ? OFF_B3 + i - 2 * NH
+                  : OFF_B4 + i - 3 * NH;
+    const float b = params[off];
+    sb[i] = (i >= NH && i < 3 * NH) ? NEG_LOG2E * b : b;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * WAVES + (threadIdx.x >> 6);
+  const int nwaves = gridDim.x * WAVES;
+  Weights W;
+  load_weights(wbf, lane, W);
+  for (int b0 = wave * 16 * IU; b0 < nb; b0 += nwaves * 16 * IU) {
+    float o[IU][3];
+    bool valid[IU];
+    infer16<IU>(W, sb, x, nb, b0, lane, o, valid);
+    if (lane >= 16) continue;
+#pragma unroll
+    for (int u = 0; u < IU; ++u) {
+      if (!valid[u]) continue;
+      const int body = b0 + 16 * u + lane;
+      if (out) {
+        out[(size_t)body * 3 + 0] = o[u][0];
+        out[(size_t)body * 3 + 1] = o[u][1];
+        out[(size_t)body * 3 + 2] = o[u][2];
+      }
+      if (ADD) {
+        // relative: the output is in units of scale * |a| of the force it
+        // corrects (read before the update)
+        const double un = relative ? (double)scale * sqrt(a0[body] * a0[body] +
+                                                          a1[body] * a1[body] +
+                                                          a2[body] * a2[body])
+                                   : (double)scale;
+        double d0 = un * (double)o[u][0], d1 = un * (double)o[u][1],
+               d2 = un * (double)o[u][2];
+        if (body < n_proj) {
+          // no component along the velocity: the correction does no work
+          const double w0 = v0[body], w1 = v1[body], w2 = v2[body];
+          const double vv = w0 * w0 + w1 * w1 + w2 * w2;
+          if (vv > 0.0) {
+            const double c = (d0 * w0 + d1 * w1 + d2 * w2) / vv;
+            d0 -= c * w0;
+            d1 -= c * w1;
+            d2 -= c * w2;
+          }
+        }
+        a0[body] += d0;
+        a1[body] += d1;
+        a2[body] += d2;
+      }
+    }
   }
 }
 
@@ -634,6 +734,21 @@ int ssim_mlp_forward(int nb, const unsigned short* feat,
   SSIM_REQUIRE(n_proj == 0 || (ax && vx && vy && vz),
                "the projection needs the accelerations and velocities");
   hipStream_t st = ssim::as_stream(stream);
+  if (zs == nullptr) {
+    const int grid = grid_waves(ssim::ceil_div(nb, IU));  // 16 IU bodies per wave
+    if (ax)
+      SSIM_TIMED("k_mlp_forward", st,
+                 hipLaunchKernelGGL(k_mlp_infer<true>, dim3(grid), dim3(64 * WAVES), 0, st,
+                                    nb, feat, wbf, params, scale, relative, out, ax, ay,
+                                    az, vx, vy, vz, n_proj));
+    else
+      SSIM_TIMED("k_mlp_forward", st,
+                 hipLaunchKernelGGL(k_mlp_infer<false>, dim3(grid), dim3(64 * WAVES), 0,
+                                    st, nb, feat, wbf, params, scale, relative, out, ax,
+                                    ay, az, vx, vy, vz, n_proj));
+    SSIM_LAUNCHED("k_mlp_infer");
+    return SSIM_OK;
+  }
   SSIM_TIMED("k_mlp_forward", st,
              hipLaunchKernelGGL(k_mlp_forward, dim3(grid_waves(nb)),
                                 dim3(64 * WAVES), 0, st, nb, feat, wbf, params,

@@ -40,10 +40,18 @@ def test_forward_matches_oracle(dev, n):
     net = LearnedCorrection(dev, p)
     out = torch.empty((n, 3), dtype=torch.float32, device=dev)
     net.predict(to_bf16_tensor(x, dev), out=out)
-    ref, _ = M.forward(p, x, rounding=True)
+    # the rounding points of the inference kernel (activations scaled by
+    # -log2 e before their BF16 rounding)
+    ref, _ = M.forward(p, x, rounding=True, act_scale=M.INFER_ACT_SCALE)
     got = out.cpu().numpy()
     assert np.linalg.norm(got - ref) <= 5e-3 * np.linalg.norm(ref)
     assert np.median(np.abs(got - ref)) <= 1e-3 * np.abs(ref).mean()
+    # accuracy: against the f64 network on the same BF16 weights and inputs,
+    # no worse than the BF16-rounded network of the training kernels
+    exact, _ = M.forward(M.bf16(p), M.bf16(x), rounding=False)
+    bf, _ = M.forward(p, x, rounding=True)
+    floor = 1e-4 * np.linalg.norm(exact)
+    assert np.linalg.norm(got - exact) <= 1.5 * np.linalg.norm(bf - exact) + floor
 
 
 def test_forward_adds_scaled_output(dev):

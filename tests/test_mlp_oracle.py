@@ -61,3 +61,25 @@ def test_features_pick_the_nearest_node_stencil():
     assert f[0, 26] == rho[4, 5, 6] / 2.0
     np.testing.assert_allclose(f[0, 27:30], [0.6, 0.6, -0.4], atol=1e-12)
     assert f[0, 30] == 0.0 and M.features(pos, rho, origin, h, 2.0, 1)[0, 30] == 1.0
+
+
+def test_scaled_activation_rounding():
+    """act_scale only moves the BF16 rounding points: a power of two commutes
+    with the rounding (identical result), and the inference kernel's scale
+    -log2(e) stays as close to the f64 network as the unscaled rounding."""
+    rng = np.random.default_rng(21)
+    p = M.init_params(3)
+    d = M.unpack(p.copy())
+    for k in ("b1", "b2", "b3"):
+        d[k][:] = rng.normal(0, 0.3, d[k].shape)
+    p = M.pack(d)
+    x = M.bf16(rng.normal(0, 1, (500, M.NIN)).astype(np.float32))
+    base, _ = M.forward(p, x)
+    two, _ = M.forward(p, x, act_scale=2.0)
+    assert np.array_equal(base, two)
+    scaled, _ = M.forward(p, x, act_scale=M.INFER_ACT_SCALE)
+    assert not np.array_equal(base, scaled)
+    exact, _ = M.forward(M.bf16(p), x, rounding=False)
+    e_base = np.linalg.norm(base - exact)
+    e_scaled = np.linalg.norm(scaled - exact)
+    assert 0.5 * e_base < e_scaled < 1.5 * e_base
