@@ -295,6 +295,17 @@ int ssim_tsc_deposit_binned(int np, const double* x, const double* y,
 int ssim_tsc_deposit_blocked(int grid_n, const double* origin, double h, int block,
                              const int* bin_start, const void* body, int mesh_prec,
                              void* mesh, void* stream);
+/* The same deposit (OVERWRITES mesh) from ssim_split_bin's block starts
+ * (cell_start) and body, for block edge 5: one workgroup per tile of 2^3
+ * blocks sums its nodes in LDS as int64 fixed point (contributions
+ * m w_x w_y w_z 2^s, rounded to nearest), so the sums are exact and do not
+ * depend on the order of the additions: bit-reproducible.  mass_bound >=
+ * sum |m| fixes s = 62 - ceil(log2 mass_bound) (no sum can overflow; each
+ * contribution is exact to 2^-(s+1)).  Only the block order of the masses
+ * is used.  No reference counterpart. */
+int ssim_tsc_deposit_tiles(int grid_n, const double* origin, double h, int block,
+                           const int* cell_start, const void* body, double mass_bound,
+                           int mesh_prec, void* mesh, void* stream);
 /* TSC interpolation of three meshes at np points: o_c = sum w f_c
  * (overwrites o_c, FP64, or adds to it when accumulate != 0); thread i handles point order[i] (order = NULL:
  * point i) — a cell-sorted order makes neighbouring threads read
@@ -392,8 +403,11 @@ int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
  * Outputs: bin_start[nb^3 block^3 + 1], cell_start[nb^3 + 1] (the block
  * starts, for ssim_pp_short), order[n] (mass indices in bin order),
  * packed[n] float4 (x, y, z relative to o - h/2, m) for ssim_pp_short and
- * body[n] double4 ((x - o)/h per axis, m) for ssim_tsc_deposit_blocked.  It replaces
- * the two cell lists (nodes and near-field cells) of ssim_cells_build.  No
+ * body[n] double4 ((x - o)/h per axis, m) for ssim_tsc_deposit_blocked and
+ * ssim_tsc_deposit_tiles.  With bin_start = NULL only the blocks are binned
+ * (key = block index, stable order, order inside a block by index): all
+ * that ssim_pp_short and ssim_tsc_deposit_tiles read.  It replaces the two
+ * cell lists (nodes and near-field cells) of ssim_cells_build.  No
  * hipMemset inside.  No reference counterpart. */
 size_t ssim_split_bin_work_bytes(int n, int grid_n, int block);
 int ssim_split_bin(int n, const double* x, const double* y, const double* z,

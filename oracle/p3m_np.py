@@ -43,19 +43,25 @@ def cell_list(pos, origin, cell, nc):
     return start, order
 
 
-def split_bins(pos, n, origin, h, block):
+def split_bins(pos, n, origin, h, block, nodes=True):
     """The split force's mass binning (ssim_split_bin, a binning of this
     build; its inputs are the reference's body arrays): per axis the TSC
     stencil centre r = floor((x - o)/h + 1/2) clamped to [0, n - 1]; nodes
     grouped in blocks of block^3 (nb = ceil(n / block) per axis), key
     ((r0//B nb + r1//B) nb + r2//B) B^3 + ((r0%B) B + r1%B) B + r2%B; stable
-    order; bin starts over all nb^3 B^3 keys; block starts."""
+    order; bin starts over all nb^3 B^3 keys; block starts.  nodes=False:
+    the key is the block index alone (bin starts None)."""
     nb = -(-n // block)
     r = [np.clip(np.floor((np.asarray(pos[c]) - origin[c]) / h + 0.5), 0,
                  n - 1).astype(np.int64) for c in range(3)]
     blk = [rc // block for rc in r]
     loc = [rc % block for rc in r]
     b3 = block ** 3
+    if not nodes:
+        cell = (blk[0] * nb + blk[1]) * nb + blk[2]
+        order = np.argsort(cell, kind="stable")
+        cstart = np.concatenate([[0], np.cumsum(np.bincount(cell, minlength=nb ** 3))])
+        return None, cstart, order
     key = (((blk[0] * nb + blk[1]) * nb + blk[2]) * b3 +
            (loc[0] * block + loc[1]) * block + loc[2])
     order = np.argsort(key, kind="stable")

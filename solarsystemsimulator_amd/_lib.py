@@ -86,6 +86,8 @@ SIGNATURES = {
                                         _P, _P, c_int, _P, _P, c_size_t, _P]),
     "ssim_tsc_deposit_blocked": (c_int, [c_int, _P, c_double, c_int, _P, _P, c_int,
                                          _P, _P]),
+    "ssim_tsc_deposit_tiles": (c_int, [c_int, _P, c_double, c_int, _P, _P, c_double,
+                                       c_int, _P, _P]),
     "ssim_tsc_gather": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double, c_int,
                                 _P, _P, _P, _P, _P, _P, c_int, _P]),
     "ssim_tsc_gather_corrected": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double,

@@ -80,7 +80,7 @@ __global__ __launch_bounds__(CB) void k_every(int ncells, int stride,
 __global__ __launch_bounds__(CB) void k_split_key(
     int n, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, double ox, double oy, double oz, double h, int gn,
-    int B, int nb, int* __restrict__ key, int* __restrict__ order,
+    int B, int nb, int nodes, int* __restrict__ key, int* __restrict__ order,
     int* __restrict__ count) {
   const int i = blockIdx.x * CB + threadIdx.x;
   if (i >= n) return;
@@ -88,7 +88,8 @@ __global__ __launch_bounds__(CB) void k_split_key(
   const int a = (int)fmin(fmax(floor((x[i] - ox) / h + 0.5), 0.0), top);
   const int b = (int)fmin(fmax(floor((y[i] - oy) / h + 0.5), 0.0), top);
   const int c = (int)fmin(fmax(floor((z[i] - oz) / h + 0.5), 0.0), top);
-  const int k = ssim_blocked_key(a, b, c, B, nb);
+  // nodes: the node bin in its block; else the block alone
+  const int k = nodes ? ssim_blocked_key(a, b, c, B, nb) : ((a / B) * nb + b / B) * nb + c / B;
   key[i] = k;
   order[i] = i;
   atomicAdd(&count[k], 1);
@@ -267,8 +268,7 @@ int ssim_split_bin(int n, const double* x, const double* y, const double* z,
   SSIM_REQUIRE(grid_n >= 1 && block >= 1 && h > 0.0 && origin, "bad mesh");
   const long long K = split_bins(grid_n, block);
   SSIM_REQUIRE(K <= (1LL << 30), "too many bins: %lld", K);
-  SSIM_REQUIRE(bin_start && cell_start && order && packed && body &&
-                   (n == 0 || (x && y && z && m)),
+  SSIM_REQUIRE(cell_start && order && packed && body && (n == 0 || (x && y && z && m)),
                "null pointer");
   SSIM_REQUIRE(work && work_bytes >= ssim_split_bin_work_bytes(n, grid_n, block),
                "workspace too small");
@@ -276,6 +276,8 @@ int ssim_split_bin(int n, const double* x, const double* y, const double* z,
   const int nb = (grid_n + block - 1) / block;
   const int ncells = nb * nb * nb;
   const int B3 = block * block * block;
+  const bool nodes = bin_start != nullptr;  // node bins, or the blocks alone
+  const long long nbins = nodes ? K : ncells;
   unsigned char* w = static_cast<unsigned char*>(work);
   int* key = reinterpret_cast<int*>(w);
   w += ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int));
@@ -283,23 +285,27 @@ int ssim_split_bin(int n, const double* x, const double* y, const double* z,
   w += ssim::align256((size_t)K * sizeof(int));
   void* scan_work = w;
   w += ssim::scan_work_bytes((int)K);
-  hipLaunchKernelGGL(k_zero_i32, dim3(ssim::ceil_div(K, CB)), dim3(CB), 0, st, (int)K,
-                     count);
+  hipLaunchKernelGGL(k_zero_i32, dim3(ssim::ceil_div(nbins, CB)), dim3(CB), 0, st,
+                     (int)nbins, count);
   SSIM_LAUNCHED("k_zero_i32");
   if (n > 0) {
     hipLaunchKernelGGL(k_split_key, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st, n, x,
-                       y, z, origin[0], origin[1], origin[2], h, grid_n, block, nb, key,
-                       order, count);
+                       y, z, origin[0], origin[1], origin[2], h, grid_n, block, nb,
+                       nodes ? 1 : 0, key, order, count);
     SSIM_LAUNCHED("k_split_key");
     int bits = 1;
-    while (bits < 31 && (1LL << bits) < K) ++bits;
+    while (bits < 31 && (1LL << bits) < nbins) ++bits;
     SSIM_TRY(ssim::radix_sort_pairs(n, bits, key, order, w,
                                     ssim::radix_sort_work_bytes(n), st));
   }
-  SSIM_TRY(ssim::scan_exclusive((int)K, count, bin_start, scan_work, st));
-  hipLaunchKernelGGL(k_every, dim3(ssim::ceil_div(ncells + 1, CB)), dim3(CB), 0, st,
-                     ncells, B3, bin_start, cell_start);
-  SSIM_LAUNCHED("k_every");
+  if (nodes) {
+    SSIM_TRY(ssim::scan_exclusive((int)K, count, bin_start, scan_work, st));
+    hipLaunchKernelGGL(k_every, dim3(ssim::ceil_div(ncells + 1, CB)), dim3(CB), 0, st,
+                       ncells, B3, bin_start, cell_start);
+    SSIM_LAUNCHED("k_every");
+  } else {
+    SSIM_TRY(ssim::scan_exclusive(ncells, count, cell_start, scan_work, st));
+  }
   if (n > 0) {
     const double hh = 0.5 * h;
     hipLaunchKernelGGL(k_split_pack, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st, n,

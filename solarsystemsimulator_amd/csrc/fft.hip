@@ -475,7 +475,9 @@ __global__ __launch_bounds__(NL * 16) void k_conv_x3(
   for (int t = threadIdx.x; t < L; t += NL * 16) stw[t] = tw[t];
   __syncthreads();
   const int l = threadIdx.x % NL, j = threadIdx.x / NL;
-  const int line = blockIdx.x * NL + l;
+  // XCD-contiguous: the two 64-byte halves of a kernel-spectrum line are
+  // read by neighbouring workgroups, which then share one L2
+  const int line = ssim_xcd_block(blockIdx.x, gridDim.x) * NL + l;
   // element (x, line) at line + x * lines (< 2^31, checked by the host)
   const unsigned int base = (unsigned int)line + (unsigned int)j * (unsigned int)lines;
   const unsigned int step = 16u * (unsigned int)lines;

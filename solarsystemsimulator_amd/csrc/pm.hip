@@ -374,6 +374,115 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_deposit(
   }
 }
 
+// Tiled TSC deposit (ssim_tsc_deposit_tiles): one workgroup per tile of
+// DT^3 blocks (E = DT B nodes a side) accumulates its nodes in LDS as
+// int64 fixed point (contributions m w_x w_y w_z 2^s, rounded): integer
+// addition is associative, so the LDS atomics give the same sums in any
+// order and the mesh is bit-reproducible.  The masses come from the
+// (DT + 2)^3 blocks around the tile (16 contiguous runs of the block-sorted
+// order); those whose stencil reaches the tile are compacted into an LDS
+// list (wave ballots), and every listed mass adds its contributions to the
+// tile's nodes, one mass a lane.  Each node is written once, as acc 2^-s.
+constexpr int DT = 2;          // blocks per tile edge
+constexpr int DEP_T = 256;     // threads of a tile workgroup
+constexpr int DEP_LCAP = 2048; // listed masses per round
+
+template <typename M, int B>
+__global__ __launch_bounds__(DEP_T) void k_tsc_deposit_tiles(
+    const double4* __restrict__ body, const int* __restrict__ cell_start, int nb,
+    int n, double scale, double inv_scale, M* __restrict__ mesh) {
+  constexpr int E = DT * B, NN = E * E * E;
+  __shared__ unsigned long long acc[NN];
+  __shared__ int list[DEP_LCAP];
+  __shared__ int nlist;
+  const int t = threadIdx.x, lane = t & 63;
+  const int ntile = (nb + DT - 1) / DT;
+  const int tz = blockIdx.x % ntile, ty = (blockIdx.x / ntile) % ntile,
+            tx = blockIdx.x / (ntile * ntile);
+  const int lo0 = tx * E, lo1 = ty * E, lo2 = tz * E;
+  for (int q = t; q < NN; q += DEP_T) acc[q] = 0ull;
+  if (t == 0) nlist = 0;
+  __syncthreads();
+  // the 16 rows of blocks around the tile: z-runs of up to DT + 2 blocks
+  const int bz0 = max(tz * DT - 1, 0), bz1 = min(tz * DT + DT, nb - 1);
+  // one list entry: add a listed mass's contributions to the tile
+  auto add_mass = [&](int sidx) {
+    const double4 p = body[sidx];
+    const double f[3] = {p.x, p.y, p.z};
+    const int lo[3] = {lo0, lo1, lo2};
+    double w[3][3];
+    int r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double rr = floor(f[c] + 0.5);
+      const double d = f[c] - rr;
+      r[c] = (int)rr - lo[c];  // centre node, tile-relative
+      const double a = 0.5 - d, b = 0.5 + d;
+      w[c][0] = 0.5 * a * a;
+      w[c][1] = 0.75 - d * d;
+      w[c][2] = 0.5 * b * b;
+    }
+    const double ms = p.w * scale;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int i = r[0] + a - 1;
+      if (i < 0 || i >= E) continue;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const int j = r[1] + b - 1;
+        if (j < 0 || j >= E) continue;
+        const double wab = ms * w[0][a] * w[1][b];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          const int k = r[2] + e - 1;
+          if (k < 0 || k >= E) continue;
+          atomicAdd(&acc[(i * E + j) * E + k],
+                    (unsigned long long)__double2ll_rn(wab * w[2][e]));
+        }
+      }
+    }
+  };
+  for (int row = 0; row < (DT + 2) * (DT + 2); ++row) {
+    const int bx = tx * DT - 1 + row / (DT + 2), by = ty * DT - 1 + row % (DT + 2);
+    if (bx < 0 || bx >= nb || by < 0 || by >= nb) continue;  // uniform
+    const int cb = (bx * nb + by) * nb;
+    const int s0 = cell_start[cb + bz0], s1 = cell_start[cb + bz1 + 1];
+    for (int base = s0; base < s1; base += DEP_T) {
+      const int sidx = base + t;
+      bool rel = false;
+      if (sidx < s1) {
+        const double4 p = body[sidx];
+        const int r0 = (int)floor(p.x + 0.5) - lo0, r1 = (int)floor(p.y + 0.5) - lo1,
+                  r2 = (int)floor(p.z + 0.5) - lo2;
+        rel = r0 >= -1 && r0 <= E && r1 >= -1 && r1 <= E && r2 >= -1 && r2 <= E;
+      }
+      const unsigned long long mask = __ballot(rel);
+      int wbase = 0;
+      if (lane == 0 && mask) wbase = atomicAdd(&nlist, __popcll(mask));
+      wbase = __shfl(wbase, 0, 64);
+      if (rel) list[wbase + __popcll(mask & ((1ull << lane) - 1ull))] = sidx;
+      __syncthreads();
+      const int cnt = nlist;
+      __syncthreads();
+      if (cnt > DEP_LCAP - DEP_T) {  // flush before the next round could overflow
+        for (int q = t; q < cnt; q += DEP_T) add_mass(list[q]);
+        __syncthreads();
+        if (t == 0) nlist = 0;
+        __syncthreads();
+      }
+    }
+  }
+  const int cnt = nlist;
+  for (int q = t; q < cnt; q += DEP_T) add_mass(list[q]);
+  __syncthreads();
+  for (int q = t; q < NN; q += DEP_T) {
+    const int i = q / (E * E), j = (q / E) % E, k = q % E;
+    const int gi = lo0 + i, gj = lo1 + j, gk = lo2 + k;
+    if (gi < n && gj < n && gk < n)
+      mesh[((long long)gi * n + gj) * n + gk] = (M)((double)(long long)acc[q] * inv_scale);
+  }
+}
+
 // The same deposit without atomics: one thread per node sums, in a fixed
 // order, the bodies binned at the 27 nodes around it (bins = nearest node,
 // stable order inside a bin), so the mesh is bit-reproducible.  A body binned
@@ -1165,6 +1274,36 @@ int ssim_tsc_deposit_blocked(int grid_n, const double* origin, double h, int blo
 #undef LAUNCH_DEPB_B
 #undef LAUNCH_DEPB
   SSIM_LAUNCHED("k_tsc_deposit_blocked");
+  return SSIM_OK;
+}
+
+int ssim_tsc_deposit_tiles(int grid_n, const double* origin, double h, int block,
+                           const int* cell_start, const void* body, double mass_bound,
+                           int mesh_prec, void* mesh, void* stream) {
+  SSIM_REQUIRE(grid_n >= 1 && h > 0.0, "bad TSC arguments");
+  SSIM_REQUIRE(block == 5, "ssim_tsc_deposit_tiles: block edge 5 only (got %d)", block);
+  SSIM_REQUIRE(mass_bound > 0.0 && std::isfinite(mass_bound), "mass_bound must be > 0");
+  SSIM_REQUIRE(origin && cell_start && body && mesh, "null pointer");
+  const int nb = (grid_n + block - 1) / block;
+  const int ntile = (nb + DT - 1) / DT;
+  int e = 0;
+  std::frexp(mass_bound, &e);  // mass_bound <= 2^e: no sum exceeds 2^62
+  e = e < -900 ? -900 : e;     // keeps 2^(62 - e) finite
+  const double scale = std::ldexp(1.0, 62 - e), inv_scale = std::ldexp(1.0, e - 62);
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid((unsigned)(ntile * ntile * ntile));
+  const double4* bp = static_cast<const double4*>(body);
+  if (mesh_prec == SSIM_PREC_F64)
+    SSIM_TIMED("k_tsc_deposit", st,
+               hipLaunchKernelGGL((k_tsc_deposit_tiles<double, 5>), grid, dim3(DEP_T), 0,
+                                  st, bp, cell_start, nb, grid_n, scale, inv_scale,
+                                  static_cast<double*>(mesh)));
+  else
+    SSIM_TIMED("k_tsc_deposit", st,
+               hipLaunchKernelGGL((k_tsc_deposit_tiles<float, 5>), grid, dim3(DEP_T), 0,
+                                  st, bp, cell_start, nb, grid_n, scale, inv_scale,
+                                  static_cast<float*>(mesh)));
+  SSIM_LAUNCHED("k_tsc_deposit_tiles");
   return SSIM_OK;
 }
 

@@ -172,37 +172,77 @@ __global__ __launch_bounds__(RT) void k_copy_pairs(int n, const int* __restrict_
   vb[i] = va[i];
 }
 
-// Exclusive scan of in[0..n) into out[0..n], one workgroup: every thread
-// sums a contiguous chunk, a Hillis-Steele scan over the chunk sums gives
-// each chunk's base, and the chunks are written out.  out[n] = total.
-// Used directly for short arrays and for the block sums of long ones.
-__global__ __launch_bounds__(SCAN_T) void k_scan(int n, const int* __restrict__ in,
-                                                 int* __restrict__ out) {
-  __shared__ int part[SCAN_T];
-  const int t = threadIdx.x;
-  const int per = (n + SCAN_T - 1) / SCAN_T;
-  const int b0 = min(n, t * per), b1 = min(n, b0 + per);
-  int s = 0;
-  for (int i = b0; i < b1; ++i) s += in[i];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < SCAN_T; off <<= 1) {
-    const int v = t >= off ? part[t - off] : 0;
+// Exclusive scan of in[0..n) into out[0..n] (out[n] = total) in one
+// workgroup: the whole scan for n <= SCAN1_MAX, the block totals of longer
+// ones (any n; a round per 32768 values).  Rounds of S1_TILES tiles of 4 SCAN_T
+// values: every thread loads its 4 consecutive values of each tile at once
+// (coalesced across the wave, one memory latency per round), scans its
+// tile sums inside its wave by shuffles, and the first wave scans the
+// S1_TILES x 16 wave totals (tile-major); a carry joins the rounds.
+constexpr int S1_TILES = 8;
+constexpr int SCAN1_MAX = 1 << 16;
+
+__global__ __launch_bounds__(SCAN_T) void k_scan1(int n, const int* __restrict__ in,
+                                                  int* __restrict__ out) {
+  constexpr int NW = SCAN_T / 64;
+  static_assert(S1_TILES * NW == 128, "two wave totals per lane of the first wave");
+  __shared__ int wsum[S1_TILES * NW];
+  __shared__ int woff[S1_TILES * NW];
+  __shared__ int rtotal;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int carry = 0;
+  for (int base = 0; base < n; base += S1_TILES * 4 * SCAN_T) {
+    int v[S1_TILES][4], x[S1_TILES], s[S1_TILES];
+#pragma unroll
+    for (int k = 0; k < S1_TILES; ++k) {
+      const int i0 = base + k * 4 * SCAN_T + 4 * t;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[k][e] = i0 + e < n ? in[i0 + e] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < S1_TILES; ++k) {
+      s[k] = v[k][0] + v[k][1] + v[k][2] + v[k][3];
+      int y = s[k];
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int z = __shfl_up(y, off, 64);
+        if (lane >= off) y += z;
+      }
+      x[k] = y;
+      if (lane == 63) wsum[k * NW + w] = y;
+    }
     __syncthreads();
-    part[t] += v;
+    if (w == 0) {
+      const int a = wsum[2 * lane], b = wsum[2 * lane + 1];
+      int y = a + b;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int z = __shfl_up(y, off, 64);
+        if (lane >= off) y += z;
+      }
+      woff[2 * lane] = y - a - b;
+      woff[2 * lane + 1] = y - b;
+      if (lane == 63) rtotal = y;
+    }
     __syncthreads();
+#pragma unroll
+    for (int k = 0; k < S1_TILES; ++k) {
+      const int i0 = base + k * 4 * SCAN_T + 4 * t;
+      int e = carry + woff[k * NW + w] + x[k] - s[k];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (i0 + q < n) out[i0 + q] = e;
+        e += v[k][q];
+      }
+    }
+    carry += rtotal;
+    __syncthreads();  // wsum, woff and rtotal are rewritten by the next round
   }
-  int run = t == 0 ? 0 : part[t - 1];
-  for (int i = b0; i < b1; ++i) {
-    const int c = in[i];
-    out[i] = run;
-    run += c;
-  }
-  if (t == SCAN_T - 1) out[n] = part[SCAN_T - 1];
+  if (t == 0) out[n] = carry;
 }
 
 // Long arrays: blocks of SB = 4 * RT elements.  k_block_sum writes each
-// block's total, k_scan turns them into block offsets, k_block_scan scans a
+// block's total, k_scan1 turns them into block offsets, k_block_scan scans a
 // block in LDS and adds its offset.
 constexpr int SB = 4 * RT;
 
@@ -264,9 +304,9 @@ size_t scan_work_bytes(int n) {
 }
 
 int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st) {
-  if (n <= 4 * SB) {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_T), 0, st, n, in, out);
-    SSIM_LAUNCHED("k_scan");
+  if (n <= SCAN1_MAX) {
+    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(SCAN_T), 0, st, n, in, out);
+    SSIM_LAUNCHED("k_scan1");
     return SSIM_OK;
   }
   SSIM_REQUIRE(work, "scan: workspace missing");
@@ -275,8 +315,8 @@ int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st) {
   int* boff = bsum + align256(((size_t)nb + 1) * sizeof(int)) / sizeof(int);
   hipLaunchKernelGGL(k_block_sum, dim3(nb), dim3(RT), 0, st, n, in, bsum);
   SSIM_LAUNCHED("k_block_sum");
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_T), 0, st, nb, bsum, boff);
-  SSIM_LAUNCHED("k_scan");
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(SCAN_T), 0, st, nb, bsum, boff);
+  SSIM_LAUNCHED("k_scan1");
   hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(RT), 0, st, n, in, boff, nb, out);
   SSIM_LAUNCHED("k_block_scan");
   return SSIM_OK;

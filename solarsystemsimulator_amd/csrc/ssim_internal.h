@@ -41,7 +41,7 @@ double yoshida_w1();
 double yoshida_w0();
 
 // Exclusive scan of n ints into out[0..n] (out[n] = total), sort.hip;
-// `work` holds scan_work_bytes(n) (unused for n <= 4096).
+// `work` holds scan_work_bytes(n) (unused for n <= 65536: one workgroup).
 size_t scan_work_bytes(int n);
 int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st);
 

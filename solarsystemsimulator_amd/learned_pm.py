@@ -80,6 +80,7 @@ class PMLearnedSystem:
                 self.v[c][self.n:].copy_(D.to_dev(tvel[c], self.dev))
         self.m[:self.n].copy_(D.to_dev(mass, self.dev))
         self.mass_unit = float(np.mean(mass))
+        self.split.set_mass_bound(self.m[:self.n])
         self.acc_valid = False
 
     def a_unit(self) -> float:

@@ -83,6 +83,9 @@ class SplitForce:
         # near-field cells of TSC runs: blocks of block^3 mesh nodes, edge
         # block * h >= rcut = rcut_rs * rs_cells * h
         self.block = max(1, math.ceil(self.rcut_rs * self.rs_cells - 1e-9))
+        # TSC deposit from the block bins by LDS tiles (block edge 5); False:
+        # the node-bin gather (ssim_tsc_deposit_blocked)
+        self.tiled_deposit = True
 
     # -- grid ---------------------------------------------------------------
     def _place(self, box):
@@ -156,9 +159,33 @@ class SplitForce:
             if not binned:
                 self.mass_bins(x, n_mass, m)
             org = _lib.double_array(self.origin)
-            _lib.call("ssim_tsc_deposit_blocked", self.n, P._addr(org), self.h,
-                      self.block, _lib.ptr(self.bin_start), _lib.ptr(self.body),
-                      self.prec, _lib.ptr(self.rho), D.stream(self.dev))
+            if self.block == 5 and self.tiled_deposit:
+                _lib.call("ssim_tsc_deposit_tiles", self.n, P._addr(org), self.h,
+                          self.block, _lib.ptr(self.cell_start), _lib.ptr(self.body),
+                          self.mass_bound(m[:n_mass]), self.prec, _lib.ptr(self.rho),
+                          D.stream(self.dev))
+            else:
+                _lib.call("ssim_tsc_deposit_blocked", self.n, P._addr(org), self.h,
+                          self.block, _lib.ptr(self.bin_start), _lib.ptr(self.body),
+                          self.prec, _lib.ptr(self.rho), D.stream(self.dev))
+
+    def set_mass_bound(self, m):
+        """Fix the bound sum |m| of the tiled deposit's fixed point (host
+        sync: call it outside graph capture, e.g. when the masses are set)."""
+        self._mass_key = (m.data_ptr(), m.numel(), m._version)
+        self._mass_bound = float(m.abs().sum()) * (1.0 + 1e-9) + 1e-300
+
+    def mass_bound(self, m):
+        """sum |m| (slightly enlarged), cached while the mass tensor is
+        unchanged; computed here only outside graph capture."""
+        import torch
+        key = (m.data_ptr(), m.numel(), m._version)
+        if getattr(self, "_mass_key", None) != key:
+            if torch.cuda.is_current_stream_capturing():
+                raise RuntimeError("mass bound unknown inside graph capture: call "
+                                   "set_mass_bound() first")
+            self.set_mass_bound(m)
+        return self._mass_bound
 
     def mesh_fields(self, x, n_mass, m, binned=False):
         """Deposit the masses and convolve: the acceleration meshes."""
@@ -228,8 +255,10 @@ class SplitForce:
                 _lib.load().ssim_split_bin_work_bytes(nm, self.n, self.block), self.dev)
             self._bin_n = nm
         org = _lib.double_array(self.origin)
+        # node bins only for the node-bin gather deposit
+        bins = None if self.block == 5 and self.tiled_deposit else _lib.ptr(self.bin_start)
         _lib.call("ssim_split_bin", nm, *[_lib.ptr(c) for c in x], _lib.ptr(m), self.n,
-                  P._addr(org), self.h, self.block, _lib.ptr(self.bin_start),
+                  P._addr(org), self.h, self.block, bins,
                   _lib.ptr(self.cell_start), _lib.ptr(self.order),
                   _lib.ptr(self.packed), _lib.ptr(self.body), _lib.ptr(self.bwork),
                   self.bwork.numel(), D.stream(self.dev))

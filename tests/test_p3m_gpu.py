@@ -44,18 +44,20 @@ def test_cell_list_bit_exact(dev):
     np.testing.assert_array_equal(pk[:, 3], m[ro].astype(np.float32))
 
 
-@pytest.mark.parametrize("n,nc,clump", [(300_000, [40, 33, 29], 0.3),
-                                         (5, [2, 1, 3], 0.0)])
-def test_cell_list_large_and_clumped(dev, n, nc, clump):
-    """Radix passes over > 4096-entry histograms (multi-block scans), a
-    clump holding a third of the bodies in one cell, and no packed copy."""
+@pytest.mark.parametrize("n,nc,clump,inv_cell", [(300_000, [40, 33, 29], 0.3, 29),
+                                                  (300_000, [50, 41, 37], 0.0, 37),
+                                                  (5, [2, 1, 3], 0.0, 29)])
+def test_cell_list_large_and_clumped(dev, n, nc, clump, inv_cell):
+    """Radix passes over > 4096-entry histograms, cell scans in one workgroup
+    (<= 65536 cells) and over several (75850 cells), a clump holding a third
+    of the bodies in one cell, and no packed copy."""
     print(f"cell list n={n} nc={nc}", flush=True)
     rng = np.random.default_rng(4)
     pos = [rng.uniform(0.0, 1.0, n) for _ in range(3)]
     k = int(clump * n)
     for c in range(3):
         pos[c][:k] = 0.5 + rng.uniform(0.0, 1e-3, k)
-    origin, cell = [0.0, 0.0, 0.0], 1.0 / 29
+    origin, cell = [0.0, 0.0, 0.0], 1.0 / inv_cell
     ncells = int(np.prod(nc))
     start = torch.empty(ncells + 1, dtype=torch.int32, device=dev)
     order = torch.empty(n, dtype=torch.int32, device=dev)
@@ -264,6 +266,73 @@ def test_split_bins_and_blocked_deposit(dev, B):
     assert np.array_equal(meshes[0], meshes[1])
     ref = p3m_np.tsc_deposit(pos, m, n, origin, h)
     np.testing.assert_allclose(meshes[0], ref, rtol=1e-12, atol=1e-12 * ref.max())
+    if B == 5:
+        _tiled_deposit_checks(dev, n, origin, h, cstart, body, m, ref)
+    # the blocks alone (bin_start = NULL): stable block order, both copies
+    cstart2 = torch.empty_like(cstart)
+    _lib.call("ssim_split_bin", nbody, *[_lib.ptr(c) for c in x], _lib.ptr(md), n,
+              ctypes.addressof(org), h, B, None, _lib.ptr(cstart2),
+              _lib.ptr(order), _lib.ptr(packed), _lib.ptr(body), _lib.ptr(w),
+              w.numel(), D.stream(dev))
+    _, rc2, ro2 = p3m_np.split_bins(pos, n, origin, h, B, nodes=False)
+    assert np.array_equal(cstart2.cpu().numpy(), rc2) and np.array_equal(rc2, rc)
+    assert np.array_equal(order.cpu().numpy(), ro2)
+    bd = body.cpu().numpy()
+    for c in range(3):
+        assert np.array_equal(bd[:, c], (pos[c][ro2] - origin[c]) / h)
+    if B == 5:
+        _tiled_deposit_checks(dev, n, origin, h, cstart2, body, m, ref)
+
+
+def _tiled_deposit_checks(dev, n, origin, h, cstart, body, m, ref):
+    """ssim_tsc_deposit_tiles: the TSC oracle in FP64 (bit-identical run to
+    run: int64 fixed-point sums) and in FP32."""
+    import ctypes
+    org = _lib.double_array(origin)
+    bound = float(np.abs(m).sum())
+    meshes = []
+    for prec in (0, 0, 1):
+        dt = torch.float64 if prec == 0 else torch.float32
+        mt = torch.full((n, n, n), 7.0, dtype=dt, device=dev)
+        _lib.call("ssim_tsc_deposit_tiles", n, ctypes.addressof(org), h, 5,
+                  _lib.ptr(cstart), _lib.ptr(body), bound, prec, _lib.ptr(mt),
+                  D.stream(dev))
+        meshes.append(mt.cpu().numpy().astype(np.float64))
+    assert np.array_equal(meshes[0], meshes[1])
+    np.testing.assert_allclose(meshes[0], ref, rtol=1e-12, atol=1e-12 * ref.max())
+    np.testing.assert_allclose(meshes[2], ref, rtol=1e-6, atol=1e-7 * ref.max())
+
+
+def test_tiled_deposit_dense_clump(dev):
+    """ssim_tsc_deposit_tiles with 30000 of 40000 masses inside one node
+    (the LDS list of a tile fills and is flushed many times) and masses on
+    the tile edges, against the TSC oracle."""
+    import ctypes
+    print("tiled deposit clump", flush=True)
+    n, h, nbody, B = 37, 0.02, 40_000, 5
+    origin = [0.0, 0.0, 0.0]
+    rng = np.random.default_rng(31)
+    pos = [rng.uniform(1.0, n - 2.0, nbody) * h for _ in range(3)]
+    for c in range(3):
+        pos[c][:30_000] = (17.3 + rng.uniform(-0.2, 0.2, 30_000)) * h
+        pos[c][30_000:31_000] = (np.arange(1000) % 3 * 10 + 9.5) * h   # tile edges
+    m = rng.uniform(0.1, 1.0, nbody)
+    x = [D.to_dev(c, dev) for c in pos]
+    md = D.to_dev(m, dev)
+    nb = -(-n // B)
+    i32 = dict(dtype=torch.int32, device=dev)
+    cstart = torch.empty(nb ** 3 + 1, **i32)
+    order = torch.empty(nbody, **i32)
+    packed = torch.empty((nbody, 4), dtype=torch.float32, device=dev)
+    body = torch.empty((nbody, 4), dtype=torch.float64, device=dev)
+    w = D.workspace(_lib.load().ssim_split_bin_work_bytes(nbody, n, B), dev)
+    org = _lib.double_array(origin)
+    _lib.call("ssim_split_bin", nbody, *[_lib.ptr(c) for c in x], _lib.ptr(md), n,
+              ctypes.addressof(org), h, B, None, _lib.ptr(cstart),
+              _lib.ptr(order), _lib.ptr(packed), _lib.ptr(body), _lib.ptr(w),
+              w.numel(), D.stream(dev))
+    ref = p3m_np.tsc_deposit(pos, m, n, origin, h)
+    _tiled_deposit_checks(dev, n, origin, h, cstart, body, m, ref)
 
 
 @pytest.mark.parametrize("prec,tol,n", [(0, 1e-12, 32), (1, 1e-5, 32), (1, 1e-5, 128),

@@ -118,6 +118,10 @@ def test_split_bins_brute_force_and_tsc_consistency():
     sizes = [bins.get((k[:3], k[3:]), 0) for k in keys]
     assert np.array_equal(np.diff(start), sizes)
     assert np.array_equal(np.diff(cstart), np.add.reduceat(sizes, np.arange(0, len(keys), B ** 3)))
+    # the blocks alone: Python's sort of (block, index) tuples, same starts
+    none, cstart_b, order_b = S.split_bins(pos, n, origin, h, B, nodes=False)
+    assert none is None and np.array_equal(cstart_b, cstart)
+    assert [i for _, i in sorted((blk, i) for blk, _, i in recs)] == list(order_b)
     # deposit from the bins
     mesh = np.zeros((n, n, n))
     key_of = {k: q for q, k in enumerate(keys)}
