@@ -463,7 +463,7 @@ __device__ __forceinline__ void hub_exchange(cpx<F> v[16], cpx<F>* s, int l, int
 }
 
 template <typename F, int NL>
-__global__ __launch_bounds__(NL * 16) void k_conv_x3(
+__global__ __launch_bounds__(NL * 16) __attribute__((amdgpu_waves_per_eu(3))) void k_conv_x3(
     const cpx<F>* in, const F* __restrict__ kim0,  // in == out0: updated in place
     const F* __restrict__ kim1, const F* __restrict__ kim2, cpx<F>* out0,
     cpx<F>* out1, cpx<F>* out2, const cpx<F>* __restrict__ tw, int lines) {
@@ -482,8 +482,11 @@ __global__ __launch_bounds__(NL * 16) void k_conv_x3(
   const unsigned int base = (unsigned int)line + (unsigned int)j * (unsigned int)lines;
   const unsigned int step = 16u * (unsigned int)lines;
   cpx<F> v[16];
+  F km[16];  // kernel spectrum of the next component, loaded one stage early
 #pragma unroll
   for (int n2 = 0; n2 < 8; ++n2) v[n2] = in[base + n2 * step];
+#pragma unroll
+  for (int k1 = 0; k1 < 16; ++k1) km[k1] = kim0[base + k1 * step];
 #pragma unroll
   for (int n2 = 8; n2 < 16; ++n2) v[n2] = {F(0), F(0)};
   dft16(v, false);
@@ -494,12 +497,13 @@ __global__ __launch_bounds__(NL * 16) void k_conv_x3(
   for (int k = 0; k < 16; ++k) X[k] = v[k];
 #pragma unroll 1
   for (int c = 0; c < 3; ++c) {
-    const F* kim = c == 0 ? kim0 : (c == 1 ? kim1 : kim2);
     cpx<F>* out = c == 0 ? out0 : (c == 1 ? out1 : out2);
 #pragma unroll
-    for (int k1 = 0; k1 < 16; ++k1) {
-      const F m = kim[base + k1 * step];
-      v[k1] = {-X[k1].im * m, X[k1].re * m};
+    for (int k1 = 0; k1 < 16; ++k1) v[k1] = {-X[k1].im * km[k1], X[k1].re * km[k1]};
+    if (c < 2) {  // the next component's loads overlap this inverse transform
+      const F* kim = c == 0 ? kim1 : kim2;
+#pragma unroll
+      for (int k1 = 0; k1 < 16; ++k1) km[k1] = kim[base + k1 * step];
     }
     dft16(v, true);
     hub_exchange<F, NL>(v, s, l, j, stw, true);
