@@ -664,6 +664,11 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather(
 // body (after the accumulate), u = scale |s| (relative) or scale, d = u corr,
 // bodies below n_proj lose the component of d along their velocity, and
 // s + d is stored: the operations of ssim_mlp_forward's own update.
+struct F3 {
+  float x, y, z;
+};
+typedef float f3_t __attribute__((ext_vector_type(3)));
+
 struct Corr {
   const float* out;
   float scale;
@@ -672,7 +677,7 @@ struct Corr {
   int n_proj;
 };
 
-template <bool CORR>
+template <bool CORR, bool BUF>
 __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
     int np, const int* __restrict__ order, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, Grid g, const float4* __restrict__ f,
@@ -683,6 +688,27 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
   const int i_ = ssim_xcd_block(blockIdx.x, gridDim.x) * PM_BLOCK + threadIdx.x;
   if (i_ >= np) return;
   const int p = order ? order[i_] : i_;
+  // every load that does not depend on the stencil is issued first
+  // (uniform branches), so its latency hides behind the mesh loads
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  if (accumulate) {
+    a0 = o0[p];
+    a1 = o1[p];
+    a2 = o2[p];
+  }
+  float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+  double w0 = 0.0, w1 = 0.0, w2 = 0.0;
+  if (CORR) {
+    r0 = cr.out[(size_t)p * 3 + 0];
+    r1 = cr.out[(size_t)p * 3 + 1];
+    r2 = cr.out[(size_t)p * 3 + 2];
+    if (cr.n_proj > 0) {
+      const int pv = p < cr.n_proj ? p : 0;
+      w0 = cr.v0[pv];
+      w1 = cr.v1[pv];
+      w2 = cr.v2[pv];
+    }
+  }
   int c[3];
   double w[3][3];
   tsc_axis(x[p], g.o[0], g.h, c[0], w[0]);
@@ -700,15 +726,37 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
       ix[d][a] = in ? i : 0;
       w[d][a] = in ? w[d][a] : 0.0;
     }
-  float4 q[3][3][3];
+  // the three fields of a node (12 of its 16 bytes: one dwordx3 load); a
+  // mesh under 2 GiB is read with buffer loads (32-bit offsets from one
+  // resource, no 64-bit address registers, so all 27 stay in flight)
+  F3 q[3][3][3];
+  if (BUF) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)f, (short)0, (int)((long long)g.n[0] * g.n[1] * g.n[2] * 16), 0x00020000);
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int b = 0; b < 3; ++b) {
-      const long long row = ((long long)ix[0][a] * g.n[1] + ix[1][b]) * g.n[2];
+      for (int b = 0; b < 3; ++b) {
+        const unsigned row = (unsigned)((ix[0][a] * g.n[1] + ix[1][b]) * g.n[2]);
 #pragma unroll
-      for (int e = 0; e < 3; ++e) q[a][b][e] = f[row + ix[2][e]];
-    }
+        for (int e = 0; e < 3; ++e) {
+          const f3_t v = __builtin_amdgcn_raw_buffer_load_b96(
+              rs, (row + (unsigned)ix[2][e]) * 16u, 0, 0);
+          q[a][b][e] = F3{v.x, v.y, v.z};
+        }
+      }
+    __builtin_amdgcn_sched_barrier(0);  // all 27 loads issued before any use
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const long long row = ((long long)ix[0][a] * g.n[1] + ix[1][b]) * g.n[2];
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+          q[a][b][e] = *reinterpret_cast<const F3*>(&f[row + ix[2][e]]);
+      }
+  }
   double v0 = 0.0, v1 = 0.0, v2 = 0.0;
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -723,17 +771,16 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
         v2 += wt * (double)q[a][b][e].z;
       }
     }
-  double s0 = accumulate ? o0[p] + v0 : v0;
-  double s1 = accumulate ? o1[p] + v1 : v1;
-  double s2 = accumulate ? o2[p] + v2 : v2;
+  double s0 = accumulate ? a0 + v0 : v0;
+  double s1 = accumulate ? a1 + v1 : v1;
+  double s2 = accumulate ? a2 + v2 : v2;
   if (CORR) {
     const double u = cr.relative ? (double)cr.scale * sqrt(s0 * s0 + s1 * s1 + s2 * s2)
                                  : (double)cr.scale;
-    double d0 = u * (double)cr.out[(size_t)p * 3 + 0];
-    double d1 = u * (double)cr.out[(size_t)p * 3 + 1];
-    double d2 = u * (double)cr.out[(size_t)p * 3 + 2];
+    double d0 = u * (double)r0;
+    double d1 = u * (double)r1;
+    double d2 = u * (double)r2;
     if (p < cr.n_proj) {
-      const double w0 = cr.v0[p], w1 = cr.v1[p], w2 = cr.v2[p];
       const double vv = w0 * w0 + w1 * w1 + w2 * w2;
       if (vv > 0.0) {
         const double cc = (d0 * w0 + d1 * w1 + d2 * w2) / vv;
@@ -1307,6 +1354,11 @@ int ssim_tsc_deposit_tiles(int grid_n, const double* origin, double h, int block
   return SSIM_OK;
 }
 
+// float4 meshes under 2 GiB are gathered with buffer loads
+static bool buffer_mesh(int grid_n) {
+  return (long long)grid_n * grid_n * grid_n * 16 <= 0x7fffffffLL;
+}
+
 int ssim_tsc_gather(int np, const int* order, const double* x, const double* y,
                     const double* z, int grid_n, const double* origin, double h,
                     int mesh_prec, const void* f0, const void* f1, const void* f2,
@@ -1323,11 +1375,20 @@ int ssim_tsc_gather(int np, const int* order, const double* x, const double* y,
     const int ns[3] = {grid_n, grid_n, grid_n};
     const Grid g = make_grid(3, ns, origin, h, 0);
     hipStream_t st = ssim::as_stream(stream);
-    SSIM_TIMED("k_tsc_gather", st,
-               hipLaunchKernelGGL(k_tsc_gather4<false>, dim3(ssim::ceil_div(np, PM_BLOCK)),
-                                  dim3(PM_BLOCK), 0, st, np, order, x, y, z, g,
-                                  static_cast<const float4*>(f0), o0, o1, o2,
-                                  accumulate, Corr{}));
+    if (buffer_mesh(grid_n))
+      SSIM_TIMED("k_tsc_gather", st,
+                 hipLaunchKernelGGL((k_tsc_gather4<false, true>),
+                                    dim3(ssim::ceil_div(np, PM_BLOCK)), dim3(PM_BLOCK), 0,
+                                    st, np, order, x, y, z, g,
+                                    static_cast<const float4*>(f0), o0, o1, o2,
+                                    accumulate, Corr{}));
+    else
+      SSIM_TIMED("k_tsc_gather", st,
+                 hipLaunchKernelGGL((k_tsc_gather4<false, false>),
+                                    dim3(ssim::ceil_div(np, PM_BLOCK)), dim3(PM_BLOCK), 0,
+                                    st, np, order, x, y, z, g,
+                                    static_cast<const float4*>(f0), o0, o1, o2,
+                                    accumulate, Corr{}));
     SSIM_LAUNCHED("k_tsc_gather4");
     return SSIM_OK;
   }
@@ -1368,11 +1429,18 @@ int ssim_tsc_gather_corrected(int np, const int* order, const double* x,
   const Grid g = make_grid(3, ns, origin, h, 0);
   hipStream_t st = ssim::as_stream(stream);
   const Corr cr{corr, scale, relative, v0, v1, v2, n_proj};
-  SSIM_TIMED("k_tsc_gather", st,
-             hipLaunchKernelGGL(k_tsc_gather4<true>, dim3(ssim::ceil_div(np, PM_BLOCK)),
-                                dim3(PM_BLOCK), 0, st, np, order, x, y, z, g,
-                                static_cast<const float4*>(f4), o0, o1, o2, accumulate,
-                                cr));
+  if (buffer_mesh(grid_n))
+    SSIM_TIMED("k_tsc_gather", st,
+               hipLaunchKernelGGL((k_tsc_gather4<true, true>),
+                                  dim3(ssim::ceil_div(np, PM_BLOCK)), dim3(PM_BLOCK), 0, st,
+                                  np, order, x, y, z, g, static_cast<const float4*>(f4), o0,
+                                  o1, o2, accumulate, cr));
+  else
+    SSIM_TIMED("k_tsc_gather", st,
+               hipLaunchKernelGGL((k_tsc_gather4<true, false>),
+                                  dim3(ssim::ceil_div(np, PM_BLOCK)), dim3(PM_BLOCK), 0, st,
+                                  np, order, x, y, z, g, static_cast<const float4*>(f4), o0,
+                                  o1, o2, accumulate, cr));
   SSIM_LAUNCHED("k_tsc_gather4");
   return SSIM_OK;
 }
