@@ -121,8 +121,8 @@ def conv_bytes(n, mesh_bytes=4):
 
 # rocprofv3 kernel names of the bench's timed kernels (PMC passes)
 PMC_KERNELS = {"k_pp_cells": "k_pp_cells<false>", "k_fft_pass": "k_line256",
-               "k_conv_x3": "k_conv_x3", "k_tsc_deposit": "k_tsc_deposit_blocked",
-               "k_tsc_gather": "k_tsc_gather4<true>", "k_mlp_forward": "k_mlp_forward",
+               "k_conv_x3": "k_conv_x3", "k_tsc_deposit": "k_tsc_deposit_tiles",
+               "k_tsc_gather": "k_tsc_gather4<true", "k_mlp_forward": "k_mlp_infer",
                "k_mlp_features": "k_mlp_features"}
 
 
@@ -134,8 +134,9 @@ def pmc_summary(fetch_dir, write_dir, out):
     import csv
     rows = {k: list(csv.DictReader(open(Path(d) / "run_counter_collection.csv")))
             for k, d in (("fetch", fetch_dir), ("write", write_dir))}
+    # the probe's copy is its first elementwise kernel after the fills
     copy = {k: [r for r in v if "elementwise" in r["Kernel_Name"]
-                and "Fill" not in r["Kernel_Name"]][-1] for k, v in rows.items()}
+                and "Fill" not in r["Kernel_Name"]][0] for k, v in rows.items()}
     copy_kib = 256 * 1024
     factor = {k: copy_kib / float(copy[k]["Counter_Value"]) for k in rows}
     n_mesh = 128
