@@ -52,7 +52,7 @@ __global__ __launch_bounds__(CB) void k_cell_key8(
     int n, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, double ox, double oy, double oz,
     double inv_cell, int nc0, int nc1, int nc2, int* __restrict__ key,
-    int* __restrict__ count) {
+    int* __restrict__ count, int* __restrict__ rank) {
   const int i = blockIdx.x * CB + threadIdx.x;
   if (i >= n) return;
   const double fa = (x[i] - ox) * inv_cell, fb = (y[i] - oy) * inv_cell,
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(CB) void k_cell_key8(
   const int k = ((a * nc1 + b) * nc2 + c) * 8 + octant_bit(fa, a) * 4 +
                 octant_bit(fb, b) * 2 + octant_bit(fc, c);
   key[i] = k;
-  atomicAdd(&count[k], 1);
+  rank[i] = atomicAdd(&count[k], 1);  // the body's slot in its bin
 }
 
 // cell_start[c] = sub_start[stride c], c = 0..ncells
@@ -114,15 +114,14 @@ __global__ __launch_bounds__(CB) void k_split_pack(
   body[s] = make_double4((px - ox) / h, (py - oy) / h, (pz - oz) / h, pm);
 }
 
-// Unstable placement: a cursor per cell hands out the slots.
-__global__ __launch_bounds__(CB) void k_scatter_any(int n, const int* __restrict__ key,
-                                                    const int* __restrict__ start,
-                                                    int* __restrict__ cursor,
-                                                    int* __restrict__ order) {
+// order[start[k] + rank] = i: the slots taken while counting
+__global__ __launch_bounds__(CB) void k_scatter_ranked(int n, const int* __restrict__ key,
+                                                       const int* __restrict__ rank,
+                                                       const int* __restrict__ start,
+                                                       int* __restrict__ order) {
   const int i = blockIdx.x * CB + threadIdx.x;
   if (i >= n) return;
-  const int k = key[i];
-  order[start[k] + atomicAdd(&cursor[k], 1)] = i;
+  order[start[key[i]] + rank[i]] = i;
 }
 
 __global__ __launch_bounds__(CB) void k_pack(
@@ -195,8 +194,8 @@ int ssim_cells_build(int n, const double* x, const double* y, const double* z,
 
 size_t ssim_cells_bin_work_bytes(int n, int ncells) {
   const size_t nsub = 8 * (size_t)(ncells > 0 ? ncells : 1);
-  return ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int)) +
-         2 * ssim::align256(nsub * sizeof(int)) +
+  return 2 * ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int)) +
+         ssim::align256(nsub * sizeof(int)) +
          ssim::align256((nsub + 1) * sizeof(int)) + ssim::scan_work_bytes((int)nsub);
 }
 
@@ -220,18 +219,17 @@ int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
   w += ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int));
   int* count = reinterpret_cast<int*>(w);
   w += ssim::align256((size_t)nsub * sizeof(int));
-  int* cursor = reinterpret_cast<int*>(w);
-  w += ssim::align256((size_t)nsub * sizeof(int));
+  int* rank = reinterpret_cast<int*>(w);
+  w += ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int));
   int* sub_start = reinterpret_cast<int*>(w);
   w += ssim::align256(((size_t)nsub + 1) * sizeof(int));
   const dim3 gc(ssim::ceil_div(nsub, CB));
   hipLaunchKernelGGL(k_zero_i32, gc, dim3(CB), 0, st, nsub, count);
-  hipLaunchKernelGGL(k_zero_i32, gc, dim3(CB), 0, st, nsub, cursor);
   SSIM_LAUNCHED("k_zero_i32");
   if (n > 0) {
     hipLaunchKernelGGL(k_cell_key8, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
                        n, x, y, z, origin[0], origin[1], origin[2], 1.0 / cell,
-                       nc[0], nc[1], nc[2], key, count);
+                       nc[0], nc[1], nc[2], key, count, rank);
     SSIM_LAUNCHED("k_cell_key8");
   }
   SSIM_TRY(ssim::scan_exclusive(nsub, count, sub_start, w, st));
@@ -239,9 +237,9 @@ int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
                      ncells, 8, sub_start, cell_start);
   SSIM_LAUNCHED("k_every");
   if (n > 0) {
-    hipLaunchKernelGGL(k_scatter_any, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
-                       n, key, sub_start, cursor, order);
-    SSIM_LAUNCHED("k_scatter_any");
+    hipLaunchKernelGGL(k_scatter_ranked, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
+                       n, key, rank, sub_start, order);
+    SSIM_LAUNCHED("k_scatter_ranked");
   }
   return SSIM_OK;
 }

@@ -95,6 +95,9 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
     const float2* __restrict__ gtab, float tab_scale, float rcut2, double G,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
     int accumulate, unsigned long long* __restrict__ counts) {
+  // the grid is sized for the most chunks possible: a workgroup past the
+  // last chunk leaves before it loads the table (uniform per workgroup)
+  if (blockIdx.x * WAVES >= cstart[ncells]) return;
   __shared__ float2 tab[GT + 1];
   for (int i = threadIdx.x; i <= GT; i += PB) tab[i] = gtab[i];
   __syncthreads();  // before any wave leaves
