@@ -240,10 +240,11 @@ int ssim_box_after_kdk(int dim, int n, const double* x, const double* y,
                        void* work, size_t work_bytes, void* stream);
 /* The closing half kick of a step, v += a half (as ssim_kick), fused with
  * ssim_box_after_kdk for the next step (out = the box of x + (v + a half)
- * full with the kicked v), in one launch.  The first 4 bytes of work (size
- * ssim_kick_box_work_bytes(n)) must be zero before the first call (a
- * zero-filled buffer); every call leaves them zero.  Calls sharing a work
- * buffer must be ordered.  No reference counterpart. */
+ * full with the kicked v): the kick kernel writes per-workgroup extrema to
+ * work (ssim_kick_box_work_bytes(n) bytes, no initial contents needed; the
+ * first 256 bytes are left untouched) and a one-workgroup launch folds them
+ * into out.  Calls sharing a work buffer must be ordered.  No reference
+ * counterpart. */
 size_t ssim_kick_box_work_bytes(int n);
 int ssim_kick_box(int dim, int n, const double* x, const double* y, const double* z,
                   double* vx, double* vy, double* vz, const double* ax,

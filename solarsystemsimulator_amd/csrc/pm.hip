@@ -207,99 +207,93 @@ __global__ __launch_bounds__(PM_BLOCK) void k_box_kdk_partial(
     }
 }
 
-// ssim_kick_box: eight bodies per thread (the minmax partial pass's grid):
-// more workgroups lengthen the last workgroup's fold more than they gain
-constexpr int KB_ITEMS = 8;
+// ssim_kick_box: bodies per thread, all three components of each loaded at
+// once (the box partial pass uses the same grid)
+#ifndef SSIM_KB_ITEMS
+#define SSIM_KB_ITEMS 4
+#endif
+constexpr int KB_ITEMS = SSIM_KB_ITEMS;
 
 // The closing half kick of a step (v += a half, stored) fused with the box
 // of the positions the next step's kick + drift will produce (as
-// k_box_kdk_partial), in one launch: every workgroup publishes its extrema,
-// and the last one to finish (a counter in work, reset by that workgroup)
-// folds them into out.
+// k_box_kdk_partial): every workgroup writes its extrema (wave shuffles,
+// then the workgroup's waves) to part, which k_minmax_final folds into out.
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
 __global__ __launch_bounds__(PM_BLOCK) void k_kick_box(
     int n, int na, const double* __restrict__ x0, const double* __restrict__ x1,
     const double* __restrict__ x2, double* __restrict__ v0, double* __restrict__ v1,
     double* __restrict__ v2, const double* __restrict__ a0,
     const double* __restrict__ a1, const double* __restrict__ a2, double half,
-    double full, double* part, unsigned int* counter, double* __restrict__ out) {
-  __shared__ double smin[3][PM_BLOCK], smax[3][PM_BLOCK];
-  __shared__ int last;
-  const double* xs[3] = {x0, x1, x2};
-  double* vs[3] = {v0, v1, v2};
-  const double* as[3] = {a0, a1, a2};
-  // a tile of KB_ITEMS bodies per thread (stride PM_BLOCK), all loads of a
-  // component issued before the arithmetic
+    double full, double* __restrict__ part) {
+  constexpr int NW = PM_BLOCK / 64;
+  __shared__ double wmin[3][NW], wmax[3][NW];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // a tile of KB_ITEMS bodies per thread (stride PM_BLOCK): every load of
+  // every component is issued before the arithmetic and the stores
   const int base = blockIdx.x * PM_BLOCK * KB_ITEMS + threadIdx.x;
-  for (int c = 0; c < na; ++c) {
-    double lo = INFINITY, hi = -INFINITY;
-    double av[KB_ITEMS], vv[KB_ITEMS], xv[KB_ITEMS];
+  double av[3][KB_ITEMS], vv[3][KB_ITEMS], xv[3][KB_ITEMS];
 #pragma unroll
-    for (int q = 0; q < KB_ITEMS; ++q) {
-      const int i = min(base + q * PM_BLOCK, n - 1);
-      av[q] = as[c][i];
-      vv[q] = vs[c][i];
-      xv[q] = xs[c][i];
+  for (int q = 0; q < KB_ITEMS; ++q) {
+    const int i = min(base + q * PM_BLOCK, n - 1);
+    av[0][q] = a0[i];
+    vv[0][q] = v0[i];
+    xv[0][q] = x0[i];
+    if (na > 1) {
+      av[1][q] = a1[i];
+      vv[1][q] = v1[i];
+      xv[1][q] = x1[i];
     }
+    if (na > 2) {
+      av[2][q] = a2[i];
+      vv[2][q] = v2[i];
+      xv[2][q] = x2[i];
+    }
+  }
+  double* vs[3] = {v0, v1, v2};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (c >= na) break;
+    double lo = INFINITY, hi = -INFINITY;
 #pragma unroll
     for (int q = 0; q < KB_ITEMS; ++q) {
       const int i = base + q * PM_BLOCK;
       if (i < n) {
-        const double v = vv[q] + av[q] * half;  // this step's closing kick
+        const double v = vv[c][q] + av[c][q] * half;  // this step's closing kick
         vs[c][i] = v;
-        const double nv = v + av[q] * half;     // the next step's opening kick
-        const double p = xv[q] + nv * full;     // and its drift
+        const double nv = v + av[c][q] * half;        // the next step's opening kick
+        const double p = xv[c][q] + nv * full;        // and its drift
         lo = fmin(lo, p);
         hi = fmax(hi, p);
       }
     }
-    smin[c][threadIdx.x] = lo;
-    smax[c][threadIdx.x] = hi;
+    lo = wave_min(lo);
+    hi = wave_max(hi);
+    if (lane == 0) {
+      wmin[c][wv] = lo;
+      wmax[c][wv] = hi;
+    }
   }
   __syncthreads();
-  for (int w = PM_BLOCK / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
-      for (int c = 0; c < na; ++c) {
-        smin[c][threadIdx.x] = fmin(smin[c][threadIdx.x], smin[c][threadIdx.x + w]);
-        smax[c][threadIdx.x] = fmax(smax[c][threadIdx.x], smax[c][threadIdx.x + w]);
-      }
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
     for (int c = 0; c < na; ++c) {
-      part[(size_t)blockIdx.x * 6 + 2 * c] = smin[c][0];
-      part[(size_t)blockIdx.x * 6 + 2 * c + 1] = smax[c][0];
-    }
-    const unsigned int done = __hip_atomic_fetch_add(
-        counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = done == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();  // acquire: every workgroup's extrema are visible here
-  for (int c = 0; c < na; ++c) {
-    double lo = INFINITY, hi = -INFINITY;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += PM_BLOCK) {
-      lo = fmin(lo, part[(size_t)b * 6 + 2 * c]);
-      hi = fmax(hi, part[(size_t)b * 6 + 2 * c + 1]);
-    }
-    smin[c][threadIdx.x] = lo;
-    smax[c][threadIdx.x] = hi;
-  }
-  __syncthreads();
-  for (int w = PM_BLOCK / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w)
-      for (int c = 0; c < na; ++c) {
-        smin[c][threadIdx.x] = fmin(smin[c][threadIdx.x], smin[c][threadIdx.x + w]);
-        smax[c][threadIdx.x] = fmax(smax[c][threadIdx.x], smax[c][threadIdx.x + w]);
+      double lo = wmin[c][0], hi = wmax[c][0];
+      for (int w = 1; w < NW; ++w) {
+        lo = fmin(lo, wmin[c][w]);
+        hi = fmax(hi, wmax[c][w]);
       }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    for (int c = 0; c < na; ++c) {
-      out[2 * c] = smin[c][0];
-      out[2 * c + 1] = smax[c][0];
+      part[(size_t)blockIdx.x * 6 + 2 * c] = lo;
+      part[(size_t)blockIdx.x * 6 + 2 * c + 1] = hi;
     }
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -1160,11 +1154,15 @@ int ssim_kick_box(int dim, int n, const double* x, const double* y, const double
   SSIM_REQUIRE(work_bytes >= ssim_kick_box_work_bytes(n), "workspace too small");
   hipStream_t st = ssim::as_stream(stream);
   const int nblk = ssim::ceil_div(n, PM_BLOCK * KB_ITEMS);
-  unsigned int* counter = static_cast<unsigned int*>(work);
+  // the workgroups' extrema are folded by a second launch: a completion
+  // counter in the first would need an agent-scope release per workgroup,
+  // i.e. an L2 write-back each
   double* part = reinterpret_cast<double*>(static_cast<unsigned char*>(work) + 256);
   hipLaunchKernelGGL(k_kick_box, dim3(nblk), dim3(PM_BLOCK), 0, st, n, dim, x, y, z, vx,
-                     vy, vz, ax, ay, az, half, full, part, counter, out);
+                     vy, vz, ax, ay, az, half, full, part);
   SSIM_LAUNCHED("k_kick_box");
+  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(PM_BLOCK), 0, st, nblk, dim, part, out);
+  SSIM_LAUNCHED("k_minmax_final");
   return SSIM_OK;
 }
 

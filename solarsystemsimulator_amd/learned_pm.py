@@ -157,9 +157,7 @@ class PMLearnedSystem:
             self._box_host = torch.empty(6, dtype=torch.float64, pin_memory=True)
             lib = _lib.load()
             self._box_ws = D.workspace(lib.ssim_minmax_work_bytes(nb), self.dev)
-            # ssim_kick_box's completion counter starts at zero
-            self._kb_ws = torch.zeros(lib.ssim_kick_box_work_bytes(nb), dtype=torch.uint8,
-                                      device=self.dev)
+            self._kb_ws = D.workspace(lib.ssim_kick_box_work_bytes(nb), self.dev)
             self._box_ev = torch.cuda.Event()
 
     def _next_box(self, half, full):

@@ -43,8 +43,8 @@ def test_box_after_kdk_is_exact(dev):
 
 def test_fused_kicks_match_separate_launches(dev):
     """ssim_kick_drift = ssim_kick + ssim_drift and ssim_kick_box = ssim_kick
-    + ssim_box_after_kdk, bit for bit, on two calls in a row (the fused box
-    resets its completion counter)."""
+    + ssim_box_after_kdk, bit for bit, on two calls in a row with a work
+    buffer that starts as garbage (no initial contents needed)."""
     print("fused kicks", flush=True)
     rng = np.random.default_rng(4)
     n = 70_001
@@ -64,8 +64,8 @@ def test_fused_kicks_match_separate_launches(dev):
         for c in range(3):
             assert torch.equal(xa[c], xb[c]) and torch.equal(va[c], vb[c])
         if call == 0:
-            kw = torch.zeros(lib.ssim_kick_box_work_bytes(n), dtype=torch.uint8,
-                             device=dev)
+            kw = torch.full((lib.ssim_kick_box_work_bytes(n),), 0xAB, dtype=torch.uint8,
+                            device=dev)
         out_a = torch.empty(6, dtype=torch.float64, device=dev)
         out_b = torch.empty(6, dtype=torch.float64, device=dev)
         _lib.call("ssim_kick_box", 3, n, *_ptrs(xa), *_ptrs(va), *_ptrs(aa), half, full,
@@ -77,7 +77,6 @@ def test_fused_kicks_match_separate_launches(dev):
         for c in range(3):
             assert torch.equal(va[c], vb[c])
         assert torch.equal(out_a, out_b)
-        assert int(kw[:4].sum()) == 0
 
 
 def _system(dev, pos, m, tpos, vel, tvel):
