@@ -66,14 +66,6 @@ __global__ __launch_bounds__(CB) void k_cell_key8(
   rank[i] = atomicAdd(&count[k], 1);  // the body's slot in its bin
 }
 
-// cell_start[c] = sub_start[stride c], c = 0..ncells
-__global__ __launch_bounds__(CB) void k_every(int ncells, int stride,
-                                              const int* __restrict__ sub,
-                                              int* __restrict__ cell_start) {
-  const int c = blockIdx.x * CB + threadIdx.x;
-  if (c <= ncells) cell_start[c] = sub[(long long)stride * c];
-}
-
 // Split-force mass bins: the TSC stencil centre r = floor((x - o)/h + 1/2)
 // of each axis (the operations of the deposit's weights, so a body's bin is
 // always its stencil's centre), clamped to the mesh, as a blocked key.
@@ -232,10 +224,7 @@ int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
                        nc[0], nc[1], nc[2], key, count, rank);
     SSIM_LAUNCHED("k_cell_key8");
   }
-  SSIM_TRY(ssim::scan_exclusive(nsub, count, sub_start, w, st));
-  hipLaunchKernelGGL(k_every, dim3(ssim::ceil_div(ncells + 1, CB)), dim3(CB), 0, st,
-                     ncells, 8, sub_start, cell_start);
-  SSIM_LAUNCHED("k_every");
+  SSIM_TRY(ssim::scan_exclusive(nsub, count, sub_start, w, st, 8, cell_start));
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter_ranked, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
                        n, key, rank, sub_start, order);
@@ -297,10 +286,7 @@ int ssim_split_bin(int n, const double* x, const double* y, const double* z,
                                     ssim::radix_sort_work_bytes(n), st));
   }
   if (nodes) {
-    SSIM_TRY(ssim::scan_exclusive((int)K, count, bin_start, scan_work, st));
-    hipLaunchKernelGGL(k_every, dim3(ssim::ceil_div(ncells + 1, CB)), dim3(CB), 0, st,
-                       ncells, B3, bin_start, cell_start);
-    SSIM_LAUNCHED("k_every");
+    SSIM_TRY(ssim::scan_exclusive((int)K, count, bin_start, scan_work, st, B3, cell_start));
   } else {
     SSIM_TRY(ssim::scan_exclusive(ncells, count, cell_start, scan_work, st));
   }

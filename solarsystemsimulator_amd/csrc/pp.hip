@@ -49,6 +49,78 @@ __global__ __launch_bounds__(PB) void k_gtab(double rs, double rcut,
   tab[i] = make_float2((float)g0, (float)(ssim_split_g(r1, rs, rcut) - g0));
 }
 
+// The chunk plan in one workgroup (ncells <= PLAN_MAX): chunk counts
+// ceil(targets / CHUNK) per cell, their exclusive scan (cstart, cstart[ncells]
+// = total), and the chunk -> cell map, in rounds of 8 tiles of 4 x 1024 cells
+// (every tile's loads at once; wave shuffles; a carry per round).
+constexpr int PLAN_T = 1024, PLAN_TILES = 8, PLAN_MAX = 1 << 16;
+
+__global__ __launch_bounds__(PLAN_T) void k_chunk_plan(int ncells,
+                                                      const int* __restrict__ tstart,
+                                                      int* __restrict__ cstart,
+                                                      int* __restrict__ map) {
+  constexpr int NW = PLAN_T / 64;
+  __shared__ int wsum[PLAN_TILES * NW];
+  __shared__ int woff[PLAN_TILES * NW];
+  __shared__ int rtotal;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int carry = 0;
+  for (int base = 0; base < ncells; base += PLAN_TILES * 4 * PLAN_T) {
+    int v[PLAN_TILES][4], x[PLAN_TILES], s[PLAN_TILES];
+#pragma unroll
+    for (int k = 0; k < PLAN_TILES; ++k) {
+      const int i0 = base + k * 4 * PLAN_T + 4 * t;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        v[k][e] = i0 + e < ncells
+                      ? (tstart[i0 + e + 1] - tstart[i0 + e] + CHUNK - 1) / CHUNK
+                      : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < PLAN_TILES; ++k) {
+      s[k] = v[k][0] + v[k][1] + v[k][2] + v[k][3];
+      int y = s[k];
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int z = __shfl_up(y, off, 64);
+        if (lane >= off) y += z;
+      }
+      x[k] = y;
+      if (lane == 63) wsum[k * NW + w] = y;
+    }
+    __syncthreads();
+    if (w == 0) {
+      const int a = wsum[2 * lane], b = wsum[2 * lane + 1];
+      int y = a + b;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int z = __shfl_up(y, off, 64);
+        if (lane >= off) y += z;
+      }
+      woff[2 * lane] = y - a - b;
+      woff[2 * lane + 1] = y - b;
+      if (lane == 63) rtotal = y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PLAN_TILES; ++k) {
+      const int i0 = base + k * 4 * PLAN_T + 4 * t;
+      int e = carry + woff[k * NW + w] + x[k] - s[k];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (i0 + q < ncells) {
+          cstart[i0 + q] = e;
+          for (int c = 0; c < v[k][q]; ++c) map[e + c] = i0 + q;
+        }
+        e += v[k][q];
+      }
+    }
+    carry += rtotal;
+    __syncthreads();  // wsum, woff and rtotal are rewritten by the next round
+  }
+  if (t == 0) cstart[ncells] = carry;
+}
+
 __global__ __launch_bounds__(PB) void k_chunk_count(int ncells,
                                                     const int* __restrict__ tstart,
                                                     int* __restrict__ nchunk) {
@@ -239,12 +311,18 @@ int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
   hipLaunchKernelGGL(k_gtab, dim3(ssim::ceil_div(GT + 1, PB)), dim3(PB), 0, st, rs,
                      rcut, gtab);
   SSIM_LAUNCHED("k_gtab");
-  const dim3 gc(ssim::ceil_div(ncells, PB));
-  hipLaunchKernelGGL(k_chunk_count, gc, dim3(PB), 0, st, ncells, tcell_start, nchunk);
-  SSIM_LAUNCHED("k_chunk_count");
-  SSIM_TRY(ssim::scan_exclusive(ncells, nchunk, cstart, w, st));
-  hipLaunchKernelGGL(k_chunk_map, gc, dim3(PB), 0, st, ncells, cstart, cmap);
-  SSIM_LAUNCHED("k_chunk_map");
+  if (ncells <= PLAN_MAX) {
+    hipLaunchKernelGGL(k_chunk_plan, dim3(1), dim3(PLAN_T), 0, st, ncells, tcell_start,
+                       cstart, cmap);
+    SSIM_LAUNCHED("k_chunk_plan");
+  } else {
+    const dim3 gc(ssim::ceil_div(ncells, PB));
+    hipLaunchKernelGGL(k_chunk_count, gc, dim3(PB), 0, st, ncells, tcell_start, nchunk);
+    SSIM_LAUNCHED("k_chunk_count");
+    SSIM_TRY(ssim::scan_exclusive(ncells, nchunk, cstart, w, st));
+    hipLaunchKernelGGL(k_chunk_map, gc, dim3(PB), 0, st, ncells, cstart, cmap);
+    SSIM_LAUNCHED("k_chunk_map");
+  }
   // chunks <= nt / 64 + ncells: launch that bound, surplus waves exit
   const dim3 grid(ssim::ceil_div((long long)max_chunks, WAVES));
   if (counts)

@@ -183,7 +183,8 @@ constexpr int S1_TILES = 8;
 constexpr int SCAN1_MAX = 1 << 16;
 
 __global__ __launch_bounds__(SCAN_T) void k_scan1(int n, const int* __restrict__ in,
-                                                  int* __restrict__ out) {
+                                                  int* __restrict__ out, int every,
+                                                  int* __restrict__ sel) {
   constexpr int NW = SCAN_T / 64;
   static_assert(S1_TILES * NW == 128, "two wave totals per lane of the first wave");
   __shared__ int wsum[S1_TILES * NW];
@@ -231,14 +232,20 @@ __global__ __launch_bounds__(SCAN_T) void k_scan1(int n, const int* __restrict__
       int e = carry + woff[k * NW + w] + x[k] - s[k];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        if (i0 + q < n) out[i0 + q] = e;
+        if (i0 + q < n) {
+          out[i0 + q] = e;
+          if (every && (i0 + q) % every == 0) sel[(i0 + q) / every] = e;
+        }
         e += v[k][q];
       }
     }
     carry += rtotal;
     __syncthreads();  // wsum, woff and rtotal are rewritten by the next round
   }
-  if (t == 0) out[n] = carry;
+  if (t == 0) {
+    out[n] = carry;
+    if (every) sel[n / every] = carry;
+  }
 }
 
 // Long arrays: blocks of SB = 4 * RT elements.  k_block_sum writes each
@@ -270,7 +277,8 @@ __global__ __launch_bounds__(RT) void k_block_sum(int n, const int* __restrict__
 
 __global__ __launch_bounds__(RT) void k_block_scan(int n, const int* __restrict__ in,
                                                    const int* __restrict__ boff,
-                                                   int nblocks, int* __restrict__ out) {
+                                                   int nblocks, int* __restrict__ out,
+                                                   int every, int* __restrict__ sel) {
   __shared__ int part[RT];
   const int t = threadIdx.x;
   const int base = blockIdx.x * SB + 4 * t;
@@ -289,10 +297,16 @@ __global__ __launch_bounds__(RT) void k_block_scan(int n, const int* __restrict_
   }
   int run = boff[blockIdx.x] + (t == 0 ? 0 : part[t - 1]);
   for (int k = 0; k < 4; ++k) {
-    if (base + k < n) out[base + k] = run;
+    if (base + k < n) {
+      out[base + k] = run;
+      if (every && (base + k) % every == 0) sel[(base + k) / every] = run;
+    }
     run += v[k];
   }
-  if (blockIdx.x == 0 && t == 0) out[n] = boff[nblocks];
+  if (blockIdx.x == 0 && t == 0) {
+    out[n] = boff[nblocks];
+    if (every) sel[n / every] = boff[nblocks];
+  }
 }
 
 }  // namespace
@@ -303,9 +317,12 @@ size_t scan_work_bytes(int n) {
   return 2 * align256(((size_t)ceil_div(n > 0 ? n : 1, SB) + 1) * sizeof(int));
 }
 
-int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st) {
+int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st, int every,
+                   int* sel) {
+  SSIM_REQUIRE(every == 0 || (every > 0 && n % every == 0 && sel),
+               "scan: every must divide n");
   if (n <= SCAN1_MAX) {
-    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(SCAN_T), 0, st, n, in, out);
+    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(SCAN_T), 0, st, n, in, out, every, sel);
     SSIM_LAUNCHED("k_scan1");
     return SSIM_OK;
   }
@@ -315,9 +332,10 @@ int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st) {
   int* boff = bsum + align256(((size_t)nb + 1) * sizeof(int)) / sizeof(int);
   hipLaunchKernelGGL(k_block_sum, dim3(nb), dim3(RT), 0, st, n, in, bsum);
   SSIM_LAUNCHED("k_block_sum");
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(SCAN_T), 0, st, nb, bsum, boff);
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(SCAN_T), 0, st, nb, bsum, boff, 0, nullptr);
   SSIM_LAUNCHED("k_scan1");
-  hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(RT), 0, st, n, in, boff, nb, out);
+  hipLaunchKernelGGL(k_block_scan, dim3(nb), dim3(RT), 0, st, n, in, boff, nb, out, every,
+                     sel);
   SSIM_LAUNCHED("k_block_scan");
   return SSIM_OK;
 }

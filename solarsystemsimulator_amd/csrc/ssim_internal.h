@@ -42,8 +42,10 @@ double yoshida_w0();
 
 // Exclusive scan of n ints into out[0..n] (out[n] = total), sort.hip;
 // `work` holds scan_work_bytes(n) (unused for n <= 65536: one workgroup).
+// every > 0 (dividing n) also writes sel[j] = out[every j], j = 0..n/every.
 size_t scan_work_bytes(int n);
-int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st);
+int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st,
+                   int every = 0, int* sel = nullptr);
 
 // Launch timing (abi.hip): SSIM_TIMED wraps a launch statement.
 bool timing_on();
