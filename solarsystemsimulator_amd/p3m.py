@@ -235,9 +235,11 @@ class SplitForce:
         return self.mesh_gather(x, a, order)
 
     def mass_bins(self, x, n_mass, m):
-        """TSC runs: one stable binning of the masses (ssim_split_bin) that
-        serves the atomic-free deposit (bins = the node each mass's stencil
-        is centred on) and the near field (the node blocks are its cells)."""
+        """TSC runs: one stable binning of the masses (ssim_split_bin) by
+        the block of nodes their stencils are centred on; the blocks are the
+        near field's cells and the tiled deposit's input.  Node bins inside
+        the blocks (bin_start) only for the node-bin deposit
+        (tiled_deposit False, or a block edge other than 5)."""
         import torch
         nm = n_mass
         if getattr(self, "_bin_n", -1) != nm:
