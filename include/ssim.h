@@ -506,6 +506,39 @@ int ssim_bounce_deltas(int n, int npairs, const long long* hi, const long long* 
                        long long* affected, long long* counts, void* work,
                        size_t work_bytes, void* stream);
 
+/* The results of the massive-body contact scan applied to the state, on the
+ * device (the host closure `apply_merges`, film_producer.py:781-861), in
+ * list order — a later pair sees the masses and flags the earlier ones
+ * left.  Per pair (pair_i, pair_j, pair_frac): a kernel contact time
+ * contact_t[j] (NaN: none) replaces frac when |contact_t[j] - (k_ev + 1)|
+ * <= 1 and is consumed (set to NaN) either way; with shatter_check, two
+ * visible massive bodies, neither a star / black hole, meeting at
+ * |v_j - v_i| >= 1.5 sqrt(2 G (m_i + m_j) / max(1e-12, r_i + r_j)) stop the
+ * whole call (result[2] = i, result[3] = j, shatter_t = t0_days + (k_ev + 1
+ * + frac) raster_days; the runaway list is not applied); otherwise, when
+ * both are visible and m_i > 0, the lighter body merges into the heavier:
+ * mass-weighted position and velocity (f32 sample, FP64 arithmetic), mass
+ * sum, radius (r_a^3 + r_b^3)^(1/3), the lighter one deactivated (mass 0,
+ * not visible).  Then every visible body of `runaway` is deactivated.
+ * sample: f32 [x|y|vx|vy] rows of n bodies; mass, vis, real_r: the mirrors
+ * by original index (updated); inv_kind / inv_pos map an original index to
+ * the massive (-1) or asteroid (0) arrays and a position in them;
+ * massive / asteroids: 7 device pointers each (x, y, vx, vy, m as double,
+ * visible as uint8, radius as double).  events receives one 32-byte record
+ * per merge or removal in the reference's ring format ("<dIIfIff": time in
+ * days, a, b, new mass, kind 0 = merge / 2 = runaway with a = 0xFFFFFFFF,
+ * x, y), at most npairs + nrun; result[0] = records, result[1] = merges.
+ * One thread; device memory for every array. */
+int ssim_apply_merges(int n, const float* sample, int npairs, const int* pair_i,
+                      const int* pair_j, const double* pair_frac, int nrun,
+                      const int* runaway, double* mass, unsigned char* vis,
+                      double* real_r, const unsigned char* is_ast,
+                      const unsigned char* is_star_bh, double* contact_t, int k_ev,
+                      int shatter_check, double G, double t0_days, double raster_days,
+                      const int* inv_kind, const int* inv_pos, void* const* massive,
+                      void* const* asteroids, unsigned char* events, int* result,
+                      double* shatter_t, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* Learned near-field correction (MLP, BF16 MFMA, FP32 accumulation)   */
 /* ------------------------------------------------------------------ */

@@ -10,11 +10,21 @@ the bounce response — TEST INFRASTRUCTURE ONLY.
   916-1019): greedy one-contact-per-body filter in list order, entry time of
   |dp + dv t| = touch, validity, approach test, restitution impulse and the
   mass-weighted 1.1x overlap push.
+* `apply_merges` — the state changes of `apply_merges` (film_producer.py:
+  781-861): pairs in list order against mirrors that earlier pairs change,
+  the kernel contact time replacing frac inside one raster, the shatter
+  stop at |dv| >= 1.5 v_esc, the mass-weighted merge into the heavier body
+  with a volume-conserving radius, the runaway removals, and the events in
+  the ring's "<dIIfIff" records (`emit_event`, film_producer.py:687-705).
 Pinned in tests/test_merge_oracle.py: a head-on pair conserves momentum and
 reverses the approach speed by e; a body crossing a star between two
-samples reports frac in (-1, 0]; the greedy filter keeps the first pair.
+samples reports frac in (-1, 0]; the greedy filter keeps the first pair;
+merges conserve mass, momentum, the mass-weighted centre and r^3; a chain
+(a, b), (b, c) merges only once; a shatter stops the call.
 """
 from __future__ import annotations
+
+import struct
 
 import numpy as np
 
@@ -127,3 +137,67 @@ def bounce_deltas(sample, hi, hj, real_r, mass, dt_y, restitution=0.6):
         d[1][oj] += pny * shift * (mass[oi] / mt)
     aff = np.unique(np.concatenate([hi, hj]))
     return hi, hj, aff.astype(np.int64), d[:, aff].T.copy()
+
+
+def apply_merges(sample, pairs, runaway, k_ev, mass, vis, real_r, is_ast, is_star_bh,
+                 contact, shatter_check, G, t0_days, raster_days):
+    """The loop of apply_merges on host arrays (updated in place: mass, vis,
+    real_r, contact (dict j -> kernel contact time, consumed)).  Returns
+    (state updates {body: (x, y, vx, vy, m)}, deactivated bodies in order,
+    radius updates {body: r}, event records (bytes), merges, shatter) with
+    shatter = (i, j, t_days) or None."""
+    n = len(mass)
+    sx, sy = sample[0:n], sample[n:2 * n]
+    svx, svy = sample[2 * n:3 * n], sample[3 * n:4 * n]
+    body, dead, radius, events = {}, [], {}, []
+    merges = 0
+
+    def event(a, b, m, kind, x, y, frac=0.0):
+        t = t0_days + (k_ev + 1 + frac) * raster_days
+        events.append(struct.pack("<dIIfIff", t, a & 0xFFFFFFFF, b, m, kind,
+                                  float(x), float(y)))
+
+    for mi, j, frac in pairs:
+        tk = contact.pop(j, None)
+        if tk is not None:
+            f_kern = tk - (k_ev + 1)
+            if -1.0 <= f_kern <= 1.0:
+                frac = f_kern
+        if (shatter_check and not is_ast[mi] and not is_ast[j]
+                and not is_star_bh[mi] and not is_star_bh[j]
+                and vis[mi] and vis[j] and mass[mi] > 0 and mass[j] > 0):
+            v_imp = float(np.hypot(svx[j] - svx[mi], svy[j] - svy[mi]))
+            touch = max(1e-12, real_r[mi] + real_r[j])
+            v_esc = float(np.sqrt(2.0 * G * (mass[mi] + mass[j]) / touch))
+            if v_imp >= 1.5 * v_esc:
+                return (body, dead, radius, events, merges,
+                        (int(mi), int(j), t0_days + (k_ev + 1 + frac) * raster_days))
+        if not vis[mi] or not vis[j] or mass[mi] <= 0:
+            continue
+        a, b = (mi, j) if mass[mi] >= mass[j] else (j, mi)
+        m_a, m_b = mass[a], mass[b]
+        m_ges = m_a + m_b
+        if m_ges <= 0:
+            continue
+        nx = (sx[a] * m_a + sx[b] * m_b) / m_ges
+        ny = (sy[a] * m_a + sy[b] * m_b) / m_ges
+        nvx = (svx[a] * m_a + svx[b] * m_b) / m_ges
+        nvy = (svy[a] * m_a + svy[b] * m_b) / m_ges
+        body[a] = (nx, ny, nvx, nvy, m_ges)
+        dead.append(b)
+        mass[a] = m_ges
+        mass[b] = 0.0
+        vis[b] = 0
+        real_r[a] = (real_r[a] ** 3 + real_r[b] ** 3) ** (1.0 / 3.0)
+        radius[a] = real_r[a]
+        event(a, b, float(m_ges), 0, nx, ny, frac)
+        merges += 1
+    for j in runaway:
+        j = int(j)
+        if not vis[j]:
+            continue
+        dead.append(j)
+        mass[j] = 0.0
+        vis[j] = 0
+        event(0xFFFFFFFF, j, 0.0, 2, float(sx[j]), float(sy[j]))
+    return body, dead, radius, events, merges, None

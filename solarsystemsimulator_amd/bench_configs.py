@@ -15,8 +15,9 @@ Prints one JSON object with a record per configuration (the headline
   deposit, R2C Poisson, central-difference gradient, CIC gather: ms per
   force evaluation and per leapfrog step, FP32 and FP64 mesh;
 * collisions_50k — star + 4 planets + 50,000 asteroids in a cloud at
-  2.5 AU: raster frames/s of the frame kernel, and the uniform-grid
-  collision search of one sample (candidates, contact pairs, ms).
+  2.5 AU: raster frames/s of the frame kernel, the uniform-grid collision
+  search of one sample (candidates, contact pairs, ms) and the merge
+  response of 40 contacts + 8 runaway bodies on the device (one call).
 """
 from __future__ import annotations
 
@@ -141,7 +142,26 @@ def collisions_50k(frames):
     out.update({"search_ms": t * 1e3, "candidates": int(cand),
                 "contact_pairs": 0 if hits is None else int(len(hits[0])),
                 "cell_h_au": h})
-    del torch, dev
+    # the merge response on the device: 40 massive-body contacts and 8
+    # runaway bodies applied to the state (one call, events read back)
+    from .detection import apply_merges
+    rng = np.random.default_rng(5)
+    massive = np.flatnonzero(np.asarray(ast) == 0)
+    pairs = [(int(massive[k % len(massive)]), int(j), 0.0)
+             for k, j in enumerate(rng.choice(np.flatnonzero(ast), 40, replace=False))]
+    runaway = [int(j) for j in rng.choice(np.flatnonzero(ast), 8, replace=False)]
+    mirrors = {"mass": D.to_dev(np.asarray(m, np.float64), dev),
+               "vis": D.to_dev(vis, dev, torch.uint8),
+               "real_r": D.to_dev(np.asarray(rr, np.float64), dev),
+               "is_ast": D.to_dev(np.asarray(ast, np.uint8), dev, torch.uint8),
+               "is_star_bh": torch.zeros(n, dtype=torch.uint8, device=dev)}
+    contact = torch.full((n,), float("nan"), dtype=torch.float64, device=dev)
+    smp = D.to_dev(np.asarray(sample, np.float32), dev, torch.float32)
+    t, (ev, merges, _) = timed(lambda: apply_merges(st, smp, pairs, runaway, 0, mirrors,
+                                                    contact, True, 2.959e-4, 0.0, 1.0))
+    out.update({"apply_merges_ms": t * 1e3, "merge_pairs": len(pairs),
+                "merges": merges, "event_records": len(ev) // 32})
+    del dev
     return out
 
 

@@ -516,6 +516,134 @@ __global__ __launch_bounds__(CB) void k_u8_to_i32(int n, const unsigned char* __
   if (i < n) b[i] = a[i];
 }
 
+// ---- apply_merges on the device ----
+// Body arrays of one kind of the nbody state (massive or asteroid): x, y,
+// vx, vy, m (FP64), visible flags, radii.
+struct BodyArrays {
+  double* f[5];
+  unsigned char* vis;
+  double* rad;
+};
+
+// One 32-byte event record of the reference's ring: "<dIIfIff" = time in
+// days, a, b, new mass, kind, x, y.
+__device__ void put_event(unsigned char* ev, int k, double t, unsigned int a,
+                          unsigned int b, double m, unsigned int kind, double x, double y) {
+  unsigned char* r = ev + 32 * (size_t)k;
+  *reinterpret_cast<double*>(r) = t;
+  *reinterpret_cast<unsigned int*>(r + 8) = a;
+  *reinterpret_cast<unsigned int*>(r + 12) = b;
+  *reinterpret_cast<float*>(r + 16) = (float)m;
+  *reinterpret_cast<unsigned int*>(r + 20) = kind;
+  *reinterpret_cast<float*>(r + 24) = (float)x;
+  *reinterpret_cast<float*>(r + 28) = (float)y;
+}
+
+// The loop of apply_merges (film_producer.py:781-861), one thread, pairs in
+// list order: a later pair sees the masses and flags the earlier ones left.
+// Per pair (i, j, frac): a contact time the nbody kernel saw for j replaces
+// frac when it lies within one raster (and is consumed either way); with
+// shatter_check, two visible massive non-star bodies meeting at
+// |v_j - v_i| >= 1.5 v_esc stop the loop (result[2..3], shatter_t), the
+// runaway list included; otherwise the lighter body merges into the heavier
+// (mass-weighted position and velocity, volume-conserving radius), which the
+// state, the mirrors and a kind-0 event receive.  Then every still visible
+// runaway body is removed (kind 2, a = 0xFFFFFFFF).
+__global__ void k_apply_merges(int n, const float* __restrict__ smp, int npairs,
+                               const int* __restrict__ pi, const int* __restrict__ pj,
+                               const double* __restrict__ pfrac, int nrun,
+                               const int* __restrict__ run, double* mass,
+                               unsigned char* vis, double* real_r,
+                               const unsigned char* __restrict__ is_ast,
+                               const unsigned char* __restrict__ is_star_bh,
+                               double* contact_t, int k_ev, int shatter_check, double G,
+                               double t0_days, double raster_days,
+                               const int* __restrict__ inv_kind,
+                               const int* __restrict__ inv_pos, BodyArrays Ms,
+                               BodyArrays As, unsigned char* events, int* result,
+                               double* shatter_t) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const float* sx = smp;
+  const float* sy = smp + n;
+  const float* svx = smp + 2 * (size_t)n;
+  const float* svy = smp + 3 * (size_t)n;
+  auto arrays = [&](int idx) -> BodyArrays& { return inv_kind[idx] == -1 ? Ms : As; };
+  auto deactivate = [&](int idx) {
+    BodyArrays& B = arrays(idx);
+    const int p = inv_pos[idx];
+    B.f[4][p] = 0.0;
+    B.vis[p] = 0;
+  };
+  int nev = 0, merged = 0;
+  result[2] = -1;
+  result[3] = -1;
+  for (int q = 0; q < npairs; ++q) {
+    const int mi = pi[q], j = pj[q];
+    double frac = pfrac[q];
+    const double tk = contact_t[j];
+    if (tk == tk) {  // present (not NaN): consumed
+      contact_t[j] = __builtin_nan("");
+      const double fk = tk - (double)(k_ev + 1);
+      if (-1.0 <= fk && fk <= 1.0) frac = fk;
+    }
+    if (shatter_check && !is_ast[mi] && !is_ast[j] && !is_star_bh[mi] && !is_star_bh[j] &&
+        vis[mi] && vis[j] && mass[mi] > 0.0 && mass[j] > 0.0) {
+      const float dvx = svx[j] - svx[mi], dvy = svy[j] - svy[mi];
+      const double v_imp = (double)hypotf(dvx, dvy);
+      const double touch = fmax(1e-12, real_r[mi] + real_r[j]);
+      const double v_esc = sqrt(2.0 * G * (mass[mi] + mass[j]) / touch);
+      if (v_imp >= 1.5 * v_esc) {
+        result[2] = mi;
+        result[3] = j;
+        *shatter_t = t0_days + ((double)(k_ev + 1) + frac) * raster_days;
+        result[0] = nev;
+        result[1] = merged;
+        return;
+      }
+    }
+    if (!vis[mi] || !vis[j] || mass[mi] <= 0.0) continue;
+    const int a = mass[mi] >= mass[j] ? mi : j;
+    const int b = a == mi ? j : mi;
+    const double ma = mass[a], mb = mass[b];
+    const double mg = ma + mb;
+    if (mg <= 0.0) continue;
+    const double nx = ((double)sx[a] * ma + (double)sx[b] * mb) / mg;
+    const double ny = ((double)sy[a] * ma + (double)sy[b] * mb) / mg;
+    const double nvx = ((double)svx[a] * ma + (double)svx[b] * mb) / mg;
+    const double nvy = ((double)svy[a] * ma + (double)svy[b] * mb) / mg;
+    {
+      BodyArrays& A = arrays(a);
+      const int p = inv_pos[a];
+      A.f[0][p] = nx;
+      A.f[1][p] = ny;
+      A.f[2][p] = nvx;
+      A.f[3][p] = nvy;
+      A.f[4][p] = mg;
+    }
+    deactivate(b);
+    mass[a] = mg;
+    mass[b] = 0.0;
+    vis[b] = 0;
+    const double ra = real_r[a], rb = real_r[b];
+    real_r[a] = pow(ra * ra * ra + rb * rb * rb, 1.0 / 3.0);
+    arrays(a).rad[inv_pos[a]] = real_r[a];
+    put_event(events, nev++, t0_days + ((double)(k_ev + 1) + frac) * raster_days,
+              (unsigned int)a, (unsigned int)b, mg, 0u, nx, ny);
+    ++merged;
+  }
+  for (int q = 0; q < nrun; ++q) {
+    const int j = run[q];
+    if (!vis[j]) continue;
+    deactivate(j);
+    mass[j] = 0.0;
+    vis[j] = 0;
+    put_event(events, nev++, t0_days + (double)(k_ev + 1) * raster_days, 0xFFFFFFFFu,
+              (unsigned int)j, 0.0, 2u, (double)sx[j], (double)sy[j]);
+  }
+  result[0] = nev;
+  result[1] = merged;
+}
+
 }  // namespace
 
 extern "C" {
@@ -746,6 +874,39 @@ int ssim_collide_hits(int n, const float* gx, const float* gy, const float* gvx,
   SSIM_HIP(hipMemcpyAsync(&np, sums, sizeof(int), hipMemcpyDeviceToHost, st));
   SSIM_HIP(hipStreamSynchronize(st));
   counts[2] = np;
+  return SSIM_OK;
+}
+
+int ssim_apply_merges(int n, const float* sample, int npairs, const int* pair_i,
+                      const int* pair_j, const double* pair_frac, int nrun,
+                      const int* runaway, double* mass, unsigned char* vis,
+                      double* real_r, const unsigned char* is_ast,
+                      const unsigned char* is_star_bh, double* contact_t, int k_ev,
+                      int shatter_check, double G, double t0_days, double raster_days,
+                      const int* inv_kind, const int* inv_pos, void* const* massive,
+                      void* const* asteroids, unsigned char* events, int* result,
+                      double* shatter_t, void* stream) {
+  SSIM_REQUIRE(n >= 1 && npairs >= 0 && nrun >= 0, "bad sizes");
+  SSIM_REQUIRE(sample && mass && vis && real_r && is_ast && is_star_bh && contact_t &&
+                   inv_kind && inv_pos && massive && asteroids && events && result &&
+                   shatter_t && (npairs == 0 || (pair_i && pair_j && pair_frac)) &&
+                   (nrun == 0 || runaway),
+               "null pointer");
+  BodyArrays M{}, A{};
+  for (int f = 0; f < 5; ++f) {
+    M.f[f] = static_cast<double*>(massive[f]);
+    A.f[f] = static_cast<double*>(asteroids[f]);
+  }
+  M.vis = static_cast<unsigned char*>(massive[5]);
+  M.rad = static_cast<double*>(massive[6]);
+  A.vis = static_cast<unsigned char*>(asteroids[5]);
+  A.rad = static_cast<double*>(asteroids[6]);
+  hipStream_t st = ssim::as_stream(stream);
+  hipLaunchKernelGGL(k_apply_merges, dim3(1), dim3(1), 0, st, n, sample, npairs, pair_i,
+                     pair_j, pair_frac, nrun, runaway, mass, vis, real_r, is_ast,
+                     is_star_bh, contact_t, k_ev, shatter_check, G, t0_days, raster_days,
+                     inv_kind, inv_pos, M, A, events, result, shatter_t);
+  SSIM_LAUNCHED("k_apply_merges");
   return SSIM_OK;
 }
 

@@ -175,6 +175,55 @@ def bounce_deltas(dev, sample, hits, real_r, mass, dt_y, restitution=0.6):
     return hi_h[a], hj_h[a], f.astype(np.int64), deltas
 
 
+def apply_merges(st, sample, pairs, runaway, k_ev, mirrors, contact_t,
+                 shatter_check, G, t0_days, raster_days):
+    """The host closure `apply_merges` (film_producer.py:781-861) on the
+    device: the merge pairs (list of (i, j, frac)) and runaway bodies applied
+    to the NBodyCuda state `st` and to the device mirrors (dict of tensors
+    "mass" f64, "vis" u8, "real_r" f64, "is_ast" u8, "is_star_bh" u8, all by
+    original index; updated), in list order.  `contact_t` is an f64 device
+    array of kernel contact times (NaN: none; consumed).  `sample` is the f32
+    (4n) [x|y|vx|vy] row (host or device).  Returns (event records as bytes
+    in the ring's "<dIIfIff" format, merges, shatter) with shatter =
+    (i, j, t_days) or None; a shatter stops the call like the reference's
+    `return` (the caller then freezes and dumps the state)."""
+    torch = _torch()
+    dev = mirrors["mass"].device
+    n = int(mirrors["mass"].numel())
+    smp = sample if torch.is_tensor(sample) else torch.from_numpy(
+        np.ascontiguousarray(sample, np.float32)).to(dev)
+    npairs, nrun = len(pairs), len(runaway)
+    pi = torch.tensor([int(p[0]) for p in pairs] or [0], dtype=torch.int32, device=dev)
+    pj = torch.tensor([int(p[1]) for p in pairs] or [0], dtype=torch.int32, device=dev)
+    pf = torch.tensor([float(p[2]) for p in pairs] or [0.0], dtype=torch.float64,
+                      device=dev)
+    run = torch.tensor([int(j) for j in runaway] or [0], dtype=torch.int32, device=dev)
+    if "inv_kind_d" not in st:
+        st["inv_kind_d"] = torch.from_numpy(st["inv_kind"].astype(np.int32)).to(dev)
+        st["inv_pos_d"] = torch.from_numpy(st["inv_pos"].astype(np.int32)).to(dev)
+    ptrs = [(_lib.ctypes.c_void_p * 7)(*[_lib.ptr(t) for t in st[k][:5]],
+                                      _lib.ptr(st[v]), _lib.ptr(st[r]))
+            for k, v, r in (("M", "mvis", "mrad"), ("a", "avis", "arad"))]
+    events = torch.empty(32 * max(npairs + nrun, 1), dtype=torch.uint8, device=dev)
+    result = torch.zeros(4, dtype=torch.int32, device=dev)
+    shatter_t = torch.zeros(1, dtype=torch.float64, device=dev)
+    _lib.call("ssim_apply_merges", n, _lib.ptr(smp), npairs, _lib.ptr(pi), _lib.ptr(pj),
+              _lib.ptr(pf), nrun, _lib.ptr(run), _lib.ptr(mirrors["mass"]),
+              _lib.ptr(mirrors["vis"]), _lib.ptr(mirrors["real_r"]),
+              _lib.ptr(mirrors["is_ast"]), _lib.ptr(mirrors["is_star_bh"]),
+              _lib.ptr(contact_t), int(k_ev), int(bool(shatter_check)), float(G),
+              float(t0_days), float(raster_days), _lib.ptr(st["inv_kind_d"]),
+              _lib.ptr(st["inv_pos_d"]), _lib.ctypes.addressof(ptrs[0]),
+              _lib.ctypes.addressof(ptrs[1]), _lib.ptr(events), _lib.ptr(result),
+              _lib.ptr(shatter_t), D.stream(dev))
+    res = result.cpu().numpy()
+    nev = int(res[0])
+    shatter = None
+    if res[2] >= 0:
+        shatter = (int(res[2]), int(res[3]), float(shatter_t.cpu().item()))
+    return bytes(events[:32 * nev].cpu().numpy()), int(res[1]), shatter
+
+
 def _streifengrenzen(gx_lebt, anteile):
     """x stripes holding `anteile` of the alive asteroids (outer bounds
     infinite), film_producer.py:382-405."""

@@ -71,3 +71,95 @@ def test_bounce_deltas_match(dev):
     assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
     assert np.array_equal(got[2], ref[2])
     np.testing.assert_allclose(got[3], ref[3], rtol=1e-12, atol=1e-300)
+
+
+def _apply_scene(seed, shatter):
+    from solarsystemsimulator_amd.nbody import NBodyCuda
+    rng = np.random.default_rng(seed)
+    n, nm = 300, 12
+    x, y = rng.uniform(-5, 5, n), rng.uniform(-5, 5, n)
+    vx, vy = rng.normal(0, 1, n), rng.normal(0, 1, n)
+    mass = np.concatenate([rng.uniform(1e-6, 1e-3, nm), rng.uniform(0, 1e-9, n - nm)])
+    mass[nm + 3] = 0.0                                   # massless asteroid
+    vis = np.ones(n, np.uint8)
+    vis[nm + 5] = 0
+    is_ast = np.zeros(n, np.uint8)
+    is_ast[nm:] = 1
+    is_star = np.zeros(n, np.uint8)
+    is_star[0] = 1
+    rr = rng.uniform(1e-5, 1e-3, n)
+    perm = rng.permutation(n)                            # kinds interleaved
+    x, y, vx, vy, mass, vis, is_ast, is_star, rr = (
+        a[perm] for a in (x, y, vx, vy, mass, vis, is_ast, is_star, rr))
+    sim = NBodyCuda(0)
+    st = sim.load_state(x, y, vx, vy, mass, vis, is_ast, real_r=rr)
+    massive = np.flatnonzero(is_ast == 0)
+    pairs = [(int(massive[rng.integers(len(massive))]), int(rng.integers(n)),
+              float(np.float32(rng.uniform(-1, 1)))) for _ in range(40)]
+    pairs.append((int(massive[1]), int(massive[2]), 0.0))
+    pairs.append((int(massive[2]), int(massive[3]), 0.5))   # chain through 2
+    pairs = [p for p in pairs if p[0] != p[1]]
+    runaway = [int(j) for j in rng.integers(0, n, 6)]
+    contact = {int(j): float(rng.uniform(6, 10)) for j in rng.integers(0, n, 25)}
+    sample = np.concatenate([x, y, vx, vy]).astype(np.float32)
+    state0 = np.stack([x, y, vx, vy])
+    return (sim, st, sample, pairs, runaway, contact, mass, vis, rr, is_ast, is_star,
+            state0)
+
+
+@pytest.mark.parametrize("seed,shatter", [(21, False), (22, False), (23, True)])
+def test_apply_merges_matches_oracle(dev, seed, shatter):
+    """detection.apply_merges (device) against the oracle loop: identical
+    event records, merge count and shatter, the state arrays and the
+    mirrors bit for bit (radii within 1e-15), consumed contact times."""
+    from solarsystemsimulator_amd.detection import apply_merges
+    print(f"apply_merges seed={seed} shatter={shatter}", flush=True)
+    sim, st, sample, pairs, runaway, contact, mass, vis, rr, is_ast, is_star, state0 = \
+        _apply_scene(seed, shatter)
+    n = len(mass)
+    G = 2.959e-4 if not shatter else 1e-9              # tiny G: every pair shatters
+    mirrors = {"mass": torch.from_numpy(mass.copy()).to(dev),
+               "vis": torch.from_numpy(vis.copy()).to(dev),
+               "real_r": torch.from_numpy(rr.copy()).to(dev),
+               "is_ast": torch.from_numpy(is_ast.copy()).to(dev),
+               "is_star_bh": torch.from_numpy(is_star.copy()).to(dev)}
+    ct = np.full(n, np.nan)
+    for j, t in contact.items():
+        ct[j] = t
+    ct_d = torch.from_numpy(ct).to(dev)
+    ev, merges, sh = apply_merges(st, sample, pairs, runaway, 7, mirrors, ct_d, shatter,
+                                  G, 1000.0, 0.5)
+    m_o, v_o, r_o, c_o = mass.copy(), vis.copy(), rr.copy(), dict(contact)
+    body, dead, radius, ev_o, merges_o, sh_o = M.apply_merges(
+        sample, pairs, runaway, 7, m_o, v_o, r_o, is_ast, is_star, c_o, shatter, G,
+        1000.0, 0.5)
+    assert ev == b"".join(ev_o) and merges == merges_o
+    assert (sh is None) == (sh_o is None)
+    if sh is not None:
+        assert sh == sh_o
+    assert np.array_equal(mirrors["mass"].cpu().numpy(), m_o)
+    assert np.array_equal(mirrors["vis"].cpu().numpy(), v_o)
+    np.testing.assert_allclose(mirrors["real_r"].cpu().numpy(), r_o, rtol=1e-15)
+    # consumed contact times: the oracle popped exactly the pairs it reached
+    got_ct = ct_d.cpu().numpy()
+    for j in range(n):
+        if j in c_o:
+            assert got_ct[j] == c_o[j]
+        else:
+            assert np.isnan(got_ct[j])
+    # the device state: merged bodies set, removed bodies mass 0 / hidden
+    out = sim.export_f64(st).reshape(4, n)
+    for b in range(n):
+        if b in body:
+            assert tuple(out[:, b]) == tuple(body[b][:4])
+        elif b not in dead:
+            assert np.array_equal(out[:, b], state0[:, b])     # untouched
+    kind, pos = st["inv_kind"], st["inv_pos"]
+    for b in set(dead) | set(body):
+        arrs = (st["M"], st["mvis"], st["mrad"]) if kind[b] == -1 else \
+               (st["a"], st["avis"], st["arad"])
+        m_dev = float(arrs[0][4][pos[b]].cpu())
+        v_dev = int(arrs[1][pos[b]].cpu())
+        assert m_dev == m_o[b] and v_dev == v_o[b]
+        if b in radius:
+            assert abs(float(arrs[2][pos[b]].cpu()) - radius[b]) <= 1e-15 * radius[b]
