@@ -108,14 +108,17 @@ def conv_bytes(n, mesh_bytes=4):
     kernel spectra real mesh_bytes): the 4 pruned FFT launches (R2C rows,
     y lines, the three components' inverse y lines, the three components'
     C2R rows written as one float4 field) and the fused x-axis kernel (x < n
-    half in, three kernel spectra in, three x < n halves out)."""
+    half in, the kx, ky <= n corners of three kernel spectra in, three x < n
+    halves out)."""
     P, H = 2 * n, n + 1
     c = 2 * mesh_bytes
     passes = [n ** 3 * mesh_bytes + n * n * H * c,          # R2C rows from rho
               n * n * H * c + n * P * H * c,                # y lines, x < n
               3 * (n * P * H * c + n * n * H * c),          # inverse y lines
               3 * n * n * H * c + n ** 3 * 4 * mesh_bytes]  # C2R rows, float4
-    x3 = n * P * H * c + 3 * P * P * H * mesh_bytes + 3 * n * P * H * c
+    # kernel spectra: odd / even in each axis, so the kx, ky <= n corner of
+    # each table ((n + 1)^2 H values) holds all of it
+    x3 = n * P * H * c + 3 * (n + 1) * (n + 1) * H * mesh_bytes + 3 * n * P * H * c
     return passes, x3
 
 
@@ -295,7 +298,7 @@ def main():
                        f"FFT passes of one {2 * n_mesh}^3 isolated convolution"),
         "k_conv_x3": ("hbm", x3_bytes, HBM_PEAK_GBS, "GB/s",
                       f"{x3_bytes / 1e6:.1f} MB: x < {n_mesh} half in, 3 kernel "
-                      f"spectra in, 3 x < {n_mesh} halves out"),
+                      f"spectrum corners in, 3 x < {n_mesh} halves out"),
         "k_tsc_deposit": ("hbm", n * 32 + 27 * 4 * n, HBM_PEAK_GBS, "GB/s",
                           "32 B/mass in + 27 FP32 node updates"),
         "k_tsc_gather": ("hbm", nb * (24 + 24) + 3 * 4 * n_mesh ** 3,
