@@ -84,7 +84,32 @@ __global__ __launch_bounds__(CB) void k_split_key(
   const int k = nodes ? ssim_blocked_key(a, b, c, B, nb) : ((a / B) * nb + b / B) * nb + c / B;
   key[i] = k;
   order[i] = i;
-  atomicAdd(&count[k], 1);
+  if (count) atomicAdd(&count[k], 1);  // node bins only (blocks: k_run_starts)
+}
+
+// start[c] = the number of sorted keys below c, c in [0, nbins]: bin c's first
+// slot, what the exclusive scan of the bin counts gives.  Wave w looks at the
+// slots p in [64w, 64w + 64) up to n, with K(n) = nbins and K(-1) = -1; the
+// bins c in (K(64w - 1), K(its last slot)] start among them, at 64w + #{p :
+// K(p) < c}, and these ranges tile [0, nbins].  A bin range the wave fills
+// 64 bins per round, each by a 6-step search over the lanes' sorted keys.
+__global__ __launch_bounds__(CB) void k_run_starts(int n, const int* __restrict__ key,
+                                                   int nbins, int* __restrict__ start) {
+  const int lane = threadIdx.x & 63;
+  const int p0 = (int)(((size_t)blockIdx.x * CB + threadIdx.x) >> 6) * 64;
+  if (p0 > n) return;  // whole waves
+  const int p = p0 + lane;
+  const int kp = p < n ? key[p] : (p == n ? nbins : nbins + 1);
+  const int lo = p0 > 0 ? key[p0 - 1] : -1;
+  const int hi = __shfl(kp, min(63, n - p0), 64);
+  for (int c0 = lo + 1; c0 <= hi; c0 += 64) {
+    const int c = c0 + lane;
+    int r = 0;  // lanes with a key below c (<= 63: the last slot's key is hi)
+#pragma unroll
+    for (int st = 32; st >= 1; st >>= 1)
+      if (__shfl(kp, r + st - 1, 64) < c) r += st;
+    if (c <= hi) start[c] = p0 + r;
+  }
 }
 
 // Both packed copies in bin order: float4 (x - cx, y - cy, z - cz, m) for
@@ -272,13 +297,15 @@ int ssim_split_bin(int n, const double* x, const double* y, const double* z,
   w += ssim::align256((size_t)K * sizeof(int));
   void* scan_work = w;
   w += ssim::scan_work_bytes((int)K);
-  hipLaunchKernelGGL(k_zero_i32, dim3(ssim::ceil_div(nbins, CB)), dim3(CB), 0, st,
-                     (int)nbins, count);
-  SSIM_LAUNCHED("k_zero_i32");
+  if (nodes) {
+    hipLaunchKernelGGL(k_zero_i32, dim3(ssim::ceil_div(nbins, CB)), dim3(CB), 0, st,
+                       (int)nbins, count);
+    SSIM_LAUNCHED("k_zero_i32");
+  }
   if (n > 0) {
     hipLaunchKernelGGL(k_split_key, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st, n, x,
                        y, z, origin[0], origin[1], origin[2], h, grid_n, block, nb,
-                       nodes ? 1 : 0, key, order, count);
+                       nodes ? 1 : 0, key, order, nodes ? count : nullptr);
     SSIM_LAUNCHED("k_split_key");
     int bits = 1;
     while (bits < 31 && (1LL << bits) < nbins) ++bits;
@@ -288,7 +315,11 @@ int ssim_split_bin(int n, const double* x, const double* y, const double* z,
   if (nodes) {
     SSIM_TRY(ssim::scan_exclusive((int)K, count, bin_start, scan_work, st, B3, cell_start));
   } else {
-    SSIM_TRY(ssim::scan_exclusive(ncells, count, cell_start, scan_work, st));
+    // the block starts from the sorted keys (n = 0: all zero)
+    const long long waves = n / 64 + 1;
+    hipLaunchKernelGGL(k_run_starts, dim3(ssim::ceil_div(waves * 64, CB)), dim3(CB), 0,
+                       st, n, key, ncells, cell_start);
+    SSIM_LAUNCHED("k_run_starts");
   }
   if (n > 0) {
     const double hh = 0.5 * h;

@@ -7,7 +7,7 @@
 //                     hist[d * hrow + b], rows padded to a multiple of 4);
 //                     it also clears the count arrays of the later passes;
 //   k_radix_scatter — the tile's first slot per digit (its own exclusive
-//                     prefix over the tile counts, or k_radix_offs's when
+//                     prefix over the tile counts, or k_radix_rows's when
 //                     there are many tiles), then per round the rank of an
 //                     element among the lanes of its wave with the same
 //                     digit (nine ballots), wave totals folded in wave
@@ -17,7 +17,8 @@
 // Tiles, rounds, waves and lanes are taken in index order, so equal keys
 // keep their input order: the sort is stable, and the result does not
 // depend on scheduling (the next-pass counts are sums of ones).  A 3-pass
-// sort is 4 launches, and the passes alternate between two scratch pairs
+// sort is 4 launches (7 over more than FUSE_TILES tiles, with k_radix_rows
+// before each scatter), and the passes alternate between two scratch pairs
 // so the last one writes the caller's arrays.
 #include "ssim_internal.h"
 
@@ -26,8 +27,9 @@ namespace {
 constexpr int RT = 256;            // threads per workgroup (one per digit)
 constexpr int RIPT = 4;            // rounds per tile
 constexpr int RTILE = RT * RIPT;   // elements per tile
-constexpr int FUSE_TILES = 256;    // up to this many tiles a scatter block
-                                   // computes its own offsets
+constexpr int FUSE_TILES = 64;     // up to this many tiles a scatter block
+                                   // computes its own offsets (its row loop
+                                   // is latency-bound; beyond, k_radix_rows)
 constexpr int SCAN_T = 1024;       // threads of the single-workgroup scan
 
 __global__ __launch_bounds__(RT) void k_radix_hist0(int n, const int* __restrict__ keys,
@@ -84,20 +86,29 @@ __device__ __forceinline__ int row_sums(const int* __restrict__ row, int hrow, i
   return tot;
 }
 
-// One workgroup: offs[d * hrow + b] = first slot of digit d from tile b.
-__global__ __launch_bounds__(RT) void k_radix_offs(const int* __restrict__ hist,
-                                                   int tiles, int hrow,
-                                                   int* __restrict__ offs) {
+// One workgroup per digit d: offs[d * hrow + b] = the digit's count in the
+// tiles before b, tot[d] = its count over all tiles.
+__global__ __launch_bounds__(RT) void k_radix_rows(const int* __restrict__ hist, int hrow,
+                                                   int* __restrict__ offs,
+                                                   int* __restrict__ tot) {
   __shared__ int s[256];
-  const int d = threadIdx.x;
-  int pre;
-  const int tot = row_sums(hist + (size_t)d * hrow, hrow, 0, &pre);
-  int run = excl_scan256(tot, s);
-  for (int j = 0; j < tiles; ++j) {
-    const int c = hist[(size_t)d * hrow + j];
-    offs[(size_t)d * hrow + j] = run;
-    run += c;
+  __shared__ int sum;
+  const int t = threadIdx.x, d = blockIdx.x;
+  const int4* r4 = reinterpret_cast<const int4*>(hist + (size_t)d * hrow);
+  int4* o4 = reinterpret_cast<int4*>(offs + (size_t)d * hrow);
+  int carry = 0;
+  for (int q0 = 0; q0 < hrow / 4; q0 += RT) {
+    const int q = q0 + t;
+    const int4 c = q < hrow / 4 ? r4[q] : make_int4(0, 0, 0, 0);
+    const int v = c.x + c.y + c.z + c.w;
+    const int ex = carry + excl_scan256(v, s);
+    if (t == RT - 1) sum = ex + v;
+    if (q < hrow / 4) o4[q] = make_int4(ex, ex + c.x, ex + c.x + c.y, ex + c.x + c.y + c.z);
+    __syncthreads();
+    carry = sum;
+    __syncthreads();
   }
+  if (t == 0) tot[d] = carry;
 }
 
 __global__ __launch_bounds__(RT) void k_radix_scatter(
@@ -120,7 +131,7 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
     v[r] = i < n ? vals[i] : 0;
   }
   if (offs) {
-    run[t] = offs[(size_t)t * hrow + blockIdx.x];
+    run[t] = excl_scan256(offs[(size_t)256 * hrow + t], s) + offs[(size_t)t * hrow + blockIdx.x];
   } else {
     int pre;
     const int tot = row_sums(hist + (size_t)t * hrow, hrow, blockIdx.x, &pre);
@@ -345,7 +356,7 @@ size_t radix_sort_work_bytes(int n) {
   const size_t tiles = (nn + RTILE - 1) / RTILE;
   const size_t hrow = (tiles + 3) & ~size_t(3);
   return 4 * align256(nn * sizeof(int)) + align256(4 * 256 * hrow * sizeof(int)) +
-         align256(256 * hrow * sizeof(int));
+         align256((256 * hrow + 256) * sizeof(int));
 }
 
 int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
@@ -383,8 +394,9 @@ int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
     const int* h = hist + p * ht;
     const int* o = nullptr;
     if (tiles > FUSE_TILES) {
-      hipLaunchKernelGGL(k_radix_offs, dim3(1), dim3(RT), 0, st, h, tiles, hrow, offs);
-      SSIM_LAUNCHED("k_radix_offs");
+      hipLaunchKernelGGL(k_radix_rows, dim3(256), dim3(RT), 0, st, h, hrow, offs,
+                         offs + ht);
+      SSIM_LAUNCHED("k_radix_rows");
       o = offs;
     }
     hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(RT), 0, st, n, sk, sv, 8 * p,
