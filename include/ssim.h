@@ -163,6 +163,25 @@ int ssim_nbody_frame(int n_ast, double* ax, double* ay, double* avx,
                      unsigned char* sub_n, float* hit_t, int* hit_m,
                      void* work, size_t work_bytes, void* stream);
 
+/* Film ring slots of k raster samples, written on the device from the
+ * outputs of ssim_nbody_frame: replaces the host side of
+ * film_producer.py:81-123 (`slot_bytes`, `schreibe_slot`) applied to
+ * nbody_kernel.py:836-859 (scatter to original order) and 894-934 (complete
+ * fine tracks, subN == m_sub).  Slot f at slots + f * slot_bytes:
+ *   x[n] f32 | y[n] f32 | u32 nh | u32 idx[sub_max] | f32 track[sub_max][m_sub][2]
+ * nh = min(totals[f], sub_max); idx ascending original indices (the lowest
+ * kept on overflow, totals[f] > sub_max); entries past nh untouched.
+ * a_idx (u32 [n_ast], strictly increasing) and m_idx (u32 [m]) map the
+ * integrator's slots to original indices, n = n_ast + m.  m_sub == 0 or
+ * sub_max == 0 writes nh = 0.  work: ssim_ring_work_bytes(k, n_ast). */
+long long ssim_ring_slot_bytes(int n, int m_sub, int sub_max);
+size_t ssim_ring_work_bytes(int k, int n_ast);
+int ssim_ring_slots(int k, int n, int n_ast, int m, int m_sub, int sub_max,
+                    const float* snap, const float* snap_m, const float* sub_pos,
+                    const unsigned char* sub_n, const unsigned* a_idx,
+                    const unsigned* m_idx, unsigned char* slots, int* totals,
+                    void* work, size_t work_bytes, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* FP64 energy diagnostic                                              */
 /* ------------------------------------------------------------------ */

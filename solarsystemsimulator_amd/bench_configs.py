@@ -61,6 +61,33 @@ def solar_system(frames):
             "frames_per_s": nf / t, "body_frames_per_s": nf * len(x) / t}
 
 
+def ring_slots_50k(batches):
+    """Raster batches of the 50,005-body asteroid cloud (m_sub 8, 16 samples
+    of 0.5 day per batch) delivered as film ring slots: step_batch (frame,
+    D2H of the rows, host scatter to original order and track collection;
+    the per-sample slot writing of the producer comes on top and is not
+    timed) against step_batch_slots (slots written on the device, one D2H)."""
+    from .constants import K_MAX
+    from .nbody import NBodyCuda
+    x, y, vx, vy, m, rr, ast = scenes.asteroid_cloud(50_000, seed=4)
+    vis = np.ones(len(x), np.uint8)
+    raster = 0.5 / 365.25
+    sub_max = min(len(x), max(2048, len(x) // 32))       # film_producer.py:75-76
+    res = {"bodies": len(x), "samples_per_batch": K_MAX, "sub_max": sub_max}
+    for name in ("step_batch", "step_batch_slots"):
+        sim = NBodyCuda(0, m_sub=8)
+        st = sim.load_state(x, y, vx, vy, m, vis, ast, real_r=rr)
+        if name == "step_batch":
+            run = lambda: sim.step_batch(st, raster, K_MAX)
+        else:
+            run = lambda: sim.step_batch_slots(st, raster, K_MAX, sub_max)
+        run()                                                # warm-up
+        t, _ = timed(lambda: [run() for _ in range(batches)])
+        res[f"{name}_ms_per_batch"] = 1e3 * t / batches
+    res["slot_bytes"] = 8 * len(x) + 4 + sub_max * (4 + 64)
+    return res
+
+
 def allpairs_65k(steps):
     from .selfgrav import SelfGravSystem
     pos, m = scenes.selfgrav_cube()
@@ -174,7 +201,8 @@ def main(argv=None):
     for name, fn, arg in (("solar_system", solar_system, 160 if q else 1000),
                           ("allpairs_65k", allpairs_65k, 5 if q else 20),
                           ("periodic_pm_262k", periodic_pm, 5 if q else 50),
-                          ("collisions_50k", collisions_50k, 32 if q else 160)):
+                          ("collisions_50k", collisions_50k, 32 if q else 160),
+                          ("ring_slots_50k", ring_slots_50k, 3 if q else 10)):
         log(f"== {name}")
         res[name] = fn(arg)
         log(json.dumps(res[name]))

@@ -127,6 +127,81 @@ class NBodyCuda:
             raise ValueError("kein Zustand geladen — FULL-Frame noetig")
         if not 1 <= k <= K_MAX:
             raise ValueError(f"k ausserhalb 1..{K_MAX}: {k}")
+        self._run_frames(st, dt_raster_years, k)
+        na, m, n = st["n_ast"], st["m"], st["N"]
+        out = np.empty((k, 4 * n), np.float32)
+        snap = st["snap"][:k * 4 * na].cpu().numpy().reshape(k, 4, na)
+        snapm = st["snapM"][:k * 4 * m].cpu().numpy().reshape(k, 4, m)
+        for f in range(4):
+            out[:, f * n + st["a_idx_h"]] = snap[:, f, :]
+            out[:, f * n + st["m_idx_h"]] = snapm[:, f, :]
+        st["sub"] = self._collect_sub(st, k)
+        st["hits"] = self._collect_hits(st)
+        return out
+
+    def step_batch_slots(self, st: dict, dt_raster_years: float, k: int,
+                         sub_max: int, out: np.ndarray | None = None):
+        """k raster samples straight into film ring slots: the slot layout of
+        film_producer.py:81-123 (`slot_bytes`, `schreibe_slot` applied to the
+        batch's rows and complete tracks) is written on the device and
+        copied to the host in one transfer, instead of the host scatter to
+        original order plus one argsort and transpose per sample.
+
+        Returns (slots uint8 (k, slot_bytes), overflow bool (k,)), overflow
+        where more complete tracks than sub_max fell due (the lowest original
+        indices are kept, as in the reference).  With `out` (uint8
+        (k, slot_bytes), e.g. k consecutive slots of the shared ring) the
+        slots land there and `out` is returned.  Bytes of the index and
+        track fields past the slot's count are unspecified.  st["hits"] is
+        collected as after step_batch; st["sub"] is None (the tracks are in
+        the slots)."""
+        torch = _torch()
+        if st is None:
+            raise ValueError("kein Zustand geladen — FULL-Frame noetig")
+        if not 1 <= k <= K_MAX:
+            raise ValueError(f"k ausserhalb 1..{K_MAX}: {k}")
+        if sub_max < 0:
+            raise ValueError(f"sub_max < 0: {sub_max}")
+        na, m, n = st["n_ast"], st["m"], st["N"]
+        lib = _lib.load()
+        sb = int(lib.ssim_ring_slot_bytes(n, self.m_sub, int(sub_max)))
+        if out is not None and (out.shape != (k, sb) or out.dtype != np.uint8):
+            raise ValueError(f"out muss uint8 ({k}, {sb}) sein, ist {out.dtype} {out.shape}")
+        self._run_frames(st, dt_raster_years, k)
+        dev = self._dev
+        if "a_idx_d" not in st:
+            for key, h in (("a_idx_d", st["a_idx_h"]), ("m_idx_d", st["m_idx_h"])):
+                h = np.asarray(h, np.int64)
+                st[key] = torch.as_tensor(h.astype(np.uint32).view(np.int32)).to(dev)
+            assert np.all(np.diff(st["a_idx_h"]) > 0)      # slot order = index order
+        if st.get("ring_dev") is None or st["ring_dev"].numel() < k * sb:
+            st["ring_dev"] = torch.empty(k * sb, dtype=torch.uint8, device=dev)
+            st["ring_host"] = torch.empty(k * sb, dtype=torch.uint8, pin_memory=True)
+            st["ring_tot"] = torch.empty(K_MAX, dtype=torch.int32, device=dev)
+            st["ring_tot_h"] = torch.empty(K_MAX, dtype=torch.int32, pin_memory=True)
+        wb = lib.ssim_ring_work_bytes(k, na)
+        if st.get("ring_work") is None or st["ring_work"].numel() < wb:
+            st["ring_work"] = D.workspace(wb, dev)
+        s = D.stream(dev)
+        _lib.call("ssim_ring_slots", k, n, na, m, self.m_sub, int(sub_max),
+                  _lib.ptr(st["snap"]), _lib.ptr(st["snapM"]), _lib.ptr(st["subpos"]),
+                  _lib.ptr(st["subn"]), _lib.ptr(st["a_idx_d"]), _lib.ptr(st["m_idx_d"]),
+                  _lib.ptr(st["ring_dev"]), _lib.ptr(st["ring_tot"]),
+                  _lib.ptr(st["ring_work"]), st["ring_work"].numel(), s)
+        host, tot = st["ring_host"][:k * sb], st["ring_tot_h"][:k]
+        host.copy_(st["ring_dev"][:k * sb], non_blocking=True)
+        tot.copy_(st["ring_tot"][:k], non_blocking=True)
+        torch.cuda.current_stream(dev).synchronize()
+        slots = host.numpy().reshape(k, sb)
+        if out is not None:
+            out[...] = slots
+        else:
+            out = slots.copy()
+        st["sub"] = None
+        st["hits"] = self._collect_hits(st)
+        return out, tot.numpy() > sub_max
+
+    def _run_frames(self, st: dict, dt_raster_years: float, k: int) -> None:
         na, m, n = st["n_ast"], st["m"], st["N"]
         s = D.stream(self._dev)
         _lib.call("ssim_fill_f32", max(na, 1), _lib.ptr(st["hitt"]), -1.0, s)
@@ -143,15 +218,6 @@ class NBodyCuda:
                   _lib.ptr(st["subn"]), _lib.ptr(st["hitt"]),
                   _lib.ptr(st["hitm"]), _lib.ptr(st["work"]),
                   st["work"].numel(), s)
-        out = np.empty((k, 4 * n), np.float32)
-        snap = st["snap"][:k * 4 * na].cpu().numpy().reshape(k, 4, na)
-        snapm = st["snapM"][:k * 4 * m].cpu().numpy().reshape(k, 4, m)
-        for f in range(4):
-            out[:, f * n + st["a_idx_h"]] = snap[:, f, :]
-            out[:, f * n + st["m_idx_h"]] = snapm[:, f, :]
-        st["sub"] = self._collect_sub(st, k)
-        st["hits"] = self._collect_hits(st)
-        return out
 
     def _collect_hits(self, st):
         """(idx, t, partner): contacts found by the fine loop

@@ -114,7 +114,9 @@ def collide_4711():
         if name == "klumpen":
             sample = np.concatenate([x, y, vx, vy]).astype(np.float32)
             mass = np.random.default_rng(2).uniform(1e-12, 3e-12, len(x))
-            hi, hj, hit, dl = M.bounce_deltas(sample, pairs[0], pairs[1], rr, mass, S.DT_Y)
+            # the producer's radii are FP64 host arrays (film_producer.py:916-1019)
+            hi, hj, hit, dl = M.bounce_deltas(sample, pairs[0], pairs[1],
+                                              rr.astype(np.float64), mass, S.DT_Y)
             d.update(bounce_mass=mass, bounce_hi=hi.astype(np.int32),
                      bounce_hj=hj.astype(np.int32),
                      bounce_hit=hit.astype(np.int32), bounce_deltas=dl)

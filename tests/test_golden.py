@@ -73,8 +73,8 @@ def test_oracle_bounce_reproduces_fixture():
     x, y, vx, vy, rr = g["klumpen_in"]
     sample = np.concatenate([x, y, vx, vy]).astype(np.float32)
     hi, hj, hit, dl = M.bounce_deltas(sample, g["klumpen_i"].astype(np.int64),
-                                      g["klumpen_j"].astype(np.int64), rr,
-                                      g["bounce_mass"], DT_Y)
+                                      g["klumpen_j"].astype(np.int64),
+                                      rr.astype(np.float64), g["bounce_mass"], DT_Y)
     assert np.array_equal(hi, g["bounce_hi"]) and np.array_equal(hj, g["bounce_hj"])
     assert np.array_equal(hit, g["bounce_hit"])
     assert np.array_equal(dl, g["bounce_deltas"])
@@ -166,7 +166,7 @@ def test_gpu_bounce_fixture(dev):
     sample = np.concatenate([x, y, vx, vy]).astype(np.float32)
     got = bounce_deltas(dev, sample, (g["klumpen_i"].astype(np.int64),
                                       g["klumpen_j"].astype(np.int64)),
-                        rr, g["bounce_mass"], DT_Y)
+                        rr.astype(np.float64), g["bounce_mass"], DT_Y)
     assert np.array_equal(got[0], g["bounce_hi"]) and np.array_equal(got[1], g["bounce_hj"])
     assert np.array_equal(got[2], g["bounce_hit"])
     np.testing.assert_allclose(got[3], g["bounce_deltas"], rtol=1e-12, atol=1e-300)
