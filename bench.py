@@ -39,8 +39,7 @@ FP64_PEAK_TFLOPS = 78.6          # MI355X vector FP64
 FP32_PEAK_TFLOPS = 157.3         # MI355X vector FP32
 BF16_PEAK_TFLOPS = 2500.0        # MI355X dense BF16 matrix
 HBM_PEAK_GBS = 8000.0
-FLOPS_PER_PAIR = 20              # 3-D pair interaction convention (direct)
-FLOPS_PER_NEAR_PAIR = 28         # near-field pair: 20 + table split function
+FLOPS_PER_PAIR = 20              # 3-D pair interaction convention
 TRAFFIC_FILE = Path(__file__).resolve().parent / "profiles" / "pmc_traffic.json"
 
 
@@ -57,6 +56,9 @@ def parse():
     ap.add_argument("--weights", default=None)
     ap.add_argument("--serial", action="store_true",
                     help="one stream: no overlap of the near field with the mesh")
+    ap.add_argument("--no-gates", action="store_true",
+                    help="skip the 1000-step energy-drift and force-error gates")
+    ap.add_argument("--gate-steps", type=int, default=1000)
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel of a step instead of replaying "
                          "the captured step")
@@ -80,6 +82,25 @@ def kernel_times(lib, names):
     return out
 
 
+def cpu_model():
+    """The host CPU's model name (lscpu "Model name", else /proc/cpuinfo)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(pos, m, tpos, eps2):
     """Plain FP64 all-pairs (the oracle's port of the reference's sum, OpenMP
     on all host cores) on a bounded sample of targets, scaled to
@@ -98,6 +119,7 @@ def cpu_baseline(pos, m, tpos, eps2):
     return {"value": (n + t) / (dt * (n + t) / k), "unit": "body-steps/s",
             "cores": int(os.environ.get("OMP_NUM_THREADS", nthreads)),
             "kind": "port", "method": "direct FP64 all-pairs",
+            "cpu_model": cpu_model(), "os_cpu_count": os.cpu_count(),
             "sample": f"{k} of {n + t} targets x {n} sources, one FP64 force "
                       f"evaluation ({dt:.1f} s), extrapolated to all targets"}
 
@@ -122,6 +144,26 @@ def conv_bytes(n, mesh_bytes=4):
     return passes, x3
 
 
+def algorithmic_bytes(n, t, n_mesh, mesh_bytes=4):
+    """Algorithmic HBM bytes per launch of the PM + learned kernels (each
+    input read once, each output written once): n masses, t tracers, an
+    n_mesh^3 mesh.  The near field reads every target's position and
+    writes its acceleration (48 B), reads the masses' packed float4
+    sources (16 B) and the target order (4 B per target)."""
+    nb = n + t
+    g3 = n_mesh ** 3
+    passes, x3 = conv_bytes(n_mesh, mesh_bytes)
+    return {
+        "k_pp_cells": nb * (24 + 24 + 4) + n * 16,
+        "k_fft_pass": sum(passes) / len(passes),
+        "k_conv_x3": x3,
+        "k_tsc_deposit": n * (3 * 8 + 8) + g3 * mesh_bytes,
+        "k_tsc_gather": nb * 3 * 8 + 3 * g3 * mesh_bytes + nb * 3 * 8,
+        "k_mlp_forward": nb * (64 + 12),
+        "k_mlp_features": nb * (24 + 64) + g3 * mesh_bytes,
+    }
+
+
 # rocprofv3 kernel names of the bench's timed kernels (PMC passes)
 PMC_KERNELS = {"k_pp_cells": "k_pp_cells<false>", "k_fft_pass": "k_line256",
                "k_conv_x3": "k_conv_x3", "k_tsc_deposit": "k_tsc_deposit_tiles",
@@ -142,9 +184,7 @@ def pmc_summary(fetch_dir, write_dir, out):
                 and "Fill" not in r["Kernel_Name"]][0] for k, v in rows.items()}
     copy_kib = 256 * 1024
     factor = {k: copy_kib / float(copy[k]["Counter_Value"]) for k in rows}
-    n_mesh = 128
-    passes, x3 = conv_bytes(n_mesh)
-    alg = {"k_fft_pass": sum(passes) / len(passes), "k_conv_x3": x3}
+    alg = algorithmic_bytes(250_000, 500_000, 128)
     rec = {"workload": "250k_cube", "copy_bytes": copy_kib * 1024,
            "copy_fetch_kib": float(copy["fetch"]["Counter_Value"]),
            "copy_write_kib": float(copy["write"]["Counter_Value"]),
@@ -170,9 +210,81 @@ def pmc_summary(fetch_dir, write_dir, out):
         rec["kernels"][name] = {"rocprof_name": pat, "fetch_kib_per_launch": kib["fetch"],
                                 "write_kib_per_launch": kib["write"],
                                 "traffic_bytes_per_launch": traffic,
-                                "algorithmic_bytes_per_launch": alg.get(name)}
+                                "algorithmic_bytes_per_launch": alg.get(name),
+                                "traffic_over_algorithmic":
+                                    traffic / alg[name] if alg.get(name) else None}
     Path(out).write_text(json.dumps(rec, indent=1))
     print(json.dumps(rec, indent=1))
+
+
+def gates(dev, pos, m, tpos, net, eps2, dt, steps, every=250):
+    """The method's accuracy gates over `steps` steps from the cold start:
+    the exact FP64 energy of the masses (ssim_energy, all pairs) at the
+    start and the end of a PM + learned run and of a direct FP64 run, and
+    the median relative force error of PM + learned (and of the split force
+    without the network) against the FP64 direct sum every `every` steps.
+    The direct run integrates the masses alone: the tracers are massless,
+    so the masses' trajectories are the same with or without them."""
+    from solarsystemsimulator_amd import _lib
+    from solarsystemsimulator_amd.learned_pm import PMLearnedSystem, force_errors
+    from solarsystemsimulator_amd.selfgrav import SelfGravSystem
+    n, t = len(m), len(tpos[0])
+    nb = n + t
+    zn, zt = [np.zeros(n)] * 3, [np.zeros(t)] * 3
+    judge = SelfGravSystem(dev, 3, n, 0, _lib.PREC_F64, _lib.SCHEME_KDK, 1.0, eps2)
+    judge.set_bodies(pos, zn, m)
+
+    def energy(x, v):
+        for c in range(3):
+            judge.x[c][:n].copy_(x[c][:n])
+            judge.v[c][:n].copy_(v[c][:n])
+        return float(judge.energy()[2])
+
+    ref = SelfGravSystem(dev, 3, n, t, _lib.PREC_F64, _lib.SCHEME_KDK, 1.0, eps2)
+    ref.set_bodies(pos, zn, m, tpos, zt)
+    fast = PMLearnedSystem(dev, n, t, eps2, 1.0, net=net)
+    fast.set_bodies(pos, zn, m, tpos, zt)
+    e0 = energy(fast.x, fast.v)
+    errs, done, t0 = [], 0, time.perf_counter()
+    for cp in list(range(0, steps + 1, every)) + ([steps] if steps % every else []):
+        if cp > done:
+            fast.run(dt, cp - done)
+            done = cp
+        for c in range(3):
+            ref.x[c][:nb].copy_(fast.x[c])
+        a_ref = [c.cpu().numpy() for c in ref.accelerations()]
+        el = force_errors(fast.force(learned=True), a_ref)
+        ep = force_errors(fast.force(learned=False), a_ref)
+        fast.force(learned=True)
+        errs.append({"step": cp, "median": float(np.median(el)),
+                     "p90": float(np.percentile(el, 90)),
+                     "median_without_mlp": float(np.median(ep))})
+        log(f"gate step {cp}: median force error {errs[-1]['median']:.3e} "
+            f"(without MLP {errs[-1]['median_without_mlp']:.3e})")
+    e_pm = energy(fast.x, fast.v)
+    pm_s = time.perf_counter() - t0
+    fast.close()
+    del fast, ref
+    direct = SelfGravSystem(dev, 3, n, 0, _lib.PREC_F64, _lib.SCHEME_KDK, 1.0, eps2)
+    direct.set_bodies(pos, zn, m)
+    t0 = time.perf_counter()
+    left = steps
+    while left > 0:                           # chunks keep progress visible
+        k = min(100, left)
+        direct.run(dt, k)
+        left -= k
+        log(f"gate: direct FP64 masses, {steps - left}/{steps} steps "
+            f"({time.perf_counter() - t0:.0f} s)")
+    e_d = energy(direct.x, direct.v)
+    d_pm = abs(e_pm - e0) / abs(e0)
+    d_d = abs(e_d - e0) / abs(e0)
+    return {"steps": steps, "dt": dt, "E0": e0,
+            "energy_drift_pm_learned": d_pm, "energy_drift_direct_f64": d_d,
+            "energy_drift_ratio": d_pm / max(d_d, 1e-300),
+            "force_err": errs,
+            "force_err_median_max": max(e["median"] for e in errs),
+            "pm_run_s": pm_s, "direct_run_s": time.perf_counter() - t0,
+            "energy": "exact FP64 sum over the masses (ssim_energy)"}
 
 
 def main():
@@ -289,9 +401,10 @@ def main():
     n_mesh = sf.n
     pass_bytes, x3_bytes = conv_bytes(n_mesh)
     mlp_flops = 2 * nb * (64 * 32 + 2 * 64 * 64 + 16 * 64)
+    alg = algorithmic_bytes(n, t, n_mesh)
     work = {
-        "k_pp_cells": ("valu", FLOPS_PER_NEAR_PAIR * inside, FP32_PEAK_TFLOPS,
-                       "TFLOP/s", f"{FLOPS_PER_NEAR_PAIR} flop x {inside:.4g} "
+        "k_pp_cells": ("valu", FLOPS_PER_PAIR * inside, FP32_PEAK_TFLOPS,
+                       "TFLOP/s", f"{FLOPS_PER_PAIR} flop x {inside:.4g} "
                        f"pairs inside rcut ({tested:.4g} pairs tested)"),
         "k_fft_pass": ("hbm", sum(pass_bytes) / len(pass_bytes), HBM_PEAK_GBS,
                        "GB/s", f"{sum(pass_bytes) / 1e6:.1f} MB over the 4 pruned "
@@ -299,20 +412,21 @@ def main():
         "k_conv_x3": ("hbm", x3_bytes, HBM_PEAK_GBS, "GB/s",
                       f"{x3_bytes / 1e6:.1f} MB: x < {n_mesh} half in, 3 kernel "
                       f"spectrum corners in, 3 x < {n_mesh} halves out"),
-        "k_tsc_deposit": ("hbm", n * 32 + 27 * 4 * n, HBM_PEAK_GBS, "GB/s",
-                          "32 B/mass in + 27 FP32 node updates"),
-        "k_tsc_gather": ("hbm", nb * (24 + 24) + 3 * 4 * n_mesh ** 3,
-                         HBM_PEAK_GBS, "GB/s",
-                         "48 B/body (position in, acceleration out) + 3 meshes"),
-        "k_cic_deposit": ("hbm", n * 32 + 8 * 4 * n, HBM_PEAK_GBS, "GB/s",
-                          "32 B/mass in + 8 FP32 node updates"),
-        "k_cic_gather": ("hbm", nb * (24 + 24) + 3 * 4 * n_mesh ** 3,
-                         HBM_PEAK_GBS, "GB/s",
-                         "48 B/body (position in, acceleration out) + 3 meshes"),
+        "k_tsc_deposit": ("hbm", alg["k_tsc_deposit"], HBM_PEAK_GBS, "GB/s",
+                          "N (3 x 8 B position + 8 B mass) + n^3 x 4 B mesh"),
+        "k_tsc_gather": ("hbm", alg["k_tsc_gather"], HBM_PEAK_GBS, "GB/s",
+                         "N x 3 x 8 B position + 3 n^3 x 4 B meshes + N x 3 x 8 B "
+                         "acceleration"),
+        "k_cic_deposit": ("hbm", alg["k_tsc_deposit"], HBM_PEAK_GBS, "GB/s",
+                          "N (3 x 8 B position + 8 B mass) + n^3 x 4 B mesh"),
+        "k_cic_gather": ("hbm", alg["k_tsc_gather"], HBM_PEAK_GBS, "GB/s",
+                         "N x 3 x 8 B position + 3 n^3 x 4 B meshes + N x 3 x 8 B "
+                         "acceleration"),
         "k_mlp_forward": ("mfma", mlp_flops, BF16_PEAK_TFLOPS, "TFLOP/s",
                           "2 x 11,264 MAC per body, BF16"),
-        "k_mlp_features": ("hbm", nb * (24 + 64), HBM_PEAK_GBS, "GB/s",
-                           "24 B position in + 64 B feature row out per body"),
+        "k_mlp_features": ("hbm", alg["k_mlp_features"], HBM_PEAK_GBS, "GB/s",
+                           "24 B position in + 64 B feature row out per body + "
+                           "n^3 x 4 B mesh"),
     }
     kernels = {}
     for k, (tot_ms, cnt) in times.items():
@@ -340,6 +454,15 @@ def main():
             "peak": d["peak"], "unit": d["unit"], "frac": d["frac"],
             "traffic": traffic, "launch_us": d["avg_us"], "work": d["work"]}
 
+    gate = None
+    if not args.no_gates and not args.small:
+        fast.close()
+        gate = gates(dev, pos, m, tpos, net, eps2, dt, args.gate_steps)
+        log(f"gates: energy drift PM + learned {gate['energy_drift_pm_learned']:.3e}, "
+            f"direct {gate['energy_drift_direct_f64']:.3e} "
+            f"(ratio {gate['energy_drift_ratio']:.2f}), worst median force error "
+            f"{gate['force_err_median_max']:.3e}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pos, m, tpos, eps2)
@@ -352,9 +475,10 @@ def main():
         "value": value, "unit": "body-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "precision": "f64 state; f32 mesh and near field; bf16 MLP with f32 "
-                     "accumulation; direct run all f64",
+        "dtype": "f32", "data": "synthetic",
+        "precision": "force arithmetic f32 (mesh, FFT, near-field pair terms); "
+                     "f64 state, kicks, drifts and energy; bf16 MLP with f32 "
+                     "accumulation; the direct run is f64 throughout",
         "config": {"workload": f"{n} masses + {t} tracers, perturbed unit cube, "
                                f"eps=1e-3, dt={dt:g}, leapfrog KDK, PM "
                                f"{n_mesh}^3 isolated + near field + learned "
@@ -368,12 +492,15 @@ def main():
         "force_err_median": float(np.median(err_l)),
         "force_err_p90": float(np.percentile(err_l, 90)),
         "force_err_median_without_mlp": float(np.median(err_p)),
+        "gates": gate,
+        "energy_drift_ratio": None if gate is None else gate["energy_drift_ratio"],
         "kernels": kernels,
         "roofline": roof,
         "roofline_direct": roof_direct,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    D.shutdown()
 
 
 if __name__ == "__main__":

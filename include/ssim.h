@@ -112,6 +112,13 @@ int ssim_kick(int dim, int n, double* vx, double* vy, double* vz,
 int ssim_drift(int dim, int n, double* x, double* y, double* z,
                const double* vx, const double* vy, const double* vz, double h,
                void* stream);
+/* Drift in the periodic box [o, o + box)^dim: x += v*h, then every
+ * coordinate is wrapped as x - box floor((x - o) / box) (unchanged while
+ * inside).  The periodic particle-mesh integrator's drift; no reference
+ * counterpart (the reference's PM is isolated, pm_kernel.py:166-293). */
+int ssim_drift_periodic(int dim, int n, double* x, double* y, double* z,
+                        const double* vx, const double* vy, const double* vz, double h,
+                        const double* origin, double box, void* stream);
 /* v += a*half then x += v*full (the opening half of a KDK step) in one
  * pass, with the rounding of ssim_kick followed by ssim_drift. */
 int ssim_kick_drift(int dim, int n, double* x, double* y, double* z, double* vx,

@@ -7,6 +7,9 @@
   — pm_kernel.py:34-163, for 2-D and 3-D.
 * `periodic_field` — the periodic Poisson flavour: rfftn, -4 pi G rho_k/k^2,
   irfftn, central difference.
+* `periodic_kdk` — KDK / Yoshida sub-steps on `periodic_accelerations`
+  (the reference's NBodyPM.step_batch order, pm_kernel.py:254-281, with
+  the drift wrapped into the box).
 * `energie_impuls` — test_pm.py:103-118.
 
 Pinned by tests/test_pm_oracle.py: mass conservation, brute-force CIC
@@ -136,6 +139,26 @@ def periodic_accelerations(pos, m, targets, n, box, origin, G=1.0):
     fields = periodic_field(pos, m, n, box, origin, G)
     h = box / n
     return [cic_gather(f, targets, [n] * len(pos), origin, h, True) for f in fields]
+
+
+def periodic_kdk(pos, vel, m, n, box, origin, G, dt, steps, weights=(1.0,)):
+    """`steps` KDK steps in the periodic box (each a composition of
+    sub-steps of the given weights: v += a w dt/2; x += v w dt wrapped as
+    x - box floor((x - o)/box); a = periodic_accelerations; v += a w dt/2).
+    Returns (pos, vel) lists of float64 arrays."""
+    x = [np.array(c, np.float64) for c in pos]
+    v = [np.array(c, np.float64) for c in vel]
+    a = periodic_accelerations(x, m, x, n, box, origin, G)
+    for _ in range(steps):
+        for w in weights:
+            full = w * dt
+            half = 0.5 * full
+            v = [v[c] + a[c] * half for c in range(3)]
+            x = [x[c] + v[c] * full for c in range(3)]
+            x = [x[c] - box * np.floor((x[c] - origin[c]) / box) for c in range(3)]
+            a = periodic_accelerations(x, m, x, n, box, origin, G)
+            v = [v[c] + a[c] * half for c in range(3)]
+    return x, v
 
 
 def energie_impuls(x, y, vx, vy, m, eps2, G=G_AU):

@@ -46,6 +46,8 @@ SIGNATURES = {
                                   _P, c_longlong, _P, c_size_t, _P]),
     "ssim_kick": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, c_double, _P]),
     "ssim_drift": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, c_double, _P]),
+    "ssim_drift_periodic": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, c_double,
+                                    _P, c_double, _P]),
     "ssim_kick_drift": (c_int, [c_int, c_int] + [_P] * 9 + [c_double, c_double, _P]),
     "ssim_kick_snap": (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P, _P,
                                _P, _P, c_double, _P, _P]),

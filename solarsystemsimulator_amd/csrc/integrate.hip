@@ -104,6 +104,26 @@ __global__ __launch_bounds__(EW_BLOCK) void k_kick_snap(
   }
 }
 
+// Drift in a periodic box: x += v*h, then x -= L floor((x - o) / L) on each
+// axis (exactly x while it stays inside [o, o + L)).
+template <int D>
+__global__ __launch_bounds__(EW_BLOCK) void k_drift_wrap(
+    int n, double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
+    const double* __restrict__ vx, const double* __restrict__ vy,
+    const double* __restrict__ vz, double h, double o0, double o1, double o2,
+    double L) {
+  const int i = blockIdx.x * EW_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const double nx = x[i] + vx[i] * h;
+  const double ny = y[i] + vy[i] * h;
+  x[i] = nx - L * floor((nx - o0) / L);
+  y[i] = ny - L * floor((ny - o1) / L);
+  if (D == 3) {
+    const double nz = z[i] + vz[i] * h;
+    z[i] = nz - L * floor((nz - o2) / L);
+  }
+}
+
 __global__ __launch_bounds__(EW_BLOCK) void k_fill_f32(long long n,
                                                        float* __restrict__ d,
                                                        float v) {
@@ -298,6 +318,27 @@ int ssim_drift(int dim, int n, double* x, double* y, double* z,
                const double* vx, const double* vy, const double* vz, double h,
                void* stream) {
   return ssim::drift(dim, n, x, y, z, vx, vy, vz, h, ssim::as_stream(stream));
+}
+
+int ssim_drift_periodic(int dim, int n, double* x, double* y, double* z,
+                        const double* vx, const double* vy, const double* vz, double h,
+                        const double* origin, double box, void* stream) {
+  SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3, got %d", dim);
+  SSIM_REQUIRE(n >= 0, "n must be >= 0");
+  if (n == 0) return SSIM_OK;
+  SSIM_REQUIRE(x && y && vx && vy && origin && (dim == 2 || (z && vz)), "null pointer");
+  SSIM_REQUIRE(box > 0.0, "box edge must be > 0, got %g", box);
+  hipStream_t st = ssim::as_stream(stream);
+  const dim3 grid(ssim::ceil_div(n, EW_BLOCK));
+  const double o2 = dim == 3 ? origin[2] : 0.0;
+  if (dim == 2)
+    hipLaunchKernelGGL(k_drift_wrap<2>, grid, dim3(EW_BLOCK), 0, st, n, x, y, z, vx,
+                       vy, vz, h, origin[0], origin[1], o2, box);
+  else
+    hipLaunchKernelGGL(k_drift_wrap<3>, grid, dim3(EW_BLOCK), 0, st, n, x, y, z, vx,
+                       vy, vz, h, origin[0], origin[1], o2, box);
+  SSIM_LAUNCHED("k_drift_wrap");
+  return SSIM_OK;
 }
 
 int ssim_kick_drift(int dim, int n, double* x, double* y, double* z, double* vx,

@@ -7,9 +7,46 @@ host fallback.
 """
 from __future__ import annotations
 
+import atexit
+import gc
+import sys
+import weakref
+
 import numpy as np
 
 from . import _lib
+
+# Objects whose close() releases GPU resources that must go while the HIP
+# runtime is still up (captured graphs, kernel-written pinned buffers).
+_closables: "weakref.WeakSet" = weakref.WeakSet()
+
+
+def on_shutdown(obj) -> None:
+    """Have obj.close() run by shutdown() (at the latest at interpreter
+    exit, before modules and the HIP runtime are torn down)."""
+    _closables.add(obj)
+
+
+def shutdown() -> None:
+    """Release the library's GPU state in a defined order: wait for the
+    device, close every registered object (graph executables, pinned host
+    stages, workspaces), collect what they referenced, wait again and hand
+    the allocator's cached blocks back.  Safe to call more than once; also
+    registered with atexit."""
+    torch = sys.modules.get("torch")
+    live = torch is not None and torch.cuda.is_initialized()
+    if live:
+        torch.cuda.synchronize()
+    for obj in list(_closables):
+        obj.close()
+    _closables.clear()
+    gc.collect()
+    if live:
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+
+
+atexit.register(shutdown)
 
 
 def require_gpu(device: int = 0):

@@ -70,6 +70,23 @@ class PMLearnedSystem:
         self.m = torch.zeros(max(self.n, 1), dtype=torch.float64, device=dev)
         self.feat = None
         self.acc_valid = False
+        D.on_shutdown(self)
+
+    def close(self):
+        """Release the captured step graph, the side streams, the pinned box
+        buffer (which a kernel writes) and the workspaces, after the device
+        has finished with them.  The system can run again afterwards (it
+        re-creates them)."""
+        torch = _torch()
+        if torch.cuda.is_initialized():
+            torch.cuda.synchronize(self.dev)
+        g = getattr(self, "_graph", None)
+        if g is not None:
+            g.reset()
+        self._graph, self._graph_key = None, None
+        self._box_dev = self._box_host = self._box_ws = self._kb_ws = None
+        self._box_ev = None
+        self.split._side = None
 
     def set_bodies(self, pos, vel, mass, tpos=None, tvel=None):
         for c in range(3):

@@ -199,7 +199,9 @@ class NBodyCuda:
             out = slots.copy()
         st["sub"] = None
         st["hits"] = self._collect_hits(st)
-        return out, tot.numpy() > sub_max
+        # no index field at all (sub_max 0): nothing overflows, as in
+        # schreibe_slot's `not sub_max` branch (film_producer.py:107-109)
+        return out, (tot.numpy() > sub_max) & (sub_max > 0)
 
     def _run_frames(self, st: dict, dt_raster_years: float, k: int) -> None:
         na, m, n = st["n_ast"], st["m"], st["N"]

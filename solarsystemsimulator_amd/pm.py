@@ -26,6 +26,7 @@ from .constants import G_AU
 
 F64 = _lib.PREC_F64
 F32 = _lib.PREC_F32C   # FP32 mesh
+MAX_FFT_LINE = 4096    # longest FFT line (points) of ssim_fft_*
 
 
 def _addr(a):
@@ -274,10 +275,15 @@ class NBodyPM:
         self.kraft_f32 = bool(kraft_f32)
         self._kernels = None
         self._kernel_h = None
+        # the reference's default grid_n = 4096 (pm_kernel.py:190) pads to
+        # 8192-point lines; the FFT here handles lines of up to 4096 points,
+        # so grid_n is limited to 2048 — rejected before any device work
+        if not (2 <= self.grid_n and 2 * self.grid_n <= MAX_FFT_LINE
+                and self.grid_n & (self.grid_n - 1) == 0):
+            raise ValueError(f"grid_n={grid_n}: NBodyPM needs a power of two with the "
+                             f"padded mesh 2N <= {MAX_FFT_LINE} (grid_n <= "
+                             f"{MAX_FFT_LINE // 2}); pass grid_n explicitly")
         self._dev = D.require_gpu(self.device)
-        if 2 * self.grid_n > 4096:
-            raise ValueError(f"grid_n={grid_n}: the padded mesh 2N must be "
-                             "<= 4096 in this build")
 
     def name(self) -> str:
         return _torch().cuda.get_device_name(self._dev) + " (PM)"
@@ -384,6 +390,80 @@ class PeriodicPM3D:
         self.field(pos, m)
         return gather(self.g, targets, self.grid, self.origin, self.h, True,
                       self.prec, outs)
+
+
+class PeriodicPMSystem:
+    """n masses in the periodic box [origin, origin + box)^3 advanced with
+    leapfrog KDK (scheme SCHEME_KDK) or Yoshida-4 (SCHEME_YOSHIDA4: three
+    KDK sub-steps weighted w1, w0, w1) on PeriodicPM3D forces, FP64 state,
+    mesh in FP64 or FP32.  A sub-step of weight w is: v += a w dt/2;
+    x += v w dt wrapped into the box (ssim_drift_periodic); a = PM force;
+    v += a w dt/2 — the KDK of the reference's NBodyPM.step_batch
+    (pm_kernel.py:254-281) in a periodic box."""
+
+    def __init__(self, dev, n_mesh: int, n: int, box: float = 1.0,
+                 origin=(0.0, 0.0, 0.0), G: float = 1.0, prec: int = F64,
+                 scheme: int = _lib.SCHEME_KDK):
+        torch = _torch()
+        if scheme not in (_lib.SCHEME_KDK, _lib.SCHEME_YOSHIDA4):
+            raise ValueError(f"unknown scheme {scheme}")
+        if n < 1:
+            raise ValueError("leerer Zustand")
+        self.dev, self.n, self.box, self.scheme = dev, int(n), float(box), scheme
+        self.solver = PeriodicPM3D(dev, n_mesh, box, origin, G, prec)
+        self._org = _lib.double_array(self.solver.origin)
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.x = [torch.zeros(self.n, **f64) for _ in range(3)]
+        self.v = [torch.zeros(self.n, **f64) for _ in range(3)]
+        self.a = [torch.zeros(self.n, **f64) for _ in range(3)]
+        self.m = torch.zeros(self.n, **f64)
+        self.acc_valid = False
+
+    def set_bodies(self, pos, vel, m):
+        for c in range(3):
+            self.x[c].copy_(D.to_dev(pos[c], self.dev))
+            self.v[c].copy_(D.to_dev(vel[c], self.dev))
+        self.m.copy_(D.to_dev(m, self.dev))
+        self.acc_valid = False
+
+    def accelerations(self):
+        """a = PM acceleration at the current positions."""
+        self.solver.accelerations(self.x, self.m, self.x, outs=self.a)
+        self.acc_valid = True
+        return self.a
+
+    def weights(self):
+        if self.scheme == _lib.SCHEME_KDK:
+            return (1.0,)
+        from .constants import YOSHIDA_W0, YOSHIDA_W1
+        return (YOSHIDA_W1, YOSHIDA_W0, YOSHIDA_W1)
+
+    def run(self, dt: float, steps: int):
+        if steps < 1:
+            raise ValueError(f"steps muss >= 1 sein: {steps}")
+        if not self.acc_valid:
+            self.accelerations()
+        st = D.stream(self.dev)
+        ps = lambda lst: [_lib.ptr(c) for c in lst]  # noqa: E731
+        for _ in range(steps):
+            for w in self.weights():
+                full = w * dt
+                half = 0.5 * full
+                _lib.call("ssim_kick", 3, self.n, *ps(self.v), *ps(self.a), half, st)
+                _lib.call("ssim_drift_periodic", 3, self.n, *ps(self.x), *ps(self.v),
+                          full, _addr(self._org), self.box, st)
+                self.accelerations()
+                _lib.call("ssim_kick", 3, self.n, *ps(self.v), *ps(self.a), half, st)
+
+    def potential(self):
+        """Mesh potential phi gathered at the bodies with the same CIC
+        stencil (device float64 array); E_pot = 1/2 sum m phi."""
+        s = self.solver
+        s.field(self.x, self.m)
+        return gather([s.phi], self.x, s.grid, s.origin, s.h, True, s.prec)[0]
+
+    def host_state(self):
+        return ([c.cpu().numpy() for c in self.x], [c.cpu().numpy() for c in self.v])
 
 
 class IsolatedPM3D:

@@ -149,6 +149,11 @@ class LiveSession:
             typ, _n, dt_years = HEADER.unpack_from(message, 0)
             if typ in FILM_TYPES:
                 if self.film is None:
+                    # FILM_STOP without an active film is ignored, with no
+                    # reply (server.py:1195-1209); FILM_SUB / FILM_START
+                    # report that no film runs
+                    if typ == MSG_FILM_STOP:
+                        return None
                     raise ValueError(f"Filmmodus nicht verfuegbar (Nachrichtentyp {typ})")
                 return self.film(message)
             sim = self.sim()

@@ -143,3 +143,46 @@ def test_periodic_poisson_single_mode(dim):
     np.testing.assert_allclose(f[0], expect, atol=1e-12 * np.abs(expect).max())
     for c in range(1, dim):
         assert np.abs(f[c]).max() < 1e-12 * np.abs(expect).max()
+
+
+def test_periodic_kdk_reversible_and_second_order():
+    """The periodic KDK restatement is time-reversible (a wrong kick weight
+    or order breaks that), conserves momentum, and its error falls ~4x when
+    dt halves (second order); Yoshida weights give a much smaller error."""
+    rng = np.random.default_rng(5)
+    npart, n = 800, 16
+    pos = [rng.uniform(0.0, 1.0, npart) for _ in range(3)]
+    vel = [rng.normal(0.0, 0.05, npart) for _ in range(3)]
+    m = np.full(npart, 1.0 / npart)
+    o = [0.0, 0.0, 0.0]
+    x1, v1 = pm_np.periodic_kdk(pos, vel, m, n, 1.0, o, 1.0, 1e-2, 4)
+    x2, v2 = pm_np.periodic_kdk(x1, [-c for c in v1], m, n, 1.0, o, 1.0, 1e-2, 4)
+    dx = np.array(x2) - np.array(pos)
+    dx -= np.round(dx)
+    assert np.abs(dx).max() < 1e-12
+    assert np.abs(np.array(v2) + np.array(vel)).max() < 1e-12
+    p0 = np.sum(m * np.array(vel), axis=1)
+    assert np.abs(np.sum(m * np.array(v1), axis=1) - p0).max() < 1e-12
+    ref, _ = pm_np.periodic_kdk(pos, vel, m, n, 1.0, o, 1.0, 0.02 / 64, 64)
+
+    def err(dt, k, w=(1.0,)):
+        x, _ = pm_np.periodic_kdk(pos, vel, m, n, 1.0, o, 1.0, dt, k, w)
+        d = np.array(x) - np.array(ref)
+        return np.abs(d - np.round(d)).max()
+    e1, e2 = err(0.01, 2), err(0.005, 4)
+    assert 3.0 < e1 / e2 < 5.0
+    w1 = 1.0 / (2.0 - 2.0 ** (1.0 / 3.0))
+    w0 = -(2.0 ** (1.0 / 3.0)) / (2.0 - 2.0 ** (1.0 / 3.0))
+    assert err(0.01, 2, (w1, w0, w1)) < 0.2 * e1
+
+
+def test_nbodypm_default_grid_is_a_clear_error():
+    """NBodyPM() with the reference's default grid_n = 4096 (8192-point
+    padded lines) raises a ValueError that names the limit, before any
+    device work; grid_n must be a power of two up to 2048."""
+    import pytest
+    from solarsystemsimulator_amd import pm as P
+    with pytest.raises(ValueError, match="grid_n <= 2048"):
+        P.NBodyPM([0])
+    with pytest.raises(ValueError, match="power of two"):
+        P.NBodyPM([0], grid_n=1000)

@@ -61,13 +61,21 @@ def _scene(n_ast, seed=11):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n_ast,m_sub,sub_max,k", [(3000, 8, 5, 4), (3000, 8, 2048, 3),
-                                                   (200, 0, 16, 2), (0, 8, 16, 2),
-                                                   (20000, 32, 7, 16)])
-def test_gpu_slots_match_host_route(dev, n_ast, m_sub, sub_max, k):
+@pytest.mark.parametrize("n_ast,m_sub,sub_max,k,massive", [
+    (3000, 8, 5, 4, True), (3000, 8, 2048, 3, True), (200, 0, 16, 2, True),
+    (0, 8, 16, 2, True), (20000, 32, 7, 16, True),
+    # sub_max 0 with tracks due: count field 0, no overflow (schreibe_slot's
+    # `not sub_max` branch)
+    (500, 8, 0, 2, True),
+    # no massive bodies at all: the massive rows and index map are empty
+    (400, 8, 16, 2, False)])
+def test_gpu_slots_match_host_route(dev, n_ast, m_sub, sub_max, k, massive):
     from solarsystemsimulator_amd.nbody import NBodyCuda
-    print(f"ring slots n_ast={n_ast} m_sub={m_sub} sub_max={sub_max} k={k}", flush=True)
+    print(f"ring slots n_ast={n_ast} m_sub={m_sub} sub_max={sub_max} k={k} "
+          f"massive={massive}", flush=True)
     st0, rr = _scene(n_ast)
+    if not massive:                     # every body an asteroid
+        st0[6] = np.ones_like(st0[6])
     # a 5-day raster: above the 0.5-day largest sub-step, so the divers get
     # complete fine tracks (a 0.5-day raster would need no sub-steps)
     raster = 5 / 365.25
@@ -95,8 +103,10 @@ def test_gpu_slots_match_host_route(dev, n_ast, m_sub, sub_max, k):
         for a, b in zip(sa["hits"], sb_["hits"]):
             assert np.array_equal(a, b)
         seen += sum(nhs)
-    if n_ast >= 40 and m_sub:
+    if n_ast >= 40 and m_sub and sub_max and massive:
         assert seen > 0                                      # the case has tracks
+    if not sub_max:
+        assert seen == 0
     assert not ring[0].any()                                 # nothing outside its slots
     assert np.array_equal(host_sim.export_f64(sa), dev_sim.export_f64(sb_))
 

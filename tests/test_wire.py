@@ -102,6 +102,10 @@ def test_session_transitions_and_errors():
     assert "unbekannter Nachrichtentyp 9" in W.parse_response(
         s.handle(struct.pack("<IId", 9, 0, 0.0)))[1]
     assert "Filmmodus" in W.parse_response(s.handle(struct.pack("<IId", 3, 0, 0.0)))[1]
+    assert "Filmmodus" in W.parse_response(s.handle(struct.pack("<IId", 7, 0, 0.0)))[1]
+    # FILM_STOP with no film running is ignored without a reply
+    # (server.py:1195-1209), so it cannot be taken for the next STEP's reply
+    assert s.handle(struct.pack("<IId", W.MSG_FILM_STOP, 0, 0.0)) is None
     assert "Protokollfehler" in W.parse_response(
         s.handle(W.build_full(0.5, x, y, vx, vy, m, vis, ast)[:-1]))[1]
     assert sim.calls == ["load", ("step", 0.5), ("upd", [2]), ("step", 0.25)]
