@@ -410,11 +410,12 @@ int ssim_cells_build(int n, const double* x, const double* y, const double* z,
                      const double* m, const double* origin, double cell,
                      const int* nc, int* cell_start, int* order, void* packed,
                      void* work, size_t work_bytes, void* stream);
-/* The same binning without the fixed order inside a cell (slots handed
- * out by atomics): for lists whose consumer does not depend on the order,
- * such as the near-field targets.  A cell's bodies come grouped by the
- * octant of the cell they lie in (x, then y, then z halves), so any run of
- * consecutive bodies of a cell is spatially compact.  No hipMemset inside. */
+/* The same binning with the bodies of a cell grouped by the octant of the
+ * cell they lie in (x, then y, then z halves), so any run of consecutive
+ * bodies of a cell is spatially compact; ascending body index inside an
+ * octant (a stable radix sort of cell * 8 + octant, so the lists, and what
+ * is computed from them, do not change from run to run).  For the
+ * near-field targets.  No hipMemset inside. */
 size_t ssim_cells_bin_work_bytes(int n, int ncells);
 int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
                    const double* origin, double cell, const int* nc,
@@ -431,7 +432,9 @@ int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
  * starts, for ssim_pp_short), order[n] (mass indices in bin order),
  * packed[n] float4 (x, y, z relative to o - h/2, m) for ssim_pp_short and
  * body[n] double4 ((x - o)/h per axis, m) for ssim_tsc_deposit_blocked and
- * ssim_tsc_deposit_tiles.  With bin_start = NULL only the blocks are binned
+ * ssim_tsc_deposit_tiles, and (near != NULL) near[n] float4 (x, y, z
+ * relative to the corner of the mass's own block, m), the precise copy
+ * ssim_pp_short's pair terms read.  With bin_start = NULL only the blocks are binned
  * (key = block index, stable order, order inside a block by index): all
  * that ssim_pp_short and ssim_tsc_deposit_tiles read.  It replaces the two
  * cell lists (nodes and near-field cells) of ssim_cells_build.  No
@@ -440,7 +443,7 @@ size_t ssim_split_bin_work_bytes(int n, int grid_n, int block);
 int ssim_split_bin(int n, const double* x, const double* y, const double* z,
                    const double* m, int grid_n, const double* origin, double h,
                    int block, int* bin_start, int* cell_start, int* order, void* packed,
-                   void* body, void* work, size_t work_bytes, void* stream);
+                   void* body, void* near, void* work, size_t work_bytes, void* stream);
 /* ADD (accumulate != 0) or STORE the near-field force G m d g(r) /
  * (r^2 + eps2)^(3/2), r < rcut, of the binned sources (cell_start +
  * packed, from ssim_cells_build or ssim_split_bin) to / into nt targets'
@@ -457,6 +460,19 @@ int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,
                   double eps2, double rs, double rcut, double* ax, double* ay,
                   double* az, int accumulate, void* work, size_t work_bytes,
                   void* stream);
+/* ssim_pp_short for masses binned by ssim_split_bin, on their
+ * block-relative copy `near` (cell_start, near from ssim_split_bin; targets
+ * from ssim_cells_bin on the same grid): each of the 27 cells is walked
+ * with the target re-expressed relative to that cell's corner, so the FP32
+ * pair terms keep the positions' low bits; the terms of a cell are summed
+ * in FP32 and the cells' partials in FP64.  Deterministic.  Work and the
+ * STORE / ADD semantics as ssim_pp_short.  New: the reference has no split
+ * force. */
+int ssim_pp_near(int nt, const double* tx, const double* ty, const double* tz,
+                 const double* origin, double cell, const int* nc, const int* cell_start,
+                 const void* near, const int* tcell_start, const int* torder, double G,
+                 double eps2, double rs, double rcut, double* ax, double* ay, double* az,
+                 int accumulate, void* work, size_t work_bytes, void* stream);
 /* Work accounting of ssim_pp_short with the same arguments: counts[0] =
  * source-target pairs tested, counts[1] = pairs closer than rcut (a body
  * with itself excluded); counts is a device array of 2, no forces. */

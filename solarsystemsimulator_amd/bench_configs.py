@@ -111,34 +111,24 @@ def allpairs_65k(steps):
 
 
 def periodic_pm(steps):
-    import torch
     from . import pm as P
     dev = D.require_gpu(0)
     pos = scenes.perturbed_cube(64, 64, 64, seed=3, periodic=True)
     n = pos[0].size
-    m = D.to_dev(np.full(n, 1.0 / n), dev)
+    m = np.full(n, 1.0 / n)
+    zeros = [np.zeros(n)] * 3
     out = {"bodies": n, "mesh": 128, "steps": steps}
     for name, prec in (("f64", P.F64), ("f32", P.F32)):
-        x = [D.to_dev(c, dev) for c in pos]
-        v = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3)]
-        a = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3)]
-        solver = P.PeriodicPM3D(dev, 128, prec=prec)
-        solver.accelerations(x, m, x, outs=a)
-        t_eval, _ = timed(lambda: solver.accelerations(x, m, x, outs=a), 5)
-        st = D.stream(dev)
-        ps = lambda lst: [_lib.ptr(c) for c in lst]  # noqa: E731
-
-        def step(dt=1e-3):
-            _lib.call("ssim_kick", 3, n, *ps(v), *ps(a), 0.5 * dt, st)
-            _lib.call("ssim_drift", 3, n, *ps(x), *ps(v), dt, st)
-            for c in range(3):                              # periodic wrap
-                torch.remainder(x[c], 1.0, out=x[c])
-            solver.accelerations(x, m, x, outs=a)
-            _lib.call("ssim_kick", 3, n, *ps(v), *ps(a), 0.5 * dt, st)
-        step()                                              # warm-up
-        t_step, _ = timed(lambda: [step() for _ in range(steps)])
-        out[f"ms_per_eval_{name}"] = t_eval * 1e3
-        out[f"ms_per_step_{name}"] = t_step / steps * 1e3
+        for sname, scheme in (("kdk", _lib.SCHEME_KDK), ("yoshida4", _lib.SCHEME_YOSHIDA4)):
+            s = P.PeriodicPMSystem(dev, 128, n, prec=prec, scheme=scheme)
+            s.set_bodies(pos, zeros, m)
+            s.accelerations()
+            if sname == "kdk":
+                t_eval, _ = timed(s.accelerations, 5)
+                out[f"ms_per_eval_{name}"] = t_eval * 1e3
+            s.run(1e-3, 1)                                  # warm-up
+            t_step, _ = timed(lambda: s.run(1e-3, steps))
+            out[f"ms_per_step_{sname}_{name}"] = t_step / steps * 1e3
     return out
 
 
@@ -195,6 +185,8 @@ def collisions_50k(frames):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--only", action="append", default=None,
+                    help="run only this configuration (repeatable)")
     args = ap.parse_args(argv)
     q = args.quick
     res = {}
@@ -203,10 +195,13 @@ def main(argv=None):
                           ("periodic_pm_262k", periodic_pm, 5 if q else 50),
                           ("collisions_50k", collisions_50k, 32 if q else 160),
                           ("ring_slots_50k", ring_slots_50k, 3 if q else 10)):
+        if args.only and name not in args.only:
+            continue
         log(f"== {name}")
         res[name] = fn(arg)
         log(json.dumps(res[name]))
-    print(json.dumps(res))
+    print(json.dumps(res), flush=True)
+    D.shutdown()
 
 
 if __name__ == "__main__":

@@ -42,8 +42,9 @@ __global__ __launch_bounds__(CB) void k_cell_key(
 }
 
 // Key of the target binning: cell * 8 + octant of the body inside its cell
-// (bit 2: upper half in x, bit 1: in y, bit 0: in z), so the unstable
-// placement below leaves each cell's targets grouped by octant.
+// (bit 2: upper half in x, bit 1: in y, bit 0: in z); a stable sort of
+// these keys leaves each cell's targets grouped by octant, ascending body
+// index inside an octant.
 __device__ __forceinline__ int octant_bit(double f, int a) {
   return clampi((int)floor(2.0 * f) - 2 * a, 0, 1);
 }
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(CB) void k_cell_key8(
     int n, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, double ox, double oy, double oz,
     double inv_cell, int nc0, int nc1, int nc2, int* __restrict__ key,
-    int* __restrict__ count, int* __restrict__ rank) {
+    int* __restrict__ order) {
   const int i = blockIdx.x * CB + threadIdx.x;
   if (i >= n) return;
   const double fa = (x[i] - ox) * inv_cell, fb = (y[i] - oy) * inv_cell,
@@ -60,10 +61,16 @@ __global__ __launch_bounds__(CB) void k_cell_key8(
   const int a = clampi((int)floor(fa), 0, nc0 - 1);
   const int b = clampi((int)floor(fb), 0, nc1 - 1);
   const int c = clampi((int)floor(fc), 0, nc2 - 1);
-  const int k = ((a * nc1 + b) * nc2 + c) * 8 + octant_bit(fa, a) * 4 +
-                octant_bit(fb, b) * 2 + octant_bit(fc, c);
-  key[i] = k;
-  rank[i] = atomicAdd(&count[k], 1);  // the body's slot in its bin
+  key[i] = ((a * nc1 + b) * nc2 + c) * 8 + octant_bit(fa, a) * 4 +
+           octant_bit(fb, b) * 2 + octant_bit(fc, c);
+  order[i] = i;
+}
+
+// cell_start[c] = sub_start[8 c], c in [0, ncells]
+__global__ __launch_bounds__(CB) void k_every8(int ncells, const int* __restrict__ sub,
+                                               int* __restrict__ cell_start) {
+  const int c = blockIdx.x * CB + threadIdx.x;
+  if (c <= ncells) cell_start[c] = sub[8 * c];
 }
 
 // Split-force mass bins: the TSC stencil centre r = floor((x - o)/h + 1/2)
@@ -121,7 +128,8 @@ __global__ __launch_bounds__(CB) void k_split_pack(
     const double* __restrict__ y, const double* __restrict__ z,
     const double* __restrict__ m, double ox, double oy, double oz, double h,
     double cx, double cy, double cz, float4* __restrict__ packed,
-    double4* __restrict__ body) {
+    double4* __restrict__ body, const int* __restrict__ key, int key_div, int nb,
+    double cell, float4* __restrict__ near) {
   const int s = blockIdx.x * CB + threadIdx.x;
   if (s >= n) return;
   const int p = order[s];
@@ -129,16 +137,15 @@ __global__ __launch_bounds__(CB) void k_split_pack(
   packed[s] = make_float4((float)(px - cx), (float)(py - cy), (float)(pz - cz),
                           (float)pm);
   body[s] = make_double4((px - ox) / h, (py - oy) / h, (pz - oz) / h, pm);
-}
-
-// order[start[k] + rank] = i: the slots taken while counting
-__global__ __launch_bounds__(CB) void k_scatter_ranked(int n, const int* __restrict__ key,
-                                                       const int* __restrict__ rank,
-                                                       const int* __restrict__ start,
-                                                       int* __restrict__ order) {
-  const int i = blockIdx.x * CB + threadIdx.x;
-  if (i >= n) return;
-  order[start[key[i]] + rank[i]] = i;
+  if (near) {
+    // relative to the corner of the block the body is binned in (its sorted
+    // key), so the near field's differences lose no bits to the magnitude
+    // of the coordinates
+    const int blk = key[s] / key_div;
+    const int A = blk / (nb * nb), B = (blk / nb) % nb, C = blk % nb;
+    near[s] = make_float4((float)(px - (cx + A * cell)), (float)(py - (cy + B * cell)),
+                          (float)(pz - (cz + C * cell)), (float)pm);
+  }
 }
 
 __global__ __launch_bounds__(CB) void k_pack(
@@ -211,9 +218,8 @@ int ssim_cells_build(int n, const double* x, const double* y, const double* z,
 
 size_t ssim_cells_bin_work_bytes(int n, int ncells) {
   const size_t nsub = 8 * (size_t)(ncells > 0 ? ncells : 1);
-  return 2 * ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int)) +
-         ssim::align256(nsub * sizeof(int)) +
-         ssim::align256((nsub + 1) * sizeof(int)) + ssim::scan_work_bytes((int)nsub);
+  return ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int)) +
+         ssim::align256((nsub + 1) * sizeof(int)) + ssim::radix_sort_work_bytes(n);
 }
 
 int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
@@ -234,27 +240,27 @@ int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
   unsigned char* w = static_cast<unsigned char*>(work);
   int* key = reinterpret_cast<int*>(w);
   w += ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int));
-  int* count = reinterpret_cast<int*>(w);
-  w += ssim::align256((size_t)nsub * sizeof(int));
-  int* rank = reinterpret_cast<int*>(w);
-  w += ssim::align256((size_t)(n > 0 ? n : 1) * sizeof(int));
   int* sub_start = reinterpret_cast<int*>(w);
   w += ssim::align256(((size_t)nsub + 1) * sizeof(int));
-  const dim3 gc(ssim::ceil_div(nsub, CB));
-  hipLaunchKernelGGL(k_zero_i32, gc, dim3(CB), 0, st, nsub, count);
-  SSIM_LAUNCHED("k_zero_i32");
   if (n > 0) {
     hipLaunchKernelGGL(k_cell_key8, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
                        n, x, y, z, origin[0], origin[1], origin[2], 1.0 / cell,
-                       nc[0], nc[1], nc[2], key, count, rank);
+                       nc[0], nc[1], nc[2], key, order);
     SSIM_LAUNCHED("k_cell_key8");
+    int bits = 1;
+    while (bits < 31 && (1LL << bits) < nsub) ++bits;
+    SSIM_TRY(ssim::radix_sort_pairs(n, bits, key, order, w,
+                                    ssim::radix_sort_work_bytes(n), st));
   }
-  SSIM_TRY(ssim::scan_exclusive(nsub, count, sub_start, w, st, 8, cell_start));
-  if (n > 0) {
-    hipLaunchKernelGGL(k_scatter_ranked, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
-                       n, key, rank, sub_start, order);
-    SSIM_LAUNCHED("k_scatter_ranked");
-  }
+  // octant-bin starts from the sorted keys (n = 0: all zero), every 8th one
+  // is a cell start
+  const long long waves = n / 64 + 1;
+  hipLaunchKernelGGL(k_run_starts, dim3(ssim::ceil_div(waves * 64, CB)), dim3(CB), 0, st,
+                     n, key, nsub, sub_start);
+  SSIM_LAUNCHED("k_run_starts");
+  hipLaunchKernelGGL(k_every8, dim3(ssim::ceil_div(ncells + 1, CB)), dim3(CB), 0, st,
+                     ncells, sub_start, cell_start);
+  SSIM_LAUNCHED("k_every8");
   return SSIM_OK;
 }
 
@@ -275,7 +281,7 @@ size_t ssim_split_bin_work_bytes(int n, int grid_n, int block) {
 int ssim_split_bin(int n, const double* x, const double* y, const double* z,
                    const double* m, int grid_n, const double* origin, double h,
                    int block, int* bin_start, int* cell_start, int* order, void* packed,
-                   void* body, void* work, size_t work_bytes, void* stream) {
+                   void* body, void* near, void* work, size_t work_bytes, void* stream) {
   SSIM_REQUIRE(n >= 0, "n must be >= 0");
   SSIM_REQUIRE(grid_n >= 1 && block >= 1 && h > 0.0 && origin, "bad mesh");
   const long long K = split_bins(grid_n, block);
@@ -326,8 +332,8 @@ int ssim_split_bin(int n, const double* x, const double* y, const double* z,
     hipLaunchKernelGGL(k_split_pack, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st, n,
                        order, x, y, z, m, origin[0], origin[1], origin[2], h,
                        origin[0] - hh, origin[1] - hh, origin[2] - hh,
-                       static_cast<float4*>(packed),
-                       static_cast<double4*>(body));
+                       static_cast<float4*>(packed), static_cast<double4*>(body), key,
+                       nodes ? B3 : 1, nb, block * h, static_cast<float4*>(near));
     SSIM_LAUNCHED("k_split_pack");
   }
   return SSIM_OK;

@@ -257,6 +257,103 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
   }
 }
 
+
+// The near field on block-relative coordinates, for masses binned by
+// ssim_split_bin (which also writes `near`: each mass relative to the
+// corner of its own block, i.e. its cell).  Same chunks, cell walk and
+// wave-uniform source groups as k_pp_cells, but the walk goes cell by cell:
+// for cell k of the 27 the target is re-expressed relative to that cell's
+// corner (t_k = t_0 - offset_k, exact to ~1e-9 of the unit), so both
+// operands of every difference are < 2 cells and keep their low bits
+// (origin-relative FP32 coordinates lose ~6e-8 absolute, ~1e-5 of a pair
+// force at r ~ rcut/2).  The FP32 terms of one cell are summed in FP32 and
+// each cell's partial is added to FP64 sums (blocked compensated
+// summation).  Fixed source order per target: deterministic.
+__global__ __launch_bounds__(PB) void k_pp_near(
+    const double* __restrict__ tx, const double* __restrict__ ty,
+    const double* __restrict__ tz, double ox, double oy, double oz, double cellsz,
+    int nc0, int nc1, int nc2, const int* __restrict__ cell_start,
+    const float4* __restrict__ near, const int* __restrict__ tstart,
+    const int* __restrict__ torder, const int* __restrict__ cstart,
+    const int* __restrict__ cmap, int ncells, float eps2,
+    const float2* __restrict__ gtab, float tab_scale, float rcut2, double G,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    int accumulate) {
+  if (blockIdx.x * WAVES >= cstart[ncells]) return;
+  __shared__ float2 tab[GT + 1];
+  for (int i = threadIdx.x; i <= GT; i += PB) tab[i] = gtab[i];
+  __syncthreads();  // before any wave leaves
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + (threadIdx.x >> 6));
+  if (w >= cstart[ncells]) return;
+  const int cell = __builtin_amdgcn_readfirstlane(cmap[w]);
+  const int first = __builtin_amdgcn_readfirstlane(
+      tstart[cell] + (w - cstart[cell]) * CHUNK);
+  const int count = __builtin_amdgcn_readfirstlane(min(CHUNK, tstart[cell + 1] - first));
+  const bool live = lane < count;
+  const int tgt = live ? torder[first + lane] : 0;
+  const int a = cell / (nc1 * nc2), b = (cell / nc2) % nc1, c = cell % nc2;
+  const float csz = (float)cellsz;
+  // the target relative to its cell's corner; idle lanes sit far away, so
+  // no source is ever within rcut of them
+  const float t0x = live ? (float)(tx[tgt] - (ox + a * cellsz)) : 1e18f;
+  const float t0y = live ? (float)(ty[tgt] - (oy + b * cellsz)) : 1e18f;
+  const float t0z = live ? (float)(tz[tgt] - (oz + c * cellsz)) : 1e18f;
+  double hx = 0.0, hy = 0.0, hz = 0.0;
+  const int c0 = max(c - 1, 0), c1 = min(c + 1, nc2 - 1);
+  for (int da = max(a - 1, 0); da <= min(a + 1, nc0 - 1); ++da)
+    for (int db = max(b - 1, 0); db <= min(b + 1, nc1 - 1); ++db) {
+      const int row = (da * nc1 + db) * nc2;
+      const float px = t0x - (float)(da - a) * csz;
+      const float py = t0y - (float)(db - b) * csz;
+      for (int cc = c0; cc <= c1; ++cc) {
+        const float pz = t0z - (float)(cc - c) * csz;
+        const int s1 = cell_start[row + cc + 1];
+        float sx = 0.f, sy = 0.f, sz = 0.f;
+        int s = cell_start[row + cc];
+        for (; s + SG <= s1; s += SG) {
+          float4 q[SG];
+#pragma unroll
+          for (int j = 0; j < SG; ++j) q[j] = near[s + j];
+          float dx[SG], dy[SG], dz[SG], r2[SG];
+          unsigned long long any = 0;
+#pragma unroll
+          for (int j = 0; j < SG; ++j) {
+            dx[j] = q[j].x - px;
+            dy[j] = q[j].y - py;
+            dz[j] = q[j].z - pz;
+            r2[j] = fmaf(dx[j], dx[j], fmaf(dy[j], dy[j], dz[j] * dz[j]));
+            any |= __ballot(r2[j] < rcut2);
+          }
+          if (any == 0) continue;
+#pragma unroll
+          for (int j = 0; j < SG; ++j)
+            pp_term(q[j].w, dx[j], dy[j], dz[j], r2[j], tab, tab_scale, eps2, sx, sy, sz);
+        }
+        for (; s < s1; ++s) {
+          const float4 q = near[s];
+          const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
+          const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          if (r2 < rcut2) pp_term(q.w, dx, dy, dz, r2, tab, tab_scale, eps2, sx, sy, sz);
+        }
+        hx += (double)sx;
+        hy += (double)sy;
+        hz += (double)sz;
+      }
+    }
+  if (live) {
+    if (accumulate) {
+      ax[tgt] += G * hx;
+      ay[tgt] += G * hy;
+      az[tgt] += G * hz;
+    } else {
+      ax[tgt] = G * hx;
+      ay[tgt] = G * hy;
+      az[tgt] = G * hz;
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -277,27 +374,11 @@ __global__ void k_zero_u64(unsigned long long* c) {
   if (threadIdx.x < 2) c[threadIdx.x] = 0;
 }
 
-int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
-            const double* origin, double cell, const int* nc,
-            const int* cell_start, const void* packed, const int* tcell_start,
-            const int* torder, double G, double eps2, double rs, double rcut,
-            double* ax, double* ay, double* az, int accumulate,
-            unsigned long long* counts, void* work, size_t work_bytes, void* stream) {
-  SSIM_REQUIRE(nt >= 0, "nt must be >= 0");
-  if (nt == 0) return SSIM_OK;
-  SSIM_REQUIRE(tx && ty && tz && origin && nc && cell_start && packed &&
-                   tcell_start && torder && work && (counts || (ax && ay && az)),
-               "null pointer");
-  SSIM_REQUIRE(nc[0] >= 1 && nc[1] >= 1 && nc[2] >= 1, "bad cell counts");
-  const long long ncl = (long long)nc[0] * nc[1] * nc[2];
-  SSIM_REQUIRE(ncl <= (1 << 26), "too many cells: %lld", ncl);
-  const int ncells = (int)ncl;
-  SSIM_REQUIRE(rs > 0.0 && rcut > 0.0 && rcut <= cell,
-               "need rs > 0 and 0 < rcut <= cell (rs=%g rcut=%g cell=%g)", rs,
-               rcut, cell);
-  SSIM_REQUIRE(eps2 > 0.0, "softening eps2 must be > 0");
-  SSIM_REQUIRE(work_bytes >= ssim_pp_work_bytes(nt, ncells), "workspace too small");
-  hipStream_t st = ssim::as_stream(stream);
+// The table and the chunk plan in `work` (ssim_pp_work_bytes): gS table,
+// chunk starts per cell and the chunk -> cell map.
+int pp_plan(int nt, int ncells, const int* tcell_start, double rs, double rcut,
+            void* work, int** cstart_out, int** cmap_out, float2** gtab_out,
+            hipStream_t st) {
   unsigned char* w = static_cast<unsigned char*>(work);
   int* nchunk = reinterpret_cast<int*>(w);
   w += ssim::align256((size_t)ncells * sizeof(int));
@@ -323,6 +404,37 @@ int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
     hipLaunchKernelGGL(k_chunk_map, gc, dim3(PB), 0, st, ncells, cstart, cmap);
     SSIM_LAUNCHED("k_chunk_map");
   }
+  *cstart_out = cstart;
+  *cmap_out = cmap;
+  *gtab_out = gtab;
+  return SSIM_OK;
+}
+
+int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
+            const double* origin, double cell, const int* nc,
+            const int* cell_start, const void* packed, const int* tcell_start,
+            const int* torder, double G, double eps2, double rs, double rcut,
+            double* ax, double* ay, double* az, int accumulate,
+            unsigned long long* counts, void* work, size_t work_bytes, void* stream) {
+  SSIM_REQUIRE(nt >= 0, "nt must be >= 0");
+  if (nt == 0) return SSIM_OK;
+  SSIM_REQUIRE(tx && ty && tz && origin && nc && cell_start && packed &&
+                   tcell_start && torder && work && (counts || (ax && ay && az)),
+               "null pointer");
+  SSIM_REQUIRE(nc[0] >= 1 && nc[1] >= 1 && nc[2] >= 1, "bad cell counts");
+  const long long ncl = (long long)nc[0] * nc[1] * nc[2];
+  SSIM_REQUIRE(ncl <= (1 << 26), "too many cells: %lld", ncl);
+  const int ncells = (int)ncl;
+  SSIM_REQUIRE(rs > 0.0 && rcut > 0.0 && rcut <= cell,
+               "need rs > 0 and 0 < rcut <= cell (rs=%g rcut=%g cell=%g)", rs,
+               rcut, cell);
+  SSIM_REQUIRE(eps2 > 0.0, "softening eps2 must be > 0");
+  SSIM_REQUIRE(work_bytes >= ssim_pp_work_bytes(nt, ncells), "workspace too small");
+  hipStream_t st = ssim::as_stream(stream);
+  int *cstart, *cmap;
+  float2* gtab;
+  SSIM_TRY(pp_plan(nt, ncells, tcell_start, rs, rcut, work, &cstart, &cmap, &gtab, st));
+  const size_t max_chunks = (size_t)nt / CHUNK + ncells + 1;
   // chunks <= nt / 64 + ncells: launch that bound, surplus waves exit
   const dim3 grid(ssim::ceil_div((long long)max_chunks, WAVES));
   if (counts)
@@ -348,6 +460,41 @@ int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
 }  // namespace
 
 extern "C" {
+
+int ssim_pp_near(int nt, const double* tx, const double* ty, const double* tz,
+                 const double* origin, double cell, const int* nc, const int* cell_start,
+                 const void* near, const int* tcell_start, const int* torder, double G,
+                 double eps2, double rs, double rcut, double* ax, double* ay, double* az,
+                 int accumulate, void* work, size_t work_bytes, void* stream) {
+  SSIM_REQUIRE(nt >= 0, "nt must be >= 0");
+  if (nt == 0) return SSIM_OK;
+  SSIM_REQUIRE(tx && ty && tz && origin && nc && cell_start && near &&
+                   tcell_start && torder && work && ax && ay && az,
+               "null pointer");
+  SSIM_REQUIRE(nc[0] >= 1 && nc[1] >= 1 && nc[2] >= 1, "bad cell counts");
+  const long long ncl = (long long)nc[0] * nc[1] * nc[2];
+  SSIM_REQUIRE(ncl <= (1 << 26), "too many cells: %lld", ncl);
+  const int ncells = (int)ncl;
+  SSIM_REQUIRE(rs > 0.0 && rcut > 0.0 && rcut <= cell,
+               "need rs > 0 and 0 < rcut <= cell (rs=%g rcut=%g cell=%g)", rs, rcut, cell);
+  SSIM_REQUIRE(eps2 > 0.0, "softening eps2 must be > 0");
+  SSIM_REQUIRE(work_bytes >= ssim_pp_work_bytes(nt, ncells), "workspace too small");
+  hipStream_t st = ssim::as_stream(stream);
+  int *cstart, *cmap;
+  float2* gtab;
+  SSIM_TRY(pp_plan(nt, ncells, tcell_start, rs, rcut, work, &cstart, &cmap, &gtab, st));
+  const int max_chunks = nt / CHUNK + ncells + 1;
+  const dim3 grid(ssim::ceil_div(max_chunks, WAVES));
+  SSIM_TIMED("k_pp_cells", st,
+             hipLaunchKernelGGL(k_pp_near, grid, dim3(PB), 0, st, tx, ty, tz, origin[0],
+                                origin[1], origin[2], cell, nc[0], nc[1], nc[2],
+                                cell_start, static_cast<const float4*>(near), tcell_start,
+                                torder, cstart, cmap, ncells, (float)eps2, gtab,
+                                (float)(GT / (rcut * rcut)), (float)(rcut * rcut), G, ax,
+                                ay, az, accumulate));
+  SSIM_LAUNCHED("k_pp_near");
+  return SSIM_OK;
+}
 
 int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,
                   const double* origin, double cell, const int* nc,

@@ -251,6 +251,8 @@ class SplitForce:
             self.order = torch.empty(max(nm, 1), **i32)
             self.packed = torch.empty((max(nm, 1), 4), dtype=torch.float32,
                                       device=self.dev)
+            self.near = torch.empty((max(nm, 1), 4), dtype=torch.float32,
+                                    device=self.dev)
             self.body = torch.empty((max(nm, 1), 4), dtype=torch.float64,
                                     device=self.dev)
             self.bwork = D.workspace(
@@ -262,8 +264,8 @@ class SplitForce:
         _lib.call("ssim_split_bin", nm, *[_lib.ptr(c) for c in x], _lib.ptr(m), self.n,
                   P._addr(org), self.h, self.block, bins,
                   _lib.ptr(self.cell_start), _lib.ptr(self.order),
-                  _lib.ptr(self.packed), _lib.ptr(self.body), _lib.ptr(self.bwork),
-                  self.bwork.numel(), D.stream(self.dev))
+                  _lib.ptr(self.packed), _lib.ptr(self.body), _lib.ptr(self.near),
+                  _lib.ptr(self.bwork), self.bwork.numel(), D.stream(self.dev))
 
     def convolve(self, interleaved=False):
         """fields = long-range acceleration meshes of self.rho (or, with
@@ -372,8 +374,19 @@ class SplitForce:
 
     def near_sum(self, x, a, accumulate=True):
         """a += (or, accumulate=False, a =) the near-field term over the
-        lists of the last bin_bodies."""
+        lists of the last bin_bodies.  TSC runs (masses binned by
+        mass_bins) use the block-relative kernel (ssim_pp_near) unless
+        near_precise is False."""
         gn, org, cell, rs, rcut = self._grid
+        if self.assign == "tsc" and self.near_precise:
+            _lib.call("ssim_pp_near", x[0].numel(),
+                      *[_lib.ptr(c) for c in x], P._addr(org), cell, P._addr(gn),
+                      _lib.ptr(self.cell_start), _lib.ptr(self.near),
+                      _lib.ptr(self.tcell_start),
+                      _lib.ptr(self.torder), self.G, self.eps2, rs, rcut,
+                      *[_lib.ptr(c) for c in a], int(accumulate), _lib.ptr(self.cwork),
+                      self.cwork.numel(), D.stream(self.dev))
+            return a
         _lib.call("ssim_pp_short", x[0].numel(), *[_lib.ptr(c) for c in x],
                   P._addr(org), cell, P._addr(gn), _lib.ptr(self.cell_start),
                   _lib.ptr(self.packed), _lib.ptr(self.tcell_start),
@@ -503,6 +516,7 @@ class SplitForce:
                 events[sig].record(st)
 
     _side = None
+    near_precise = True   # TSC: the near field on block-relative coordinates
     overlap = True        # TSC: near field on a second stream
     side_first = False    # enqueue the target binning before the mass binning
 
