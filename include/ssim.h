@@ -239,6 +239,22 @@ int ssim_cic_deposit(int dim, int periodic, int np, const double* x,
                      const double* y, const double* z, const double* m,
                      const int* grid_n, const double* origin, double h,
                      int mesh_prec, void* mesh, int* cells, void* stream);
+/* The same CIC deposit, reproducible and without floating-point
+ * atomics: the mesh is OVERWRITTEN (every node written once, no zeroing
+ * needed), bit-identical from run to run.  The masses are filed by tiles of
+ * 8^3 (3-D) / 32^2 (2-D) nodes (one integer count per distinct tile of a
+ * wave, a scan, a ranked scatter), and one workgroup per tile sums its
+ * nodes in LDS as int64 fixed point (each of the reference's FP64 corner
+ * products rounded to 2^(e-62), 2^e >= masses the tile scans x max |m|)
+ * before writing them.  Same cell rule and `cells` output as
+ * ssim_cic_deposit.  work: ssim_cic_deposit_tiles_work_bytes.  Replaces
+ * the reference's `cp.bincount` (pm_kernel.py:89). */
+size_t ssim_cic_deposit_tiles_work_bytes(int dim, int np, const int* grid_n);
+int ssim_cic_deposit_tiles(int dim, int periodic, int np, const double* x,
+                           const double* y, const double* z, const double* m,
+                           const int* grid_n, const double* origin, double h,
+                           int mesh_prec, void* mesh, int* cells, void* work,
+                           size_t work_bytes, void* stream);
 /* CIC interpolation of nfields (1..3) mesh fields at np positions, same
  * stencil as the deposit (`_cic_gather`, pm_kernel.py:93-107). */
 int ssim_cic_gather(int dim, int periodic, int np, const double* x,
@@ -487,6 +503,22 @@ int ssim_pp_count(int nt, const double* tx, const double* ty, const double* tz,
 /* Uniform-grid collision search (bit-exact ordered pair list)         */
 /* ------------------------------------------------------------------ */
 
+/* out[0] (device) = the q-th percentile of the speeds |(gvx, gvy)| of
+ * the alive bodies (is_ast and vis != 0), NumPy's 'linear' rule on FP32
+ * data: the speeds sorted, virtual index (m - 1) (q / 100) in FP32 and the
+ * two-sided lerp of numpy's percentile; NaN if no body is alive.  The cell
+ * size of the collision search comes from its 95th percentile
+ * (`cp.percentile(cp.hypot(gvx, gvy), 95)`, film_producer.py:357).  Each
+ * speed is sqrt of the FP64 sum of squares rounded to FP32. */
+size_t ssim_speed_percentile_work_bytes(int n);
+int ssim_speed_percentile(int n, const float* gvx, const float* gvy,
+                          const unsigned char* is_ast, const unsigned char* vis, float q,
+                          float* out, void* work, size_t work_bytes, void* stream);
+/* Candidates per batch of ssim_collide_hits for the following calls (a
+ * multiple of 32 up to 2^25, the default; the workspace holds one batch's
+ * hit bits).  The pair list does not depend on it.  A knob for tests of
+ * the batching and for memory-tight callers. */
+int ssim_collide_batch(long long candidates);
 /* Asteroid-asteroid contact pairs of one sample, exactly the list of
  * `Erkennungskarte._hits` (film_producer.py:223-317) for the stripe
  * [lo, hi) with halo (single device: lo = -inf, hi = +inf): same alive
@@ -499,6 +531,9 @@ int ssim_pp_count(int nt, const double* tx, const double* ty, const double* tz,
  * pairs (or -1 when f32_hit_capacity was too small: retry with
  * counts[1]).  Pairs are written to out_i/out_j (int64) up to
  * pair_capacity; the workspace depends on n and f32_hit_capacity.
+ * Candidate pairs must stay below 2^31 (SSIM_EINVAL otherwise).  One
+ * prefilter evaluation per candidate (load-balanced runs of 32 over the
+ * flattened candidate list, batched), no atomics: deterministic.
  * Synchronises the stream (counts are returned on the host). */
 size_t ssim_collide_work_bytes(int n, long long f32_hit_capacity);
 int ssim_collide_hits(int n, const float* gx, const float* gy, const float* gvx,

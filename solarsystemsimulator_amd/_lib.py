@@ -70,6 +70,9 @@ SIGNATURES = {
     "ssim_fft_c2r": (c_int, [c_int, _P, c_int, _P, _P, _P, c_size_t, _P]),
     "ssim_cic_deposit": (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P,
                                  c_double, c_int, _P, _P, _P]),
+    "ssim_cic_deposit_tiles_work_bytes": (c_size_t, [c_int, c_int, _P]),
+    "ssim_cic_deposit_tiles": (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P, _P,
+                                       c_double, c_int, _P, _P, _P, c_size_t, _P]),
     "ssim_cic_gather": (c_int, [c_int, c_int, c_int, _P, _P, _P, _P, _P,
                                 c_double, c_int, c_int, _P, _P, _P, _P, _P, _P,
                                 _P]),
@@ -125,6 +128,10 @@ SIGNATURES = {
     "ssim_pp_count": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P,
                               _P, c_double, c_double, c_double, _P, _P, c_size_t,
                               _P]),
+    "ssim_speed_percentile_work_bytes": (c_size_t, [c_int]),
+    "ssim_speed_percentile": (c_int, [c_int, _P, _P, _P, _P, c_float, _P, _P, c_size_t,
+                                      _P]),
+    "ssim_collide_batch": (c_int, [c_longlong]),
     "ssim_collide_work_bytes": (c_size_t, [c_int, c_longlong]),
     "ssim_collide_hits": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, c_float,
                                   c_float, c_float, c_double, c_double,

@@ -16,10 +16,18 @@
 //    runs one CuPy elementwise kernel per operation; built here without FMA
 //    contraction);
 //  * ownership min(x_a, x_b) < hi, then the FP64 recheck on the FP32 inputs.
-// The output is ordered by (offset, row, sorted position): count pass, an
-// exclusive scan of the per-row hit counts, fill pass — no atomics, so the
-// list is deterministic.  The sort is a bitonic network on the 96-bit
-// composite (key, row), which is a stable sort of the keys.
+// The output is ordered by (offset, row, sorted position).  The rows are
+// sorted by key with a stable LSD radix sort (three passes of the 31-bit
+// sorter over the 64-bit key); each (offset, row) has one contiguous range
+// of sorted candidates, so the candidate list is the concatenation of those
+// ranges in output order.  Flattened that way, the candidates are cut into
+// runs of 32 per thread (load-balanced however the cells are filled): one
+// pass evaluates the FP32 prefilter once per candidate into a bitmask, a
+// scan of the per-thread hit counts gives every hit its place, and a second
+// pass writes the hits — the same list as the reference's, in its order,
+// without atomics.  The candidates go in batches of 2^25 so the workspace
+// does not depend on their number.
+#include <algorithm>
 #include <climits>
 
 #include "ssim_internal.h"
@@ -88,53 +96,61 @@ __device__ __forceinline__ long long cell_key(long long ix, long long iy) {
   return (long long)(a ^ b);
 }
 
-__global__ __launch_bounds__(CB) void k_keys(int na, int np2, const int* __restrict__ ai,
+// Cell and key of every alive row; the key's sign bit is flipped so that
+// unsigned order is the reference's signed order.
+__global__ __launch_bounds__(CB) void k_keys(int na, const int* __restrict__ ai,
                                              const float* __restrict__ gx,
                                              const float* __restrict__ gy, double h,
                                              long long* __restrict__ ix,
                                              long long* __restrict__ iy,
-                                             unsigned long long* __restrict__ skey,
+                                             unsigned long long* __restrict__ ukey,
+                                             int* __restrict__ rkey,
                                              int* __restrict__ srow) {
   const int p = blockIdx.x * CB + threadIdx.x;
-  if (p >= np2) return;
-  if (p < na) {
-    const int b = ai[p];
-    const long long cx = (long long)floor((double)gx[b] / h);
-    const long long cy = (long long)floor((double)gy[b] / h);
-    ix[p] = cx;
-    iy[p] = cy;
-    // signed order -> unsigned order by flipping the sign bit
-    skey[p] = (unsigned long long)cell_key(cx, cy) ^ 0x8000000000000000ULL;
-    srow[p] = p;
-  } else {  // padding sorts last
-    skey[p] = ULLONG_MAX;
-    srow[p] = INT_MAX;
-  }
+  if (p >= na) return;
+  const int b = ai[p];
+  const long long cx = (long long)floor((double)gx[b] / h);
+  const long long cy = (long long)floor((double)gy[b] / h);
+  ix[p] = cx;
+  iy[p] = cy;
+  const unsigned long long k = (unsigned long long)cell_key(cx, cy) ^ 0x8000000000000000ULL;
+  ukey[p] = k;
+  rkey[p] = (int)(k & 0x7fffffffULL);  // bits 0..30: the first radix pass
+  srow[p] = p;
 }
 
-__device__ __forceinline__ bool pair_less(unsigned long long ka, int ra,
-                                          unsigned long long kb, int rb) {
-  return ka < kb || (ka == kb && ra < rb);
+// The next 31-bit (or last 2-bit) slice of the key, in the current order.
+__global__ __launch_bounds__(CB) void k_key_slice(int na, int shift, unsigned mask,
+                                                  const unsigned long long* __restrict__ ukey,
+                                                  const int* __restrict__ srow,
+                                                  int* __restrict__ rkey) {
+  const int c = blockIdx.x * CB + threadIdx.x;
+  if (c >= na) return;
+  rkey[c] = (int)((ukey[srow[c]] >> shift) & mask);
 }
 
-// One bitonic compare-exchange step over the power-of-two array.
-__global__ __launch_bounds__(CB) void k_bitonic(int np2, int j, int k,
-                                                unsigned long long* __restrict__ key,
-                                                int* __restrict__ row) {
-  const int i = blockIdx.x * CB + threadIdx.x;
-  if (i >= np2) return;
-  const int l = i ^ j;
-  if (l <= i) return;
-  const bool up = (i & k) == 0;
-  const unsigned long long ki = key[i], kl = key[l];
-  const int ri = row[i], rl = row[l];
-  const bool swap = up ? pair_less(kl, rl, ki, ri) : pair_less(ki, ri, kl, rl);
-  if (swap) {
-    key[i] = kl;
-    key[l] = ki;
-    row[i] = rl;
-    row[l] = ri;
-  }
+// Sorted keys and the candidates' data in sorted order: (x, y, vx, vy),
+// radius and body index of sorted entry c; and the same data of the rows
+// (alive bodies in index order), ra4 / rar.
+__global__ __launch_bounds__(CB) void k_sorted_pack(
+    int na, const int* __restrict__ ai, const int* __restrict__ srow,
+    const unsigned long long* __restrict__ ukey, const float* __restrict__ gx,
+    const float* __restrict__ gy, const float* __restrict__ gvx,
+    const float* __restrict__ gvy, const float* __restrict__ rr,
+    unsigned long long* __restrict__ skey, float4* __restrict__ sb,
+    float* __restrict__ srr, int* __restrict__ sidx, float4* __restrict__ ra4,
+    float* __restrict__ rar) {
+  const int c = blockIdx.x * CB + threadIdx.x;
+  if (c >= na) return;
+  const int r = srow[c];
+  const int b = ai[r];
+  skey[c] = ukey[r];
+  sb[c] = make_float4(gx[b], gy[b], gvx[b], gvy[b]);
+  srr[c] = rr[b];
+  sidx[c] = b;
+  const int a = ai[c];
+  ra4[c] = make_float4(gx[a], gy[a], gvx[a], gvy[a]);
+  rar[c] = rr[a];
 }
 
 __device__ __forceinline__ int lower_bound(const unsigned long long* a, int n,
@@ -162,30 +178,38 @@ struct Pf {
   float dt, tol;
 };
 
-// FP32 prefilter of one candidate pair, one rounding per operation.
-__device__ __forceinline__ bool prefilter(const Pf& P, int a, int b) {
-  const float dpx = P.gx[b] - P.gx[a];
-  const float dpy = P.gy[b] - P.gy[a];
-  const float dvx = P.gvx[b] - P.gvx[a];
-  const float dvy = P.gvy[b] - P.gvy[a];
+// FP32 prefilter of one candidate pair (a: row body, b: candidate), one
+// rounding per operation.
+__device__ __forceinline__ bool prefilter_v(float4 A, float ra, float4 B, float rb, float dt,
+                                           float tol) {
+  const float dpx = B.x - A.x;
+  const float dpy = B.y - A.y;
+  const float dvx = B.z - A.z;
+  const float dvy = B.w - A.w;
   const float dv2 = dvx * dvx + dvy * dvy;
   const float num = -(dpx * dvx + dpy * dvy);
   const float den = dv2 > 0.0f ? dv2 : 1.0f;
   float t = num / den;
-  t = t < -P.dt ? -P.dt : (t > P.dt ? P.dt : t);
+  t = t < -dt ? -dt : (t > dt ? dt : t);
   const float cx = dpx + dvx * t;
   const float cy = dpy + dvy * t;
-  const float rs = P.rr[a] + P.rr[b] + P.tol;
+  const float rs = ra + rb + tol;
   return cx * cx + cy * cy <= rs * rs;
 }
 
-// Count (pass 0) or write (pass 1) the prefilter hits of every (offset, row).
-__global__ __launch_bounds__(CB) void k_candidates(
-    int na, const int* __restrict__ ai, const long long* __restrict__ ix,
-    const long long* __restrict__ iy, const unsigned long long* __restrict__ skey,
-    const int* __restrict__ srow, Pf P, int pass, int* __restrict__ hcount,
-    unsigned long long* __restrict__ ccount, const int* __restrict__ hoff,
-    int* __restrict__ outa, int* __restrict__ outb) {
+__device__ __forceinline__ bool prefilter(const Pf& P, int a, int b) {
+  return prefilter_v(make_float4(P.gx[a], P.gy[a], P.gvx[a], P.gvy[a]), P.rr[a],
+                     make_float4(P.gx[b], P.gy[b], P.gvx[b], P.gvy[b]), P.rr[b], P.dt,
+                     P.tol);
+}
+
+// Candidate range of every (offset, row) t = o na + p: the sorted entries
+// whose key is the offset cell's key (start, length).
+__global__ __launch_bounds__(CB) void k_segments(int na, const long long* __restrict__ ix,
+                                                 const long long* __restrict__ iy,
+                                                 const unsigned long long* __restrict__ skey,
+                                                 int* __restrict__ sstart,
+                                                 int* __restrict__ slen) {
   const int t = blockIdx.x * CB + threadIdx.x;
   if (t >= 5 * na) return;
   const int o = t / na, p = t % na;
@@ -193,26 +217,91 @@ __global__ __launch_bounds__(CB) void k_candidates(
   const unsigned long long k2 =
       (unsigned long long)cell_key(ix[p] + ox[o], iy[p] + oy[o]) ^ 0x8000000000000000ULL;
   const int lo = lower_bound(skey, na, k2);
-  const int hi = upper_bound(skey, na, k2);
-  const int a = ai[p];
-  int w = pass ? hoff[t] : 0;
+  sstart[t] = lo;
+  slen[t] = upper_bound(skey, na, k2) - lo;
+}
+
+// A wave takes WCH = 64 CPT consecutive candidates of the flattened list,
+// in CPT rounds of 64 consecutive ones (lane l, round q: candidate
+// gw + 64 q + l, so the sorted candidates' loads are coalesced); each lane
+// follows the segment (offset, row) of its candidate forward through the
+// segment offsets soff (5 na + 1 of them).  pass 0 stores each round's hit
+// ballot and the wave's hit count; pass 1 writes the hits in list order:
+// place = hits of earlier waves (wpos) + earlier rounds + earlier lanes.
+constexpr int CPT = 32;
+constexpr int WCH = 64 * CPT;
+constexpr long long CBATCH = 1LL << 25;  // candidates per batch (workspace size)
+long long g_batch = CBATCH;              // the batch in use (ssim_collide_batch)
+
+__global__ __launch_bounds__(CB) void k_cand_waves(
+    int na, long long base, long long batch, long long total, const int* __restrict__ soff,
+    const int* __restrict__ sstart, const int* __restrict__ ai,
+    const float4* __restrict__ ra4, const float* __restrict__ rar,
+    const float4* __restrict__ sb, const float* __restrict__ srr,
+    const int* __restrict__ sidx, float dt, float tol, int pass,
+    unsigned long long* __restrict__ bits, int* __restrict__ wcnt,
+    const int* __restrict__ wpos, int hoff, int* __restrict__ outa, int* __restrict__ outb) {
+  const int lane = threadIdx.x & 63;
+  const long long wave = ((long long)blockIdx.x * CB + threadIdx.x) >> 6;
+  const long long gw = base + wave * WCH;
+  const long long gend = min(total, base + batch);
+  if (gw >= gend) return;  // whole waves
+  const int nseg = 5 * na;
+  // the segment of this lane's first candidate (or of the last one)
+  const long long g0 = min(gw + lane, gend - 1);
+  int lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if ((long long)soff[mid] <= g0) lo = mid; else hi = mid;
+  }
+  int t = lo;
+  long long sbeg = soff[t], send = soff[t + 1];
+  int o = t / na, p = t % na;
+  int a = ai[p];
+  float4 A = ra4[p];
+  float ra = rar[p];
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  unsigned long long* wb = bits + wave * CPT;
   int cnt = 0;
-  for (int c = lo; c < hi; ++c) {
-    const int b = ai[srow[c]];
-    const bool keep = (o == 0) ? (a < b) : (a != b);
-    if (keep && prefilter(P, a, b)) {
-      if (pass) {
-        outa[w] = a;
-        outb[w] = b;
-        ++w;
+  int w = pass ? hoff + wpos[wave] : 0;
+  for (int q = 0; q < CPT; ++q) {
+    const long long g = gw + (long long)q * 64 + lane;
+    const bool in = g < gend;
+    bool hit = false;
+    int b = 0;
+    if (in) {
+      while (g >= send) {  // the next non-empty segment
+        ++t;
+        sbeg = send;
+        send = soff[t + 1];
+        o = t / na;
+        p = t % na;
+        a = ai[p];
+        A = ra4[p];
+        ra = rar[p];
       }
-      ++cnt;
+      const int c = sstart[t] + (int)(g - sbeg);
+      b = sidx[c];
+      if (pass == 0) {
+        const bool keep = (o == 0) ? (a < b) : (a != b);
+        hit = keep && prefilter_v(A, ra, sb[c], srr[c], dt, tol);
+      }
+    }
+    if (pass == 0) {
+      const unsigned long long bal = __ballot(hit);
+      if (lane == 0) wb[q] = bal;
+      cnt += __popcll(bal);
+    } else {
+      const unsigned long long bal = wb[q];
+      if ((bal >> lane) & 1ull) {
+        const int at = w + __popcll(bal & lt);
+        outa[at] = a;
+        outb[at] = b;
+      }
+      w += __popcll(bal);
     }
   }
-  if (!pass) {
-    hcount[t] = cnt;
-    ccount[t] = (unsigned long long)(hi - lo);
-  }
+  if (pass == 0 && lane == 0) wcnt[wave] = cnt;
 }
 
 // Ownership rule + FP64 recheck on the FP32 inputs -> flag per f32 hit.
@@ -256,19 +345,32 @@ __global__ __launch_bounds__(CB) void k_compact_pairs(int nh, const int* __restr
   }
 }
 
-__global__ __launch_bounds__(1024) void k_sum_u64(int n, const unsigned long long* __restrict__ a,
-                                                  unsigned long long* __restrict__ out) {
-  __shared__ unsigned long long s[1024];
-  unsigned long long v = 0;
-  for (int i = threadIdx.x; i < n; i += 1024) v += a[i];
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for (int w = 512; w > 0; w >>= 1) {
-    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = s[0];
+// *out (zeroed first) += the sum of a[0..n): 16 values per thread, one
+// 64-bit atomic per workgroup (integer adds: the order does not matter)
+__global__ void k_zero_u64_1(unsigned long long* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = 0ull;
 }
+
+__global__ __launch_bounds__(CB) void k_sum_i32_u64(int n, const int* __restrict__ a,
+                                                    unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long s[CB / 64];
+  const int base = blockIdx.x * CB * 16 + threadIdx.x;
+  unsigned long long v = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const int i = base + k * CB;
+    if (i < n) v += (unsigned long long)a[i];
+  }
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int q = 0; q < CB / 64; ++q) t += s[q];
+    atomicAdd(out, t);
+  }
+}
+
 
 // Exclusive scan of n ints (any n) with the 3-phase kernels.
 int scan(int n, const int* in, int* out, int* totals, int* sum, hipStream_t st) {
@@ -280,21 +382,16 @@ int scan(int n, const int* in, int* out, int* totals, int* sum, hipStream_t st) 
   return SSIM_OK;
 }
 
-int next_pow2(int v) {
-  int p = 1;
-  while (p < v) p <<= 1;
-  return p;
-}
-
 struct Layout {
-  size_t flag, pos, ai, ix, iy, skey, srow, hcount, hoff, ccount, totals, sums,
-      ha, hb, hflag, hpos, total;
+  size_t flag, pos, ai, ix, iy, ukey, rkey, srow, skey, sb, srr, sidx, ra4, rar, sstart,
+      slen, soff, bits, wcnt, wpos, sums, scan, radix, ha, hb, hflag, hpos, total;
 };
 
 Layout layout(int n, long long hcap) {
   Layout L{};
   const size_t nn = (size_t)(n > 0 ? n : 1);
-  const size_t np2 = (size_t)next_pow2((int)nn);
+  const size_t nw = (size_t)(CBATCH / WCH);
+  const size_t hc = (size_t)(hcap > 0 ? hcap : 1);
   size_t o = 0;
   auto take = [&](size_t bytes) {
     const size_t at = o;
@@ -302,25 +399,96 @@ Layout layout(int n, long long hcap) {
     return at;
   };
   L.flag = take(nn * 4);
-  L.pos = take(nn * 4);
+  L.pos = take((nn + 1) * 4);
   L.ai = take(nn * 4);
   L.ix = take(nn * 8);
   L.iy = take(nn * 8);
-  L.skey = take(np2 * 8);
-  L.srow = take(np2 * 4);
-  L.hcount = take(5 * nn * 4);
-  L.hoff = take(5 * nn * 4);
-  L.ccount = take(5 * nn * 8);
-  // block totals of the largest scan: 5 rows per body or the FP32 hits
-  const size_t scan_max = 5 * nn > (size_t)hcap ? 5 * nn : (size_t)hcap;
-  L.totals = take((scan_max / 1024 + 2) * 4);
+  L.ukey = take(nn * 8);
+  L.rkey = take(nn * 4);
+  L.srow = take(nn * 4);
+  L.skey = take(nn * 8);
+  L.sb = take(nn * 16);
+  L.srr = take(nn * 4);
+  L.sidx = take(nn * 4);
+  L.ra4 = take(nn * 16);
+  L.rar = take(nn * 4);
+  L.sstart = take(5 * nn * 4);
+  L.slen = take(5 * nn * 4);
+  L.soff = take((5 * nn + 1) * 4);
+  L.bits = take(nw * CPT * 8);
+  L.wcnt = take(nw * 4);
+  L.wpos = take((nw + 1) * 4);
   L.sums = take(64);
-  L.ha = take((size_t)(hcap > 0 ? hcap : 1) * 4);
-  L.hb = take((size_t)(hcap > 0 ? hcap : 1) * 4);
-  L.hflag = take((size_t)(hcap > 0 ? hcap : 1) * 4);
-  L.hpos = take((size_t)(hcap > 0 ? hcap : 1) * 4);
+  // the largest scan: 5 rows per body, a batch's waves or the FP32 hits
+  size_t scan_max = 5 * nn > nw ? 5 * nn : nw;
+  scan_max = scan_max > hc ? scan_max : hc;
+  L.scan = take(ssim::scan_work_bytes((int)scan_max));
+  L.radix = take(ssim::radix_sort_work_bytes((int)nn));
+  L.ha = take(hc * 4);
+  L.hb = take(hc * 4);
+  L.hflag = take(hc * 4);
+  L.hpos = take((hc + 1) * 4);
   L.total = o;
   return L;
+}
+
+// ---- the cell size's speed percentile (film_producer.py:357) ----
+
+// alive speeds as float bits (>= 0, so their integer order is their order):
+// sqrt of the FP64 sum of squares, rounded once to FP32
+__global__ __launch_bounds__(CB) void k_alive_speed(int n, const float* __restrict__ gvx,
+                                                    const float* __restrict__ gvy,
+                                                    const unsigned char* __restrict__ ast,
+                                                    const unsigned char* __restrict__ vis,
+                                                    const int* __restrict__ pos,
+                                                    int* __restrict__ bits,
+                                                    int* __restrict__ idx) {
+  const int i = blockIdx.x * CB + threadIdx.x;
+  if (i >= n || !(ast[i] && vis[i] != 0)) return;
+  const double vx = gvx[i], vy = gvy[i];
+  const float sp = (float)sqrt(vx * vx + vy * vy);
+  bits[pos[i]] = __float_as_int(sp);
+  idx[pos[i]] = pos[i];
+}
+
+// slots m..n-1 (m = the alive count, on the device) get the largest key
+__global__ __launch_bounds__(CB) void k_pad_bits(int n, const int* __restrict__ mcount,
+                                                 int* __restrict__ bits, int* __restrict__ idx) {
+  const int i = blockIdx.x * CB + threadIdx.x;
+  if (i < n && i >= *mcount) {
+    bits[i] = 0x7fffffff;
+    idx[i] = i;
+  }
+}
+
+__global__ __launch_bounds__(CB) void k_flag_alive(int n, const unsigned char* __restrict__ ast,
+                                                   const unsigned char* __restrict__ vis,
+                                                   int* __restrict__ flag) {
+  const int i = blockIdx.x * CB + threadIdx.x;
+  if (i < n) flag[i] = (ast[i] && vis[i] != 0) ? 1 : 0;
+}
+
+// NumPy's 'linear' percentile of m sorted FP32 values, in FP32 as NumPy
+// computes it for an FP32 array: virtual index (m - 1) * (q / 100), gamma
+// its fraction, a + (b - a) gamma (b - (b - a)(1 - gamma) for gamma >= 1/2)
+__global__ void k_percentile_lerp(const int* __restrict__ sorted_bits,
+                                  const int* __restrict__ mcount, float qf,
+                                  float* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int m = *mcount;
+  if (m < 1) {
+    out[0] = __int_as_float(0x7fc00000);
+    return;
+  }
+  const float vi = (float)(m - 1) * qf;
+  const float fl = floorf(vi);
+  int lo = (int)fl;
+  lo = lo < 0 ? 0 : (lo > m - 1 ? m - 1 : lo);
+  const int hi = lo + 1 < m ? lo + 1 : m - 1;
+  const float g = vi - fl;
+  const float a = __int_as_float(sorted_bits[lo]), b = __int_as_float(sorted_bits[hi]);
+  const float d = b - a;
+  out[0] = g >= 0.5f ? b - d * (1.0f - g) : a + d * g;
 }
 
 // ---- massive-body merges and runaway guard (`analyze_merge`,
@@ -783,6 +951,59 @@ size_t ssim_collide_work_bytes(int n, long long f32_hit_capacity) {
   return layout(n, f32_hit_capacity).total;
 }
 
+size_t ssim_speed_percentile_work_bytes(int n) {
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  return 4 * ssim::align256(nn * 4) + ssim::align256((nn / 1024 + 2) * 4) +
+         ssim::align256(64) + ssim::radix_sort_work_bytes((int)nn);
+}
+
+int ssim_speed_percentile(int n, const float* gvx, const float* gvy,
+                          const unsigned char* is_ast, const unsigned char* vis, float q,
+                          float* out, void* work, size_t work_bytes, void* stream) {
+  SSIM_REQUIRE(n >= 1, "n must be >= 1");
+  SSIM_REQUIRE(q >= 0.0f && q <= 100.0f, "q must be in [0, 100], got %g", q);
+  SSIM_REQUIRE(gvx && gvy && is_ast && vis && out, "null pointer");
+  SSIM_REQUIRE(work && work_bytes >= ssim_speed_percentile_work_bytes(n),
+               "workspace too small");
+  hipStream_t st = ssim::as_stream(stream);
+  const size_t nn = (size_t)n;
+  unsigned char* w = static_cast<unsigned char*>(work);
+  int* flag = reinterpret_cast<int*>(w);
+  w += ssim::align256(nn * 4);
+  int* pos = reinterpret_cast<int*>(w);
+  w += ssim::align256(nn * 4);
+  int* bits = reinterpret_cast<int*>(w);
+  w += ssim::align256(nn * 4);
+  int* idx = reinterpret_cast<int*>(w);
+  w += ssim::align256(nn * 4);
+  int* totals = reinterpret_cast<int*>(w);
+  w += ssim::align256((nn / 1024 + 2) * 4);
+  int* sums = reinterpret_cast<int*>(w);
+  w += ssim::align256(64);
+  const dim3 g(ssim::ceil_div(n, CB));
+  hipLaunchKernelGGL(k_flag_alive, g, dim3(CB), 0, st, n, is_ast, vis, flag);
+  SSIM_TRY(scan(n, flag, pos, totals, sums, st));
+  hipLaunchKernelGGL(k_alive_speed, g, dim3(CB), 0, st, n, gvx, gvy, is_ast, vis, pos, bits,
+                     idx);
+  SSIM_LAUNCHED("k_alive_speed");
+  // sort the m alive speeds (m on the device: sort all n slots with the
+  // unused ones holding the largest key, so the first m are the alive ones)
+  hipLaunchKernelGGL(k_pad_bits, g, dim3(CB), 0, st, n, sums, bits, idx);
+  SSIM_TRY(ssim::radix_sort_pairs(n, 31, bits, idx, w, ssim::radix_sort_work_bytes(n), st));
+  const float qf = q / 100.0f;
+  hipLaunchKernelGGL(k_percentile_lerp, dim3(1), dim3(64), 0, st, bits, sums, qf, out);
+  SSIM_LAUNCHED("k_percentile_lerp");
+  return SSIM_OK;
+}
+
+int ssim_collide_batch(long long candidates) {
+  SSIM_REQUIRE(candidates >= CPT && candidates <= CBATCH && candidates % CPT == 0,
+               "collision batch must be a multiple of %d in [%d, %lld], got %lld", CPT, CPT,
+               CBATCH, candidates);
+  g_batch = candidates;
+  return SSIM_OK;
+}
+
 int ssim_collide_hits(int n, const float* gx, const float* gy, const float* gvx,
                       const float* gvy, const float* rr,
                       const unsigned char* is_ast, const unsigned char* vis,
@@ -807,72 +1028,120 @@ int ssim_collide_hits(int n, const float* gx, const float* gy, const float* gvx,
   int* ai = reinterpret_cast<int*>(w + L.ai);
   long long* ix = reinterpret_cast<long long*>(w + L.ix);
   long long* iy = reinterpret_cast<long long*>(w + L.iy);
-  unsigned long long* skey = reinterpret_cast<unsigned long long*>(w + L.skey);
+  unsigned long long* ukey = reinterpret_cast<unsigned long long*>(w + L.ukey);
+  int* rkey = reinterpret_cast<int*>(w + L.rkey);
   int* srow = reinterpret_cast<int*>(w + L.srow);
-  int* hcount = reinterpret_cast<int*>(w + L.hcount);
-  int* hoff = reinterpret_cast<int*>(w + L.hoff);
-  unsigned long long* ccount = reinterpret_cast<unsigned long long*>(w + L.ccount);
-  int* totals = reinterpret_cast<int*>(w + L.totals);
-  int* sums = reinterpret_cast<int*>(w + L.sums);
-  unsigned long long* csum = reinterpret_cast<unsigned long long*>(w + L.sums + 8);
+  unsigned long long* skey = reinterpret_cast<unsigned long long*>(w + L.skey);
+  float4* sb = reinterpret_cast<float4*>(w + L.sb);
+  float* srr = reinterpret_cast<float*>(w + L.srr);
+  int* sidx = reinterpret_cast<int*>(w + L.sidx);
+  float4* ra4 = reinterpret_cast<float4*>(w + L.ra4);
+  float* rar = reinterpret_cast<float*>(w + L.rar);
+  int* sstart = reinterpret_cast<int*>(w + L.sstart);
+  int* slen = reinterpret_cast<int*>(w + L.slen);
+  int* soff = reinterpret_cast<int*>(w + L.soff);
+  unsigned long long* bits = reinterpret_cast<unsigned long long*>(w + L.bits);
+  int* wcnt = reinterpret_cast<int*>(w + L.wcnt);
+  int* wpos = reinterpret_cast<int*>(w + L.wpos);
+  unsigned long long* csum = reinterpret_cast<unsigned long long*>(w + L.sums);
+  void* swork = w + L.scan;
+  void* radix = w + L.radix;
+  const size_t radix_bytes = ssim::radix_sort_work_bytes(n);
   int* ha = reinterpret_cast<int*>(w + L.ha);
   int* hb = reinterpret_cast<int*>(w + L.hb);
   int* hflag = reinterpret_cast<int*>(w + L.hflag);
   int* hpos = reinterpret_cast<int*>(w + L.hpos);
+  auto read_int = [&](const int* at, int* v) -> int {
+    SSIM_HIP(hipMemcpyAsync(v, at, sizeof(int), hipMemcpyDeviceToHost, st));
+    SSIM_HIP(hipStreamSynchronize(st));
+    return SSIM_OK;
+  };
 
   // alive rows in ascending body index
   hipLaunchKernelGGL(k_alive_flags, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0, st,
                      n, gx, is_ast, vis, lo, hi + halo, flag);
-  SSIM_TRY(scan(n, flag, pos, totals, sums, st));
+  SSIM_LAUNCHED("k_alive_flags");
+  SSIM_TRY(ssim::scan_exclusive(n, flag, pos, swork, st));
   int na = 0;
-  SSIM_HIP(hipMemcpyAsync(&na, sums, sizeof(int), hipMemcpyDeviceToHost, st));
-  SSIM_HIP(hipStreamSynchronize(st));
+  SSIM_TRY(read_int(pos + n, &na));
   if (na < 2) return SSIM_OK;
+  const dim3 ga(ssim::ceil_div(na, CB));
   hipLaunchKernelGGL(k_compact_alive, dim3(ssim::ceil_div(n, CB)), dim3(CB), 0,
                      st, n, flag, pos, ai);
-  // keys, stable sort by (key, row)
-  const int np2 = next_pow2(na);
-  hipLaunchKernelGGL(k_keys, dim3(ssim::ceil_div(np2, CB)), dim3(CB), 0, st, na,
-                     np2, ai, gx, gy, h, ix, iy, skey, srow);
-  for (int k = 2; k <= np2; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1)
-      hipLaunchKernelGGL(k_bitonic, dim3(ssim::ceil_div(np2, CB)), dim3(CB), 0,
-                         st, np2, j, k, skey, srow);
-  SSIM_LAUNCHED("k_bitonic");
-  // count pass
-  Pf P{gx, gy, gvx, gvy, rr, (float)dt_y, 1e-6f};
-  const int nrows = 5 * na;
-  hipLaunchKernelGGL(k_candidates, dim3(ssim::ceil_div(nrows, CB)), dim3(CB), 0,
-                     st, na, ai, ix, iy, skey, srow, P, 0, hcount, ccount,
-                     nullptr, nullptr, nullptr);
-  SSIM_LAUNCHED("k_candidates");
-  SSIM_TRY(scan(nrows, hcount, hoff, totals, sums, st));
-  hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(1024), 0, st, nrows, ccount, csum);
-  int nh = 0;
+  // keys, then a stable LSD radix sort of the rows by the 64-bit key: bits
+  // 0..30, 31..61 and 62..63
+  hipLaunchKernelGGL(k_keys, ga, dim3(CB), 0, st, na, ai, gx, gy, h, ix, iy, ukey, rkey,
+                     srow);
+  SSIM_LAUNCHED("k_keys");
+  SSIM_TRY(ssim::radix_sort_pairs(na, 31, rkey, srow, radix, radix_bytes, st));
+  hipLaunchKernelGGL(k_key_slice, ga, dim3(CB), 0, st, na, 31, 0x7fffffffu, ukey, srow,
+                     rkey);
+  SSIM_TRY(ssim::radix_sort_pairs(na, 31, rkey, srow, radix, radix_bytes, st));
+  hipLaunchKernelGGL(k_key_slice, ga, dim3(CB), 0, st, na, 62, 0x3u, ukey, srow, rkey);
+  SSIM_TRY(ssim::radix_sort_pairs(na, 2, rkey, srow, radix, radix_bytes, st));
+  hipLaunchKernelGGL(k_sorted_pack, ga, dim3(CB), 0, st, na, ai, srow, ukey, gx, gy, gvx,
+                     gvy, rr, skey, sb, srr, sidx, ra4, rar);
+  SSIM_LAUNCHED("k_sorted_pack");
+  // the candidate range of every (offset, row) and their offsets in the
+  // flattened list
+  const int nseg = 5 * na;
+  hipLaunchKernelGGL(k_segments, dim3(ssim::ceil_div(nseg, CB)), dim3(CB), 0, st, na, ix,
+                     iy, skey, sstart, slen);
+  SSIM_LAUNCHED("k_segments");
+  hipLaunchKernelGGL(k_zero_u64_1, dim3(1), dim3(64), 0, st, csum);
+  hipLaunchKernelGGL(k_sum_i32_u64, dim3(ssim::ceil_div(nseg, CB * 16)), dim3(CB), 0, st,
+                     nseg, slen, csum);
+  SSIM_LAUNCHED("k_sum_i32_u64");
   unsigned long long ncand = 0;
-  SSIM_HIP(hipMemcpyAsync(&nh, sums, sizeof(int), hipMemcpyDeviceToHost, st));
   SSIM_HIP(hipMemcpyAsync(&ncand, csum, sizeof(ncand), hipMemcpyDeviceToHost, st));
   SSIM_HIP(hipStreamSynchronize(st));
   counts[0] = (long long)ncand;
+  SSIM_REQUIRE(ncand < (unsigned long long)INT_MAX,
+               "collision search: %llu candidate pairs exceed the 2^31 list limit "
+               "(cell size too large for the cloud)", ncand);
+  SSIM_TRY(ssim::scan_exclusive(nseg, slen, soff, swork, st));  // soff[nseg] = total
+  // the prefilter once per candidate, in batches
+  Pf P{gx, gy, gvx, gvy, rr, (float)dt_y, 1e-6f};
+  long long nh = 0;
+  bool room = true;
+  const long long batch = g_batch;
+  for (long long base = 0; base < (long long)ncand; base += batch) {
+    const long long nb = std::min<long long>(batch, (long long)ncand - base);
+    const int nwave = (int)((nb + WCH - 1) / WCH);
+    const dim3 gb(ssim::ceil_div((long long)nwave * 64, CB));
+    hipLaunchKernelGGL(k_cand_waves, gb, dim3(CB), 0, st, na, base, batch, (long long)ncand,
+                       soff, sstart, ai, ra4, rar, sb, srr, sidx, P.dt, P.tol, 0, bits, wcnt,
+                       nullptr, 0, nullptr, nullptr);
+    SSIM_LAUNCHED("k_cand_waves");
+    SSIM_TRY(ssim::scan_exclusive(nwave, wcnt, wpos, swork, st));
+    int nbh = 0;
+    SSIM_TRY(read_int(wpos + nwave, &nbh));
+    if (room && nh + nbh <= f32_hit_capacity) {
+      hipLaunchKernelGGL(k_cand_waves, gb, dim3(CB), 0, st, na, base, batch,
+                         (long long)ncand, soff, sstart, ai, ra4, rar, sb, srr, sidx, P.dt,
+                         P.tol, 1, bits, wcnt, wpos, (int)nh, ha, hb);
+      SSIM_LAUNCHED("k_cand_waves");
+    } else {
+      room = false;
+    }
+    nh += nbh;
+  }
   counts[1] = nh;
   if (nh == 0) return SSIM_OK;
-  if (nh > f32_hit_capacity) {  // caller retries with a larger workspace
+  if (!room) {  // caller retries with a larger workspace
     counts[2] = -1;
     return SSIM_OK;
   }
-  hipLaunchKernelGGL(k_candidates, dim3(ssim::ceil_div(nrows, CB)), dim3(CB), 0,
-                     st, na, ai, ix, iy, skey, srow, P, 1, hcount, ccount, hoff,
-                     ha, hb);
-  hipLaunchKernelGGL(k_recheck, dim3(ssim::ceil_div(nh, CB)), dim3(CB), 0, st, nh,
+  const int nhi = (int)nh;
+  hipLaunchKernelGGL(k_recheck, dim3(ssim::ceil_div(nhi, CB)), dim3(CB), 0, st, nhi,
                      ha, hb, P, dt_y, hi, hflag);
-  SSIM_TRY(scan(nh, hflag, hpos, totals, sums, st));
-  hipLaunchKernelGGL(k_compact_pairs, dim3(ssim::ceil_div(nh, CB)), dim3(CB), 0,
-                     st, nh, hflag, hpos, ha, hb, out_i, out_j,
+  SSIM_TRY(ssim::scan_exclusive(nhi, hflag, hpos, swork, st));
+  hipLaunchKernelGGL(k_compact_pairs, dim3(ssim::ceil_div(nhi, CB)), dim3(CB), 0,
+                     st, nhi, hflag, hpos, ha, hb, out_i, out_j,
                      (out_i && out_j) ? pair_capacity : 0);
   SSIM_LAUNCHED("k_compact_pairs");
   int np = 0;
-  SSIM_HIP(hipMemcpyAsync(&np, sums, sizeof(int), hipMemcpyDeviceToHost, st));
-  SSIM_HIP(hipStreamSynchronize(st));
+  SSIM_TRY(read_int(hpos + nhi, &np));
   counts[2] = np;
   return SSIM_OK;
 }

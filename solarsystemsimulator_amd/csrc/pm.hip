@@ -93,6 +93,251 @@ __global__ __launch_bounds__(PM_BLOCK) void k_cic_deposit(
   }
 }
 
+// ---------------------------------------------------------------------
+// Reproducible CIC deposit without floating-point atomics
+// (ssim_cic_deposit_tiles).  The mesh is cut into tiles of CE^D nodes
+// (CE = 8 in 3-D, 32 in 2-D).  Each mass is filed under the tile of its
+// cell (cell_of, the reference's cell rule): a key kernel counts the tiles
+// with one integer atomic per distinct tile of a wave (masses arrive in
+// tile runs when they are spatially ordered), issued back to back, a scan gives the tile starts
+// and a ranked scatter lays the masses out in tile order.  One workgroup
+// per tile then scans the masses of its tile and of the tiles just below it
+// on each axis (a cell's upper corner is the next node, so those are the
+// only masses that reach the tile), adds every contribution that lands in
+// the tile to an LDS accumulator as int64 fixed point, and writes each node
+// once.  Integer sums are exact in any order, so the mesh is bit-identical
+// from run to run; each contribution is the reference's FP64 product
+// ((m ux) uy) uz rounded to 2^(e - 62), with 2^e >= (masses the tile
+// scans) x max |m| (over all masses: the key kernel's workgroup maxima,
+// folded by the scatter), and each node is the nearest double of its exact
+// fixed-point sum.
+// ---------------------------------------------------------------------
+template <int D>
+struct CicTile {
+  static constexpr int E = D == 3 ? 8 : 32;           // nodes per tile edge
+  static constexpr int NN = D == 3 ? 512 : 1024;      // nodes per tile
+};
+constexpr int CT_THREADS = 64;   // one wave per tile
+
+__device__ __forceinline__ int tiles_of(int n, int E) { return (n + E - 1) / E; }
+
+template <int D>
+__global__ __launch_bounds__(PM_BLOCK) void k_cic_tile_key(
+    int np, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ m, Grid g,
+    int* __restrict__ key, int* __restrict__ rank, int* __restrict__ count,
+    int* __restrict__ cells, double4* __restrict__ rec, int4* __restrict__ reci,
+    double* __restrict__ part_max) {
+  constexpr int E = CicTile<D>::E;
+  const int p = blockIdx.x * PM_BLOCK + threadIdx.x;
+  const bool on = p < np;
+  int k = 0;
+  if (on) {
+    int i, i1, j, j1, c = 0, c1 = 0;
+    double tx, ty, tz = 0.0;
+    cell_of(x[p], g.o[0], g.h, g.n[0], g.periodic, i, i1, tx);
+    cell_of(y[p], g.o[1], g.h, g.n[1], g.periodic, j, j1, ty);
+    if (D == 3) cell_of(z[p], g.o[2], g.h, g.n[2], g.periodic, c, c1, tz);
+    // the mass's cell and weights, computed once (the tiles only compare)
+    rec[p] = make_double4(tx, ty, tz, m[p]);
+    reci[p] = make_int4(i, j, c, 0);
+    if (cells) {
+      cells[(size_t)D * p] = i;
+      cells[(size_t)D * p + 1] = j;
+      if (D == 3) cells[(size_t)D * p + 2] = c;
+    }
+    const int t1 = tiles_of(g.n[1], E), t2 = D == 3 ? tiles_of(g.n[2], E) : 1;
+    k = ((i / E) * t1 + j / E) * t2 + (D == 3 ? c / E : 0);
+    key[p] = k;
+  }
+  // max |m| of this workgroup's masses (the tiles' fixed-point scale; the
+  // scatter's first workgroup folds the workgroups' maxima)
+  __shared__ double wmax[PM_BLOCK / 64];
+  const int lane = threadIdx.x & 63;
+  double am = on ? fabs(m[p]) : 0.0;
+  for (int off = 32; off > 0; off >>= 1) am = fmax(am, __shfl_xor(am, off, 64));
+  if (lane == 0) wmax[threadIdx.x >> 6] = am;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bm = wmax[0];
+    for (int q = 1; q < PM_BLOCK / 64; ++q) bm = fmax(bm, wmax[q]);
+    part_max[blockIdx.x] = bm;
+  }
+  // one atomic per distinct key of the wave, issued back to back; the
+  // ranks (slots below in the same tile) once they have all returned
+  unsigned long long pending = __ballot(on), mine = 0ull;
+  int base = 0, leader_of = 0;
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const int lk = __shfl(k, leader, 64);
+    const unsigned long long same = __ballot(on && k == lk) & pending;
+    if (lane == leader) base = atomicAdd(&count[lk], __popcll(same));
+    if ((same >> lane) & 1ull) {
+      mine = same;
+      leader_of = leader;
+    }
+    pending &= ~same;
+  }
+  const int b0 = __shfl(base, leader_of, 64);
+  if (on) rank[p] = b0 + __popcll(mine & ((1ull << lane) - 1ull));
+}
+
+// count[0..n) = 0 and the 64-bit maximum's two words (mx) = 0
+__global__ __launch_bounds__(PM_BLOCK) void k_zero_count(int n, int* __restrict__ c,
+                                                         int* __restrict__ mx) {
+  const int i = blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (i < n) c[i] = 0;
+  else if (i < n + 2) mx[i - n] = 0;
+}
+
+// the masses' records in tile order: slot = start[key] + rank; workgroup 0
+// also folds the key kernel's nparts workgroup maxima of |m| into *mmax
+__global__ __launch_bounds__(PM_BLOCK) void k_cic_tile_scatter(
+    int np, const int* __restrict__ key, const int* __restrict__ rank,
+    const int* __restrict__ start, const double4* __restrict__ rec,
+    const int4* __restrict__ reci, double4* __restrict__ trec, int4* __restrict__ treci,
+    int nparts, const double* __restrict__ part_max, double* __restrict__ mmax) {
+  if (blockIdx.x == 0) {
+    __shared__ double wm[PM_BLOCK / 64];
+    double bm = 0.0;
+    for (int q = threadIdx.x; q < nparts; q += PM_BLOCK) bm = fmax(bm, part_max[q]);
+    for (int off = 32; off > 0; off >>= 1) bm = fmax(bm, __shfl_xor(bm, off, 64));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = bm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int q = 1; q < PM_BLOCK / 64; ++q) bm = fmax(bm, wm[q]);
+      *mmax = fmax(wm[0], bm);
+    }
+  }
+  const int p = blockIdx.x * PM_BLOCK + threadIdx.x;
+  if (p >= np) return;
+  const int slot = start[key[p]] + rank[p];
+  trec[slot] = rec[p];
+  treci[slot] = reci[p];
+}
+
+template <int D, typename M>
+__global__ __launch_bounds__(CT_THREADS) void k_cic_deposit_tiles(
+    Grid g, const int* __restrict__ start, const double4* __restrict__ trec,
+    const int4* __restrict__ treci, const double* __restrict__ mmax,
+    M* __restrict__ mesh) {
+  constexpr int E = CicTile<D>::E, NN = CicTile<D>::NN;
+  __shared__ unsigned long long acc[NN];
+  const int tid = threadIdx.x;
+  const int nt[3] = {tiles_of(g.n[0], E), tiles_of(g.n[1], E),
+                     D == 3 ? tiles_of(g.n[2], E) : 1};
+  const int T[3] = {(int)blockIdx.x / (nt[1] * nt[2]), ((int)blockIdx.x / nt[2]) % nt[1],
+                    (int)blockIdx.x % nt[2]};
+  const int lo[3] = {T[0] * E, T[1] * E, T[2] * E};
+  // the source tiles: this one and the one below on each axis (wrapped when
+  // periodic, none below tile 0 otherwise; never the tile itself again);
+  // from a tile below only the masses in its top cell layer reach this one
+  int src[8], need[8];
+  int nsrc = 0;
+  for (int b = 0; b < (1 << D); ++b) {
+    int s3[3] = {T[0], T[1], T[2]};
+    bool ok = true;
+    for (int a = 0; a < D; ++a) {
+      if (!((b >> a) & 1)) continue;
+      if (nt[a] == 1) ok = false;
+      else if (T[a] > 0) s3[a] = T[a] - 1;
+      else if (g.periodic) s3[a] = nt[a] - 1;
+      else ok = false;
+    }
+    if (ok) {
+      src[nsrc] = (s3[0] * nt[1] + s3[1]) * nt[2] + s3[2];
+      need[nsrc++] = b;
+    }
+  }
+  // fixed-point scale of this tile: 2^e >= (masses scanned) x max |m| (the
+  // global maximum, from the key kernel)
+  int cnt = 0;
+  for (int q = 0; q < nsrc; ++q) cnt += start[src[q] + 1] - start[src[q]];
+  for (int q = tid; q < NN; q += CT_THREADS) acc[q] = 0ull;
+  const double mx = *mmax;
+  int ex = 0;
+  frexp(mx * (double)(cnt > 0 ? cnt : 1), &ex);
+  ex = ex < -900 ? -900 : ex;
+  __syncthreads();
+  const double scale = ldexp(1.0, 62 - ex), inv_scale = ldexp(1.0, ex - 62);
+  // one mass: its contributions that land in this tile (nb: the axes on
+  // which its source tile lies below this one)
+  auto add_mass = [&](const int4 ci, const double4 r, int nb) {
+    const int c3[3] = {ci.x, ci.y, ci.z};
+    bool reach = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      if ((nb >> a) & 1) reach = reach && ((c3[a] + 1 == lo[a]) ||
+                                           (g.periodic && lo[a] == 0 && c3[a] == g.n[a] - 1));
+    if (!reach) return;
+    const int i = ci.x, j = ci.y, c = ci.z;
+    const int i1 = (g.periodic && i + 1 == g.n[0]) ? 0 : i + 1;
+    const int j1 = (g.periodic && j + 1 == g.n[1]) ? 0 : j + 1;
+    const double tx = r.x, ty = r.y, tz = r.z;
+    const double ux = 1.0 - tx, uy = 1.0 - ty;
+    const int ni[2] = {i - lo[0], i1 - lo[0]}, nj[2] = {j - lo[1], j1 - lo[1]};
+    if (D == 2) {
+      const double v[2][2] = {{r.w * ux * uy, r.w * ux * ty}, {r.w * tx * uy, r.w * tx * ty}};
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+          if (ni[a] >= 0 && ni[a] < E && nj[e] >= 0 && nj[e] < E)
+            atomicAdd(&acc[ni[a] * E + nj[e]],
+                      (unsigned long long)__double2ll_rn(v[a][e] * scale));
+    } else {
+      const int c1 = (g.periodic && c + 1 == g.n[2]) ? 0 : c + 1;
+      const double uz = 1.0 - tz;
+      const int nk[2] = {c - lo[2], c1 - lo[2]};
+      const double a2[2][2] = {{r.w * ux * uy, r.w * ux * ty}, {r.w * tx * uy, r.w * tx * ty}};
+      const double wz[2] = {uz, tz};
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+#pragma unroll
+          for (int f = 0; f < 2; ++f)
+            if (ni[a] >= 0 && ni[a] < E && nj[e] >= 0 && nj[e] < E && nk[f] >= 0 &&
+                nk[f] < E)
+              atomicAdd(&acc[(ni[a] * E + nj[e]) * E + nk[f]],
+                        (unsigned long long)__double2ll_rn(a2[a][e] * wz[f] * scale));
+    }
+  };
+  // the first CT_THREADS masses of every source tile, all loads issued
+  // before any is used, then the rest (dense tiles) tile by tile
+  int s0q[8], s1q[8];
+  int4 ciq[8];
+  double4 rq[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    s0q[q] = q < nsrc ? start[src[q]] : 0;
+    s1q[q] = q < nsrc ? start[src[q] + 1] : 0;
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int sidx = s0q[q] + tid;
+    if (sidx < s1q[q]) {
+      ciq[q] = treci[sidx];
+      rq[q] = trec[sidx];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (s0q[q] + tid < s1q[q]) add_mass(ciq[q], rq[q], need[q]);
+  for (int q = 0; q < nsrc; ++q)
+    for (int sidx = s0q[q] + CT_THREADS + tid; sidx < s1q[q]; sidx += CT_THREADS)
+      add_mass(treci[sidx], trec[sidx], need[q]);
+  __syncthreads();
+  for (int q = tid; q < NN; q += CT_THREADS) {
+    const int gi = lo[0] + (D == 3 ? q / (E * E) : q / E);
+    const int gj = lo[1] + (D == 3 ? (q / E) % E : q % E);
+    const int gk = D == 3 ? lo[2] + q % E : 0;
+    if (gi < g.n[0] && gj < g.n[1] && (D == 2 || gk < g.n[2]))
+      mesh[((long long)gi * g.n[1] + gj) * g.n[2] + gk] =
+          (M)((double)(long long)acc[q] * inv_scale);
+  }
+}
+
 // out_c[p] = sum over the CIC corners of field_c * weight, c < nf.
 template <int D, typename M>
 __global__ __launch_bounds__(PM_BLOCK) void k_cic_gather(
@@ -1054,6 +1299,95 @@ int ssim_cic_deposit(int dim, int periodic, int np, const double* x,
   }
 #undef LAUNCH_DEP
   SSIM_LAUNCHED("k_cic_deposit");
+  return SSIM_OK;
+}
+
+size_t ssim_cic_deposit_tiles_work_bytes(int dim, int np, const int* grid_n) {
+  if ((dim != 2 && dim != 3) || !grid_n) return 0;
+  const int E = dim == 3 ? CicTile<3>::E : CicTile<2>::E;
+  long long nt = 1;
+  for (int c = 0; c < dim; ++c) nt *= (grid_n[c] + E - 1) / E;
+  const size_t n = (size_t)(np > 0 ? np : 1);
+  return 2 * ssim::align256(n * sizeof(int)) + ssim::align256((size_t)nt * sizeof(int)) +
+         ssim::align256(((size_t)nt + 1) * sizeof(int)) + ssim::scan_work_bytes((int)nt) +
+         2 * ssim::align256(n * sizeof(double4)) + 2 * ssim::align256(n * sizeof(int4)) +
+         ssim::align256(sizeof(double)) +
+         ssim::align256((size_t)((n + PM_BLOCK - 1) / PM_BLOCK) * sizeof(double));
+}
+
+int ssim_cic_deposit_tiles(int dim, int periodic, int np, const double* x,
+                           const double* y, const double* z, const double* m,
+                           const int* grid_n, const double* origin, double h,
+                           int mesh_prec, void* mesh, int* cells, void* work,
+                           size_t work_bytes, void* stream) {
+  SSIM_TRY(check_grid(dim, grid_n, h, periodic));
+  SSIM_REQUIRE(np >= 0, "np must be >= 0");
+  SSIM_REQUIRE(mesh && origin && (np == 0 || (x && y && m && (dim == 2 || z))),
+               "null pointer");
+  SSIM_REQUIRE(work && work_bytes >= ssim_cic_deposit_tiles_work_bytes(dim, np, grid_n),
+               "workspace too small");
+  const Grid g = make_grid(dim, grid_n, origin, h, periodic);
+  const int E = dim == 3 ? CicTile<3>::E : CicTile<2>::E;
+  long long ntl = 1;
+  for (int c = 0; c < dim; ++c) ntl *= (grid_n[c] + E - 1) / E;
+  SSIM_REQUIRE(ntl <= (1LL << 30), "too many tiles: %lld", ntl);
+  const int nt = (int)ntl;
+  hipStream_t st = ssim::as_stream(stream);
+  const size_t n = (size_t)(np > 0 ? np : 1);
+  unsigned char* w = static_cast<unsigned char*>(work);
+  int* key = reinterpret_cast<int*>(w);
+  w += ssim::align256(n * sizeof(int));
+  int* rank = reinterpret_cast<int*>(w);
+  w += ssim::align256(n * sizeof(int));
+  int* count = reinterpret_cast<int*>(w);
+  w += ssim::align256((size_t)nt * sizeof(int));
+  int* start = reinterpret_cast<int*>(w);
+  w += ssim::align256(((size_t)nt + 1) * sizeof(int));
+  void* scan_work = w;
+  w += ssim::scan_work_bytes(nt);
+  double4* rec = reinterpret_cast<double4*>(w);
+  w += ssim::align256(n * sizeof(double4));
+  double4* trec = reinterpret_cast<double4*>(w);
+  w += ssim::align256(n * sizeof(double4));
+  int4* reci = reinterpret_cast<int4*>(w);
+  w += ssim::align256(n * sizeof(int4));
+  int4* treci = reinterpret_cast<int4*>(w);
+  w += ssim::align256(n * sizeof(int4));
+  double* mmax = reinterpret_cast<double*>(w);
+  w += ssim::align256(sizeof(double));
+  double* part_max = reinterpret_cast<double*>(w);
+  // counts zeroed by a kernel (graph-capturable), then keys, starts, order
+  hipLaunchKernelGGL(k_zero_count, dim3(ssim::ceil_div(nt + 2, PM_BLOCK)), dim3(PM_BLOCK), 0,
+                     st, nt, count, reinterpret_cast<int*>(mmax));  // np = 0: mmax = 0
+  SSIM_LAUNCHED("k_zero_count");
+  if (np > 0) {
+    const dim3 gp(ssim::ceil_div(np, PM_BLOCK));
+    if (dim == 2)
+      hipLaunchKernelGGL(k_cic_tile_key<2>, gp, dim3(PM_BLOCK), 0, st, np, x, y, z, m, g,
+                         key, rank, count, cells, rec, reci, part_max);
+    else
+      hipLaunchKernelGGL(k_cic_tile_key<3>, gp, dim3(PM_BLOCK), 0, st, np, x, y, z, m, g,
+                         key, rank, count, cells, rec, reci, part_max);
+    SSIM_LAUNCHED("k_cic_tile_key");
+  }
+  SSIM_TRY(ssim::scan_exclusive(nt, count, start, scan_work, st));
+  if (np > 0) {
+    const int nparts = ssim::ceil_div(np, PM_BLOCK);
+    hipLaunchKernelGGL(k_cic_tile_scatter, dim3(nparts), dim3(PM_BLOCK), 0, st, np, key,
+                       rank, start, rec, reci, trec, treci, nparts, part_max, mmax);
+    SSIM_LAUNCHED("k_cic_tile_scatter");
+  }
+#define LAUNCH_TILES(D, M)                                                         \
+  SSIM_TIMED("k_cic_deposit", st,                                                   \
+             hipLaunchKernelGGL((k_cic_deposit_tiles<D, M>), dim3(nt), dim3(CT_THREADS), \
+                                0, st, g, start, trec, treci, mmax, static_cast<M*>(mesh)))
+  if (mesh_prec == SSIM_PREC_F64) {
+    if (dim == 2) LAUNCH_TILES(2, double); else LAUNCH_TILES(3, double);
+  } else {
+    if (dim == 2) LAUNCH_TILES(2, float); else LAUNCH_TILES(3, float);
+  }
+#undef LAUNCH_TILES
+  SSIM_LAUNCHED("k_cic_deposit_tiles");
   return SSIM_OK;
 }
 

@@ -162,13 +162,27 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
     }
     run[t] = r0;
     __syncthreads();
+    int slot = 0;
     if (valid) {
-      const int slot = wb[w][d] + rank;
+      slot = wb[w][d] + rank;
       keys_out[slot] = k[r];
       vals_out[slot] = v[r];
-      if (hist_next)
-        atomicAdd(&hist_next[(size_t)((k[r] >> shift_next) & 255) * hrow + slot / RTILE],
-                  1);
+    }
+    if (hist_next) {
+      // the next pass's count of (digit, destination tile): lanes of the
+      // wave that share one add it together (up to four such groups;
+      // keys with repetitive digits would otherwise serialise on one
+      // counter), the rest one by one
+      const int combo = valid ? ((k[r] >> shift_next) & 255) * hrow + slot / RTILE : -1;
+      unsigned long long pend = __ballot(valid);
+      for (int it = 0; it < 4 && pend; ++it) {
+        const int leader = __ffsll((long long)pend) - 1;
+        const int lc = __shfl(combo, leader, 64);
+        const unsigned long long same = __ballot(valid && combo == lc) & pend;
+        if (lane == leader) atomicAdd(&hist_next[lc], __popcll(same));
+        pend &= ~same;
+      }
+      if ((pend >> lane) & 1ull) atomicAdd(&hist_next[combo], 1);
     }
   }
 }

@@ -259,6 +259,25 @@ def _lastanteile(anteile, last):
     return new
 
 
+def speed_percentile(karte: Erkennungskarte, gvx, gvy, q: float) -> float:
+    """The q-th percentile of the alive bodies' speeds, on the device
+    (ssim_speed_percentile: NumPy's linear rule on the FP32 speeds, like
+    `cp.percentile(cp.hypot(gvx[lebt], gvy[lebt]), q)`, film_producer.py:357);
+    one float comes back."""
+    torch = _torch()
+    n = gvx.numel()
+    lib = _lib.load()
+    wb = lib.ssim_speed_percentile_work_bytes(n)
+    if getattr(karte, "_pwork", None) is None or karte._pwork.numel() < wb:
+        karte._pwork = D.workspace(wb, karte._dev)
+        karte._pout = torch.empty(1, dtype=torch.float32, device=karte._dev)
+    cols = [a.contiguous() for a in (gvx, gvy)]
+    _lib.call("ssim_speed_percentile", n, _lib.ptr(cols[0]), _lib.ptr(cols[1]),
+              _lib.ptr(karte.g_ast), _lib.ptr(karte.g_vis), float(q), _lib.ptr(karte._pout),
+              _lib.ptr(karte._pwork), karte._pwork.numel(), D.stream(karte._dev))
+    return float(karte._pout.cpu()[0])
+
+
 def bounce_suche(det, det_pool, rows, dt_y: float, rr_max: float, anteile=None):
     """Collision pairs of one sample over the stripes (film_producer.py:
     324-379): returns (pairs | None, candidates, h, new shares).  On one
@@ -270,9 +289,9 @@ def bounce_suche(det, det_pool, rows, dt_y: float, rr_max: float, anteile=None):
     idx = np.flatnonzero((k0._ast_h != 0) & (k0._vis_h != 0))
     if idx.size < 2:
         return None, 0, 0.0, anteile
-    speed = np.hypot(gvx.cpu().numpy()[idx], gvy.cpu().numpy()[idx])
-    v95 = float(np.percentile(speed, 95))
-    bounds = _streifengrenzen(gx.cpu().numpy()[idx], anteile)
+    v95 = speed_percentile(k0, gvx, gvy, 95.0)
+    bounds = (_streifengrenzen(gx.cpu().numpy()[idx], anteile) if len(rows) > 1
+              else [(-np.inf, np.inf)])
     h = max(1e-4, 2.0 * v95 * dt_y)
     halo = 2.0 * h + 2.0 * rr_max
     parts, per_card = [], []

@@ -112,22 +112,41 @@ def minmax(arrays):
     return [(float(h[2 * c]), float(h[2 * c + 1])) for c in range(len(arrays))]
 
 
+_DEPOSIT_WORK: dict = {}
+
+
 def deposit(pos, m, grid_n, origin, h, periodic, prec=F64, mesh=None,
-            cells=None):
-    """CIC mass deposit; returns the mesh (shape grid_n, FP64/FP32)."""
+            cells=None, tiled=True):
+    """CIC mass deposit; returns the mesh (shape grid_n, FP64/FP32).
+    tiled (default): ssim_cic_deposit_tiles — int64 fixed-point sums in
+    LDS tiles, every node written once, bit-identical from run to run;
+    False: ssim_cic_deposit, FP atomics on the mesh (order not fixed, like
+    the reference's cp.bincount)."""
     torch = _torch()
     dim = len(pos)
     dev = _dev_of(pos[0])
     if mesh is None:
-        mesh = torch.zeros(list(grid_n), dtype=_mesh_dtype(prec), device=dev)
-    else:
+        mesh = (torch.empty if tiled else torch.zeros)(list(grid_n), dtype=_mesh_dtype(prec),
+                                                       device=dev)
+    elif not tiled:
         mesh.zero_()
     gn = _lib.int_array(grid_n)
     org = _lib.double_array(origin)
     ps = [_lib.ptr(p) for p in pos] + [None] * (3 - dim)
-    _lib.call("ssim_cic_deposit", dim, int(periodic), pos[0].numel(), *ps,
+    n = pos[0].numel()
+    if not tiled:
+        _lib.call("ssim_cic_deposit", dim, int(periodic), n, *ps,
+                  _lib.ptr(m), _addr(gn), _addr(org), float(h), prec,
+                  _lib.ptr(mesh), _lib.ptr(cells), D.stream(dev))
+        return mesh
+    key = (str(dev), dim, n, tuple(int(g) for g in grid_n))
+    w = _DEPOSIT_WORK.get(key)
+    if w is None:
+        nb = _lib.load().ssim_cic_deposit_tiles_work_bytes(dim, n, _addr(gn))
+        w = _DEPOSIT_WORK[key] = D.workspace(nb, dev)
+    _lib.call("ssim_cic_deposit_tiles", dim, int(periodic), n, *ps,
               _lib.ptr(m), _addr(gn), _addr(org), float(h), prec,
-              _lib.ptr(mesh), _lib.ptr(cells), D.stream(dev))
+              _lib.ptr(mesh), _lib.ptr(cells), _lib.ptr(w), w.numel(), D.stream(dev))
     return mesh
 
 

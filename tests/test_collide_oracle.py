@@ -53,3 +53,32 @@ def test_order_is_offset_then_row():
     # decreases inside one offset block, so it can only drop 4 times
     drops = int(np.sum(np.diff(ci) < 0))
     assert drops <= 4
+
+
+def percentile_f32(values, q):
+    """NumPy's 'linear' percentile of an FP32 array restated in FP32 (the
+    rule ssim_speed_percentile runs on the device): virtual index
+    (m - 1) * float32(q / 100), gamma its fraction, a + (b - a) gamma, or
+    b - (b - a)(1 - gamma) when gamma >= 1/2."""
+    v = np.sort(np.asarray(values, np.float32))
+    m = len(v)
+    qf = np.float32(q) / np.float32(100)
+    vi = np.float32(m - 1) * qf
+    lo = int(np.floor(vi))
+    hi = min(lo + 1, m - 1)
+    g = np.float32(vi - np.float32(lo))
+    a, b = v[lo], v[hi]
+    d = np.float32(b - a)
+    return np.float32(b - d * np.float32(1 - g)) if g >= 0.5 else np.float32(a + d * g)
+
+
+def test_percentile_restatement_matches_numpy():
+    """The FP32 restatement equals np.percentile bit for bit (sizes where
+    the virtual index has a fraction above, below and at 1/2, and m = 1)."""
+    rng = np.random.default_rng(3)
+    for m in (1, 2, 3, 7, 20, 21, 101, 1000, 50_000):
+        x = rng.gamma(2.0, 1.5, m).astype(np.float32)
+        for q in (95, 50, 5, 100, 0, 37.5):
+            ref = np.percentile(x, q)
+            assert ref.dtype == np.float32
+            assert percentile_f32(x, q) == ref, (m, q)
