@@ -205,6 +205,23 @@ int ssim_energy(int dim, int n, const double* x, const double* y,
                 const double* vz, const double* m, double G, double eps2,
                 double* out, void* work, size_t work_bytes, void* stream);
 
+/* ssim_energy without a workspace and without any allocation: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops that report the
+ * energy every step, harnesses that forbid hipMalloc).  The per-workgroup
+ * partials live in a fixed device array inside the library (2 MiB per GPU,
+ * held until the process ends), so n is limited to
+ * SSIM_ENERGY_NOALLOC_MAX_N bodies (larger n: SSIM_EINVAL, use
+ * ssim_energy).  Results are bit-identical to ssim_energy's.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (different
+ * streams, or host threads without ordering) race and must be serialised
+ * by the caller. */
+#define SSIM_ENERGY_NOALLOC_MAX_N (1 << 24)
+int ssim_energy_noalloc(int dim, int n, const double* x, const double* y,
+                        const double* z, const double* vx, const double* vy,
+                        const double* vz, const double* m, double G, double eps2,
+                        double* out, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* FFT (real <-> half-complex, NumPy layout and normalisation)         */
 /* ------------------------------------------------------------------ */

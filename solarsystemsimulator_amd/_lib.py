@@ -21,6 +21,9 @@ SSIM_OK = 0
 SSIM_EINVAL = -1
 SSIM_EHIP = -2
 
+# largest n of ssim_energy_noalloc (SSIM_ENERGY_NOALLOC_MAX_N in ssim.h)
+SSIM_ENERGY_NOALLOC_MAX_N = 1 << 24
+
 PREC_F64 = 0
 PREC_F32C = 1
 SCHEME_KDK = 0
@@ -64,6 +67,8 @@ SIGNATURES = {
     "ssim_energy_work_bytes": (c_size_t, [c_int, c_int]),
     "ssim_energy": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, _P, c_double,
                             c_double, _P, _P, c_size_t, _P]),
+    "ssim_energy_noalloc": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, _P,
+                                    c_double, c_double, _P, _P]),
     "ssim_fft_work_bytes": (c_size_t, [c_int, _P, c_int]),
     "ssim_fft_prepare": (c_int, [c_int, _P, c_int, _P, c_size_t, _P]),
     "ssim_fft_r2c": (c_int, [c_int, _P, c_int, _P, _P, _P, c_size_t, _P]),

@@ -11,6 +11,8 @@
 // partials in block order, so the result does not depend on scheduling.
 #include "ssim_internal.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int EN_BLOCK = 256;
@@ -120,21 +122,28 @@ size_t ssim_energy_work_bytes(int dim, int n) {
   return ssim::align256((size_t)ssim::ceil_div(n, EN_BLOCK) * 4 * sizeof(double));
 }
 
-int ssim_energy(int dim, int n, const double* x, const double* y,
-                const double* z, const double* vx, const double* vy,
-                const double* vz, const double* m, double G, double eps2,
-                double* out, void* work, size_t work_bytes, void* stream) {
+}  // extern "C"
+
+namespace {
+
+int energy_args_ok(int dim, int n, const double* x, const double* y, const double* z,
+                   const double* vx, const double* vy, const double* vz, const double* m,
+                   double eps2, const double* out) {
   SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3, got %d", dim);
   SSIM_REQUIRE(n >= 1, "empty state (n=%d)", n);
   SSIM_REQUIRE(eps2 > 0.0, "softening eps2 must be > 0, got %g", eps2);
   SSIM_REQUIRE(x && y && vx && vy && m && out, "null pointer");
   SSIM_REQUIRE(dim == 2 || (z && vz), "dim 3 needs z pointers");
-  const size_t need = ssim_energy_work_bytes(dim, n);
-  SSIM_REQUIRE(work && work_bytes >= need, "workspace too small: %zu < %zu",
-               work_bytes, need);
-  hipStream_t st = ssim::as_stream(stream);
+  return SSIM_OK;
+}
+
+// The two launches of one energy evaluation; part holds
+// ceil(n / EN_BLOCK) * 4 doubles, every one of them written by the first
+// launch before the second reads it (no initial contents needed).
+int energy_launch(int dim, int n, const double* x, const double* y, const double* z,
+                  const double* vx, const double* vy, const double* vz, const double* m,
+                  double G, double eps2, double* out, double* part, hipStream_t st) {
   const int nblk = ssim::ceil_div(n, EN_BLOCK);
-  double* part = static_cast<double*>(work);
   if (dim == 2)
     hipLaunchKernelGGL(k_energy_partial<2>, dim3(nblk), dim3(EN_BLOCK), 0, st,
                        n, x, y, z, vx, vy, vz, m, eps2, part);
@@ -146,6 +155,65 @@ int ssim_energy(int dim, int n, const double* x, const double* y,
                      out);
   SSIM_LAUNCHED("k_energy_final");
   return SSIM_OK;
+}
+
+// Scratch of ssim_energy_noalloc: the per-workgroup partials of up to
+// SSIM_ENERGY_NOALLOC_MAX_N bodies (2 MiB), one copy per GPU for the life of the
+// process.  Host code never passes the array's name to a kernel (on the
+// host it is only a stand-in for the symbol): the device address comes from
+// hipGetSymbolAddress, once per device, and is kept in g_scratch_dev.
+constexpr int NOALLOC_MAX_BLK = SSIM_ENERGY_NOALLOC_MAX_N / EN_BLOCK;
+__device__ __align__(256) double scratch[NOALLOC_MAX_BLK * 4];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[MAX_DEVICES] = {};
+
+// The device address of this GPU's scratch array (looked up on the first
+// call on each device, with that device current).
+int scratch_for_current_device(double** out) {
+  int dev = -1;
+  SSIM_HIP(hipGetDevice(&dev));
+  SSIM_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    SSIM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SSIM_REQUIRE(p != nullptr, "no device address for the energy scratch array");
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return SSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssim_energy(int dim, int n, const double* x, const double* y,
+                const double* z, const double* vx, const double* vy,
+                const double* vz, const double* m, double G, double eps2,
+                double* out, void* work, size_t work_bytes, void* stream) {
+  SSIM_TRY(energy_args_ok(dim, n, x, y, z, vx, vy, vz, m, eps2, out));
+  const size_t need = ssim_energy_work_bytes(dim, n);
+  SSIM_REQUIRE(work && work_bytes >= need, "workspace too small: %zu < %zu",
+               work_bytes, need);
+  return energy_launch(dim, n, x, y, z, vx, vy, vz, m, G, eps2, out,
+                       static_cast<double*>(work), ssim::as_stream(stream));
+}
+
+int ssim_energy_noalloc(int dim, int n, const double* x, const double* y,
+                        const double* z, const double* vx, const double* vy,
+                        const double* vz, const double* m, double G, double eps2,
+                        double* out, void* stream) {
+  SSIM_TRY(energy_args_ok(dim, n, x, y, z, vx, vy, vz, m, eps2, out));
+  SSIM_REQUIRE(n <= SSIM_ENERGY_NOALLOC_MAX_N,
+               "ssim_energy_noalloc: n = %d exceeds its scratch limit of %d bodies "
+               "(use ssim_energy with a workspace)", n, SSIM_ENERGY_NOALLOC_MAX_N);
+  double* part = nullptr;
+  SSIM_TRY(scratch_for_current_device(&part));
+  return energy_launch(dim, n, x, y, z, vx, vy, vz, m, G, eps2, out, part,
+                       ssim::as_stream(stream));
 }
 
 }  // extern "C"
