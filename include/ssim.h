@@ -170,6 +170,27 @@ int ssim_nbody_frame(int n_ast, double* ax, double* ay, double* avx,
                      unsigned char* sub_n, float* hit_t, int* hit_m,
                      void* work, size_t work_bytes, void* stream);
 
+/* The same raster integrator in space (D = 3): every formula of
+ * ssim_nbody_frame with a z component added (|d|^2 = dx^2 + dy^2 + dz^2,
+ * hot classification, encounter times and contact distances in 3-D); the
+ * reference runs only the plane (nbody_kernel.py:97-560 has x, y), so this
+ * entry is its 3-D generalisation for scenes with inclinations.  Layouts:
+ * snap f32 [k][6][n_ast] and snap_m f32 [k][6][m] as x, y, z, vx, vy, vz;
+ * sub_pos f32 [k][m_sub][3][n_ast]; work: ssim_nbody_work_bytes(n_ast). */
+int ssim_nbody_frame3d(int n_ast, double* ax, double* ay, double* az,
+                       double* avx, double* avy, double* avz, double* aacc_x,
+                       double* aacc_y, double* aacc_z, const double* am,
+                       const unsigned char* avis, const double* arad, int m,
+                       double* mx, double* my, double* mz, double* mvx,
+                       double* mvy, double* mvz, double* macc_x, double* macc_y,
+                       double* macc_z, const double* mm,
+                       const unsigned char* mvis, const double* mrad, double G,
+                       double soft, double dt_raster, int k, double max_sub_dt,
+                       int max_sub_steps, double w1, double w0, int m_sub,
+                       float* snap, float* snap_m, float* sub_pos,
+                       unsigned char* sub_n, float* hit_t, int* hit_m,
+                       void* work, size_t work_bytes, void* stream);
+
 /* Film ring slots of k raster samples, written on the device from the
  * outputs of ssim_nbody_frame: replaces the host side of
  * film_producer.py:81-123 (`slot_bytes`, `schreibe_slot`) applied to

@@ -114,48 +114,65 @@ def nbody_frames(x, y, vx, vy, mass, visible, is_ast, real_r, dt_raster, k,
     returns (out f32 (k, 4n) in original order, state f64 (4n), sub,
     hits) with sub/hits shaped like NBodyCuda's st["sub"] / st["hits"].
     The inputs are not modified."""
+    return _frames(2, [x, y], [vx, vy], mass, visible, is_ast, real_r,
+                   dt_raster, k, m_sub, G, soft, max_sub_dt, max_sub_steps)
+
+
+def nbody_frames3d(x, y, z, vx, vy, vz, mass, visible, is_ast, real_r,
+                   dt_raster, k, m_sub=8, G=4 * np.pi ** 2, soft=1e-6,
+                   max_sub_dt=0.5 / 365.25, max_sub_steps=100_000):
+    """The same batch in space: out f32 (k, 6n) [x|y|z|vx|vy|vz], state f64
+    (6n), sub pos (m_sub, 3, nh)."""
+    return _frames(3, [x, y, z], [vx, vy, vz], mass, visible, is_ast, real_r,
+                   dt_raster, k, m_sub, G, soft, max_sub_dt, max_sub_steps)
+
+
+def _frames(dim, pos, vel, mass, visible, is_ast, real_r, dt_raster, k,
+            m_sub, G, soft, max_sub_dt, max_sub_steps):
     c = ctypes
+    F = 2 * dim
     ast = np.asarray(is_ast) != 0
     m_idx, a_idx = np.flatnonzero(~ast), np.flatnonzero(ast)
     na, m = len(a_idx), len(m_idx)
-    n = len(x)
-    cols = [np.asarray(v, np.float64) for v in (x, y, vx, vy, mass)]
+    n = len(pos[0])
+    cols = [np.asarray(v, np.float64) for v in list(pos) + list(vel) + [mass]]
     a = [_f64(v[a_idx]).copy() for v in cols]
     M = [_f64(v[m_idx]).copy() for v in cols]
     vis = np.asarray(visible).astype(np.uint8)
     rr = np.zeros(n) if real_r is None else np.asarray(real_r, np.float64)
     avis, mvis = np.ascontiguousarray(vis[a_idx]), np.ascontiguousarray(vis[m_idx])
     arad, mrad = _f64(rr[a_idx]).copy(), _f64(rr[m_idx]).copy()
-    aacc = [np.zeros(max(na, 1)) for _ in range(2)]
-    macc = [np.zeros(max(m, 1)) for _ in range(2)]
+    aacc = [np.zeros(max(na, 1)) for _ in range(dim)]
+    macc = [np.zeros(max(m, 1)) for _ in range(dim)]
     ms = max(m_sub, 1)
-    snap = np.zeros(k * 4 * max(na, 1), np.float32)
-    snapm = np.zeros(k * 4 * max(m, 1), np.float32)
-    subpos = np.zeros(k * ms * 2 * max(na, 1), np.float32)
+    snap = np.zeros(k * F * max(na, 1), np.float32)
+    snapm = np.zeros(k * F * max(m, 1), np.float32)
+    subpos = np.zeros(k * ms * dim * max(na, 1), np.float32)
     subn = np.zeros(k * max(na, 1), np.uint8)
     hitt = np.full(max(na, 1), -1.0, np.float32)
     hitm = np.zeros(max(na, 1), np.int32)
     hot = np.zeros(max(na, 1), np.uint8)
     w1 = 1.0 / (2.0 - 2.0 ** (1.0 / 3.0))
     w0 = -(2.0 ** (1.0 / 3.0)) / (2.0 - 2.0 ** (1.0 / 3.0))
-    fn = lib().oracle_nbody_frame
+    fn = lib().oracle_nbody_frame if dim == 2 else lib().oracle_nbody_frame3d
     P = c.c_void_p
-    fn.argtypes = ([c.c_int] + [P] * 9 + [c.c_int] + [P] * 9 +
+    nv = 3 * dim + 3                     # pos, vel, acc, mass, vis, radius
+    fn.argtypes = ([c.c_int] + [P] * nv + [c.c_int] + [P] * nv +
                    [c.c_double, c.c_double, c.c_double, c.c_int, c.c_double,
                     c.c_int, c.c_double, c.c_double, c.c_int] + [P] * 7)
-    fn(na, *[_p(v) for v in a[:4]], _p(aacc[0]), _p(aacc[1]), _p(a[4]),
-       _p(avis), _p(arad), m, *[_p(v) for v in M[:4]], _p(macc[0]),
-       _p(macc[1]), _p(M[4]), _p(mvis), _p(mrad), G, soft, dt_raster, k,
+    fn(na, *[_p(v) for v in a[:F]], *[_p(v) for v in aacc], _p(a[F]),
+       _p(avis), _p(arad), m, *[_p(v) for v in M[:F]], *[_p(v) for v in macc],
+       _p(M[F]), _p(mvis), _p(mrad), G, soft, dt_raster, k,
        max_sub_dt, max_sub_steps, w1, w0, m_sub, _p(snap), _p(snapm),
        _p(subpos), _p(subn), _p(hitt), _p(hitm), _p(hot))
-    out = np.empty((k, 4 * n), np.float32)
-    sn = snap[:k * 4 * na].reshape(k, 4, na)
-    smn = snapm[:k * 4 * m].reshape(k, 4, m)
-    for f in range(4):
+    out = np.empty((k, F * n), np.float32)
+    sn = snap[:k * F * na].reshape(k, F, na)
+    smn = snapm[:k * F * m].reshape(k, F, m)
+    for f in range(F):
         out[:, f * n + a_idx] = sn[:, f, :]
         out[:, f * n + m_idx] = smn[:, f, :]
-    state = np.empty(4 * n)
-    for f in range(4):
+    state = np.empty(F * n)
+    for f in range(F):
         state[f * n + a_idx] = a[f][:na]
         state[f * n + m_idx] = M[f][:m]
     sel = np.flatnonzero(hitt[:na] >= 0)
@@ -164,8 +181,8 @@ def nbody_frames(x, y, vx, vy, mass, visible, is_ast, real_r, dt_raster, k,
     if m_sub > 0:
         sub = []
         subn_k = subn[:k * na].reshape(k, na)
-        pos = subpos[:k * ms * 2 * na].reshape(k, ms, 2, na)
+        pos_ = subpos[:k * ms * dim * na].reshape(k, ms, dim, na)
         for ks in range(k):
             s = np.flatnonzero(subn_k[ks] == m_sub)
-            sub.append((a_idx[s], pos[ks][:, :, s]))
+            sub.append((a_idx[s], pos_[ks][:, :, s]))
     return out, state, sub, hits

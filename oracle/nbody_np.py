@@ -19,54 +19,71 @@ YOSHIDA_W0 = -(2.0 ** (1.0 / 3.0)) / (2.0 - 2.0 ** (1.0 / 3.0))
 
 
 def reference_advance(dt_years, x, y, vx, vy, mass, visible, is_ast):
-    x, y, vx, vy = (np.asarray(a, np.float64).copy() for a in (x, y, vx, vy))
+    (x, y), (vx, vy) = advance_nd(dt_years, [x, y], [vx, vy], mass, visible,
+                                  is_ast)
+    return x, y, vx, vy
+
+
+def advance_nd(dt_years, pos, vel, mass, visible, is_ast, G=G_AU,
+               soft=SOFTENING, max_sub_dt=MAX_SUB_DT_YEARS):
+    """`reference_advance` for 2 or 3 components (positions `pos`, velocities
+    `vel`: lists of arrays); |d|^2 summed x, y, (z), and for 2 components
+    exactly the reference's operations (hypot distances)."""
+    dim = len(pos)
+    p = [np.asarray(a, np.float64).copy() for a in pos]
+    v = [np.asarray(a, np.float64).copy() for a in vel]
     vis = np.asarray(visible) != 0
     ast = np.asarray(is_ast) != 0
     mas_i = np.flatnonzero(~ast & vis)
 
+    def norm(d):
+        if dim == 2:
+            return np.hypot(d[0], d[1])
+        return np.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])
+
     def accel():
-        ax = np.zeros_like(x)
-        ay = np.zeros_like(y)
+        acc = [np.zeros_like(p[0]) for _ in range(dim)]
         for i in mas_i:
-            dx = x - x[i]
-            dy = y - y[i]
-            r2 = dx * dx + dy * dy + SOFTENING
-            f = G_AU / (r2 * np.sqrt(r2))
+            d = [c - c[i] for c in p]
+            r2 = d[0] * d[0]
+            for c in range(1, dim):
+                r2 = r2 + d[c] * d[c]
+            r2 = r2 + soft
+            f = G / (r2 * np.sqrt(r2))
             f[i] = 0.0
             f[~vis] = 0.0
             sel = np.ones_like(f, dtype=bool)
             sel[~vis] = False
             sel[i] = False
             sel[mas_i[mas_i < i]] = False
-            ax[sel] += f[sel] * mass[i] * (-dx[sel])
-            ay[sel] += f[sel] * mass[i] * (-dy[sel])
-            ax[i] += np.sum(f[sel] * mass[sel] * dx[sel])
-            ay[i] += np.sum(f[sel] * mass[sel] * dy[sel])
-        return ax, ay
+            for c in range(dim):
+                acc[c][sel] += f[sel] * mass[i] * (-d[c][sel])
+                acc[c][i] += np.sum(f[sel] * mass[sel] * d[c][sel])
+        return acc
 
     def adapt_dt():
-        dt = MAX_SUB_DT_YEARS
+        dt = max_sub_dt
         for i in mas_i:
             sel = vis.copy()
             sel[i] = False
             sel[mas_i[mas_i < i]] = False
-            dist = np.maximum(np.hypot(x[sel] - x[i], y[sel] - y[i]), 1e-12)
-            vrel = np.hypot(vx[sel] - vx[i], vy[sel] - vy[i])
+            dist = np.maximum(norm([c[sel] - c[i] for c in p]), 1e-12)
+            vrel = norm([c[sel] - c[i] for c in v])
             ok = vrel > 1e-9
             if ok.any():
                 dt = min(dt, float(np.min(dist[ok] / vrel[ok])) / 20.0)
-        return max(dt, MAX_SUB_DT_YEARS / 1000.0)
+        return max(dt, max_sub_dt / 1000.0)
 
-    ax, ay = accel()
+    acc = accel()
 
     def verlet(dt):
-        nonlocal ax, ay
-        x[vis] += vx[vis] * dt + 0.5 * ax[vis] * dt * dt
-        y[vis] += vy[vis] * dt + 0.5 * ay[vis] * dt * dt
-        nax, nay = accel()
-        vx[vis] += 0.5 * (ax[vis] + nax[vis]) * dt
-        vy[vis] += 0.5 * (ay[vis] + nay[vis]) * dt
-        ax, ay = nax, nay
+        nonlocal acc
+        for c in range(dim):
+            p[c][vis] += v[c][vis] * dt + 0.5 * acc[c][vis] * dt * dt
+        nacc = accel()
+        for c in range(dim):
+            v[c][vis] += 0.5 * (acc[c][vis] + nacc[c][vis]) * dt
+        acc = nacc
 
     remaining, guard = dt_years, 0
     while remaining > 1e-12 and guard < 100_000:
@@ -75,7 +92,7 @@ def reference_advance(dt_years, x, y, vx, vy, mass, visible, is_ast):
         for w in (YOSHIDA_W1, YOSHIDA_W0, YOSHIDA_W1):
             verlet(w * sub)
         remaining -= sub
-    return x, y, vx, vy
+    return p, v
 
 
 def make_system(n_ast: int, seed: int = 7):

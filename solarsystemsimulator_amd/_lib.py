@@ -64,6 +64,10 @@ SIGNATURES = {
                          [c_double, c_double, c_double, c_int, c_double, c_int,
                           c_double, c_double, c_int] + [_P] * 6 +
                          [_P, c_size_t, _P]),
+    "ssim_nbody_frame3d": (c_int, [c_int] + [_P] * 12 + [c_int] + [_P] * 12 +
+                           [c_double, c_double, c_double, c_int, c_double, c_int,
+                            c_double, c_double, c_int] + [_P] * 6 +
+                           [_P, c_size_t, _P]),
     "ssim_energy_work_bytes": (c_size_t, [c_int, c_int]),
     "ssim_energy": (c_int, [c_int, c_int, _P, _P, _P, _P, _P, _P, _P, c_double,
                             c_double, _P, _P, c_size_t, _P]),

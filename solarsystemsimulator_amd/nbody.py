@@ -56,6 +56,12 @@ def pick_detect_devices(exclude, rank: int = 0, count: int = 1) -> list[int]:
 class NBodyCuda:
     """Drop-in for nbody_kernel.NBodyCuda."""
 
+    dim = 2
+    # physical constants of the reference's scheme (nbody_kernel.py:58-65)
+    G = G_AU
+    soft = SOFTENING
+    max_sub_dt = MAX_SUB_DT_YEARS
+
     def __init__(self, devices, m_sub=SUB_SAMPLES):
         if isinstance(devices, (int, np.integer)):
             devices = [int(devices)]
@@ -75,17 +81,21 @@ class NBodyCuda:
     # ---------------- state ----------------
     def load_state(self, x, y, vx, vy, mass, visible, is_ast,
                    real_r=None) -> dict:
+        return self._load([x, y], [vx, vy], mass, visible, is_ast, real_r)
+
+    def _load(self, pos, vel, mass, visible, is_ast, real_r) -> dict:
         torch = _torch()
+        F = 2 * self.dim
         ast = np.asarray(is_ast) != 0
         m_idx = np.flatnonzero(~ast)
         a_idx = np.flatnonzero(ast)
         m = len(m_idx)
         if m > M_MAX:
             raise ValueError(f"zu viele massive Koerper: {m} > {M_MAX}")
-        n = len(x)
+        n = len(pos[0])
         na = len(a_idx)
         dev = self._dev
-        cols = [np.asarray(c, np.float64) for c in (x, y, vx, vy, mass)]
+        cols = [np.asarray(c, np.float64) for c in list(pos) + list(vel) + [mass]]
         vis = np.asarray(visible).astype(np.uint8)
         rr = (np.zeros(n) if real_r is None else np.asarray(real_r, np.float64))
 
@@ -93,20 +103,21 @@ class NBodyCuda:
             return D.to_dev(arr, dev, dtype) if len(arr) else \
                 torch.zeros(1, dtype=dtype, device=dev)
         st = {"N": n, "m": m, "n_ast": na, "m_idx_h": m_idx, "a_idx_h": a_idx,
-              "a": [up(c[a_idx]) for c in cols],          # x y vx vy m
+              "dim": self.dim,
+              "a": [up(c[a_idx]) for c in cols],          # positions velocities m
               "M": [up(c[m_idx]) for c in cols],
               "avis": up(vis[a_idx], torch.uint8),
               "mvis": up(vis[m_idx], torch.uint8),
               "arad": up(rr[a_idx]), "mrad": up(rr[m_idx]),
               "aacc": [torch.zeros(max(na, 1), dtype=torch.float64, device=dev)
-                       for _ in range(2)],
+                       for _ in range(self.dim)],
               "macc": [torch.zeros(max(m, 1), dtype=torch.float64, device=dev)
-                       for _ in range(2)]}
+                       for _ in range(self.dim)]}
         ms = max(self.m_sub, 1)
-        st["snap"] = torch.empty(K_MAX * 4 * max(na, 1), dtype=torch.float32, device=dev)
-        st["snapM"] = torch.empty(K_MAX * 4 * max(m, 1), dtype=torch.float32, device=dev)
-        st["subpos"] = torch.zeros(K_MAX * ms * 2 * max(na, 1), dtype=torch.float32,
-                                   device=dev)
+        st["snap"] = torch.empty(K_MAX * F * max(na, 1), dtype=torch.float32, device=dev)
+        st["snapM"] = torch.empty(K_MAX * F * max(m, 1), dtype=torch.float32, device=dev)
+        st["subpos"] = torch.zeros(K_MAX * ms * self.dim * max(na, 1),
+                                   dtype=torch.float32, device=dev)
         st["subn"] = torch.zeros(K_MAX * max(na, 1), dtype=torch.uint8, device=dev)
         st["hitt"] = torch.full((max(na, 1),), -1.0, dtype=torch.float32, device=dev)
         st["hitm"] = torch.zeros(max(na, 1), dtype=torch.int32, device=dev)
@@ -129,10 +140,11 @@ class NBodyCuda:
             raise ValueError(f"k ausserhalb 1..{K_MAX}: {k}")
         self._run_frames(st, dt_raster_years, k)
         na, m, n = st["n_ast"], st["m"], st["N"]
-        out = np.empty((k, 4 * n), np.float32)
-        snap = st["snap"][:k * 4 * na].cpu().numpy().reshape(k, 4, na)
-        snapm = st["snapM"][:k * 4 * m].cpu().numpy().reshape(k, 4, m)
-        for f in range(4):
+        F = 2 * self.dim
+        out = np.empty((k, F * n), np.float32)
+        snap = st["snap"][:k * F * na].cpu().numpy().reshape(k, F, na)
+        snapm = st["snapM"][:k * F * m].cpu().numpy().reshape(k, F, m)
+        for f in range(F):
             out[:, f * n + st["a_idx_h"]] = snap[:, f, :]
             out[:, f * n + st["m_idx_h"]] = snapm[:, f, :]
         st["sub"] = self._collect_sub(st, k)
@@ -158,6 +170,8 @@ class NBodyCuda:
         torch = _torch()
         if st is None:
             raise ValueError("kein Zustand geladen — FULL-Frame noetig")
+        if self.dim != 2:
+            raise ValueError("film ring slots carry the plane (x, y) only")
         if not 1 <= k <= K_MAX:
             raise ValueError(f"k ausserhalb 1..{K_MAX}: {k}")
         if sub_max < 0:
@@ -204,22 +218,25 @@ class NBodyCuda:
         return out, (tot.numpy() > sub_max) & (sub_max > 0)
 
     def _run_frames(self, st: dict, dt_raster_years: float, k: int) -> None:
-        na, m, n = st["n_ast"], st["m"], st["N"]
+        na, m = st["n_ast"], st["m"]
+        F = 2 * self.dim
         s = D.stream(self._dev)
         _lib.call("ssim_fill_f32", max(na, 1), _lib.ptr(st["hitt"]), -1.0, s)
         a, M_ = st["a"], st["M"]
-        _lib.call("ssim_nbody_frame", na, *[_lib.ptr(c) for c in a[:4]],
-                  _lib.ptr(st["aacc"][0]), _lib.ptr(st["aacc"][1]),
-                  _lib.ptr(a[4]), _lib.ptr(st["avis"]), _lib.ptr(st["arad"]),
-                  m, *[_lib.ptr(c) for c in M_[:4]], _lib.ptr(st["macc"][0]),
-                  _lib.ptr(st["macc"][1]), _lib.ptr(M_[4]), _lib.ptr(st["mvis"]),
-                  _lib.ptr(st["mrad"]), G_AU, SOFTENING, float(dt_raster_years),
-                  int(k), MAX_SUB_DT_YEARS, MAX_SUB_STEPS_PER_FRAME, YOSHIDA_W1,
-                  YOSHIDA_W0, self.m_sub, _lib.ptr(st["snap"]),
-                  _lib.ptr(st["snapM"]), _lib.ptr(st["subpos"]),
-                  _lib.ptr(st["subn"]), _lib.ptr(st["hitt"]),
-                  _lib.ptr(st["hitm"]), _lib.ptr(st["work"]),
-                  st["work"].numel(), s)
+        entry = "ssim_nbody_frame" if self.dim == 2 else "ssim_nbody_frame3d"
+        _lib.call(entry, na, *[_lib.ptr(c) for c in a[:F]],
+                  *[_lib.ptr(c) for c in st["aacc"]],
+                  _lib.ptr(a[F]), _lib.ptr(st["avis"]), _lib.ptr(st["arad"]),
+                  m, *[_lib.ptr(c) for c in M_[:F]],
+                  *[_lib.ptr(c) for c in st["macc"]],
+                  _lib.ptr(M_[F]), _lib.ptr(st["mvis"]),
+                  _lib.ptr(st["mrad"]), float(self.G), float(self.soft),
+                  float(dt_raster_years), int(k), float(self.max_sub_dt),
+                  MAX_SUB_STEPS_PER_FRAME, YOSHIDA_W1, YOSHIDA_W0, self.m_sub,
+                  _lib.ptr(st["snap"]), _lib.ptr(st["snapM"]),
+                  _lib.ptr(st["subpos"]), _lib.ptr(st["subn"]),
+                  _lib.ptr(st["hitt"]), _lib.ptr(st["hitm"]),
+                  _lib.ptr(st["work"]), st["work"].numel(), s)
 
     def _collect_hits(self, st):
         """(idx, t, partner): contacts found by the fine loop
@@ -235,23 +252,23 @@ class NBodyCuda:
                 st["m_idx_h"][hm[sel]].astype(np.int64))
 
     def _collect_sub(self, st, k):
-        """Per raster (idx, pos (mSub, 2, nh)) of asteroids with a complete
+        """Per raster (idx, pos (mSub, dim, nh)) of asteroids with a complete
         fine track, subN == mSub (nbody_kernel.py:894-934)."""
         if self.m_sub <= 0:
             return None
-        ms, na = self.m_sub, st["n_ast"]
+        ms, na, dim = self.m_sub, st["n_ast"], self.dim
         frames = []
         if not na:
-            return [(np.empty(0, np.int64), np.empty((ms, 2, 0), np.float32))] * k
+            return [(np.empty(0, np.int64), np.empty((ms, dim, 0), np.float32))] * k
         subn = st["subn"][:k * na].cpu().numpy().reshape(k, na)
         pos = None
         for ks in range(k):
             sel = np.flatnonzero(subn[ks] == ms)
             if sel.size == 0:
-                frames.append((np.empty(0, np.int64), np.empty((ms, 2, 0), np.float32)))
+                frames.append((np.empty(0, np.int64), np.empty((ms, dim, 0), np.float32)))
                 continue
             if pos is None:
-                pos = st["subpos"][:k * ms * 2 * na].cpu().numpy().reshape(k, ms, 2, na)
+                pos = st["subpos"][:k * ms * dim * na].cpu().numpy().reshape(k, ms, dim, na)
             frames.append((st["a_idx_h"][sel].astype(np.int64),
                            np.ascontiguousarray(pos[ks][:, :, sel])))
         return frames
@@ -278,12 +295,13 @@ class NBodyCuda:
                           D.stream(self._dev))
 
     def apply_updates(self, st, idx, vals) -> None:
-        """Set x/y/vx/vy of bodies (original indices) absolutely."""
-        self._scatter(st, idx, np.asarray(vals)[:, :4], add=False)
+        """Set x/y/vx/vy (x/y/z/vx/vy/vz in space) of bodies (original
+        indices) absolutely."""
+        self._scatter(st, idx, np.asarray(vals)[:, :2 * self.dim], add=False)
 
     def apply_deltas(self, st, idx, deltas) -> None:
-        """Add x/y/vx/vy corrections; idx must be unique."""
-        self._scatter(st, idx, np.asarray(deltas)[:, :4], add=True)
+        """Add position/velocity corrections; idx must be unique."""
+        self._scatter(st, idx, np.asarray(deltas)[:, :2 * self.dim], add=True)
 
     def apply_body_state(self, st, idx, x, y, vx, vy, mass) -> None:
         """Set one body completely (merges in the film producer)."""
@@ -308,15 +326,49 @@ class NBodyCuda:
         gi = torch.as_tensor(np.array([pos], np.int64)).to(self._dev)
         gv = torch.zeros(1, dtype=torch.float64, device=self._dev)
         _lib.call("ssim_scatter_f64", 1, _lib.ptr(gi), _lib.ptr(gv),
-                  _lib.ptr(st[which][4]), 0, D.stream(self._dev))
+                  _lib.ptr(st[which][2 * self.dim]), 0, D.stream(self._dev))
         vis[pos:pos + 1].zero_()
 
     def export_f64(self, st) -> np.ndarray:
-        n = st["N"]
-        out = np.empty(4 * n, dtype="<f8")
-        for f in range(4):
+        n, F = st["N"], 2 * self.dim
+        out = np.empty(F * n, dtype="<f8")
+        for f in range(F):
             if st["n_ast"]:
                 out[f * n + st["a_idx_h"]] = st["a"][f][:st["n_ast"]].cpu().numpy()
             if st["m"]:
                 out[f * n + st["m_idx_h"]] = st["M"][f][:st["m"]].cpu().numpy()
         return out
+
+
+class NBodyCuda3D(NBodyCuda):
+    """The raster integrator in space: the scheme of NBodyCuda with a z
+    component (libssim's ssim_nbody_frame3d).  The reference integrates the
+    plane only; this class serves scenes with inclinations.  Arrays gain z:
+    `load_state(x, y, z, vx, vy, vz, mass, visible, is_ast, real_r)`,
+    `step_batch` returns f32 (k, 6n) [x|y|z|vx|vy|vz], sub-samples are
+    (idx, pos (mSub, 3, nh)), `export_f64` is <f8 (6n), updates take six
+    columns.  G, the softening added to r^2 and the largest sub-step are
+    parameters (defaults: the reference's G = 4 pi^2, 1e-6, half a day in
+    years), so a scene can run in its own units (e.g. G = 1 with the time
+    unit 1 yr / 2 pi and Plummer softening eps^2 = 1e-12 for eps = 1e-6 AU)."""
+
+    dim = 3
+
+    def __init__(self, devices, m_sub=SUB_SAMPLES, G=G_AU, softening=SOFTENING,
+                 max_sub_dt=MAX_SUB_DT_YEARS):
+        super().__init__(devices, m_sub)
+        if not softening > 0.0:
+            raise ValueError(f"softening muss > 0 sein: {softening}")
+        if not max_sub_dt > 0.0:
+            raise ValueError(f"max_sub_dt muss > 0 sein: {max_sub_dt}")
+        self.G = float(G)
+        self.soft = float(softening)
+        self.max_sub_dt = float(max_sub_dt)
+
+    def load_state(self, x, y, z, vx, vy, vz, mass, visible, is_ast,
+                   real_r=None) -> dict:
+        return self._load([x, y, z], [vx, vy, vz], mass, visible, is_ast, real_r)
+
+    def apply_body_state(self, st, idx, x, y, z, vx, vy, vz, mass) -> None:
+        self._scatter(st, [int(idx)], np.array([[x, y, z, vx, vy, vz, mass]]),
+                      add=False)

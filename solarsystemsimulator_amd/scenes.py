@@ -106,6 +106,48 @@ def solar_system(n_test: int = 10_000, seed: int = 1):
     return x, y, vx, vy, m, visible, is_ast
 
 
+def _kepler_3d(a, e, inc, node, peri, nu, gm):
+    """Position and velocity on the ellipse (a, e) at true anomaly nu, in
+    the plane inclined by inc with ascending node `node` and argument of
+    periapsis `peri`: the perifocal state rotated by Rz(node) Rx(inc)
+    Rz(peri).  Returns (x, y, z, vx, vy, vz)."""
+    p = a * (1.0 - e * e)
+    r = p / (1.0 + e * np.cos(nu))
+    xp, yp = r * np.cos(nu), r * np.sin(nu)
+    s = np.sqrt(gm / p)
+    vxp, vyp = -s * np.sin(nu), s * (e + np.cos(nu))
+    cO, sO, ci, si = np.cos(node), np.sin(node), np.cos(inc), np.sin(inc)
+    cw, sw = np.cos(peri), np.sin(peri)
+    r11, r12 = cO * cw - sO * sw * ci, -cO * sw - sO * cw * ci
+    r21, r22 = sO * cw + cO * sw * ci, -sO * sw + cO * cw * ci
+    r31, r32 = sw * si, cw * si
+    return (r11 * xp + r12 * yp, r21 * xp + r22 * yp, r31 * xp + r32 * yp,
+            r11 * vxp + r12 * vyp, r21 * vxp + r22 * vyp, r31 * vxp + r32 * vyp)
+
+
+def solar_system_3d(n_test: int = 10_000, seed: int = 1, G: float = 1.0):
+    """The solar system in space, in units G = 1, AU, solar masses (time
+    unit 1 yr / 2 pi): star of mass 1 at rest, the 8 planets with
+    e ~ U[0, 0.1], i ~ U[0, 3 deg], n_test massless test particles with
+    a ~ U[2.1, 3.3] AU, e ~ U[0, 0.2], i ~ U[0, 10 deg]; nodes, periapses
+    and phases uniform.  Returns (x, y, z, vx, vy, vz, mass, visible,
+    is_ast)."""
+    rng = np.random.default_rng(seed)
+    m = np.array([1.0] + [pm for pm, _ in PLANETS] + [0.0] * n_test)
+    a = np.array([pa for _, pa in PLANETS] + list(rng.uniform(2.1, 3.3, n_test)))
+    e = np.concatenate([rng.uniform(0.0, 0.1, 8), rng.uniform(0.0, 0.2, n_test)])
+    inc = np.radians(np.concatenate([rng.uniform(0.0, 3.0, 8),
+                                     rng.uniform(0.0, 10.0, n_test)]))
+    node, peri, nu = (rng.uniform(0.0, 2 * np.pi, 8 + n_test) for _ in range(3))
+    st = _kepler_3d(a, e, inc, node, peri, nu, G)
+    x, y, z, vx, vy, vz = (np.concatenate([[0.0], c]) for c in st)
+    n = len(m)
+    visible = np.ones(n, np.uint8)
+    is_ast = np.zeros(n, np.uint8)
+    is_ast[9:] = 1
+    return x, y, z, vx, vy, vz, m, visible, is_ast
+
+
 def asteroid_cloud(n: int = 50_000, seed: int = 4):
     """Star (mass 1) + Mercury..Mars + n asteroids in a Gaussian cloud
     (sigma 0.05 AU) at 2.5 AU moving with the circular velocity, dispersion
