@@ -210,6 +210,14 @@ int ssim_ring_slots(int k, int n, int n_ast, int m, int m_sub, int sub_max,
                     const unsigned* m_idx, unsigned char* slots, int* totals,
                     void* work, size_t work_bytes, void* stream);
 
+/* The k raster samples of ssim_nbody_frame as rows f32 [k][4][n]
+ * (x, y, vx, vy) in original body order: the array step_batch returns
+ * (nbody_kernel.py:836-856 scatters it on the host), kept on the device
+ * for the producer's detection.  a_idx / m_idx as for ssim_ring_slots. */
+int ssim_frame_rows(int k, int n, int na, int m, const float* snap,
+                    const float* snap_m, const unsigned* a_idx,
+                    const unsigned* m_idx, float* rows, void* stream);
+
 /* ------------------------------------------------------------------ */
 /* FP64 energy diagnostic                                              */
 /* ------------------------------------------------------------------ */

@@ -58,6 +58,7 @@ SIGNATURES = {
     "ssim_scatter_f64": (c_int, [c_int, _P, _P, _P, c_int, _P]),
     "ssim_ring_slot_bytes": (ctypes.c_longlong, [c_int, c_int, c_int]),
     "ssim_ring_work_bytes": (c_size_t, [c_int, c_int]),
+    "ssim_frame_rows": (c_int, [c_int, c_int, c_int, c_int, _P, _P, _P, _P, _P, _P]),
     "ssim_ring_slots": (c_int, [c_int] * 6 + [_P] * 8 + [_P, c_size_t, _P]),
     "ssim_nbody_work_bytes": (c_size_t, [c_int]),
     "ssim_nbody_frame": (c_int, [c_int] + [_P] * 9 + [c_int] + [_P] * 9 +

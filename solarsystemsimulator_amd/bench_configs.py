@@ -8,6 +8,12 @@ Prints one JSON object with a record per configuration (the headline
 * solar_system — star + 8 planets + 10,000 test particles (2-D), raster
   integrator (frame kernel, Yoshida-4 sub-steps of <= 0.5 day), dt = 1 day,
   1000 frames: frames/s and body-steps/s;
+* solar_system_3d — the same scene in space with inclinations, G = 1
+  units, Plummer eps = 1e-6 AU (the 3-D frame kernel);
+* film_belt_200k / film_clumps_200k — the film producer's batch loop on
+  the film benchmark's belt and interpenetrating-clumps scenes with
+  200,000 asteroids (200,001 bodies), 0.5-day raster: simulated days per
+  second with ring slots, collision search, merges and bounces;
 * allpairs_65k — 65,536 equal masses in the perturbed unit cube, Yoshida-4,
   dt = 1e-3, eps = 1e-3: ms/step in FP64 and in FP32 with compensated
   sums, and the FP32 path's force error against FP64 (median / p99);
@@ -59,6 +65,57 @@ def solar_system(frames):
     nf = batches * K_MAX
     return {"bodies": len(x), "frames": nf, "dt": "1 day", "s": t,
             "frames_per_s": nf / t, "body_frames_per_s": nf * len(x) / t}
+
+
+def solar_system_3d(frames):
+    """The solar system in space in its own units (G = 1, AU, solar masses,
+    one day = 2 pi / 365.25), inclinations, Plummer eps = 1e-6 AU."""
+    from .nbody import NBodyCuda3D
+    from .constants import K_MAX
+    sc = scenes.solar_system_3d(10_000, seed=1)
+    day = 2 * np.pi / 365.25
+    sim = NBodyCuda3D(0, G=1.0, softening=1e-12, max_sub_dt=0.5 * day)
+    st = sim.load_state(*sc)
+    sim.step_batch(st, day, K_MAX)                        # warm-up
+    batches = max(1, frames // K_MAX)
+    t, _ = timed(lambda: [sim.step_batch(st, day, K_MAX) for _ in range(batches)])
+    nf = batches * K_MAX
+    return {"bodies": len(sc[0]), "frames": nf, "dt": "1 day", "s": t,
+            "frames_per_s": nf / t, "body_frames_per_s": nf * len(sc[0]) / t}
+
+
+def film(scene, n_ast, batches):
+    """The film producer's loop (FilmProducer) on the film benchmark's seeded
+    scene: 0.5-day raster, K = 8 samples per batch, 8 sub-samples, the
+    collision search and bounces on, a ring of 1024 slots."""
+    from .nbody import NBodyCuda
+    from .producer import FilmProducer, sub_max_fuer
+    s = scene(n_ast)
+    n = len(s["x"])
+    sim = NBodyCuda(0, m_sub=8)
+    st = sim.load_state(s["x"], s["y"], s["vx"], s["vy"], s["mass"], s["visible"],
+                        s["is_ast"], s["real_r"])
+    sub_max = sub_max_fuer(n)
+    sb = int(_lib.load().ssim_ring_slot_bytes(n, 8, sub_max))
+    ring = np.zeros((1024, sb), np.uint8)
+    prod = FilmProducer(sim, st, ring, 0.5, mass=s["mass"], real_r=s["real_r"],
+                        visible=s["visible"], is_ast=s["is_ast"],
+                        is_star_bh=s["is_star_bh"], sub_max=sub_max, K=8)
+    prod.run(2)                                            # warm-up
+    k0, c0 = prod.k, prod.candidates
+    times0 = dict(prod.times)
+    t0 = time.perf_counter()
+    prod.run(batches)
+    t = time.perf_counter() - t0
+    samples = prod.k - k0
+    share = {key: (prod.times[key] - times0[key]) / t for key in prod.times}
+    out = {"bodies": n, "raster_days": 0.5, "samples": samples, "s": t,
+           "sim_days_per_s": samples * 0.5 / t,
+           "candidates_per_sample": (prod.candidates - c0) / samples,
+           "events": prod.n_events, "collisions": prod.collisions,
+           "time_share": {k: round(v, 3) for k, v in share.items()}}
+    prod.close()
+    return out
 
 
 def ring_slots_50k(batches):
@@ -191,6 +248,11 @@ def main(argv=None):
     q = args.quick
     res = {}
     for name, fn, arg in (("solar_system", solar_system, 160 if q else 1000),
+                          ("solar_system_3d", solar_system_3d, 160 if q else 1000),
+                          ("film_belt_200k", lambda b: film(scenes.film_belt, 200_000, b),
+                           4 if q else 20),
+                          ("film_clumps_200k",
+                           lambda b: film(scenes.film_clumps, 200_000, b), 4 if q else 20),
                           ("allpairs_65k", allpairs_65k, 5 if q else 20),
                           ("periodic_pm_262k", periodic_pm, 5 if q else 50),
                           ("collisions_50k", collisions_50k, 32 if q else 160),

@@ -20,3 +20,12 @@ BLOCK = 256                     # selfgrav tile width (kept for callers)
 KALIBRIER_N = 4096
 KALIBRIER_SCHRITTE = 12
 KALIBRIER_RUNDEN = 3
+
+# film producer (film_producer.py:36-78)
+EV_BYTES = 32                   # event record: "<dIIfIff"
+EV_FORMAT = "<dIIfIff"          # t_days f64 | a u32 | b u32 | mass f32 | kind u32 | x f32 | y f32
+BOUNCE_E = 0.6                  # restitution of asteroid bounces
+SUB_MAX_ANTEIL = 32             # one sub-sample track slot per this many bodies
+SUB_MAX_MIN = 2048
+SUB_SAMPLES_DEFAULT = 8
+M_SUB_MAX = 32

@@ -129,9 +129,39 @@ __global__ __launch_bounds__(RING_BLOCK) void k_ring_write(
   }
 }
 
+// rows[f][c][idx[i]] = snap[f][c][i] for the asteroid (a_idx) and massive
+// (m_idx) slots: the (k, 4n) [x|y|vx|vy] rows of step_batch in original
+// order.  blockIdx.y = frame * 4 + component; lanes walk the slots.
+__global__ __launch_bounds__(RING_BLOCK) void k_frame_rows(
+    int n, int na, int m, const float* __restrict__ snap,
+    const float* __restrict__ snapM, const unsigned* __restrict__ a_idx,
+    const unsigned* __restrict__ m_idx, float* __restrict__ rows) {
+  const int fc = blockIdx.y;                   // frame * 4 + component
+  float* row = rows + (long long)fc * n;
+  const int i = blockIdx.x * RING_BLOCK + threadIdx.x;
+  if (i < na) row[a_idx[i]] = snap[(long long)fc * na + i];
+  else if (i < na + m) row[m_idx[i - na]] = snapM[(long long)fc * m + (i - na)];
+}
+
 }  // namespace
 
 extern "C" {
+
+int ssim_frame_rows(int k, int n, int na, int m, const float* snap,
+                    const float* snap_m, const unsigned* a_idx,
+                    const unsigned* m_idx, float* rows, void* stream) {
+  SSIM_REQUIRE(k >= 1 && n >= 0 && na >= 0 && m >= 0 && na + m == n,
+               "bad sizes k=%d n=%d na=%d m=%d", k, n, na, m);
+  SSIM_REQUIRE(rows, "null rows");
+  SSIM_REQUIRE(na == 0 || (snap && a_idx), "null asteroid pointer");
+  SSIM_REQUIRE(m == 0 || (snap_m && m_idx), "null massive pointer");
+  if (n == 0) return SSIM_OK;
+  hipLaunchKernelGGL(k_frame_rows, dim3(ssim::ceil_div(n, RING_BLOCK), 4 * k),
+                     dim3(RING_BLOCK), 0, ssim::as_stream(stream), n, na, m, snap,
+                     snap_m, a_idx, m_idx, rows);
+  SSIM_LAUNCHED("k_frame_rows");
+  return SSIM_OK;
+}
 
 long long ssim_ring_slot_bytes(int n, int m_sub, int sub_max) {
   return 8LL * n + 4 + (long long)sub_max * (4 + 8LL * m_sub);

@@ -142,7 +142,8 @@ def merge_scan(dev, gx, gy, gvx, gvy, px, py, g_rr, g_vis, g_ast, m_alive,
 def bounce_deltas(dev, sample, hits, real_r, mass, dt_y, restitution=0.6):
     """Bounce response of a contact list on the device (the host function
     `bounce_deltas`, film_producer.py:916-1019): (hi, hj, affected,
-    deltas[k, 4]) or None.  `sample` is the f32 (4n) [x|y|vx|vy] host row."""
+    deltas[k, 4]) or None.  `sample` is the f32 (4n) [x|y|vx|vy] row, on
+    the host or as a device tensor; `real_r` and `mass` (f64, n) likewise."""
     torch = _torch()
     n = len(real_r)
     hi_h = np.asarray(hits[0], np.int64)
@@ -150,10 +151,17 @@ def bounce_deltas(dev, sample, hits, real_r, mass, dt_y, restitution=0.6):
     npairs = len(hi_h)
     if npairs == 0:
         return None
-    s = np.asarray(sample, np.float32)
-    x, y, vx, vy = (D.to_dev(s[k * n:(k + 1) * n], dev, torch.float32) for k in range(4))
-    rr = D.to_dev(np.asarray(real_r, np.float64), dev)
-    ms = D.to_dev(np.asarray(mass, np.float64), dev)
+    if torch.is_tensor(sample):
+        s = sample.reshape(-1)
+        if s.dtype != torch.float32 or s.numel() != 4 * n:
+            raise ValueError(f"sample muss float32 (4n = {4 * n}) sein")
+        x, y, vx, vy = (s[k * n:(k + 1) * n] for k in range(4))
+    else:
+        s = np.asarray(sample, np.float32)
+        x, y, vx, vy = (D.to_dev(s[k * n:(k + 1) * n], dev, torch.float32)
+                        for k in range(4))
+    rr = real_r if torch.is_tensor(real_r) else D.to_dev(np.asarray(real_r, np.float64), dev)
+    ms = mass if torch.is_tensor(mass) else D.to_dev(np.asarray(mass, np.float64), dev)
     hi, hj = D.to_dev(hi_h, dev, torch.int64), D.to_dev(hj_h, dev, torch.int64)
     d = [torch.empty(n, dtype=torch.float64, device=dev) for _ in range(4)]
     act = torch.empty(npairs, dtype=torch.int64, device=dev)

@@ -167,7 +167,12 @@ class NBodyCuda:
         track fields past the slot's count are unspecified.  st["hits"] is
         collected as after step_batch; st["sub"] is None (the tracks are in
         the slots)."""
-        torch = _torch()
+        sb = self._slots_check(st, k, sub_max, out)
+        self._run_frames(st, dt_raster_years, k)
+        self._slots_enqueue(st, k, sub_max)
+        return self._slots_collect(st, k, sb, sub_max, out)
+
+    def _slots_check(self, st, k, sub_max, out) -> int:
         if st is None:
             raise ValueError("kein Zustand geladen — FULL-Frame noetig")
         if self.dim != 2:
@@ -176,18 +181,30 @@ class NBodyCuda:
             raise ValueError(f"k ausserhalb 1..{K_MAX}: {k}")
         if sub_max < 0:
             raise ValueError(f"sub_max < 0: {sub_max}")
-        na, m, n = st["n_ast"], st["m"], st["N"]
-        lib = _lib.load()
-        sb = int(lib.ssim_ring_slot_bytes(n, self.m_sub, int(sub_max)))
+        sb = int(_lib.load().ssim_ring_slot_bytes(st["N"], self.m_sub, int(sub_max)))
         if out is not None and (out.shape != (k, sb) or out.dtype != np.uint8):
             raise ValueError(f"out muss uint8 ({k}, {sb}) sein, ist {out.dtype} {out.shape}")
-        self._run_frames(st, dt_raster_years, k)
-        dev = self._dev
+        return sb
+
+    def _idx_dev(self, st) -> None:
+        """Original indices of the integrator's slots as u32 device arrays
+        (ascending for the asteroids, which the slot order relies on)."""
+        torch = _torch()
         if "a_idx_d" not in st:
             for key, h in (("a_idx_d", st["a_idx_h"]), ("m_idx_d", st["m_idx_h"])):
                 h = np.asarray(h, np.int64)
-                st[key] = torch.as_tensor(h.astype(np.uint32).view(np.int32)).to(dev)
+                st[key] = torch.as_tensor(h.astype(np.uint32).view(np.int32)).to(self._dev)
             assert np.all(np.diff(st["a_idx_h"]) > 0)      # slot order = index order
+
+    def _slots_enqueue(self, st, k, sub_max) -> None:
+        """The ring-slot kernels of the last k frames and their copy to
+        pinned host memory, queued on the current stream."""
+        torch = _torch()
+        na, m, n = st["n_ast"], st["m"], st["N"]
+        lib = _lib.load()
+        sb = int(lib.ssim_ring_slot_bytes(n, self.m_sub, int(sub_max)))
+        dev = self._dev
+        self._idx_dev(st)
         if st.get("ring_dev") is None or st["ring_dev"].numel() < k * sb:
             st["ring_dev"] = torch.empty(k * sb, dtype=torch.uint8, device=dev)
             st["ring_host"] = torch.empty(k * sb, dtype=torch.uint8, pin_memory=True)
@@ -202,10 +219,14 @@ class NBodyCuda:
                   _lib.ptr(st["subn"]), _lib.ptr(st["a_idx_d"]), _lib.ptr(st["m_idx_d"]),
                   _lib.ptr(st["ring_dev"]), _lib.ptr(st["ring_tot"]),
                   _lib.ptr(st["ring_work"]), st["ring_work"].numel(), s)
+        st["ring_host"][:k * sb].copy_(st["ring_dev"][:k * sb], non_blocking=True)
+        st["ring_tot_h"][:k].copy_(st["ring_tot"][:k], non_blocking=True)
+
+    def _slots_collect(self, st, k, sb, sub_max, out):
+        """Wait for the queued slots and hand them out (see step_batch_slots)."""
+        torch = _torch()
+        torch.cuda.current_stream(self._dev).synchronize()
         host, tot = st["ring_host"][:k * sb], st["ring_tot_h"][:k]
-        host.copy_(st["ring_dev"][:k * sb], non_blocking=True)
-        tot.copy_(st["ring_tot"][:k], non_blocking=True)
-        torch.cuda.current_stream(dev).synchronize()
         slots = host.numpy().reshape(k, sb)
         if out is not None:
             out[...] = slots
@@ -216,6 +237,21 @@ class NBodyCuda:
         # no index field at all (sub_max 0): nothing overflows, as in
         # schreibe_slot's `not sub_max` branch (film_producer.py:107-109)
         return out, (tot.numpy() > sub_max) & (sub_max > 0)
+
+    def frame_rows(self, st: dict, k: int, rows) -> None:
+        """After a batch: its k samples as device rows f32 (k, 4n)
+        [x|y|vx|vy] in original order (libssim's ssim_frame_rows), the
+        array step_batch returns, without leaving the device.  `rows` is a
+        float32 device tensor of at least k * 4n elements."""
+        if self.dim != 2:
+            raise ValueError("frame rows carry the plane (x, y) only")
+        n, na, m = st["N"], st["n_ast"], st["m"]
+        if rows.numel() < k * 4 * n or rows.dtype != _torch().float32:
+            raise ValueError(f"rows muss float32 mit >= {k * 4 * n} Elementen sein")
+        self._idx_dev(st)
+        _lib.call("ssim_frame_rows", int(k), n, na, m, _lib.ptr(st["snap"]),
+                  _lib.ptr(st["snapM"]), _lib.ptr(st["a_idx_d"]), _lib.ptr(st["m_idx_d"]),
+                  _lib.ptr(rows), D.stream(self._dev))
 
     def _run_frames(self, st: dict, dt_raster_years: float, k: int) -> None:
         na, m = st["n_ast"], st["m"]

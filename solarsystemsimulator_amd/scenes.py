@@ -175,3 +175,48 @@ def asteroid_cloud(n: int = 50_000, seed: int = 4):
     radius = np.concatenate([[0.00465], [1.6e-5, 4.0e-5, 4.3e-5, 2.3e-5], rad])
     is_ast = np.concatenate([np.zeros(5, np.uint8), np.ones(n, np.uint8)])
     return x, y, vx, vy, mass, radius, is_ast
+
+
+def _star_first(ax, ay, avx, avy) -> dict:
+    """Sun (mass 1, radius 0.00465 AU) followed by the asteroids (mass
+    1e-14, radius 2e-5 AU), the film benchmark's layout (bench_film.py:
+    30-42)."""
+    n_ast = len(ax)
+    one = np.ones(n_ast + 1, np.uint8)
+    return {"x": np.concatenate(([0.0], ax)), "y": np.concatenate(([0.0], ay)),
+            "vx": np.concatenate(([0.0], avx)), "vy": np.concatenate(([0.0], avy)),
+            "mass": np.concatenate(([1.0], np.full(n_ast, 1e-14))),
+            "real_r": np.concatenate(([0.00465], np.full(n_ast, 2e-5))),
+            "visible": one,
+            "is_ast": np.concatenate(([0], np.ones(n_ast, np.uint8))),
+            "is_star_bh": np.concatenate(([1], np.zeros(n_ast, np.uint8)))}
+
+
+def film_belt(n_ast: int, seed: int = 20260720) -> dict:
+    """The film benchmark's relaxed belt (bench_film.py:45-54): radii
+    1.5 + 2.5 sqrt(U) AU, uniform angles, circular speed with 3 % scatter."""
+    rng = np.random.default_rng(seed)
+    r = 1.5 + 2.5 * np.sqrt(rng.random(n_ast))
+    th = rng.random(n_ast) * 2 * np.pi
+    v = np.sqrt(G_AU / r) * (1.0 + rng.normal(0, 0.03, n_ast))
+    return _star_first(r * np.cos(th), r * np.sin(th), -v * np.sin(th), v * np.cos(th))
+
+
+def film_clumps(n_ast: int, clumps: int = 5, seed: int = 20260720) -> dict:
+    """The film benchmark's interpenetrating clouds (bench_film.py:57-83):
+    `clumps` Gaussian clouds (sigma 0.12 AU, velocity scatter 0.4 AU/yr)
+    with centres at r = 2 + 0.05 i AU, angle 0.35 i, moving with the
+    circular speed; the dense case of the collision search."""
+    rng = np.random.default_rng(seed)
+    each = n_ast // clumps
+    parts = [[], [], [], []]
+    for i in range(clumps):
+        m = each if i < clumps - 1 else n_ast - each * (clumps - 1)
+        r0, th0 = 2.0 + 0.05 * i, 0.35 * i
+        cx, cy = r0 * np.cos(th0), r0 * np.sin(th0)
+        v0 = np.sqrt(G_AU / r0)
+        parts[0].append(rng.normal(cx, 0.12, m))
+        parts[1].append(rng.normal(cy, 0.12, m))
+        parts[2].append(-v0 * np.sin(th0) + rng.normal(0, 0.4, m))
+        parts[3].append(v0 * np.cos(th0) + rng.normal(0, 0.4, m))
+    return _star_first(*(np.concatenate(p) for p in parts))
