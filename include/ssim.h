@@ -629,6 +629,17 @@ int ssim_bounce_deltas(int n, int npairs, const long long* hi, const long long* 
                        long long* affected, long long* counts, void* work,
                        size_t work_bytes, void* stream);
 
+/* Bounce deltas onto the resident state: body aff[q] (original index)
+ * gets x, y, vx, vy += ddx/ddy/ddvx/ddvy[aff[q]] (the f64 length-n outputs
+ * of ssim_bounce_deltas) in its integrator slot; the device form of the
+ * host's apply_deltas after bounce_deltas (film_producer.py:1021-1026,
+ * nbody_kernel.py:952-986).  aff: distinct indices; inv_kind /
+ * inv_pos / massive / asteroids as for ssim_apply_merges. */
+int ssim_add_body_deltas(int naff, const long long* aff, const double* ddx,
+                         const double* ddy, const double* ddvx, const double* ddvy,
+                         const int* inv_kind, const int* inv_pos, void* const* massive,
+                         void* const* asteroids, void* stream);
+
 /* The results of the massive-body contact scan applied to the state, on the
  * device (the host closure `apply_merges`, film_producer.py:781-861), in
  * list order — a later pair sees the masses and flags the earlier ones

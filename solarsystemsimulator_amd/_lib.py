@@ -154,6 +154,7 @@ SIGNATURES = {
     "ssim_bounce_work_bytes": (c_size_t, [c_int, c_int]),
     "ssim_bounce_deltas": (c_int, [c_int, c_int] + [_P] * 8 + [c_double, c_double]
                            + [_P] * 7 + [_P, c_size_t, _P]),
+    "ssim_add_body_deltas": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "ssim_apply_merges": (c_int, [c_int, _P, c_int, _P, _P, _P, c_int, _P, _P, _P, _P,
                                   _P, _P, _P, c_int, c_int, c_double, c_double,
                                   c_double, _P, _P, _P, _P, _P, _P, _P, _P]),

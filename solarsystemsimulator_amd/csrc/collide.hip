@@ -221,15 +221,17 @@ __global__ __launch_bounds__(CB) void k_segments(int na, const long long* __rest
   slen[t] = upper_bound(skey, na, k2) - lo;
 }
 
-// A wave takes WCH = 64 CPT consecutive candidates of the flattened list,
-// in CPT rounds of 64 consecutive ones (lane l, round q: candidate
-// gw + 64 q + l, so the sorted candidates' loads are coalesced); each lane
-// follows the segment (offset, row) of its candidate forward through the
-// segment offsets soff (5 na + 1 of them).  pass 0 stores each round's hit
-// ballot and the wave's hit count; pass 1 writes the hits in list order:
-// place = hits of earlier waves (wpos) + earlier rounds + earlier lanes.
+// A wave takes 64 cpt consecutive candidates of the flattened list, in cpt
+// rounds of 64 consecutive ones (lane l, round q: candidate gw + 64 q + l,
+// so the sorted candidates' loads are coalesced); each lane follows the
+// segment (offset, row) of its candidate forward through the segment
+// offsets soff (5 na + 1 of them).  pass 0 stores each round's hit ballot
+// and the wave's hit count; pass 1 writes the hits in list order: place =
+// hits of earlier waves (wpos) + earlier rounds + earlier lanes.  cpt (1 ..
+// CPT) is chosen per batch so that a launch has at least WAVES_MIN waves:
+// few candidates take few rounds each instead of filling few waves.
 constexpr int CPT = 32;
-constexpr int WCH = 64 * CPT;
+constexpr long long WAVES_MIN = 8192;
 constexpr long long CBATCH = 1LL << 25;  // candidates per batch (workspace size)
 long long g_batch = CBATCH;              // the batch in use (ssim_collide_batch)
 
@@ -238,12 +240,12 @@ __global__ __launch_bounds__(CB) void k_cand_waves(
     const int* __restrict__ sstart, const int* __restrict__ ai,
     const float4* __restrict__ ra4, const float* __restrict__ rar,
     const float4* __restrict__ sb, const float* __restrict__ srr,
-    const int* __restrict__ sidx, float dt, float tol, int pass,
+    const int* __restrict__ sidx, float dt, float tol, int pass, int cpt,
     unsigned long long* __restrict__ bits, int* __restrict__ wcnt,
     const int* __restrict__ wpos, int hoff, int* __restrict__ outa, int* __restrict__ outb) {
   const int lane = threadIdx.x & 63;
   const long long wave = ((long long)blockIdx.x * CB + threadIdx.x) >> 6;
-  const long long gw = base + wave * WCH;
+  const long long gw = base + wave * 64 * cpt;
   const long long gend = min(total, base + batch);
   if (gw >= gend) return;  // whole waves
   const int nseg = 5 * na;
@@ -261,10 +263,10 @@ __global__ __launch_bounds__(CB) void k_cand_waves(
   float4 A = ra4[p];
   float ra = rar[p];
   const unsigned long long lt = (1ull << lane) - 1ull;
-  unsigned long long* wb = bits + wave * CPT;
+  unsigned long long* wb = bits + wave * cpt;
   int cnt = 0;
   int w = pass ? hoff + wpos[wave] : 0;
-  for (int q = 0; q < CPT; ++q) {
+  for (int q = 0; q < cpt; ++q) {
     const long long g = gw + (long long)q * 64 + lane;
     const bool in = g < gend;
     bool hit = false;
@@ -390,7 +392,7 @@ struct Layout {
 Layout layout(int n, long long hcap) {
   Layout L{};
   const size_t nn = (size_t)(n > 0 ? n : 1);
-  const size_t nw = (size_t)(CBATCH / WCH);
+  const size_t nw = (size_t)(CBATCH / 64);     // waves of a batch at cpt = 1
   const size_t hc = (size_t)(hcap > 0 ? hcap : 1);
   size_t o = 0;
   auto take = [&](size_t bytes) {
@@ -415,7 +417,7 @@ Layout layout(int n, long long hcap) {
   L.sstart = take(5 * nn * 4);
   L.slen = take(5 * nn * 4);
   L.soff = take((5 * nn + 1) * 4);
-  L.bits = take(nw * CPT * 8);
+  L.bits = take(nw * 8);
   L.wcnt = take(nw * 4);
   L.wpos = take((nw + 1) * 4);
   L.sums = take(64);
@@ -812,6 +814,29 @@ __global__ void k_apply_merges(int n, const float* __restrict__ smp, int npairs,
   result[1] = merged;
 }
 
+// Bounce deltas onto the state: body aff[q] (original index) gets
+// x, y, vx, vy += d[0..3][aff[q]] in its integrator slot (the host's
+// apply_deltas after bounce_deltas, film_producer.py:1021-1026); the
+// affected bodies are distinct, so every slot sees one addition.
+__global__ void k_add_body_deltas(int naff, const long long* __restrict__ aff,
+                                  const double* __restrict__ d0,
+                                  const double* __restrict__ d1,
+                                  const double* __restrict__ d2,
+                                  const double* __restrict__ d3,
+                                  const int* __restrict__ inv_kind,
+                                  const int* __restrict__ inv_pos, BodyArrays Ms,
+                                  BodyArrays As) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= naff) return;
+  const long long b = aff[q];
+  BodyArrays& B = inv_kind[b] == -1 ? Ms : As;
+  const int p = inv_pos[b];
+  B.f[0][p] += d0[b];
+  B.f[1][p] += d1[b];
+  B.f[2][p] += d2[b];
+  B.f[3][p] += d3[b];
+}
+
 }  // namespace
 
 extern "C" {
@@ -1107,11 +1132,13 @@ int ssim_collide_hits(int n, const float* gx, const float* gy, const float* gvx,
   const long long batch = g_batch;
   for (long long base = 0; base < (long long)ncand; base += batch) {
     const long long nb = std::min<long long>(batch, (long long)ncand - base);
-    const int nwave = (int)((nb + WCH - 1) / WCH);
+    long long cpt = (nb + 64 * WAVES_MIN - 1) / (64 * WAVES_MIN);
+    cpt = cpt < 1 ? 1 : (cpt > CPT ? CPT : cpt);
+    const int nwave = (int)((nb + 64 * cpt - 1) / (64 * cpt));
     const dim3 gb(ssim::ceil_div((long long)nwave * 64, CB));
     hipLaunchKernelGGL(k_cand_waves, gb, dim3(CB), 0, st, na, base, batch, (long long)ncand,
-                       soff, sstart, ai, ra4, rar, sb, srr, sidx, P.dt, P.tol, 0, bits, wcnt,
-                       nullptr, 0, nullptr, nullptr);
+                       soff, sstart, ai, ra4, rar, sb, srr, sidx, P.dt, P.tol, 0, (int)cpt,
+                       bits, wcnt, nullptr, 0, nullptr, nullptr);
     SSIM_LAUNCHED("k_cand_waves");
     SSIM_TRY(ssim::scan_exclusive(nwave, wcnt, wpos, swork, st));
     int nbh = 0;
@@ -1119,7 +1146,7 @@ int ssim_collide_hits(int n, const float* gx, const float* gy, const float* gvx,
     if (room && nh + nbh <= f32_hit_capacity) {
       hipLaunchKernelGGL(k_cand_waves, gb, dim3(CB), 0, st, na, base, batch,
                          (long long)ncand, soff, sstart, ai, ra4, rar, sb, srr, sidx, P.dt,
-                         P.tol, 1, bits, wcnt, wpos, (int)nh, ha, hb);
+                         P.tol, 1, (int)cpt, bits, wcnt, wpos, (int)nh, ha, hb);
       SSIM_LAUNCHED("k_cand_waves");
     } else {
       room = false;
@@ -1143,6 +1170,27 @@ int ssim_collide_hits(int n, const float* gx, const float* gy, const float* gvx,
   int np = 0;
   SSIM_TRY(read_int(hpos + nhi, &np));
   counts[2] = np;
+  return SSIM_OK;
+}
+
+int ssim_add_body_deltas(int naff, const long long* aff, const double* ddx,
+                         const double* ddy, const double* ddvx, const double* ddvy,
+                         const int* inv_kind, const int* inv_pos, void* const* massive,
+                         void* const* asteroids, void* stream) {
+  SSIM_REQUIRE(naff >= 0, "naff must be >= 0");
+  if (naff == 0) return SSIM_OK;
+  SSIM_REQUIRE(aff && ddx && ddy && ddvx && ddvy && inv_kind && inv_pos && massive &&
+                   asteroids,
+               "null pointer");
+  BodyArrays M{}, A{};
+  for (int f = 0; f < 5; ++f) {
+    M.f[f] = static_cast<double*>(massive[f]);
+    A.f[f] = static_cast<double*>(asteroids[f]);
+  }
+  hipLaunchKernelGGL(k_add_body_deltas, dim3(ssim::ceil_div(naff, 256)), dim3(256), 0,
+                     ssim::as_stream(stream), naff, aff, ddx, ddy, ddvx, ddvy, inv_kind,
+                     inv_pos, M, A);
+  SSIM_LAUNCHED("k_add_body_deltas");
   return SSIM_OK;
 }
 

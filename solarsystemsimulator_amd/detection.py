@@ -144,6 +144,25 @@ def bounce_deltas(dev, sample, hits, real_r, mass, dt_y, restitution=0.6):
     `bounce_deltas`, film_producer.py:916-1019): (hi, hj, affected,
     deltas[k, 4]) or None.  `sample` is the f32 (4n) [x|y|vx|vy] row, on
     the host or as a device tensor; `real_r` and `mass` (f64, n) likewise."""
+    r = bounce_deltas_device(dev, sample, hits, real_r, mass, dt_y, restitution)
+    if r is None:
+        return None
+    hi, hj, aff, d = r
+    deltas = torch_stack_cols(d, aff).cpu().numpy()
+    return hi, hj, aff.cpu().numpy().astype(np.int64), deltas
+
+
+def torch_stack_cols(cols, idx):
+    """(len(idx), len(cols)) from device columns gathered at idx."""
+    torch = _torch()
+    return torch.stack([c.index_select(0, idx) for c in cols], dim=1)
+
+
+def bounce_deltas_device(dev, sample, hits, real_r, mass, dt_y, restitution=0.6):
+    """bounce_deltas with the results left on the device: (hi, hj) host
+    int64 of the pairs that act, the affected bodies as a device int64
+    tensor and the four f64 delta columns (length n, by original index) —
+    for add_body_deltas.  None when no pair acts."""
     torch = _torch()
     n = len(real_r)
     hi_h = np.asarray(hits[0], np.int64)
@@ -178,9 +197,36 @@ def bounce_deltas(dev, sample, hits, real_r, mass, dt_y, restitution=0.6):
     if na == 0:
         return None
     a = act[:na].cpu().numpy()
-    f = aff[:nf].cpu().numpy()
-    deltas = np.stack([c.cpu().numpy()[f] for c in d], axis=1)
-    return hi_h[a], hj_h[a], f.astype(np.int64), deltas
+    return hi_h[a], hj_h[a], aff[:nf], d
+
+
+def add_body_deltas(st, aff, d, dev) -> None:
+    """Add bounce deltas (device columns d, length n by original index) to
+    the NBodyCuda state `st` for the affected bodies `aff` (device int64,
+    distinct) — apply_deltas without the host round trip."""
+    if aff.numel() == 0:
+        return
+    _body_index(st, dev)
+    ptrs = _body_ptrs(st)
+    _lib.call("ssim_add_body_deltas", int(aff.numel()), _lib.ptr(aff),
+              *[_lib.ptr(c) for c in d], _lib.ptr(st["inv_kind_d"]),
+              _lib.ptr(st["inv_pos_d"]), _lib.ctypes.addressof(ptrs[0]),
+              _lib.ctypes.addressof(ptrs[1]), D.stream(dev))
+
+
+def _body_index(st, dev) -> None:
+    torch = _torch()
+    if "inv_kind_d" not in st:
+        st["inv_kind_d"] = torch.from_numpy(st["inv_kind"].astype(np.int32)).to(dev)
+        st["inv_pos_d"] = torch.from_numpy(st["inv_pos"].astype(np.int32)).to(dev)
+
+
+def _body_ptrs(st):
+    """The 7 device pointers (x, y, vx, vy, m, visible, radius) of the
+    massive and of the asteroid arrays of a 2-D NBodyCuda state."""
+    return [(_lib.ctypes.c_void_p * 7)(*[_lib.ptr(t) for t in st[k][:5]],
+                                      _lib.ptr(st[v]), _lib.ptr(st[r]))
+            for k, v, r in (("M", "mvis", "mrad"), ("a", "avis", "arad"))]
 
 
 def apply_merges(st, sample, pairs, runaway, k_ev, mirrors, contact_t,
@@ -206,12 +252,8 @@ def apply_merges(st, sample, pairs, runaway, k_ev, mirrors, contact_t,
     pf = torch.tensor([float(p[2]) for p in pairs] or [0.0], dtype=torch.float64,
                       device=dev)
     run = torch.tensor([int(j) for j in runaway] or [0], dtype=torch.int32, device=dev)
-    if "inv_kind_d" not in st:
-        st["inv_kind_d"] = torch.from_numpy(st["inv_kind"].astype(np.int32)).to(dev)
-        st["inv_pos_d"] = torch.from_numpy(st["inv_pos"].astype(np.int32)).to(dev)
-    ptrs = [(_lib.ctypes.c_void_p * 7)(*[_lib.ptr(t) for t in st[k][:5]],
-                                      _lib.ptr(st[v]), _lib.ptr(st[r]))
-            for k, v, r in (("M", "mvis", "mrad"), ("a", "avis", "arad"))]
+    _body_index(st, dev)
+    ptrs = _body_ptrs(st)
     events = torch.empty(32 * max(npairs + nrun, 1), dtype=torch.uint8, device=dev)
     result = torch.zeros(4, dtype=torch.int32, device=dev)
     shatter_t = torch.zeros(1, dtype=torch.float64, device=dev)

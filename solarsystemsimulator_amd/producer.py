@@ -39,8 +39,8 @@ from . import _lib
 from . import device as D
 from .constants import (BOUNCE_E, EV_BYTES, EV_FORMAT, G_AU, K_MAX,
                         SUB_MAX_ANTEIL, SUB_MAX_MIN)
-from .detection import (Erkennungskarte, apply_merges, bounce_deltas,
-                        bounce_suche, merge_scan)
+from .detection import (Erkennungskarte, add_body_deltas, apply_merges,
+                        bounce_deltas_device, bounce_suche, merge_scan)
 
 
 def _torch():
@@ -255,9 +255,9 @@ class FilmProducer:
                 self.anteile)
             self.candidates += cand
             if hits is not None:
-                bounce = bounce_deltas(self.dev, row.reshape(-1), hits,
-                                       self.mirrors["real_r"], self.mirrors["mass"],
-                                       self.dt_years, BOUNCE_E)
+                bounce = bounce_deltas_device(self.dev, row.reshape(-1), hits,
+                                              self.mirrors["real_r"], self.mirrors["mass"],
+                                              self.dt_years, BOUNCE_E)
         return {"k": k_ev, "row": row, "pairs": pairs, "runaways": runaway,
                 "bounce": bounce}
 
@@ -289,17 +289,28 @@ class FilmProducer:
             self.events += ev
             self.collisions += merges
             if ev:
-                self.mass[:] = self.mirrors["mass"].cpu().numpy()
-                self.vis[:] = self.mirrors["vis"].cpu().numpy()
-                self.real_r[:] = self.mirrors["real_r"].cpu().numpy()
+                # only the bodies named by the records changed: a merge
+                # changes a (mass, radius) and b (mass, visibility), a
+                # removal its body b
+                rec = np.frombuffer(ev, EV_RECORD)
+                merged = rec["kind"] == 0
+                touched = np.unique(np.concatenate([rec["a"][merged], rec["b"]]))
+                sel = torch.as_tensor(touched.astype(np.int64)).to(self.dev)
+                self.mass[touched] = self.mirrors["mass"].index_select(0, sel).cpu().numpy()
+                self.vis[touched] = self.mirrors["vis"].index_select(0, sel).cpu().numpy()
+                self.real_r[touched] = \
+                    self.mirrors["real_r"].index_select(0, sel).cpu().numpy()
                 self._det_dirty = True
             if shatter is not None:
                 self.shatter = shatter
                 return
         if res["bounce"] is None:
             return
-        hi, hj, aff, deltas = res["bounce"]
-        self.sim.apply_deltas(self.st, aff, deltas)
+        hi, hj, aff, cols = res["bounce"]
+        main = torch.cuda.current_stream(self.dev)
+        for t in (aff, *cols):               # made on the side stream, used here
+            t.record_stream(main)
+        add_body_deltas(self.st, aff, cols, self.dev)
         heavy = np.where(self.mass[hi] >= self.mass[hj], hi, hj)
         light = np.where(heavy == hi, hj, hi)
         sel = torch.as_tensor(heavy.astype(np.int64)).to(self.dev)
