@@ -689,6 +689,8 @@ __global__ __launch_bounds__(CB) void k_u8_to_i32(int n, const unsigned char* __
 // ---- apply_merges on the device ----
 // Body arrays of one kind of the nbody state (massive or asteroid): x, y,
 // vx, vy, m (FP64), visible flags, radii.
+constexpr int AM_THREADS = 256;  // workgroup of k_apply_merges
+
 struct BodyArrays {
   double* f[5];
   unsigned char* vis;
@@ -709,8 +711,11 @@ __device__ void put_event(unsigned char* ev, int k, double t, unsigned int a,
   *reinterpret_cast<float*>(r + 28) = (float)y;
 }
 
-// The loop of apply_merges (film_producer.py:781-861), one thread, pairs in
-// list order: a later pair sees the masses and flags the earlier ones left.
+// The loop of apply_merges (film_producer.py:781-861), pairs in list order
+// on one thread: a later pair sees the masses and flags the earlier ones
+// left.  The runaway removals after it are independent of one another (each
+// touches only its own body), so the workgroup takes them 256 at a time,
+// their records placed in list order by a scan of the still visible ones.
 // Per pair (i, j, frac): a contact time the nbody kernel saw for j replaces
 // frac when it lies within one raster (and is consumed either way); with
 // shatter_check, two visible massive non-star bodies meeting at
@@ -732,7 +737,8 @@ __global__ void k_apply_merges(int n, const float* __restrict__ smp, int npairs,
                                const int* __restrict__ inv_pos, BodyArrays Ms,
                                BodyArrays As, unsigned char* events, int* result,
                                double* shatter_t) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  __shared__ int s_state[3];       // records, merges, stopped
+  __shared__ int s_wave[AM_THREADS / 64];
   const float* sx = smp;
   const float* sy = smp + n;
   const float* svx = smp + 2 * (size_t)n;
@@ -744,97 +750,99 @@ __global__ void k_apply_merges(int n, const float* __restrict__ smp, int npairs,
     B.f[4][p] = 0.0;
     B.vis[p] = 0;
   };
-  int nev = 0, merged = 0;
-  result[2] = -1;
-  result[3] = -1;
-  for (int q = 0; q < npairs; ++q) {
-    const int mi = pi[q], j = pj[q];
-    double frac = pfrac[q];
-    const double tk = contact_t[j];
-    if (tk == tk) {  // present (not NaN): consumed
-      contact_t[j] = __builtin_nan("");
-      const double fk = tk - (double)(k_ev + 1);
-      if (-1.0 <= fk && fk <= 1.0) frac = fk;
-    }
-    if (shatter_check && !is_ast[mi] && !is_ast[j] && !is_star_bh[mi] && !is_star_bh[j] &&
-        vis[mi] && vis[j] && mass[mi] > 0.0 && mass[j] > 0.0) {
-      const float dvx = svx[j] - svx[mi], dvy = svy[j] - svy[mi];
-      const double v_imp = (double)hypotf(dvx, dvy);
-      const double touch = fmax(1e-12, real_r[mi] + real_r[j]);
-      const double v_esc = sqrt(2.0 * G * (mass[mi] + mass[j]) / touch);
-      if (v_imp >= 1.5 * v_esc) {
-        result[2] = mi;
-        result[3] = j;
-        *shatter_t = t0_days + ((double)(k_ev + 1) + frac) * raster_days;
-        result[0] = nev;
-        result[1] = merged;
-        return;
+  if (threadIdx.x == 0) {
+    int nev = 0, merged = 0;
+    s_state[2] = 0;
+    result[2] = -1;
+    result[3] = -1;
+    for (int q = 0; q < npairs; ++q) {
+      const int mi = pi[q], j = pj[q];
+      double frac = pfrac[q];
+      const double tk = contact_t[j];
+      if (tk == tk) {  // present (not NaN): consumed
+        contact_t[j] = __builtin_nan("");
+        const double fk = tk - (double)(k_ev + 1);
+        if (-1.0 <= fk && fk <= 1.0) frac = fk;
       }
+      if (shatter_check && !is_ast[mi] && !is_ast[j] && !is_star_bh[mi] && !is_star_bh[j] &&
+          vis[mi] && vis[j] && mass[mi] > 0.0 && mass[j] > 0.0) {
+        const float dvx = svx[j] - svx[mi], dvy = svy[j] - svy[mi];
+        const double v_imp = (double)hypotf(dvx, dvy);
+        const double touch = fmax(1e-12, real_r[mi] + real_r[j]);
+        const double v_esc = sqrt(2.0 * G * (mass[mi] + mass[j]) / touch);
+        if (v_imp >= 1.5 * v_esc) {
+          result[2] = mi;
+          result[3] = j;
+          *shatter_t = t0_days + ((double)(k_ev + 1) + frac) * raster_days;
+          result[0] = nev;
+          result[1] = merged;
+          s_state[2] = 1;
+          break;
+        }
+      }
+      if (!vis[mi] || !vis[j] || mass[mi] <= 0.0) continue;
+      const int a = mass[mi] >= mass[j] ? mi : j;
+      const int b = a == mi ? j : mi;
+      const double ma = mass[a], mb = mass[b];
+      const double mg = ma + mb;
+      if (mg <= 0.0) continue;
+      const double nx = ((double)sx[a] * ma + (double)sx[b] * mb) / mg;
+      const double ny = ((double)sy[a] * ma + (double)sy[b] * mb) / mg;
+      const double nvx = ((double)svx[a] * ma + (double)svx[b] * mb) / mg;
+      const double nvy = ((double)svy[a] * ma + (double)svy[b] * mb) / mg;
+      {
+        BodyArrays& A = arrays(a);
+        const int p = inv_pos[a];
+        A.f[0][p] = nx;
+        A.f[1][p] = ny;
+        A.f[2][p] = nvx;
+        A.f[3][p] = nvy;
+        A.f[4][p] = mg;
+      }
+      deactivate(b);
+      mass[a] = mg;
+      mass[b] = 0.0;
+      vis[b] = 0;
+      const double ra = real_r[a], rb = real_r[b];
+      real_r[a] = pow(ra * ra * ra + rb * rb * rb, 1.0 / 3.0);
+      arrays(a).rad[inv_pos[a]] = real_r[a];
+      put_event(events, nev++, t0_days + ((double)(k_ev + 1) + frac) * raster_days,
+                (unsigned int)a, (unsigned int)b, mg, 0u, nx, ny);
+      ++merged;
     }
-    if (!vis[mi] || !vis[j] || mass[mi] <= 0.0) continue;
-    const int a = mass[mi] >= mass[j] ? mi : j;
-    const int b = a == mi ? j : mi;
-    const double ma = mass[a], mb = mass[b];
-    const double mg = ma + mb;
-    if (mg <= 0.0) continue;
-    const double nx = ((double)sx[a] * ma + (double)sx[b] * mb) / mg;
-    const double ny = ((double)sy[a] * ma + (double)sy[b] * mb) / mg;
-    const double nvx = ((double)svx[a] * ma + (double)svx[b] * mb) / mg;
-    const double nvy = ((double)svy[a] * ma + (double)svy[b] * mb) / mg;
-    {
-      BodyArrays& A = arrays(a);
-      const int p = inv_pos[a];
-      A.f[0][p] = nx;
-      A.f[1][p] = ny;
-      A.f[2][p] = nvx;
-      A.f[3][p] = nvy;
-      A.f[4][p] = mg;
+    s_state[0] = nev;
+    s_state[1] = merged;
+  }
+  __syncthreads();
+  if (s_state[2]) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int nev = s_state[0];
+  for (int base = 0; base < nrun; base += AM_THREADS) {
+    const int q = base + (int)threadIdx.x;
+    const int j = q < nrun ? run[q] : 0;
+    const bool act = q < nrun && vis[j] != 0;
+    const unsigned long long bal = __ballot(act);
+    if (lane == 0) s_wave[wave] = __popcll(bal);
+    __syncthreads();
+    int before = __popcll(bal & ((1ull << lane) - 1ull)), total = 0;
+    for (int w = 0; w < AM_THREADS / 64; ++w) {
+      if (w < wave) before += s_wave[w];
+      total += s_wave[w];
     }
-    deactivate(b);
-    mass[a] = mg;
-    mass[b] = 0.0;
-    vis[b] = 0;
-    const double ra = real_r[a], rb = real_r[b];
-    real_r[a] = pow(ra * ra * ra + rb * rb * rb, 1.0 / 3.0);
-    arrays(a).rad[inv_pos[a]] = real_r[a];
-    put_event(events, nev++, t0_days + ((double)(k_ev + 1) + frac) * raster_days,
-              (unsigned int)a, (unsigned int)b, mg, 0u, nx, ny);
-    ++merged;
+    if (act) {
+      deactivate(j);
+      mass[j] = 0.0;
+      vis[j] = 0;
+      put_event(events, nev + before, t0_days + (double)(k_ev + 1) * raster_days,
+                0xFFFFFFFFu, (unsigned int)j, 0.0, 2u, (double)sx[j], (double)sy[j]);
+    }
+    nev += total;
+    __syncthreads();
   }
-  for (int q = 0; q < nrun; ++q) {
-    const int j = run[q];
-    if (!vis[j]) continue;
-    deactivate(j);
-    mass[j] = 0.0;
-    vis[j] = 0;
-    put_event(events, nev++, t0_days + (double)(k_ev + 1) * raster_days, 0xFFFFFFFFu,
-              (unsigned int)j, 0.0, 2u, (double)sx[j], (double)sy[j]);
+  if (threadIdx.x == 0) {
+    result[0] = nev;
+    result[1] = s_state[1];
   }
-  result[0] = nev;
-  result[1] = merged;
-}
-
-// Bounce deltas onto the state: body aff[q] (original index) gets
-// x, y, vx, vy += d[0..3][aff[q]] in its integrator slot (the host's
-// apply_deltas after bounce_deltas, film_producer.py:1021-1026); the
-// affected bodies are distinct, so every slot sees one addition.
-__global__ void k_add_body_deltas(int naff, const long long* __restrict__ aff,
-                                  const double* __restrict__ d0,
-                                  const double* __restrict__ d1,
-                                  const double* __restrict__ d2,
-                                  const double* __restrict__ d3,
-                                  const int* __restrict__ inv_kind,
-                                  const int* __restrict__ inv_pos, BodyArrays Ms,
-                                  BodyArrays As) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= naff) return;
-  const long long b = aff[q];
-  BodyArrays& B = inv_kind[b] == -1 ? Ms : As;
-  const int p = inv_pos[b];
-  B.f[0][p] += d0[b];
-  B.f[1][p] += d1[b];
-  B.f[2][p] += d2[b];
-  B.f[3][p] += d3[b];
 }
 
 }  // namespace
@@ -1219,7 +1227,7 @@ int ssim_apply_merges(int n, const float* sample, int npairs, const int* pair_i,
   A.vis = static_cast<unsigned char*>(asteroids[5]);
   A.rad = static_cast<double*>(asteroids[6]);
   hipStream_t st = ssim::as_stream(stream);
-  hipLaunchKernelGGL(k_apply_merges, dim3(1), dim3(1), 0, st, n, sample, npairs, pair_i,
+  hipLaunchKernelGGL(k_apply_merges, dim3(1), dim3(AM_THREADS), 0, st, n, sample, npairs, pair_i,
                      pair_j, pair_frac, nrun, runaway, mass, vis, real_r, is_ast,
                      is_star_bh, contact_t, k_ev, shatter_check, G, t0_days, raster_days,
                      inv_kind, inv_pos, M, A, events, result, shatter_t);
