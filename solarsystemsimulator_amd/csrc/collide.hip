@@ -715,7 +715,8 @@ __device__ void put_event(unsigned char* ev, int k, double t, unsigned int a,
 // on one thread: a later pair sees the masses and flags the earlier ones
 // left.  The runaway removals after it are independent of one another (each
 // touches only its own body), so the workgroup takes them 256 at a time,
-// their records placed in list order by a scan of the still visible ones.
+// their records placed in list order by a scan of the still visible ones
+// (a body listed twice only at its first entry, as the loop would).
 // Per pair (i, j, frac): a contact time the nbody kernel saw for j replaces
 // frac when it lies within one raster (and is consumed either way); with
 // shatter_check, two visible massive non-star bodies meeting at
@@ -817,10 +818,16 @@ __global__ void k_apply_merges(int n, const float* __restrict__ smp, int npairs,
   if (s_state[2]) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int nev = s_state[0];
+  __shared__ int s_j[AM_THREADS];
   for (int base = 0; base < nrun; base += AM_THREADS) {
     const int q = base + (int)threadIdx.x;
-    const int j = q < nrun ? run[q] : 0;
-    const bool act = q < nrun && vis[j] != 0;
+    const int j = q < nrun ? run[q] : -1;
+    s_j[threadIdx.x] = j;
+    __syncthreads();
+    // a body listed twice is removed once, at its first entry (earlier
+    // chunks already cleared its flag; earlier entries of this chunk here)
+    bool act = q < nrun && vis[j] != 0;
+    for (int e = 0; act && e < (int)threadIdx.x; ++e) act = s_j[e] != j;
     const unsigned long long bal = __ballot(act);
     if (lane == 0) s_wave[wave] = __popcll(bal);
     __syncthreads();
@@ -843,6 +850,29 @@ __global__ void k_apply_merges(int n, const float* __restrict__ smp, int npairs,
     result[0] = nev;
     result[1] = s_state[1];
   }
+}
+
+// Bounce deltas onto the state: body aff[q] (original index) gets
+// x, y, vx, vy += d[0..3][aff[q]] in its integrator slot (the host's
+// apply_deltas after bounce_deltas, film_producer.py:1021-1026); the
+// affected bodies are distinct, so every slot sees one addition.
+__global__ void k_add_body_deltas(int naff, const long long* __restrict__ aff,
+                                  const double* __restrict__ d0,
+                                  const double* __restrict__ d1,
+                                  const double* __restrict__ d2,
+                                  const double* __restrict__ d3,
+                                  const int* __restrict__ inv_kind,
+                                  const int* __restrict__ inv_pos, BodyArrays Ms,
+                                  BodyArrays As) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= naff) return;
+  const long long b = aff[q];
+  BodyArrays& B = inv_kind[b] == -1 ? Ms : As;
+  const int p = inv_pos[b];
+  B.f[0][p] += d0[b];
+  B.f[1][p] += d1[b];
+  B.f[2][p] += d2[b];
+  B.f[3][p] += d3[b];
 }
 
 }  // namespace

@@ -485,7 +485,7 @@ int ssim_pp_near(int nt, const double* tx, const double* ty, const double* tz,
   SSIM_TRY(pp_plan(nt, ncells, tcell_start, rs, rcut, work, &cstart, &cmap, &gtab, st));
   const int max_chunks = nt / CHUNK + ncells + 1;
   const dim3 grid(ssim::ceil_div(max_chunks, WAVES));
-  SSIM_TIMED("k_pp_cells", st,
+  SSIM_TIMED("k_pp_near", st,
              hipLaunchKernelGGL(k_pp_near, grid, dim3(PB), 0, st, tx, ty, tz, origin[0],
                                 origin[1], origin[2], cell, nc[0], nc[1], nc[2],
                                 cell_start, static_cast<const float4*>(near), tcell_start,
