@@ -3,23 +3,21 @@
 //
 // The sort takes 8 key bits per pass over tiles of RTILE = 1024 elements
 // (4 rounds of one element per thread of a 256-thread workgroup):
-//   k_radix_hist0   — per-tile digit counts of the first pass (digit-major,
+//   k_radix_hist    — per-tile digit counts of the pass in LDS (digit-major,
 //                     hist[d * hrow + b], rows padded to a multiple of 4);
-//                     it also clears the count arrays of the later passes;
 //   k_radix_scatter — the tile's first slot per digit (its own exclusive
 //                     prefix over the tile counts, or k_radix_rows's when
 //                     there are many tiles), then per round the rank of an
 //                     element among the lanes of its wave with the same
 //                     digit (nine ballots), wave totals folded in wave
-//                     order, and the store; the destination tile's count
-//                     of the NEXT pass's digit is added as it goes, so only
-//                     the first pass needs a counting kernel.
+//                     order, and the store.
 // Tiles, rounds, waves and lanes are taken in index order, so equal keys
 // keep their input order: the sort is stable, and the result does not
-// depend on scheduling (the next-pass counts are sums of ones).  A 3-pass
-// sort is 4 launches (7 over more than FUSE_TILES tiles, with k_radix_rows
-// before each scatter), and the passes alternate between two scratch pairs
-// so the last one writes the caller's arrays.
+// depend on scheduling.  A pass is 2 launches (3 over more than FUSE_TILES
+// tiles, with k_radix_rows before the scatter), and the passes alternate
+// between two scratch pairs so the last one writes the caller's arrays.
+// (Counting the next pass's digits inside the scatter, by destination tile,
+// costs global atomics; the extra pass over the keys counts in LDS.)
 #include "ssim_internal.h"
 
 namespace {
@@ -32,39 +30,47 @@ constexpr int FUSE_TILES = 64;     // up to this many tiles a scatter block
                                    // is latency-bound; beyond, k_radix_rows)
 constexpr int SCAN_T = 1024;       // threads of the single-workgroup scan
 
-__global__ __launch_bounds__(RT) void k_radix_hist0(int n, const int* __restrict__ keys,
-                                                    int hrow, int* __restrict__ hist,
-                                                    int* __restrict__ zero, int nzero) {
+// launched with hrow workgroups: the ones past the last tile write the
+// zero padding of the rows
+__global__ __launch_bounds__(RT) void k_radix_hist(int n, const int* __restrict__ keys,
+                                                   int shift, int hrow,
+                                                   int* __restrict__ hist) {
   __shared__ int cnt[256];
   const int t = threadIdx.x;
-  for (int i = blockIdx.x * RT + t; i < nzero; i += gridDim.x * RT) zero[i] = 0;
   cnt[t] = 0;
   __syncthreads();
   const int base = blockIdx.x * RTILE;
+  int k[RIPT];
 #pragma unroll
   for (int r = 0; r < RIPT; ++r) {
     const int i = base + r * RT + t;
-    if (i < n) atomicAdd(&cnt[keys[i] & 255], 1);
+    k[r] = i < n ? keys[i] : 0;
   }
+#pragma unroll
+  for (int r = 0; r < RIPT; ++r)
+    if (base + r * RT + t < n) atomicAdd(&cnt[(k[r] >> shift) & 255], 1);
   __syncthreads();
   hist[(size_t)t * hrow + blockIdx.x] = cnt[t];
 }
 
-// Exclusive prefix of v over the 256 threads of the workgroup (s: 256 ints
-// of LDS, free again on return).
+// Exclusive prefix of v over the 256 threads of the workgroup: inclusive
+// scans of the four waves by shuffles, then the totals of the waves before
+// (s: 4 ints of LDS, free again on return).
 __device__ int excl_scan256(int v, int* s) {
-  const int t = threadIdx.x;
-  s[t] = v;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    const int a = t >= off ? s[t - off] : 0;
-    __syncthreads();
-    s[t] += a;
-    __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int y = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int z = __shfl_up(y, off, 64);
+    if (lane >= off) y += z;
   }
-  const int incl = s[t];
+  if (lane == 63) s[w] = y;
   __syncthreads();
-  return incl - v;
+  int pre = 0;
+#pragma unroll
+  for (int q = 0; q < RT / 64; ++q) pre += q < w ? s[q] : 0;
+  __syncthreads();
+  return pre + y - v;
 }
 
 // Counts are digit-major, hist[d * hrow + b], rows padded to hrow (a
@@ -114,8 +120,7 @@ __global__ __launch_bounds__(RT) void k_radix_rows(const int* __restrict__ hist,
 __global__ __launch_bounds__(RT) void k_radix_scatter(
     int n, const int* __restrict__ keys, const int* __restrict__ vals, int shift,
     int hrow, const int* __restrict__ hist, const int* __restrict__ offs,
-    int* __restrict__ keys_out, int* __restrict__ vals_out, int* __restrict__ hist_next,
-    int shift_next) {
+    int* __restrict__ keys_out, int* __restrict__ vals_out) {
   constexpr int W = RT / 64;
   __shared__ int s[256];
   __shared__ int run[256];
@@ -162,27 +167,10 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
     }
     run[t] = r0;
     __syncthreads();
-    int slot = 0;
     if (valid) {
-      slot = wb[w][d] + rank;
+      const int slot = wb[w][d] + rank;
       keys_out[slot] = k[r];
       vals_out[slot] = v[r];
-    }
-    if (hist_next) {
-      // the next pass's count of (digit, destination tile): lanes of the
-      // wave that share one add it together (up to four such groups;
-      // keys with repetitive digits would otherwise serialise on one
-      // counter), the rest one by one
-      const int combo = valid ? ((k[r] >> shift_next) & 255) * hrow + slot / RTILE : -1;
-      unsigned long long pend = __ballot(valid);
-      for (int it = 0; it < 4 && pend; ++it) {
-        const int leader = __ffsll((long long)pend) - 1;
-        const int lc = __shfl(combo, leader, 64);
-        const unsigned long long same = __ballot(valid && combo == lc) & pend;
-        if (lane == leader) atomicAdd(&hist_next[lc], __popcll(same));
-        pend &= ~same;
-      }
-      if ((pend >> lane) & 1ull) atomicAdd(&hist_next[combo], 1);
     }
   }
 }
@@ -395,17 +383,15 @@ int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
   int* hist = reinterpret_cast<int*>(w);
   w += align256(4 * ht * sizeof(int));
   int* offs = reinterpret_cast<int*>(w);
-  // pass 0's rows: the padding stays zero (tiles >= hrow - 3 write the rest)
-  hipLaunchKernelGGL(k_radix_hist0, dim3(hrow), dim3(RT), 0, st, n, keys, hrow, hist,
-                     hist + ht, (int)((passes - 1) * ht));
-  SSIM_LAUNCHED("k_radix_hist0");
   const int* sk = keys;
   const int* sv = vals;
   for (int p = 0; p < passes; ++p) {
     const bool last = p == passes - 1 && passes >= 2;
     int* dk = last ? keys : pk[p & 1];
     int* dv = last ? vals : pv[p & 1];
-    const int* h = hist + p * ht;
+    const int* h = hist;
+    hipLaunchKernelGGL(k_radix_hist, dim3(hrow), dim3(RT), 0, st, n, sk, 8 * p, hrow, hist);
+    SSIM_LAUNCHED("k_radix_hist");
     const int* o = nullptr;
     if (tiles > FUSE_TILES) {
       hipLaunchKernelGGL(k_radix_rows, dim3(256), dim3(RT), 0, st, h, hrow, offs,
@@ -414,8 +400,7 @@ int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
       o = offs;
     }
     hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(RT), 0, st, n, sk, sv, 8 * p,
-                       hrow, h, o, dk, dv,
-                       p + 1 < passes ? hist + (p + 1) * ht : nullptr, 8 * (p + 1));
+                       hrow, h, o, dk, dv);
     SSIM_LAUNCHED("k_radix_scatter");
     sk = dk;
     sv = dv;
