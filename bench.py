@@ -191,11 +191,12 @@ def pmc_summary(fetch_dir, write_dir, out):
            "factor_fetch": factor["fetch"], "factor_write": factor["write"],
            "kernels": {},
            "commands": [
-               f"rocprofv3 --pmc FETCH_SIZE -d {fetch_dir} -o run -f csv -- "
+               f"rocprofv3 --pmc FETCH_SIZE -d {Path(fetch_dir).name} -o run -f csv -- "
                "python -u bench.py --probe",
-               f"rocprofv3 --pmc WRITE_SIZE -d {write_dir} -o run -f csv -- "
+               f"rocprofv3 --pmc WRITE_SIZE -d {Path(write_dir).name} -o run -f csv -- "
                "python -u bench.py --probe",
-               f"python bench.py --pmc-summary {fetch_dir} {write_dir} {out}"]}
+               f"python bench.py --pmc-summary {Path(fetch_dir).name} "
+               f"{Path(write_dir).name} {Path(out).name}"]}
     for name, pat in PMC_KERNELS.items():
         kib = {}
         for k, v in rows.items():
