@@ -95,20 +95,31 @@ __global__ __launch_bounds__(CB) void k_split_key(
 }
 
 // start[c] = the number of sorted keys below c, c in [0, nbins]: bin c's first
-// slot, what the exclusive scan of the bin counts gives.  Wave w looks at the
-// slots p in [64w, 64w + 64) up to n, with K(n) = nbins and K(-1) = -1; the
-// bins c in (K(64w - 1), K(its last slot)] start among them, at 64w + #{p :
-// K(p) < c}, and these ranges tile [0, nbins].  A bin range the wave fills
-// 64 bins per round, each by a 6-step search over the lanes' sorted keys.
+// slot, what the exclusive scan of the bin counts gives.  The bins up to the
+// first key start at 0 and those past the last key at n (k_run_fill, one
+// thread per bin: a mesh margin leaves thousands of empty bins at both
+// ends).  In between, wave w looks at the slots p in [64w, 64w + 64) below
+// n; the bins c in (K(64w - 1), K(its last slot)] start among them, at
+// 64w + #{p : K(p) < c} (K(-1) taken as the first key), and these ranges
+// tile (first key, last key].  A bin range the wave fills 64 bins per
+// round, each by a 6-step search over the lanes' sorted keys.
+__global__ __launch_bounds__(CB) void k_run_fill(int n, const int* __restrict__ key,
+                                                 int nbins, int* __restrict__ start) {
+  const int c = blockIdx.x * CB + threadIdx.x;
+  if (c > nbins) return;
+  if (n == 0 || c <= key[0]) start[c] = 0;
+  else if (c > key[n - 1]) start[c] = n;
+}
+
 __global__ __launch_bounds__(CB) void k_run_starts(int n, const int* __restrict__ key,
                                                    int nbins, int* __restrict__ start) {
   const int lane = threadIdx.x & 63;
   const int p0 = (int)(((size_t)blockIdx.x * CB + threadIdx.x) >> 6) * 64;
-  if (p0 > n) return;  // whole waves
+  if (p0 >= n) return;  // whole waves
   const int p = p0 + lane;
-  const int kp = p < n ? key[p] : (p == n ? nbins : nbins + 1);
-  const int lo = p0 > 0 ? key[p0 - 1] : -1;
-  const int hi = __shfl(kp, min(63, n - p0), 64);
+  const int kp = p < n ? key[p] : nbins;
+  const int lo = p0 > 0 ? key[p0 - 1] : key[0];
+  const int hi = __shfl(kp, min(63, n - 1 - p0), 64);
   for (int c0 = lo + 1; c0 <= hi; c0 += 64) {
     const int c = c0 + lane;
     int r = 0;  // lanes with a key below c (<= 63: the last slot's key is hi)
@@ -117,6 +128,18 @@ __global__ __launch_bounds__(CB) void k_run_starts(int n, const int* __restrict_
       if (__shfl(kp, r + st - 1, 64) < c) r += st;
     if (c <= hi) start[c] = p0 + r;
   }
+}
+
+int run_starts(int n, const int* key, int nbins, int* start, hipStream_t st) {
+  hipLaunchKernelGGL(k_run_fill, dim3(ssim::ceil_div(nbins + 1, CB)), dim3(CB), 0, st, n,
+                     key, nbins, start);
+  SSIM_LAUNCHED("k_run_fill");
+  if (n == 0) return SSIM_OK;
+  const long long waves = ssim::ceil_div(n, 64);
+  hipLaunchKernelGGL(k_run_starts, dim3(ssim::ceil_div(waves * 64, CB)), dim3(CB), 0, st,
+                     n, key, nbins, start);
+  SSIM_LAUNCHED("k_run_starts");
+  return SSIM_OK;
 }
 
 // Both packed copies in bin order: float4 (x - cx, y - cy, z - cz, m) for
@@ -254,10 +277,7 @@ int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
   }
   // octant-bin starts from the sorted keys (n = 0: all zero), every 8th one
   // is a cell start
-  const long long waves = n / 64 + 1;
-  hipLaunchKernelGGL(k_run_starts, dim3(ssim::ceil_div(waves * 64, CB)), dim3(CB), 0, st,
-                     n, key, nsub, sub_start);
-  SSIM_LAUNCHED("k_run_starts");
+  SSIM_TRY(run_starts(n, key, nsub, sub_start, st));
   hipLaunchKernelGGL(k_every8, dim3(ssim::ceil_div(ncells + 1, CB)), dim3(CB), 0, st,
                      ncells, sub_start, cell_start);
   SSIM_LAUNCHED("k_every8");
@@ -322,10 +342,7 @@ int ssim_split_bin(int n, const double* x, const double* y, const double* z,
     SSIM_TRY(ssim::scan_exclusive((int)K, count, bin_start, scan_work, st, B3, cell_start));
   } else {
     // the block starts from the sorted keys (n = 0: all zero)
-    const long long waves = n / 64 + 1;
-    hipLaunchKernelGGL(k_run_starts, dim3(ssim::ceil_div(waves * 64, CB)), dim3(CB), 0,
-                       st, n, key, ncells, cell_start);
-    SSIM_LAUNCHED("k_run_starts");
+    SSIM_TRY(run_starts(n, key, ncells, cell_start, st));
   }
   if (n > 0) {
     const double hh = 0.5 * h;
