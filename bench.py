@@ -16,12 +16,18 @@ one force evaluation, half-kick, all on the GPU with inputs resident in HBM.
 
 `value` is the PM + learned rate (the method the metric asks for); the
 direct rate, the ratio and the force errors are in the same JSON line.
+`gates` (skipped with --no-gates) checks the method's accuracy targets in
+the same run: 1000 PM + learned steps and 1000 direct FP64 steps of the
+masses from the cold start, the exact FP64 energy at both ends of each
+(`energy_drift_ratio` = PM + learned drift / direct drift), and the median
+force error of PM + learned against the FP64 direct sum every 250 steps.
 Per-kernel durations come from libssim's launch timing (HIP events around
 each launch on its stream) over `--profile-steps` extra steps after the
 timed region; `roofline` is the kernel with the largest time per step.
 Options: --gpus N (1 only), --steps K (PM steps timed), --warmup W,
---direct-steps, --small (1/50 of the bodies), --no-cpu, --probe (a short
-run for hardware-counter passes: a 256 MiB copy, then three PM steps).
+--direct-steps, --small (1/50 of the bodies), --no-cpu, --no-gates,
+--gate-steps, --probe (a short run for hardware-counter passes: a 256 MiB
+copy, then three PM steps).
 """
 from __future__ import annotations
 
