@@ -213,6 +213,93 @@ __device__ void four_step_256(cpx<F>* s, int LS, int nl, const cpx<F>* tw,
   __syncthreads();
 }
 
+// in-place 8-point DFT, natural order out: 4-point DFTs of the even and the
+// odd inputs, the odd ones twiddled by W8^k1, then one radix-2 step
+template <typename F>
+__device__ __forceinline__ void dft8(cpx<F> v[8], bool inverse) {
+  constexpr double C2 = 0.70710678118654752440;
+  const F sg = inverse ? F(1) : F(-1);
+  dft4(v[0], v[2], v[4], v[6], inverse);
+  dft4(v[1], v[3], v[5], v[7], inverse);
+  v[3] = cmul(v[3], cpx<F>{F(C2), sg * F(C2)});
+  v[5] = rot(v[5], inverse);
+  v[7] = cmul(v[7], cpx<F>{F(-C2), sg * F(C2)});
+  cpx<F> o[8];
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    o[k1] = cadd(v[2 * k1], v[2 * k1 + 1]);
+    o[k1 + 4] = csub(v[2 * k1], v[2 * k1 + 1]);
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = o[k];
+}
+
+// ---- 128-point lines: the four-step FFT with 128 = 16 x 8 ----
+// x[n1 + 16 n2] -> Y[n1][k2] = DFT8 over n2 (thread n1), times
+// W128^(n1 k2), then X[k2 + 8 k1] = DFT16 over n1 of Y[.][k2] (thread k2):
+// one LDS exchange instead of a radix-2 and three radix-4 stages.
+// Y[n1][k2] sits at 8 n1 + (k2 ^ (n1 & 7)).
+template <typename F>
+__device__ void four_step_128(cpx<F>* s, int LS, int nl, const cpx<F>* tw,
+                              bool inverse) {
+  constexpr int MAXJ1 = 4096 / 8 / FFT_THREADS, MAXJ3 = 4096 / 16 / FFT_THREADS;
+  cpx<F> v[MAXJ1][8];
+  const int jobs1 = nl * 16, jobs3 = nl * 8;
+#pragma unroll
+  for (int q = 0; q < MAXJ1; ++q) {
+    const int job = threadIdx.x + q * FFT_THREADS;
+    if (job < jobs1) {
+      const cpx<F>* a = s + (job >> 4) * LS;
+      const int n1 = job & 15;
+#pragma unroll
+      for (int n2 = 0; n2 < 8; ++n2) v[q][n2] = a[n1 + 16 * n2];
+      dft8(v[q], inverse);
+#pragma unroll
+      for (int k2 = 1; k2 < 8; ++k2) {
+        cpx<F> w = tw[n1 * k2];
+        if (inverse) w.im = -w.im;
+        v[q][k2] = cmul(v[q][k2], w);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXJ1; ++q) {
+    const int job = threadIdx.x + q * FFT_THREADS;
+    if (job < jobs1) {
+      cpx<F>* a = s + (job >> 4) * LS;
+      const int n1 = job & 15;
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) a[8 * n1 + (k2 ^ (n1 & 7))] = v[q][k2];
+    }
+  }
+  __syncthreads();
+  cpx<F> u[MAXJ3][16];
+#pragma unroll
+  for (int q = 0; q < MAXJ3; ++q) {
+    const int job = threadIdx.x + q * FFT_THREADS;
+    if (job < jobs3) {
+      const cpx<F>* a = s + (job >> 3) * LS;
+      const int k2 = job & 7;
+#pragma unroll
+      for (int n1 = 0; n1 < 16; ++n1) u[q][n1] = a[8 * n1 + (k2 ^ (n1 & 7))];
+      dft16(u[q], inverse);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXJ3; ++q) {
+    const int job = threadIdx.x + q * FFT_THREADS;
+    if (job < jobs3) {
+      cpx<F>* a = s + (job >> 3) * LS;
+      const int k2 = job & 7;
+#pragma unroll
+      for (int k1 = 0; k1 < 16; ++k1) a[k2 + 8 * k1] = u[q][k1];
+    }
+  }
+  __syncthreads();
+}
+
 template <typename F, int MODE>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     const void* __restrict__ vin, void* __restrict__ vout, Pass P,
@@ -305,6 +392,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
 
   if (L == 256) {
     four_step_256(s, LS, nl, twp, inverse != 0);
+  } else if (L == 128) {
+    four_step_128(s, LS, nl, twp, inverse != 0);
   } else {
     // ---- Stockham stages over all lines of the tile ----
     int p = 1;
