@@ -75,6 +75,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def quiet_gc():
+    """Collect the set-up's garbage and freeze what survives before a timed
+    region: a full collection of the interpreter's heap (~50 ms with torch
+    loaded) would otherwise land at a random step of the timed loop."""
+    import gc
+    gc.collect()
+    gc.freeze()
+
+
 def kernel_times(lib, names):
     """{name: (total ms, launches)} of the launches recorded since
     ssim_timing(1)."""
@@ -343,6 +352,7 @@ def main():
     snap = torch.empty((1, direct.snap_width()), dtype=torch.float32, device=dev)
     direct.run(dt, 1, snap)                         # warm-up (+ initial force)
     torch.cuda.synchronize(dev)
+    quiet_gc()
     t0 = time.perf_counter()
     for _ in range(args.direct_steps):
         direct.run(dt, 1, snap)
@@ -372,6 +382,7 @@ def main():
     graph = not args.eager
     fast.run(dt, max(args.warmup, 1), graph=graph)
     torch.cuda.synchronize(dev)
+    quiet_gc()
     t0 = time.perf_counter()
     fast.run(dt, args.steps, graph=graph)
     torch.cuda.synchronize(dev)
