@@ -316,19 +316,20 @@ __global__ __launch_bounds__(PB) void k_pp_near(
 #pragma unroll
           for (int j = 0; j < SG; ++j) q[j] = near[s + j];
           float dx[SG], dy[SG], dz[SG], r2[SG];
-          unsigned long long any = 0;
+          unsigned long long need[SG];
 #pragma unroll
           for (int j = 0; j < SG; ++j) {
             dx[j] = q[j].x - px;
             dy[j] = q[j].y - py;
             dz[j] = q[j].z - pz;
             r2[j] = fmaf(dx[j], dx[j], fmaf(dy[j], dy[j], dz[j] * dz[j]));
-            any |= __ballot(r2[j] < rcut2);
+            need[j] = __ballot(r2[j] < rcut2);
           }
-          if (any == 0) continue;
+          // a source beyond rcut of every lane costs its distance only
 #pragma unroll
           for (int j = 0; j < SG; ++j)
-            pp_term(q[j].w, dx[j], dy[j], dz[j], r2[j], tab, tab_scale, eps2, sx, sy, sz);
+            if (need[j])
+              pp_term(q[j].w, dx[j], dy[j], dz[j], r2[j], tab, tab_scale, eps2, sx, sy, sz);
         }
         for (; s < s1; ++s) {
           const float4 q = near[s];
