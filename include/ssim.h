@@ -446,6 +446,18 @@ int ssim_pm_conv_isolated3(int n, int prec, const void* rho, const void* kim_x,
  * node: phi_k = -4 pi G (m_k / h^dim) / k^2, phi_0 = 0. */
 int ssim_pm_poisson(int dim, const int* grid_n, double h, double G, int prec,
                     void* spectrum, void* stream);
+/* The periodic potential of a 3-D mesh of mass per node in one call:
+ * rfftn, the ssim_pm_poisson factor, irfftn, with the x axis's forward
+ * transform, factor and inverse transform fused in one pass (the two
+ * spectrum round trips of that axis saved); the result equals
+ * ssim_fft_r2c + ssim_pm_poisson + ssim_fft_c2r bit for bit.  The
+ * reference's PM is 2-D and isolated (pm_kernel.py:127-147); this serves
+ * the periodic 3-D plan (TODO.md:225-253).  rho, phi: real grid_n^3;
+ * spec: the half spectrum (grid_n[2]/2 + 1 in the last axis) as scratch;
+ * fft_work prepared for grid_n. */
+int ssim_pm_potential_periodic(const int* grid_n, int prec, const void* rho, void* spec,
+                               void* phi, double h, double G, const void* fft_work,
+                               size_t fft_work_bytes, void* stream);
 /* Periodic central difference g_c = -(phi[+1] - phi[-1]) / (2h). */
 int ssim_pm_gradient(int dim, const int* grid_n, double h, int prec,
                      const void* phi, void* g0, void* g1, void* g2,

@@ -115,6 +115,8 @@ SIGNATURES = {
     "ssim_spectrum_imag": (c_int, [c_longlong, c_int, _P, _P, _P]),
     "ssim_pm_conv_isolated3": (c_int, [c_int, c_int] + [_P] * 9 + [_P, c_size_t,
                                                                   _P]),
+    "ssim_pm_potential_periodic": (c_int, [_P, c_int, _P, _P, _P, c_double, c_double, _P,
+                                           c_size_t, _P]),
     "ssim_pm_poisson": (c_int, [c_int, _P, c_double, c_double, c_int, _P, _P]),
     "ssim_pm_gradient": (c_int, [c_int, _P, c_double, c_int, _P, _P, _P, _P,
                                  _P]),

@@ -29,7 +29,10 @@ constexpr int FFT_TILE_BYTES = 32768;  // LDS tile of one workgroup
 constexpr int FFT_MAXB = 64;           // lines per tile along a strided axis
 constexpr int FFT_MAX_L = 4096;
 
-enum { MODE_C2C = 0, MODE_R2C = 1, MODE_C2R = 2 };
+// MODE_POISSON: a strided C2C pass along the first axis of a 3-D half
+// spectrum that runs forward, multiplies by the periodic Poisson factor
+// and runs inverse on the same tile (see poisson_lines)
+enum { MODE_C2C = 0, MODE_R2C = 1, MODE_C2R = 2, MODE_POISSON = 3 };
 
 template <typename F>
 struct cpx {
@@ -94,6 +97,10 @@ struct Pass {
   long long R1, s_in_hi, s_out_hi;
   int tw_lds;                 // 1: the twiddle table is staged in LDS
   int out_es, out_c;          // C2R: element e goes to out[(off + e) * es + c]
+  // MODE_POISSON: real grid n0 x n1 x n2 (the pass runs along n0, the
+  // inner index is (ky, kz) over n1 x (n2 / 2 + 1)), cell h, G
+  int pn0, pn1, pn2;
+  double ph, pG;
 };
 
 __device__ __forceinline__ long long row_off(long long r, long long R1,
@@ -300,6 +307,127 @@ __device__ void four_step_128(cpx<F>* s, int LS, int nl, const cpx<F>* tw,
   __syncthreads();
 }
 
+// The periodic Poisson factor on a tile of first-axis lines of the 3-D half
+// spectrum, after their forward transform: element e of line l is mode
+// (kx = e, ky, kz) with (ky, kz) the line's inner index; times
+// -4 pi G / (k^2 h^3), k = 2 pi f / (n h) with f the signed frequency, and
+// 0 for k = 0 — the operations of k_poisson (pm.hip), so the fused solve
+// rounds exactly like the separate passes.
+template <typename F>
+__device__ void poisson_lines(cpx<F>* s, int LS, int nl, int L, const Pass& P, long long o0,
+                              long long b0, int nvalid) {
+  const int W = P.pn2 / 2 + 1;
+  for (int t = threadIdx.x; t < nl * L; t += FFT_THREADS) {
+    const int l = t % nl, e = t / nl;
+    if (l >= nvalid) continue;
+    const long long inner = b0 + l;
+    const int b = (int)(inner / W), c = (int)(inner % W);
+    const int f[3] = {e <= P.pn0 / 2 ? e : e - P.pn0, b <= P.pn1 / 2 ? b : b - P.pn1, c};
+    const int ns[3] = {P.pn0, P.pn1, P.pn2};
+    double k2 = 0.0;
+    for (int q = 0; q < 3; ++q) {
+      const double k = 2.0 * M_PI * (double)f[q] / ((double)ns[q] * P.ph);
+      k2 += k * k;
+    }
+    const double vol = P.ph * P.ph * P.ph;
+    const double g = (k2 > 0.0) ? (-4.0 * M_PI * P.pG) / (k2 * vol) : 0.0;
+    cpx<F>& v = s[l * LS + e];
+    v = {(F)(v.re * g), (F)(v.im * g)};
+  }
+  (void)o0;
+}
+
+// All lines of a tile transformed in LDS (forward or inverse): the
+// four-step forms for 128 and 256 points, Stockham radix-4 (+ one radix-2
+// stage) otherwise.
+template <typename F>
+__device__ void tile_fft(cpx<F>* s, int LS, int nl, int L, int log2L, const cpx<F>* twp,
+                         int inverse) {
+  const int E = nl * L;
+  if (L == 256) {
+    four_step_256(s, LS, nl, twp, inverse != 0);
+  } else if (L == 128) {
+    four_step_128(s, LS, nl, twp, inverse != 0);
+  } else {
+    // ---- Stockham stages over all lines of the tile ----
+    int p = 1;
+    if (log2L & 1) {  // one radix-2 stage (p == 1: k == 0, no twiddle)
+      constexpr int NB = max_elems<F>() / 2 / FFT_THREADS;
+      const int T = L / 2;
+      cpx<F> r0[NB], r1[NB];
+  #pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int u = threadIdx.x + nb * FFT_THREADS;
+        if (u < E / 2) {
+          const cpx<F>* a = s + (u / T) * LS;
+          const int i = u % T;
+          r0[nb] = a[i];
+          r1[nb] = a[i + T];
+        }
+      }
+      __syncthreads();
+  #pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int u = threadIdx.x + nb * FFT_THREADS;
+        if (u < E / 2) {
+          cpx<F>* a = s + (u / T) * LS;
+          const int i = u % T;
+          a[2 * i] = cadd(r0[nb], r1[nb]);
+          a[2 * i + 1] = csub(r0[nb], r1[nb]);
+        }
+      }
+      __syncthreads();
+      p = 2;
+    }
+    for (; p < L; p *= 4) {
+      constexpr int NB = max_elems<F>() / 4 / FFT_THREADS;
+      const int T = L / 4;
+      const int twstep = L / (4 * p);
+      cpx<F> r[NB][4];
+  #pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int u = threadIdx.x + nb * FFT_THREADS;
+        if (u < E / 4) {
+          const int i = u % T;
+          const int k = i & (p - 1);
+          const cpx<F>* a = s + (u / T) * LS;
+  #pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            cpx<F> x = a[i + q * T];
+            if (q > 0 && k > 0) {
+              cpx<F> w = twp[q * k * twstep];
+              if (inverse) w.im = -w.im;
+              x = cmul(x, w);
+            }
+            r[nb][q] = x;
+          }
+        }
+      }
+      __syncthreads();
+  #pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const int u = threadIdx.x + nb * FFT_THREADS;
+        if (u < E / 4) {
+          const int i = u % T;
+          const int k = i & (p - 1);
+          cpx<F>* a = s + (u / T) * LS;
+          const cpx<F> s02 = cadd(r[nb][0], r[nb][2]);
+          const cpx<F> d02 = csub(r[nb][0], r[nb][2]);
+          const cpx<F> s13 = cadd(r[nb][1], r[nb][3]);
+          const cpx<F> d13 = rot(csub(r[nb][1], r[nb][3]), inverse != 0);
+          const int j = (i - k) * 4 + k;
+          a[j] = cadd(s02, s13);
+          a[j + p] = cadd(d02, d13);
+          a[j + 2 * p] = csub(s02, s13);
+          a[j + 3 * p] = csub(d02, d13);
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+}
+
 template <typename F, int MODE>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     const void* __restrict__ vin, void* __restrict__ vout, Pass P,
@@ -390,86 +518,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
   }
   __syncthreads();
 
-  if (L == 256) {
-    four_step_256(s, LS, nl, twp, inverse != 0);
-  } else if (L == 128) {
-    four_step_128(s, LS, nl, twp, inverse != 0);
-  } else {
-    // ---- Stockham stages over all lines of the tile ----
-    int p = 1;
-    if (P.log2L & 1) {  // one radix-2 stage (p == 1: k == 0, no twiddle)
-      constexpr int NB = max_elems<F>() / 2 / FFT_THREADS;
-      const int T = L / 2;
-      cpx<F> r0[NB], r1[NB];
-  #pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const int u = threadIdx.x + nb * FFT_THREADS;
-        if (u < E / 2) {
-          const cpx<F>* a = s + (u / T) * LS;
-          const int i = u % T;
-          r0[nb] = a[i];
-          r1[nb] = a[i + T];
-        }
-      }
-      __syncthreads();
-  #pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const int u = threadIdx.x + nb * FFT_THREADS;
-        if (u < E / 2) {
-          cpx<F>* a = s + (u / T) * LS;
-          const int i = u % T;
-          a[2 * i] = cadd(r0[nb], r1[nb]);
-          a[2 * i + 1] = csub(r0[nb], r1[nb]);
-        }
-      }
-      __syncthreads();
-      p = 2;
-    }
-    for (; p < L; p *= 4) {
-      constexpr int NB = max_elems<F>() / 4 / FFT_THREADS;
-      const int T = L / 4;
-      const int twstep = L / (4 * p);
-      cpx<F> r[NB][4];
-  #pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const int u = threadIdx.x + nb * FFT_THREADS;
-        if (u < E / 4) {
-          const int i = u % T;
-          const int k = i & (p - 1);
-          const cpx<F>* a = s + (u / T) * LS;
-  #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            cpx<F> x = a[i + q * T];
-            if (q > 0 && k > 0) {
-              cpx<F> w = twp[q * k * twstep];
-              if (inverse) w.im = -w.im;
-              x = cmul(x, w);
-            }
-            r[nb][q] = x;
-          }
-        }
-      }
-      __syncthreads();
-  #pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const int u = threadIdx.x + nb * FFT_THREADS;
-        if (u < E / 4) {
-          const int i = u % T;
-          const int k = i & (p - 1);
-          cpx<F>* a = s + (u / T) * LS;
-          const cpx<F> s02 = cadd(r[nb][0], r[nb][2]);
-          const cpx<F> d02 = csub(r[nb][0], r[nb][2]);
-          const cpx<F> s13 = cadd(r[nb][1], r[nb][3]);
-          const cpx<F> d13 = rot(csub(r[nb][1], r[nb][3]), inverse != 0);
-          const int j = (i - k) * 4 + k;
-          a[j] = cadd(s02, s13);
-          a[j + p] = cadd(d02, d13);
-          a[j + 2 * p] = csub(s02, s13);
-          a[j + 3 * p] = csub(d02, d13);
-        }
-      }
-      __syncthreads();
-    }
+  tile_fft(s, LS, nl, L, P.log2L, twp, inverse);
+  if (MODE == MODE_POISSON) {
+    poisson_lines(s, LS, nl, L, P, o0, b0, nvalid);
+    __syncthreads();
+    tile_fft(s, LS, nl, L, P.log2L, twp, 1);
   }
 
   // ---- store ----
@@ -885,6 +938,37 @@ int fft_c2r(int ndim, const int* dims, cpx<F>* in, F* out,
   return SSIM_OK;
 }
 
+// The periodic potential of a 3-D density in five passes: R2C rows, forward
+// y lines, the fused x pass (forward, Poisson factor, inverse), inverse y
+// lines, C2R rows — the same operations as fft_r2c + k_poisson + fft_c2r
+// without the two spectrum round trips of the x axis.
+template <typename F>
+int poisson3(const int* dims, const F* rho, cpx<F>* spec, F* phi, double h, double G,
+             const unsigned char* work, int prec, hipStream_t st) {
+  const int cdims[3] = {dims[0], dims[1], dims[2] / 2 + 1};
+  const long long rows = (long long)dims[0] * dims[1];
+  const int L = dims[2];
+  const double total = (double)dims[0] * dims[1] * dims[2];
+  SSIM_TRY((launch_pass<F, MODE_R2C>(rho, spec, row_pass(rows, L, L, cdims[2], tile_elems<F>()),
+                                     work + tw_offset(3, dims, 2, prec), 0, F(1), st)));
+  SSIM_TRY((launch_pass<F, MODE_C2C>(spec, spec, strided_pass(3, cdims, 1, tile_elems<F>()),
+                                     work + tw_offset(3, dims, 1, prec), 0, F(1), st)));
+  Pass px = strided_pass(3, cdims, 0, tile_elems<F>());
+  px.pn0 = dims[0];
+  px.pn1 = dims[1];
+  px.pn2 = dims[2];
+  px.ph = h;
+  px.pG = G;
+  SSIM_TRY((launch_pass<F, MODE_POISSON>(spec, spec, px, work + tw_offset(3, dims, 0, prec),
+                                         0, F(1), st)));
+  SSIM_TRY((launch_pass<F, MODE_C2C>(spec, spec, strided_pass(3, cdims, 1, tile_elems<F>()),
+                                     work + tw_offset(3, dims, 1, prec), 1, F(1), st)));
+  SSIM_TRY((launch_pass<F, MODE_C2R>(spec, phi, row_pass(rows, L, cdims[2], L, tile_elems<F>()),
+                                     work + tw_offset(3, dims, 2, prec), 1, F(1.0 / total),
+                                     st)));
+  return SSIM_OK;
+}
+
 // The 256^3 padded convolution (n = 128) from register-direct kernels:
 // R2C rows, forward y lines, the fused x axis, and per component inverse y
 // lines + C2R rows; spec and tmp as in conv_isolated3.
@@ -1078,12 +1162,37 @@ int fft_c2r(int ndim, const int* dims, int prec, void* in, void* out,
                           static_cast<float*>(out), w, prec, st);
 }
 
+int pm_potential_periodic(const int* dims, int prec, const void* rho, void* spec,
+                          void* phi, double h, double G, const void* work,
+                          size_t work_bytes, hipStream_t st) {
+  SSIM_REQUIRE(valid_dims(3, dims),
+               "FFT dims must be powers of two in [2, %d]", FFT_MAX_L);
+  SSIM_REQUIRE(rho && spec && phi && work, "null pointer");
+  SSIM_REQUIRE(h > 0.0, "h must be > 0");
+  SSIM_REQUIRE(work_bytes >= fft_work_bytes(3, dims, prec), "FFT workspace too small");
+  const unsigned char* w = static_cast<const unsigned char*>(work);
+  if (prec == SSIM_PREC_F64)
+    return ::poisson3<double>(dims, static_cast<const double*>(rho),
+                              static_cast<cpx<double>*>(spec), static_cast<double*>(phi), h,
+                              G, w, prec, st);
+  return ::poisson3<float>(dims, static_cast<const float*>(rho),
+                           static_cast<cpx<float>*>(spec), static_cast<float*>(phi), h, G, w,
+                           prec, st);
+}
+
 }  // namespace ssim
 
 extern "C" {
 
 size_t ssim_fft_work_bytes(int ndim, const int* dims, int prec) {
   return ssim::fft_work_bytes(ndim, dims, prec);
+}
+
+int ssim_pm_potential_periodic(const int* grid_n, int prec, const void* rho, void* spec,
+                               void* phi, double h, double G, const void* fft_work,
+                               size_t fft_work_bytes, void* stream) {
+  return ssim::pm_potential_periodic(grid_n, prec, rho, spec, phi, h, G, fft_work,
+                                     fft_work_bytes, ssim::as_stream(stream));
 }
 
 int ssim_pm_conv_isolated3(int n, int prec, const void* rho, const void* kim_x,

@@ -392,14 +392,25 @@ class PeriodicPM3D:
         self.g = [torch.empty(self.grid, dtype=dt, device=dev) for _ in range(3)]
         self._gn = _lib.int_array(self.grid)
 
+    # True: the potential by ssim_pm_potential_periodic (the x axis's forward
+    # transform, Poisson factor and inverse transform in one pass); False:
+    # R2C, ssim_pm_poisson, C2R as three calls (the same result bit for bit)
+    fused = True
+
     def field(self, pos, m):
         """Acceleration field (3 meshes) of masses m at pos."""
         deposit(pos, m, self.grid, self.origin, self.h, True, self.prec,
                 mesh=self.rho)
-        self.plan.r2c(self.rho, self.spec)
-        _lib.call("ssim_pm_poisson", 3, _addr(self._gn), self.h, self.G,
-                  self.prec, _lib.ptr(self.spec), D.stream(self.dev))
-        self.plan.c2r(self.spec, self.phi)
+        if self.fused:
+            _lib.call("ssim_pm_potential_periodic", _addr(self._gn), self.prec,
+                      _lib.ptr(self.rho), _lib.ptr(self.spec), _lib.ptr(self.phi),
+                      self.h, self.G, _lib.ptr(self.plan.work), self.plan.work_bytes,
+                      D.stream(self.dev))
+        else:
+            self.plan.r2c(self.rho, self.spec)
+            _lib.call("ssim_pm_poisson", 3, _addr(self._gn), self.h, self.G,
+                      self.prec, _lib.ptr(self.spec), D.stream(self.dev))
+            self.plan.c2r(self.spec, self.phi)
         _lib.call("ssim_pm_gradient", 3, _addr(self._gn), self.h, self.prec,
                   _lib.ptr(self.phi), *[_lib.ptr(g) for g in self.g],
                   D.stream(self.dev))
