@@ -233,6 +233,45 @@ def pmc_summary(fetch_dir, write_dir, out):
     print(json.dumps(rec, indent=1))
 
 
+def near_hilo_cost(lib, fast, dt, args, direct, nb, ms_step, kernels):
+    """The same run with the near field's hi/lo form (ssim_pp_near_hilo:
+    FP64 pair arithmetic, p99 error of the near field ~3e-7 against ~3e-6):
+    its step time in the timed run's launch mode, its kernel's duration
+    beside the plain kernel's, and the method's force error with it."""
+    from solarsystemsimulator_amd.learned_pm import force_errors
+    sf = fast.split
+    graph = not args.eager
+    sf.near_hilo = True
+    try:
+        fast.run(dt, 2, graph=graph)
+        import torch
+        torch.cuda.synchronize(fast.dev)
+        quiet_gc()
+        t0 = time.perf_counter()
+        fast.run(dt, args.steps, graph=graph)
+        torch.cuda.synchronize(fast.dev)
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        lib.ssim_timing(1)
+        fast.run(dt, args.profile_steps, graph=False)
+        tot, cnt = kernel_times(lib, ["k_pp_near_hilo"])["k_pp_near_hilo"]
+        lib.ssim_timing(0)
+        for c in range(3):
+            direct.x[c][:nb].copy_(fast.x[c])
+        ref = [c.cpu().numpy() for c in direct.accelerations()]
+        err = force_errors(fast.force(learned=True), ref)
+    finally:
+        sf.near_hilo = False
+    fast.force(learned=True)
+    plain = kernels.get("k_pp_near", {}).get("avg_us")
+    us = tot / max(cnt, 1) * 1e3
+    log(f"near field hi/lo: {ms:.3f} ms/step ({ms / ms_step:.2f}x), kernel "
+        f"{us:.1f} us against {plain or 0:.1f} us, median force error {np.median(err):.3e}")
+    return {"ms_per_step": ms, "step_ratio": ms / ms_step, "kernel": "k_pp_near_hilo",
+            "kernel_us": us, "kernel_us_plain": plain,
+            "force_err_median": float(np.median(err)),
+            "near_field_p99_err": "<= 1e-6 (tests/test_p3m_gpu.py)"}
+
+
 def gates(dev, pos, m, tpos, net, eps2, dt, steps, every=250):
     """The method's accuracy gates over `steps` steps from the cold start:
     the exact FP64 energy of the masses (ssim_energy, all pairs) at the
@@ -476,6 +515,9 @@ def main():
             "peak": d["peak"], "unit": d["unit"], "frac": d["frac"],
             "traffic": traffic, "launch_us": d["avg_us"], "work": d["work"]}
 
+    # ---- the hi/lo near field (FP64 pair arithmetic): its cost -------------
+    hilo = near_hilo_cost(lib, fast, dt, args, direct, nb, ms_step, kernels)
+
     gate = None
     if not args.no_gates and not args.small:
         fast.close()
@@ -517,6 +559,7 @@ def main():
         "gates": gate,
         "energy_drift_ratio": None if gate is None else gate["energy_drift_ratio"],
         "kernels": kernels,
+        "near_hilo": hilo,
         "roofline": roof,
         "roofline_direct": roof_direct,
         "cpu_baseline": cpu,

@@ -507,8 +507,10 @@ int ssim_cells_bin(int n, const double* x, const double* y, const double* z,
  * packed[n] float4 (x, y, z relative to o - h/2, m) for ssim_pp_short and
  * body[n] double4 ((x - o)/h per axis, m) for ssim_tsc_deposit_blocked and
  * ssim_tsc_deposit_tiles, and (near != NULL) near[n] float4 (x, y, z
- * relative to the corner of the mass's own block, m), the precise copy
- * ssim_pp_short's pair terms read.  With bin_start = NULL only the blocks are binned
+ * relative to the corner of the mass's own block, m), the copy
+ * ssim_pp_near's pair terms read, and (near_lo != NULL) near_lo[n] float4
+ * (the remainders of those three coordinates, 0): hi + lo for
+ * ssim_pp_near_hilo.  With bin_start = NULL only the blocks are binned
  * (key = block index, stable order, order inside a block by index): all
  * that ssim_pp_short and ssim_tsc_deposit_tiles read.  It replaces the two
  * cell lists (nodes and near-field cells) of ssim_cells_build.  No
@@ -517,7 +519,8 @@ size_t ssim_split_bin_work_bytes(int n, int grid_n, int block);
 int ssim_split_bin(int n, const double* x, const double* y, const double* z,
                    const double* m, int grid_n, const double* origin, double h,
                    int block, int* bin_start, int* cell_start, int* order, void* packed,
-                   void* body, void* near, void* work, size_t work_bytes, void* stream);
+                   void* body, void* near, void* near_lo, void* work, size_t work_bytes,
+                   void* stream);
 /* ADD (accumulate != 0) or STORE the near-field force G m d g(r) /
  * (r^2 + eps2)^(3/2), r < rcut, of the binned sources (cell_start +
  * packed, from ssim_cells_build or ssim_split_bin) to / into nt targets'
@@ -547,6 +550,19 @@ int ssim_pp_near(int nt, const double* tx, const double* ty, const double* tz,
                  const void* near, const int* tcell_start, const int* torder, double G,
                  double eps2, double rs, double rcut, double* ax, double* ay, double* az,
                  int accumulate, void* work, size_t work_bytes, void* stream);
+/* ssim_pp_near with each mass position as an FP32 pair: near (hi) from
+ * ssim_split_bin and near_lo (lo, its near_lo output), and each target as
+ * FP64 relative to the cell's corner, so that the pair difference (t - hi)
+ * - lo keeps about 48 bits; the pair factor is interpolated quadratically
+ * in a table of GT + 2 entries.  Same order of summation, same semantics;
+ * the force error against the FP64 sum drops from p99 ~2e-6 to < 1e-6.
+ * New: the reference has no split force. */
+int ssim_pp_near_hilo(int nt, const double* tx, const double* ty, const double* tz,
+                      const double* origin, double cell, const int* nc,
+                      const int* cell_start, const void* near, const void* near_lo,
+                      const int* tcell_start, const int* torder, double G, double eps2,
+                      double rs, double rcut, double* ax, double* ay, double* az,
+                      int accumulate, void* work, size_t work_bytes, void* stream);
 /* Work accounting of ssim_pp_short with the same arguments: counts[0] =
  * source-target pairs tested, counts[1] = pairs closer than rcut (a body
  * with itself excluded); counts is a device array of 2, no forces. */

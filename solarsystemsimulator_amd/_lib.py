@@ -129,7 +129,7 @@ SIGNATURES = {
                                c_size_t, _P]),
     "ssim_split_bin_work_bytes": (c_size_t, [c_int, c_int, c_int]),
     "ssim_split_bin": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double, c_int, _P,
-                               _P, _P, _P, _P, _P, _P, c_size_t, _P]),
+                               _P, _P, _P, _P, _P, _P, _P, c_size_t, _P]),
     "ssim_pp_work_bytes": (c_size_t, [c_int, c_int]),
     "ssim_pp_short": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P,
                               _P, c_double, c_double, c_double, c_double, _P,
@@ -137,6 +137,9 @@ SIGNATURES = {
     "ssim_pp_near": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P, _P,
                              c_double, c_double, c_double, c_double, _P, _P, _P, c_int,
                              _P, c_size_t, _P]),
+    "ssim_pp_near_hilo": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P, _P,
+                                  _P, c_double, c_double, c_double, c_double, _P, _P, _P,
+                                  c_int, _P, c_size_t, _P]),
     "ssim_pp_count": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P,
                               _P, c_double, c_double, c_double, _P, _P, c_size_t,
                               _P]),

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(CB) void k_split_pack(
     const double* __restrict__ m, double ox, double oy, double oz, double h,
     double cx, double cy, double cz, float4* __restrict__ packed,
     double4* __restrict__ body, const int* __restrict__ key, int key_div, int nb,
-    double cell, float4* __restrict__ near) {
+    double cell, float4* __restrict__ near, float4* __restrict__ near_lo) {
   const int s = blockIdx.x * CB + threadIdx.x;
   if (s >= n) return;
   const int p = order[s];
@@ -166,8 +166,14 @@ __global__ __launch_bounds__(CB) void k_split_pack(
     // of the coordinates
     const int blk = key[s] / key_div;
     const int A = blk / (nb * nb), B = (blk / nb) % nb, C = blk % nb;
-    near[s] = make_float4((float)(px - (cx + A * cell)), (float)(py - (cy + B * cell)),
-                          (float)(pz - (cz + C * cell)), (float)pm);
+    const double rx = px - (cx + A * cell), ry = py - (cy + B * cell),
+                 rz = pz - (cz + C * cell);
+    const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
+    near[s] = make_float4(hx, hy, hz, (float)pm);
+    // the remainders: hi + lo is the block-relative position to ~2^-48
+    if (near_lo)
+      near_lo[s] = make_float4((float)(rx - (double)hx), (float)(ry - (double)hy),
+                               (float)(rz - (double)hz), 0.0f);
   }
 }
 
@@ -301,13 +307,15 @@ size_t ssim_split_bin_work_bytes(int n, int grid_n, int block) {
 int ssim_split_bin(int n, const double* x, const double* y, const double* z,
                    const double* m, int grid_n, const double* origin, double h,
                    int block, int* bin_start, int* cell_start, int* order, void* packed,
-                   void* body, void* near, void* work, size_t work_bytes, void* stream) {
+                   void* body, void* near, void* near_lo, void* work, size_t work_bytes,
+                   void* stream) {
   SSIM_REQUIRE(n >= 0, "n must be >= 0");
   SSIM_REQUIRE(grid_n >= 1 && block >= 1 && h > 0.0 && origin, "bad mesh");
   const long long K = split_bins(grid_n, block);
   SSIM_REQUIRE(K <= (1LL << 30), "too many bins: %lld", K);
   SSIM_REQUIRE(cell_start && order && packed && body && (n == 0 || (x && y && z && m)),
                "null pointer");
+  SSIM_REQUIRE(!near_lo || near, "near_lo needs near");
   SSIM_REQUIRE(work && work_bytes >= ssim_split_bin_work_bytes(n, grid_n, block),
                "workspace too small");
   hipStream_t st = ssim::as_stream(stream);
@@ -350,7 +358,8 @@ int ssim_split_bin(int n, const double* x, const double* y, const double* z,
                        order, x, y, z, m, origin[0], origin[1], origin[2], h,
                        origin[0] - hh, origin[1] - hh, origin[2] - hh,
                        static_cast<float4*>(packed), static_cast<double4*>(body), key,
-                       nodes ? B3 : 1, nb, block * h, static_cast<float4*>(near));
+                       nodes ? B3 : 1, nb, block * h, static_cast<float4*>(near),
+                       static_cast<float4*>(near_lo));
     SSIM_LAUNCHED("k_split_pack");
   }
   return SSIM_OK;

@@ -27,7 +27,10 @@
 // force (largest at r ~ rs/20, where g - 1 ~ r^3 is least smooth in u) and
 // ~1e-7 elsewhere, and a tested pair then costs one table read and one
 // hardware reciprocal square root (v_rsq_f32, 1 ulp) instead of erfc, exp
-// and two square roots.
+// and two square roots.  The hi/lo near field (ssim_pp_near_hilo) does the
+// pair arithmetic in FP64 on positions given as FP32 hi + lo pairs and
+// interpolates the same table quadratically (table entries GT + 1 and
+// GT + 2 are zero).
 #include "ssim_internal.h"
 
 namespace {
@@ -43,7 +46,7 @@ constexpr int GT = 4096;           // intervals of the gS(sqrt(u)) table
 __global__ __launch_bounds__(PB) void k_gtab(double rs, double rcut,
                                              float2* __restrict__ tab) {
   const int i = blockIdx.x * PB + threadIdx.x;
-  if (i > GT) return;
+  if (i > GT + 1) return;
   const double r0 = rcut * sqrt((double)i / GT), r1 = rcut * sqrt((double)(i + 1) / GT);
   const double g0 = ssim_split_g(r0, rs, rcut);
   tab[i] = make_float2((float)g0, (float)(ssim_split_g(r1, rs, rcut) - g0));
@@ -154,6 +157,32 @@ __device__ __forceinline__ void pp_term(float m, float dx, float dy, float dz, f
   sz = fmaf(f, dz, sz);
 }
 
+// The term in FP64 for the hi/lo kernel: d and r^2 in FP64, gS from the
+// table in FP32 but interpolated quadratically through the nodes i, i + 1,
+// i + 2 (g_i + t d_i + t (t - 1) / 2 (d_{i+1} - d_i), d = the stored
+// differences: the table's error drops from ~1e-6 to ~1e-8 of g), and
+// (r^2 + eps2)^(-1/2) from the FP32 estimate and one Newton step in FP64
+// (relative error ~1e-13).  With packed FP32 off, an FP64 FMA issues at the
+// rate of an FP32 one.
+__device__ __forceinline__ void pp_term_d(float m, double dx, double dy, double dz,
+                                          double r2, const float2* tab, float tab_scale,
+                                          double eps2, double& sx, double& sy, double& sz) {
+  const float t = fminf((float)r2 * tab_scale, (float)GT);
+  const int i = (int)t;
+  const float2 e0 = tab[i], e1 = tab[i + 1];
+  const float fr = __builtin_amdgcn_fractf(t);
+  const float c = fmaf((fr - 1.0f) * (e1.y - e0.y), 0.5f, e0.y);
+  const float g = fmaf(fr, c, e0.x);
+  const double x = r2 + eps2;
+  double y = (double)__builtin_amdgcn_rsqf((float)x);
+  const double res = fma(-(x * y), y, 1.0);
+  y = fma(0.5 * y, res, y);
+  const double f = (double)m * (double)g * (y * y * y);
+  sx = fma(f, dx, sx);
+  sy = fma(f, dy, sy);
+  sz = fma(f, dz, sz);
+}
+
 // COUNT: instead of forces, count the pairs tested and the pairs inside
 // rcut (excluding a body with itself), for the work accounting of a run.
 template <bool COUNT>
@@ -170,8 +199,8 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
   // the grid is sized for the most chunks possible: a workgroup past the
   // last chunk leaves before it loads the table (uniform per workgroup)
   if (blockIdx.x * WAVES >= cstart[ncells]) return;
-  __shared__ float2 tab[GT + 1];
-  for (int i = threadIdx.x; i <= GT; i += PB) tab[i] = gtab[i];
+  __shared__ float2 tab[GT + 2];
+  for (int i = threadIdx.x; i <= GT + 1; i += PB) tab[i] = gtab[i];
   __syncthreads();  // before any wave leaves
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + (threadIdx.x >> 6));
@@ -269,19 +298,21 @@ __global__ __launch_bounds__(PB) void k_pp_cells(
 // force at r ~ rcut/2).  The FP32 terms of one cell are summed in FP32 and
 // each cell's partial is added to FP64 sums (blocked compensated
 // summation).  Fixed source order per target: deterministic.
+template <bool HILO>
 __global__ __launch_bounds__(PB) void k_pp_near(
     const double* __restrict__ tx, const double* __restrict__ ty,
     const double* __restrict__ tz, double ox, double oy, double oz, double cellsz,
     int nc0, int nc1, int nc2, const int* __restrict__ cell_start,
-    const float4* __restrict__ near, const int* __restrict__ tstart,
+    const float4* __restrict__ near, const float4* __restrict__ near_lo,
+    const int* __restrict__ tstart,
     const int* __restrict__ torder, const int* __restrict__ cstart,
-    const int* __restrict__ cmap, int ncells, float eps2,
+    const int* __restrict__ cmap, int ncells, double eps2,
     const float2* __restrict__ gtab, float tab_scale, float rcut2, double G,
     double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
     int accumulate) {
   if (blockIdx.x * WAVES >= cstart[ncells]) return;
-  __shared__ float2 tab[GT + 1];
-  for (int i = threadIdx.x; i <= GT; i += PB) tab[i] = gtab[i];
+  __shared__ float2 tab[GT + 2];
+  for (int i = threadIdx.x; i <= GT + 1; i += PB) tab[i] = gtab[i];
   __syncthreads();  // before any wave leaves
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * WAVES + (threadIdx.x >> 6));
@@ -293,24 +324,69 @@ __global__ __launch_bounds__(PB) void k_pp_near(
   const bool live = lane < count;
   const int tgt = live ? torder[first + lane] : 0;
   const int a = cell / (nc1 * nc2), b = (cell / nc2) % nc1, c = cell % nc2;
-  const float csz = (float)cellsz;
+  const float csz = (float)cellsz, eps2f = (float)eps2;
+  const float pre2 = rcut2 * 1.0001f;  // HILO's FP32 distance test
   // the target relative to its cell's corner; idle lanes sit far away, so
   // no source is ever within rcut of them
-  const float t0x = live ? (float)(tx[tgt] - (ox + a * cellsz)) : 1e18f;
-  const float t0y = live ? (float)(ty[tgt] - (oy + b * cellsz)) : 1e18f;
-  const float t0z = live ? (float)(tz[tgt] - (oz + c * cellsz)) : 1e18f;
+  const double d0x = live ? tx[tgt] - (ox + a * cellsz) : 1e18;
+  const double d0y = live ? ty[tgt] - (oy + b * cellsz) : 1e18;
+  const double d0z = live ? tz[tgt] - (oz + c * cellsz) : 1e18;
+  const float t0x = (float)d0x, t0y = (float)d0y, t0z = (float)d0z;
   double hx = 0.0, hy = 0.0, hz = 0.0;
   const int c0 = max(c - 1, 0), c1 = min(c + 1, nc2 - 1);
   for (int da = max(a - 1, 0); da <= min(a + 1, nc0 - 1); ++da)
     for (int db = max(b - 1, 0); db <= min(b + 1, nc1 - 1); ++db) {
       const int row = (da * nc1 + db) * nc2;
+      // HILO: the target in this cell's frame in FP64
+      const double qx = d0x - (double)(da - a) * cellsz, qy = d0y - (double)(db - b) * cellsz;
       const float px = t0x - (float)(da - a) * csz;
       const float py = t0y - (float)(db - b) * csz;
       for (int cc = c0; cc <= c1; ++cc) {
+        const double qz = d0z - (double)(cc - c) * cellsz;
         const float pz = t0z - (float)(cc - c) * csz;
         const int s1 = cell_start[row + cc + 1];
-        float sx = 0.f, sy = 0.f, sz = 0.f;
         int s = cell_start[row + cc];
+        if (HILO) {
+          // the distance test in FP32 on the hi parts with a margin (their
+          // r^2 is within ~1e-6 of the exact one), then (hi - q) + lo and
+          // the term in FP64 for the sources some lane has inside; a term
+          // beyond rcut adds exactly zero (table clamp), so the test only
+          // decides what is computed, never what is summed
+          for (; s + SG <= s1; s += SG) {
+            float4 q[SG], ql[SG];
+#pragma unroll
+            for (int j = 0; j < SG; ++j) {
+              q[j] = near[s + j];
+              ql[j] = near_lo[s + j];
+            }
+            unsigned long long need[SG];
+#pragma unroll
+            for (int j = 0; j < SG; ++j) {
+              const float ex = q[j].x - px, ey = q[j].y - py, ez = q[j].z - pz;
+              need[j] = __ballot(fmaf(ex, ex, fmaf(ey, ey, ez * ez)) < pre2);
+            }
+#pragma unroll
+            for (int j = 0; j < SG; ++j)
+              if (need[j]) {
+                const double dx = ((double)q[j].x - qx) + (double)ql[j].x;
+                const double dy = ((double)q[j].y - qy) + (double)ql[j].y;
+                const double dz = ((double)q[j].z - qz) + (double)ql[j].z;
+                pp_term_d(q[j].w, dx, dy, dz, fma(dx, dx, fma(dy, dy, dz * dz)), tab,
+                          tab_scale, eps2, hx, hy, hz);
+              }
+          }
+          for (; s < s1; ++s) {
+            const float4 q = near[s], l = near_lo[s];
+            const double dx = ((double)q.x - qx) + (double)l.x;
+            const double dy = ((double)q.y - qy) + (double)l.y;
+            const double dz = ((double)q.z - qz) + (double)l.z;
+            const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+            if (r2 < (double)pre2)  // as the groups: every term with t < GT
+              pp_term_d(q.w, dx, dy, dz, r2, tab, tab_scale, eps2, hx, hy, hz);
+          }
+          continue;
+        }
+        float sx = 0.f, sy = 0.f, sz = 0.f;
         for (; s + SG <= s1; s += SG) {
           float4 q[SG];
 #pragma unroll
@@ -329,13 +405,13 @@ __global__ __launch_bounds__(PB) void k_pp_near(
 #pragma unroll
           for (int j = 0; j < SG; ++j)
             if (need[j])
-              pp_term(q[j].w, dx[j], dy[j], dz[j], r2[j], tab, tab_scale, eps2, sx, sy, sz);
+              pp_term(q[j].w, dx[j], dy[j], dz[j], r2[j], tab, tab_scale, eps2f, sx, sy, sz);
         }
         for (; s < s1; ++s) {
           const float4 q = near[s];
           const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
           const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          if (r2 < rcut2) pp_term(q.w, dx, dy, dz, r2, tab, tab_scale, eps2, sx, sy, sz);
+          if (r2 < rcut2) pp_term(q.w, dx, dy, dz, r2, tab, tab_scale, eps2f, sx, sy, sz);
         }
         hx += (double)sx;
         hy += (double)sy;
@@ -363,7 +439,7 @@ size_t ssim_pp_work_bytes(int nt, int ncells) {
   const size_t nc = (size_t)(ncells > 0 ? ncells : 1);
   const size_t chunks = (size_t)(nt > 0 ? nt : 1) / CHUNK + nc + 1;
   return ssim::align256(nc * sizeof(int)) + ssim::align256((nc + 1) * sizeof(int)) +
-         ssim::align256(chunks * sizeof(int)) + ssim::align256((GT + 1) * sizeof(float2)) +
+         ssim::align256(chunks * sizeof(int)) + ssim::align256((GT + 2) * sizeof(float2)) +
          ssim::scan_work_bytes((int)nc);
 }
 
@@ -389,8 +465,8 @@ int pp_plan(int nt, int ncells, const int* tcell_start, double rs, double rcut,
   const size_t max_chunks = (size_t)nt / CHUNK + ncells + 1;
   w += ssim::align256(max_chunks * sizeof(int));
   float2* gtab = reinterpret_cast<float2*>(w);
-  w += ssim::align256((GT + 1) * sizeof(float2));
-  hipLaunchKernelGGL(k_gtab, dim3(ssim::ceil_div(GT + 1, PB)), dim3(PB), 0, st, rs,
+  w += ssim::align256((GT + 2) * sizeof(float2));
+  hipLaunchKernelGGL(k_gtab, dim3(ssim::ceil_div(GT + 2, PB)), dim3(PB), 0, st, rs,
                      rcut, gtab);
   SSIM_LAUNCHED("k_gtab");
   if (ncells <= PLAN_MAX) {
@@ -462,9 +538,9 @@ int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
 
 extern "C" {
 
-int ssim_pp_near(int nt, const double* tx, const double* ty, const double* tz,
+static int pp_near_impl(int nt, const double* tx, const double* ty, const double* tz,
                  const double* origin, double cell, const int* nc, const int* cell_start,
-                 const void* near, const int* tcell_start, const int* torder, double G,
+                 const void* near, const void* near_lo, const int* tcell_start, const int* torder, double G,
                  double eps2, double rs, double rcut, double* ax, double* ay, double* az,
                  int accumulate, void* work, size_t work_bytes, void* stream) {
   SSIM_REQUIRE(nt >= 0, "nt must be >= 0");
@@ -486,15 +562,48 @@ int ssim_pp_near(int nt, const double* tx, const double* ty, const double* tz,
   SSIM_TRY(pp_plan(nt, ncells, tcell_start, rs, rcut, work, &cstart, &cmap, &gtab, st));
   const int max_chunks = nt / CHUNK + ncells + 1;
   const dim3 grid(ssim::ceil_div(max_chunks, WAVES));
-  SSIM_TIMED("k_pp_near", st,
-             hipLaunchKernelGGL(k_pp_near, grid, dim3(PB), 0, st, tx, ty, tz, origin[0],
-                                origin[1], origin[2], cell, nc[0], nc[1], nc[2],
-                                cell_start, static_cast<const float4*>(near), tcell_start,
-                                torder, cstart, cmap, ncells, (float)eps2, gtab,
-                                (float)(GT / (rcut * rcut)), (float)(rcut * rcut), G, ax,
-                                ay, az, accumulate));
+  const float4* nl4 = static_cast<const float4*>(near_lo);
+  if (near_lo) {
+    SSIM_TIMED("k_pp_near_hilo", st,
+               hipLaunchKernelGGL(k_pp_near<true>, grid, dim3(PB), 0, st, tx, ty, tz,
+                                  origin[0], origin[1], origin[2], cell, nc[0], nc[1], nc[2],
+                                  cell_start, static_cast<const float4*>(near), nl4,
+                                  tcell_start, torder, cstart, cmap, ncells, eps2,
+                                  gtab, (float)(GT / (rcut * rcut)), (float)(rcut * rcut), G,
+                                  ax, ay, az, accumulate));
+  } else {
+    SSIM_TIMED("k_pp_near", st,
+               hipLaunchKernelGGL(k_pp_near<false>, grid, dim3(PB), 0, st, tx, ty, tz,
+                                  origin[0], origin[1], origin[2], cell, nc[0], nc[1], nc[2],
+                                  cell_start, static_cast<const float4*>(near), nl4,
+                                  tcell_start, torder, cstart, cmap, ncells, eps2,
+                                  gtab, (float)(GT / (rcut * rcut)), (float)(rcut * rcut), G,
+                                  ax, ay, az, accumulate));
+  }
   SSIM_LAUNCHED("k_pp_near");
   return SSIM_OK;
+}
+
+int ssim_pp_near(int nt, const double* tx, const double* ty, const double* tz,
+                 const double* origin, double cell, const int* nc, const int* cell_start,
+                 const void* near, const int* tcell_start, const int* torder, double G,
+                 double eps2, double rs, double rcut, double* ax, double* ay, double* az,
+                 int accumulate, void* work, size_t work_bytes, void* stream) {
+  return pp_near_impl(nt, tx, ty, tz, origin, cell, nc, cell_start, near, nullptr,
+                      tcell_start, torder, G, eps2, rs, rcut, ax, ay, az, accumulate, work,
+                      work_bytes, stream);
+}
+
+int ssim_pp_near_hilo(int nt, const double* tx, const double* ty, const double* tz,
+                      const double* origin, double cell, const int* nc,
+                      const int* cell_start, const void* near, const void* near_lo,
+                      const int* tcell_start, const int* torder, double G, double eps2,
+                      double rs, double rcut, double* ax, double* ay, double* az,
+                      int accumulate, void* work, size_t work_bytes, void* stream) {
+  SSIM_REQUIRE(near_lo || nt == 0, "null near_lo");
+  return pp_near_impl(nt, tx, ty, tz, origin, cell, nc, cell_start, near, near_lo,
+                      tcell_start, torder, G, eps2, rs, rcut, ax, ay, az, accumulate, work,
+                      work_bytes, stream);
 }
 
 int ssim_pp_short(int nt, const double* tx, const double* ty, const double* tz,

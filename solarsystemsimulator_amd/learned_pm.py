@@ -220,7 +220,7 @@ class PMLearnedSystem:
         the placement, dt or the learned flag changed)."""
         torch = _torch()
         key = (self.split.placement(), float(dt), bool(learned), self.fold,
-               self.split.overlap)
+               self.split.overlap, self.split.near_hilo)
         if getattr(self, "_graph_key", None) != key:
             self._graph = None
             g = torch.cuda.CUDAGraph()

@@ -257,14 +257,19 @@ class SplitForce:
                                     device=self.dev)
             self.bwork = D.workspace(
                 _lib.load().ssim_split_bin_work_bytes(nm, self.n, self.block), self.dev)
+            # allocated with the rest so that switching near_hilo on never
+            # allocates inside a captured step
+            self.near_lo = torch.empty((max(nm, 1), 4), dtype=torch.float32,
+                                       device=self.dev)
             self._bin_n = nm
+        lo = _lib.ptr(self.near_lo) if self.near_hilo else None
         org = _lib.double_array(self.origin)
         # node bins only for the node-bin gather deposit
         bins = None if self.block == 5 and self.tiled_deposit else _lib.ptr(self.bin_start)
         _lib.call("ssim_split_bin", nm, *[_lib.ptr(c) for c in x], _lib.ptr(m), self.n,
                   P._addr(org), self.h, self.block, bins,
                   _lib.ptr(self.cell_start), _lib.ptr(self.order),
-                  _lib.ptr(self.packed), _lib.ptr(self.body), _lib.ptr(self.near),
+                  _lib.ptr(self.packed), _lib.ptr(self.body), _lib.ptr(self.near), lo,
                   _lib.ptr(self.bwork), self.bwork.numel(), D.stream(self.dev))
 
     def convolve(self, interleaved=False):
@@ -376,12 +381,15 @@ class SplitForce:
         """a += (or, accumulate=False, a =) the near-field term over the
         lists of the last bin_bodies.  TSC runs (masses binned by
         mass_bins) use the block-relative kernel (ssim_pp_near) unless
-        near_precise is False."""
+        near_precise is False, and its hi/lo form (ssim_pp_near_hilo) when
+        near_hilo is set."""
         gn, org, cell, rs, rcut = self._grid
         if self.assign == "tsc" and self.near_precise:
-            _lib.call("ssim_pp_near", x[0].numel(),
+            hl = self.near_hilo
+            _lib.call("ssim_pp_near_hilo" if hl else "ssim_pp_near", x[0].numel(),
                       *[_lib.ptr(c) for c in x], P._addr(org), cell, P._addr(gn),
                       _lib.ptr(self.cell_start), _lib.ptr(self.near),
+                      *([_lib.ptr(self.near_lo)] if hl else []),
                       _lib.ptr(self.tcell_start),
                       _lib.ptr(self.torder), self.G, self.eps2, rs, rcut,
                       *[_lib.ptr(c) for c in a], int(accumulate), _lib.ptr(self.cwork),
@@ -517,6 +525,8 @@ class SplitForce:
 
     _side = None
     near_precise = True   # TSC: the near field on block-relative coordinates
+    near_hilo = False     # TSC: positions as hi + lo FP32 pairs in the near
+                          # field (p99 force error <= 1e-6, about 1.3x its time)
     overlap = True        # TSC: near field on a second stream
     side_first = False    # enqueue the target binning before the mass binning
 

@@ -245,26 +245,33 @@ def test_split_bins_and_blocked_deposit(dev, B):
     packed = torch.empty((nbody, 4), dtype=torch.float32, device=dev)
     body = torch.empty((nbody, 4), dtype=torch.float64, device=dev)
     near = torch.empty((nbody, 4), dtype=torch.float32, device=dev)
+    near_lo = torch.empty((nbody, 4), dtype=torch.float32, device=dev)
     w = D.workspace(_lib.load().ssim_split_bin_work_bytes(nbody, n, B), dev)
     org = _lib.double_array(origin)
     _lib.call("ssim_split_bin", nbody, *[_lib.ptr(c) for c in x], _lib.ptr(md), n,
               ctypes.addressof(org), h, B, _lib.ptr(start), _lib.ptr(cstart),
               _lib.ptr(order), _lib.ptr(packed), _lib.ptr(body), _lib.ptr(near),
-              _lib.ptr(w), w.numel(), D.stream(dev))
+              _lib.ptr(near_lo), _lib.ptr(w), w.numel(), D.stream(dev))
     rs, rc, ro = p3m_np.split_bins(pos, n, origin, h, B)
     assert np.array_equal(start.cpu().numpy(), rs)
     assert np.array_equal(cstart.cpu().numpy(), rc)
     assert np.array_equal(order.cpu().numpy(), ro)
     pk, bd, nr = packed.cpu().numpy(), body.cpu().numpy(), near.cpu().numpy()
+    lo = near_lo.cpu().numpy()
     for c in range(3):
         assert np.array_equal(bd[:, c], (pos[c][ro] - origin[c]) / h)
         assert np.array_equal(pk[:, c], (pos[c][ro] - (origin[c] - 0.5 * h)).astype(np.float32))
         # relative to the corner of the mass's block (its TSC centre node // B)
         blk = np.clip(np.floor((pos[c][ro] - origin[c]) / h + 0.5), 0, n - 1).astype(int) // B
         corner = (origin[c] - 0.5 * h) + blk * (B * h)
-        assert np.array_equal(nr[:, c], (pos[c][ro] - corner).astype(np.float32))
+        rel = pos[c][ro] - corner
+        assert np.array_equal(nr[:, c], rel.astype(np.float32))
+        # hi + lo: the block-relative position to the rounding of the remainder
+        assert np.array_equal(lo[:, c], (rel - nr[:, c].astype(np.float64)).astype(np.float32))
+        assert np.abs(nr[:, c].astype(np.float64) + lo[:, c] - rel).max() <= 2.0 ** -48 * B * h
     assert np.array_equal(bd[:, 3], m[ro])
     assert np.array_equal(nr[:, 3], m[ro].astype(np.float32))
+    assert not lo[:, 3].any()
     meshes = []
     for _ in range(2):
         mb = torch.full((n, n, n), 7.0, dtype=torch.float64, device=dev)
@@ -280,7 +287,7 @@ def test_split_bins_and_blocked_deposit(dev, B):
     cstart2 = torch.empty_like(cstart)
     _lib.call("ssim_split_bin", nbody, *[_lib.ptr(c) for c in x], _lib.ptr(md), n,
               ctypes.addressof(org), h, B, None, _lib.ptr(cstart2),
-              _lib.ptr(order), _lib.ptr(packed), _lib.ptr(body), None, _lib.ptr(w),
+              _lib.ptr(order), _lib.ptr(packed), _lib.ptr(body), None, None, _lib.ptr(w),
               w.numel(), D.stream(dev))
     _, rc2, ro2 = p3m_np.split_bins(pos, n, origin, h, B, nodes=False)
     assert np.array_equal(cstart2.cpu().numpy(), rc2) and np.array_equal(rc2, rc)
@@ -337,7 +344,7 @@ def test_tiled_deposit_dense_clump(dev):
     org = _lib.double_array(origin)
     _lib.call("ssim_split_bin", nbody, *[_lib.ptr(c) for c in x], _lib.ptr(md), n,
               ctypes.addressof(org), h, B, None, _lib.ptr(cstart),
-              _lib.ptr(order), _lib.ptr(packed), _lib.ptr(body), None, _lib.ptr(w),
+              _lib.ptr(order), _lib.ptr(packed), _lib.ptr(body), None, None, _lib.ptr(w),
               w.numel(), D.stream(dev))
     ref = p3m_np.tsc_deposit(pos, m, n, origin, h)
     _tiled_deposit_checks(dev, n, origin, h, cstart, body, m, ref)
@@ -447,6 +454,69 @@ def test_near_precise_matches_oracle(dev, clump):
           f"max {err.max():.2e}; origin-relative p99 {np.percentile(err_old, 99):.2e}")
     assert np.median(err) <= 1e-6 and np.percentile(err, 99) <= 5e-6
     assert np.percentile(err, 99) * 5 <= np.percentile(err_old, 99)
+
+
+def _near_time(sf, x, a, reps=20):
+    """Average ms of one near_sum (targets already binned)."""
+    st = torch.cuda.current_stream(x[0].device)
+    sf.near_sum(x, a, accumulate=False)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        sf.near_sum(x, a, accumulate=False)
+    e1.record(st)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+@pytest.mark.parametrize("clump", [False, True])
+def test_near_hilo_matches_oracle(dev, clump):
+    """The hi/lo near field (ssim_pp_near_hilo: masses as FP32 hi + lo
+    pairs from ssim_split_bin, targets from FP64, pair arithmetic and sums
+    in FP64, the table interpolated quadratically) against the exact oracle
+    sum (p3m_np.near_field): p99 of the per-body relative error <= 1e-6,
+    median <= 1e-7, at least 5x below the plain block-relative kernel's
+    p99; bit-identical when run twice.  Prints both kernels' times on the
+    same lists (its cost)."""
+    print(f"near hilo clump={clump}", flush=True)
+    n, t = 12000, 6000
+    rng = np.random.default_rng(21)
+    pos = [rng.uniform(0.0, 1.0, n) for _ in range(3)]
+    if clump:
+        for c in range(3):
+            pos[c][:4000] = 0.5 + rng.normal(0.0, 0.01, 4000)
+    tpos = [rng.uniform(0.0, 1.0, t) for _ in range(3)]
+    m = rng.uniform(0.5, 1.5, n) / n
+    allx = [np.concatenate([pos[c], tpos[c]]) for c in range(3)]
+    sf = SplitForce(dev, mesh_n=32, eps2=1e-6, rs_cells=1.0, rcut_rs=5.0,
+                    prec=P.F32, assign="tsc", margin=4.0, freeze=True)
+    sf.near_hilo = True
+    x = [D.to_dev(c, dev) for c in allx]
+    md = D.to_dev(m, dev)
+    sf.set_mass_bound(md)
+    sf.place(x)
+    sf.bin_bodies(x, n, md)
+    rs = sf.rs_cells * sf.h
+    ref = p3m_np.near_field(pos, m, allx, 1.0, 1e-6, rs, sf.rcut_rs * rs)
+    runs = []
+    for _ in range(2):
+        a = [torch.full((n + t,), 3.0, dtype=torch.float64, device=dev) for _ in range(3)]
+        sf.near_sum(x, a, accumulate=False)
+        runs.append(np.array([c.cpu().numpy() for c in a]))
+    assert np.array_equal(runs[0], runs[1])
+    err = np.linalg.norm(runs[0] - ref, axis=0) / np.linalg.norm(ref, axis=0)
+    t_hilo = _near_time(sf, x, a)
+    sf.near_hilo = False
+    a = [torch.zeros(n + t, dtype=torch.float64, device=dev) for _ in range(3)]
+    sf.near_sum(x, a, accumulate=False)
+    plain = np.array([c.cpu().numpy() for c in a])
+    t_plain = _near_time(sf, x, a)
+    err_plain = np.linalg.norm(plain - ref, axis=0) / np.linalg.norm(ref, axis=0)
+    print(f"near hilo: median {np.median(err):.2e} p99 {np.percentile(err, 99):.2e} "
+          f"max {err.max():.2e} {t_hilo * 1e3:.1f} us; plain p99 "
+          f"{np.percentile(err_plain, 99):.2e} {t_plain * 1e3:.1f} us")
+    assert np.median(err) <= 1e-7 and np.percentile(err, 99) <= 1e-6
+    assert np.percentile(err, 99) * 5 <= np.percentile(err_plain, 99)
 
 
 def test_near_precise_accumulates(dev):
