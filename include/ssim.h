@@ -521,6 +521,24 @@ int ssim_split_bin(int n, const double* x, const double* y, const double* z,
                    int block, int* bin_start, int* cell_start, int* order, void* packed,
                    void* body, void* near, void* near_lo, void* work, size_t work_bytes,
                    void* stream);
+/* One binning of all n_all bodies (the n_mass masses first, then the
+ * tracers) on ssim_split_bin's blocks, for the split force's near field
+ * and tiled deposit: every body gets the block its TSC stencil is centred
+ * on (ssim_split_bin's rule), one stable sort orders them by block, index
+ * order inside a block.  Outputs: tcell_start[nb^3 + 1] and torder[n_all]
+ * (the target lists of ssim_pp_near, which may be any binning of the
+ * targets on this grid) and, for the masses, exactly ssim_split_bin's
+ * outputs with bin_start = NULL: cell_start[nb^3 + 1], order[n_mass],
+ * packed, body, near, near_lo (the masses lead every block's segment of
+ * torder, so their order is its restriction to indices < n_mass).  It
+ * replaces ssim_split_bin + ssim_cells_bin (two sorts) by one.  No hipMemset
+ * inside.  No reference counterpart. */
+size_t ssim_split_bin_all_work_bytes(int n_all, int grid_n, int block);
+int ssim_split_bin_all(int n_all, int n_mass, const double* x, const double* y,
+                       const double* z, const double* m, int grid_n, const double* origin,
+                       double h, int block, int* cell_start, int* order, void* packed,
+                       void* body, void* near, void* near_lo, int* tcell_start,
+                       int* torder, void* work, size_t work_bytes, void* stream);
 /* ADD (accumulate != 0) or STORE the near-field force G m d g(r) /
  * (r^2 + eps2)^(3/2), r < rcut, of the binned sources (cell_start +
  * packed, from ssim_cells_build or ssim_split_bin) to / into nt targets'

@@ -130,6 +130,9 @@ SIGNATURES = {
     "ssim_split_bin_work_bytes": (c_size_t, [c_int, c_int, c_int]),
     "ssim_split_bin": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double, c_int, _P,
                                _P, _P, _P, _P, _P, _P, _P, c_size_t, _P]),
+    "ssim_split_bin_all_work_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "ssim_split_bin_all": (c_int, [c_int, c_int, _P, _P, _P, _P, c_int, _P, c_double, c_int,
+                                   _P, _P, _P, _P, _P, _P, _P, _P, _P, c_size_t, _P]),
     "ssim_pp_work_bytes": (c_size_t, [c_int, c_int]),
     "ssim_pp_short": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P,
                               _P, c_double, c_double, c_double, c_double, _P,

@@ -146,25 +146,26 @@ int run_starts(int n, const int* key, int nbins, int* start, hipStream_t st) {
 // the near field (relative to the cell grid's origin) and double4
 // ((x - ox)/h, (y - oy)/h, (z - oz)/h, m) for the deposit (the node-unit
 // coordinates its weights start from, computed with the same operations).
-__global__ __launch_bounds__(CB) void k_split_pack(
-    int n, const int* __restrict__ order, const double* __restrict__ x,
-    const double* __restrict__ y, const double* __restrict__ z,
-    const double* __restrict__ m, double ox, double oy, double oz, double h,
-    double cx, double cy, double cz, float4* __restrict__ packed,
-    double4* __restrict__ body, const int* __restrict__ key, int key_div, int nb,
-    double cell, float4* __restrict__ near, float4* __restrict__ near_lo) {
-  const int s = blockIdx.x * CB + threadIdx.x;
-  if (s >= n) return;
-  const int p = order[s];
+// Pack the body p into slot s: float4 (x - cx, y - cy, z - cz, m) for the
+// near field on origin-relative coordinates, double4 ((x - ox)/h, (y - oy)/h,
+// (z - oz)/h, m) for the deposit (the node-unit coordinates its weights
+// start from, computed with the same operations) and, with `near`, float4
+// (x, y, z relative to the corner of block blk, m) with their FP32
+// remainders in near_lo.
+__device__ __forceinline__ void split_pack_one(
+    int s, int p, int blk, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ m, double ox, double oy,
+    double oz, double h, double cx, double cy, double cz, float4* __restrict__ packed,
+    double4* __restrict__ body, int nb, double cell, float4* __restrict__ near,
+    float4* __restrict__ near_lo) {
   const double px = x[p], py = y[p], pz = z[p], pm = m[p];
   packed[s] = make_float4((float)(px - cx), (float)(py - cy), (float)(pz - cz),
                           (float)pm);
   body[s] = make_double4((px - ox) / h, (py - oy) / h, (pz - oz) / h, pm);
   if (near) {
-    // relative to the corner of the block the body is binned in (its sorted
-    // key), so the near field's differences lose no bits to the magnitude
-    // of the coordinates
-    const int blk = key[s] / key_div;
+    // relative to the corner of the block the body is binned in, so the
+    // near field's differences lose no bits to the magnitude of the
+    // coordinates
     const int A = blk / (nb * nb), B = (blk / nb) % nb, C = blk % nb;
     const double rx = px - (cx + A * cell), ry = py - (cy + B * cell),
                  rz = pz - (cz + C * cell);
@@ -175,6 +176,61 @@ __global__ __launch_bounds__(CB) void k_split_pack(
       near_lo[s] = make_float4((float)(rx - (double)hx), (float)(ry - (double)hy),
                                (float)(rz - (double)hz), 0.0f);
   }
+}
+
+// Both packed copies in bin order (slot s holds order[s], binned under
+// key[s], whose block is key[s] / key_div).
+__global__ __launch_bounds__(CB) void k_split_pack(
+    int n, const int* __restrict__ order, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, double ox, double oy, double oz, double h,
+    double cx, double cy, double cz, float4* __restrict__ packed,
+    double4* __restrict__ body, const int* __restrict__ key, int key_div, int nb,
+    double cell, float4* __restrict__ near, float4* __restrict__ near_lo) {
+  const int s = blockIdx.x * CB + threadIdx.x;
+  if (s >= n) return;
+  split_pack_one(s, order[s], key[s] / key_div, x, y, z, m, ox, oy, oz, h, cx, cy, cz,
+                 packed, body, nb, cell, near, near_lo);
+}
+
+// All bodies binned by block (tstart, torder: index order inside a block,
+// masses = indices below n_mass first): mcount[c] = the masses of block c,
+// by a binary search for n_mass in its sorted segment.
+__global__ __launch_bounds__(CB) void k_mass_counts(int ncells, const int* __restrict__ tstart,
+                                                    const int* __restrict__ torder,
+                                                    int n_mass, int* __restrict__ mcount) {
+  const int c = blockIdx.x * CB + threadIdx.x;
+  if (c >= ncells) return;
+  const int s0 = tstart[c];
+  int lo = s0, hi = tstart[c + 1];  // first slot with an index >= n_mass
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (torder[mid] < n_mass) lo = mid + 1;
+    else hi = mid;
+  }
+  mcount[c] = lo - s0;
+}
+
+// The masses' slots from the all-body order: the mass at sorted position p
+// of block c goes to mstart[c] + (p - tstart[c]) (the masses lead every
+// block's segment), with order and both packed copies.
+__global__ __launch_bounds__(CB) void k_split_pack_all(
+    int n_all, int n_mass, const int* __restrict__ torder, const int* __restrict__ key,
+    const int* __restrict__ tstart, const int* __restrict__ mstart,
+    int* __restrict__ order, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ m, double ox, double oy,
+    double oz, double h, double cx, double cy, double cz, float4* __restrict__ packed,
+    double4* __restrict__ body, int nb, double cell, float4* __restrict__ near,
+    float4* __restrict__ near_lo) {
+  const int p = blockIdx.x * CB + threadIdx.x;
+  if (p >= n_all) return;
+  const int i = torder[p];
+  if (i >= n_mass) return;
+  const int c = key[p];
+  const int s = mstart[c] + (p - tstart[c]);
+  order[s] = i;
+  split_pack_one(s, i, c, x, y, z, m, ox, oy, oz, h, cx, cy, cz, packed, body, nb, cell,
+                 near, near_lo);
 }
 
 __global__ __launch_bounds__(CB) void k_pack(
@@ -361,6 +417,69 @@ int ssim_split_bin(int n, const double* x, const double* y, const double* z,
                        nodes ? B3 : 1, nb, block * h, static_cast<float4*>(near),
                        static_cast<float4*>(near_lo));
     SSIM_LAUNCHED("k_split_pack");
+  }
+  return SSIM_OK;
+}
+
+
+size_t ssim_split_bin_all_work_bytes(int n_all, int grid_n, int block) {
+  if (grid_n < 1 || block < 1) return 0;
+  const long long nb = (grid_n + block - 1) / block;
+  const long long ncells = nb * nb * nb;
+  if (ncells > (1LL << 30)) return 0;
+  return ssim::align256((size_t)(n_all > 0 ? n_all : 1) * sizeof(int)) +
+         ssim::align256((size_t)ncells * sizeof(int)) + ssim::scan_work_bytes((int)ncells) +
+         ssim::radix_sort_work_bytes(n_all);
+}
+
+int ssim_split_bin_all(int n_all, int n_mass, const double* x, const double* y,
+                       const double* z, const double* m, int grid_n, const double* origin,
+                       double h, int block, int* cell_start, int* order, void* packed,
+                       void* body, void* near, void* near_lo, int* tcell_start,
+                       int* torder, void* work, size_t work_bytes, void* stream) {
+  SSIM_REQUIRE(n_all >= 0 && n_mass >= 0 && n_mass <= n_all, "need 0 <= n_mass <= n_all");
+  SSIM_REQUIRE(grid_n >= 1 && block >= 1 && h > 0.0 && origin, "bad mesh");
+  const long long nbl = (grid_n + block - 1) / block;
+  SSIM_REQUIRE(nbl * nbl * nbl <= (1LL << 30), "too many blocks: %lld", nbl * nbl * nbl);
+  SSIM_REQUIRE(cell_start && order && packed && body && tcell_start && torder &&
+               (n_all == 0 || (x && y && z)) && (n_mass == 0 || m), "null pointer");
+  SSIM_REQUIRE(!near_lo || near, "near_lo needs near");
+  SSIM_REQUIRE(work && work_bytes >= ssim_split_bin_all_work_bytes(n_all, grid_n, block),
+               "workspace too small");
+  hipStream_t st = ssim::as_stream(stream);
+  const int nb = (int)nbl;
+  const int ncells = nb * nb * nb;
+  unsigned char* w = static_cast<unsigned char*>(work);
+  int* key = reinterpret_cast<int*>(w);
+  w += ssim::align256((size_t)(n_all > 0 ? n_all : 1) * sizeof(int));
+  int* mcount = reinterpret_cast<int*>(w);
+  w += ssim::align256((size_t)ncells * sizeof(int));
+  void* scan_work = w;
+  w += ssim::scan_work_bytes(ncells);
+  if (n_all > 0) {
+    hipLaunchKernelGGL(k_split_key, dim3(ssim::ceil_div(n_all, CB)), dim3(CB), 0, st, n_all,
+                       x, y, z, origin[0], origin[1], origin[2], h, grid_n, block, nb, 0,
+                       key, torder, nullptr);
+    SSIM_LAUNCHED("k_split_key");
+    int bits = 1;
+    while (bits < 31 && (1LL << bits) < ncells) ++bits;
+    SSIM_TRY(ssim::radix_sort_pairs(n_all, bits, key, torder, w,
+                                    ssim::radix_sort_work_bytes(n_all), st));
+  }
+  SSIM_TRY(run_starts(n_all, key, ncells, tcell_start, st));
+  hipLaunchKernelGGL(k_mass_counts, dim3(ssim::ceil_div(ncells, CB)), dim3(CB), 0, st, ncells,
+                     tcell_start, torder, n_mass, mcount);
+  SSIM_LAUNCHED("k_mass_counts");
+  SSIM_TRY(ssim::scan_exclusive(ncells, mcount, cell_start, scan_work, st));
+  if (n_mass > 0) {
+    const double hh = 0.5 * h;
+    hipLaunchKernelGGL(k_split_pack_all, dim3(ssim::ceil_div(n_all, CB)), dim3(CB), 0, st,
+                       n_all, n_mass, torder, key, tcell_start, cell_start, order, x, y, z,
+                       m, origin[0], origin[1], origin[2], h, origin[0] - hh,
+                       origin[1] - hh, origin[2] - hh, static_cast<float4*>(packed),
+                       static_cast<double4*>(body), nb, block * h,
+                       static_cast<float4*>(near), static_cast<float4*>(near_lo));
+    SSIM_LAUNCHED("k_split_pack_all");
   }
   return SSIM_OK;
 }

@@ -669,13 +669,16 @@ __global__ __launch_bounds__(MB) void k_mlp_features(
     o[v] = make_uint4(wd[4 * v], wd[4 * v + 1], wd[4 * v + 2], wd[4 * v + 3]);
 }
 
+#ifndef SSIM_MLP_WG_CAP
+#define SSIM_MLP_WG_CAP 1024
+#endif
 int grid_waves(int nb) {
-  // workgroups: one per 16-body batch per wave, at most 1024 (4096 waves,
-  // about one resident set on 256 CUs): every wave then loads its 88
-  // registers of weights once for several batches
+  // workgroups: one per 16-body batch per wave, at most SSIM_MLP_WG_CAP:
+  // every wave then loads its 88 registers of weights once for several
+  // batches
   const int batches = ssim::ceil_div(nb, 16);
   int wg = ssim::ceil_div(batches, WAVES);
-  return wg < 1 ? 1 : (wg > 1024 ? 1024 : wg);
+  return wg < 1 ? 1 : (wg > SSIM_MLP_WG_CAP ? SSIM_MLP_WG_CAP : wg);
 }
 
 }  // namespace

@@ -431,6 +431,18 @@ __global__ __launch_bounds__(PB) void k_pp_near(
   }
 }
 
+// The table scale s = GT / rcut^2 rounded up until rcut2 * s >= GT in
+// FP32 (rcut2 = rcut^2 in FP32): r2 >= rcut2 then gives t = min(r2 s, GT) =
+// GT exactly, whose entries are zero, so a source at or beyond rcut adds
+// exactly nothing whether or not another lane of the wave needs it, and a
+// target's sum does not depend on which targets share its wave.
+float tab_scale_for(double rcut) {
+  const float rcut2 = (float)(rcut * rcut);
+  float s = (float)(GT / (rcut * rcut));
+  while (rcut2 * s < (float)GT) s = nextafterf(s, INFINITY);
+  return s;
+}
+
 }  // namespace
 
 extern "C" {
@@ -519,7 +531,7 @@ int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
                        origin[0], origin[1], origin[2], nc[0], nc[1], nc[2],
                        cell_start, static_cast<const float4*>(packed), tcell_start,
                        torder, cstart, cmap, ncells, (float)eps2, gtab,
-                       (float)(GT / (rcut * rcut)), (float)(rcut * rcut), G, ax, ay, az,
+                       tab_scale_for(rcut), (float)(rcut * rcut), G, ax, ay, az,
                        accumulate, counts);
   else
     SSIM_TIMED("k_pp_cells", st,
@@ -528,7 +540,7 @@ int pp_impl(int nt, const double* tx, const double* ty, const double* tz,
                                   nc[2], cell_start,
                                   static_cast<const float4*>(packed), tcell_start,
                                   torder, cstart, cmap, ncells, (float)eps2, gtab,
-                                  (float)(GT / (rcut * rcut)), (float)(rcut * rcut), G, ax,
+                                  tab_scale_for(rcut), (float)(rcut * rcut), G, ax,
                                   ay, az, accumulate, nullptr));
   SSIM_LAUNCHED("k_pp_cells");
   return SSIM_OK;
@@ -569,7 +581,7 @@ static int pp_near_impl(int nt, const double* tx, const double* ty, const double
                                   origin[0], origin[1], origin[2], cell, nc[0], nc[1], nc[2],
                                   cell_start, static_cast<const float4*>(near), nl4,
                                   tcell_start, torder, cstart, cmap, ncells, eps2,
-                                  gtab, (float)(GT / (rcut * rcut)), (float)(rcut * rcut), G,
+                                  gtab, tab_scale_for(rcut), (float)(rcut * rcut), G,
                                   ax, ay, az, accumulate));
   } else {
     SSIM_TIMED("k_pp_near", st,
@@ -577,7 +589,7 @@ static int pp_near_impl(int nt, const double* tx, const double* ty, const double
                                   origin[0], origin[1], origin[2], cell, nc[0], nc[1], nc[2],
                                   cell_start, static_cast<const float4*>(near), nl4,
                                   tcell_start, torder, cstart, cmap, ncells, eps2,
-                                  gtab, (float)(GT / (rcut * rcut)), (float)(rcut * rcut), G,
+                                  gtab, tab_scale_for(rcut), (float)(rcut * rcut), G,
                                   ax, ay, az, accumulate));
   }
   SSIM_LAUNCHED("k_pp_near");

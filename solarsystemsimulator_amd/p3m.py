@@ -243,6 +243,7 @@ class SplitForce:
         import torch
         nm = n_mass
         if getattr(self, "_bin_n", -1) != nm:
+            self._all_key = None
             nb = -(-self.n // self.block)
             K = nb ** 3 * self.block ** 3
             i32 = dict(dtype=torch.int32, device=self.dev)
@@ -309,8 +310,7 @@ class SplitForce:
         rs = self.rs_cells * self.h
         rcut = self.rcut_rs * rs
         if self.assign == "tsc":
-            self.mass_bins(x, n_mass, m)
-            return self.bin_targets(x)
+            return self.tsc_bins(x, n_mass, m)
         lo = list(self.origin)
         extent = (self.n - 1) * self.h
         k = max(1, int(extent // rcut))
@@ -347,6 +347,50 @@ class SplitForce:
         self._grid = (gn, org, cell, rs, rcut)
         return self._grid
 
+    def tsc_bins(self, x, n_mass, m):
+        """TSC runs: the mass bins and the target lists, by one binning of
+        all bodies (ssim_split_bin_all) when the deposit needs no node bins,
+        else by mass_bins and bin_targets; returns the grid."""
+        if not (self.unified_bins and self.block == 5 and self.tiled_deposit):
+            self.mass_bins(x, n_mass, m)
+            return self.bin_targets(x)
+        import torch
+        nt = x[0].numel()
+        nb = -(-self.n // self.block)
+        ncells = nb ** 3
+        if getattr(self, "_all_key", None) != (n_mass, nt, ncells):
+            i32 = dict(dtype=torch.int32, device=self.dev)
+            f32 = dict(dtype=torch.float32, device=self.dev)
+            nm = max(n_mass, 1)
+            self.cell_start = torch.empty(ncells + 1, **i32)
+            self.order = torch.empty(nm, **i32)
+            self.packed = torch.empty((nm, 4), **f32)
+            self.near = torch.empty((nm, 4), **f32)
+            self.near_lo = torch.empty((nm, 4), **f32)
+            self.body = torch.empty((nm, 4), dtype=torch.float64, device=self.dev)
+            self.tcell_start = torch.empty(ncells + 1, **i32)
+            self.torder = torch.empty(max(nt, 1), **i32)
+            lib = _lib.load()
+            self.awork = D.workspace(
+                lib.ssim_split_bin_all_work_bytes(nt, self.n, self.block), self.dev)
+            self.cwork = D.workspace(lib.ssim_pp_work_bytes(nt, ncells), self.dev)
+            self._all_key = (n_mass, nt, ncells)
+            self._bin_n = -1          # mass_bins / bin_targets reallocate
+            self._nt = -1
+        org = _lib.double_array(self.origin)
+        lo = _lib.ptr(self.near_lo) if self.near_hilo else None
+        _lib.call("ssim_split_bin_all", nt, n_mass, *[_lib.ptr(c) for c in x], _lib.ptr(m),
+                  self.n, P._addr(org), self.h, self.block, _lib.ptr(self.cell_start),
+                  _lib.ptr(self.order), _lib.ptr(self.packed), _lib.ptr(self.body),
+                  _lib.ptr(self.near), lo, _lib.ptr(self.tcell_start),
+                  _lib.ptr(self.torder), _lib.ptr(self.awork), self.awork.numel(),
+                  D.stream(self.dev))
+        rs = self.rs_cells * self.h
+        gn = _lib.int_array([nb, nb, nb])
+        corner = _lib.double_array([o - 0.5 * self.h for o in self.origin])
+        self._grid = (gn, corner, self.block * self.h, rs, self.rcut_rs * rs)
+        return self._grid
+
     def bin_targets(self, x):
         """TSC runs: cell list of all targets on the grid of node blocks
         (origin o - h/2, edge block * h); returns the grid."""
@@ -360,6 +404,7 @@ class SplitForce:
         lo = [o - 0.5 * self.h for o in self.origin]
         nt = x[0].numel()
         if self._nt != nt or self._cells_cap != ncells:
+            self._all_key = None
             i32 = dict(dtype=torch.int32, device=self.dev)
             self.tcell_start = torch.empty(ncells + 1, **i32)
             self.torder = torch.empty(max(nt, 1), **i32)
@@ -434,23 +479,22 @@ class SplitForce:
         on the current stream and returns mesh_gather's `corr` dict; the
         gather then adds it.
 
-        With TSC the work runs on two streams: the side stream bins the
-        targets while the current one bins the masses (one binning
-        for the deposit and the near-field sources); then the side stream
-        adds the near field (bound by the vector ALU) and, once the mass
-        mesh is deposited, runs the correction, while the current one
-        deposits and convolves (bound by memory and latency); the near field
-        is stored into a, and the current stream waits for the side stream
-        and adds the mesh gather (in cell order).  With
-        CIC everything runs in order on one stream."""
+        With TSC the work runs on two streams: the current one bins all
+        bodies once (the deposit's mass bins, the near field's sources and
+        targets: tsc_bins); then the side stream adds the near field (bound
+        by the vector ALU) and, once the mass mesh is deposited, runs the
+        correction, while the current one deposits and convolves (bound by
+        memory and latency); the near field is stored into a, and the
+        current stream waits for the side stream and adds the mesh gather
+        (in cell order).  With CIC everything runs in order on one
+        stream."""
         import torch
         if not placed:
             self.place(x)
         if correction is not None and not self.can_fold_correction():
             raise ValueError("correction needs TSC with interleaved FP32 fields")
         if self.assign == "tsc" and not self.overlap:
-            self.bin_targets(x)
-            self.mass_bins(x, n_mass, m)
+            self.tsc_bins(x, n_mass, m)
             self.near_sum(x, a, accumulate=False)
             self.deposit(x, n_mass, m, binned=True)
             corr = correction() if correction is not None else None
@@ -480,25 +524,30 @@ class SplitForce:
         signal): stream "main" (the current one) or "side"; before a
         segment runs, its stream waits for the named events, and afterwards
         records `signal`.  The caller records "start" on main before the
-        first segment.  The mass binning comes first (it feeds both the
-        deposit and the near field; side_first puts the target binning
-        before it, the order a captured graph needs); the target binning
-        and the near field run on the side stream beside the deposit and
-        the convolution, and so does the correction once the mass mesh
-        exists; the gather waits for the side stream."""
+        first segment.  The binning comes first (one binning of all
+        bodies, tsc_bins; without unified bins the mass binning on main and
+        the target binning on the side stream, side_first putting the
+        target binning first, the order a captured graph needs); the near
+        field runs on the side stream beside the deposit and the
+        convolution, and so does the correction once the mass mesh exists;
+        the gather waits for the side stream."""
         held = {}
 
         def corr():
             held["corr"] = correction()
 
-        mass = ("main", lambda: self.mass_bins(x, n_mass, m), (), "bins")
-        targets = ("side", lambda: self.bin_targets(x), ("start",), None)
-        segs = [targets, mass] if self.side_first else [mass, targets]
+        if self.unified_bins and self.block == 5 and self.tiled_deposit:
+            segs = [("main", lambda: self.tsc_bins(x, n_mass, m), (), "bins")]
+        else:
+            mass = ("main", lambda: self.mass_bins(x, n_mass, m), (), "bins")
+            targets = ("side", lambda: self.bin_targets(x), ("start",), None)
+            segs = [targets, mass] if self.side_first else [mass, targets]
         segs += [("side", lambda: self.near_sum(x, a, accumulate=False), ("bins",), None),
                  ("main", lambda: self.deposit(x, n_mass, m, binned=True), (), "deposited")]
         if correction is not None:
             segs.append(("side", corr, ("deposited",), None))
-        last = segs[-1] if correction is not None else segs[2]
+        last = segs[-1] if correction is not None else \
+            next(sg for sg in segs if sg[0] == "side" and sg[2] == ("bins",))
         segs[segs.index(last)] = last[:3] + ("side_done",)
         segs += [("main", lambda: self.convolve(interleaved=self.f4 is not None), (), None),
                  ("main", lambda: self.mesh_gather(x, a, order=self.torder, accumulate=True,
@@ -528,6 +577,7 @@ class SplitForce:
     near_hilo = False     # TSC: positions as hi + lo FP32 pairs in the near
                           # field (p99 force error <= 1e-6, about 1.3x its time)
     overlap = True        # TSC: near field on a second stream
+    unified_bins = True   # TSC: masses and targets from one binning
     side_first = False    # enqueue the target binning before the mass binning
 
 

@@ -318,6 +318,111 @@ def _tiled_deposit_checks(dev, n, origin, h, cstart, body, m, ref):
     np.testing.assert_allclose(meshes[2], ref, rtol=1e-6, atol=1e-7 * ref.max())
 
 
+def _split_bin_call(dev, pos, m, n, origin, h, B):
+    """ssim_split_bin (blocks only) on the masses; host copies."""
+    import ctypes
+    nbody = len(m)
+    nb = -(-n // B)
+    i32 = dict(dtype=torch.int32, device=dev)
+    out = dict(cstart=torch.empty(nb ** 3 + 1, **i32), order=torch.empty(max(nbody, 1), **i32))
+    for k in ("packed", "near", "near_lo"):
+        out[k] = torch.empty((max(nbody, 1), 4), dtype=torch.float32, device=dev)
+    out["body"] = torch.empty((max(nbody, 1), 4), dtype=torch.float64, device=dev)
+    x = [D.to_dev(c, dev) for c in pos]
+    md = D.to_dev(m, dev)
+    w = D.workspace(_lib.load().ssim_split_bin_work_bytes(nbody, n, B), dev)
+    org = _lib.double_array(origin)
+    _lib.call("ssim_split_bin", nbody, *[_lib.ptr(c) for c in x], _lib.ptr(md), n,
+              ctypes.addressof(org), h, B, None, _lib.ptr(out["cstart"]),
+              _lib.ptr(out["order"]), _lib.ptr(out["packed"]), _lib.ptr(out["body"]),
+              _lib.ptr(out["near"]), _lib.ptr(out["near_lo"]), _lib.ptr(w), w.numel(),
+              D.stream(dev))
+    return {k: v.cpu().numpy()[:nbody] if k not in ("cstart",) else v.cpu().numpy()
+            for k, v in out.items()}
+
+
+@pytest.mark.parametrize("case", ["uniform", "clump", "no_tracers", "no_masses", "empty"])
+def test_split_bin_all_equals_two_binnings(dev, case):
+    """ssim_split_bin_all (one binning of masses and tracers): its target
+    lists equal the oracle's block binning of all bodies (stable, index
+    order inside a block), and its mass outputs equal ssim_split_bin's on
+    the masses alone bit for bit (starts, order, all packed copies)."""
+    print(f"split bin all {case}", flush=True)
+    import ctypes
+    n, h, B = 30, 0.04, 5
+    origin = [0.1, -0.2, 0.3]
+    rng = np.random.default_rng(31)
+    nm, nt = {"uniform": (20_000, 45_000), "clump": (20_000, 45_000),
+              "no_tracers": (7_000, 0), "no_masses": (0, 9_000), "empty": (0, 0)}[case]
+    pos = [origin[a] + rng.uniform(0.5, n - 1.5, nm + nt) * h for a in range(3)]
+    if case == "clump":
+        for a in range(3):
+            pos[a][:8000] = origin[a] + 12.3 * h + rng.normal(0.0, 0.3 * h, 8000)
+            pos[a][nm:nm + 9000] = origin[a] + 12.3 * h + rng.normal(0.0, 0.3 * h, 9000)
+    nall = nm + nt
+    k = min(300, nall)
+    pos[0][:k] = origin[0] + (np.arange(k) % 25 + 0.5) * h            # on bin edges
+    m = rng.uniform(0.5, 2.0, nm)
+    x = [D.to_dev(c, dev) for c in pos]
+    md = D.to_dev(m, dev)
+    nb = -(-n // B)
+    i32 = dict(dtype=torch.int32, device=dev)
+    cstart = torch.empty(nb ** 3 + 1, **i32)
+    tstart = torch.empty(nb ** 3 + 1, **i32)
+    order = torch.empty(max(nm, 1), **i32)
+    torder = torch.empty(max(nall, 1), **i32)
+    bufs = {k: torch.empty((max(nm, 1), 4), dtype=torch.float32, device=dev)
+            for k in ("packed", "near", "near_lo")}
+    body = torch.empty((max(nm, 1), 4), dtype=torch.float64, device=dev)
+    w = D.workspace(_lib.load().ssim_split_bin_all_work_bytes(nall, n, B), dev)
+    org = _lib.double_array(origin)
+    _lib.call("ssim_split_bin_all", nall, nm, *[_lib.ptr(c) for c in x], _lib.ptr(md), n,
+              ctypes.addressof(org), h, B, _lib.ptr(cstart), _lib.ptr(order),
+              _lib.ptr(bufs["packed"]), _lib.ptr(body), _lib.ptr(bufs["near"]),
+              _lib.ptr(bufs["near_lo"]), _lib.ptr(tstart), _lib.ptr(torder), _lib.ptr(w),
+              w.numel(), D.stream(dev))
+    _, rc, ro = p3m_np.split_bins(pos, n, origin, h, B, nodes=False)
+    assert np.array_equal(tstart.cpu().numpy(), rc)
+    assert np.array_equal(torder.cpu().numpy()[:nall], ro)
+    ref = _split_bin_call(dev, [c[:nm] for c in pos], m, n, origin, h, B)
+    assert np.array_equal(cstart.cpu().numpy(), ref["cstart"])
+    assert np.array_equal(order.cpu().numpy()[:nm], ref["order"])
+    assert np.array_equal(body.cpu().numpy()[:nm], ref["body"])
+    for k in ("packed", "near", "near_lo"):
+        assert np.array_equal(bufs[k].cpu().numpy()[:nm], ref[k])
+    if case == "clump":
+        assert np.diff(rc).max() > 8000                   # one dense block
+
+
+def test_unified_bins_same_forces(dev):
+    """The split force (TSC, overlapped streams, FP32 mesh) with the unified
+    binning is bit-identical to the one with the two separate binnings: the
+    mass bins are the same, and each target's near-field sum and gather do
+    not depend on how the targets are grouped."""
+    print("unified bins forces", flush=True)
+    rng = np.random.default_rng(33)
+    n, t = 20_000, 30_000
+    pos = [rng.uniform(0.0, 1.0, n) for _ in range(3)]
+    for c in range(3):
+        pos[c][:3000] = 0.4 + rng.normal(0.0, 0.02, 3000)
+    tpos = [rng.uniform(0.0, 1.0, t) for _ in range(3)]
+    m = rng.uniform(0.5, 1.5, n) / n
+    allx = [np.concatenate([pos[c], tpos[c]]) for c in range(3)]
+    out = []
+    for unified in (True, False, True):
+        sf = SplitForce(dev, mesh_n=64, eps2=1e-6, rs_cells=1.0, rcut_rs=5.0,
+                        prec=P.F32, assign="tsc", margin=4.0, freeze=True)
+        sf.unified_bins = unified
+        x = [D.to_dev(c, dev) for c in allx]
+        md = D.to_dev(m, dev)
+        sf.set_mass_bound(md)
+        a = [torch.zeros(n + t, dtype=torch.float64, device=dev) for _ in range(3)]
+        sf.accelerations(x, n, md, a)
+        out.append(np.array([c.cpu().numpy() for c in a]))
+    assert np.array_equal(out[0], out[2])
+    assert np.array_equal(out[0], out[1])
+
+
 def test_tiled_deposit_dense_clump(dev):
     """ssim_tsc_deposit_tiles with 30000 of 40000 masses inside one node
     (the LDS list of a tile fills and is flushed many times) and masses on
