@@ -170,6 +170,8 @@ def algorithmic_bytes(n, t, n_mesh, mesh_bytes=4):
     passes, x3 = conv_bytes(n_mesh, mesh_bytes)
     return {
         "k_pp_near": nb * (24 + 24 + 4) + n * 16,
+        # the filtered near field reads the masses' hi and lo float4 copies
+        "k_pp_bits": nb * (24 + 24 + 4) + n * 32,
         "k_fft_pass": sum(passes) / len(passes),
         "k_conv_x3": x3,
         "k_tsc_deposit": n * (3 * 8 + 8) + g3 * mesh_bytes,
@@ -180,7 +182,8 @@ def algorithmic_bytes(n, t, n_mesh, mesh_bytes=4):
 
 
 # rocprofv3 kernel names of the bench's timed kernels (PMC passes)
-PMC_KERNELS = {"k_pp_near": "k_pp_near", "k_fft_pass": "k_line256",
+PMC_KERNELS = {"k_pp_bits": "k_pp_bits<false", "k_pp_near": "k_pp_near",
+               "k_fft_pass": "k_line256",
                "k_conv_x3": "k_conv_x3", "k_tsc_deposit": "k_tsc_deposit_tiles",
                "k_tsc_gather": "k_tsc_gather4<true", "k_mlp_forward": "k_mlp_infer",
                "k_mlp_features": "k_mlp_features"}
@@ -234,8 +237,9 @@ def pmc_summary(fetch_dir, write_dir, out):
 
 
 def near_hilo_cost(lib, fast, dt, args, direct, nb, ms_step, kernels):
-    """The same run with the near field's hi/lo form (ssim_pp_near_hilo:
-    FP64 pair arithmetic, p99 error of the near field ~3e-7 against ~3e-6):
+    """The same run with the near field's precise form (ssim_pp_near_bits
+    with FP64 positions (hi + lo) and pair terms: p99 error of the near
+    field ~2e-7 against ~2e-6):
     its step time in the timed run's launch mode, its kernel's duration
     beside the plain kernel's, and the method's force error with it."""
     from solarsystemsimulator_amd.learned_pm import force_errors
@@ -253,7 +257,7 @@ def near_hilo_cost(lib, fast, dt, args, direct, nb, ms_step, kernels):
         ms = (time.perf_counter() - t0) / args.steps * 1e3
         lib.ssim_timing(1)
         fast.run(dt, args.profile_steps, graph=False)
-        tot, cnt = kernel_times(lib, ["k_pp_near_hilo"])["k_pp_near_hilo"]
+        tot, cnt = kernel_times(lib, ["k_pp_bits_precise"])["k_pp_bits_precise"]
         lib.ssim_timing(0)
         for c in range(3):
             direct.x[c][:nb].copy_(fast.x[c])
@@ -262,11 +266,11 @@ def near_hilo_cost(lib, fast, dt, args, direct, nb, ms_step, kernels):
     finally:
         sf.near_hilo = False
     fast.force(learned=True)
-    plain = kernels.get("k_pp_near", {}).get("avg_us")
+    plain = kernels.get("k_pp_bits", {}).get("avg_us")
     us = tot / max(cnt, 1) * 1e3
     log(f"near field hi/lo: {ms:.3f} ms/step ({ms / ms_step:.2f}x), kernel "
         f"{us:.1f} us against {plain or 0:.1f} us, median force error {np.median(err):.3e}")
-    return {"ms_per_step": ms, "step_ratio": ms / ms_step, "kernel": "k_pp_near_hilo",
+    return {"ms_per_step": ms, "step_ratio": ms / ms_step, "kernel": "k_pp_bits_precise",
             "kernel_us": us, "kernel_us_plain": plain,
             "force_err_median": float(np.median(err)),
             "near_field_p99_err": "<= 1e-6 (tests/test_p3m_gpu.py)"}
@@ -445,7 +449,7 @@ def main():
     sf = fast.split
     sf.bin_bodies(fast.x, n, fast.m)
     tested, inside = sf.near_pair_counts(fast.x)
-    names = ["k_pp_near", "k_pp_cells", "k_fft_pass", "k_conv_x3", "k_tsc_deposit",
+    names = ["k_pp_bits", "k_pp_near", "k_pp_cells", "k_fft_pass", "k_conv_x3", "k_tsc_deposit",
              "k_tsc_gather", "k_cic_deposit", "k_cic_gather", "k_mlp_forward",
              "k_mlp_features"]
     # HIP events per launch over eager steps in the timed run's stream
@@ -461,6 +465,10 @@ def main():
     mlp_flops = 2 * nb * (64 * 32 + 2 * 64 * 64 + 16 * 64)
     alg = algorithmic_bytes(n, t, n_mesh)
     work = {
+        "k_pp_bits": ("valu", FLOPS_PER_PAIR * inside, FP32_PEAK_TFLOPS,
+                      "TFLOP/s", f"{FLOPS_PER_PAIR} flop x {inside:.4g} "
+                      f"pairs inside rcut ({tested:.4g} pairs through the FP16 "
+                      f"MFMA filter)"),
         "k_pp_near": ("valu", FLOPS_PER_PAIR * inside, FP32_PEAK_TFLOPS,
                       "TFLOP/s", f"{FLOPS_PER_PAIR} flop x {inside:.4g} "
                       f"pairs inside rcut ({tested:.4g} pairs tested)"),

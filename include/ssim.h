@@ -581,6 +581,28 @@ int ssim_pp_near_hilo(int nt, const double* tx, const double* ty, const double* 
                       const int* tcell_start, const int* torder, double G, double eps2,
                       double rs, double rcut, double* ax, double* ay, double* az,
                       int accumulate, void* work, size_t work_bytes, void* stream);
+/* The near field of ssim_pp_near / ssim_pp_near_hilo (same inputs, same
+ * STORE / ADD semantics; near_lo may be NULL: lo = 0) as a filter and a
+ * sparse sum: per chunk of <= 64 targets, the 27 cells' masses are staged
+ * in LDS relative to the target cell's corner in FP64 (hi + lo + cell
+ * offset), in batches of <= 256 that take every B-th mass of each row; an
+ * FP16 MFMA evaluates r^2 - thr for 64 x 32 pairs at a time and packs the
+ * signs into per-target bit masks (thr = (rcut/cell)^2 + 0.02 cell^2, more
+ * than 4x the FP16 error bound, so no pair inside rcut is missed); each
+ * lane then sums the terms of its own set bits only, in FP64 positions and
+ * sums.  precise != 0: quadratic table interpolation and an FP64-refined
+ * (r^2 + eps2)^(-1/2), as ssim_pp_near_hilo; 0: linear interpolation and
+ * the FP32 v_rsq.  Deterministic; a target's result depends only on its
+ * position and its cell's masses.  Replaces the reference's tile loop over
+ * all sources (backend/selfgrav_kernel.py:231-259) inside rcut.  New: the
+ * reference has no split force. */
+int ssim_pp_near_bits(int nt, const double* tx, const double* ty, const double* tz,
+                      const double* origin, double cell, const int* nc,
+                      const int* cell_start, const void* near, const void* near_lo,
+                      const int* tcell_start, const int* torder, double G, double eps2,
+                      double rs, double rcut, double* ax, double* ay, double* az,
+                      int accumulate, int precise, void* work, size_t work_bytes,
+                      void* stream);
 /* Work accounting of ssim_pp_short with the same arguments: counts[0] =
  * source-target pairs tested, counts[1] = pairs closer than rcut (a body
  * with itself excluded); counts is a device array of 2, no forces. */

@@ -143,6 +143,9 @@ SIGNATURES = {
     "ssim_pp_near_hilo": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P, _P,
                                   _P, c_double, c_double, c_double, c_double, _P, _P, _P,
                                   c_int, _P, c_size_t, _P]),
+    "ssim_pp_near_bits": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P, _P,
+                                  _P, c_double, c_double, c_double, c_double, _P, _P, _P,
+                                  c_int, c_int, _P, c_size_t, _P]),
     "ssim_pp_count": (c_int, [c_int, _P, _P, _P, _P, c_double, _P, _P, _P, _P,
                               _P, c_double, c_double, c_double, _P, _P, c_size_t,
                               _P]),

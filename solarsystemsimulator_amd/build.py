@@ -31,9 +31,16 @@ CFLAGS = [
 ]
 
 
+# Per-file code generation options.  pp.hip: MFMA results in VGPRs rather
+# than AGPRs, so the near field's filter reads its r^2 tiles without
+# v_accvgpr_read copies.
+FILE_FLAGS = {"pp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [HIPCC, "-x", "hip", "-c", str(src), "-o", str(obj), *CFLAGS]
+    cmd = [HIPCC, "-x", "hip", "-c", str(src), "-o", str(obj), *CFLAGS,
+           *FILE_FLAGS.get(src.name, [])]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stdout}\n{r.stderr}")

@@ -263,7 +263,7 @@ class SplitForce:
             self.near_lo = torch.empty((max(nm, 1), 4), dtype=torch.float32,
                                        device=self.dev)
             self._bin_n = nm
-        lo = _lib.ptr(self.near_lo) if self.near_hilo else None
+        lo = _lib.ptr(self.near_lo) if (self.near_hilo or self.near_bits) else None
         org = _lib.double_array(self.origin)
         # node bins only for the node-bin gather deposit
         bins = None if self.block == 5 and self.tiled_deposit else _lib.ptr(self.bin_start)
@@ -378,7 +378,7 @@ class SplitForce:
             self._bin_n = -1          # mass_bins / bin_targets reallocate
             self._nt = -1
         org = _lib.double_array(self.origin)
-        lo = _lib.ptr(self.near_lo) if self.near_hilo else None
+        lo = _lib.ptr(self.near_lo) if (self.near_hilo or self.near_bits) else None
         _lib.call("ssim_split_bin_all", nt, n_mass, *[_lib.ptr(c) for c in x], _lib.ptr(m),
                   self.n, P._addr(org), self.h, self.block, _lib.ptr(self.cell_start),
                   _lib.ptr(self.order), _lib.ptr(self.packed), _lib.ptr(self.body),
@@ -429,6 +429,15 @@ class SplitForce:
         near_precise is False, and its hi/lo form (ssim_pp_near_hilo) when
         near_hilo is set."""
         gn, org, cell, rs, rcut = self._grid
+        if self.assign == "tsc" and self.near_precise and self.near_bits:
+            _lib.call("ssim_pp_near_bits", x[0].numel(),
+                      *[_lib.ptr(c) for c in x], P._addr(org), cell, P._addr(gn),
+                      _lib.ptr(self.cell_start), _lib.ptr(self.near),
+                      _lib.ptr(self.near_lo), _lib.ptr(self.tcell_start),
+                      _lib.ptr(self.torder), self.G, self.eps2, rs, rcut,
+                      *[_lib.ptr(c) for c in a], int(accumulate), int(self.near_hilo),
+                      _lib.ptr(self.cwork), self.cwork.numel(), D.stream(self.dev))
+            return a
         if self.assign == "tsc" and self.near_precise:
             hl = self.near_hilo
             _lib.call("ssim_pp_near_hilo" if hl else "ssim_pp_near", x[0].numel(),
@@ -574,8 +583,12 @@ class SplitForce:
 
     _side = None
     near_precise = True   # TSC: the near field on block-relative coordinates
-    near_hilo = False     # TSC: positions as hi + lo FP32 pairs in the near
-                          # field (p99 force error <= 1e-6, about 1.3x its time)
+    near_bits = True      # TSC: the near field as MFMA filter + sparse sum
+                          # (ssim_pp_near_bits); False: the cell walk
+                          # (ssim_pp_near / ssim_pp_near_hilo)
+    near_hilo = False     # TSC: FP64 positions (hi + lo) and pair terms in
+                          # the near field (p99 force error <= 1e-6, about
+                          # 1.35x its time)
     overlap = True        # TSC: near field on a second stream
     unified_bins = True   # TSC: masses and targets from one binning
     side_first = False    # enqueue the target binning before the mass binning

@@ -540,6 +540,7 @@ def test_near_precise_matches_oracle(dev, clump):
     tpos = [rng.uniform(0.0, 1.0, t) for _ in range(3)]
     m = rng.uniform(0.5, 1.5, n) / n
     sf, x, allx = _tsc_split(dev, pos, m, tpos)
+    sf.near_bits = False          # the cell walk (ssim_pp_near)
     rs = sf.rs_cells * sf.h
     ref = p3m_np.near_field(pos, m, allx, 1.0, 1e-6, rs, sf.rcut_rs * rs)
     runs = []
@@ -595,7 +596,7 @@ def test_near_hilo_matches_oracle(dev, clump):
     allx = [np.concatenate([pos[c], tpos[c]]) for c in range(3)]
     sf = SplitForce(dev, mesh_n=32, eps2=1e-6, rs_cells=1.0, rcut_rs=5.0,
                     prec=P.F32, assign="tsc", margin=4.0, freeze=True)
-    sf.near_hilo = True
+    sf.near_hilo, sf.near_bits = True, False   # the cell walk's hi/lo kernel
     x = [D.to_dev(c, dev) for c in allx]
     md = D.to_dev(m, dev)
     sf.set_mass_bound(md)
@@ -624,6 +625,78 @@ def test_near_hilo_matches_oracle(dev, clump):
     assert np.percentile(err, 99) * 5 <= np.percentile(err_plain, 99)
 
 
+@pytest.mark.parametrize("clump", [False, True])
+@pytest.mark.parametrize("precise", [False, True])
+def test_near_bits_matches_oracle(dev, clump, precise):
+    """The filtered near field (ssim_pp_near_bits: FP16 MFMA distance
+    filter, per-lane sums over the set bits, FP64 positions) against the
+    exact oracle sum (p3m_np.near_field), per-body relative error: FP64
+    positions and terms (precise) p99 <= 1e-6 and median <= 1e-7; FP32
+    positions relative to the cell and FP32 terms the plain kernel's p99 <=
+    5e-6 and median <= 1e-6; bit-identical when run twice.  The clumped case puts 4000 masses in a ball of sigma 0.01
+    (several batches per chunk).  Prints the kernels' times."""
+    print(f"near bits clump={clump} precise={precise}", flush=True)
+    n, t = 12000, 6000
+    rng = np.random.default_rng(23)
+    pos = [rng.uniform(0.0, 1.0, n) for _ in range(3)]
+    if clump:
+        for c in range(3):
+            pos[c][:4000] = 0.5 + rng.normal(0.0, 0.01, 4000)
+    tpos = [rng.uniform(0.0, 1.0, t) for _ in range(3)]
+    m = rng.uniform(0.5, 1.5, n) / n
+    allx = [np.concatenate([pos[c], tpos[c]]) for c in range(3)]
+    sf = SplitForce(dev, mesh_n=32, eps2=1e-6, rs_cells=1.0, rcut_rs=5.0,
+                    prec=P.F32, assign="tsc", margin=4.0, freeze=True)
+    sf.near_bits, sf.near_hilo = True, precise
+    x = [D.to_dev(c, dev) for c in allx]
+    md = D.to_dev(m, dev)
+    sf.set_mass_bound(md)
+    sf.place(x)
+    sf.bin_bodies(x, n, md)
+    rs = sf.rs_cells * sf.h
+    ref = p3m_np.near_field(pos, m, allx, 1.0, 1e-6, rs, sf.rcut_rs * rs)
+    runs = []
+    for _ in range(2):
+        a = [torch.full((n + t,), 3.0, dtype=torch.float64, device=dev) for _ in range(3)]
+        sf.near_sum(x, a, accumulate=False)
+        runs.append(np.array([c.cpu().numpy() for c in a]))
+    assert np.array_equal(runs[0], runs[1])
+    err = np.linalg.norm(runs[0] - ref, axis=0) / np.linalg.norm(ref, axis=0)
+    t_bits = _near_time(sf, x, a)
+    sf.near_bits, sf.near_hilo = False, True
+    t_hilo = _near_time(sf, x, a)
+    sf.near_hilo = False
+    t_plain = _near_time(sf, x, a)
+    print(f"near bits: median {np.median(err):.2e} p99 {np.percentile(err, 99):.2e} "
+          f"max {err.max():.2e} {t_bits * 1e3:.1f} us; hilo {t_hilo * 1e3:.1f} us, "
+          f"plain {t_plain * 1e3:.1f} us")
+    if precise:
+        assert np.median(err) <= 1e-7 and np.percentile(err, 99) <= 1e-6
+    else:
+        # FP32 positions relative to the cell: the plain kernel's bounds
+        assert np.median(err) <= 1e-6 and np.percentile(err, 99) <= 5e-6
+
+
+def test_near_bits_accumulates(dev):
+    """ssim_pp_near_bits with accumulate = 1 adds its sum to what is there,
+    the same sum as a STORE."""
+    rng = np.random.default_rng(24)
+    n, t = 3000, 1000
+    pos = [rng.uniform(0.0, 1.0, n) for _ in range(3)]
+    tpos = [rng.uniform(0.0, 1.0, t) for _ in range(3)]
+    m = np.full(n, 1.0 / n)
+    sf, x, _ = _tsc_split(dev, pos, m, tpos)
+    sf.near_bits = True
+    sf.bin_bodies(x, n, D.to_dev(m, dev))
+    a0 = [torch.zeros(n + t, dtype=torch.float64, device=dev) for _ in range(3)]
+    sf.near_sum(x, a0, accumulate=False)
+    a1 = [torch.full((n + t,), 0.25, dtype=torch.float64, device=dev) for _ in range(3)]
+    sf.near_sum(x, a1, accumulate=True)
+    for c in range(3):
+        assert torch.equal(a1[c], a0[c] + 0.25)
+    assert float(a0[0].abs().max()) > 0.0
+
+
 def test_near_precise_accumulates(dev):
     """accumulate = 1 adds the near field to what is there."""
     rng = np.random.default_rng(22)
@@ -632,6 +705,7 @@ def test_near_precise_accumulates(dev):
     tpos = [rng.uniform(0.0, 1.0, t) for _ in range(3)]
     m = np.full(n, 1.0 / n)
     sf, x, _ = _tsc_split(dev, pos, m, tpos)
+    sf.near_bits = False          # the cell walk (ssim_pp_near)
     a0 = [torch.zeros(n + t, dtype=torch.float64, device=dev) for _ in range(3)]
     sf.near_sum(x, a0, accumulate=False)
     a1 = [torch.full((n + t,), 0.25, dtype=torch.float64, device=dev) for _ in range(3)]
