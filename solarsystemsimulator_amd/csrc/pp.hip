@@ -73,76 +73,57 @@ __global__ __launch_bounds__(PB) void k_gtab_s(double rs, double rcut, double ep
   tab[i] = make_float2((float)g0, (float)(g1 - g0));
 }
 
-// The chunk plan in one workgroup (ncells <= PLAN_MAX): chunk counts
-// ceil(targets / CHUNK) per cell, their exclusive scan (cstart, cstart[ncells]
-// = total), and the chunk -> cell map, in rounds of 8 tiles of 4 x 1024 cells
-// (every tile's loads at once; wave shuffles; a carry per round).
-constexpr int PLAN_T = 1024, PLAN_TILES = 8, PLAN_MAX = 1 << 16;
+// The chunk plan by two fully parallel launches (ncells <= 1024 x 1024):
+// k_chunk_sum writes each cell's chunk count and each 256-thread
+// workgroup's total over its 1024 cells, k_chunk_scan adds the totals of
+// the workgroups before its own to its scan and writes cstart and the chunk
+// -> cell map.  Same plan as k_chunk_plan.
+constexpr int CP_T = 256, CP_BLOCKS = 1024;
 
-__global__ __launch_bounds__(PLAN_T) void k_chunk_plan(int ncells,
-                                                      const int* __restrict__ tstart,
-                                                      int* __restrict__ cstart,
-                                                      int* __restrict__ map) {
-  constexpr int NW = PLAN_T / 64;
-  __shared__ int wsum[PLAN_TILES * NW];
-  __shared__ int woff[PLAN_TILES * NW];
-  __shared__ int rtotal;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  int carry = 0;
-  for (int base = 0; base < ncells; base += PLAN_TILES * 4 * PLAN_T) {
-    int v[PLAN_TILES][4], x[PLAN_TILES], s[PLAN_TILES];
+__global__ __launch_bounds__(CP_T) void k_chunk_sum(int ncells,
+                                                    const int* __restrict__ tstart,
+                                                    int* __restrict__ nchunk,
+                                                    int* __restrict__ bsum) {
+  __shared__ int red[4];
+  const int c0 = (blockIdx.x * CP_T + threadIdx.x) * 4;
+  int v = 0;
 #pragma unroll
-    for (int k = 0; k < PLAN_TILES; ++k) {
-      const int i0 = base + k * 4 * PLAN_T + 4 * t;
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        v[k][e] = i0 + e < ncells
-                      ? (tstart[i0 + e + 1] - tstart[i0 + e] + CHUNK - 1) / CHUNK
-                      : 0;
+  for (int e = 0; e < 4; ++e)
+    if (c0 + e < ncells) {
+      const int k = (tstart[c0 + e + 1] - tstart[c0 + e] + CHUNK - 1) / CHUNK;
+      nchunk[c0 + e] = k;
+      v += k;
     }
+  int tot;
+  ssim_block256_excl(v, red, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(CP_T) void k_chunk_scan(int ncells,
+                                                     const int* __restrict__ nchunk,
+                                                     const int* __restrict__ bsum,
+                                                     int* __restrict__ cstart,
+                                                     int* __restrict__ map) {
+  __shared__ int red[4];
+  const int off = ssim_block256_offset(bsum, blockIdx.x, red);
+  const int c0 = (blockIdx.x * CP_T + threadIdx.x) * 4;
+  int v[4], s = 0;
 #pragma unroll
-    for (int k = 0; k < PLAN_TILES; ++k) {
-      s[k] = v[k][0] + v[k][1] + v[k][2] + v[k][3];
-      int y = s[k];
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int z = __shfl_up(y, off, 64);
-        if (lane >= off) y += z;
-      }
-      x[k] = y;
-      if (lane == 63) wsum[k * NW + w] = y;
-    }
-    __syncthreads();
-    if (w == 0) {
-      const int a = wsum[2 * lane], b = wsum[2 * lane + 1];
-      int y = a + b;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const int z = __shfl_up(y, off, 64);
-        if (lane >= off) y += z;
-      }
-      woff[2 * lane] = y - a - b;
-      woff[2 * lane + 1] = y - b;
-      if (lane == 63) rtotal = y;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < PLAN_TILES; ++k) {
-      const int i0 = base + k * 4 * PLAN_T + 4 * t;
-      int e = carry + woff[k * NW + w] + x[k] - s[k];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (i0 + q < ncells) {
-          cstart[i0 + q] = e;
-          for (int c = 0; c < v[k][q]; ++c) map[e + c] = i0 + q;
-        }
-        e += v[k][q];
-      }
-    }
-    carry += rtotal;
-    __syncthreads();  // wsum, woff and rtotal are rewritten by the next round
+  for (int e = 0; e < 4; ++e) {
+    v[e] = c0 + e < ncells ? nchunk[c0 + e] : 0;
+    s += v[e];
   }
-  if (t == 0) cstart[ncells] = carry;
+  int tot;
+  int run = off + ssim_block256_excl(s, red, &tot);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    if (c0 + e < ncells) {
+      cstart[c0 + e] = run;
+      for (int k = 0; k < v[e]; ++k) map[run + k] = c0 + e;
+    }
+    run += v[e];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) cstart[ncells] = off + tot;
 }
 
 __global__ __launch_bounds__(PB) void k_chunk_count(int ncells,
@@ -853,10 +834,16 @@ int pp_plan(int nt, int ncells, const int* tcell_start, double rs, double rcut,
                        rcut, gt, gtab);
     SSIM_LAUNCHED("k_gtab");
   }
-  if (ncells <= PLAN_MAX) {
-    hipLaunchKernelGGL(k_chunk_plan, dim3(1), dim3(PLAN_T), 0, st, ncells, tcell_start,
+  const int cblocks = ssim::ceil_div(ncells, 4 * CP_T);
+  if (cblocks <= CP_BLOCKS) {
+    // w: the scan workspace, >= ceil(ncells / 1024) + 1 ints
+    int* bsum = reinterpret_cast<int*>(w);
+    hipLaunchKernelGGL(k_chunk_sum, dim3(cblocks), dim3(CP_T), 0, st, ncells, tcell_start,
+                       nchunk, bsum);
+    SSIM_LAUNCHED("k_chunk_sum");
+    hipLaunchKernelGGL(k_chunk_scan, dim3(cblocks), dim3(CP_T), 0, st, ncells, nchunk, bsum,
                        cstart, cmap);
-    SSIM_LAUNCHED("k_chunk_plan");
+    SSIM_LAUNCHED("k_chunk_scan");
   } else {
     const dim3 gc(ssim::ceil_div(ncells, PB));
     hipLaunchKernelGGL(k_chunk_count, gc, dim3(PB), 0, st, ncells, tcell_start, nchunk);

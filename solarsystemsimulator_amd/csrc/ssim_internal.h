@@ -172,6 +172,41 @@ __host__ __device__ inline int ssim_blocked_key(int i, int j, int k, int B, int 
          ((i - bi * B) * B + (j - bj * B)) * B + (k - bk * B);
 }
 
+// Exclusive prefix of v over a 256-thread workgroup (wave shuffles and one
+// LDS exchange of the 4 wave totals, `red` = 4 ints of LDS); *total = the
+// workgroup's sum.  Every thread of the workgroup must call it.
+__device__ __forceinline__ int ssim_block256_excl(int v, int* red, int* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int y = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int z = __shfl_up(y, off, 64);
+    if (lane >= off) y += z;
+  }
+  if (lane == 63) red[w] = y;
+  __syncthreads();
+  int pre = 0, tot = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = red[q];
+    pre += q < w ? r : 0;
+    tot += r;
+  }
+  *total = tot;
+  __syncthreads();  // red is free again
+  return pre + y - v;
+}
+
+// The offset of workgroup g among ng workgroup totals bsum[0..ng) (the sum
+// of those before it), by all 256 threads; `red` = 4 ints of LDS.
+__device__ __forceinline__ int ssim_block256_offset(const int* bsum, int g, int* red) {
+  int v = 0;
+  for (int i = threadIdx.x; i < g; i += 256) v += bsum[i];
+  int tot;
+  ssim_block256_excl(v, red, &tot);
+  return tot;
+}
+
 // Knuth TwoSum: s + err == a + b exactly (6 flops, no ordering needed).
 template <typename F>
 __device__ __forceinline__ void ssim_two_sum(F a, F b, F& s, F& err) {
