@@ -123,12 +123,21 @@ class PMLearnedSystem:
 
     def _correction(self):
         """Features and network outputs (no update) for the gather."""
+        self._correction_features()
+        return self._correction_outputs()
+
+    def _correction_features(self):
+        """The feature rows (read the deposited mass mesh)."""
+        self.features()
+
+    def _correction_outputs(self):
+        """The network's outputs for the feature rows, as the gather's
+        `corr` dict."""
         torch = _torch()
-        feat = self.features()
         if getattr(self, "_corr_out", None) is None:
             self._corr_out = torch.empty((self.n + self.t, 3), dtype=torch.float32,
                                          device=self.dev)
-        self.net.predict(feat, out=self._corr_out)
+        self.net.predict(self.feat, out=self._corr_out)
         return dict(out=self._corr_out, scale=self.out_unit(),
                     relative=self.net.relative, vel=self.v, n_proj=self.n)
 
@@ -137,8 +146,10 @@ class PMLearnedSystem:
         `placed`: the mesh is placed for them already (run's steps)."""
         fold = (learned and self.net is not None and self.fold and
                 self.split.can_fold_correction())
+        # features behind the deposit, the network behind the near field
+        corr = (self._correction_features, self._correction_outputs) if fold else None
         self.split.accelerations(self.x, self.n, self.m, self.a, placed=placed,
-                                 correction=self._correction if fold else None)
+                                 correction=corr)
         if learned and self.net is not None and not fold:
             feat = self.features()
             # masses: no component along the velocity (no work done on

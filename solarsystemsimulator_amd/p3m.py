@@ -486,7 +486,10 @@ class SplitForce:
         `correction` (can_fold_correction() only): a callable, launched
         once the mass mesh is deposited, that enqueues a learned correction
         on the current stream and returns mesh_gather's `corr` dict; the
-        gather then adds it.
+        gather then adds it.  A pair (prepare, finish) splits it: prepare
+        (the features, which read the mass mesh) runs right after the
+        deposit on the current stream, finish (the network) after it on the
+        side stream behind the near field and returns the dict.
 
         With TSC the work runs on two streams: the current one bins all
         bodies once (the deposit's mass bins, the near field's sources and
@@ -506,7 +509,11 @@ class SplitForce:
             self.tsc_bins(x, n_mass, m)
             self.near_sum(x, a, accumulate=False)
             self.deposit(x, n_mass, m, binned=True)
-            corr = correction() if correction is not None else None
+            if isinstance(correction, tuple):
+                correction[0]()
+                corr = correction[1]()
+            else:
+                corr = correction() if correction is not None else None
             self.convolve(interleaved=self.f4 is not None)
             self.mesh_gather(x, a, order=self.torder, accumulate=True, corr=corr)
             return a
@@ -542,8 +549,10 @@ class SplitForce:
         the gather waits for the side stream."""
         held = {}
 
+        split = isinstance(correction, tuple)
+
         def corr():
-            held["corr"] = correction()
+            held["corr"] = correction[1]() if split else correction()
 
         if self.unified_bins and self.block == 5 and self.tiled_deposit:
             segs = [("main", lambda: self.tsc_bins(x, n_mass, m), (), "bins")]
@@ -553,7 +562,12 @@ class SplitForce:
             segs = [targets, mass] if self.side_first else [mass, targets]
         segs += [("side", lambda: self.near_sum(x, a, accumulate=False), ("bins",), None),
                  ("main", lambda: self.deposit(x, n_mass, m, binned=True), (), "deposited")]
-        if correction is not None:
+        if correction is not None and split:
+            # the features behind the deposit on the current stream, the
+            # network behind the near field on the side stream
+            segs.append(("main", correction[0], (), "prepared"))
+            segs.append(("side", corr, ("prepared",), None))
+        elif correction is not None:
             segs.append(("side", corr, ("deposited",), None))
         last = segs[-1] if correction is not None else \
             next(sg for sg in segs if sg[0] == "side" and sg[2] == ("bins",))
