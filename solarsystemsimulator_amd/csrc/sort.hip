@@ -322,6 +322,43 @@ __global__ __launch_bounds__(RT) void k_block_scan(int n, const int* __restrict_
   }
 }
 
+// Mid-size arrays (SCAN1_FAST < n <= SB x SB2_MAX): k_block_sum writes each
+// block's total, then every block of k_block_scan2 sums the totals before
+// its own (all threads, <= SB2_MAX / RT loads each) and scans its block:
+// two fully parallel launches instead of a single-workgroup pass.
+constexpr int SCAN1_FAST = 4096;
+constexpr int SB2_MAX = 1024;
+
+__global__ __launch_bounds__(RT) void k_block_scan2(int n, const int* __restrict__ in,
+                                                    const int* __restrict__ bsum,
+                                                    int* __restrict__ out, int every,
+                                                    int* __restrict__ sel) {
+  __shared__ int red[4];
+  static_assert(RT == 256, "ssim_block256 helpers");
+  const int off = ssim_block256_offset(bsum, blockIdx.x, red);
+  const int base = blockIdx.x * SB + 4 * threadIdx.x;
+  int v[4], s = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0;
+    s += v[k];
+  }
+  int tot;
+  int run = off + ssim_block256_excl(s, red, &tot);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) {
+      out[base + k] = run;
+      if (every && (base + k) % every == 0) sel[(base + k) / every] = run;
+    }
+    run += v[k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    out[n] = off + tot;
+    if (every) sel[n / every] = off + tot;
+  }
+}
+
 }  // namespace
 
 namespace ssim {
@@ -334,13 +371,22 @@ int scan_exclusive(int n, const int* in, int* out, void* work, hipStream_t st, i
                    int* sel) {
   SSIM_REQUIRE(every == 0 || (every > 0 && n % every == 0 && sel),
                "scan: every must divide n");
-  if (n <= SCAN1_MAX) {
+  const int nb = ceil_div(n, SB);
+  if (n <= SCAN1_FAST || (n <= SCAN1_MAX && !work)) {
     hipLaunchKernelGGL(k_scan1, dim3(1), dim3(SCAN_T), 0, st, n, in, out, every, sel);
     SSIM_LAUNCHED("k_scan1");
     return SSIM_OK;
   }
   SSIM_REQUIRE(work, "scan: workspace missing");
-  const int nb = ceil_div(n, SB);
+  if (nb <= SB2_MAX) {
+    int* bsum = static_cast<int*>(work);
+    hipLaunchKernelGGL(k_block_sum, dim3(nb), dim3(RT), 0, st, n, in, bsum);
+    SSIM_LAUNCHED("k_block_sum");
+    hipLaunchKernelGGL(k_block_scan2, dim3(nb), dim3(RT), 0, st, n, in, bsum, out, every,
+                       sel);
+    SSIM_LAUNCHED("k_block_scan2");
+    return SSIM_OK;
+  }
   int* bsum = static_cast<int*>(work);
   int* boff = bsum + align256(((size_t)nb + 1) * sizeof(int)) / sizeof(int);
   hipLaunchKernelGGL(k_block_sum, dim3(nb), dim3(RT), 0, st, n, in, bsum);
