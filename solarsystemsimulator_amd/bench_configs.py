@@ -19,7 +19,8 @@ Prints one JSON object with a record per configuration (the headline
   sums, and the FP32 path's force error against FP64 (median / p99);
 * periodic_pm_262k — 262,144 bodies in the periodic cube, 128^3 mesh, CIC
   deposit, R2C Poisson, central-difference gradient, CIC gather: ms per
-  force evaluation and per leapfrog step, FP32 and FP64 mesh;
+  force evaluation and per leapfrog step, FP32 and FP64 mesh, and µs per
+  CIC deposit alone, tiled (reproducible) and with FP atomics;
 * collisions_50k — star + 4 planets + 50,000 asteroids in a cloud at
   2.5 AU: raster frames/s of the frame kernel, the uniform-grid collision
   search of one sample (candidates, contact pairs, ms) and the merge
@@ -194,6 +195,16 @@ def periodic_pm(steps):
             s.run(1e-3, 1)                                  # warm-up
             t_step, _ = timed(lambda: s.run(1e-3, steps))
             out[f"ms_per_step_{sname}_{name}"] = t_step / steps * 1e3
+        # the CIC deposit alone on the same bodies: the reproducible tiled
+        # one (each node written once) and the FP atomic one (+ its fill)
+        x = [D.to_dev(c, dev) for c in pos]
+        md = D.to_dev(m, dev)
+        for kind, tiled in (("tiled", True), ("atomic", False)):
+            dep = lambda: P.deposit(x, md, [128] * 3, [0.0] * 3, 1.0 / 128, True,  # noqa: E731
+                                    prec, tiled=tiled)
+            dep()
+            t_dep, _ = timed(dep, 20)
+            out[f"us_per_deposit_{kind}_{name}"] = t_dep * 1e6
     return out
 
 

@@ -212,38 +212,31 @@ __global__ __launch_bounds__(CB) void k_mass_counts(int ncells, const int* __res
 }
 
 // Mass counts and their exclusive scan by two fully parallel launches
-// (ncells <= 256 * 4 * SCAN_BLOCKS): k_mass_sum writes mcount[c] (the
-// binary search of k_mass_counts) and each workgroup's total of 1024 cells;
-// k_cells_scan adds the totals of the workgroups before its own to the
-// workgroup's scan.  Same result as k_mass_counts + ssim::scan_exclusive.
+// (ncells <= CB * SCAN_BLOCKS): k_mass_sum writes mcount[c] (the binary
+// search of k_mass_counts, one cell per thread: one search's latency) and
+// each workgroup's total of CB cells; k_cells_scan adds the totals of the
+// workgroups before its own to the workgroup's scan.  Same result as
+// k_mass_counts + ssim::scan_exclusive.
 constexpr int SCAN_BLOCKS = 1024;
-
-__device__ __forceinline__ int mass_count_of(int c, const int* __restrict__ tstart,
-                                             const int* __restrict__ torder, int n_mass) {
-  const int s0 = tstart[c];
-  int lo = s0, hi = tstart[c + 1];
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (torder[mid] < n_mass) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo - s0;
-}
 
 __global__ __launch_bounds__(CB) void k_mass_sum(int ncells, const int* __restrict__ tstart,
                                                  const int* __restrict__ torder, int n_mass,
                                                  int* __restrict__ mcount,
                                                  int* __restrict__ bsum) {
   __shared__ int red[4];
-  const int c0 = (blockIdx.x * CB + threadIdx.x) * 4;
+  const int c = blockIdx.x * CB + threadIdx.x;
   int v = 0;
-#pragma unroll
-  for (int e = 0; e < 4; ++e)
-    if (c0 + e < ncells) {
-      const int k = mass_count_of(c0 + e, tstart, torder, n_mass);
-      mcount[c0 + e] = k;
-      v += k;
+  if (c < ncells) {
+    const int s0 = tstart[c];
+    int lo = s0, hi = tstart[c + 1];  // first slot with an index >= n_mass
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (torder[mid] < n_mass) lo = mid + 1;
+      else hi = mid;
     }
+    v = lo - s0;
+    mcount[c] = v;
+  }
   int tot;
   ssim_block256_excl(v, red, &tot);
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
@@ -254,20 +247,11 @@ __global__ __launch_bounds__(CB) void k_cells_scan(int ncells, const int* __rest
                                                    int* __restrict__ start) {
   __shared__ int red[4];
   const int off = ssim_block256_offset(bsum, blockIdx.x, red);
-  const int c0 = (blockIdx.x * CB + threadIdx.x) * 4;
-  int v[4], s = 0;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    v[e] = c0 + e < ncells ? cnt[c0 + e] : 0;
-    s += v[e];
-  }
+  const int c = blockIdx.x * CB + threadIdx.x;
+  const int v = c < ncells ? cnt[c] : 0;
   int tot;
-  int run = off + ssim_block256_excl(s, red, &tot);
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    if (c0 + e < ncells) start[c0 + e] = run;
-    run += v[e];
-  }
+  const int run = off + ssim_block256_excl(v, red, &tot);
+  if (c < ncells) start[c] = run;
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[ncells] = off + tot;
 }
 
@@ -527,9 +511,9 @@ int ssim_split_bin_all(int n_all, int n_mass, const double* x, const double* y,
                                     ssim::radix_sort_work_bytes(n_all), st));
   }
   SSIM_TRY(run_starts(n_all, key, ncells, tcell_start, st));
-  const int sblocks = ssim::ceil_div(ncells, 4 * CB);
-  if (sblocks <= SCAN_BLOCKS) {
-    // scan_work holds >= ceil(ncells / 1024) + 1 ints (ssim::scan_work_bytes)
+  const int sblocks = ssim::ceil_div(ncells, CB);
+  if (sblocks <= SCAN_BLOCKS && ssim::scan_work_bytes(ncells) >= (sblocks + 1) * sizeof(int)) {
+    // the workgroup totals in scan_work
     int* bsum = static_cast<int*>(scan_work);
     hipLaunchKernelGGL(k_mass_sum, dim3(sblocks), dim3(CB), 0, st, ncells, tcell_start,
                        torder, n_mass, mcount, bsum);
