@@ -21,6 +21,10 @@ Prints one JSON object with a record per configuration (the headline
   deposit, R2C Poisson, central-difference gradient, CIC gather: ms per
   force evaluation and per leapfrog step, FP32 and FP64 mesh, and µs per
   CIC deposit alone, tiled (reproducible) and with FP atomics;
+* near_field_250k — the near field of the bench.py workload (250k masses +
+  500k tracers) after one binning: µs per call of the filtered sum (FP32 /
+  FP64 terms) and of the cell walk (FP32 / hi-lo), and their per-body
+  differences from the FP64 filtered sum;
 * collisions_50k — star + 4 planets + 50,000 asteroids in a cloud at
   2.5 AU: raster frames/s of the frame kernel, the uniform-grid collision
   search of one sample (candidates, contact pairs, ms) and the merge
@@ -208,6 +212,43 @@ def periodic_pm(steps):
     return out
 
 
+def near_field_250k(reps):
+    """The near field of the 250k-mass + 500k-tracer cube (the bench.py
+    workload) after one binning of all bodies: µs per call of the filtered
+    sum (ssim_pp_near_bits, FP32 and FP64 terms) and of the cell walk
+    (ssim_pp_near and its hi/lo form) on the same lists, and each one's
+    per-body difference from the FP64 filtered sum (median / p99)."""
+    import torch
+    from . import pm as P
+    from .p3m import SplitForce
+    dev = D.require_gpu(0)
+    pos, m, tpos = scenes.learned_scene()
+    n, t = len(m), len(tpos[0])
+    sf = SplitForce(dev, mesh_n=128, eps2=1e-6, rs_cells=1.0, rcut_rs=5.0, prec=P.F32,
+                    assign="tsc", margin=4.0, freeze=True)
+    x = [D.to_dev(np.concatenate([pos[c], tpos[c]]), dev) for c in range(3)]
+    md = D.to_dev(m, dev)
+    sf.set_mass_bound(md)
+    sf.place(x)
+    sf.bin_bodies(x, n, md)
+    a = [torch.zeros(n + t, dtype=torch.float64, device=dev) for _ in range(3)]
+    res, acc = {"bodies": n + t, "masses": n}, {}
+    for name, bits, hilo in (("bits_f32", True, False), ("bits_f64", True, True),
+                             ("cells_f32", False, False), ("cells_hilo", False, True)):
+        sf.near_bits, sf.near_hilo = bits, hilo
+        sf.near_sum(x, a, accumulate=False)
+        acc[name] = np.array([c.cpu().numpy() for c in a])
+        t_call, _ = timed(lambda: sf.near_sum(x, a, accumulate=False), reps)
+        res[f"us_{name}"] = t_call * 1e6
+    ref = acc["bits_f64"]
+    nrm = np.linalg.norm(ref, axis=0)
+    for name, v in acc.items():
+        if name != "bits_f64":
+            e = np.linalg.norm(v - ref, axis=0) / nrm
+            res[f"diff_{name}"] = [float(np.median(e)), float(np.percentile(e, 99))]
+    return res
+
+
 def collisions_50k(frames):
     import torch
     from .constants import K_MAX
@@ -274,6 +315,7 @@ def main(argv=None):
                            lambda b: film(scenes.film_clumps, 200_000, b), 4 if q else 20),
                           ("allpairs_65k", allpairs_65k, 5 if q else 20),
                           ("periodic_pm_262k", periodic_pm, 5 if q else 50),
+                          ("near_field_250k", near_field_250k, 5 if q else 20),
                           ("collisions_50k", collisions_50k, 32 if q else 160),
                           ("ring_slots_50k", ring_slots_50k, 3 if q else 10)):
         if args.only and name not in args.only:
