@@ -31,10 +31,11 @@ CFLAGS = [
 ]
 
 
-# Per-file code generation options.  pp.hip: MFMA results in VGPRs rather
-# than AGPRs, so the near field's filter reads its r^2 tiles without
-# v_accvgpr_read copies.
-FILE_FLAGS = {"pp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+# Per-file code generation options: MFMA results in VGPRs rather than AGPRs,
+# so the near field's filter (pp.hip) reads its r^2 tiles and the network's
+# activations (mlp.hip) read their accumulators without v_accvgpr_read copies.
+_VGPR_FORM = ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]
+FILE_FLAGS = {"pp.hip": _VGPR_FORM, "mlp.hip": _VGPR_FORM}
 
 
 def _compile(src: Path) -> Path:

@@ -244,13 +244,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_forward(
 typedef __bf16 bf2_t __attribute__((ext_vector_type(2)));
 typedef float f2_t __attribute__((ext_vector_type(2)));
 
-#ifndef SSIM_IU
-#define SSIM_IU 2
-#endif
-#ifndef SSIM_INFER_ATTR
-#define SSIM_INFER_ATTR
-#endif
-constexpr int IU = SSIM_IU;  // column blocks per iteration
+constexpr int IU = 4;  // 16-body column blocks per iteration (independent MFMA chains)
 constexpr float NEG_LOG2E = -1.4426950408889634f;
 constexpr float NEG_LN2 = -0.6931471805599453f;
 
@@ -271,34 +265,68 @@ __device__ __forceinline__ s4_t act_t(f4_t t) {
   return out;
 }
 
-// U 16-body column blocks per iteration (independent MFMA chains);
-// sb: b1 | -log2(e) b2 | -log2(e) b3 | b4
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f4_t mfma32(bf8_t a, bf8_t b, f4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ bf8_t cat8(s4_t lo, s4_t hi) {
+  typedef short s8_t __attribute__((ext_vector_type(8)));
+  const s8_t v = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+  return __builtin_bit_cast(bf8_t, v);
+}
+// The inference kernel uses v_mfma_f32_16x16x32_bf16 (half the MFMA count of
+// the K = 16 form; lane l: A row l%16 / B column l%16, k 8(l/16)+i, i < 8;
+// accumulator as above).  A tiles: 8 BF16 of one weight row.
+// The B operand of a hidden layer is the pair of accumulator tiles of row
+// blocks 2j, 2j+1 (lane l: units 32j + 4(l/16) + i and 32j + 16 + 4(l/16) + i),
+// so the weight tile takes its columns in that order.
+struct Weights32 {
+  bf8_t w1[4], w2[4][2], w3[4][2], w4[2];
+};
+__device__ __forceinline__ bf8_t load_a_pair(const unsigned short* __restrict__ w, int K,
+                                             int rt, int j, int lane) {
+  const unsigned short* p = w + (size_t)(16 * rt + (lane & 15)) * K + 32 * j + 4 * (lane >> 4);
+  const s4_t lo = *reinterpret_cast<const s4_t*>(p);
+  const s4_t hi = *reinterpret_cast<const s4_t*>(p + 16);
+  return cat8(lo, hi);
+}
+__device__ __forceinline__ void load_weights32(const unsigned short* __restrict__ wb,
+                                               int lane, Weights32& W) {
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt) {
+    const unsigned short* p = wb + OFF_W1 + (size_t)(16 * rt + (lane & 15)) * NIN + 8 * (lane >> 4);
+    W.w1[rt] = *reinterpret_cast<const bf8_t*>(p);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      W.w2[rt][j] = load_a_pair(wb + OFF_W2, NH, rt, j, lane);
+      W.w3[rt][j] = load_a_pair(wb + OFF_W3, NH, rt, j, lane);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 2; ++j) W.w4[j] = load_a_pair(wb + OFF_W4, NH, 0, j, lane);
+}
+
 template <int U>
-__device__ __forceinline__ void infer16(const Weights& W, const float* __restrict__ sb,
+__device__ __forceinline__ void infer16(const Weights32& W, const float* __restrict__ sb,
                                         const unsigned short* __restrict__ x, int nb,
                                         int b0, int lane, float out3[U][3], bool valid[U]) {
-  s4_t xb[U][2], h1[U][4], h2[U][4], h3[U][4];
+  bf8_t xb[U];
+  s4_t h1[U][4], h2[U][4], h3[U][4];
   const int g4 = 4 * (lane >> 4);
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int body = b0 + 16 * u + (lane & 15);
     valid[u] = body < nb;
     const int bl = valid[u] ? body : 0;
-#pragma unroll
-    for (int kb = 0; kb < 2; ++kb) {
-      const s4_t v = *reinterpret_cast<const s4_t*>(x + (size_t)bl * NIN + 16 * kb + g4);
-      xb[u][kb] = valid[u] ? v : s4_t{0, 0, 0, 0};
-    }
+    typedef short s8_t __attribute__((ext_vector_type(8)));
+    const s8_t v = *reinterpret_cast<const s8_t*>(x + (size_t)bl * NIN + 8 * (lane >> 4));
+    xb[u] = __builtin_bit_cast(bf8_t, valid[u] ? v : s8_t{0, 0, 0, 0, 0, 0, 0, 0});
   }
 #pragma unroll
   for (int rt = 0; rt < 4; ++rt) {
     const f4_t b = *reinterpret_cast<const f4_t*>(sb + 16 * rt + g4);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      f4_t acc = mfma(W.w1[rt][0], xb[u][0], b);
-      acc = mfma(W.w1[rt][1], xb[u][1], acc);
-      h1[u][rt] = act_t(acc * NEG_LOG2E);
-    }
+    for (int u = 0; u < U; ++u) h1[u][rt] = act_t(mfma32(W.w1[rt], xb[u], b) * NEG_LOG2E);
   }
 #pragma unroll
   for (int rt = 0; rt < 4; ++rt) {
@@ -307,9 +335,10 @@ __device__ __forceinline__ void infer16(const Weights& W, const float* __restric
 #pragma unroll
     for (int u = 0; u < U; ++u) acc[u] = b;
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb)
+    for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int u = 0; u < U; ++u) acc[u] = mfma(W.w2[rt][kb], h1[u][kb], acc[u]);
+      for (int u = 0; u < U; ++u)
+        acc[u] = mfma32(W.w2[rt][j], cat8(h1[u][2 * j], h1[u][2 * j + 1]), acc[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) h2[u][rt] = act_t(acc[u]);
   }
@@ -320,9 +349,10 @@ __device__ __forceinline__ void infer16(const Weights& W, const float* __restric
 #pragma unroll
     for (int u = 0; u < U; ++u) acc[u] = b;
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb)
+    for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int u = 0; u < U; ++u) acc[u] = mfma(W.w3[rt][kb], h2[u][kb], acc[u]);
+      for (int u = 0; u < U; ++u)
+        acc[u] = mfma32(W.w3[rt][j], cat8(h2[u][2 * j], h2[u][2 * j + 1]), acc[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) h3[u][rt] = act_t(acc[u]);
   }
@@ -330,10 +360,9 @@ __device__ __forceinline__ void infer16(const Weights& W, const float* __restric
 #pragma unroll
   for (int u = 0; u < U; ++u) acc[u] = f4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int kb = 0; kb < 4; ++kb)
+  for (int j = 0; j < 2; ++j)
 #pragma unroll
-    for (int u = 0; u < U; ++u) acc[u] = mfma(W.w4[kb], h3[u][kb], acc[u]);
-  // rows 0..2 live in lanes 0..15 (4(l/16) + i with l/16 == 0)
+    for (int u = 0; u < U; ++u) acc[u] = mfma32(W.w4[j], cat8(h3[u][2 * j], h3[u][2 * j + 1]), acc[u]);
 #pragma unroll
   for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -343,7 +372,7 @@ __device__ __forceinline__ void infer16(const Weights& W, const float* __restric
 // ADD: add the (scaled, projected) correction to a as k_mlp_forward does;
 // otherwise only store the raw outputs.
 template <bool ADD>
-__global__ __launch_bounds__(64 * WAVES) SSIM_INFER_ATTR void k_mlp_infer(
+__global__ __launch_bounds__(64 * WAVES) void k_mlp_infer(
     int nb, const unsigned short* __restrict__ x, const unsigned short* __restrict__ wbf,
     const float* __restrict__ params, float scale, int relative, float* __restrict__ out,
     double* __restrict__ a0, double* __restrict__ a1, double* __restrict__ a2,
@@ -362,8 +391,8 @@ __global__ __launch_bounds__(64 * WAVES) SSIM_INFER_ATTR void k_mlp_infer(
   const int lane = threadIdx.x & 63;
   const int wave = blockIdx.x * WAVES + (threadIdx.x >> 6);
   const int nwaves = gridDim.x * WAVES;
-  Weights W;
-  load_weights(wbf, lane, W);
+  Weights32 W;
+  load_weights32(wbf, lane, W);
   for (int b0 = wave * 16 * IU; b0 < nb; b0 += nwaves * 16 * IU) {
     float o[IU][3];
     bool valid[IU];
