@@ -338,6 +338,16 @@ int ssim_kick_box(int dim, int n, const double* x, const double* y, const double
                   double* vx, double* vy, double* vz, const double* ax,
                   const double* ay, const double* az, double half, double full,
                   double* out, void* work, size_t work_bytes, void* stream);
+/* ssim_kick_box followed by the next step's opening kick and drift, in one
+ * pass: v += a half, v += a half, x += v full (the operations, in order, of
+ * ssim_kick_box then ssim_kick_drift, so the state is bit-identical), and
+ * out = the box of the new positions.  For a step that another step with
+ * the same accelerations' kick follows; work as ssim_kick_box.  No
+ * reference counterpart. */
+int ssim_kick_drift_box(int dim, int n, double* x, double* y, double* z, double* vx,
+                        double* vy, double* vz, const double* ax, const double* ay,
+                        const double* az, double half, double full, double* out,
+                        void* work, size_t work_bytes, void* stream);
 /* Real-space force Green's function component `comp` on the zero-padded
  * (2n)^dim mesh: -G d_comp / (|d|^2 + eps2)^(3/2) with wrapped offsets
  * (build_force_kernels, pm_kernel.py:34-60, generalised to 3-D).  rs > 0

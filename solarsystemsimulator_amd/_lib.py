@@ -93,6 +93,8 @@ SIGNATURES = {
     "ssim_kick_box_work_bytes": (c_size_t, [c_int]),
     "ssim_kick_box": (c_int, [c_int, c_int] + [_P] * 9 + [c_double, c_double, _P, _P,
                                                         c_size_t, _P]),
+    "ssim_kick_drift_box": (c_int, [c_int, c_int] + [_P] * 9 + [c_double, c_double, _P,
+                                                              _P, c_size_t, _P]),
     "ssim_pm_force_kernel": (c_int, [c_int, c_int, c_int, c_double, c_double,
                                      c_double, c_double, c_double, c_int, _P,
                                      _P]),

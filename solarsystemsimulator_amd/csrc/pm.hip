@@ -475,6 +475,9 @@ constexpr int KB_ITEMS = SSIM_KB_ITEMS;
 // of the positions the next step's kick + drift will produce (as
 // k_box_kdk_partial): every workgroup writes its extrema (wave shuffles,
 // then the workgroup's waves) to part, which k_minmax_final folds into out.
+// DRIFT: the next step's opening half kick and drift are stored too (v +=
+// a half twice, x += v full: the operations of k_kick_box followed by
+// k_kick_drift), and the box is that of the stored positions.
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
@@ -486,9 +489,10 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+template <bool DRIFT>
 __global__ __launch_bounds__(PM_BLOCK) void k_kick_box(
-    int n, int na, const double* __restrict__ x0, const double* __restrict__ x1,
-    const double* __restrict__ x2, double* __restrict__ v0, double* __restrict__ v1,
+    int n, int na, double* __restrict__ x0, double* __restrict__ x1,
+    double* __restrict__ x2, double* __restrict__ v0, double* __restrict__ v1,
     double* __restrict__ v2, const double* __restrict__ a0,
     const double* __restrict__ a1, const double* __restrict__ a2, double half,
     double full, double* __restrict__ part) {
@@ -517,6 +521,7 @@ __global__ __launch_bounds__(PM_BLOCK) void k_kick_box(
     }
   }
   double* vs[3] = {v0, v1, v2};
+  double* xs[3] = {x0, x1, x2};
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     if (c >= na) break;
@@ -526,9 +531,14 @@ __global__ __launch_bounds__(PM_BLOCK) void k_kick_box(
       const int i = base + q * PM_BLOCK;
       if (i < n) {
         const double v = vv[c][q] + av[c][q] * half;  // this step's closing kick
-        vs[c][i] = v;
         const double nv = v + av[c][q] * half;        // the next step's opening kick
         const double p = xv[c][q] + nv * full;        // and its drift
+        if (DRIFT) {
+          vs[c][i] = nv;
+          xs[c][i] = p;
+        } else {
+          vs[c][i] = v;
+        }
         lo = fmin(lo, p);
         hi = fmax(hi, p);
       }
@@ -1491,10 +1501,10 @@ size_t ssim_kick_box_work_bytes(int n) {
   return 256 + ssim::align256((size_t)nblk * 6 * sizeof(double));
 }
 
-int ssim_kick_box(int dim, int n, const double* x, const double* y, const double* z,
-                  double* vx, double* vy, double* vz, const double* ax,
-                  const double* ay, const double* az, double half, double full,
-                  double* out, void* work, size_t work_bytes, void* stream) {
+static int kick_box_impl(bool drift, int dim, int n, double* x, double* y, double* z,
+                         double* vx, double* vy, double* vz, const double* ax,
+                         const double* ay, const double* az, double half, double full,
+                         double* out, void* work, size_t work_bytes, void* stream) {
   SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
   SSIM_REQUIRE(n >= 1, "box of an empty set");
   SSIM_REQUIRE(x && y && vx && vy && ax && ay && out && work &&
@@ -1507,12 +1517,34 @@ int ssim_kick_box(int dim, int n, const double* x, const double* y, const double
   // counter in the first would need an agent-scope release per workgroup,
   // i.e. an L2 write-back each
   double* part = reinterpret_cast<double*>(static_cast<unsigned char*>(work) + 256);
-  hipLaunchKernelGGL(k_kick_box, dim3(nblk), dim3(PM_BLOCK), 0, st, n, dim, x, y, z, vx,
-                     vy, vz, ax, ay, az, half, full, part);
+  if (drift)
+    hipLaunchKernelGGL(k_kick_box<true>, dim3(nblk), dim3(PM_BLOCK), 0, st, n, dim, x, y, z,
+                       vx, vy, vz, ax, ay, az, half, full, part);
+  else
+    hipLaunchKernelGGL(k_kick_box<false>, dim3(nblk), dim3(PM_BLOCK), 0, st, n, dim, x, y,
+                       z, vx, vy, vz, ax, ay, az, half, full, part);
   SSIM_LAUNCHED("k_kick_box");
   hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(PM_BLOCK), 0, st, nblk, dim, part, out);
   SSIM_LAUNCHED("k_minmax_final");
   return SSIM_OK;
+}
+
+int ssim_kick_box(int dim, int n, const double* x, const double* y, const double* z,
+                  double* vx, double* vy, double* vz, const double* ax,
+                  const double* ay, const double* az, double half, double full,
+                  double* out, void* work, size_t work_bytes, void* stream) {
+  // the positions are only read without the drift
+  return kick_box_impl(false, dim, n, const_cast<double*>(x), const_cast<double*>(y),
+                       const_cast<double*>(z), vx, vy, vz, ax, ay, az, half, full, out,
+                       work, work_bytes, stream);
+}
+
+int ssim_kick_drift_box(int dim, int n, double* x, double* y, double* z, double* vx,
+                        double* vy, double* vz, const double* ax, const double* ay,
+                        const double* az, double half, double full, double* out,
+                        void* work, size_t work_bytes, void* stream) {
+  return kick_box_impl(true, dim, n, x, y, z, vx, vy, vz, ax, ay, az, half, full, out,
+                       work, work_bytes, stream);
 }
 
 int ssim_pm_force_kernel(int dim, int n, int comp, double h, double eps2,

@@ -80,10 +80,9 @@ class PMLearnedSystem:
         torch = _torch()
         if torch.cuda.is_initialized():
             torch.cuda.synchronize(self.dev)
-        g = getattr(self, "_graph", None)
-        if g is not None:
+        for g in (getattr(self, "_graphs", None) or {}).values():
             g.reset()
-        self._graph, self._graph_key = None, None
+        self._graphs, self._graph_key = {}, None
         self._box_dev = self._box_host = self._box_ws = self._kb_ws = None
         self._box_ev = None
         self.split._side = None
@@ -173,9 +172,12 @@ class PMLearnedSystem:
     # the branches of a two-stream graph in capture order rather than side
     # by side, so the overlapped force is captured with the side stream's
     # binning first (split.side_first): the near field then runs beside the
-    # deposit and the convolution.
+    # deposit and the convolution.  Inside a run() call each step but the
+    # last ends with the next step's opening kick and drift fused into its
+    # closing kick (ssim_kick_drift_box), so only the first step launches
+    # its own (ssim_kick_drift).
     use_graph = True
-    captures = 0          # step graphs captured so far (one per placement)
+    captures = 0          # step graphs captured so far (two per placement at most)
 
     def _box_buffers(self):
         torch = _torch()
@@ -204,49 +206,61 @@ class PMLearnedSystem:
         b = self._box_host.numpy()
         return [(float(b[2 * c]), float(b[2 * c + 1])) for c in range(3)]
 
-    def _step(self, dt, learned, snap_ptr=None):
-        """Launches of one KDK step (the mesh already placed for the drifted
-        positions), ending with the box of the next step's drift."""
+    def _opening(self, dt):
+        """The opening half kick and drift of a step (ssim_kick_drift)."""
+        ps = lambda lst: [_lib.ptr(c) for c in lst]  # noqa: E731
+        _lib.call("ssim_kick_drift", 3, self.n + self.t, *ps(self.x), *ps(self.v),
+                  *ps(self.a), 0.5 * dt, dt, D.stream(self.dev))
+
+    def _step(self, dt, learned, snap_ptr=None, drift_next=False):
+        """Launches of one KDK step after its opening kick and drift (the
+        mesh already placed for the drifted positions), ending with the box
+        of the next step's drift.  drift_next: another step follows, and its
+        opening kick and drift are fused into this step's closing kick
+        (ssim_kick_drift_box: the same operations in the same order)."""
         nb = self.n + self.t
         st = D.stream(self.dev)
         ps = lambda lst: [_lib.ptr(c) for c in lst]  # noqa: E731
         half = 0.5 * dt
-        _lib.call("ssim_kick_drift", 3, nb, *ps(self.x), *ps(self.v), *ps(self.a),
-                  half, dt, st)
         self.force(learned, placed=True)
         if snap_ptr is None:
-            # closing kick + next box in one launch; the box goes straight
-            # into the pinned host buffer (kernels can write it), so reading
-            # it needs no copy
-            _lib.call("ssim_kick_box", 3, nb, *ps(self.x), *ps(self.v), *ps(self.a),
-                      half, dt, _lib.ptr(self._box_host), _lib.ptr(self._kb_ws),
+            # closing kick (+ the next opening kick and drift) + next box in
+            # one launch; the box goes straight into the pinned host buffer
+            # (kernels can write it), so reading it needs no copy
+            _lib.call("ssim_kick_drift_box" if drift_next else "ssim_kick_box", 3, nb,
+                      *ps(self.x), *ps(self.v), *ps(self.a), half, dt,
+                      _lib.ptr(self._box_host), _lib.ptr(self._kb_ws),
                       self._kb_ws.numel(), st)
             return
         _lib.call("ssim_kick_snap", 3, self.n, self.t, *ps(self.x), *ps(self.v),
                   *ps(self.a), half, snap_ptr, st)
         self._next_box(half, dt)
 
-    def _graph_for(self, dt, learned):
+    def _graph_for(self, dt, learned, drift_next=False):
         """The captured step for the current placement (captured anew when
-        the placement, dt or the learned flag changed)."""
+        the placement, dt or the learned flag changed); both forms of the
+        step's end (drift_next) are kept."""
         torch = _torch()
         key = (self.split.placement(), float(dt), bool(learned), self.fold,
                self.split.overlap, self.split.near_hilo)
         if getattr(self, "_graph_key", None) != key:
-            self._graph = None
+            for g in (getattr(self, "_graphs", None) or {}).values():
+                g.reset()
+            self._graphs, self._graph_key = {}, key
+        if drift_next not in self._graphs:
             g = torch.cuda.CUDAGraph()
             side = torch.cuda.Stream(device=self.dev)    # default priority
             side.wait_stream(torch.cuda.current_stream(self.dev))
             self.split.side_first = True
             try:
                 with torch.cuda.graph(g, stream=side):
-                    self._step(dt, learned)
+                    self._step(dt, learned, drift_next=drift_next)
             finally:
                 self.split.side_first = False
             torch.cuda.current_stream(self.dev).wait_stream(side)
-            self._graph, self._graph_key = g, key
+            self._graphs[drift_next] = g
             self.captures += 1
-        return self._graph
+        return self._graphs[drift_next]
 
     def run(self, dt: float, steps: int, snap=None, learned: bool = True,
             graph: bool | None = None):
@@ -263,10 +277,16 @@ class PMLearnedSystem:
         for s in range(steps):
             self.split.place_box(self._read_box(in_host))
             in_host = snap is None    # the step's ssim_kick_box writes it
+            # the first step opens with its own kick and drift; each later
+            # one's were fused into the step before (no snapshots)
+            if s == 0 or snap is not None:
+                self._opening(dt)
+            drift_next = snap is None and s + 1 < steps
             if use:
-                self._graph_for(dt, learned).replay()
+                self._graph_for(dt, learned, drift_next).replay()
             else:
-                self._step(dt, learned, None if snap is None else _lib.ptr(snap[s]))
+                self._step(dt, learned, None if snap is None else _lib.ptr(snap[s]),
+                           drift_next)
 
     def host_state(self):
         return ([c.cpu().numpy() for c in self.x], [c.cpu().numpy() for c in self.v])

@@ -77,6 +77,19 @@ def test_fused_kicks_match_separate_launches(dev):
         for c in range(3):
             assert torch.equal(va[c], vb[c])
         assert torch.equal(out_a, out_b)
+        # ssim_kick_drift_box = ssim_kick_box then ssim_kick_drift (state),
+        # and its box is the one ssim_kick_box gives for that drift
+        xc, vc = [c.clone() for c in xa], [c.clone() for c in va]
+        out_c = torch.empty(6, dtype=torch.float64, device=dev)
+        _lib.call("ssim_kick_drift_box", 3, n, *_ptrs(xc), *_ptrs(vc), *_ptrs(aa), half,
+                  full, _lib.ptr(out_c), _lib.ptr(kw), kw.numel(), st)
+        _lib.call("ssim_kick_box", 3, n, *_ptrs(xa), *_ptrs(va), *_ptrs(aa), half, full,
+                  _lib.ptr(out_a), _lib.ptr(kw), kw.numel(), st)
+        _lib.call("ssim_kick_drift", 3, n, *_ptrs(xa), *_ptrs(va), *_ptrs(aa), half, full,
+                  st)
+        for c in range(3):
+            assert torch.equal(xa[c], xc[c]) and torch.equal(va[c], vc[c])
+        assert torch.equal(out_a, out_c)
 
 
 def _system(dev, pos, m, tpos, vel, tvel):
@@ -108,6 +121,30 @@ def test_replayed_steps_equal_eager_steps(dev):
         assert torch.equal(eager.a[c], replay.a[c])
     # the eager run re-placed the mesh the same way
     assert eager.split.placement() == replay.split.placement()
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_fused_step_ends_equal_single_steps(dev, graph):
+    """One run of 6 steps (each step but the last ends with the next one's
+    opening kick and drift, ssim_kick_drift_box) equals 6 runs of one step
+    (each opens with ssim_kick_drift and closes with ssim_kick_box), bit
+    for bit."""
+    print(f"fused step ends graph={graph}", flush=True)
+    pos, m, tpos = scenes.learned_scene((20, 20, 10), (20, 20, 20), seed=9,
+                                        tracer_seed=99)
+    n, t = len(m), len(tpos[0])
+    rng = np.random.default_rng(10)
+    vel = [rng.normal(0.0, 2.0, n) for _ in range(3)]
+    tvel = [rng.normal(0.0, 2.0, t) for _ in range(3)]
+    one = _system(dev, pos, m, tpos, vel, tvel)
+    many = _system(dev, pos, m, tpos, vel, tvel)
+    one.run(2.5e-4, 6, graph=graph)
+    for _ in range(6):
+        many.run(2.5e-4, 1, graph=graph)
+    for c in range(3):
+        assert torch.equal(one.x[c], many.x[c])
+        assert torch.equal(one.v[c], many.v[c])
+        assert torch.equal(one.a[c], many.a[c])
 
 
 @pytest.mark.parametrize("overlap", [True, False])
