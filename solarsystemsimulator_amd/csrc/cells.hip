@@ -94,25 +94,60 @@ __global__ __launch_bounds__(CB) void k_split_key(
   if (count) atomicAdd(&count[k], 1);  // node bins only (blocks: k_run_starts)
 }
 
-// start[c] = the number of sorted keys below c, c in [0, nbins]: bin c's first
-// slot, what the exclusive scan of the bin counts gives.  The bins up to the
-// first key start at 0 and those past the last key at n (k_run_fill, one
-// thread per bin: a mesh margin leaves thousands of empty bins at both
-// ends).  In between, wave w looks at the slots p in [64w, 64w + 64) below
-// n; the bins c in (K(64w - 1), K(its last slot)] start among them, at
-// 64w + #{p : K(p) < c} (K(-1) taken as the first key), and these ranges
-// tile (first key, last key].  A bin range the wave fills 64 bins per
-// round, each by a 6-step search over the lanes' sorted keys.
-__global__ __launch_bounds__(CB) void k_run_fill(int n, const int* __restrict__ key,
-                                                 int nbins, int* __restrict__ start) {
-  const int c = blockIdx.x * CB + threadIdx.x;
-  if (c > nbins) return;
-  if (n == 0 || c <= key[0]) start[c] = 0;
-  else if (c > key[n - 1]) start[c] = n;
+// The block keys of k_split_key (nodes = 0) fused with the radix sort's
+// first-pass counts: workgroup b takes the keys of tile b as
+// radix_sort_pairs's counting kernel does (RADIX_T threads, keys base +
+// r RADIX_T + t), writes them and its tile's counts of the 8 low key bits
+// (digit-major, row stride hrow); workgroups past the last tile write the
+// rows' zero padding.  The values are the identity (radix_sort_pairs with
+// vals_identity), so no order array is written here.
+__global__ __launch_bounds__(ssim::RADIX_T) void k_split_key_count(
+    int n, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, double ox, double oy, double oz, double h, int gn,
+    int B, int nb, int hrow, int* __restrict__ key, int* __restrict__ hist) {
+  constexpr int T = ssim::RADIX_T, R = ssim::RADIX_TILE / ssim::RADIX_T;
+  __shared__ int cnt[256];
+  const int t = threadIdx.x;
+  cnt[t] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * ssim::RADIX_TILE;
+  const double top = gn - 1;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int i = base + r * T + t;
+    if (i < n) {
+      const int a = (int)fmin(fmax(floor((x[i] - ox) / h + 0.5), 0.0), top);
+      const int b = (int)fmin(fmax(floor((y[i] - oy) / h + 0.5), 0.0), top);
+      const int c = (int)fmin(fmax(floor((z[i] - oz) / h + 0.5), 0.0), top);
+      const int k = ((a / B) * nb + b / B) * nb + c / B;
+      key[i] = k;
+      atomicAdd(&cnt[k & 255], 1);
+    }
+  }
+  __syncthreads();
+  hist[(size_t)t * hrow + blockIdx.x] = cnt[t];
 }
 
+// start[c] = the number of sorted keys below c, c in [0, nbins]: bin c's first
+// slot, what the exclusive scan of the bin counts gives.  One launch, two
+// independent parts.  The first nrun workgroups: wave w looks at the slots p
+// in [64w, 64w + 64) below n; the bins c in (K(64w - 1), K(its last slot)]
+// start among them, at 64w + #{p : K(p) < c} (K(-1) taken as the first
+// key), and these ranges tile (first key, last key].  A bin range the wave
+// fills 64 bins per round, each by a 6-step search over the lanes' sorted
+// keys.  The other workgroups, one thread per bin: the bins up to the first
+// key start at 0 and those past the last key at n (a mesh margin leaves
+// thousands of empty bins at both ends).
 __global__ __launch_bounds__(CB) void k_run_starts(int n, const int* __restrict__ key,
-                                                   int nbins, int* __restrict__ start) {
+                                                   int nbins, int nrun,
+                                                   int* __restrict__ start) {
+  if ((int)blockIdx.x >= nrun) {
+    const int c = (blockIdx.x - nrun) * CB + threadIdx.x;
+    if (c > nbins) return;
+    if (n == 0 || c <= key[0]) start[c] = 0;
+    else if (c > key[n - 1]) start[c] = n;
+    return;
+  }
   const int lane = threadIdx.x & 63;
   const int p0 = (int)(((size_t)blockIdx.x * CB + threadIdx.x) >> 6) * 64;
   if (p0 >= n) return;  // whole waves
@@ -131,13 +166,10 @@ __global__ __launch_bounds__(CB) void k_run_starts(int n, const int* __restrict_
 }
 
 int run_starts(int n, const int* key, int nbins, int* start, hipStream_t st) {
-  hipLaunchKernelGGL(k_run_fill, dim3(ssim::ceil_div(nbins + 1, CB)), dim3(CB), 0, st, n,
-                     key, nbins, start);
-  SSIM_LAUNCHED("k_run_fill");
-  if (n == 0) return SSIM_OK;
-  const long long waves = ssim::ceil_div(n, 64);
-  hipLaunchKernelGGL(k_run_starts, dim3(ssim::ceil_div(waves * 64, CB)), dim3(CB), 0, st,
-                     n, key, nbins, start);
+  const int nrun = n > 0 ? (int)ssim::ceil_div(ssim::ceil_div(n, 64) * 64, CB) : 0;
+  const int nfill = (int)ssim::ceil_div(nbins + 1, CB);
+  hipLaunchKernelGGL(k_run_starts, dim3(nrun + nfill), dim3(CB), 0, st, n, key, nbins, nrun,
+                     start);
   SSIM_LAUNCHED("k_run_starts");
   return SSIM_OK;
 }
@@ -500,15 +532,22 @@ int ssim_split_bin_all(int n_all, int n_mass, const double* x, const double* y,
   w += ssim::align256((size_t)ncells * sizeof(int));
   void* scan_work = w;
   w += ssim::scan_work_bytes(ncells);
-  if (n_all > 0) {
-    hipLaunchKernelGGL(k_split_key, dim3(ssim::ceil_div(n_all, CB)), dim3(CB), 0, st, n_all,
-                       x, y, z, origin[0], origin[1], origin[2], h, grid_n, block, nb, 0,
-                       key, torder, nullptr);
-    SSIM_LAUNCHED("k_split_key");
-    int bits = 1;
-    while (bits < 31 && (1LL << bits) < ncells) ++bits;
+  int bits = 1;
+  while (bits < 31 && (1LL << bits) < ncells) ++bits;
+  if (n_all >= 2) {
+    // keys and the sort's first counts in one launch, identity values
+    int hrow = 0;
+    int* hist = ssim::radix_first_counts(w, n_all, &hrow);
+    hipLaunchKernelGGL(k_split_key_count, dim3(hrow), dim3(ssim::RADIX_T), 0, st, n_all, x,
+                       y, z, origin[0], origin[1], origin[2], h, grid_n, block, nb, hrow, key,
+                       hist);
+    SSIM_LAUNCHED("k_split_key_count");
     SSIM_TRY(ssim::radix_sort_pairs(n_all, bits, key, torder, w,
-                                    ssim::radix_sort_work_bytes(n_all), st));
+                                    ssim::radix_sort_work_bytes(n_all), st, true, true));
+  } else if (n_all == 1) {
+    hipLaunchKernelGGL(k_split_key, dim3(1), dim3(CB), 0, st, n_all, x, y, z, origin[0],
+                       origin[1], origin[2], h, grid_n, block, nb, 0, key, torder, nullptr);
+    SSIM_LAUNCHED("k_split_key");
   }
   SSIM_TRY(run_starts(n_all, key, ncells, tcell_start, st));
   const int sblocks = ssim::ceil_div(ncells, CB);

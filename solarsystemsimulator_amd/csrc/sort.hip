@@ -22,9 +22,10 @@
 
 namespace {
 
-constexpr int RT = 256;            // threads per workgroup (one per digit)
+constexpr int RT = ssim::RADIX_T;  // threads per workgroup (one per digit)
 constexpr int RIPT = 4;            // rounds per tile
 constexpr int RTILE = RT * RIPT;   // elements per tile
+static_assert(RTILE == ssim::RADIX_TILE, "tile of the exported layout");
 constexpr int FUSE_TILES = 64;     // up to this many tiles a scatter block
                                    // computes its own offsets (its row loop
                                    // is latency-bound; beyond, k_radix_rows)
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(RT) void k_radix_scatter(
   for (int r = 0; r < RIPT; ++r) {
     const int i = base + r * RT + t;
     k[r] = i < n ? keys[i] : 0;
-    v[r] = i < n ? vals[i] : 0;
+    v[r] = vals ? (i < n ? vals[i] : 0) : i;  // vals == NULL: the identity
   }
   if (offs) {
     run[t] = excl_scan256(offs[(size_t)256 * hrow + t], s) + offs[(size_t)t * hrow + blockIdx.x];
@@ -407,8 +408,18 @@ size_t radix_sort_work_bytes(int n) {
          align256((256 * hrow + 256) * sizeof(int));
 }
 
+int* radix_first_counts(void* work, int n, int* hrow) {
+  const int tiles = ceil_div(n > 0 ? n : 1, RTILE);
+  *hrow = (tiles + 3) & ~3;
+  return reinterpret_cast<int*>(static_cast<unsigned char*>(work) +
+                                4 * align256((size_t)(n > 0 ? n : 1) * sizeof(int)));
+}
+
 int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
-                     size_t work_bytes, hipStream_t st) {
+                     size_t work_bytes, hipStream_t st, bool first_counted,
+                     bool vals_identity) {
+  SSIM_REQUIRE(!(first_counted || vals_identity) || n >= 2,
+               "radix sort: counted keys / identity values need n >= 2");
   if (n <= 1) return SSIM_OK;
   SSIM_REQUIRE(bits >= 1 && bits <= 31, "radix sort: bad key width %d", bits);
   SSIM_REQUIRE(work && work_bytes >= radix_sort_work_bytes(n),
@@ -430,14 +441,17 @@ int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
   w += align256(4 * ht * sizeof(int));
   int* offs = reinterpret_cast<int*>(w);
   const int* sk = keys;
-  const int* sv = vals;
+  const int* sv = vals_identity ? nullptr : vals;
   for (int p = 0; p < passes; ++p) {
     const bool last = p == passes - 1 && passes >= 2;
     int* dk = last ? keys : pk[p & 1];
     int* dv = last ? vals : pv[p & 1];
     const int* h = hist;
-    hipLaunchKernelGGL(k_radix_hist, dim3(hrow), dim3(RT), 0, st, n, sk, 8 * p, hrow, hist);
-    SSIM_LAUNCHED("k_radix_hist");
+    if (p > 0 || !first_counted) {
+      hipLaunchKernelGGL(k_radix_hist, dim3(hrow), dim3(RT), 0, st, n, sk, 8 * p, hrow,
+                         hist);
+      SSIM_LAUNCHED("k_radix_hist");
+    }
     const int* o = nullptr;
     if (tiles > FUSE_TILES) {
       hipLaunchKernelGGL(k_radix_rows, dim3(256), dim3(RT), 0, st, h, hrow, offs,

@@ -53,9 +53,19 @@ void* timing_start(hipStream_t st);
 void timing_stop(const char* name, void* start, hipStream_t st);
 
 // Stable LSD radix sort of (keys, vals) in place on keys in [0, 2^bits).
+// first_counted: the caller's key kernel has already written the first
+// pass's per-tile digit counts (8 low key bits) into radix_first_counts(work,
+// n), in k_radix_hist's layout: tiles of RADIX_TILE keys, RADIX_T threads
+// each taking keys base + r RADIX_T + t, r < RADIX_TILE / RADIX_T, counts
+// digit-major (row stride *hrow, the tile count rounded up to 4, zero past
+// the last tile).  vals_identity: the input values are 0..n-1 (not read;
+// vals is output only).
+constexpr int RADIX_T = 256, RADIX_TILE = 1024;
 size_t radix_sort_work_bytes(int n);
+int* radix_first_counts(void* work, int n, int* hrow);
 int radix_sort_pairs(int n, int bits, int* keys, int* vals, void* work,
-                     size_t work_bytes, hipStream_t st);
+                     size_t work_bytes, hipStream_t st, bool first_counted = false,
+                     bool vals_identity = false);
 
 }  // namespace ssim
 

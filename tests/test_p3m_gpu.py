@@ -341,7 +341,8 @@ def _split_bin_call(dev, pos, m, n, origin, h, B):
             for k, v in out.items()}
 
 
-@pytest.mark.parametrize("case", ["uniform", "clump", "no_tracers", "no_masses", "empty"])
+@pytest.mark.parametrize("case", ["uniform", "clump", "no_tracers", "no_masses", "empty",
+                                  "one", "two"])
 def test_split_bin_all_equals_two_binnings(dev, case):
     """ssim_split_bin_all (one binning of masses and tracers): its target
     lists equal the oracle's block binning of all bodies (stable, index
@@ -353,7 +354,8 @@ def test_split_bin_all_equals_two_binnings(dev, case):
     origin = [0.1, -0.2, 0.3]
     rng = np.random.default_rng(31)
     nm, nt = {"uniform": (20_000, 45_000), "clump": (20_000, 45_000),
-              "no_tracers": (7_000, 0), "no_masses": (0, 9_000), "empty": (0, 0)}[case]
+              "no_tracers": (7_000, 0), "no_masses": (0, 9_000), "empty": (0, 0),
+              "one": (1, 0), "two": (1, 1)}[case]
     pos = [origin[a] + rng.uniform(0.5, n - 1.5, nm + nt) * h for a in range(3)]
     if case == "clump":
         for a in range(3):
