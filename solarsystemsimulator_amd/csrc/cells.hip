@@ -100,14 +100,18 @@ __global__ __launch_bounds__(CB) void k_split_key(
 // r RADIX_T + t), writes them and its tile's counts of the 8 low key bits
 // (digit-major, row stride hrow); workgroups past the last tile write the
 // rows' zero padding.  The values are the identity (radix_sort_pairs with
-// vals_identity), so no order array is written here.
+// vals_identity), so no order array is written here.  zero[0..nzero) is
+// cleared on the way (the mass counts k_run_starts adds to).
 __global__ __launch_bounds__(ssim::RADIX_T) void k_split_key_count(
     int n, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, double ox, double oy, double oz, double h, int gn,
-    int B, int nb, int hrow, int* __restrict__ key, int* __restrict__ hist) {
+    int B, int nb, int hrow, int* __restrict__ key, int* __restrict__ hist,
+    int nzero, int* __restrict__ zero) {
   constexpr int T = ssim::RADIX_T, R = ssim::RADIX_TILE / ssim::RADIX_T;
   __shared__ int cnt[256];
   const int t = threadIdx.x;
+  // zero[0..nzero): counts that a later launch accumulates into
+  for (int q = blockIdx.x * T + t; q < nzero; q += gridDim.x * T) zero[q] = 0;
   cnt[t] = 0;
   __syncthreads();
   const int base = blockIdx.x * ssim::RADIX_TILE;
@@ -140,7 +144,9 @@ __global__ __launch_bounds__(ssim::RADIX_T) void k_split_key_count(
 // thousands of empty bins at both ends).
 __global__ __launch_bounds__(CB) void k_run_starts(int n, const int* __restrict__ key,
                                                    int nbins, int nrun,
-                                                   int* __restrict__ start) {
+                                                   int* __restrict__ start,
+                                                   const int* __restrict__ vals, int nv,
+                                                   int* __restrict__ vcount) {
   if ((int)blockIdx.x >= nrun) {
     const int c = (blockIdx.x - nrun) * CB + threadIdx.x;
     if (c > nbins) return;
@@ -163,13 +169,30 @@ __global__ __launch_bounds__(CB) void k_run_starts(int n, const int* __restrict_
       if (__shfl(kp, r + st - 1, 64) < c) r += st;
     if (c <= hi) start[c] = p0 + r;
   }
+  if (vcount) {
+    // vcount[k] += the slots of key k whose value is below nv: the first
+    // lane of each run of equal keys in the wave adds the run's count
+    const bool below = p < n && vals[p] < nv;
+    const int prev = __shfl_up(kp, 1, 64);
+    const bool head = p < n && (lane == 0 || prev != kp);
+    const unsigned long long heads = __ballot(head), hits = __ballot(below);
+    if (head) {
+      const unsigned long long later = heads & ~((2ull << lane) - 1ull);
+      const unsigned long long upto = later ? (1ull << __builtin_ctzll(later)) - 1ull : ~0ull;
+      const int cnt = __popcll(hits & upto & ~((1ull << lane) - 1ull));
+      if (cnt) atomicAdd(&vcount[kp], cnt);
+    }
+  }
 }
 
-int run_starts(int n, const int* key, int nbins, int* start, hipStream_t st) {
+// vals / nv / vcount (optional, vcount zeroed beforehand): vcount[k] = the
+// number of slots of key k whose value is below nv.
+int run_starts(int n, const int* key, int nbins, int* start, hipStream_t st,
+               const int* vals = nullptr, int nv = 0, int* vcount = nullptr) {
   const int nrun = n > 0 ? (int)ssim::ceil_div(ssim::ceil_div(n, 64) * 64, CB) : 0;
   const int nfill = (int)ssim::ceil_div(nbins + 1, CB);
   hipLaunchKernelGGL(k_run_starts, dim3(nrun + nfill), dim3(CB), 0, st, n, key, nbins, nrun,
-                     start);
+                     start, vals, nv, vcount);
   SSIM_LAUNCHED("k_run_starts");
   return SSIM_OK;
 }
@@ -279,6 +302,34 @@ __global__ __launch_bounds__(CB) void k_cells_scan(int ncells, const int* __rest
                                                    int* __restrict__ start) {
   __shared__ int red[4];
   const int off = ssim_block256_offset(bsum, blockIdx.x, red);
+  const int c = blockIdx.x * CB + threadIdx.x;
+  const int v = c < ncells ? cnt[c] : 0;
+  int tot;
+  const int run = off + ssim_block256_excl(v, red, &tot);
+  if (c < ncells) start[c] = run;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[ncells] = off + tot;
+}
+
+// start = the exclusive scan of cnt[0..ncells) (start[ncells] = the total),
+// one launch: each workgroup sums the counts before its CB cells itself
+// (ncells <= DIRECT_SCAN_CELLS keeps those re-reads small).
+constexpr int DIRECT_SCAN_CELLS = 32768;
+__global__ __launch_bounds__(CB) void k_cells_scan_direct(int ncells,
+                                                          const int* __restrict__ cnt,
+                                                          int* __restrict__ start) {
+  __shared__ int red[4];
+  // the counts before this workgroup's cells (a multiple of CB of them):
+  // 16-byte loads, eight in flight per thread
+  const int4* c4 = reinterpret_cast<const int4*>(cnt);
+  const int n4 = blockIdx.x * (CB / 4);
+  int pre = 0;
+#pragma unroll 8
+  for (int i = threadIdx.x; i < n4; i += CB) {
+    const int4 q = c4[i];
+    pre += (q.x + q.y) + (q.z + q.w);
+  }
+  int off;
+  ssim_block256_excl(pre, red, &off);
   const int c = blockIdx.x * CB + threadIdx.x;
   const int v = c < ncells ? cnt[c] : 0;
   int tot;
@@ -534,13 +585,16 @@ int ssim_split_bin_all(int n_all, int n_mass, const double* x, const double* y,
   w += ssim::scan_work_bytes(ncells);
   int bits = 1;
   while (bits < 31 && (1LL << bits) < ncells) ++bits;
+  // the common case: keys, the sort's first counts and the zeroed mass
+  // counts in one launch (identity values), the mass counts with the run
+  // starts, and a one-launch scan of them
+  const bool fused = n_all >= 2 && ncells <= DIRECT_SCAN_CELLS;
   if (n_all >= 2) {
-    // keys and the sort's first counts in one launch, identity values
     int hrow = 0;
     int* hist = ssim::radix_first_counts(w, n_all, &hrow);
     hipLaunchKernelGGL(k_split_key_count, dim3(hrow), dim3(ssim::RADIX_T), 0, st, n_all, x,
                        y, z, origin[0], origin[1], origin[2], h, grid_n, block, nb, hrow, key,
-                       hist);
+                       hist, fused ? ncells : 0, mcount);
     SSIM_LAUNCHED("k_split_key_count");
     SSIM_TRY(ssim::radix_sort_pairs(n_all, bits, key, torder, w,
                                     ssim::radix_sort_work_bytes(n_all), st, true, true));
@@ -549,22 +603,30 @@ int ssim_split_bin_all(int n_all, int n_mass, const double* x, const double* y,
                        origin[1], origin[2], h, grid_n, block, nb, 0, key, torder, nullptr);
     SSIM_LAUNCHED("k_split_key");
   }
-  SSIM_TRY(run_starts(n_all, key, ncells, tcell_start, st));
   const int sblocks = ssim::ceil_div(ncells, CB);
-  if (sblocks <= SCAN_BLOCKS && ssim::scan_work_bytes(ncells) >= (sblocks + 1) * sizeof(int)) {
-    // the workgroup totals in scan_work
-    int* bsum = static_cast<int*>(scan_work);
-    hipLaunchKernelGGL(k_mass_sum, dim3(sblocks), dim3(CB), 0, st, ncells, tcell_start,
-                       torder, n_mass, mcount, bsum);
-    SSIM_LAUNCHED("k_mass_sum");
-    hipLaunchKernelGGL(k_cells_scan, dim3(sblocks), dim3(CB), 0, st, ncells, mcount, bsum,
+  if (fused) {
+    SSIM_TRY(run_starts(n_all, key, ncells, tcell_start, st, torder, n_mass, mcount));
+    hipLaunchKernelGGL(k_cells_scan_direct, dim3(sblocks), dim3(CB), 0, st, ncells, mcount,
                        cell_start);
-    SSIM_LAUNCHED("k_cells_scan");
+    SSIM_LAUNCHED("k_cells_scan_direct");
   } else {
-    hipLaunchKernelGGL(k_mass_counts, dim3(ssim::ceil_div(ncells, CB)), dim3(CB), 0, st,
-                       ncells, tcell_start, torder, n_mass, mcount);
-    SSIM_LAUNCHED("k_mass_counts");
-    SSIM_TRY(ssim::scan_exclusive(ncells, mcount, cell_start, scan_work, st));
+    SSIM_TRY(run_starts(n_all, key, ncells, tcell_start, st));
+    if (sblocks <= SCAN_BLOCKS &&
+        ssim::scan_work_bytes(ncells) >= (sblocks + 1) * sizeof(int)) {
+      // the workgroup totals in scan_work
+      int* bsum = static_cast<int*>(scan_work);
+      hipLaunchKernelGGL(k_mass_sum, dim3(sblocks), dim3(CB), 0, st, ncells, tcell_start,
+                         torder, n_mass, mcount, bsum);
+      SSIM_LAUNCHED("k_mass_sum");
+      hipLaunchKernelGGL(k_cells_scan, dim3(sblocks), dim3(CB), 0, st, ncells, mcount,
+                         bsum, cell_start);
+      SSIM_LAUNCHED("k_cells_scan");
+    } else {
+      hipLaunchKernelGGL(k_mass_counts, dim3(ssim::ceil_div(ncells, CB)), dim3(CB), 0, st,
+                         ncells, tcell_start, torder, n_mass, mcount);
+      SSIM_LAUNCHED("k_mass_counts");
+      SSIM_TRY(ssim::scan_exclusive(ncells, mcount, cell_start, scan_work, st));
+    }
   }
   if (n_mass > 0) {
     const double hh = 0.5 * h;
