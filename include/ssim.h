@@ -338,6 +338,15 @@ int ssim_kick_box(int dim, int n, const double* x, const double* y, const double
                   double* vx, double* vy, double* vz, const double* ax,
                   const double* ay, const double* az, double half, double full,
                   double* out, void* work, size_t work_bytes, void* stream);
+/* The closing half kick of a step and the next step's opening half kick
+ * and drift in one pass: v += a half, v += a half, x += v full (the
+ * operations, in order, of ssim_kick then ssim_kick_drift).  For a caller
+ * that computes the next box beforehand (ssim_box_after_kdk on the same
+ * state gives the box of the positions stored here).  No reference
+ * counterpart. */
+int ssim_kicks_drift(int dim, int n, double* x, double* y, double* z, double* vx,
+                     double* vy, double* vz, const double* ax, const double* ay,
+                     const double* az, double half, double full, void* stream);
 /* ssim_kick_box followed by the next step's opening kick and drift, in one
  * pass: v += a half, v += a half, x += v full (the operations, in order, of
  * ssim_kick_box then ssim_kick_drift, so the state is bit-identical), and
@@ -429,6 +438,24 @@ int ssim_tsc_gather_corrected(int np, const int* order, const double* x,
                               const float* corr, float scale, int relative,
                               const double* v0, const double* v1, const double* v2,
                               int n_proj, void* stream);
+/* ssim_tsc_gather of the interleaved FP32 fields (corr == NULL) or
+ * ssim_tsc_gather_corrected (corr != NULL; n_proj and the velocities as
+ * there), the same results, and out[0..6) = the box (min, max per axis) of
+ * the positions x + ((v + a half) + a half) full, a the stored sum: where
+ * the bodies go with this step's closing half kick and the next step's
+ * opening half kick and drift (ssim_kick_drift_box, ssim_kicks_drift).  v0,
+ * v1, v2: the velocities of all np points.  The gather writes
+ * per-workgroup extrema to work (ssim_tsc_gather_box_work_bytes(np)) and a
+ * one-workgroup launch folds them into out, which may be pinned host
+ * memory.  np >= 1.  No reference counterpart. */
+size_t ssim_tsc_gather_box_work_bytes(int np);
+int ssim_tsc_gather_box(int np, const int* order, const double* x, const double* y,
+                        const double* z, int grid_n, const double* origin, double h,
+                        const void* f4, double* o0, double* o1, double* o2, int accumulate,
+                        const float* corr, float scale, int relative, const double* v0,
+                        const double* v1, const double* v2, int n_proj, double half,
+                        double full, double* out, void* work, size_t work_bytes,
+                        void* stream);
 /* out = a * b over ncomplex interleaved complex values (the spectral
  * products of pm_force_field, pm_kernel.py:144-146). */
 int ssim_spectrum_mul(long long ncomplex, int prec, const void* a,

@@ -93,6 +93,7 @@ SIGNATURES = {
     "ssim_kick_box_work_bytes": (c_size_t, [c_int]),
     "ssim_kick_box": (c_int, [c_int, c_int] + [_P] * 9 + [c_double, c_double, _P, _P,
                                                         c_size_t, _P]),
+    "ssim_kicks_drift": (c_int, [c_int, c_int] + [_P] * 9 + [c_double, c_double, _P]),
     "ssim_kick_drift_box": (c_int, [c_int, c_int] + [_P] * 9 + [c_double, c_double, _P,
                                                               _P, c_size_t, _P]),
     "ssim_pm_force_kernel": (c_int, [c_int, c_int, c_int, c_double, c_double,
@@ -113,6 +114,10 @@ SIGNATURES = {
     "ssim_tsc_gather_corrected": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double,
                                           _P, _P, _P, _P, c_int, _P, c_float, c_int,
                                           _P, _P, _P, c_int, _P]),
+    "ssim_tsc_gather_box_work_bytes": (c_size_t, [c_int]),
+    "ssim_tsc_gather_box": (c_int, [c_int, _P, _P, _P, _P, c_int, _P, c_double, _P, _P, _P,
+                                    _P, c_int, _P, c_float, c_int, _P, _P, _P, c_int,
+                                    c_double, c_double, _P, _P, c_size_t, _P]),
     "ssim_spectrum_mul": (c_int, [c_longlong, c_int, _P, _P, _P, _P]),
     "ssim_spectrum_imag": (c_int, [c_longlong, c_int, _P, _P, _P]),
     "ssim_pm_conv_isolated3": (c_int, [c_int, c_int] + [_P] * 9 + [_P, c_size_t,

@@ -477,7 +477,8 @@ constexpr int KB_ITEMS = SSIM_KB_ITEMS;
 // then the workgroup's waves) to part, which k_minmax_final folds into out.
 // DRIFT: the next step's opening half kick and drift are stored too (v +=
 // a half twice, x += v full: the operations of k_kick_box followed by
-// k_kick_drift), and the box is that of the stored positions.
+// k_kick_drift), and the box is that of the stored positions.  BOX false:
+// no box (part is not written).
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
@@ -489,7 +490,7 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-template <bool DRIFT>
+template <bool DRIFT, bool BOX>
 __global__ __launch_bounds__(PM_BLOCK) void k_kick_box(
     int n, int na, double* __restrict__ x0, double* __restrict__ x1,
     double* __restrict__ x2, double* __restrict__ v0, double* __restrict__ v1,
@@ -543,6 +544,7 @@ __global__ __launch_bounds__(PM_BLOCK) void k_kick_box(
         hi = fmax(hi, p);
       }
     }
+    if (!BOX) continue;
     lo = wave_min(lo);
     hi = wave_max(hi);
     if (lane == 0) {
@@ -550,6 +552,7 @@ __global__ __launch_bounds__(PM_BLOCK) void k_kick_box(
       wmax[c][wv] = hi;
     }
   }
+  if (!BOX) return;
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int c = 0; c < na; ++c) {
@@ -938,16 +941,29 @@ struct Corr {
   int n_proj;
 };
 
-template <bool CORR, bool BUF>
+// BOX (k_tsc_gather4<., ., true>): with the stored acceleration a of each
+// body, the position the next closing and opening half kicks and drift
+// give, x + ((v + a half) + a half) full (the operations of
+// ssim_kick_drift_box), is folded into per-workgroup extrema written to
+// bx.part (the layout k_minmax_final folds).
+struct BoxOut {
+  const double *v0, *v1, *v2;
+  double half, full;
+  double* part;
+};
+
+template <bool CORR, bool BUF, bool BOX>
 __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
     int np, const int* __restrict__ order, const double* __restrict__ x, const double* __restrict__ y,
     const double* __restrict__ z, Grid g, const float4* __restrict__ f,
     double* __restrict__ o0, double* __restrict__ o1, double* __restrict__ o2,
-    int accumulate, Corr cr) {
+    int accumulate, Corr cr, BoxOut bx) {
   // XCD-contiguous ranges of the (cell-ordered) bodies: an XCD's L2 then
   // holds the part of the mesh its bodies touch
   const int i_ = ssim_xcd_block(blockIdx.x, gridDim.x) * PM_BLOCK + threadIdx.x;
-  if (i_ >= np) return;
+  if (!BOX && i_ >= np) return;
+  double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  if (i_ < np) {
   const int p = order ? order[i_] : i_;
   // every load that does not depend on the stencil is issued first
   // (uniform branches), so its latency hides behind the mesh loads
@@ -970,11 +986,18 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
       w2 = cr.v2[pv];
     }
   }
+  double bv0 = 0.0, bv1 = 0.0, bv2 = 0.0;
+  if (BOX) {
+    bv0 = bx.v0[p];
+    bv1 = bx.v1[p];
+    bv2 = bx.v2[p];
+  }
+  const double px = x[p], py = y[p], pz = z[p];
   int c[3];
   double w[3][3];
-  tsc_axis(x[p], g.o[0], g.h, c[0], w[0]);
-  tsc_axis(y[p], g.o[1], g.h, c[1], w[1]);
-  tsc_axis(z[p], g.o[2], g.h, c[2], w[2]);
+  tsc_axis(px, g.o[0], g.h, c[0], w[0]);
+  tsc_axis(py, g.o[1], g.h, c[1], w[1]);
+  tsc_axis(pz, g.o[2], g.h, c[2], w[2]);
   // branch-free stencil: nodes outside the mesh are clamped into it with
   // weight 0 (an exact zero term), so all 27 loads can be in flight at once
   int ix[3][3];
@@ -1057,6 +1080,39 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
   o0[p] = s0;
   o1[p] = s1;
   o2[p] = s2;
+  if (BOX) {
+    const double sv[3] = {s0, s1, s2}, vv[3] = {bv0, bv1, bv2}, xv[3] = {px, py, pz};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double nv = (vv[q] + sv[q] * bx.half) + sv[q] * bx.half;
+      blo[q] = bhi[q] = xv[q] + nv * bx.full;
+    }
+  }
+  }
+  if (BOX) {
+    constexpr int NW = PM_BLOCK / 64;
+    __shared__ double wmin[3][NW], wmax[3][NW];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double lo = wave_min(blo[q]), hi = wave_max(bhi[q]);
+      if (lane == 0) {
+        wmin[q][wv] = lo;
+        wmax[q][wv] = hi;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int q = 0; q < 3; ++q) {
+        double lo = wmin[q][0], hi = wmax[q][0];
+        for (int w2 = 1; w2 < NW; ++w2) {
+          lo = fmin(lo, wmin[q][w2]);
+          hi = fmax(hi, wmax[q][w2]);
+        }
+        bx.part[(size_t)blockIdx.x * 6 + 2 * q] = lo;
+        bx.part[(size_t)blockIdx.x * 6 + 2 * q + 1] = hi;
+      }
+  }
 }
 
 // One workgroup folds the per-block extrema (min and max are exact, so the
@@ -1518,11 +1574,11 @@ static int kick_box_impl(bool drift, int dim, int n, double* x, double* y, doubl
   // i.e. an L2 write-back each
   double* part = reinterpret_cast<double*>(static_cast<unsigned char*>(work) + 256);
   if (drift)
-    hipLaunchKernelGGL(k_kick_box<true>, dim3(nblk), dim3(PM_BLOCK), 0, st, n, dim, x, y, z,
-                       vx, vy, vz, ax, ay, az, half, full, part);
+    hipLaunchKernelGGL((k_kick_box<true, true>), dim3(nblk), dim3(PM_BLOCK), 0, st, n, dim,
+                       x, y, z, vx, vy, vz, ax, ay, az, half, full, part);
   else
-    hipLaunchKernelGGL(k_kick_box<false>, dim3(nblk), dim3(PM_BLOCK), 0, st, n, dim, x, y,
-                       z, vx, vy, vz, ax, ay, az, half, full, part);
+    hipLaunchKernelGGL((k_kick_box<false, true>), dim3(nblk), dim3(PM_BLOCK), 0, st, n,
+                       dim, x, y, z, vx, vy, vz, ax, ay, az, half, full, part);
   SSIM_LAUNCHED("k_kick_box");
   hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(PM_BLOCK), 0, st, nblk, dim, part, out);
   SSIM_LAUNCHED("k_minmax_final");
@@ -1537,6 +1593,22 @@ int ssim_kick_box(int dim, int n, const double* x, const double* y, const double
   return kick_box_impl(false, dim, n, const_cast<double*>(x), const_cast<double*>(y),
                        const_cast<double*>(z), vx, vy, vz, ax, ay, az, half, full, out,
                        work, work_bytes, stream);
+}
+
+int ssim_kicks_drift(int dim, int n, double* x, double* y, double* z, double* vx,
+                     double* vy, double* vz, const double* ax, const double* ay,
+                     const double* az, double half, double full, void* stream) {
+  SSIM_REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3");
+  SSIM_REQUIRE(n >= 0, "n must be >= 0");
+  if (n == 0) return SSIM_OK;
+  SSIM_REQUIRE(x && y && vx && vy && ax && ay && (dim == 2 || (z && vz && az)),
+               "null pointer");
+  hipStream_t st = ssim::as_stream(stream);
+  hipLaunchKernelGGL((k_kick_box<true, false>), dim3(ssim::ceil_div(n, PM_BLOCK * KB_ITEMS)),
+                     dim3(PM_BLOCK), 0, st, n, dim, x, y, z, vx, vy, vz, ax, ay, az, half,
+                     full, nullptr);
+  SSIM_LAUNCHED("k_kick_box");
+  return SSIM_OK;
 }
 
 int ssim_kick_drift_box(int dim, int n, double* x, double* y, double* z, double* vx,
@@ -1756,18 +1828,18 @@ int ssim_tsc_gather(int np, const int* order, const double* x, const double* y,
     hipStream_t st = ssim::as_stream(stream);
     if (buffer_mesh(grid_n))
       SSIM_TIMED("k_tsc_gather", st,
-                 hipLaunchKernelGGL((k_tsc_gather4<false, true>),
+                 hipLaunchKernelGGL((k_tsc_gather4<false, true, false>),
                                     dim3(ssim::ceil_div(np, PM_BLOCK)), dim3(PM_BLOCK), 0,
                                     st, np, order, x, y, z, g,
                                     static_cast<const float4*>(f0), o0, o1, o2,
-                                    accumulate, Corr{}));
+                                    accumulate, Corr{}, BoxOut{}));
     else
       SSIM_TIMED("k_tsc_gather", st,
-                 hipLaunchKernelGGL((k_tsc_gather4<false, false>),
+                 hipLaunchKernelGGL((k_tsc_gather4<false, false, false>),
                                     dim3(ssim::ceil_div(np, PM_BLOCK)), dim3(PM_BLOCK), 0,
                                     st, np, order, x, y, z, g,
                                     static_cast<const float4*>(f0), o0, o1, o2,
-                                    accumulate, Corr{}));
+                                    accumulate, Corr{}, BoxOut{}));
     SSIM_LAUNCHED("k_tsc_gather4");
     return SSIM_OK;
   }
@@ -1810,17 +1882,59 @@ int ssim_tsc_gather_corrected(int np, const int* order, const double* x,
   const Corr cr{corr, scale, relative, v0, v1, v2, n_proj};
   if (buffer_mesh(grid_n))
     SSIM_TIMED("k_tsc_gather", st,
-               hipLaunchKernelGGL((k_tsc_gather4<true, true>),
+               hipLaunchKernelGGL((k_tsc_gather4<true, true, false>),
                                   dim3(ssim::ceil_div(np, PM_BLOCK)), dim3(PM_BLOCK), 0, st,
                                   np, order, x, y, z, g, static_cast<const float4*>(f4), o0,
-                                  o1, o2, accumulate, cr));
+                                  o1, o2, accumulate, cr, BoxOut{}));
   else
     SSIM_TIMED("k_tsc_gather", st,
-               hipLaunchKernelGGL((k_tsc_gather4<true, false>),
+               hipLaunchKernelGGL((k_tsc_gather4<true, false, false>),
                                   dim3(ssim::ceil_div(np, PM_BLOCK)), dim3(PM_BLOCK), 0, st,
                                   np, order, x, y, z, g, static_cast<const float4*>(f4), o0,
-                                  o1, o2, accumulate, cr));
+                                  o1, o2, accumulate, cr, BoxOut{}));
   SSIM_LAUNCHED("k_tsc_gather4");
+  return SSIM_OK;
+}
+
+size_t ssim_tsc_gather_box_work_bytes(int np) {
+  return ssim::align256((size_t)ssim::ceil_div(np > 0 ? np : 1, PM_BLOCK) * 6 *
+                        sizeof(double));
+}
+
+int ssim_tsc_gather_box(int np, const int* order, const double* x, const double* y,
+                        const double* z, int grid_n, const double* origin, double h,
+                        const void* f4, double* o0, double* o1, double* o2, int accumulate,
+                        const float* corr, float scale, int relative, const double* v0,
+                        const double* v1, const double* v2, int n_proj, double half,
+                        double full, double* out, void* work, size_t work_bytes,
+                        void* stream) {
+  SSIM_REQUIRE(np >= 1 && grid_n >= 1 && h > 0.0, "bad TSC arguments (np >= 1 for a box)");
+  SSIM_REQUIRE(x && y && z && origin && f4 && o0 && o1 && o2 && v0 && v1 && v2 && out &&
+                   work,
+               "null pointer");
+  SSIM_REQUIRE(work_bytes >= ssim_tsc_gather_box_work_bytes(np), "workspace too small");
+  const int ns[3] = {grid_n, grid_n, grid_n};
+  const Grid g = make_grid(3, ns, origin, h, 0);
+  hipStream_t st = ssim::as_stream(stream);
+  const int nblk = (int)ssim::ceil_div(np, PM_BLOCK);
+  double* part = static_cast<double*>(work);
+  const Corr cr{corr, scale, relative, v0, v1, v2, corr ? n_proj : 0};
+  const BoxOut bo{v0, v1, v2, half, full, part};
+  const float4* fp = static_cast<const float4*>(f4);
+#define SSIM_GB_LAUNCH(C, B)                                                             \
+  SSIM_TIMED("k_tsc_gather", st,                                                        \
+             hipLaunchKernelGGL((k_tsc_gather4<C, B, true>), dim3(nblk), dim3(PM_BLOCK), 0, \
+                                st, np, order, x, y, z, g, fp, o0, o1, o2, accumulate, cr, \
+                                bo))
+  const bool buf = buffer_mesh(grid_n);
+  if (corr && buf) SSIM_GB_LAUNCH(true, true);
+  else if (corr) SSIM_GB_LAUNCH(true, false);
+  else if (buf) SSIM_GB_LAUNCH(false, true);
+  else SSIM_GB_LAUNCH(false, false);
+#undef SSIM_GB_LAUNCH
+  SSIM_LAUNCHED("k_tsc_gather4");
+  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(PM_BLOCK), 0, st, nblk, 3, part, out);
+  SSIM_LAUNCHED("k_minmax_final");
   return SSIM_OK;
 }
 

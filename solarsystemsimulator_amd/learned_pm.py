@@ -140,15 +140,19 @@ class PMLearnedSystem:
         return dict(out=self._corr_out, scale=self.out_unit(),
                     relative=self.net.relative, vel=self.v, n_proj=self.n)
 
-    def force(self, learned: bool = True, placed: bool = False):
+    def force(self, learned: bool = True, placed: bool = False, box=None):
         """a = split force (+ learned correction) at the current positions;
-        `placed`: the mesh is placed for them already (run's steps)."""
+        `placed`: the mesh is placed for them already (run's steps).  `box`
+        (_gather_box only): the gather also writes the box of the next
+        drift (SplitForce.mesh_gather)."""
         fold = (learned and self.net is not None and self.fold and
                 self.split.can_fold_correction())
+        if box is not None and learned and self.net is not None and not fold:
+            raise ValueError("the box comes from the gather only with the correction folded")
         # features behind the deposit, the network behind the near field
         corr = (self._correction_features, self._correction_outputs) if fold else None
         self.split.accelerations(self.x, self.n, self.m, self.a, placed=placed,
-                                 correction=corr)
+                                 correction=corr, box=box)
         if learned and self.net is not None and not fold:
             feat = self.features()
             # masses: no component along the velocity (no work done on
@@ -175,9 +179,13 @@ class PMLearnedSystem:
     # deposit and the convolution.  Inside a run() call each step but the
     # last ends with the next step's opening kick and drift fused into its
     # closing kick (ssim_kick_drift_box), so only the first step launches
-    # its own (ssim_kick_drift).
+    # its own (ssim_kick_drift).  When the gather gives the final
+    # accelerations (the correction folded into it, or none), it also
+    # writes the next box (ssim_tsc_gather_box): the host waits for that
+    # part of the step only, and places the mesh and queues the next step
+    # while the step's kicks and drift (ssim_kicks_drift) still run.
     use_graph = True
-    captures = 0          # step graphs captured so far (two per placement at most)
+    captures = 0          # step graphs captured so far (three per placement at most)
 
     def _box_buffers(self):
         torch = _torch()
@@ -188,6 +196,7 @@ class PMLearnedSystem:
             lib = _lib.load()
             self._box_ws = D.workspace(lib.ssim_minmax_work_bytes(nb), self.dev)
             self._kb_ws = D.workspace(lib.ssim_kick_box_work_bytes(nb), self.dev)
+            self._gb_ws = D.workspace(lib.ssim_tsc_gather_box_work_bytes(nb), self.dev)
             self._box_ev = torch.cuda.Event()
 
     def _next_box(self, half, full):
@@ -196,12 +205,15 @@ class PMLearnedSystem:
                   *ps(self.a), half, full, _lib.ptr(self._box_dev),
                   _lib.ptr(self._box_ws), self._box_ws.numel(), D.stream(self.dev))
 
-    def _read_box(self, in_host=False):
+    def _read_box(self, in_host=False, recorded=False):
         """The box of the next drift: from the pinned host buffer (in_host:
-        written there by ssim_kick_box) or copied from the device one."""
+        written there by the step) or copied from the device one.
+        recorded: the box event was recorded after the launch that writes
+        it (later work may still run)."""
         if not in_host:
             self._box_host.copy_(self._box_dev, non_blocking=True)
-        self._box_ev.record()
+        if not (in_host and recorded):
+            self._box_ev.record()
         self._box_ev.synchronize()
         b = self._box_host.numpy()
         return [(float(b[2 * c]), float(b[2 * c + 1])) for c in range(3)]
@@ -236,10 +248,37 @@ class PMLearnedSystem:
                   *ps(self.a), half, snap_ptr, st)
         self._next_box(half, dt)
 
+    def _gather_box(self, learned):
+        """The gather can write the next box: TSC with interleaved FP32
+        fields, and the correction (if any) folded into the gather."""
+        return self.split.can_fold_correction() and (
+            not learned or self.net is None or self.fold)
+
+    def _force_box(self, dt, learned):
+        """A step's force (its opening kick and drift done), the gather
+        writing the box of the next drift into the pinned host buffer."""
+        self.force(learned, placed=True,
+                   box={"vel": self.v, "half": 0.5 * dt, "full": dt,
+                        "out": _lib.ptr(self._box_host), "work": self._gb_ws})
+
+    def _tail(self, dt, drift_next):
+        """A step's closing half kick and, drift_next, the next step's
+        opening half kick and drift (ssim_kicks_drift: the operations of
+        ssim_kick then ssim_kick_drift)."""
+        ps = lambda lst: [_lib.ptr(c) for c in lst]  # noqa: E731
+        st = D.stream(self.dev)
+        if drift_next:
+            _lib.call("ssim_kicks_drift", 3, self.n + self.t, *ps(self.x), *ps(self.v),
+                      *ps(self.a), 0.5 * dt, dt, st)
+        else:
+            _lib.call("ssim_kick", 3, self.n + self.t, *ps(self.v), *ps(self.a), 0.5 * dt,
+                      st)
+
     def _graph_for(self, dt, learned, drift_next=False):
         """The captured step for the current placement (captured anew when
         the placement, dt or the learned flag changed); both forms of the
-        step's end (drift_next) are kept."""
+        step's end (drift_next) are kept, and the force with the gather's
+        box (drift_next = "box", _force_box)."""
         torch = _torch()
         key = (self.split.placement(), float(dt), bool(learned), self.fold,
                self.split.overlap, self.split.near_hilo)
@@ -254,7 +293,10 @@ class PMLearnedSystem:
             self.split.side_first = True
             try:
                 with torch.cuda.graph(g, stream=side):
-                    self._step(dt, learned, drift_next=drift_next)
+                    if drift_next == "box":
+                        self._force_box(dt, learned)
+                    else:
+                        self._step(dt, learned, drift_next=drift_next)
             finally:
                 self.split.side_first = False
             torch.cuda.current_stream(self.dev).wait_stream(side)
@@ -272,17 +314,27 @@ class PMLearnedSystem:
             self.force(learned)
         use = self.use_graph if graph is None else graph
         use = use and snap is None and self.split.freeze
+        # the gather writes the next box, and the host does not wait for
+        # the step's kicks and drift
+        early = snap is None and self._gather_box(learned)
         self._next_box(0.5 * dt, dt)
         in_host = False
         for s in range(steps):
-            self.split.place_box(self._read_box(in_host))
-            in_host = snap is None    # the step's ssim_kick_box writes it
+            self.split.place_box(self._read_box(in_host, recorded=early))
+            in_host = snap is None    # the step writes it into the host buffer
             # the first step opens with its own kick and drift; each later
             # one's were fused into the step before (no snapshots)
             if s == 0 or snap is not None:
                 self._opening(dt)
             drift_next = snap is None and s + 1 < steps
-            if use:
+            if early:
+                if use:
+                    self._graph_for(dt, learned, "box").replay()
+                else:
+                    self._force_box(dt, learned)
+                self._box_ev.record()
+                self._tail(dt, drift_next)
+            elif use:
                 self._graph_for(dt, learned, drift_next).replay()
             else:
                 self._step(dt, learned, None if snap is None else _lib.ptr(snap[s]),

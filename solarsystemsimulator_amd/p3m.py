@@ -197,9 +197,26 @@ class SplitForce:
         with the interleaved FP32 fields)."""
         return self.assign == "tsc" and self.f4 is not None
 
-    def mesh_gather(self, x, a, order=None, accumulate=False, corr=None):
+    def mesh_gather(self, x, a, order=None, accumulate=False, corr=None, box=None):
         """a (+)= the meshes interpolated at the bodies x; `corr` (a dict of
-        out, scale, relative, vel, n_proj) folds a learned correction in."""
+        out, scale, relative, vel, n_proj) folds a learned correction in.
+        `box` (a dict of vel, half, full, out, work; can_fold_correction()
+        only): the gather also writes the box of the positions the next
+        kicks and drift give (ssim_tsc_gather_box) to out."""
+        if box is not None:
+            if not self.can_fold_correction():
+                raise ValueError("the box folds into the TSC float4 gather only")
+            org = _lib.double_array(self.origin)
+            c = corr or {}
+            _lib.call("ssim_tsc_gather_box", x[0].numel(), _lib.ptr(order),
+                      *[_lib.ptr(q) for q in x], self.n, P._addr(org), self.h,
+                      _lib.ptr(self.f4), *[_lib.ptr(q) for q in a], int(accumulate),
+                      _lib.ptr(c.get("out")), float(c.get("scale", 0.0)),
+                      int(c.get("relative", 0)), *[_lib.ptr(q) for q in box["vel"]],
+                      int(c.get("n_proj", 0)), float(box["half"]), float(box["full"]),
+                      box["out"], _lib.ptr(box["work"]), box["work"].numel(),
+                      D.stream(self.dev))
+            return a
         if corr is not None:
             if not self.can_fold_correction():
                 raise ValueError("the correction folds into the TSC float4 gather only")
@@ -479,7 +496,7 @@ class SplitForce:
     _nt = -1
     _grid = None
 
-    def accelerations(self, x, n_mass, m, a, placed=False, correction=None):
+    def accelerations(self, x, n_mass, m, a, placed=False, correction=None, box=None):
         """a = mesh + near-field acceleration of all bodies in x (n_mass
         masses first, tracers after them); `placed`: the caller has placed
         the mesh for x already (place_box), so nothing waits for the host.
@@ -489,7 +506,9 @@ class SplitForce:
         gather then adds it.  A pair (prepare, finish) splits it: prepare
         (the features, which read the mass mesh) runs right after the
         deposit on the current stream, finish (the network) after it on the
-        side stream behind the near field and returns the dict.
+        side stream behind the near field and returns the dict.  `box`
+        (TSC with interleaved FP32 fields): the gather also writes the box
+        of the next drift (mesh_gather).
 
         With TSC the work runs on two streams: the current one bins all
         bodies once (the deposit's mass bins, the near field's sources and
@@ -503,8 +522,8 @@ class SplitForce:
         import torch
         if not placed:
             self.place(x)
-        if correction is not None and not self.can_fold_correction():
-            raise ValueError("correction needs TSC with interleaved FP32 fields")
+        if (correction is not None or box is not None) and not self.can_fold_correction():
+            raise ValueError("correction / box need TSC with interleaved FP32 fields")
         if self.assign == "tsc" and not self.overlap:
             self.tsc_bins(x, n_mass, m)
             self.near_sum(x, a, accumulate=False)
@@ -515,7 +534,7 @@ class SplitForce:
             else:
                 corr = correction() if correction is not None else None
             self.convolve(interleaved=self.f4 is not None)
-            self.mesh_gather(x, a, order=self.torder, accumulate=True, corr=corr)
+            self.mesh_gather(x, a, order=self.torder, accumulate=True, corr=corr, box=box)
             return a
         if self.assign != "tsc":
             self.bin_bodies(x, n_mass, m)
@@ -525,7 +544,7 @@ class SplitForce:
         main = torch.cuda.current_stream(self.dev)
         start = torch.cuda.Event()
         start.record(main)
-        self.run_segments(self.tsc_segments(x, n_mass, m, a, correction),
+        self.run_segments(self.tsc_segments(x, n_mass, m, a, correction, box),
                           {"start": start})
         return a
 
@@ -535,7 +554,7 @@ class SplitForce:
             self._side = torch.cuda.Stream(device=self.dev)  # default priority
         return self._side
 
-    def tsc_segments(self, x, n_mass, m, a, correction=None):
+    def tsc_segments(self, x, n_mass, m, a, correction=None, box=None):
         """The overlapped TSC force as ordered segments (stream, fn, waits,
         signal): stream "main" (the current one) or "side"; before a
         segment runs, its stream waits for the named events, and afterwards
@@ -574,7 +593,7 @@ class SplitForce:
         segs[segs.index(last)] = last[:3] + ("side_done",)
         segs += [("main", lambda: self.convolve(interleaved=self.f4 is not None), (), None),
                  ("main", lambda: self.mesh_gather(x, a, order=self.torder, accumulate=True,
-                                                   corr=held.get("corr")),
+                                                   corr=held.get("corr"), box=box),
                   ("side_done",), None)]
         return segs
 

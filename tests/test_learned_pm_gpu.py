@@ -7,6 +7,7 @@ import torch
 
 from solarsystemsimulator_amd import _lib, scenes
 from solarsystemsimulator_amd import device as D
+from solarsystemsimulator_amd import pm as P
 from solarsystemsimulator_amd.learned_pm import PMLearnedSystem, load_weights
 
 pytestmark = pytest.mark.gpu
@@ -169,3 +170,77 @@ def test_folded_correction_equals_separate_update(dev, overlap):
     for c in range(3):
         assert torch.equal(folded[c], separate[c])
         assert not torch.equal(folded[c], plain[c])
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_gather_box_steps_equal_step_box(dev, graph):
+    """Steps whose gather writes the next box (the host waits for it and
+    not for the kicks and drift) equal steps whose closing kick writes it,
+    bit for bit, with the same mesh placements."""
+    print(f"gather box steps graph={graph}", flush=True)
+    pos, m, tpos = scenes.learned_scene((20, 20, 10), (20, 20, 20), seed=11,
+                                        tracer_seed=111)
+    n, t = len(m), len(tpos[0])
+    rng = np.random.default_rng(12)
+    vel = [rng.normal(0.0, 2.0, n) for _ in range(3)]
+    tvel = [rng.normal(0.0, 2.0, t) for _ in range(3)]
+    early = _system(dev, pos, m, tpos, vel, tvel)
+    late = _system(dev, pos, m, tpos, vel, tvel)
+    assert early._gather_box(True)
+    late._gather_box = lambda learned: False
+    early.run(2.5e-4, 7, graph=graph)
+    late.run(2.5e-4, 7, graph=graph)
+    for c in range(3):
+        assert torch.equal(early.x[c], late.x[c])
+        assert torch.equal(early.v[c], late.v[c])
+        assert torch.equal(early.a[c], late.a[c])
+    assert early.split.placement() == late.split.placement()
+
+
+def test_gather_box_matches_kick_box(dev):
+    """ssim_tsc_gather_box: the accelerations of ssim_tsc_gather_corrected
+    and of ssim_tsc_gather (no correction), and the box ssim_kick_drift_box
+    gives for the state they leave, bit for bit."""
+    print("gather box C-ABI", flush=True)
+    pos, m, tpos = scenes.learned_scene((20, 20, 10), (20, 20, 20), seed=13,
+                                        tracer_seed=113)
+    n, t = len(m), len(tpos[0])
+    nb = n + t
+    rng = np.random.default_rng(14)
+    vel = [rng.normal(0.0, 2.0, n) for _ in range(3)]
+    tvel = [rng.normal(0.0, 2.0, t) for _ in range(3)]
+    s = _system(dev, pos, m, tpos, vel, tvel)
+    s.force(learned=True)                 # mesh fields, bins, network outputs
+    sf = s.split
+    lib = _lib.load()
+    st = D.stream(dev)
+    org = _lib.double_array(sf.origin)
+    import ctypes
+    corr = torch.from_numpy(rng.normal(0.0, 1.0, (nb, 3)).astype(np.float32)).to(dev)
+    base = [D.to_dev(rng.normal(0.0, 1.0, nb), dev) for _ in range(3)]
+    half, full = 1.25e-4, 2.5e-4
+    gw = D.workspace(lib.ssim_tsc_gather_box_work_bytes(nb), dev)
+    kw = D.workspace(lib.ssim_kick_box_work_bytes(nb), dev)
+    for use_corr in (True, False):
+        ref = [b.clone() for b in base]
+        got = [b.clone() for b in base]
+        if use_corr:
+            _lib.call("ssim_tsc_gather_corrected", nb, _lib.ptr(sf.torder),
+                      *_ptrs(s.x), sf.n, ctypes.addressof(org), sf.h, _lib.ptr(sf.f4),
+                      *_ptrs(ref), 1, _lib.ptr(corr), 0.01, 1, *_ptrs(s.v), n, st)
+        else:
+            _lib.call("ssim_tsc_gather", nb, _lib.ptr(sf.torder), *_ptrs(s.x), sf.n,
+                      ctypes.addressof(org), sf.h, P.F32, _lib.ptr(sf.f4), None, None,
+                      *_ptrs(ref), 1, st)
+        box = torch.empty(6, dtype=torch.float64, device=dev)
+        _lib.call("ssim_tsc_gather_box", nb, _lib.ptr(sf.torder), *_ptrs(s.x), sf.n,
+                  ctypes.addressof(org), sf.h, _lib.ptr(sf.f4), *_ptrs(got), 1,
+                  _lib.ptr(corr) if use_corr else None, 0.01, 1, *_ptrs(s.v), n, half,
+                  full, _lib.ptr(box), _lib.ptr(gw), gw.numel(), st)
+        for c in range(3):
+            assert torch.equal(got[c], ref[c]), (use_corr, c)
+        xk, vk = [c.clone() for c in s.x], [c.clone() for c in s.v]
+        kbox = torch.empty(6, dtype=torch.float64, device=dev)
+        _lib.call("ssim_kick_drift_box", 3, nb, *_ptrs(xk), *_ptrs(vk), *_ptrs(got), half,
+                  full, _lib.ptr(kbox), _lib.ptr(kw), kw.numel(), st)
+        assert torch.equal(box, kbox), use_corr
