@@ -164,7 +164,12 @@ def algorithmic_bytes(n, t, n_mesh, mesh_bytes=4):
     input read once, each output written once): n masses, t tracers, an
     n_mesh^3 mesh.  The near field reads every target's position and
     writes its acceleration (48 B), reads the masses' packed float4
-    sources (16 B) and the target order (4 B per target)."""
+    sources (16 B) and the target order (4 B per target).  The step's
+    gather (correction folded in, adding to the near field's sum, writing
+    the next box) reads per body its slot in the cell order (4 B), its
+    position (24 B), the sum it adds to (24 B), the network's outputs
+    (12 B) and its velocity (24 B), writes the sum (24 B), and reads the
+    three fields of every node (12 of the float4's 16 B)."""
     nb = n + t
     g3 = n_mesh ** 3
     passes, x3 = conv_bytes(n_mesh, mesh_bytes)
@@ -175,7 +180,8 @@ def algorithmic_bytes(n, t, n_mesh, mesh_bytes=4):
         "k_fft_pass": sum(passes) / len(passes),
         "k_conv_x3": x3,
         "k_tsc_deposit": n * (3 * 8 + 8) + g3 * mesh_bytes,
-        "k_tsc_gather": nb * 3 * 8 + 3 * g3 * mesh_bytes + nb * 3 * 8,
+        "k_tsc_gather": nb * (4 + 24 + 24 + 12 + 24 + 24) + 3 * g3 * mesh_bytes,
+        "k_cic_gather": nb * 3 * 8 + 3 * g3 * mesh_bytes + nb * 3 * 8,
         "k_mlp_forward": nb * (64 + 12),
         "k_mlp_features": nb * (24 + 64) + g3 * mesh_bytes,
     }
@@ -484,11 +490,11 @@ def main():
         "k_tsc_deposit": ("hbm", alg["k_tsc_deposit"], HBM_PEAK_GBS, "GB/s",
                           "N (3 x 8 B position + 8 B mass) + n^3 x 4 B mesh"),
         "k_tsc_gather": ("hbm", alg["k_tsc_gather"], HBM_PEAK_GBS, "GB/s",
-                         "N x 3 x 8 B position + 3 n^3 x 4 B meshes + N x 3 x 8 B "
-                         "acceleration"),
+                         "N (4 B slot + 24 B position + 24 B sum in + 12 B network "
+                         "outputs + 24 B velocity + 24 B sum out) + 3 n^3 x 4 B meshes"),
         "k_cic_deposit": ("hbm", alg["k_tsc_deposit"], HBM_PEAK_GBS, "GB/s",
                           "N (3 x 8 B position + 8 B mass) + n^3 x 4 B mesh"),
-        "k_cic_gather": ("hbm", alg["k_tsc_gather"], HBM_PEAK_GBS, "GB/s",
+        "k_cic_gather": ("hbm", alg["k_cic_gather"], HBM_PEAK_GBS, "GB/s",
                          "N x 3 x 8 B position + 3 n^3 x 4 B meshes + N x 3 x 8 B "
                          "acceleration"),
         "k_mlp_forward": ("mfma", mlp_flops, BF16_PEAK_TFLOPS, "TFLOP/s",
