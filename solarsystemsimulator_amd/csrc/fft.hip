@@ -27,7 +27,9 @@ namespace {
 constexpr int FFT_THREADS = 256;
 constexpr int FFT_TILE_BYTES = 32768;  // LDS tile of one workgroup
 constexpr int FFT_MAXB = 64;           // lines per tile along a strided axis
-constexpr int FFT_MAX_L = 4096;
+constexpr int FFT_MAX_L = 8192;        // longest line
+constexpr int FFT_LINE_L = 4096;       // longest line of the regular pass; longer
+                                       // lines: one per workgroup (k_fft_pass<., ., 8192>)
 
 // MODE_POISSON: a strided C2C pass along the first axis of a 3-D half
 // spectrum that runs forward, multiplies by the periodic Poisson factor
@@ -48,10 +50,11 @@ constexpr int tile_elems() {
 }
 
 // the largest tile a workgroup may hold: a single line can be longer than
-// the tile (up to FFT_MAX_L points, 64 KB in FP64)
-template <typename F>
+// the tile (up to MAXL points: 64 KB in FP64 for the regular pass, 128 KB
+// for the 8192-point one)
+template <typename F, int MAXL = FFT_LINE_L>
 constexpr int max_elems() {
-  return tile_elems<F>() > FFT_MAX_L ? tile_elems<F>() : FFT_MAX_L;
+  return tile_elems<F>() > MAXL ? tile_elems<F>() : MAXL;
 }
 
 template <typename F>
@@ -340,7 +343,7 @@ __device__ void poisson_lines(cpx<F>* s, int LS, int nl, int L, const Pass& P, l
 // All lines of a tile transformed in LDS (forward or inverse): the
 // four-step forms for 128 and 256 points, Stockham radix-4 (+ one radix-2
 // stage) otherwise.
-template <typename F>
+template <typename F, int MAXL = FFT_LINE_L>
 __device__ void tile_fft(cpx<F>* s, int LS, int nl, int L, int log2L, const cpx<F>* twp,
                          int inverse) {
   const int E = nl * L;
@@ -352,7 +355,7 @@ __device__ void tile_fft(cpx<F>* s, int LS, int nl, int L, int log2L, const cpx<
     // ---- Stockham stages over all lines of the tile ----
     int p = 1;
     if (log2L & 1) {  // one radix-2 stage (p == 1: k == 0, no twiddle)
-      constexpr int NB = max_elems<F>() / 2 / FFT_THREADS;
+      constexpr int NB = max_elems<F, MAXL>() / 2 / FFT_THREADS;
       const int T = L / 2;
       cpx<F> r0[NB], r1[NB];
   #pragma unroll
@@ -380,7 +383,7 @@ __device__ void tile_fft(cpx<F>* s, int LS, int nl, int L, int log2L, const cpx<
       p = 2;
     }
     for (; p < L; p *= 4) {
-      constexpr int NB = max_elems<F>() / 4 / FFT_THREADS;
+      constexpr int NB = max_elems<F, MAXL>() / 4 / FFT_THREADS;
       const int T = L / 4;
       const int twstep = L / (4 * p);
       cpx<F> r[NB][4];
@@ -428,7 +431,7 @@ __device__ void tile_fft(cpx<F>* s, int LS, int nl, int L, int log2L, const cpx<
 
 }
 
-template <typename F, int MODE>
+template <typename F, int MODE, int MAXL = FFT_LINE_L>
 __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
     const void* __restrict__ vin, void* __restrict__ vout, Pass P,
     const cpx<F>* __restrict__ tw, int inverse, F scale,
@@ -518,11 +521,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_pass(
   }
   __syncthreads();
 
-  tile_fft(s, LS, nl, L, P.log2L, twp, inverse);
+  tile_fft<F, MAXL>(s, LS, nl, L, P.log2L, twp, inverse);
   if (MODE == MODE_POISSON) {
     poisson_lines(s, LS, nl, L, P, o0, b0, nvalid);
     __syncthreads();
-    tile_fft(s, LS, nl, L, P.log2L, twp, 1);
+    tile_fft<F, MAXL>(s, LS, nl, L, P.log2L, twp, 1);
   }
 
   // ---- store ----
@@ -839,10 +842,29 @@ int launch_pass(const void* in, void* out, Pass P, const void* tw, int inverse,
                (P.lines_per_tile > 1 ? P.L + 1 : P.L) * sizeof(cpx<F>);
   P.tw_lds = lds + (size_t)P.L * sizeof(cpx<F>) <= 65536 ? 1 : 0;
   if (P.tw_lds) lds += (size_t)P.L * sizeof(cpx<F>);
-  SSIM_TIMED("k_fft_pass", st,
-             hipLaunchKernelGGL((k_fft_pass<F, MODE>), dim3(nwg), dim3(FFT_THREADS),
-                                lds, st, in, out, P, static_cast<const cpx<F>*>(tw),
-                                inverse, scale, mul));
+  if (P.L <= FFT_LINE_L) {
+    SSIM_TIMED("k_fft_pass", st,
+               hipLaunchKernelGGL((k_fft_pass<F, MODE>), dim3(nwg), dim3(FFT_THREADS),
+                                  lds, st, in, out, P, static_cast<const cpx<F>*>(tw),
+                                  inverse, scale, mul));
+  } else {
+    // one line per workgroup in more than 64 KB of LDS (8192 points: 64 KB
+    // in FP32, 128 KB in FP64), which the device must offer a workgroup
+    SSIM_REQUIRE(P.lines_per_tile == 1, "long FFT lines go one per workgroup");
+    int dev = 0, max_lds = 0;
+    SSIM_HIP(hipGetDevice(&dev));
+    SSIM_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    SSIM_REQUIRE(lds <= (size_t)max_lds,
+                 "%d-point FFT lines need %zu bytes of LDS per workgroup, the device "
+                 "offers %d", P.L, lds, max_lds);
+    SSIM_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&k_fft_pass<F, MODE, FFT_MAX_L>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    SSIM_TIMED("k_fft_pass", st,
+               hipLaunchKernelGGL((k_fft_pass<F, MODE, FFT_MAX_L>), dim3(nwg),
+                                  dim3(FFT_THREADS), lds, st, in, out, P,
+                                  static_cast<const cpx<F>*>(tw), inverse, scale, mul));
+  }
   SSIM_LAUNCHED("k_fft_pass");
   return SSIM_OK;
 }

@@ -26,7 +26,7 @@ from .constants import G_AU
 
 F64 = _lib.PREC_F64
 F32 = _lib.PREC_F32C   # FP32 mesh
-MAX_FFT_LINE = 4096    # longest FFT line (points) of ssim_fft_*
+MAX_FFT_LINE = 8192    # longest FFT line (points) of ssim_fft_*
 
 
 def _addr(a):
@@ -65,7 +65,7 @@ class FFTPlan:
         self.work_bytes = lib.ssim_fft_work_bytes(nd, _addr(self._c_dims), prec)
         if self.work_bytes == 0:
             raise ValueError(f"unsupported FFT dims {self.dims} (powers of two "
-                             "up to 4096)")
+                             f"up to {MAX_FFT_LINE})")
         self.work = D.workspace(self.work_bytes, dev)
         _lib.call("ssim_fft_prepare", nd, _addr(self._c_dims), prec,
                   _lib.ptr(self.work), self.work_bytes, D.stream(dev))
@@ -295,8 +295,8 @@ class NBodyPM:
         self._kernels = None
         self._kernel_h = None
         # the reference's default grid_n = 4096 (pm_kernel.py:190) pads to
-        # 8192-point lines; the FFT here handles lines of up to 4096 points,
-        # so grid_n is limited to 2048 — rejected before any device work
+        # 8192-point lines, the longest the FFT here handles; larger grids
+        # are rejected before any device work
         if not (2 <= self.grid_n and 2 * self.grid_n <= MAX_FFT_LINE
                 and self.grid_n & (self.grid_n - 1) == 0):
             raise ValueError(f"grid_n={grid_n}: NBodyPM needs a power of two with the "

@@ -20,9 +20,10 @@ def dev_t(a, dev, dtype=torch.float64):
 
 
 @pytest.mark.parametrize("prec", [P.F64, P.F32])
-@pytest.mark.parametrize("dims", [[2], [8], [256], [4096], [2, 4], [16, 8],
+@pytest.mark.parametrize("dims", [[2], [8], [256], [4096], [8192], [2, 4], [16, 8],
                                   [64, 128], [256, 16], [1024, 512], [4, 8, 2],
-                                  [32, 32, 32], [128, 64, 256], [256, 4, 256]])
+                                  [32, 32, 32], [128, 64, 256], [256, 4, 256],
+                                  [8192, 8], [4, 8192]])
 def test_fft_matches_numpy(dev, dims, prec):
     print(f"fft dims={dims} prec={prec}", flush=True)
     rng = np.random.default_rng(len(dims) * 100 + dims[-1])
@@ -238,6 +239,42 @@ def test_ref_3_4_cloud_vs_allpairs(dev, eps_in_h, schwelle):
     rx, ry = oracle.allpairs([xs, ys], ms, [xs, ys], G_AU, eps2)
     rel = np.hypot(ax.cpu().numpy() - rx, ay.cpu().numpy() - ry) / np.hypot(rx, ry)
     assert np.median(rel) < schwelle
+
+
+def test_nbodypm_default_grid(dev):
+    """NBodyPM() with the reference's defaults (grid_n = 4096: 8192-point
+    padded FFT lines, pm_kernel.py:190): the PM accelerations of a clumped
+    disc against the direct sum (median error below the 2048-grid one's
+    bound) and 20 KDK steps conserving momentum."""
+    print("NBodyPM default grid", flush=True)
+    rng = np.random.default_rng(43)
+    n, R, Mtot = 4000, 30000.0, 5.6e9
+    r = R * np.sqrt(rng.uniform(0, 1, n))
+    th = rng.uniform(0, 2 * np.pi, n)
+    x, y = r * np.cos(th), r * np.sin(th)
+    m = np.full(n, Mtot / n)
+    kern = P.NBodyPM([0])
+    assert kern.grid_n == 4096
+    gx, gy = dev_t(x, dev), dev_t(y, dev)
+    x0, y0, h = P.grid_fuer(gx, gy, 4096, 4.0)
+    eps2 = (kern.softening_zellen * h) ** 2
+    ax, ay = P.pm_accelerations(gx, gy, dev_t(m, dev), 4096, x0, y0, h, eps2)
+    rx, ry = oracle.allpairs([x, y], m, [x, y], G_AU, eps2)
+    rel = np.hypot(ax.cpu().numpy() - rx, ay.cpu().numpy() - ry) / np.hypot(rx, ry)
+    assert np.median(rel) < 0.05
+    vvir = np.sqrt(G_AU * Mtot / R)
+    ang = rng.uniform(0, 2 * np.pi, n)
+    vx, vy = 0.7 * vvir * np.cos(ang), 0.7 * vvir * np.sin(ang)
+    vx -= np.sum(m * vx) / Mtot
+    vy -= np.sum(m * vy) / Mtot
+    st = kern.load_state(x, y, vx, vy, m)
+    out = kern.step_batch(st, R / vvir / 1000.0, steps=20)
+    assert out.shape == (20, 4 * n)
+    f = kern.export_f64(st)
+    _, p1 = pm_np.energie_impuls(f[0:n], f[n:2 * n], f[2 * n:3 * n], f[3 * n:4 * n],
+                                 m, eps2)
+    p_rel = np.hypot(*p1) / np.hypot(np.sum(m * np.abs(vx)), np.sum(m * np.abs(vy)))
+    assert p_rel < 1e-6
 
 
 def test_ref_5_dynamics(dev):

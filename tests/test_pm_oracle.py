@@ -176,13 +176,14 @@ def test_periodic_kdk_reversible_and_second_order():
     assert err(0.01, 2, (w1, w0, w1)) < 0.2 * e1
 
 
-def test_nbodypm_default_grid_is_a_clear_error():
-    """NBodyPM() with the reference's default grid_n = 4096 (8192-point
-    padded lines) raises a ValueError that names the limit, before any
-    device work; grid_n must be a power of two up to 2048."""
+def test_nbodypm_grid_limits_are_clear_errors():
+    """NBodyPM beyond the longest FFT line (grid_n = 8192: 16384-point padded
+    lines) or with a grid that is not a power of two raises a ValueError
+    that names the limit, before any device work; the reference's default
+    grid_n = 4096 (8192-point lines) is within it (tests/test_pm_gpu.py)."""
     import pytest
     from solarsystemsimulator_amd import pm as P
-    with pytest.raises(ValueError, match="grid_n <= 2048"):
-        P.NBodyPM([0])
+    with pytest.raises(ValueError, match="grid_n <= 4096"):
+        P.NBodyPM([0], grid_n=8192)
     with pytest.raises(ValueError, match="power of two"):
         P.NBodyPM([0], grid_n=1000)
