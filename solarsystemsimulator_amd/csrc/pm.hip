@@ -958,8 +958,9 @@ __global__ __launch_bounds__(PM_BLOCK) void k_tsc_gather4(
     const double* __restrict__ z, Grid g, const float4* __restrict__ f,
     double* __restrict__ o0, double* __restrict__ o1, double* __restrict__ o2,
     int accumulate, Corr cr, BoxOut bx) {
-  // XCD-contiguous ranges of the (cell-ordered) bodies: an XCD's L2 then
-  // holds the part of the mesh its bodies touch
+  // XCD-contiguous ranges of the visited bodies (in cell order, or in
+  // index order when that is spatially coherent): an XCD's L2 then holds
+  // the part of the mesh its bodies touch
   const int i_ = ssim_xcd_block(blockIdx.x, gridDim.x) * PM_BLOCK + threadIdx.x;
   if (!BOX && i_ >= np) return;
   double blo[3] = {INFINITY, INFINITY, INFINITY}, bhi[3] = {-INFINITY, -INFINITY, -INFINITY};

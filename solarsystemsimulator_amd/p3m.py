@@ -517,8 +517,11 @@ class SplitForce:
         correction, while the current one deposits and convolves (bound by
         memory and latency); the near field is stored into a, and the
         current stream waits for the side stream and adds the mesh gather
-        (in cell order).  With CIC everything runs in order on one
-        stream."""
+        (in index order: each body's sum and the box do not depend on the
+        visit order, and index order reads the bodies' arrays contiguously;
+        measured as fast as the cell order on randomly ordered bodies and
+        faster in the step on the lattice-ordered cube).  With CIC
+        everything runs in order on one stream."""
         import torch
         if not placed:
             self.place(x)
@@ -534,7 +537,7 @@ class SplitForce:
             else:
                 corr = correction() if correction is not None else None
             self.convolve(interleaved=self.f4 is not None)
-            self.mesh_gather(x, a, order=self.torder, accumulate=True, corr=corr, box=box)
+            self.mesh_gather(x, a, order=None, accumulate=True, corr=corr, box=box)
             return a
         if self.assign != "tsc":
             self.bin_bodies(x, n_mass, m)
@@ -592,7 +595,7 @@ class SplitForce:
             next(sg for sg in segs if sg[0] == "side" and sg[2] == ("bins",))
         segs[segs.index(last)] = last[:3] + ("side_done",)
         segs += [("main", lambda: self.convolve(interleaved=self.f4 is not None), (), None),
-                 ("main", lambda: self.mesh_gather(x, a, order=self.torder, accumulate=True,
+                 ("main", lambda: self.mesh_gather(x, a, order=None, accumulate=True,
                                                    corr=held.get("corr"), box=box),
                   ("side_done",), None)]
         return segs
