@@ -310,32 +310,69 @@ __global__ __launch_bounds__(CB) void k_cells_scan(int ncells, const int* __rest
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[ncells] = off + tot;
 }
 
-// start = the exclusive scan of cnt[0..ncells) (start[ncells] = the total),
-// one launch: each workgroup sums the counts before its CB cells itself
-// (ncells <= DIRECT_SCAN_CELLS keeps those re-reads small).
+// The masses' starts and their packed copies in one launch (ncells <=
+// DIRECT_SCAN_CELLS): workgroup b takes the PK_CELLS blocks from c0 =
+// PK_CELLS b, sums the mass counts before them itself (16-byte loads, eight
+// in flight per thread), scans its own counts in one wave and writes their
+// starts (start[ncells] = the total), then packs its blocks' masses: the
+// w-th of them is the k-th mass of block c, at sorted position tstart[c] + k
+// (the masses lead every block's segment), slot start[c] + k — the slots and
+// records of k_split_pack_all, with a thread for each mass.
 constexpr int DIRECT_SCAN_CELLS = 32768;
-__global__ __launch_bounds__(CB) void k_cells_scan_direct(int ncells,
-                                                          const int* __restrict__ cnt,
-                                                          int* __restrict__ start) {
+constexpr int PK_CELLS = 64;
+__global__ __launch_bounds__(CB) void k_cells_scan_pack(
+    int ncells, int n_mass, const int* __restrict__ cnt, const int* __restrict__ tstart,
+    const int* __restrict__ torder, int* __restrict__ start, int* __restrict__ order,
+    const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const double* __restrict__ m, double ox, double oy,
+    double oz, double h, double cx, double cy, double cz, float4* __restrict__ packed,
+    double4* __restrict__ body, int nb, double cell, float4* __restrict__ near,
+    float4* __restrict__ near_lo) {
+  static_assert(PK_CELLS == 64, "one wave scans the workgroup's counts");
   __shared__ int red[4];
-  // the counts before this workgroup's cells (a multiple of CB of them):
-  // 16-byte loads, eight in flight per thread
+  __shared__ int lpre[PK_CELLS + 1], lts[PK_CELLS];
+  const int t = threadIdx.x;
+  const int c0 = blockIdx.x * PK_CELLS;
   const int4* c4 = reinterpret_cast<const int4*>(cnt);
-  const int n4 = blockIdx.x * (CB / 4);
+  const int n4 = c0 / 4;
   int pre = 0;
 #pragma unroll 8
-  for (int i = threadIdx.x; i < n4; i += CB) {
+  for (int i = t; i < n4; i += CB) {
     const int4 q = c4[i];
     pre += (q.x + q.y) + (q.z + q.w);
   }
   int off;
   ssim_block256_excl(pre, red, &off);
-  const int c = blockIdx.x * CB + threadIdx.x;
-  const int v = c < ncells ? cnt[c] : 0;
-  int tot;
-  const int run = off + ssim_block256_excl(v, red, &tot);
-  if (c < ncells) start[c] = run;
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[ncells] = off + tot;
+  if (t < PK_CELLS) {
+    const int c = c0 + t;
+    const int v = c < ncells ? cnt[c] : 0;
+    int run = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int u = __shfl_up(run, d, 64);
+      if (t >= d) run += u;
+    }
+    lpre[t] = run - v;
+    if (t == PK_CELLS - 1) lpre[PK_CELLS] = run;
+    lts[t] = c < ncells ? tstart[c] : 0;
+    if (c < ncells) start[c] = off + run - v;
+    if (c == ncells - 1) start[ncells] = off + run;
+  }
+  __syncthreads();
+  const int total = lpre[PK_CELLS];
+  for (int w = t; w < total; w += CB) {
+    // the block holding mass w: the last j with lpre[j] <= w (empty blocks
+    // repeat the prefix, so the last one is the block that has masses)
+    int j = 0;
+#pragma unroll
+    for (int st = 32; st >= 1; st >>= 1)
+      if (lpre[j + st] <= w) j += st;
+    const int i = torder[lts[j] + (w - lpre[j])];
+    const int s = off + w;
+    order[s] = i;
+    split_pack_one(s, i, c0 + j, x, y, z, m, ox, oy, oz, h, cx, cy, cz, packed, body, nb,
+                   cell, near, near_lo);
+  }
 }
 
 // The masses' slots from the all-body order: the mass at sorted position p
@@ -604,11 +641,17 @@ int ssim_split_bin_all(int n_all, int n_mass, const double* x, const double* y,
     SSIM_LAUNCHED("k_split_key");
   }
   const int sblocks = ssim::ceil_div(ncells, CB);
+  const double hh = 0.5 * h;
   if (fused) {
     SSIM_TRY(run_starts(n_all, key, ncells, tcell_start, st, torder, n_mass, mcount));
-    hipLaunchKernelGGL(k_cells_scan_direct, dim3(sblocks), dim3(CB), 0, st, ncells, mcount,
-                       cell_start);
-    SSIM_LAUNCHED("k_cells_scan_direct");
+    hipLaunchKernelGGL(k_cells_scan_pack, dim3(ssim::ceil_div(ncells, PK_CELLS)), dim3(CB), 0,
+                       st, ncells, n_mass, mcount, tcell_start, torder, cell_start, order, x,
+                       y, z, m, origin[0], origin[1], origin[2], h, origin[0] - hh,
+                       origin[1] - hh, origin[2] - hh, static_cast<float4*>(packed),
+                       static_cast<double4*>(body), nb, block * h, static_cast<float4*>(near),
+                       static_cast<float4*>(near_lo));
+    SSIM_LAUNCHED("k_cells_scan_pack");
+    return SSIM_OK;
   } else {
     SSIM_TRY(run_starts(n_all, key, ncells, tcell_start, st));
     if (sblocks <= SCAN_BLOCKS &&
@@ -629,7 +672,6 @@ int ssim_split_bin_all(int n_all, int n_mass, const double* x, const double* y,
     }
   }
   if (n_mass > 0) {
-    const double hh = 0.5 * h;
     hipLaunchKernelGGL(k_split_pack_all, dim3(ssim::ceil_div(n_all, CB)), dim3(CB), 0, st,
                        n_all, n_mass, torder, key, tcell_start, cell_start, order, x, y, z,
                        m, origin[0], origin[1], origin[2], h, origin[0] - hh,
