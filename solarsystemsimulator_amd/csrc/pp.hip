@@ -604,6 +604,10 @@ __global__ __launch_bounds__(BT) void k_pp_bits(
 
   double hx = 0.0, hy = 0.0, hz = 0.0;
   const float eps2f = (float)eps2;
+  // the table offset in a VGPR once: the pair loop's fma already takes
+  // tab_scale from an SGPR, and a VOP3 reads at most one
+  float toff;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(toff) : "s"(tab_off));
   for (int bt = 0; bt < nbatch; ++bt) {
     // slots of this batch per row and their exclusive prefix
     const int nr = (lane < 9 && len > bt) ? (len - bt + nbatch - 1) / nbatch : 0;
@@ -741,7 +745,7 @@ __global__ __launch_bounds__(BT) void k_pp_bits(
         const float xx = r2 + eps2f;
         const float inv = __builtin_amdgcn_rsqf(xx);
         // table in s = sqrt(r^2 + eps2) = xx inv, linear interpolation
-        const float tt = __builtin_amdgcn_fmed3f(fmaf(xx * inv, tab_scale, tab_off), 0.0f,
+        const float tt = __builtin_amdgcn_fmed3f(fmaf(xx * inv, tab_scale, toff), 0.0f,
                                                  (float)GTB);
         const float2 e = tab[(int)tt];
         const float g = fmaf(__builtin_amdgcn_fractf(tt), e.y, e.x);
